@@ -1,0 +1,201 @@
+/*
+ * speckle_amd.h -- C ABI of the MI355X speckle-tracking library.
+ *
+ * This is the drop-in boundary for the GrIMP speckle-tracking hot path
+ * (fastice/speckleSource).  The reference exposes three in-process entry
+ * points that all take one fat struct:
+ *
+ *   void speckleTrack(TrackParams *)      Strack/strack.h:170,  called at Strack/strack.c:134
+ *   void corrTrackFast(TrackParams *)     Strackw/strackw.h:1,  called at Strackw/strackw.c:126
+ *   void cullSTData/cullStats/cullSmooth/cullIslands(CullParams *)
+ *                                         Cullst/cullst.h:60-67, called at Cullst/cullst.c:75-100
+ *
+ * TrackParams embeds FFTW plan and GDAL handle types (Strack/strack.h:84-85,
+ * 135-142), so it cannot be cloned as-is.  st_track_params carries the same
+ * fields as plain C types; images live on the device as "pairs" that the
+ * caller uploads once and tracks many times.  Output grids keep the
+ * reference's layout (row-major nA x nR, float32 / byte) and sentinels
+ * (-2e9 offsets, type 0, Strack/speckleTrack.c:306-323).
+ *
+ * Every function returns ST_OK (0) or a negative ST_ERR_* code; none exits
+ * the process (the reference's error() does).  st_last_error() returns a
+ * message for the last failing call on a context.
+ *
+ * Threading: one st_ctx per device; calls on one context are serialised by
+ * the library.
+ */
+#ifndef SPECKLE_AMD_H
+#define SPECKLE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ST_OK 0
+#define ST_ERR_ARG (-1)         /* invalid argument or size */
+#define ST_ERR_HIP (-2)         /* HIP runtime failure */
+#define ST_ERR_UNSUPPORTED (-3) /* option not implemented on this path */
+#define ST_ERR_NOMEM (-4)       /* device allocation failed / over budget */
+#define ST_ERR_NODEV (-5)       /* no usable GPU */
+
+/* SLC sample types (Strack/strack.h:10-16, speckleTrack.c:1221-1232) */
+#define ST_SAMPLE_C64 0  /* complex float32, fftw_complex {re, im} */
+#define ST_SAMPLE_CI16 1 /* complex int16, strackComplex {r, i} */
+
+/* Match types written to .mt (Strack/speckleTrack.c:19-22) */
+#define ST_BAD 0
+#define ST_CMATCH 1
+#define ST_AMPMATCH 2
+#define ST_AMPMATCHLARGE 3
+
+/* "no value" sentinel, -LARGEINT (Strack/writeVrt.c:69) */
+#define ST_NODATA (-2.0e9f)
+
+/* Precision of the patch correlations.  ST_PREC_FP32 is the plain FP32 FFT
+ * path.  ST_PREC_MFMA forms the small window DFTs as BF16 hi+lo split
+ * contractions on the matrix cores (FP32 accumulate). */
+#define ST_PREC_FP32 0
+#define ST_PREC_MFMA 1
+
+typedef struct st_ctx st_ctx;
+
+/* Tracking parameters: the fields of TrackParams (Strack/strack.h:56-165)
+ * that the strack / strackw hot path reads. */
+typedef struct {
+    /* estimate grid, already divided by scaleFactor (parseTrack.c:207-210) */
+    int32_t rStart, aStart;   /* strack.h:91-92 */
+    int32_t deltaR, deltaA;   /* strack.h:94-95 */
+    int32_t nR, nA;           /* strack.h:97-98 */
+    /* windows */
+    int32_t wR, wA;           /* complex window, strack.h:100-101 */
+    int32_t wRa, wAa;         /* amplitude window, strack.h:103-104 */
+    /* strackw only */
+    int32_t edgePadR, edgePadA; /* strack.h:107-108 (caller value; +5 is added internally, corrTrackFast.c:116-117) */
+    int32_t navgR, navgA;       /* strack.h:110-111 */
+    int32_t scaleFactor;        /* strack.h:109 */
+    /* flags, defaults in Strack/strack.c:143-154 */
+    int32_t noComplex;          /* -noComplex */
+    int32_t hanningFlag;        /* default 1; -noHanning clears it */
+    int32_t gaussFlag;          /* -gauss: not supported (ST_ERR_UNSUPPORTED) */
+    int32_t maxTries;           /* 2, -singleAmp gives 1 */
+    int32_t checkAzFocus;       /* -checkAzFocus */
+    float azDefocusThresh;      /* 1.8 (strack.c:45) */
+    int32_t precision;          /* ST_PREC_* */
+    /* initial offsets (strack.h:115,131-133,154; speckleTrack.c:810-847) */
+    int32_t polyShift;          /* 1: use the polynomials only */
+    double rShiftPoly[3];       /* rShift = p0 + p1*r + p2*a */
+    double aShiftPoly[3];
+    const float *initDr;        /* optional initial-offset grids, initNa x initNr */
+    const float *initDa;
+    int32_t initNr, initNa;
+    double initR0, initA0, initDeltaR, initDeltaA;
+    /* optional NN mask (strack.h:36-45, speckleTrack.c:1951-1965) */
+    const uint8_t *mask;        /* maskNa x maskNr, NULL = no mask */
+    int32_t maskNr, maskNa, maskR0, maskA0, maskNrl, maskNal;
+    /* baseline phase ramp on patch 1 (speckleTrack.c:868-907) */
+    int32_t baseFlag;           /* 0: the ramp is the identity and is skipped */
+    double Bn, Bp, dBn, dBp, dBnQ, dBpQ, lambda;
+    int32_t nSlpA1;             /* image-1 azimuth size used by the ramp */
+    const double *sinThetaD;    /* per-range tables, length nr of image 1 */
+    const double *cosThetaD;
+    const double *overRange;
+} st_track_params;
+
+/* Output grids, caller-owned host arrays of nA*nR elements (row-major). */
+typedef struct {
+    float *offR;   /* .dr */
+    float *offA;   /* .da */
+    float *corr;   /* .cc */
+    float *azd;    /* .azd (strack only; may be NULL) */
+    uint8_t *type; /* .mt */
+    int32_t nComplex, nAmp, nAmpL, nFail; /* strack.h:150-153 */
+} st_track_out;
+
+/* Optional per-point diagnostics (any pointer may be NULL).  Attempt index
+ * 0 = complex match, 1 = amplitude small window, 2 = amplitude large window.
+ * Peaks are in the fftshifted 2x-oversampled correlation surface
+ * (findCPeak, speckleTrack.c:744-805); -1000 = rejected by findCPeak,
+ * -2000 = attempt not run. */
+typedef struct {
+    int32_t *r2, *a2;          /* integer position of window 2 (findImage2Pos) */
+    int32_t *azShift;          /* azimuth spectral shift used by the complex match */
+    int32_t *peakI[3], *peakJ[3];
+    int32_t *fineI[3], *fineJ[3]; /* peak in 1/(2*NOVER) px units after oversampling */
+    double *cmax[3];           /* normalised correlation of the attempt */
+    float *sigRg[3], *sigAz[3];/* 1-D Gaussian widths (gaussPeak1D) */
+    /* (best - runner-up) / best over the coarse surface and over the
+     * oversampled search box: small values mark near-tied peaks whose
+     * argmax may legitimately differ between FP32 implementations. */
+    float *peakGap[3], *fineGap[3];
+} st_track_diag;
+
+/* ---- contexts and image pairs ---------------------------------------- */
+
+int st_ctx_create(int32_t device, st_ctx **ctx);
+int st_ctx_destroy(st_ctx *ctx);
+const char *st_last_error(st_ctx *ctx);
+
+/* Upload an SLC pair (image 1 na1 x nr1, image 2 na2 x nr2 samples,
+ * native byte order) and keep it resident.  Replaces the strip buffers of
+ * loadBuffer / updateSLCBuffer (speckleTrack.c:1369-1399,
+ * corrTrackFast.c:554-602): the whole scene stays in HBM. */
+int st_pair_upload(st_ctx *ctx, const void *img1, int32_t nr1, int32_t na1,
+                   const void *img2, int32_t nr2, int32_t na2,
+                   int32_t sample_type, int32_t *pair);
+int st_pair_release(st_ctx *ctx, int32_t pair);
+/* Copy rows [row0, row0+nrows) of both images of a resident pair back to
+ * the host (complex float32). */
+int st_pair_download(st_ctx *ctx, int32_t pair, int32_t row0, int32_t nrows,
+                     void *img1, void *img2);
+
+/* ---- tracking ---------------------------------------------------------- */
+
+/* strack: complex match, then amplitude small / large retries.
+ * Replaces speckleTrack (Strack/strack.h:170, speckleTrack.c:159-346). */
+int st_speckle_track(st_ctx *ctx, int32_t pair, const st_track_params *par,
+                     st_track_out *out, st_track_diag *diag);
+
+/* strackw: edge-padded normalised cross-correlation tracker.
+ * Replaces corrTrackFast (Strackw/strackw.h:1, corrTrackFast.c:89-231). */
+int st_corr_track_fast(st_ctx *ctx, int32_t pair, const st_track_params *par,
+                       st_track_out *out, st_track_diag *diag);
+
+/* ---- culling (Cullst/cullst.h:3-67) ------------------------------------ */
+
+typedef struct {
+    int32_t nR, nA;
+    float *offR, *offA;     /* in/out, culled in place */
+    float *corr;            /* in */
+    uint8_t *type;          /* in/out */
+    const uint8_t *mask;    /* offsets.mask, may be NULL (maskFlag = 0) */
+    float *offRS, *offAS;   /* out: smoothed offsets */
+    float *sigmaR, *sigmaA; /* out: plane-fit sigmas */
+    int32_t bR, bA;         /* median box (boxSize) */
+    int32_t nGood;          /* cullst.c:144 default 17 */
+    float maxA, maxR;       /* 1.0, 0.75 */
+    int32_t sR, sA;         /* smoothing window (2, 8) */
+    int32_t islandThresh;   /* -1 = off */
+    int32_t singleMT;       /* -1 = off */
+    float corrThresh;       /* 0 */
+    int32_t nCulled;        /* out: points culled by the last st_cull_st_data pass */
+} st_cull_params;
+
+/* One pass of cullSTData (Cullst/cullSTData.c:13-126). */
+int st_cull_st_data(st_ctx *ctx, st_cull_params *cp);
+/* cullStats (Cullst/cullStats.c:19-144). */
+int st_cull_stats(st_ctx *ctx, st_cull_params *cp);
+/* cullSmooth (Cullst/cullSmooth.c:29-103). */
+int st_cull_smooth(st_ctx *ctx, st_cull_params *cp);
+/* cullIslands (Cullst/cullIslands.c:13-198). */
+int st_cull_islands(st_ctx *ctx, st_cull_params *cp);
+/* The cullst call order (Cullst/cullst.c:75-100): cullSTData x3,
+ * cullStats, cullSmooth, cullIslands when islandThresh > 0. */
+int st_cull_run(st_ctx *ctx, st_cull_params *cp);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPECKLE_AMD_H */

@@ -1,0 +1,389 @@
+/*
+ * ORACLE -- test infrastructure (see orc_common.h).  Not the product.
+ *
+ * CPU restatement of cullst's culling stages, Cullst/cullSTData.c,
+ * cullStats.c, cullSmooth.c and cullIslands.c of fastice/speckleSource.
+ *
+ * - cullSTData: box median test, in place, raster order.  The median is
+ *   the floor(n/2)-th smallest of the 1-based list (selectForCull,
+ *   cullSTData.c:132-192); here it is taken by sorting, which selects the
+ *   same order statistic.
+ * - cullStats: plane fit {mean, row, col} by truncated-SVD least squares
+ *   (NR svdfit contract: singular values below 1e-5 * max are zeroed;
+ *   cRecipes is not vendored, the tolerance is NR's published one).  The
+ *   SVD is one-sided Jacobi in double.
+ * - cullSmooth: weighted box mean with effective looks.
+ * - cullIslands: labelRegions (unwrapSource) is not vendored; components
+ *   are 4-connected and labelled 1.. in raster order of their first pixel
+ *   ("parity unpinned"); the last label is skipped as in the reference
+ *   (cullIslands.c:152,179).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "orc_common.h"
+#include "../include/speckle_amd.h"
+
+#define NARROWREGION 4
+#define NEG_LARGE (-2000000000.0f)
+
+static inline int valid2(float r, float a)
+{
+    /* offR > (1 - LARGEINT) with LARGEINT = 2e9 (cullSTData.c:63) */
+    return r > (1 - ORC_LARGEINT) && a > (1 - ORC_LARGEINT);
+}
+
+static int cmpf(const void *x, const void *y)
+{
+    float a = *(const float *)x, b = *(const float *)y;
+    return (a > b) - (a < b);
+}
+
+int orc_cull_st_data(st_cull_params *cp)
+{
+    int nA = cp->nA, nR = cp->nR;
+    float *offR = cp->offR, *offA = cp->offA;
+    int maskFlag = cp->mask != NULL;
+    for (int i = 0; i < nA; i++)
+        for (int j = 0; j < nR; j++) {
+            size_t o = (size_t)i * nR + j;
+            if (cp->corr[o] < cp->corrThresh) {
+                offR[o] = NEG_LARGE;
+                offA[o] = NEG_LARGE;
+                cp->type[o] = 0;
+            }
+            if (maskFlag) {
+                unsigned char mv = (unsigned char)cp->mask[o] & NARROWREGION;
+                if (mv > 0 && cp->type[o] > 2) {
+                    offR[o] = NEG_LARGE;
+                    offA[o] = NEG_LARGE;
+                    cp->type[o] = 0;
+                }
+                if (cp->singleMT > 0 && cp->type[o] != cp->singleMT) {
+                    offR[o] = NEG_LARGE;
+                    offA[o] = NEG_LARGE;
+                    cp->type[o] = 0;
+                }
+            }
+        }
+    size_t nlist = 1000 + (size_t)(cp->bR + 1) * (cp->bA + 1);
+    float *listR = (float *)malloc(sizeof(float) * nlist), *listA = (float *)malloc(sizeof(float) * nlist);
+    int nCulled = 0;
+    for (int i = 0; i < nA; i++) {
+        int i1 = i - cp->bA / 2 > 0 ? i - cp->bA / 2 : 0;
+        int i2 = i + cp->bA / 2 < nA - 1 ? i + cp->bA / 2 : nA - 1;
+        for (int j = 0; j < nR; j++) {
+            int j1 = j - cp->bR / 2 > 0 ? j - cp->bR / 2 : 0;
+            int j2 = j + cp->bR / 2 < nR - 1 ? j + cp->bR / 2 : nR - 1;
+            unsigned ngood = 0;
+            for (int ii = i1; ii <= i2; ii++)
+                for (int jj = j1; jj <= j2; jj++) {
+                    size_t q = (size_t)ii * nR + jj;
+                    if (valid2(offR[q], offA[q])) {
+                        listR[ngood] = offR[q];
+                        listA[ngood] = offA[q];
+                        ngood++;
+                    }
+                }
+            size_t o = (size_t)i * nR + j;
+            if (ngood > (unsigned)cp->nGood) {
+                unsigned k = ngood / 2; /* 1-based index k -> 0-based k-1 */
+                if (k == 0)
+                    k = 1; /* ngood == 1: the reference reads the unused slot 0 */
+                qsort(listR, ngood, sizeof(float), cmpf);
+                qsort(listA, ngood, sizeof(float), cmpf);
+                float medianR = listR[k - 1], medianA = listA[k - 1];
+                double diffA = (double)(offA[o] - medianA);
+                double diffR = (double)(offR[o] - medianR);
+                if (fabs(diffA) > cp->maxA || fabs(diffR) > cp->maxR) {
+                    offR[o] = NEG_LARGE;
+                    offA[o] = NEG_LARGE;
+                    nCulled++;
+                }
+            } else {
+                offR[o] = NEG_LARGE;
+                offA[o] = NEG_LARGE;
+                nCulled++;
+            }
+        }
+    }
+    free(listR);
+    free(listA);
+    cp->nCulled = nCulled;
+    return ST_OK;
+}
+
+/* Least squares y ~ A x, A = n x 3, by one-sided Jacobi SVD with the
+ * svdfit truncation rule.  Returns x (3). */
+static void lsq3_svd(const double *A, const double *y, int n, double x[3])
+{
+    double U[81 * 3], V[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    memcpy(U, A, sizeof(double) * (size_t)n * 3);
+    for (int sweep = 0; sweep < 30; sweep++) {
+        double off = 0.0;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                double alpha = 0, beta = 0, gamma = 0;
+                for (int i = 0; i < n; i++) {
+                    alpha += U[i * 3 + p] * U[i * 3 + p];
+                    beta += U[i * 3 + q] * U[i * 3 + q];
+                    gamma += U[i * 3 + p] * U[i * 3 + q];
+                }
+                if (gamma == 0.0)
+                    continue;
+                double conv = fabs(gamma) / sqrt(alpha * beta);
+                if (conv > off)
+                    off = conv;
+                if (conv < 1e-15)
+                    continue;
+                double zeta = (beta - alpha) / (2.0 * gamma);
+                double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                for (int i = 0; i < n; i++) {
+                    double up = U[i * 3 + p], uq = U[i * 3 + q];
+                    U[i * 3 + p] = c * up - s * uq;
+                    U[i * 3 + q] = s * up + c * uq;
+                }
+                for (int i = 0; i < 3; i++) {
+                    double vp = V[i * 3 + p], vq = V[i * 3 + q];
+                    V[i * 3 + p] = c * vp - s * vq;
+                    V[i * 3 + q] = s * vp + c * vq;
+                }
+            }
+        if (off < 1e-15)
+            break;
+    }
+    double w[3], wmax = 0;
+    for (int k = 0; k < 3; k++) {
+        double s2 = 0;
+        for (int i = 0; i < n; i++)
+            s2 += U[i * 3 + k] * U[i * 3 + k];
+        w[k] = sqrt(s2);
+        if (w[k] > wmax)
+            wmax = w[k];
+    }
+    double thresh = 1.0e-5 * wmax;
+    double coef[3];
+    for (int k = 0; k < 3; k++) {
+        coef[k] = 0;
+        if (w[k] < thresh || w[k] == 0.0)
+            continue;
+        /* (u_k . y) / w_k with u_k = U[:,k] / w_k */
+        double s = 0;
+        for (int i = 0; i < n; i++)
+            s += U[i * 3 + k] * y[i];
+        coef[k] = s / (w[k] * w[k]);
+    }
+    for (int r = 0; r < 3; r++)
+        x[r] = V[r * 3 + 0] * coef[0] + V[r * 3 + 1] * coef[1] + V[r * 3 + 2] * coef[2];
+}
+
+int orc_cull_stats(st_cull_params *cp)
+{
+    int nA = cp->nA, nR = cp->nR;
+    const int b = 9; /* cullStats.c:38-39 forces 9 x 9 */
+    cp->bR = b;
+    cp->bA = b;
+    double listR[81], listA[81], A[81 * 3];
+    int xi[81], xj[81];
+    double aR[3] = {0, 0, 0}, aA[3] = {0, 0, 0};
+    for (int i = 0; i < nA; i++) {
+        int i1 = i - b / 2 > 0 ? i - b / 2 : 0;
+        int i2 = i + b / 2 < nA - 1 ? i + b / 2 : nA - 1;
+        for (int j = 0; j < nR; j++) {
+            int j1 = j - b / 2 > 0 ? j - b / 2 : 0;
+            int j2 = j + b / 2 < nR - 1 ? j + b / 2 : nR - 1;
+            int ngood = 0;
+            double meanA = 0.0, meanR = 0.0;
+            for (int ii = i1; ii <= i2; ii++)
+                for (int jj = j1; jj <= j2; jj++) {
+                    size_t q = (size_t)ii * nR + jj;
+                    if (valid2(cp->offR[q], cp->offA[q])) {
+                        listR[ngood] = cp->offR[q];
+                        listA[ngood] = cp->offA[q];
+                        meanA += cp->offA[q];
+                        meanR += cp->offR[q];
+                        xi[ngood] = ii;
+                        xj[ngood] = jj;
+                        ngood++;
+                    }
+                }
+            meanA /= (double)ngood;
+            meanR /= (double)ngood;
+            if (ngood > 8) {
+                /* the basis value is stored in a float field (cullStats.c:80) */
+                for (int k = 0; k < ngood; k++) {
+                    A[k * 3 + 0] = (double)(float)meanR;
+                    A[k * 3 + 1] = (double)(float)xi[k];
+                    A[k * 3 + 2] = (double)(float)xj[k];
+                }
+                lsq3_svd(A, listR, ngood, aR);
+                for (int k = 0; k < ngood; k++)
+                    A[k * 3 + 0] = (double)(float)meanA;
+                lsq3_svd(A, listA, ngood, aA);
+                aR[0] *= meanR;
+                aA[0] *= meanA;
+            }
+            int ng = 0;
+            double mA = 0.0, mR = 0.0, varR = 0.0, varA = 0.0;
+            for (int ii = i1; ii <= i2; ii++)
+                for (int jj = j1; jj <= j2; jj++) {
+                    size_t q = (size_t)ii * nR + jj;
+                    if (valid2(cp->offR[q], cp->offA[q])) {
+                        ng++;
+                        float oR = cp->offR[q], oA = cp->offA[q];
+                        oR = (float)(oR - (aR[0] + aR[1] * (float)ii + aR[2] * (float)jj));
+                        oA = (float)(oA - (aA[0] + aA[1] * (float)ii + aA[2] * (float)jj));
+                        mA += oA;
+                        mR += oR;
+                        varA += oA * oA;
+                        varR += oR * oR;
+                    }
+                }
+            size_t o = (size_t)i * nR + j;
+            if (ng > 8 && cp->offR[o] > (1 - ORC_LARGEINT)) {
+                float sA = (float)(varA / (float)(ng - 3));
+                float sR = (float)(varR / (float)(ng - 3));
+                sA = (float)(sA - mA * mA / (float)((ng - 3) * ng));
+                sR = (float)(sR - mR * mR / (float)((ng - 3) * ng));
+                sA = (float)(0.00001 > sA ? 0.00001 : sA);
+                sR = (float)(0.00001 > sR ? 0.00001 : sR);
+                cp->sigmaA[o] = (float)sqrt((double)sA);
+                cp->sigmaR[o] = (float)sqrt((double)sR);
+            } else {
+                cp->sigmaA[o] = NEG_LARGE;
+                cp->sigmaR[o] = NEG_LARGE;
+            }
+        }
+    }
+    return ST_OK;
+}
+
+/* sWeights, cullSmooth.c:10-27 (allocated at its full size here) */
+static void s_weights(int s, float *w /* 2*(s/2)+1 entries, centred */)
+{
+    float w2 = (float)((double)s / 2.0 - 0.01);
+    for (int m = -s / 2; m <= s / 2; m++)
+        w[m + s / 2] = (m < -w2 || m > w2) ? 0.5f : 1.0f;
+}
+
+int orc_cull_smooth(st_cull_params *cp)
+{
+    int nA = cp->nA, nR = cp->nR;
+    float nlooks[4] = {0.0f, 1.0f, 0.25f, 0.015f};
+    float wA[513], wR[513];
+    if (cp->sA > 1000 || cp->sR > 1000)
+        return ST_ERR_ARG;
+    s_weights(cp->sA, wA);
+    s_weights(cp->sR, wR);
+    for (int i = 0; i < nA; i++) {
+        int i1 = i - cp->sA / 2 > 0 ? i - cp->sA / 2 : 0;
+        int i2 = i + cp->sA / 2 < nA - 1 ? i + cp->sA / 2 : nA - 1;
+        int iwStart = i1 - i;
+        for (int j = 0; j < nR; j++) {
+            int j1 = j - cp->sR / 2 > 0 ? j - cp->sR / 2 : 0;
+            int j2 = j + cp->sR / 2 < nR - 1 ? j + cp->sR / 2 : nR - 1;
+            int jwStart = j1 - j;
+            double ngood = 0, meanA = 0.0, meanR = 0.0;
+            float nLooksEff = 0.0f;
+            for (int ii = i1, iw = iwStart; ii <= i2; ii++, iw++)
+                for (int jj = j1, jw = jwStart; jj <= j2; jj++, jw++) {
+                    size_t q = (size_t)ii * nR + jj;
+                    if (valid2(cp->offR[q], cp->offA[q])) {
+                        double w = wA[iw + cp->sA / 2] * wR[jw + cp->sR / 2];
+                        ngood += w;
+                        nLooksEff = (float)(nLooksEff + w * nlooks[(int)cp->type[q] & 3]);
+                        meanA += w * cp->offA[q];
+                        meanR += w * cp->offR[q];
+                    }
+                }
+            size_t o = (size_t)i * nR + j;
+            if (((ngood > 1) || (cp->sA == 1) || (cp->sR == 1)) && cp->offR[o] > (1 - ORC_LARGEINT) &&
+                cp->sigmaA[o] > (1 - ORC_LARGEINT)) {
+                nLooksEff = (float)(nLooksEff > 1. ? nLooksEff : 1.);
+                cp->sigmaA[o] = (float)(cp->sigmaA[o] / sqrt((double)nLooksEff));
+                cp->sigmaR[o] = (float)(cp->sigmaR[o] / sqrt((double)nLooksEff));
+                cp->offRS[o] = (float)(meanR / (float)ngood);
+                cp->offAS[o] = (float)(meanA / (float)ngood);
+            } else {
+                cp->offRS[o] = NEG_LARGE;
+                cp->offAS[o] = NEG_LARGE;
+            }
+        }
+    }
+    return ST_OK;
+}
+
+int orc_cull_islands(st_cull_params *cp)
+{
+    int nA = cp->nA, nR = cp->nR;
+    size_t n = (size_t)nA * nR;
+    int32_t *lab = (int32_t *)calloc(n, sizeof(int32_t));
+    int32_t *stack = (int32_t *)malloc(n * sizeof(int32_t));
+    int nlab = 0;
+    for (size_t p0 = 0; p0 < n; p0++) {
+        if (lab[p0] || !(cp->offRS[p0] >= (float)(-ORC_LARGEINT + 1000)))
+            continue;
+        nlab++;
+        size_t sp = 0;
+        stack[sp++] = (int32_t)p0;
+        lab[p0] = nlab;
+        while (sp) {
+            int32_t p = stack[--sp];
+            int i = p / nR, j = p % nR;
+            const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+            for (int k = 0; k < 4; k++) {
+                int ii = i + di[k], jj = j + dj[k];
+                if (ii < 0 || ii >= nA || jj < 0 || jj >= nR)
+                    continue;
+                size_t q = (size_t)ii * nR + jj;
+                if (!lab[q] && cp->offRS[q] >= (float)(-ORC_LARGEINT + 1000)) {
+                    lab[q] = nlab;
+                    stack[sp++] = (int32_t)q;
+                }
+            }
+        }
+    }
+    int *minX = (int *)malloc(sizeof(int) * (nlab + 1)), *maxX = (int *)malloc(sizeof(int) * (nlab + 1));
+    int *minY = (int *)malloc(sizeof(int) * (nlab + 1)), *maxY = (int *)malloc(sizeof(int) * (nlab + 1));
+    for (int l = 0; l <= nlab; l++) {
+        minX[l] = minY[l] = 2000000000;
+        maxX[l] = maxY[l] = -2000000000;
+    }
+    for (size_t p = 0; p < n; p++)
+        if (lab[p]) {
+            int l = lab[p], i = (int)(p / nR), j = (int)(p % nR);
+            if (j < minX[l]) minX[l] = j;
+            if (j > maxX[l]) maxX[l] = j;
+            if (i < minY[l]) minY[l] = i;
+            if (i > maxY[l]) maxY[l] = i;
+        }
+    for (size_t p = 0; p < n; p++) {
+        int l = lab[p];
+        if (l >= 1 && l < nlab && (maxX[l] - minX[l]) <= cp->islandThresh && (maxY[l] - minY[l]) <= cp->islandThresh) {
+            cp->offRS[p] = NEG_LARGE;
+            if (cp->offAS)
+                cp->offAS[p] = NEG_LARGE;
+        }
+    }
+    free(lab);
+    free(stack);
+    free(minX);
+    free(maxX);
+    free(minY);
+    free(maxY);
+    return ST_OK;
+}
+
+int orc_cull_run(st_cull_params *cp)
+{
+    orc_cull_st_data(cp);
+    orc_cull_st_data(cp);
+    orc_cull_st_data(cp);
+    orc_cull_stats(cp);
+    orc_cull_smooth(cp);
+    if (cp->islandThresh > 0)
+        orc_cull_islands(cp);
+    return ST_OK;
+}

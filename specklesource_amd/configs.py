@@ -1,0 +1,134 @@
+"""Synthetic workloads of the five benchmark configurations.
+
+Each function returns (spec, params) -- an abi.SynthSpec describing the
+scene and the abi.TrackParams of the tracking run -- at full size or
+scaled down for tests (`scale` divides the scene sides; windows, grid
+steps and offsets keep their full-size values so the per-patch work is
+the same).
+
+  cfg1  strack -noComplex on 2048 x 4096, w=32, grid 16, affine offsets
+  cfg2  strack complex+amplitude on 20000 x 14000, w=64, grid 24,
+        glacier ridge (12 px az / 4 px rg peak), coherence 0.3-0.9,
+        10% zero-filled
+  cfg3  64 pairs 1024-8192 px, w in {16,32,64,128}, grid w/2, affine
+        offsets within +/-20 px
+  cfg4  strackw on 8192 x 8192, coherence 0.15-0.4, shear to +/-120 px,
+        5x5 boxcar (navg=2), chip 128 or 256 with edge pad 123
+"""
+import numpy as np
+
+from . import abi
+from .track import default_params
+
+
+def _spec(nr, na, seed, **kw):
+    s = abi.SynthSpec()
+    s.nr, s.na, s.seed = int(nr), int(na), int(seed)
+    s.gmin, s.gmax = 0.7, 0.7
+    s.sigmaLn, s.corrLen = 0.5, 64.0
+    s.bwR, s.bwA, s.fdc = 0.85, 0.8, 0.12
+    s.maskR0 = s.maskR1 = s.maskA0 = s.maskA1 = 0
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def grid_for(nr, na, w, step, pad):
+    """Grid start/count covering the scene with `pad` px margins."""
+    r0 = pad + w
+    a0 = pad + w
+    nR = max(1, (nr - 2 * r0) // step + 1)
+    nA = max(1, (na - 2 * a0) // step + 1)
+    return r0, a0, nR, nA
+
+
+def affine_poly(spec):
+    """Initial-offset polynomials p0 + p1*r + p2*a equal to the affine
+    part of the synthetic field (the reference's initialoffsetfile)."""
+    return [spec.dr0, spec.drR, spec.drA], [spec.da0, spec.daR, spec.daA]
+
+
+def cfg1(scale=1, seed=1):
+    nr, na = 2048 // scale, 4096 // scale
+    s = _spec(nr, na, seed, dr0=1.0, drR=2.0 / 2048, da0=0.5, daA=2.0 / 4096, gmin=0.7, gmax=0.7)
+    w, step = 32, 16
+    r0, a0, nR, nA = grid_for(nr, na, 2 * w, step, 16)
+    rp, ap = [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]
+    p = default_params(rStart=r0, aStart=a0, deltaR=step, deltaA=step, nR=nR, nA=nA,
+                       wR=w, wA=w, wRa=w, wAa=w, noComplex=1, rShiftPoly=rp, aShiftPoly=ap)
+    return s, p
+
+
+def cfg2(scale=1, seed=2, masked=True):
+    nr, na = 20000 // scale, 14000 // scale
+    kw = dict(dr0=0.3, drR=0.7 / 20000, da0=0.2, daA=0.8 / 14000,
+              ridgeR=4.0, ridgeA=12.0, ridgeC=0.55 * nr, ridgeW=0.08 * nr,
+              gmin=0.3, gmax=0.9, gLr=nr / 1.5, gLa=na / 1.25)
+    if masked:
+        kw.update(maskR0=int(0.75 * nr), maskR1=int(0.95 * nr), maskA0=int(0.25 * na),
+                  maskA1=int(0.75 * na))
+    s = _spec(nr, na, seed, **kw)
+    w, step = 64, 24
+    r0, a0, nR, nA = grid_for(nr, na, 2 * w, step, 24)
+    rp, ap = affine_poly(s)
+    p = default_params(rStart=r0, aStart=a0, deltaR=step, deltaA=step, nR=nR, nA=nA,
+                       wR=w, wA=w, wRa=w, wAa=w, rShiftPoly=rp, aShiftPoly=ap)
+    return s, p
+
+
+def cfg3_pairs(n_pairs=64, scale=1, seed=3):
+    """List of (spec, params) for the batched uneven workload."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_pairs):
+        nr = int(rng.integers(1024, 8193)) // scale
+        na = int(rng.integers(1024, 8193)) // scale
+        w = int(rng.choice([16, 32, 64, 128]))
+        step = w // 2
+        c = rng.uniform(-20, 20, size=2)
+        g = rng.uniform(-2e-3, 2e-3, size=4)
+        s = _spec(nr, na, seed * 1000 + k, dr0=c[0], drR=g[0], drA=g[1], da0=c[1], daR=g[2],
+                  daA=g[3], gmin=0.5, gmax=0.8, gLr=nr / 1.3, gLa=na / 1.7)
+        pad = 2 * w + 40
+        r0, a0, nR, nA = grid_for(nr, na, w, step, pad)
+        rp, ap = affine_poly(s)
+        p = default_params(rStart=r0, aStart=a0, deltaR=step, deltaA=step, nR=nR, nA=nA,
+                           wR=w, wA=w, wRa=w, wAa=w, rShiftPoly=rp, aShiftPoly=ap)
+        out.append((s, p))
+    return out
+
+
+def cfg4(chip=128, scale=1, seed=4):
+    nr = na = 8192 // scale
+    s = _spec(nr, na, seed, shearA=120.0 * min(1.0, 1.0), dr0=0.5, gmin=0.15, gmax=0.4,
+              gLr=nr / 1.1, gLa=na / 1.3)
+    edge = 123
+    wRa = chip + 2 * (edge + 5)
+    step = chip // 2
+    r0, a0, nR, nA = grid_for(nr, na, wRa, step, 150)
+    p = default_params(rStart=r0, aStart=a0, deltaR=step, deltaA=step, nR=nR, nA=nA,
+                       wR=wRa, wA=wRa, wRa=wRa, wAa=wRa, edgePadR=edge, edgePadA=edge,
+                       navgR=2, navgA=2, noComplex=1)
+    return s, p
+
+
+def initial_grid(spec, par, factor=8, err_sigma=2.5, err_clip=5.0, seed=44):
+    """Initial-offset grid at 1/factor of the output resolution with
+    clipped Gaussian error (cfg4's supplied estimate)."""
+    from .track import synth_offset
+    rng = np.random.default_rng(seed)
+    nR0 = max(2, par.nR // factor + 2)
+    nA0 = max(2, par.nA // factor + 2)
+    dR = par.deltaR * factor
+    dA = par.deltaA * factor
+    r0 = par.rStart - par.wRa // 2
+    a0 = par.aStart - par.wAa // 2
+    gr = np.zeros((nA0, nR0), np.float32)
+    ga = np.zeros((nA0, nR0), np.float32)
+    for i in range(nA0):
+        for j in range(nR0):
+            dr, da = synth_offset(spec, r0 + j * dR + par.wRa / 2, a0 + i * dA + par.wAa / 2)
+            e = np.clip(rng.normal(0, err_sigma, 2), -err_clip, err_clip)
+            gr[i, j] = dr + e[0]
+            ga[i, j] = da + e[1]
+    return gr, ga, (r0, a0, dR, dA)

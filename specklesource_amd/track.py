@@ -1,0 +1,213 @@
+"""Host-side mirror of the reference tracking interface.
+
+The reference drives its hot path through three calls on fat structs:
+``speckleTrack(TrackParams*)`` (Strack/strack.h:170),
+``corrTrackFast(TrackParams*)`` (Strackw/strackw.h:1) and the cull
+functions on ``CullParams*`` (Cullst/cullst.h:60-67).  This module keeps
+those names and argument meanings for Python callers (tests, bench,
+serving) and runs them on the MI355X library through its C ABI.  The
+flag defaults are the reference's (Strack/strack.c:143-154,
+Cullst/cullst.c:140-150).
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libspeckle_amd.so")
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            "libspeckle_amd.so is not built (%s); run __graft_entry__.build()" % path)
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.st_ctx_create.argtypes = [C.c_int32, C.POINTER(vp)]
+    lib.st_ctx_destroy.argtypes = [vp]
+    lib.st_last_error.argtypes = [vp]
+    lib.st_last_error.restype = C.c_char_p
+    lib.st_pair_upload.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int32,
+                                   C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
+    lib.st_pair_release.argtypes = [vp, C.c_int32]
+    lib.st_pair_download.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, vp]
+    lib.st_speckle_track.argtypes = [vp, C.c_int32, C.POINTER(abi.TrackParams),
+                                     C.POINTER(abi.TrackOut), C.POINTER(abi.TrackDiag)]
+    lib.st_corr_track_fast.argtypes = lib.st_speckle_track.argtypes
+    for name in ("st_cull_st_data", "st_cull_stats", "st_cull_smooth",
+                 "st_cull_islands", "st_cull_run"):
+        getattr(lib, name).argtypes = [vp, C.POINTER(abi.CullParams)]
+    lib.st_synth_host.argtypes = [C.POINTER(abi.SynthSpec), vp, vp]
+    lib.st_synth_device.argtypes = [vp, C.POINTER(abi.SynthSpec), C.POINTER(C.c_int32)]
+    lib.st_synth_offset.argtypes = [C.POINTER(abi.SynthSpec), C.c_double, C.c_double,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.st_synth_offset.restype = None
+    _lib = lib
+    return lib
+
+
+def default_params(**kw):
+    """TrackParams with the reference defaults (strack.c:143-154)."""
+    p = abi.TrackParams()
+    p.scaleFactor = 1
+    p.hanningFlag = 1
+    p.maxTries = 2
+    p.azDefocusThresh = 1.8
+    p.polyShift = 1
+    p.precision = abi.ST_PREC_FP32
+    for k, v in kw.items():
+        if k in ("rShiftPoly", "aShiftPoly"):
+            getattr(p, k)[:] = list(v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+@dataclass
+class TrackResult:
+    offR: np.ndarray
+    offA: np.ndarray
+    corr: np.ndarray
+    azd: np.ndarray
+    type: np.ndarray
+    nComplex: int = 0
+    nAmp: int = 0
+    nAmpL: int = 0
+    nFail: int = 0
+    diag: dict = field(default_factory=dict)
+
+
+def alloc_outputs(par, want_diag=False):
+    shape = (par.nA, par.nR)
+    res = TrackResult(offR=np.zeros(shape, np.float32), offA=np.zeros(shape, np.float32),
+                      corr=np.zeros(shape, np.float32), azd=np.zeros(shape, np.float32),
+                      type=np.zeros(shape, np.uint8))
+    out = abi.TrackOut()
+    out.offR = abi.ptr(res.offR, C.c_float)
+    out.offA = abi.ptr(res.offA, C.c_float)
+    out.corr = abi.ptr(res.corr, C.c_float)
+    out.azd = abi.ptr(res.azd, C.c_float)
+    out.type = abi.ptr(res.type, C.c_uint8)
+    dg = None
+    if want_diag:
+        dg = abi.TrackDiag()
+        d = {}
+        for nm in ("r2", "a2", "azShift"):
+            d[nm] = np.zeros(shape, np.int32)
+            setattr(dg, nm, abi.ptr(d[nm], C.c_int32))
+        for nm, dt, ct in (("peakI", np.int32, C.c_int32), ("peakJ", np.int32, C.c_int32),
+                           ("fineI", np.int32, C.c_int32), ("fineJ", np.int32, C.c_int32),
+                           ("cmax", np.float64, C.c_double), ("sigRg", np.float32, C.c_float),
+                           ("sigAz", np.float32, C.c_float), ("peakGap", np.float32, C.c_float),
+                           ("fineGap", np.float32, C.c_float)):
+            arrs = [np.zeros(shape, dt) for _ in range(3)]
+            d[nm] = arrs
+            fld = getattr(dg, nm)
+            for t in range(3):
+                fld[t] = abi.ptr(arrs[t], ct)
+        res.diag = d
+    return res, out, dg
+
+
+def finish(res, out):
+    res.nComplex, res.nAmp, res.nAmpL, res.nFail = out.nComplex, out.nAmp, out.nAmpL, out.nFail
+    return res
+
+
+class Device:
+    """One MI355X context (st_ctx) with resident SLC pairs."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self.ctx = C.c_void_p()
+        rc = self.lib.st_ctx_create(device, C.byref(self.ctx))
+        if rc != abi.ST_OK:
+            raise RuntimeError("st_ctx_create failed (%d)" % rc)
+
+    def _check(self, rc, what):
+        if rc != abi.ST_OK:
+            msg = self.lib.st_last_error(self.ctx)
+            raise RuntimeError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def close(self):
+        if self.ctx:
+            self.lib.st_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_pair(self, img1, img2, sample_type=abi.ST_SAMPLE_C64):
+        img1 = np.ascontiguousarray(img1)
+        img2 = np.ascontiguousarray(img2)
+        pid = C.c_int32()
+        rc = self.lib.st_pair_upload(self.ctx, img1.ctypes.data, img1.shape[1], img1.shape[0],
+                                     img2.ctypes.data, img2.shape[1], img2.shape[0],
+                                     sample_type, C.byref(pid))
+        self._check(rc, "st_pair_upload")
+        return pid.value
+
+    def synth_pair(self, spec):
+        pid = C.c_int32()
+        self._check(self.lib.st_synth_device(self.ctx, C.byref(spec), C.byref(pid)),
+                    "st_synth_device")
+        return pid.value
+
+    def download_pair(self, pair, row0, nrows, nr):
+        a = np.zeros((nrows, nr), np.complex64)
+        b = np.zeros((nrows, nr), np.complex64)
+        self._check(self.lib.st_pair_download(self.ctx, pair, row0, nrows, a.ctypes.data,
+                                              b.ctypes.data), "st_pair_download")
+        return a, b
+
+    def release_pair(self, pair):
+        self._check(self.lib.st_pair_release(self.ctx, pair), "st_pair_release")
+
+    def speckle_track(self, pair, par, diag=False):
+        res, out, dg = alloc_outputs(par, diag)
+        rc = self.lib.st_speckle_track(self.ctx, pair, C.byref(par), C.byref(out),
+                                       C.byref(dg) if dg is not None else None)
+        self._check(rc, "st_speckle_track")
+        return finish(res, out)
+
+    def corr_track_fast(self, pair, par, diag=False):
+        res, out, dg = alloc_outputs(par, diag)
+        rc = self.lib.st_corr_track_fast(self.ctx, pair, C.byref(par), C.byref(out),
+                                         C.byref(dg) if dg is not None else None)
+        self._check(rc, "st_corr_track_fast")
+        return finish(res, out)
+
+    def cull(self, name, cp):
+        self._check(getattr(self.lib, name)(self.ctx, C.byref(cp)), name)
+        return cp
+
+
+def synth_host(spec):
+    """Generate a synthetic pair on the host (complex64 arrays na x nr)."""
+    lib = load_library()
+    a = np.zeros((spec.na, spec.nr), np.complex64)
+    b = np.zeros((spec.na, spec.nr), np.complex64)
+    rc = lib.st_synth_host(C.byref(spec), a.ctypes.data, b.ctypes.data)
+    if rc != abi.ST_OK:
+        raise RuntimeError("st_synth_host failed (%d)" % rc)
+    return a, b
+
+
+def synth_offset(spec, r, a):
+    lib = load_library()
+    dr, da = C.c_double(), C.c_double()
+    lib.st_synth_offset(C.byref(spec), float(r), float(a), C.byref(dr), C.byref(da))
+    return dr.value, da.value
