@@ -162,6 +162,27 @@ int st_speckle_track(st_ctx *ctx, int32_t pair, const st_track_params *par,
 int st_corr_track_fast(st_ctx *ctx, int32_t pair, const st_track_params *par,
                        st_track_out *out, st_track_diag *diag);
 
+/* Work counters and device timings of the last tracking call on a context.
+ * Not in the reference (it prints line summaries and stage timers instead,
+ * speckleTrack.c:1426-1447, corrTrackFast.c:219-224).  A "patch
+ * correlation" is a window pair that reaches the FFT stage: every complex
+ * match whose patches pass the data tests, every amplitude try whose
+ * windows are in bounds, every strackw patch. */
+typedef struct {
+    int64_t nPoints;
+    int64_t nCorr[3];      /* patch correlations: complex, amp small, amp large */
+    double stageMs[4];     /* device time: geometry, complex, amp small, amp large */
+    double totalMs;        /* device time of the whole call */
+    double kernelMs[8];    /* summed duration of the dominant kernels (see kernelNames) */
+    int64_t kernelLaunches[8];
+    char kernelNames[8][32];
+} st_track_stats;
+
+int st_last_stats(st_ctx *ctx, st_track_stats *stats);
+/* Enable per-kernel timing with HIP events (adds host synchronisation;
+ * off by default). */
+int st_set_profiling(st_ctx *ctx, int32_t on);
+
 /* ---- culling (Cullst/cullst.h:3-67) ------------------------------------ */
 
 typedef struct {

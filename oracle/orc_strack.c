@@ -992,3 +992,82 @@ int orc_speckle_track(const st_track_params *par, const void *img1, int nr1, int
     free(azdLocal);
     return ST_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* Test hooks: internals exposed so the oracle can be pinned against
+ * closed forms (tests/test_oracle.py). */
+void orc_oversample8(const double *P, double *Qmag)
+{
+    orc_st s;
+    memset(&s, 0, sizeof(s));
+    s.cFastOver = (ocplx *)calloc((size_t)NFAST * NOVER * NFAST * NOVER, sizeof(ocplx));
+    for (int k = 0; k < NFAST * NFAST; k++) {
+        s.cFast[k].re = (oreal)P[k];
+        s.cFast[k].im = 0;
+    }
+    over_sample_c(&s);
+    for (int k = 0; k < NFAST * NOVER * NFAST * NOVER; k++)
+        Qmag[k] = (double)absc(s.cFastOver[k]);
+    free(s.cFastOver);
+}
+
+float orc_gauss_sigma(const float *y, float x0, int azOrRg)
+{
+    oreal mag[NGAUSS * NGAUSS];
+    memset(mag, 0, sizeof(mag));
+    /* place the samples on row / column 9 of an 18 x 18 surface */
+    for (int t = 0; t < NGAUSS; t++) {
+        if (azOrRg == AZ)
+            mag[t * NGAUSS + NGAUSS / 2] = (oreal)y[t];
+        else
+            mag[(NGAUSS / 2) * NGAUSS + t] = (oreal)y[t];
+    }
+    return gauss_peak_1d(mag, NGAUSS, NGAUSS / 2, NGAUSS / 2, azOrRg) + 0.0f * x0;
+}
+
+void orc_hanning(int w, float *out)
+{
+    orc_st s;
+    memset(&s, 0, sizeof(s));
+    s.wA = s.wR = w;
+    s.hanning = (oreal *)calloc((size_t)w * w, sizeof(oreal));
+    compute_hanning(&s);
+    for (int k = 0; k < w * w; k++)
+        out[k] = (float)s.hanning[k];
+    free(s.hanning);
+}
+
+/* cmpPSWithPad + the 2w x 2w inverse + findCPeak's fftshifted magnitude,
+ * for two w x w patches already in the frequency domain's input form. */
+void orc_corr_surface(const ocplx *p1, const ocplx *p2, int w, int azShift, oreal *magOut)
+{
+    orc_st s;
+    memset(&s, 0, sizeof(s));
+    st_track_params par;
+    memset(&par, 0, sizeof(par));
+    par.nR = par.nA = 1;
+    par.hanningFlag = 0;
+    s.par = &par;
+    s.wA = s.wR = w;
+    s.azShift = azShift;
+    s.p1 = s.p2 = 1.0;
+    s.patch1 = (ocplx *)calloc((size_t)w * w, sizeof(ocplx));
+    s.patch2 = (ocplx *)calloc((size_t)w * w, sizeof(ocplx));
+    s.ps = (ocplx *)calloc((size_t)4 * w * w, sizeof(ocplx));
+    s.cbuf = (ocplx *)calloc((size_t)4 * w * w, sizeof(ocplx));
+    s.mag = magOut;
+    s.cFastOver = (ocplx *)calloc((size_t)NFAST * NOVER * NFAST * NOVER, sizeof(ocplx));
+    float azd;
+    s.azd = &azd;
+    memcpy(s.patch1, p1, sizeof(ocplx) * (size_t)w * w);
+    memcpy(s.patch2, p2, sizeof(ocplx) * (size_t)w * w);
+    int pI, pJ, fI, fJ;
+    double cMax;
+    float r1, a1;
+    complex_match(&s, 0, 0, 0, 0, &pI, &pJ, &fI, &fJ, &cMax, &r1, &a1);
+    free(s.patch1);
+    free(s.patch2);
+    free(s.ps);
+    free(s.cbuf);
+    free(s.cFastOver);
+}

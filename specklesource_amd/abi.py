@@ -83,6 +83,14 @@ class TrackDiag(C.Structure):
     ]
 
 
+class TrackStats(C.Structure):
+    _fields_ = [
+        ("nPoints", C.c_int64), ("nCorr", C.c_int64 * 3), ("stageMs", C.c_double * 4),
+        ("totalMs", C.c_double), ("kernelMs", C.c_double * 8), ("kernelLaunches", C.c_int64 * 8),
+        ("kernelNames", (C.c_char * 32) * 8),
+    ]
+
+
 class CullParams(C.Structure):
     """st_cull_params: CullParams fields of Cullst/cullst.h:3-60."""
 

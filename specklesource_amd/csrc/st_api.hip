@@ -1,0 +1,212 @@
+// Contexts, resident pairs and the tracking entry points of the C ABI
+// (include/speckle_amd.h).
+#include <string.h>
+
+#include "st_ctx.h"
+
+void st_ctx::arena_reserve(size_t bytes)
+{
+    if (arena.cap < bytes) {
+        if (arena.base)
+            ST_HIP(hipFree(arena.base));
+        arena.base = nullptr;
+        arena.cap = 0;
+        size_t cap = bytes + (bytes >> 3);
+        ST_HIP(hipMalloc(&arena.base, cap));
+        arena.cap = cap;
+    }
+    arena.reset();
+}
+
+void st_ctx::stats_reset()
+{
+    stats = st_track_stats{};
+    npend = 0;
+    if (!evInit) {
+        for (int k = 0; k < 72; k++)
+            ST_HIP(hipEventCreate(&ev[k]));
+        evInit = true;
+    }
+}
+
+int st_ctx::kernel_slot(const char *name)
+{
+    for (int k = 0; k < 8; k++) {
+        if (stats.kernelNames[k][0] == 0) {
+            strncpy(stats.kernelNames[k], name, 31);
+            return k;
+        }
+        if (strncmp(stats.kernelNames[k], name, 31) == 0)
+            return k;
+    }
+    return 7;
+}
+
+void st_ctx::time_begin(int slot)
+{
+    if (!profiling)
+        return;
+    if (npend >= 31)
+        flush_timings();
+    pendSlot[npend] = slot;
+    ST_HIP(hipEventRecord(ev[2 * npend], stream));
+}
+
+void st_ctx::time_end(int slot)
+{
+    (void)slot;
+    if (!profiling)
+        return;
+    ST_HIP(hipEventRecord(ev[2 * npend + 1], stream));
+    npend++;
+}
+
+void st_ctx::flush_timings()
+{
+    if (!npend)
+        return;
+    ST_HIP(hipEventSynchronize(ev[2 * npend - 1]));
+    for (int k = 0; k < npend; k++) {
+        float ms = 0;
+        ST_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+        stats.kernelMs[pendSlot[k]] += ms;
+        stats.kernelLaunches[pendSlot[k]] += 1;
+    }
+    npend = 0;
+}
+
+StPair &st_ctx::pair(int id)
+{
+    auto it = pairs.find(id);
+    ST_REQUIRE(it != pairs.end(), ST_ERR_ARG, "unknown pair id " + std::to_string(id));
+    return it->second;
+}
+
+extern "C" int st_ctx_create(int32_t device, st_ctx **out)
+{
+    if (!out)
+        return ST_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return ST_ERR_NODEV;
+    if (device < 0 || device >= n)
+        return ST_ERR_ARG;
+    st_ctx *c = new st_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return ST_ERR_HIP;
+    }
+    *out = c;
+    return ST_OK;
+}
+
+extern "C" int st_ctx_destroy(st_ctx *c)
+{
+    if (!c)
+        return ST_ERR_ARG;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto &kv : c->pairs) {
+        hipFree(kv.second.d1);
+        hipFree(kv.second.d2);
+    }
+    if (c->arena.base)
+        hipFree(c->arena.base);
+    if (c->evInit)
+        for (int k = 0; k < 72; k++)
+            hipEventDestroy(c->ev[k]);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return ST_OK;
+}
+
+extern "C" const char *st_last_error(st_ctx *c)
+{
+    return c ? c->err.c_str() : "null context";
+}
+
+extern "C" int st_pair_upload(st_ctx *c, const void *img1, int32_t nr1, int32_t na1, const void *img2, int32_t nr2,
+                              int32_t na2, int32_t stype, int32_t *pair)
+{
+    return st_guard(c, [&]() {
+        ST_REQUIRE(img1 && img2 && pair && nr1 > 0 && na1 > 0 && nr2 > 0 && na2 > 0, ST_ERR_ARG, "bad pair arguments");
+        ST_REQUIRE(stype == ST_SAMPLE_C64 || stype == ST_SAMPLE_CI16, ST_ERR_ARG, "bad sample type");
+        size_t es = stype == ST_SAMPLE_C64 ? 8 : 4;
+        StPair p;
+        p.nr1 = nr1;
+        p.na1 = na1;
+        p.nr2 = nr2;
+        p.na2 = na2;
+        p.stype = stype;
+        size_t b1 = es * (size_t)nr1 * na1, b2 = es * (size_t)nr2 * na2;
+        ST_HIP(hipMalloc(&p.d1, b1));
+        ST_HIP(hipMalloc(&p.d2, b2));
+        ST_HIP(hipMemcpyAsync(p.d1, img1, b1, hipMemcpyHostToDevice, c->stream));
+        ST_HIP(hipMemcpyAsync(p.d2, img2, b2, hipMemcpyHostToDevice, c->stream));
+        ST_HIP(hipStreamSynchronize(c->stream));
+        int id = c->nextPair++;
+        c->pairs[id] = p;
+        *pair = id;
+    });
+}
+
+extern "C" int st_pair_release(st_ctx *c, int32_t id)
+{
+    return st_guard(c, [&]() {
+        StPair &p = c->pair(id);
+        ST_HIP(hipStreamSynchronize(c->stream));
+        ST_HIP(hipFree(p.d1));
+        ST_HIP(hipFree(p.d2));
+        c->pairs.erase(id);
+    });
+}
+
+extern "C" int st_pair_download(st_ctx *c, int32_t id, int32_t row0, int32_t nrows, void *img1, void *img2)
+{
+    return st_guard(c, [&]() {
+        StPair &p = c->pair(id);
+        ST_REQUIRE(p.stype == ST_SAMPLE_C64, ST_ERR_UNSUPPORTED, "download of complex int16 pairs");
+        ST_REQUIRE(row0 >= 0 && nrows >= 0 && row0 + nrows <= p.na1 && row0 + nrows <= p.na2, ST_ERR_ARG,
+                   "row range outside the pair");
+        if (img1)
+            ST_HIP(hipMemcpyAsync(img1, (char *)p.d1 + (size_t)row0 * p.nr1 * 8, (size_t)nrows * p.nr1 * 8,
+                                  hipMemcpyDeviceToHost, c->stream));
+        if (img2)
+            ST_HIP(hipMemcpyAsync(img2, (char *)p.d2 + (size_t)row0 * p.nr2 * 8, (size_t)nrows * p.nr2 * 8,
+                                  hipMemcpyDeviceToHost, c->stream));
+        ST_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int st_speckle_track(st_ctx *c, int32_t pair, const st_track_params *par, st_track_out *out,
+                                st_track_diag *diag)
+{
+    return st_guard(c, [&]() {
+        ST_REQUIRE(par && out && out->offR && out->offA && out->corr && out->type, ST_ERR_ARG, "null argument");
+        st_speckle_track_impl(c, c->pair(pair), *par, out, diag);
+    });
+}
+
+extern "C" int st_corr_track_fast(st_ctx *c, int32_t pair, const st_track_params *par, st_track_out *out,
+                                  st_track_diag *diag)
+{
+    return st_guard(c, [&]() {
+        ST_REQUIRE(par && out && out->offR && out->offA && out->corr && out->type, ST_ERR_ARG, "null argument");
+        st_corr_track_fast_impl(c, c->pair(pair), *par, out, diag);
+    });
+}
+
+extern "C" int st_last_stats(st_ctx *c, st_track_stats *s)
+{
+    return st_guard(c, [&]() {
+        ST_REQUIRE(s, ST_ERR_ARG, "null stats");
+        *s = c->stats;
+    });
+}
+
+extern "C" int st_set_profiling(st_ctx *c, int32_t on)
+{
+    return st_guard(c, [&]() { c->profiling = on != 0; });
+}

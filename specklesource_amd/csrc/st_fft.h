@@ -1,0 +1,44 @@
+// Batched FP32 complex FFT engine for CDNA4 (no rocFFT).
+//
+// Rows of length n (n = 2^a 3^b 5^c 7^d <= 4096) are transformed in LDS by
+// a mixed-radix Stockham autosort (radices 8, 4, 2, 3, 5, 7), several rows
+// per workgroup.  2-D transforms are row passes plus tiled transposes.
+// Conventions follow FFTW 2, which the reference links: sign -1 is
+// exp(-2 pi i jk/n) ("forward"), +1 is exp(+2 pi i jk/n) ("backward"),
+// no normalisation.
+#pragma once
+
+#include <map>
+#include <utility>
+
+#include "st_common.h"
+
+namespace stfft {
+
+constexpr int kMaxStages = 14;
+
+struct PlanArgs {
+    int n, nst, rowsPerBlock;
+    int radix[kMaxStages], ns[kMaxStages], twoff[kMaxStages];
+    const float2 *tw;
+};
+
+class Engine {
+  public:
+    ~Engine();
+    const PlanArgs &plan(int n, int sign);
+    // In-place batched 1-D FFT of nrows contiguous rows of length n.
+    void rows(float2 *data, long nrows, int n, int sign, hipStream_t st);
+    // out[b][c][r] = in[b][r][c] for batch b of rows x cols arrays.
+    static void transpose(const float2 *in, float2 *out, int batch, int rows, int cols, hipStream_t st);
+    // In-place batched 2-D FFT of batch arrays na x nr (row-major); tmp must
+    // hold batch*na*nr elements.
+    void fft2d(float2 *data, float2 *tmp, int batch, int na, int nr, int sign, hipStream_t st);
+
+  private:
+    std::map<std::pair<int, int>, PlanArgs> plans_;
+};
+
+bool supported_size(int n);
+
+}  // namespace stfft

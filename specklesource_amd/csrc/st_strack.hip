@@ -1,0 +1,1207 @@
+// strack on the GPU: the decision cascade of speckleTrack
+// (Strack/speckleTrack.c:159-346) run as batched stages over work lists.
+//
+//   geometry   findImage2Pos / boundsCheck / maskValue for every point
+//   complex    getPatches -> cmpPSWithPad -> cmpTrackFast -> gaussPeak1D x2
+//              -> finalizeCmpxMatch                      (speckleTrack.c:349-394)
+//   amp small  getAmpPatches -> ampMatch -> ampTrackFast  (speckleTrack.c:398-597)
+//   amp large  the same with the window doubled (LA = 2)
+//   compose    offsets, corr, type, nFail                 (speckleTrack.c:301-323)
+//
+// Each attempt stage works on a compacted list of the points that reach it,
+// in batches sized to the scratch budget.  Per-patch FFTs go through the
+// batched LDS FFT engine; the peak stage fuses the fftshifted magnitude,
+// the scan-order argmax, the FP64 8x8 -> 96x96 refinement and the sample
+// extraction for the Gaussian fits.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "st_ctx.h"
+#include "st_track_common.h"
+
+using namespace sttrack;
+
+namespace sttrack {
+__constant__ double2 c_dir96[193];
+}
+
+namespace {
+
+struct KPar {
+    int nR, nA, rStart, aStart, deltaR, deltaA, wR, wA, wRa, wAa, sf;
+    int noComplex, hanning, maxTries, checkAz;
+    float azThresh;
+    double rPoly[3], aPoly[3];
+    const uint8_t *mask;
+    int maskNr, maskNa, maskR0, maskA0, maskNrl, maskNal;
+    int baseFlag;
+    double Bn, Bp, dBn, dBp, dBnQ, dBpQ, lambda;
+    int nSlpA1;
+    const double *sinT, *cosT, *overR;
+    const float *hann;
+    Images im;
+};
+
+// Per-point state, one entry per grid point (row-major nA x nR).
+struct Pts {
+    int *r1, *a1, *r2, *a2;
+    uint8_t *maskv, *haveData, *good, *type, *azValid;
+    int *azShift, *azUse;
+    float *corr, *rShift1, *aShift1, *azd;
+    int *peakI, *peakJ, *fineI, *fineJ; // 3 x npts
+    double *cmax;
+    float *sigRg, *sigAz, *peakGap, *fineGap;
+    int *counts;  // nComplex, nAmp, nAmpL, nFail, list sizes...
+};
+
+enum { CNT_COMPLEX = 0, CNT_AMP, CNT_AMPL, CNT_FAIL, CNT_LIST, CNT_CORR0, CNT_CORR1, CNT_CORR2, CNT_N };
+
+__device__ inline bool in_bounds(int r1, int a1, int wR, int wA, int sR, int sA)
+{
+    return r1 > 0 && (r1 + wR) < sR && a1 > 0 && (a1 + wA) < sA;
+}
+
+// ---------------------------------------------------------------------
+// geometry: findImage2Pos (polynomial) or host-computed positions,
+// maskValue, complex eligibility; initialises the point state.
+__global__ void k_geometry(KPar P, Pts S, const int *hostR2, const int *hostA2, int *list)
+{
+    const int npts = P.nR * P.nA;
+    const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pt >= npts)
+        return;
+    const int i = pt / P.nR, j = pt - i * P.nR;
+    const int a1 = P.aStart + i * P.deltaA - P.wA / 2;
+    const int r1 = P.rStart + j * P.deltaR - P.wR / 2;
+    int r2, a2;
+    if (hostR2) {
+        r2 = hostR2[pt];
+        a2 = hostA2[pt];
+    } else {
+        const double sc = (double)P.sf;
+        double rs = P.rPoly[0] + (double)(r1 * sc) * P.rPoly[1] + (double)(a1 * sc) * P.rPoly[2];
+        double as = P.aPoly[0] + (double)(r1 * sc) * P.aPoly[1] + (double)(a1 * sc) * P.aPoly[2];
+        rs /= sc;
+        as /= sc;
+        r2 = (rs >= 0) ? r1 + (int)(rs + 0.5) : r1 + (int)(rs - 0.5);
+        a2 = (as >= 0) ? a1 + (int)(as + 0.5) : a1 + (int)(as - 0.5);
+    }
+    int mv = 1;
+    if (P.mask) {
+        const int rr = r1 * P.sf, aa = a1 * P.sf;
+        const int ia1 = (aa + P.wA / 2 - P.maskA0) / P.maskNal;
+        const int ir1 = (rr + P.wR / 2 - P.maskR0) / P.maskNrl;
+        if (ia1 < 0 || ia1 >= P.maskNa || ir1 < 0 || ir1 >= P.maskNr)
+            mv = 0;
+        else
+            mv = P.mask[(size_t)ia1 * P.maskNr + ir1];
+        mv = mv < 1 ? mv : 1;
+    }
+    S.r1[pt] = r1;
+    S.a1[pt] = a1;
+    S.r2[pt] = r2;
+    S.a2[pt] = a2;
+    S.maskv[pt] = (uint8_t)mv;
+    S.haveData[pt] = 1;
+    S.good[pt] = 0;
+    S.type[pt] = 0;
+    S.azValid[pt] = 0;
+    S.azShift[pt] = 0;
+    S.azUse[pt] = 0;
+    S.corr[pt] = 0.f;
+    S.rShift1[pt] = 0.f;
+    S.aShift1[pt] = 0.f;
+    S.azd[pt] = 0.f;
+    for (int t = 0; t < 3; t++) {
+        const size_t o = (size_t)t * npts + pt;
+        S.peakI[o] = S.peakJ[o] = S.fineI[o] = S.fineJ[o] = -2000;
+        S.cmax[o] = 0;
+        S.sigRg[o] = S.sigAz[o] = S.peakGap[o] = S.fineGap[o] = 0.f;
+    }
+    const bool ok = in_bounds(r1, a1, P.wR, P.wA, P.im.nr1, P.im.na1) && in_bounds(r2, a2, P.wR, P.wA, P.im.nr2, P.im.na2);
+    if (ok && mv > 0 && !P.noComplex) {
+        int k = atomicAdd(&S.counts[CNT_LIST], 1);
+        list[k] = pt;
+    }
+}
+
+// Doppler estimate of a w x w patch pair (estDopCarrier1,
+// Strack/dopplerEstimation.c:25-88): lag-1 azimuth autocorrelation after
+// complex mean removal, in double.
+template <class Get>
+__device__ inline int doppler_shift(Get get, int wA, int wR, int nAz, int wAnorm, double *sh)
+{
+    const int N = wA * wR;
+    double m[4] = {0, 0, 0, 0};
+    for (int e = threadIdx.x; e < N; e += NT) {
+        float2 a, b;
+        get(e, a, b);
+        m[0] += a.x;
+        m[1] += a.y;
+        m[2] += b.x;
+        m[3] += b.y;
+    }
+    for (int k = 0; k < 4; k++)
+        m[k] = block_sum_d<NT>(m[k], sh) / ((double)wA * (double)wR);
+    double z[4] = {0, 0, 0, 0};
+    for (int e = threadIdx.x; e < N - wR; e += NT) {
+        float2 a0, b0, a1, b1;
+        get(e, a0, b0);
+        get(e + wR, a1, b1);
+        double ar = a1.x - m[0], ai = a1.y - m[1], br = a0.x - m[0], bi = a0.y - m[1];
+        z[0] += ar * br + ai * bi;
+        z[1] += ai * br - ar * bi;
+        ar = b1.x - m[2], ai = b1.y - m[3], br = b0.x - m[2], bi = b0.y - m[3];
+        z[2] += ar * br + ai * bi;
+        z[3] += ai * br - ar * bi;
+    }
+    for (int k = 0; k < 4; k++)
+        z[k] = block_sum_d<NT>(z[k], sh);
+    double ang1 = (z[0] == 0.0 && z[1] == 0.0) ? NAN : atan2(z[1], z[0]);
+    double ang2 = (z[2] == 0.0 && z[3] == 0.0) ? NAN : atan2(z[3], z[2]);
+    double v = (double)(nAz / 2) * (ang1 + ang2) * .5 / 3.14159265358979323846;
+    int shift = isnan(v) ? 0 : ((v >= 0.0) ? (int)(v + 0.5) : (int)(v - 0.5));
+    if (shift < 0)
+        shift += nAz;
+    return (int)((float)shift * (float)wAnorm / (float)nAz);
+}
+
+// ---------------------------------------------------------------------
+// complex stage: getPatches (speckleTrack.c:1194-1330) + Doppler +
+// phaseCorrectBaseline (speckleTrack.c:868-907).  One block per point.
+struct CpxBatch {
+    float2 *P;        // [2][B][w*w]: patch 1 then patch 2
+    float2 *S;        // [B][4w^2] padded cross spectrum / correlation
+    float2 *tmp;      // transpose scratch
+    float *mag;       // [B][4w^2]
+    PeakRes *res;     // [B]
+    float *sig;       // [B][2]
+    uint8_t *have;    // [B]
+    int *az;          // [B]
+    double *p;        // [B][2]
+};
+
+__global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *list, int off, int B, CpxBatch C)
+{
+    __shared__ double sh[NT / 64];
+    __shared__ int shi[NT / 64];
+    const int b = blockIdx.x;
+    const int pt = list[off + b];
+    const int w = P.wA, N = w * w;
+    const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
+    const bool isf = P.im.stype == ST_SAMPLE_C64;
+    const float thr = isf ? 1e-15f : 1e-9f;
+    const float thrAll = thr * P.wA * P.wR;
+    auto get = [&](int e, float2 &x, float2 &y) {
+        const int i = e / w, j = e - (e / w) * w;
+        x = load_sample(P.im.img1, P.im.stype, P.im.nr1, a1 + i, r1 + j);
+        y = load_sample(P.im.img2, P.im.stype, P.im.nr2, a2 + i, r2 + j);
+        if (isf) {
+            x.x *= 1000;
+            x.y *= 1000;
+            y.x *= 1000;
+            y.y *= 1000;
+        }
+        if (P.hanning) {
+            y.x *= P.hann[e];
+            y.y *= P.hann[e];
+        }
+    };
+    float2 *P1 = C.P + (size_t)b * N, *P2 = C.P + ((size_t)B + b) * N;
+    int nZero = 0;
+    double p1 = 0, p2 = 0;
+    for (int e = threadIdx.x; e < N; e += NT) {
+#pragma clang fp contract(off)
+        float2 x, y;
+        get(e, x, y);
+        const float pw1 = x.x * x.x + x.y * x.y, pw2 = y.x * y.x + y.y * y.y;
+        nZero += (pw1 < thr || pw2 < thr);
+        p1 += pw1;
+        p2 += pw2;
+        P2[e] = y;
+        if (P.baseFlag) {
+            const int i = e / w, j = e - (e / w) * w;
+            double xx = (a1 - P.wA / 2 - (P.nSlpA1 / 2)) / (double)P.nSlpA1;
+            xx += i * (1. / (double)P.nSlpA1);
+            double x0 = (a1 - P.wA / 2 - (P.nSlpA1 / 2)) / (double)P.nSlpA1;
+            double bn = P.Bn + x0 * P.dBn + x0 * x0 * P.dBnQ;
+            double bp = P.Bp + xx * P.dBp + xx * xx * P.dBpQ;
+            double bsq = bn * bn + bp * bp;
+            int j1 = (r1 - P.wR / 2 > 0 ? r1 - P.wR / 2 : 0) + j;
+            double ph = -bn * P.sinT[j1] - bp * P.cosT[j1] + (0.5 * bsq * P.overR[j1]);
+            ph *= -(4.0 * 3.14159265358979323846) / P.lambda;
+            double pr = cos(ph), pim = sin(ph);
+            double cr = x.x, ci = x.y;
+            x = make_float2((float)(cr * pr - ci * pim), (float)(cr * pim + ci * pr));
+        }
+        P1[e] = x;
+    }
+    nZero = block_sum_i<NT>(nZero, shi);
+    p1 = block_sum_d<NT>(p1, sh);
+    p2 = block_sum_d<NT>(p2, sh);
+    if (nZero > 20 || p1 < thrAll || p2 < thrAll) {
+        if (threadIdx.x == 0)
+            C.have[b] = 0;
+        return;
+    }
+    const int az = doppler_shift(get, w, w, w, P.wA, sh);
+    if (threadIdx.x == 0) {
+        C.have[b] = 1;
+        C.az[b] = az;
+        C.p[2 * b] = p1 / (double)(P.wA * P.wR);
+        C.p[2 * b + 1] = p2 / (double)(P.wA * P.wR);
+    }
+}
+
+// cmpPSWithPad (speckleTrack.c:604-643): F1 conj(F2) into the four corners
+// of the 2w x 2w spectrum, azimuth rows rotated by azShift, Nyquist on the
+// negative side.
+__global__ void k_cpx_pad(const float2 *P, int B, int w, const int *az, float2 *S)
+{
+    const int W = 2 * w, h = w / 2;
+    const int b = blockIdx.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= W * W)
+        return;
+    const int n = e / W, m = e - n * W;
+    float2 v = make_float2(0.f, 0.f);
+    const int k = n < h ? n : (n >= W - h ? n - W : 1 << 20);
+    const int l = m < h ? m : (m >= W - h ? m - W : 1 << 20);
+    if (k != (1 << 20) && l != (1 << 20)) {
+        const int ka = (k + w + az[b]) % w, la = (l + w) % w;
+        const float2 f1 = P[(size_t)b * w * w + ka * w + la];
+        const float2 f2 = P[((size_t)B + b) * w * w + ka * w + la];
+        v = cmulconj(f1, f2);
+    }
+    S[(size_t)b * W * W + e] = v;
+}
+
+// ---------------------------------------------------------------------
+// Peak stage (findCPeak speckleTrack.c:744-805, overSampleC :852-863,
+// argmax over the central box :686-700, sample extraction for
+// gaussPeak1D :2158-2180).  mode 0 = complex (Gaussian fits at the
+// refined peak / NOVER), 1 = amplitude (Gaussian fits at the coarse peak).
+__global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W, int mode, PeakRes *res)
+{
+    __shared__ ArgMax sha[NT / 64];
+    __shared__ ArgMaxD shd[NT / 64];
+    __shared__ float shf[NT / 64];
+    __shared__ double shdd[NT / 64];
+    __shared__ double Pm[NFAST * NFAST];
+    __shared__ double2 T[NFAST * kFine];
+    const int b = blockIdx.x;
+    const float2 *C = S + (size_t)b * W * W;
+    float *M = mag + (size_t)b * W * W;
+    const int H = W / 2;
+    ArgMax best{0.0f, 0x7fffffff};
+    for (int e = threadIdx.x; e < W * W; e += NT) {
+#pragma clang fp contract(off)
+        const int n = e / W, m = e - n * W;
+        const float2 z = C[e];
+        const float x2 = z.x * z.x + z.y * z.y;
+        const float v = (float)sqrt((double)x2);
+        const int I = n < H ? n + H : n - H, J = m < H ? m + H : m - H;
+        M[I * W + J] = v;
+        const int i = I < H ? I : I - H, j = J < H ? J : J - H;
+        const int q = (I < H) ? (J < H ? 0 : 3) : (J < H ? 2 : 1);
+        const int rank = (i * H + j) * 4 + q;
+        if (v > 0.0f) {
+            ArgMax c{v, rank};
+            best = argmax_combine(best, c);
+        }
+    }
+    best = block_argmax<NT>(best, sha);
+    __syncthreads();
+    int I = -1000, J = -1000;
+    float raw = 0.f;
+    if (best.rank != 0x7fffffff) {
+        const int q = best.rank & 3, ij = best.rank >> 2;
+        const int i = ij / H, j = ij - (ij / H) * H;
+        I = (q == 0 || q == 3) ? i : i + H;
+        J = (q == 0 || q == 2) ? j : j + H;
+        raw = best.v;
+    }
+    // runner-up on the coarse surface
+    float sec = 0.f;
+    const int bestIdx = (I >= 0) ? I * W + J : -1;
+    for (int e = threadIdx.x; e < W * W; e += NT)
+        if (e != bestIdx)
+            sec = fmaxf(sec, M[e]);
+    sec = block_max_f<NT>(sec, shf);
+    PeakRes r;
+    r.gap = raw > 0 ? (raw - sec) / raw : 0.f;
+    r.lmOk = 0;
+    r.fi = r.fj = -1;
+    r.fineMax = 0;
+    r.fineGap = 0;
+    if (I < NFAST / 2 || I > W - NFAST / 2 || J < NFAST / 2 || J > W - NFAST / 2) {
+        r.I = r.J = -1000;
+        r.raw = -1.f;
+        if (threadIdx.x == 0)
+            res[b] = r;
+        return;
+    }
+    r.I = I;
+    r.J = J;
+    r.raw = raw;
+    // FP64 refinement: Q = A P A^T on the searched 48 x 48 box
+    if (threadIdx.x < NFAST * NFAST)
+        Pm[threadIdx.x] = (double)M[(I - NFAST / 2 + threadIdx.x / NFAST) * W + (J - NFAST / 2 + threadIdx.x % NFAST)];
+    __syncthreads();
+    for (int t = threadIdx.x; t < NFAST * kFine; t += NT) {
+        const int mm = t / kFine, v = t - mm * kFine;
+        double re = 0, im = 0;
+        for (int nn = 0; nn < NFAST; nn++) {
+            const double2 d = c_dir96[(v + 24) - 12 * nn + 96];
+            const double p = Pm[mm * NFAST + nn];
+            re += p * d.x;
+            im += p * d.y;
+        }
+        T[t] = make_double2(re, im);
+    }
+    __syncthreads();
+    ArgMaxD fb{0.0, 0x7fffffff};
+    double qv[(kFine * kFine + NT - 1) / NT];
+    int nq = 0;
+    for (int t = threadIdx.x; t < kFine * kFine; t += NT) {
+        const int u = t / kFine, v = t - u * kFine;
+        double re = 0, im = 0;
+        for (int mm = 0; mm < NFAST; mm++) {
+            const double2 d = c_dir96[(u + 24) - 12 * mm + 96];
+            const double2 tv = T[mm * kFine + v];
+            re += d.x * tv.x - d.y * tv.y;
+            im += d.x * tv.y + d.y * tv.x;
+        }
+        const double a = sqrt(re * re + im * im);
+        qv[nq++] = a;
+        if (a > 0.0) {
+            ArgMaxD c{a, t};
+            fb = argmaxd_combine(fb, c);
+        }
+    }
+    fb = block_argmax_d<NT>(fb, shd);
+    double fsec = 0;
+    nq = 0;
+    for (int t = threadIdx.x; t < kFine * kFine; t += NT, nq++)
+        if (t != fb.rank)
+            fsec = fmax(fsec, qv[nq]);
+    fsec = block_max_d<NT>(fsec, shdd);
+    if (fb.rank != 0x7fffffff) {
+        r.fi = fb.rank / kFine;
+        r.fj = fb.rank - r.fi * kFine;
+        r.fineMax = fb.v;
+        r.fineGap = (float)((fb.v - fsec) / fb.v);
+    } else {
+        r.fi = r.fj = 0;
+        r.fineMax = 0;
+    }
+    // samples for the two Gaussian fits
+    int cI, cJ;
+    bool ok;
+    if (mode == 0) {
+        const int iF = I * NOVER + (r.fi + 24) - (NOVER * NFAST) / 2;
+        const int jF = J * NOVER + (r.fj + 24) - (NOVER * NFAST) / 2;
+        cI = iF / NOVER;
+        cJ = jF / NOVER;
+        ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < W && (cJ - NGAUSS / 2) >= 0 && (cJ + NGAUSS / 2) < W;
+    } else {
+        cI = I;
+        cJ = J;
+        ok = I > NGAUSS / 2 && I < W - NGAUSS / 2 && J > NGAUSS / 2 && J < W - NGAUSS / 2;
+    }
+    r.lmOk = ok ? 1 : 0;
+    if (ok) {
+        for (int t = 0; t < NGAUSS; t++) {
+            r.lmY[0][t] = M[cI * W + (cJ - NGAUSS / 2 + t)];
+            r.lmY[1][t] = M[(cI - NGAUSS / 2 + t) * W + cJ];
+        }
+        r.lmX0[0] = (float)(cJ - NGAUSS / 2);
+        r.lmX0[1] = (float)(cI - NGAUSS / 2);
+    }
+    if (threadIdx.x == 0)
+        res[b] = r;
+}
+
+// Two fits per attempt: [0] along range, [1] along azimuth.
+__global__ void k_lm(const PeakRes *res, int B, float *sig)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * B)
+        return;
+    const int b = t >> 1, k = t & 1;
+    const PeakRes &r = res[b];
+    sig[t] = r.lmOk ? gauss_peak_1d(r.lmY[k], r.lmX0[k]) : 0.f;
+}
+
+// complexMatch tail + finalizeCmpxMatch (speckleTrack.c:377-394,714-739).
+__global__ void k_cpx_finalize(KPar P, Pts S, const int *list, int off, int B, CpxBatch C)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B)
+        return;
+    const int pt = list[off + b];
+    const int npts = P.nR * P.nA;
+    if (!C.have[b]) {
+        S.haveData[pt] = 0;
+        return;
+    }
+    S.azValid[pt] = 1;
+    S.azShift[pt] = C.az[b];
+    atomicAdd(&S.counts[CNT_CORR0], 1);
+    const PeakRes &r = C.res[b];
+    const double p1 = C.p[2 * b], p2 = C.p[2 * b + 1];
+    double cMax;
+    int iMax, jMax;
+    if (r.I < -999) {
+        cMax = -1.0;
+        iMax = jMax = -1000;
+    } else {
+        const double hc = P.hanning ? 1.5 : 1.0;
+        cMax = r.fineMax / (NFAST * NFAST);
+        cMax = hc * cMax / (sqrt(p1 * p2) * pow((double)(P.wR * P.wA), 2));
+        iMax = r.I * NOVER + (r.fi + 24) - (NOVER * NFAST) / 2;
+        jMax = r.J * NOVER + (r.fj + 24) - (NOVER * NFAST) / 2;
+    }
+    float azd = -2.0e9f;
+    int good = 0;
+    float sRg = C.sig[2 * b], sAz = C.sig[2 * b + 1];
+    if (r.lmOk) {
+        azd = sAz / sRg;
+        if (fabsf(azd) < P.azThresh || !P.checkAz) {
+            const double invNOVER = 1.0 / (float)NOVER;
+            const double wRo = P.wR / 2. * NOVER * 2, wAo = P.wA / 2. * NOVER * 2;
+            if (cMax > .185 && !P.noComplex) {
+                float rs = (float)(((float)jMax - wRo) * invNOVER / 2);
+                float as = (float)(((float)iMax - wAo) * invNOVER / 2);
+                if (fabs((double)rs) < 0.125 * P.wR * 2 && fabs((double)as) < 0.125 * P.wA * 2) {
+                    good = 1;
+                    S.rShift1[pt] = rs;
+                    S.aShift1[pt] = as;
+                    atomicAdd(&S.counts[CNT_COMPLEX], 1);
+                }
+            }
+        }
+    }
+    S.azd[pt] = azd;
+    S.corr[pt] = (float)cMax;
+    if (good) {
+        S.good[pt] = 1;
+        S.type[pt] = ST_CMATCH;
+    }
+    S.peakI[pt] = r.I;
+    S.peakJ[pt] = r.J;
+    S.fineI[pt] = iMax;
+    S.fineJ[pt] = jMax;
+    S.cmax[pt] = cMax;
+    S.sigRg[pt] = r.lmOk ? sRg : 0.f;
+    S.sigAz[pt] = r.lmOk ? sAz : 0.f;
+    S.peakGap[pt] = r.gap;
+    S.fineGap[pt] = r.fineGap;
+    (void)npts;
+}
+
+// Carry the last estimated azShift forward in raster order (the
+// reference's trackPar->azShift persists between points).
+__global__ void k_carry_az(Pts S, int npts)
+{
+    __shared__ int last[1024];
+    __shared__ int has[1024];
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int chunk = (npts + nt - 1) / nt;
+    const int lo = t * chunk, hi = imin(npts, lo + chunk);
+    int lv = 0, lh = 0;
+    for (int p = lo; p < hi; p++)
+        if (S.azValid[p]) {
+            lv = S.azShift[p];
+            lh = 1;
+        }
+    last[t] = lv;
+    has[t] = lh;
+    __syncthreads();
+    if (t == 0) {
+        int cur = 0;
+        for (int k = 0; k < nt; k++) {
+            int v = last[k], h = has[k];
+            last[k] = cur;  // value entering chunk k
+            if (h)
+                cur = v;
+        }
+    }
+    __syncthreads();
+    int cur = last[t];
+    for (int p = lo; p < hi; p++) {
+        if (S.azValid[p])
+            cur = S.azShift[p];
+        S.azUse[p] = cur;
+    }
+}
+
+// Work list of the amplitude tries (the while loop of speckleTrack.c:246).
+__global__ void k_amp_list(KPar P, Pts S, int npts, int *list)
+{
+    const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pt >= npts)
+        return;
+    if (!S.good[pt] && S.haveData[pt] && S.maskv[pt] > 0) {
+        int k = atomicAdd(&S.counts[CNT_LIST], 1);
+        list[k] = pt;
+    }
+}
+
+// ---------------------------------------------------------------------
+// amplitude stages (getAmpPatches speckleTrack.c:1006-1189, ampMatch
+// :398-484, ampTrackFast :489-597)
+struct AmpBatch {
+    float2 *I;     // [2][B][ws^2]
+    float2 *U;     // [2][B][4 ws^2]
+    float2 *S;     // [B][4 ws^2]
+    float2 *tmp;
+    float *mag;
+    PeakRes *res;
+    float *sig;
+    uint8_t *state;  // 0 = bounds failed, 1 = no data, 2 = matched
+    int *az;
+    double *p;
+};
+
+__global__ void __launch_bounds__(NT) k_amp_gather(KPar P, Pts S, const int *list, int off, int B, int large,
+                                                    AmpBatch A)
+{
+    __shared__ double sh[NT / 64];
+    const int b = blockIdx.x;
+    const int pt = list[off + b];
+    const int sc = large ? LA : 1;
+    const int ws = P.wAa * sc;
+    const int i = pt / P.nR, j = pt - (pt / P.nR) * P.nR;
+    const int a1 = P.aStart + i * P.deltaA - P.wA / 2 * sc;
+    const int r1 = P.rStart + j * P.deltaR - P.wR / 2 * sc;
+    const int r2 = r1 + (S.r2[pt] - S.r1[pt]), a2 = a1 + (S.a2[pt] - S.a1[pt]);
+    const bool ok = in_bounds(r1, a1, P.wRa * sc, P.wAa * sc, P.im.nr1, P.im.na1) &&
+                    in_bounds(r2, a2, P.wRa * sc, P.wAa * sc, P.im.nr2, P.im.na2);
+    if (!ok) {
+        if (threadIdx.x == 0)
+            A.state[b] = 0;
+        return;
+    }
+    const int N = ws * ws;
+    float2 *I1 = A.I + (size_t)b * N, *I2 = A.I + ((size_t)B + b) * N;
+    for (int e = threadIdx.x; e < N; e += NT) {
+        const int ii = e / ws, jj = e - (e / ws) * ws;
+        I1[e] = load_sample(P.im.img1, P.im.stype, P.im.nr1, a1 + ii, r1 + jj);
+        I2[e] = load_sample(P.im.img2, P.im.stype, P.im.nr2, a2 + ii, r2 + jj);
+    }
+    int azUse = S.azUse[pt];
+    if (P.noComplex && !large) {
+        const int wsA = P.wAa;
+        auto get = [&](int e, float2 &x, float2 &y) {
+            const int ii = e / wsA, jj = e - (e / wsA) * wsA;
+            x = load_sample(P.im.img1, P.im.stype, P.im.nr1, a1 + ii, r1 + jj);
+            y = load_sample(P.im.img2, P.im.stype, P.im.nr2, a2 + ii, r2 + jj);
+        };
+        azUse = doppler_shift(get, P.wAa, P.wRa, P.wAa, P.wA, sh);
+        if (threadIdx.x == 0)
+            S.azUse[pt] = azUse;
+    }
+    if (threadIdx.x == 0) {
+        A.state[b] = 2;
+        A.az[b] = (int)((float)azUse * (float)ws / (float)P.wA);
+    }
+}
+
+// zeroPadFFT (speckleTrack.c:2014-2038) of both spectra into 2ws x 2ws.
+__global__ void k_amp_pad(const float2 *I, int B, int ws, const int *az, const uint8_t *state, float2 *U)
+{
+    const int W = 2 * ws, h = ws / 2;
+    const int b = blockIdx.y % B, which = blockIdx.y / B;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= W * W || state[b] != 2)
+        return;
+    const int n = e / W, m = e - n * W;
+    float2 v = make_float2(0.f, 0.f);
+    const int k = n < h ? n : (n >= W - h ? n - W : 1 << 20);
+    const int l = m < h ? m : (m >= W - h ? m - W : 1 << 20);
+    if (k != (1 << 20) && l != (1 << 20)) {
+        const int ka = (k + ws + az[b]) % ws, la = (l + ws) % ws;
+        v = I[((size_t)which * B + b) * ws * ws + ka * ws + la];
+    }
+    U[((size_t)which * B + b) * W * W + e] = v;
+}
+
+// Detection, zero test, amplitude, mean removal and power
+// (speckleTrack.c:1166-1188, 437-467).
+__global__ void __launch_bounds__(NT) k_amp_detect(KPar P, int B, int ws, int large, AmpBatch A)
+{
+    __shared__ double sh[NT / 64];
+    __shared__ int shi[NT / 64];
+    const int b = blockIdx.x;
+    if (A.state[b] != 2)
+        return;
+    const int W = 2 * ws, N = W * W;
+    float2 *U1 = A.U + (size_t)b * N, *U2 = A.U + ((size_t)B + b) * N;
+    const size_t sws = (size_t)ws;
+    const float scale = (float)(1. / (double)(sws * sws * sws * sws * 8));
+    const float thr = P.im.stype == ST_SAMPLE_C64 ? 1.0e-15f : 1.0e-6f;
+    int nZero = 0;
+    double s1 = 0, s2 = 0;
+    for (int e = threadIdx.x; e < N; e += NT) {
+#pragma clang fp contract(off)
+        const float2 u = U1[e], v = U2[e];
+        const float x1 = (u.x * u.x + u.y * u.y) * scale, x2 = (v.x * v.x + v.y * v.y) * scale;
+        nZero += (x1 < thr || x2 < thr);
+        s1 += x1;
+        s2 += x2;
+        U1[e] = make_float2(x1, 0.f);
+        U2[e] = make_float2(x2, 0.f);
+    }
+    nZero = block_sum_i<NT>(nZero, shi);
+    s1 = block_sum_d<NT>(s1, sh);
+    s2 = block_sum_d<NT>(s2, sh);
+    if (nZero > 20 || s1 < 1.0e-6 || s2 < 1.0e-6) {
+        if (threadIdx.x == 0)
+            A.state[b] = 1;
+        return;
+    }
+    double m1 = 0, m2 = 0;
+    for (int e = threadIdx.x; e < N; e += NT) {
+        const float v1 = (float)sqrt((double)U1[e].x), v2 = (float)sqrt((double)U2[e].x);
+        U1[e].x = v1;
+        U2[e].x = v2;
+        m1 += v1;
+        m2 += v2;
+    }
+    m1 = block_sum_d<NT>(m1, sh) / (double)N;
+    m2 = block_sum_d<NT>(m2, sh) / (double)N;
+    double p1 = 0, p2 = 0;
+    for (int e = threadIdx.x; e < N; e += NT) {
+#pragma clang fp contract(off)
+        float v1 = U1[e].x, v2 = U2[e].x;
+        if (v1 > 1.0e-20)
+            v1 = (float)(v1 - m1);
+        v2 = (float)(v2 - m2);
+        U1[e].x = v1;
+        U2[e].x = v2;
+        p1 += v1 * v1;
+        p2 += v2 * v2;
+    }
+    p1 = block_sum_d<NT>(p1, sh);
+    p2 = block_sum_d<NT>(p2, sh);
+    if (threadIdx.x == 0) {
+        A.p[2 * b] = p1 / (double)N;
+        A.p[2 * b + 1] = p2 / (double)N;
+    }
+}
+
+__global__ void k_amp_cross(const float2 *U, int B, int N, const uint8_t *state, float2 *S)
+{
+    const int b = blockIdx.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N || state[b] != 2)
+        return;
+    const float2 a = U[(size_t)b * N + e], c = U[((size_t)B + b) * N + e];
+    S[(size_t)b * N + e] = cmulconj(a, c);
+}
+
+// ampTrackFast tail and the driver's acceptance test
+// (speckleTrack.c:537-597, 272-298).
+__global__ void k_amp_finalize(KPar P, Pts S, const int *list, int off, int B, int large, AmpBatch A)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B)
+        return;
+    const int pt = list[off + b];
+    const int npts = P.nR * P.nA;
+    const int t = large ? 2 : 1;
+    const size_t o = (size_t)t * npts + pt;
+    if (A.state[b] == 0) {
+        S.corr[pt] = 0.f;
+        return;
+    }
+    atomicAdd(&S.counts[CNT_CORR0 + t], 1);
+    if (A.state[b] == 1) {
+        S.haveData[pt] = 0;
+        S.corr[pt] = 0.f;
+        return;
+    }
+    const int sc = large ? LA : 1;
+    const int W = 2 * P.wAa * sc;
+    const PeakRes &r = A.res[b];
+    float sRg = A.sig[2 * b], sAz = A.sig[2 * b + 1];
+    if (r.lmOk)
+        S.azd[pt] = sAz / sRg;
+    const float azdNow = S.azd[pt];
+    double cMax;
+    int iMax, jMax;
+    if (r.I < -999 || (fabsf(azdNow) >= P.azThresh && P.checkAz)) {
+        cMax = r.I < -999 ? -1.0 : (double)r.raw;
+        iMax = r.I;
+        jMax = r.J;
+    } else {
+        const double p1 = A.p[2 * b], p2 = A.p[2 * b + 1];
+        cMax = r.fineMax / (pow((double)W, 2) * pow((double)W, 2) * pow((double)NFAST, 2));
+        cMax = cMax / sqrt(p1 * p2);
+        iMax = r.I * NOVER + (r.fi + 24) - (NOVER * NFAST) / 2;
+        jMax = r.J * NOVER + (r.fj + 24) - (NOVER * NFAST) / 2;
+    }
+    const double invNOVER = 1.0 / (float)NOVER;
+    const float rs = (float)(((float)jMax - P.wRa * 0.5 * 2 * NOVER * sc) * invNOVER / 2);
+    const float as = (float)(((float)iMax - P.wAa * 0.5 * 2 * NOVER * sc) * invNOVER / 2);
+    const double thr = large ? 0.028 : 0.07;
+    S.corr[pt] = (float)cMax;
+    if (fabs((double)rs) < 0.15 * P.wRa * 2 * sc && fabs((double)as) < 0.15 * P.wAa * 2 * sc && cMax > thr) {
+        S.good[pt] = 1;
+        S.type[pt] = large ? ST_AMPMATCHLARGE : ST_AMPMATCH;
+        S.rShift1[pt] = rs;
+        S.aShift1[pt] = as;
+        atomicAdd(&S.counts[large ? CNT_AMPL : CNT_AMP], 1);
+    }
+    S.peakI[o] = r.I;
+    S.peakJ[o] = r.J;
+    S.fineI[o] = iMax;
+    S.fineJ[o] = jMax;
+    S.cmax[o] = cMax;
+    S.sigRg[o] = r.lmOk ? sRg : 0.f;
+    S.sigAz[o] = r.lmOk ? sAz : 0.f;
+    S.peakGap[o] = r.gap;
+    S.fineGap[o] = r.fineGap;
+    (void)W;
+}
+
+// Final offsets (speckleTrack.c:301-323).
+__global__ void k_compose(KPar P, Pts S, float *offR, float *offA, int npts)
+{
+    const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pt >= npts)
+        return;
+    float rs, as;
+    if (S.good[pt] && S.maskv[pt] > 0) {
+        rs = (float)(S.r2[pt] - S.r1[pt]) - S.rShift1[pt];
+        as = (float)(S.a2[pt] - S.a1[pt]) - S.aShift1[pt];
+    } else {
+        rs = -2.0e9f;
+        as = -2.0e9f;
+        if (S.haveData[pt])
+            atomicAdd(&S.counts[CNT_FAIL], 1);
+    }
+    if (rs > -100000) {
+        rs *= (float)P.sf;
+        as *= (float)P.sf;
+    }
+    offR[pt] = rs;
+    offA[pt] = as;
+}
+
+// ---------------------------------------------------------------------
+// host side
+
+void host_hanning(int wA, int wR, std::vector<float> &h)
+{
+    // computeHanning, speckleTrack.c:1830-1852 (square windows)
+    h.assign((size_t)wA * wR, 0.f);
+    for (int i = 0; i < wA / 2; i++) {
+        int i1 = wA / 2 + i, i2 = wA / 2 - i - 1;
+        double wi = 0.5 * (1.0 + cos((2.0 * M_PI) * (double)i / (double)(wA - 1)));
+        for (int j = 0; j < wR / 2; j++) {
+            int j1 = wA / 2 + j, j2 = wA / 2 - j - 1;
+            double wj = 0.5 * (1.0 + cos((2.0 * M_PI) * (double)j / (double)(wR - 1)));
+            float v = (float)(wi * wj);
+            h[(size_t)i1 * wR + j1] = v;
+            h[(size_t)i2 * wR + j2] = v;
+            h[(size_t)i2 * wR + j1] = v;
+            h[(size_t)i1 * wR + j2] = v;
+        }
+    }
+}
+
+// findImage2Pos with an initial-offset grid (speckleTrack.c:810-847,
+// 1857-1940), evaluated on the host because points without a value keep
+// the previous point's position.
+void host_positions(const st_track_params &p, std::vector<int> &r2v, std::vector<int> &a2v)
+{
+    const int npts = p.nR * p.nA;
+    r2v.resize(npts);
+    a2v.resize(npts);
+    int r2 = 0, a2 = 0;
+    const float minvalue = (float)-ST_LARGEINT * 0.9f;
+    for (int i = 0; i < p.nA; i++) {
+        int a1 = p.aStart + i * p.deltaA - p.wA / 2;
+        for (int j = 0; j < p.nR; j++) {
+            int r1 = p.rStart + j * p.deltaR - p.wR / 2;
+            double sc = (double)p.scaleFactor;
+            double dr = -ST_LARGEINT, da = -ST_LARGEINT;
+            bool have = false;
+            {
+                double range = (r1 - p.initR0) / p.initDeltaR, azimuth = (a1 - p.initA0) / p.initDeltaA;
+                if (!(range < 0.0 || azimuth < 0.0 || (int)range >= p.initNr || (int)azimuth >= p.initNa)) {
+                    const float *rim = p.initDr, *aim = p.initDa;
+                    int nr = p.initNr, jj = (int)range, ii = (int)azimuth;
+                    if (jj == p.initNr - 1 || ii == p.initNa - 1) {
+                        dr = rim[ii * nr + jj];
+                        da = aim[ii * nr + jj];
+                        if (dr < minvalue || da < minvalue)
+                            dr = da = -ST_LARGEINT;
+                    } else {
+                        float t = (float)(range - (double)jj), u = (float)(azimuth - (double)ii);
+                        float q1 = rim[ii * nr + jj], q2 = rim[ii * nr + jj + 1], q3 = rim[(ii + 1) * nr + jj + 1],
+                              q4 = rim[(ii + 1) * nr + jj];
+                        bool bad = false;
+                        if (q1 <= minvalue || q2 <= minvalue || q3 <= minvalue || q4 <= minvalue) {
+                            dr = fmaxf(fmaxf(fmaxf(q1, q2), q3), q4);
+                            if (dr < minvalue) {
+                                dr = da = -ST_LARGEINT;
+                                bad = true;
+                            }
+                        } else
+                            dr = (float)((1.0 - t) * (1.0 - u) * q1 + t * (1.0 - u) * q2 + t * u * q3 + (1.0 - t) * u * q4);
+                        if (!bad) {
+                            q1 = aim[ii * nr + jj];
+                            q2 = aim[ii * nr + jj + 1];
+                            q3 = aim[(ii + 1) * nr + jj + 1];
+                            q4 = aim[(ii + 1) * nr + jj];
+                            if (q1 <= minvalue || q2 <= minvalue || q3 <= minvalue || q4 <= minvalue) {
+                                da = fmaxf(fmaxf(fmaxf(q1, q2), q3), q4);
+                                if (da < minvalue)
+                                    dr = da = -ST_LARGEINT;
+                            } else
+                                da = (float)((1.0 - t) * (1.0 - u) * q1 + t * (1.0 - u) * q2 + t * u * q3 +
+                                             (1.0 - t) * u * q4);
+                        }
+                    }
+                }
+            }
+            if (dr > (-ST_LARGEINT + 1))
+                have = true;
+            if (have) {
+                dr /= sc;
+                da /= sc;
+                r2 = (dr >= 0) ? r1 + (int)(dr + 0.5) : r1 + (int)(dr - 0.5);
+                a2 = (da >= 0) ? a1 + (int)(da + 0.5) : a1 + (int)(da - 0.5);
+            }
+            r2v[(size_t)i * p.nR + j] = r2;
+            a2v[(size_t)i * p.nR + j] = a2;
+        }
+    }
+}
+
+void init_dirichlet(st_ctx *ctx)
+{
+    static int done_dev = -1;
+    if (done_dev == ctx->device)
+        return;
+    double2 tab[193];
+    for (int t = -96; t <= 96; t++) {
+        double re = 0, im = 0;
+        for (int k = -4; k <= 3; k++) {
+            double ang = 2.0 * M_PI * (double)k * (double)t / 96.0;
+            re += cos(ang);
+            im += sin(ang);
+        }
+        tab[t + 96] = make_double2(re, im);
+    }
+    ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dir96), tab, sizeof(tab)));
+    done_dev = ctx->device;
+}
+
+template <class T>
+T *upload(st_ctx *ctx, const T *h, size_t n)
+{
+    T *d = ctx->arena.take<T>(n);
+    ST_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return d;
+}
+
+int read_count(st_ctx *ctx, int *dcounts, int idx)
+{
+    int v = 0;
+    ST_HIP(hipMemcpyAsync(&v, dcounts + idx, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    return v;
+}
+
+}  // namespace
+
+void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, st_track_out *out,
+                           st_track_diag *diag)
+{
+    ST_REQUIRE(!par.gaussFlag, ST_ERR_UNSUPPORTED, "-gauss peak mode is not implemented");
+    ST_REQUIRE(par.wR == par.wA && par.wRa == par.wAa, ST_ERR_UNSUPPORTED, "non-square windows");
+    ST_REQUIRE(par.wR >= 8 && par.wRa >= 8 && !(par.wR & 1) && !(par.wRa & 1), ST_ERR_ARG, "window sizes");
+    ST_REQUIRE(stfft::supported_size(par.wR) && stfft::supported_size(2 * par.wR) &&
+                   stfft::supported_size(par.wRa * LA) && stfft::supported_size(4 * par.wRa),
+               ST_ERR_UNSUPPORTED, "window size not supported by the FFT engine");
+    ST_REQUIRE(par.nR > 0 && par.nA > 0 && par.scaleFactor > 0, ST_ERR_ARG, "grid");
+    ST_REQUIRE(par.polyShift || (par.initDr && par.initDa && par.initNr > 0 && par.initNa > 0), ST_ERR_ARG,
+               "initial offsets: polyShift=0 needs initDr/initDa");
+    hipStream_t st = ctx->stream;
+    init_dirichlet(ctx);
+    ctx->stats_reset();
+    hipEvent_t *sev = ctx->ev + 64;  // stage boundary events
+    ST_HIP(hipEventRecord(sev[0], st));
+    const int kGather = ctx->kernel_slot("cpx_gather"), kFft = ctx->kernel_slot("fft2d"),
+              kPeak = ctx->kernel_slot("peak_refine"), kAmpDet = ctx->kernel_slot("amp_detect");
+    const int npts = par.nR * par.nA;
+    const int w = par.wR;
+
+    // per-point state + inputs
+    std::vector<float> hann;
+    host_hanning(par.wA, par.wR, hann);
+    std::vector<int> hr2, ha2;
+    if (!par.polyShift)
+        host_positions(par, hr2, ha2);
+    size_t fixed = (size_t)npts * 160 + hann.size() * 4 + (hr2.size() * 8) + 4096;
+    if (par.mask)
+        fixed += (size_t)par.maskNr * par.maskNa;
+    if (par.baseFlag)
+        fixed += (size_t)pr.nr1 * 24;
+    // batch sizes
+    const size_t perCpx = (size_t)w * w * (16 + 32 + 32 + 16) + 2048;
+    int ws_max = par.wAa * (par.maxTries >= 2 ? LA : 1);
+    const size_t perAmp = (size_t)ws_max * ws_max * 192 + 2048;
+    size_t budget = ctx->scratchBudget;
+    const size_t kMaxBatch = 8192;  // keeps 2 x batch inside the 65535 grid-z limit of the transposes
+    int Bc = (int)std::max<size_t>(1, std::min<size_t>({(size_t)npts, budget / perCpx, kMaxBatch}));
+    int Ba = (int)std::max<size_t>(1, std::min<size_t>({(size_t)npts, budget / perAmp, kMaxBatch}));
+    size_t need = fixed + std::max((size_t)Bc * perCpx, (size_t)Ba * perAmp) + 65536;
+    ctx->arena_reserve(need);
+    Arena &ar = ctx->arena;
+
+    Pts S;
+    S.r1 = ar.take<int>(npts);
+    S.a1 = ar.take<int>(npts);
+    S.r2 = ar.take<int>(npts);
+    S.a2 = ar.take<int>(npts);
+    S.maskv = ar.take<uint8_t>(npts);
+    S.haveData = ar.take<uint8_t>(npts);
+    S.good = ar.take<uint8_t>(npts);
+    S.type = ar.take<uint8_t>(npts);
+    S.azValid = ar.take<uint8_t>(npts);
+    S.azShift = ar.take<int>(npts);
+    S.azUse = ar.take<int>(npts);
+    S.corr = ar.take<float>(npts);
+    S.rShift1 = ar.take<float>(npts);
+    S.aShift1 = ar.take<float>(npts);
+    S.azd = ar.take<float>(npts);
+    S.peakI = ar.take<int>(3 * (size_t)npts);
+    S.peakJ = ar.take<int>(3 * (size_t)npts);
+    S.fineI = ar.take<int>(3 * (size_t)npts);
+    S.fineJ = ar.take<int>(3 * (size_t)npts);
+    S.cmax = ar.take<double>(3 * (size_t)npts);
+    S.sigRg = ar.take<float>(3 * (size_t)npts);
+    S.sigAz = ar.take<float>(3 * (size_t)npts);
+    S.peakGap = ar.take<float>(3 * (size_t)npts);
+    S.fineGap = ar.take<float>(3 * (size_t)npts);
+    S.counts = ar.take<int>(CNT_N);
+    int *list = ar.take<int>(npts);
+    float *dOffR = ar.take<float>(npts), *dOffA = ar.take<float>(npts);
+
+    KPar P{};
+    P.nR = par.nR;
+    P.nA = par.nA;
+    P.rStart = par.rStart;
+    P.aStart = par.aStart;
+    P.deltaR = par.deltaR;
+    P.deltaA = par.deltaA;
+    P.wR = par.wR;
+    P.wA = par.wA;
+    P.wRa = par.wRa;
+    P.wAa = par.wAa;
+    P.sf = par.scaleFactor;
+    P.noComplex = par.noComplex;
+    P.hanning = par.hanningFlag;
+    P.maxTries = par.maxTries;
+    P.checkAz = par.checkAzFocus;
+    P.azThresh = par.azDefocusThresh;
+    for (int k = 0; k < 3; k++) {
+        P.rPoly[k] = par.rShiftPoly[k];
+        P.aPoly[k] = par.aShiftPoly[k];
+    }
+    P.mask = par.mask ? upload(ctx, par.mask, (size_t)par.maskNr * par.maskNa) : nullptr;
+    P.maskNr = par.maskNr;
+    P.maskNa = par.maskNa;
+    P.maskR0 = par.maskR0;
+    P.maskA0 = par.maskA0;
+    P.maskNrl = par.maskNrl > 0 ? par.maskNrl : 1;
+    P.maskNal = par.maskNal > 0 ? par.maskNal : 1;
+    P.baseFlag = par.baseFlag;
+    if (par.baseFlag) {
+        ST_REQUIRE(par.sinThetaD && par.cosThetaD && par.overRange && par.nSlpA1 > 0 && par.lambda != 0, ST_ERR_ARG,
+                   "baseline correction needs the range tables");
+        P.Bn = par.Bn;
+        P.Bp = par.Bp;
+        P.dBn = par.dBn;
+        P.dBp = par.dBp;
+        P.dBnQ = par.dBnQ;
+        P.dBpQ = par.dBpQ;
+        P.lambda = par.lambda;
+        P.nSlpA1 = par.nSlpA1;
+        P.sinT = upload(ctx, par.sinThetaD, (size_t)pr.nr1);
+        P.cosT = upload(ctx, par.cosThetaD, (size_t)pr.nr1);
+        P.overR = upload(ctx, par.overRange, (size_t)pr.nr1);
+    }
+    P.hann = upload(ctx, hann.data(), hann.size());
+    P.im = Images{pr.d1, pr.d2, pr.nr1, pr.na1, pr.nr2, pr.na2, pr.stype};
+    const int *dR2 = nullptr, *dA2 = nullptr;
+    if (!par.polyShift) {
+        dR2 = upload(ctx, hr2.data(), hr2.size());
+        dA2 = upload(ctx, ha2.data(), ha2.size());
+    }
+    const size_t mark = ar.used;
+
+    ST_HIP(hipMemsetAsync(S.counts, 0, CNT_N * sizeof(int), st));
+    hipLaunchKernelGGL(k_geometry, dim3((npts + 255) / 256), dim3(256), 0, st, P, S, dR2, dA2, list);
+    ST_HIP(hipGetLastError());
+    ST_HIP(hipEventRecord(sev[1], st));
+    const int nCpx = read_count(ctx, S.counts, CNT_LIST);
+
+    // ---- complex attempts
+    if (nCpx > 0) {
+        const int B = std::min(Bc, nCpx);
+        CpxBatch C;
+        C.P = ar.take<float2>((size_t)2 * B * w * w);
+        C.S = ar.take<float2>((size_t)B * 4 * w * w);
+        C.tmp = ar.take<float2>((size_t)B * 4 * w * w);
+        C.mag = ar.take<float>((size_t)B * 4 * w * w);
+        C.res = ar.take<PeakRes>(B);
+        C.sig = ar.take<float>(2 * (size_t)B);
+        C.have = ar.take<uint8_t>(B);
+        C.az = ar.take<int>(B);
+        C.p = ar.take<double>(2 * (size_t)B);
+        for (int off = 0; off < nCpx; off += B) {
+            const int nb = std::min(B, nCpx - off);
+            ctx->time_begin(kGather);
+            hipLaunchKernelGGL(k_cpx_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, C);
+            ctx->time_end(kGather);
+            ctx->time_begin(kFft);
+            ctx->fft.fft2d(C.P, C.tmp, 2 * nb, w, w, -1, st);
+            ctx->time_end(kFft);
+            hipLaunchKernelGGL(k_cpx_pad, dim3((4 * w * w + 255) / 256, nb), dim3(256), 0, st, C.P, nb, w, C.az, C.S);
+            ctx->time_begin(kFft);
+            ctx->fft.fft2d(C.S, C.tmp, nb, 2 * w, 2 * w, +1, st);
+            ctx->time_end(kFft);
+            ctx->time_begin(kPeak);
+            hipLaunchKernelGGL(k_peak, dim3(nb), dim3(NT), 0, st, C.S, C.mag, 2 * w, 0, C.res);
+            ctx->time_end(kPeak);
+            hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, C.res, nb, C.sig);
+            hipLaunchKernelGGL(k_cpx_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, C);
+            ST_HIP(hipGetLastError());
+        }
+        ar.used = mark;
+    }
+    hipLaunchKernelGGL(k_carry_az, dim3(1), dim3(1024), 0, st, S, npts);
+    ST_HIP(hipEventRecord(sev[2], st));
+    ST_HIP(hipEventRecord(sev[3], st));
+    ST_HIP(hipEventRecord(sev[4], st));
+
+    // ---- amplitude attempts
+    for (int t = 0; t < par.maxTries && t < 2; t++) {
+        const int large = t;
+        ST_HIP(hipMemsetAsync(S.counts + CNT_LIST, 0, sizeof(int), st));
+        hipLaunchKernelGGL(k_amp_list, dim3((npts + 255) / 256), dim3(256), 0, st, P, S, npts, list);
+        const int nA = read_count(ctx, S.counts, CNT_LIST);
+        if (nA == 0)
+            break;
+        const int ws = par.wAa * (large ? LA : 1), W = 2 * ws;
+        const size_t per = (size_t)ws * ws * 192 + 2048;
+        const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nA, (ar.cap - ar.used) / per, kMaxBatch}));
+        AmpBatch A;
+        A.I = ar.take<float2>((size_t)2 * B * ws * ws);
+        A.U = ar.take<float2>((size_t)2 * B * W * W);
+        A.S = ar.take<float2>((size_t)B * W * W);
+        A.tmp = ar.take<float2>((size_t)2 * B * W * W);
+        A.mag = ar.take<float>((size_t)B * W * W);
+        A.res = ar.take<PeakRes>(B);
+        A.sig = ar.take<float>(2 * (size_t)B);
+        A.state = ar.take<uint8_t>(B);
+        A.az = ar.take<int>(B);
+        A.p = ar.take<double>(2 * (size_t)B);
+        for (int off = 0; off < nA; off += B) {
+            const int nb = std::min(B, nA - off);
+            hipLaunchKernelGGL(k_amp_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, large, A);
+            ctx->fft.fft2d(A.I, A.tmp, 2 * nb, ws, ws, -1, st);
+            hipLaunchKernelGGL(k_amp_pad, dim3((W * W + 255) / 256, 2 * nb), dim3(256), 0, st, A.I, nb, ws, A.az,
+                               A.state, A.U);
+            ctx->fft.fft2d(A.U, A.tmp, 2 * nb, W, W, +1, st);
+            ctx->time_begin(kAmpDet);
+            hipLaunchKernelGGL(k_amp_detect, dim3(nb), dim3(NT), 0, st, P, nb, ws, large, A);
+            ctx->time_end(kAmpDet);
+            ctx->fft.fft2d(A.U, A.tmp, 2 * nb, W, W, -1, st);
+            hipLaunchKernelGGL(k_amp_cross, dim3((W * W + 255) / 256, nb), dim3(256), 0, st, A.U, nb, W * W, A.state,
+                               A.S);
+            ctx->fft.fft2d(A.S, A.tmp, nb, W, W, +1, st);
+            ctx->time_begin(kPeak);
+            hipLaunchKernelGGL(k_peak, dim3(nb), dim3(NT), 0, st, A.S, A.mag, W, 1, A.res);
+            ctx->time_end(kPeak);
+            hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, A.res, nb, A.sig);
+            hipLaunchKernelGGL(k_amp_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, large, A);
+            ST_HIP(hipGetLastError());
+        }
+        ar.used = mark;
+        ST_HIP(hipEventRecord(sev[3 + large], st));
+    }
+
+    hipLaunchKernelGGL(k_compose, dim3((npts + 255) / 256), dim3(256), 0, st, P, S, dOffR, dOffA, npts);
+    ST_HIP(hipGetLastError());
+    int counts[CNT_N];
+    ST_HIP(hipMemcpyAsync(counts, S.counts, sizeof(counts), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipMemcpyAsync(out->offR, dOffR, npts * sizeof(float), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipMemcpyAsync(out->offA, dOffA, npts * sizeof(float), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipMemcpyAsync(out->corr, S.corr, npts * sizeof(float), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipMemcpyAsync(out->type, S.type, npts, hipMemcpyDeviceToHost, st));
+    if (out->azd)
+        ST_HIP(hipMemcpyAsync(out->azd, S.azd, npts * sizeof(float), hipMemcpyDeviceToHost, st));
+    if (diag) {
+        auto cp = [&](void *h, const void *d, size_t bytes) {
+            if (h)
+                ST_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
+        };
+        cp(diag->r2, S.r2, npts * 4);
+        cp(diag->a2, S.a2, npts * 4);
+        for (int t = 0; t < 3; t++) {
+            size_t o = (size_t)t * npts;
+            cp(diag->peakI[t], S.peakI + o, npts * 4);
+            cp(diag->peakJ[t], S.peakJ + o, npts * 4);
+            cp(diag->fineI[t], S.fineI + o, npts * 4);
+            cp(diag->fineJ[t], S.fineJ + o, npts * 4);
+            cp(diag->cmax[t], S.cmax + o, npts * 8);
+            cp(diag->sigRg[t], S.sigRg + o, npts * 4);
+            cp(diag->sigAz[t], S.sigAz + o, npts * 4);
+            cp(diag->peakGap[t], S.peakGap + o, npts * 4);
+            cp(diag->fineGap[t], S.fineGap + o, npts * 4);
+        }
+    }
+    std::vector<uint8_t> azv;
+    std::vector<int> azs;
+    if (diag && diag->azShift) {
+        azv.resize(npts);
+        azs.resize(npts);
+        ST_HIP(hipMemcpyAsync(azv.data(), S.azValid, npts, hipMemcpyDeviceToHost, st));
+        ST_HIP(hipMemcpyAsync(azs.data(), S.azShift, npts * 4, hipMemcpyDeviceToHost, st));
+    }
+    ST_HIP(hipEventRecord(sev[5], st));
+    ST_HIP(hipStreamSynchronize(st));
+    ctx->flush_timings();
+    {
+        float ms;
+        ST_HIP(hipEventElapsedTime(&ms, sev[0], sev[1]));
+        ctx->stats.stageMs[0] = ms;
+        ST_HIP(hipEventElapsedTime(&ms, sev[1], sev[2]));
+        ctx->stats.stageMs[1] = ms;
+        ST_HIP(hipEventElapsedTime(&ms, sev[2], sev[3]));
+        ctx->stats.stageMs[2] = ms;
+        ST_HIP(hipEventElapsedTime(&ms, sev[3], sev[4]));
+        ctx->stats.stageMs[3] = ms;
+        ST_HIP(hipEventElapsedTime(&ms, sev[0], sev[5]));
+        ctx->stats.totalMs = ms;
+        ctx->stats.nPoints = npts;
+        for (int t = 0; t < 3; t++)
+            ctx->stats.nCorr[t] = counts[CNT_CORR0 + t];
+    }
+    if (diag && diag->azShift)
+        for (int k = 0; k < npts; k++)
+            diag->azShift[k] = azv[k] ? azs[k] : -1;
+    out->nComplex = counts[CNT_COMPLEX];
+    out->nAmp = counts[CNT_AMP];
+    out->nAmpL = counts[CNT_AMPL];
+    out->nFail = counts[CNT_FAIL];
+}

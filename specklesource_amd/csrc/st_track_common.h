@@ -1,0 +1,195 @@
+// Device pieces shared by the strack stages: sample access, the
+// correlation-peak search with the reference scan order, the FP64
+// sub-pixel refinement and the 1-D Gaussian Levenberg-Marquardt fit.
+#pragma once
+
+#include "st_common.h"
+
+namespace sttrack {
+
+constexpr int NOVER = 12;  // Strack/speckleTrack.c:15
+constexpr int NFAST = 8;   // speckleTrack.c:16
+constexpr int LA = 2;      // speckleTrack.c:18
+constexpr int NGAUSS = 18; // speckleTrack.c:29
+constexpr int kFine = 2 * NOVER * 2;  // 48: side of the searched oversampled box
+constexpr int NT = 256;
+
+struct Images {
+    const void *img1, *img2;
+    int nr1, na1, nr2, na2, stype;
+};
+
+__device__ inline float2 load_sample(const void *img, int stype, int nr, int a, int r)
+{
+    const size_t o = (size_t)a * nr + r;
+    if (stype == ST_SAMPLE_C64)
+        return ((const float2 *)img)[o];
+    const short2 s = ((const short2 *)img)[o];
+    return make_float2((float)s.x, (float)s.y);
+}
+
+// Per-attempt result of the peak stage.
+struct PeakRes {
+    int I, J;         // coarse peak in the fftshifted surface, -1000 = rejected
+    float raw;        // |C| at the coarse peak (findCPeak's cMax), -1 when rejected
+    float gap;        // (best - runner-up) / best on the coarse surface
+    int fi, fj;       // oversampled-box argmax (0..47) or -1
+    double fineMax;   // |Q| at the refined peak
+    float fineGap;
+    int lmOk;         // 1 when the 18-sample Gaussian fits are defined
+    float lmX0[2];    // first abscissa of the range / azimuth samples
+    float lmY[2][NGAUSS];
+};
+
+// Dirichlet kernel of the 8 -> 96 spectral zero pad (speckleTrack.c:852-863,
+// 2014-2038): D(t) = sum_{k=-4}^{3} exp(2 pi i k t / 96), t = -96 .. 96.
+extern __constant__ double2 c_dir96[193];
+
+#pragma clang fp contract(off)
+// 1-D Gaussian LM fit (gaussPeak1D / mrqminMod / mrqcofMod / myGauss1D,
+// speckleTrack.c:2137-2202, 2309-2465), all in float like the reference.
+__device__ inline void lm_gauss(float x, const float a[3], float *y, float dyda[3])
+{
+    float sig = a[2];
+    float t = x - a[1];
+    float argx = (float)(0.5 * (double)t * (double)t / (double)(sig * sig));
+    float ex = (float)exp(-(double)argx);
+    *y = a[0] * ex;
+    dyda[0] = ex;
+    dyda[1] = *y * t / (sig * sig);
+    dyda[2] = *y * t * t / (sig * sig * sig);
+}
+
+__device__ inline void lm_coef(const float *x, const float *y, const float a[3], float alpha[3][3], float beta[3],
+                               float *chisq)
+{
+    for (int j = 0; j < 3; j++) {
+        for (int k = 0; k <= j; k++)
+            alpha[j][k] = 0.0f;
+        beta[j] = 0.0f;
+    }
+    *chisq = 0.0f;
+    const float sg = 0.1f;
+    const float sig2i = (float)(1.0 / (double)(sg * sg));
+    for (int i = 0; i < NGAUSS; i++) {
+        float ymod, dyda[3];
+        lm_gauss(x[i], a, &ymod, dyda);
+        float dy = y[i] - ymod;
+        for (int l = 0; l < 3; l++) {
+            float wt = dyda[l] * sig2i;
+            for (int m = 0; m <= l; m++)
+                alpha[l][m] += wt * dyda[m];
+            beta[l] += dy * wt;
+        }
+        *chisq += dy * dy * sig2i;
+    }
+    for (int j = 1; j < 3; j++)
+        for (int k = 0; k < j; k++)
+            alpha[k][j] = alpha[j][k];
+}
+
+// Gauss-Jordan with full pivoting on a 3x3 system; 0 when singular.
+__device__ inline int solve3(float m[3][3], float b[3])
+{
+    int used[3] = {0, 0, 0};
+    for (int it = 0; it < 3; it++) {
+        float big = 0;
+        int r = -1, c = -1;
+        for (int j = 0; j < 3; j++) {
+            if (used[j])
+                continue;
+            for (int k = 0; k < 3; k++)
+                if (!used[k] && fabsf(m[j][k]) >= big) {
+                    big = fabsf(m[j][k]);
+                    r = j;
+                    c = k;
+                }
+        }
+        if (r < 0)
+            return 0;
+        used[c] = 1;
+        if (r != c) {
+            for (int l = 0; l < 3; l++) {
+                float t = m[r][l];
+                m[r][l] = m[c][l];
+                m[c][l] = t;
+            }
+            float t = b[r];
+            b[r] = b[c];
+            b[c] = t;
+        }
+        if (m[c][c] == 0.0f)
+            return 0;
+        float pinv = (float)(1.0 / (double)m[c][c]);
+        m[c][c] = 1.0f;
+        for (int l = 0; l < 3; l++)
+            m[c][l] *= pinv;
+        b[c] *= pinv;
+        for (int ll = 0; ll < 3; ll++)
+            if (ll != c) {
+                float d = m[ll][c];
+                m[ll][c] = 0.0f;
+                for (int l = 0; l < 3; l++)
+                    m[ll][l] -= m[c][l] * d;
+                b[ll] -= b[c] * d;
+            }
+    }
+    return 1;
+}
+
+// gaussPeak1D: returns sigma = width / 2 of the fit through y[0..17] at
+// abscissae x0, x0+1, ...
+__device__ inline float gauss_peak_1d(const float *yin, float x0)
+{
+    float x[NGAUSS], y[NGAUSS];
+    float maxC = 0.0f;
+    for (int i = 0; i < NGAUSS; i++) {
+        x[i] = x0 + (float)i;
+        y[i] = yin[i];
+        maxC = y[i] > maxC ? y[i] : maxC;
+    }
+    for (int i = 0; i < NGAUSS; i++)
+        y[i] /= maxC;
+    float a[3] = {1.0f, x[NGAUSS / 2], 2.0f};
+    float alpha[3][3], beta[3], ochisq, chisq;
+    float lambda = 0.001f;
+    lm_coef(x, y, a, alpha, beta, &chisq);
+    ochisq = chisq;
+    float lasti = 2e9f;
+    for (int it = 0; it < 5; it++) {
+        float covar[3][3], da[3];
+        for (int j = 0; j < 3; j++) {
+            for (int k = 0; k < 3; k++)
+                covar[j][k] = alpha[j][k];
+            covar[j][j] = (float)((double)alpha[j][j] * (1.0 + (double)lambda));
+            da[j] = beta[j];
+        }
+        if (!solve3(covar, da))
+            break;
+        float atry[3];
+        for (int l = 0; l < 3; l++)
+            atry[l] = a[l] + da[l];
+        float nalpha[3][3], nbeta[3];
+        lm_coef(x, y, atry, nalpha, nbeta, &chisq);
+        if (chisq < ochisq) {
+            lambda = (float)((double)lambda * 0.1);
+            ochisq = chisq;
+            for (int j = 0; j < 3; j++) {
+                for (int k = 0; k < 3; k++)
+                    alpha[j][k] = nalpha[j][k];
+                beta[j] = nbeta[j];
+            }
+            for (int l = 0; l < 3; l++)
+                a[l] = atry[l];
+        } else {
+            lambda = (float)((double)lambda * 10.0);
+        }
+        if (fabs((double)(a[1] - lasti)) < 0.001)
+            break;
+        lasti = a[1];
+    }
+    return (float)(a[2] * 0.5f);
+}
+#pragma clang fp contract(on)
+
+}  // namespace sttrack

@@ -47,6 +47,8 @@ def load_library(path=LIB_PATH):
     for name in ("st_cull_st_data", "st_cull_stats", "st_cull_smooth",
                  "st_cull_islands", "st_cull_run"):
         getattr(lib, name).argtypes = [vp, C.POINTER(abi.CullParams)]
+    lib.st_last_stats.argtypes = [vp, C.POINTER(abi.TrackStats)]
+    lib.st_set_profiling.argtypes = [vp, C.c_int32]
     lib.st_synth_host.argtypes = [C.POINTER(abi.SynthSpec), vp, vp]
     lib.st_synth_device.argtypes = [vp, C.POINTER(abi.SynthSpec), C.POINTER(C.c_int32)]
     lib.st_synth_offset.argtypes = [C.POINTER(abi.SynthSpec), C.c_double, C.c_double,
@@ -189,6 +191,14 @@ class Device:
                                          C.byref(dg) if dg is not None else None)
         self._check(rc, "st_corr_track_fast")
         return finish(res, out)
+
+    def last_stats(self):
+        st = abi.TrackStats()
+        self._check(self.lib.st_last_stats(self.ctx, C.byref(st)), "st_last_stats")
+        return st
+
+    def set_profiling(self, on):
+        self._check(self.lib.st_set_profiling(self.ctx, 1 if on else 0), "st_set_profiling")
 
     def cull(self, name, cp):
         self._check(getattr(self.lib, name)(self.ctx, C.byref(cp)), name)

@@ -1,0 +1,120 @@
+"""Shared helpers for the oracle pinning tests and the GPU parity tests."""
+import ctypes as C
+
+import numpy as np
+
+from specklesource_amd import abi, configs, track
+
+# Relative gap between the best and the runner-up correlation value below
+# which two FP32 implementations may legitimately pick different peaks
+# (FP32 FFT rounding on surfaces of <= 512^2 points is ~1e-6 relative).
+TIE = 2e-5
+# Relative distance to an acceptance threshold that FP32 rounding can cross.
+THRESH_TOL = 1e-4
+
+
+def periodic_pair(w, nr, na, dr, da, seed=0, band=True):
+    """Image 1 tiles one w x w pattern; image 2 is image 1 shifted by the
+    integer offset (dr, da), so every w x w window of image 2 is a circular
+    shift of the matching window of image 1 (a closed-form case: the
+    correlation peak is exactly at the shift)."""
+    rng = np.random.default_rng(seed)
+    pat = (rng.standard_normal((w, w)) + 1j * rng.standard_normal((w, w))) / np.sqrt(2)
+    if band:
+        # keep the pattern band-limited around a Doppler centroid
+        F = np.fft.fft2(pat)
+        fa = np.fft.fftfreq(w)[:, None]
+        fr = np.fft.fftfreq(w)[None, :]
+        keep = (np.abs(((fa - 0.12 + 0.5) % 1.0) - 0.5) < 0.4) & (np.abs(fr) < 0.43)
+        pat = np.fft.ifft2(F * keep)
+    reps = (na // w + 2, nr // w + 2)
+    big = np.tile(pat, reps)
+    img1 = big[:na, :nr].astype(np.complex64)
+    img2 = np.roll(np.roll(big, da, axis=0), dr, axis=1)[:na, :nr].astype(np.complex64)
+    return np.ascontiguousarray(img1), np.ascontiguousarray(img2)
+
+
+def small_params(w, nr, na, step=None, **kw):
+    step = step or w
+    pad = 2 * w + 8
+    nR = max(1, (nr - 2 * pad) // step + 1)
+    nA = max(1, (na - 2 * pad) // step + 1)
+    base = dict(rStart=pad, aStart=pad, deltaR=step, deltaA=step, nR=nR, nA=nA, wR=w, wA=w, wRa=w,
+                wAa=w, rShiftPoly=[0, 0, 0], aShiftPoly=[0, 0, 0])
+    base.update(kw)
+    return track.default_params(**base)
+
+
+def truth_grid(spec, par):
+    """True offset at each grid point (patch centre)."""
+    tr = np.zeros((par.nA, par.nR))
+    ta = np.zeros((par.nA, par.nR))
+    for i in range(par.nA):
+        for j in range(par.nR):
+            tr[i, j], ta[i, j] = track.synth_offset(spec, par.rStart + j * par.deltaR,
+                                                    par.aStart + i * par.deltaA)
+    return tr, ta
+
+
+def compare_strack(par, g, o32, o64):
+    """Parity report of a GPU run g against the FP32-faithful (o32) and FP64
+    (o64) oracle runs (all with diagnostics).  Returns (report, problems):
+    problems lists points whose disagreement is not explained by a near-tie
+    or a near-threshold value."""
+    rep = {}
+    probs = []
+    n = par.nR * par.nA
+    d32, dg = o32.diag, g.diag
+    # integer-peak agreement per attempt
+    tot = agree = ties = 0
+    tie_pts = set()
+    for t in range(3):
+        ran = (d32["peakI"][t] > -2000) & (dg["peakI"][t] > -2000)
+        same = ran & (d32["peakI"][t] == dg["peakI"][t]) & (d32["peakJ"][t] == dg["peakJ"][t])
+        diff = ran & ~same
+        near = diff & ((d32["peakGap"][t] < TIE) | (dg["peakGap"][t] < TIE))
+        tot += int(ran.sum())
+        agree += int(same.sum())
+        ties += int(near.sum())
+        for idx in zip(*np.nonzero(near)):
+            tie_pts.add(idx)
+        for idx in zip(*np.nonzero(diff & ~near)):
+            probs.append(("peak", t, idx, int(d32["peakI"][t][idx]), int(d32["peakJ"][t][idx]),
+                          int(dg["peakI"][t][idx]), int(dg["peakJ"][t][idx]),
+                          float(d32["peakGap"][t][idx])))
+    rep["attempts"] = tot
+    rep["peak_agree"] = agree / max(tot, 1)
+    rep["near_ties"] = ties
+    # match types
+    tmis = g.type != o32.type
+    expl = np.zeros_like(tmis)
+    for idx in zip(*np.nonzero(tmis)):
+        if idx in tie_pts:
+            expl[idx] = True
+            continue
+        c = [d32["cmax"][t][idx] for t in range(3)]
+        for t, thr in ((0, 0.185), (1, 0.07), (2, 0.028)):
+            if abs(c[t] - thr) < THRESH_TOL * thr:
+                expl[idx] = True
+        # fine peak moved by one quantum near the acceptance limit
+        if any(d32["fineGap"][t][idx] < 1e-3 for t in range(3)):
+            expl[idx] = True
+    for idx in zip(*np.nonzero(tmis & ~expl)):
+        probs.append(("type", idx, int(o32.type[idx]), int(g.type[idx])))
+    rep["type_agree"] = 1.0 - tmis.sum() / n
+    rep["type_mismatch_explained"] = int((tmis & expl).sum())
+    # sub-pixel offsets against the FP64 oracle
+    both = (g.type > 0) & (o64.type > 0) & (g.type == o64.type)
+    if both.any():
+        er = (g.offR - o64.offR)[both]
+        ea = (g.offA - o64.offA)[both]
+        rep["rms_r_vs_f64"] = float(np.sqrt(np.mean(er.astype(np.float64) ** 2)))
+        rep["rms_a_vs_f64"] = float(np.sqrt(np.mean(ea.astype(np.float64) ** 2)))
+        cr = np.abs(g.corr[both].astype(np.float64) - o64.corr[both]) / np.maximum(np.abs(o64.corr[both]), 1e-12)
+        rep["corr_relerr_median"] = float(np.median(cr))
+        rep["corr_relerr_max"] = float(np.max(cr))
+        same32 = both & (g.type == o32.type)
+        rep["offset_equal_f32"] = float(np.mean((g.offR == o32.offR)[same32] & (g.offA == o32.offA)[same32]))
+    rep["counts_gpu"] = (g.nComplex, g.nAmp, g.nAmpL, g.nFail)
+    rep["counts_f32"] = (o32.nComplex, o32.nAmp, o32.nAmpL, o32.nFail)
+    return rep, probs

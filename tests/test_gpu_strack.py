@@ -1,0 +1,139 @@
+"""GPU parity of st_speckle_track (strack) against the oracle.
+
+Integer-peak locations and match types must agree with the FP32-faithful
+oracle except where the oracle's own diagnostics show a near-tie (top two
+correlation values within TIE) or a value within THRESH_TOL of an
+acceptance threshold; those are counted.  Sub-pixel offsets are compared
+with the FP64 oracle: RMS <= 0.02 px per axis.  Correlation relative
+error against FP64 is reported and bounded by 1e-3 (median)."""
+import numpy as np
+import pytest
+
+from specklesource_amd import abi, configs, track
+from tests.helpers import compare_strack, periodic_pair, small_params
+
+pytestmark = pytest.mark.gpu
+
+RMS_TOL = 0.02
+
+
+def _run_all(device, oracle32, oracle64, par, a, b, stype=abi.ST_SAMPLE_C64):
+    pid = device.upload_pair(a, b, stype)
+    try:
+        g = device.speckle_track(pid, par, diag=True)
+    finally:
+        device.release_pair(pid)
+    o32 = oracle32.speckle_track(par, a, b, stype, diag=True)
+    o64 = oracle64.speckle_track(par, a, b, stype, diag=True)
+    rep, probs = compare_strack(par, g, o32, o64)
+    print("\n", rep)
+    return g, o32, o64, rep, probs
+
+
+def _check(rep, probs, rms=True):
+    assert not probs, probs[:10]
+    assert rep["peak_agree"] >= 0.999 or rep["attempts"] - rep["near_ties"] == round(rep["peak_agree"] * rep["attempts"])
+    if rms and "rms_r_vs_f64" in rep:
+        assert rep["rms_r_vs_f64"] <= RMS_TOL and rep["rms_a_vs_f64"] <= RMS_TOL
+        assert rep["corr_relerr_median"] < 1e-3
+
+
+@pytest.mark.parametrize("w", [16, 32, 64])
+def test_circular_shift_known_answer(device, w):
+    img1, img2 = periodic_pair(w, 8 * w, 8 * w, 3, -2, seed=w)
+    par = small_params(w, 8 * w, 8 * w, step=w // 2, hanningFlag=0, rShiftPoly=[3, 0, 0], aShiftPoly=[-2, 0, 0])
+    pid = device.upload_pair(img1, img2)
+    r = device.speckle_track(pid, par)
+    device.release_pair(pid)
+    assert (r.type == abi.ST_CMATCH).all()
+    assert np.all(r.offR == 3) and np.all(r.offA == -2)
+    assert np.allclose(r.corr, 1.0, atol=2e-4)
+
+
+def test_parity_full_scene_shape(device, oracle32, oracle64):
+    """cfg2-shaped scene (w=64 complex + amplitude retries, ridge flow,
+    coherence 0.3-0.9, zero-filled block), scaled to 1000 x 700."""
+    s, p = configs.cfg2(scale=20)
+    a, b = track.synth_host(s)
+    g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
+    assert (g.type == 0).sum() > 0 and (g.type == 1).sum() > 0
+    _check(rep, probs)
+
+
+def test_parity_amplitude_only(device, oracle32, oracle64):
+    """cfg1-shaped (-noComplex, w=32, large retry 64) on 512 x 1024."""
+    s, p = configs.cfg1(scale=4)
+    a, b = track.synth_host(s)
+    g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
+    assert (g.type == abi.ST_AMPMATCH).mean() > 0.5
+    _check(rep, probs)
+
+
+@pytest.mark.parametrize("w", [16, 32])
+def test_parity_window_sizes(device, oracle32, oracle64, w):
+    s = configs._spec(480, 480, 100 + w, dr0=0.8, drR=1e-3, da0=-1.3, daA=2e-3, gmin=0.35, gmax=0.85,
+                      gLr=300, gLa=250)
+    p = small_params(w, 480, 480, step=w // 2, rShiftPoly=[0.8, 1e-3, 0], aShiftPoly=[-1.3, 0, 2e-3])
+    a, b = track.synth_host(s)
+    g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
+    _check(rep, probs)
+
+
+def test_parity_complex_int16(device, oracle32, oracle64):
+    s = configs._spec(400, 400, 21, dr0=1.2, da0=0.4, gmin=0.6, gmax=0.6)
+    a, b = track.synth_host(s)
+    a16 = np.zeros(a.shape + (2,), np.int16)
+    b16 = np.zeros_like(a16)
+    a16[..., 0] = np.round(a.real * 300)
+    a16[..., 1] = np.round(a.imag * 300)
+    b16[..., 0] = np.round(b.real * 300)
+    b16[..., 1] = np.round(b.imag * 300)
+    p = small_params(32, 400, 400, step=16, rShiftPoly=[1, 0, 0])
+    g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a16, b16, abi.ST_SAMPLE_CI16)
+    _check(rep, probs)
+
+
+def test_parity_initial_grid_with_holes(device, oracle32, oracle64):
+    """findImage2Pos from an initial-offset grid; grid holes keep the
+    previous point's window position (speckleTrack.c:823-826)."""
+    s = configs._spec(400, 400, 22, dr0=2.4, da0=-1.7, gmin=0.7, gmax=0.7)
+    a, b = track.synth_host(s)
+    p = small_params(32, 400, 400, step=16, polyShift=0)
+    gr = np.full((10, 10), 2.4, np.float32)
+    ga = np.full((10, 10), -1.7, np.float32)
+    gr[3:5, 2:6] = -2e9
+    ga[3:5, 2:6] = -2e9
+    p.initDr = abi.ptr(gr, abi.C.c_float)
+    p.initDa = abi.ptr(ga, abi.C.c_float)
+    p.initNr = p.initNa = 10
+    p.initR0, p.initA0, p.initDeltaR, p.initDeltaA = 0.0, 0.0, 40.0, 40.0
+    g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
+    assert np.array_equal(g.diag["r2"], o32.diag["r2"]) and np.array_equal(g.diag["a2"], o32.diag["a2"])
+    _check(rep, probs)
+
+
+def test_parity_mask_and_flags(device, oracle32, oracle64):
+    s = configs._spec(400, 400, 23, dr0=0.7, da0=1.1, gmin=0.3, gmax=0.7, gLr=200, gLa=300)
+    a, b = track.synth_host(s)
+    mask = np.ones((20, 20), np.uint8)
+    mask[5:9, :] = 0
+    for kw in (dict(maxTries=1), dict(hanningFlag=0), dict(checkAzFocus=1, azDefocusThresh=1.2),
+               dict(scaleFactor=2, rStart=40, aStart=40, deltaR=8, deltaA=8)):
+        p = small_params(32, 400, 400, step=16, rShiftPoly=[1, 0, 0], aShiftPoly=[1, 0, 0])
+        p.mask = abi.ptr(mask, abi.C.c_uint8)
+        p.maskNr = p.maskNa = 20
+        p.maskNrl = p.maskNal = 20
+        for k, v in kw.items():
+            setattr(p, k, v)
+        g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
+        _check(rep, probs)
+
+
+def test_parity_zero_filled_block(device, oracle32, oracle64):
+    s = configs._spec(400, 400, 24, dr0=0.5, da0=0.5, gmin=0.7, gmax=0.7, maskR0=100, maskR1=220,
+                      maskA0=80, maskA1=300)
+    a, b = track.synth_host(s)
+    p = small_params(32, 400, 400, step=12)
+    g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
+    assert np.array_equal(g.type == 0, o32.type == 0)
+    _check(rep, probs)
