@@ -1,0 +1,138 @@
+// Register-resident radix-2 FFTs for the fused per-patch kernels.
+//
+// A length-W transform is split over a pair of adjacent lanes (lane ^ 1):
+// each lane runs a W/2-point FFT in registers and one exchange of W/2
+// values combines the halves.  Forward transforms split the input into
+// even / odd samples (decimation in time) and leave the first / second half
+// of the output in lanes 0 / 1; inverse transforms take the first / second
+// half of the input (decimation in frequency) and leave the even / odd
+// outputs.  Twiddles come from constant tables indexed by compile-time
+// constants, so they are scalar loads.
+#pragma once
+
+#include "st_common.h"
+
+namespace fftreg {
+
+// exp(-2 pi i j / 64), j = 0..63
+extern __constant__ float2 c_tw64[64];
+// exp(+2 pi i j / 128), j = 0..127
+extern __constant__ float2 c_tw128[128];
+
+__device__ inline float2 cmul_f(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// exp(SIGN 2 pi i k / len) for len dividing 64
+template <int SIGN>
+__device__ inline float2 tw(int k, int len)
+{
+    float2 w = c_tw64[k * (64 / len)];
+    if (SIGN > 0)
+        w.y = -w.y;
+    return w;
+}
+
+template <int N>
+constexpr int bitrev(int i)
+{
+    int j = 0;
+    for (int b = 0; (1 << b) < N; b++)
+        if (i & (1 << b))
+            j |= (N >> (b + 1));
+    return j;
+}
+
+// In-place N-point FFT, natural order in and out.
+template <int N, int SIGN>
+__device__ inline void fft(float2 (&x)[N])
+{
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int j = bitrev<N>(i);
+        if (j > i) {
+            float2 t = x[i];
+            x[i] = x[j];
+            x[j] = t;
+        }
+    }
+#pragma unroll
+    for (int len = 2; len <= N; len <<= 1) {
+#pragma unroll
+        for (int i = 0; i < N; i += len) {
+#pragma unroll
+            for (int k = 0; k < len / 2; k++) {
+                const float2 a = x[i + k], b = x[i + k + len / 2];
+                float2 tb;
+                if (k == 0)
+                    tb = b;
+                else if (4 * k == len)
+                    tb = SIGN < 0 ? make_float2(b.y, -b.x) : make_float2(-b.y, b.x);
+                else
+                    tb = cmul_f(b, tw<SIGN>(k, len));
+                x[i + k] = make_float2(a.x + tb.x, a.y + tb.y);
+                x[i + k + len / 2] = make_float2(a.x - tb.x, a.y - tb.y);
+            }
+        }
+    }
+}
+
+// Value of the partner lane (lane ^ 1) via DPP quad_perm [1,0,3,2].
+__device__ inline float dpp_swap1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
+__device__ inline float2 shfl_pair(float2 v)
+{
+    return make_float2(dpp_swap1(v.x), dpp_swap1(v.y));
+}
+
+// Forward W-point FFT over a lane pair.  Lane h holds samples 2m + h
+// (m = 0..W/2-1); on return it holds outputs h*W/2 + k.
+template <int W>
+__device__ inline void fwd_pair(float2 (&x)[W / 2], int h)
+{
+    constexpr int H = W / 2;
+    fft<H, -1>(x);
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+        const float2 y = shfl_pair(x[k]);
+        const float2 E = h ? y : x[k];
+        const float2 O = h ? x[k] : y;
+        float2 t;
+        if (k == 0)
+            t = O;
+        else if (4 * k == W)
+            t = make_float2(O.y, -O.x);
+        else
+            t = cmul_f(O, tw<-1>(k, W));
+        x[k] = h ? make_float2(E.x - t.x, E.y - t.y) : make_float2(E.x + t.x, E.y + t.y);
+    }
+}
+
+// Inverse (exp(+2 pi i jk/W)) W-point FFT over a lane pair.  Lane h holds
+// inputs h*W/2 + n; on return it holds outputs 2k + h.
+template <int W>
+__device__ inline void inv_pair(float2 (&x)[W / 2], int h)
+{
+    constexpr int H = W / 2;
+#pragma unroll
+    for (int n = 0; n < H; n++) {
+        const float2 y = shfl_pair(x[n]);
+        const float2 a = make_float2(x[n].x + y.x, x[n].y + y.y);
+        const float2 d = make_float2(y.x - x[n].x, y.y - x[n].y);
+        float2 b;
+        if (n == 0)
+            b = d;
+        else if (4 * n == W)
+            b = make_float2(-d.y, d.x);
+        else
+            b = cmul_f(d, tw<+1>(n, W));
+        x[n] = h ? b : a;
+    }
+    fft<H, +1>(x);
+}
+
+}  // namespace fftreg

@@ -20,12 +20,17 @@
 
 #include "st_ctx.h"
 #include "st_track_common.h"
+#include "st_fftreg.h"
 
 using namespace sttrack;
 
 namespace sttrack {
 __constant__ double2 c_dir96[193];
 }
+namespace fftreg {
+__constant__ float2 c_tw64[64];
+__constant__ float2 c_tw128[128];
+}  // namespace fftreg
 
 namespace {
 
@@ -793,6 +798,485 @@ __global__ void k_compose(KPar P, Pts S, float *offR, float *offA, int npts)
 }
 
 // ---------------------------------------------------------------------
+// Fused complex match for w in {16, 32, 64}: one workgroup of 4w threads
+// per patch pair, persistent over the work list.  The whole
+// getPatches -> cmpPSWithPad -> 2w x 2w inverse -> findCPeak ->
+// overSampleC chain stays on chip:
+//   * the w x w forward FFTs run in registers, each length-w transform
+//     split over a lane pair (st_fftreg.h), with one LDS transpose;
+//   * the zero-padded 2w x 2w inverse is done as its four polyphase
+//     components C[2n+p][2m+q] = IFFT_w(X[(k+az) mod w][l] exp(i pi (k p +
+//     l q) / w)) (k, l signed), so no padded array is ever formed;
+//   * the fftshifted magnitude surface goes to a per-workgroup scratch
+//     slot (L2 resident) for the argmax neighbourhood and the Gaussian-fit
+//     samples.
+// Thread t: group g = t / 2w (patch g in the forward part, column parity q
+// = g in the inverse part), index r = (t / 2) mod w, lane half h = t & 1.
+template <int W>
+__device__ inline double group_sum(double v, double *sh)
+{
+    constexpr int G = 2 * W;
+#pragma unroll
+    for (int o = 1; o < (G < 64 ? G : 64); o <<= 1)
+        v += __shfl_xor(v, o);
+    if constexpr (G <= 64) {
+        return v;
+    } else {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        __syncthreads();
+        if (lane == 0)
+            sh[wv] = v;
+        __syncthreads();
+        const int g = threadIdx.x / G;
+        const double r = sh[2 * g] + sh[2 * g + 1];
+        __syncthreads();
+        return r;
+    }
+}
+
+// Sum over the whole workgroup (4w threads).
+template <int W>
+__device__ inline double wg_sum(double v, double *sh)
+{
+    constexpr int NTH = 4 * W;
+#pragma unroll
+    for (int o = 1; o < (NTH < 64 ? NTH : 64); o <<= 1)
+        v += __shfl_xor(v, o);
+    if constexpr (NTH <= 64) {
+        return v;
+    } else {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        __syncthreads();
+        if (lane == 0)
+            sh[wv] = v;
+        __syncthreads();
+        double r = 0;
+        for (int k = 0; k < NTH / 64; k++)
+            r += sh[k];
+        __syncthreads();
+        return r;
+    }
+}
+
+struct Top2 {
+    float v1;
+    int r1;
+    float v2;
+};
+
+__device__ inline Top2 top2_combine(Top2 a, Top2 b)
+{
+    Top2 o;
+    if (b.v1 > a.v1 || (b.v1 == a.v1 && b.r1 < a.r1)) {
+        o.v1 = b.v1;
+        o.r1 = b.r1;
+        o.v2 = fmaxf(a.v1, b.v2);
+    } else {
+        o.v1 = a.v1;
+        o.r1 = a.r1;
+        o.v2 = fmaxf(b.v1, a.v2);
+    }
+    return o;
+}
+
+template <int NTH>
+__device__ inline Top2 wg_top2(Top2 a, Top2 *sh)
+{
+#pragma unroll
+    for (int o = 1; o < (NTH < 64 ? NTH : 64); o <<= 1) {
+        Top2 b;
+        b.v1 = __shfl_xor(a.v1, o);
+        b.r1 = __shfl_xor(a.r1, o);
+        b.v2 = __shfl_xor(a.v2, o);
+        a = top2_combine(a, b);
+    }
+    if constexpr (NTH <= 64) {
+        return a;
+    } else {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        __syncthreads();
+        if (lane == 0)
+            sh[wv] = a;
+        __syncthreads();
+        Top2 r = sh[0];
+        for (int k = 1; k < NTH / 64; k++)
+            r = top2_combine(r, sh[k]);
+        __syncthreads();
+        return r;
+    }
+}
+
+template <int W, int STYPE>
+__global__ void __launch_bounds__(4 * W) k_cpx_fused(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
+                                                     const int *list, int count, const CpxBatch *__restrict__ Cp,
+                                                     float *magScratch)
+{
+    const KPar &P = *Pp;
+    const Pts &S = *Sp;
+    const CpxBatch &C = *Cp;
+    using namespace ::fftreg;
+    constexpr int H = W / 2, NTH = 4 * W, LD = W + 1, WW = 2 * W;
+    extern __shared__ float2 lds[];  // [2][W][LD]
+    __shared__ double shd[16];
+    __shared__ Top2 sht[16];
+    __shared__ unsigned shmask[2][2 * W];
+    __shared__ double shang[2][4];
+    __shared__ int shflag;
+    constexpr bool isf = STYPE == ST_SAMPLE_C64;
+    const float thr = isf ? 1e-15f : 1e-9f;
+    const float thrAll = thr * W * W;
+
+    for (int it = blockIdx.x; it < count; it += gridDim.x) {
+        // Lane indices are re-derived behind an opaque copy every item so
+        // the compiler does not hoist per-lane addresses out of the work
+        // loop (they would stay live in registers for the whole kernel).
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        const int g = t / (2 * W);
+        const int r = (t >> 1) % W;
+        const int h = t & 1;
+        float *M = magScratch + (size_t)blockIdx.x * WW * WW;
+        const int pt = list[it];
+        const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
+        // ---------------- load column r of patch g: samples a = 2m + h
+        float2 x[H];
+        const void *img = g ? P.im.img2 : P.im.img1;
+        const int nr = g ? P.im.nr2 : P.im.nr1;
+        const int ra = (g ? r2 : r1) + r, aa = (g ? a2 : a1) + h;
+        unsigned low = 0;
+        double psum = 0, mre = 0, mim = 0;
+        const bool useH = (g == 1) && P.hanning;
+#pragma unroll
+        for (int m = 0; m < H; m++) {
+            const size_t o = (size_t)(aa + 2 * m) * nr + ra;
+            if constexpr (isf) {
+                x[m] = ((const float2 *)img)[o];
+            } else {
+                const short2 q = ((const short2 *)img)[o];
+                x[m] = make_float2((float)q.x, (float)q.y);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < H; m++) {
+#pragma clang fp contract(off)
+            float2 v = x[m];
+            if constexpr (isf) {
+                v.x *= 1000;
+                v.y *= 1000;
+            }
+            const float hw = P.hann[(2 * m + h) * W + r];
+            v.x = useH ? v.x * hw : v.x;
+            v.y = useH ? v.y * hw : v.y;
+            x[m] = v;
+            const float pw = v.x * v.x + v.y * v.y;
+            low |= (pw < thr ? 1u : 0u) << m;
+            psum += pw;
+            mre += v.x;
+            mim += v.y;
+        }
+        shmask[g][2 * r + h] = low;
+        psum = group_sum<W>(psum, shd);
+        mre = group_sum<W>(mre, shd);
+        mim = group_sum<W>(mim, shd);
+        __syncthreads();
+        int nz = 0;
+        if (g == 0)
+            nz = __popc(shmask[0][2 * r + h] | shmask[1][2 * r + h]);
+        const double nzd = wg_sum<W>((double)nz, shd);
+        if (t == 0)
+            shang[0][0] = shang[1][0] = 0;
+        __syncthreads();
+        if ((t & (2 * W - 1)) == 0)
+            shang[g][0] = psum;
+        __syncthreads();
+        const double p1s = shang[0][0], p2s = shang[1][0];
+        if (nzd > 20.5 || p1s < thrAll || p2s < thrAll) {
+            if (t == 0)
+                C.have[it] = 0;
+            __syncthreads();
+            continue;
+        }
+        // ---------------- Doppler (estDopCarrier1): lag-1 azimuth products
+        {
+            const double cnt = (double)W * (double)W;
+            const double mr = mre / cnt, mi = mim / cnt;
+            double zr = 0, zi = 0;
+            // a = 2m + h; next sample a + 1 lives in the partner lane:
+            // h = 0 -> partner m, h = 1 -> partner m + 1
+#pragma unroll
+            for (int m = 0; m < H; m++) {
+                const float2 yp = shfl_pair(x[m]);            // partner's sample m
+                const float2 yn = shfl_pair(x[m + 1 < H ? m + 1 : m]);  // partner's sample m+1
+                const bool valid = (h == 0) || (m + 1 < H);
+                const float2 nxt = h ? yn : yp;
+                if (valid) {
+                    const double ar = (double)nxt.x - mr, ai = (double)nxt.y - mi;
+                    const double br = (double)x[m].x - mr, bi = (double)x[m].y - mi;
+                    zr += ar * br + ai * bi;
+                    zi += ai * br - ar * bi;
+                }
+            }
+            zr = group_sum<W>(zr, shd);
+            zi = group_sum<W>(zi, shd);
+            if ((t & (2 * W - 1)) == 0) {
+                shang[g][1] = zr;
+                shang[g][2] = zi;
+            }
+            __syncthreads();
+            if (t == 0) {
+                const double z0r = shang[0][1], z0i = shang[0][2], z1r = shang[1][1], z1i = shang[1][2];
+                const double ang1 = (z0r == 0.0 && z0i == 0.0) ? NAN : atan2(z0i, z0r);
+                const double ang2 = (z1r == 0.0 && z1i == 0.0) ? NAN : atan2(z1i, z1r);
+                const double v = (double)(W / 2) * (ang1 + ang2) * .5 / 3.14159265358979323846;
+                int shift = isnan(v) ? 0 : ((v >= 0.0) ? (int)(v + 0.5) : (int)(v - 0.5));
+                if (shift < 0)
+                    shift += W;
+                shflag = (int)((float)shift * (float)P.wA / (float)W);
+            }
+            __syncthreads();
+        }
+        const int az = shflag;
+        // ---------------- baseline phase ramp on patch 1
+        if (P.baseFlag && g == 0) {
+#pragma unroll
+            for (int m = 0; m < H; m++) {
+                const int i = 2 * m + h;
+                const double x0 = (a1 - P.wA / 2 - (P.nSlpA1 / 2)) / (double)P.nSlpA1;
+                const double xx = x0 + i * (1. / (double)P.nSlpA1);
+                const double bn = P.Bn + x0 * P.dBn + x0 * x0 * P.dBnQ;
+                const double bp = P.Bp + xx * P.dBp + xx * xx * P.dBpQ;
+                const double bsq = bn * bn + bp * bp;
+                const int j1 = (r1 - P.wR / 2 > 0 ? r1 - P.wR / 2 : 0) + r;
+                double ph = -bn * P.sinT[j1] - bp * P.cosT[j1] + (0.5 * bsq * P.overR[j1]);
+                ph *= -(4.0 * 3.14159265358979323846) / P.lambda;
+                const double pr = cos(ph), pim = sin(ph);
+                const double cr = x[m].x, ci = x[m].y;
+                x[m] = make_float2((float)(cr * pr - ci * pim), (float)(cr * pim + ci * pr));
+            }
+        }
+        // ---------------- forward FFT along azimuth (column r), then LDS
+        fwd_pair<W>(x, h);
+        float2 *F = lds + (size_t)g * W * LD;
+#pragma unroll
+        for (int k = 0; k < H; k++)
+            F[(h * H + k) * LD + r] = x[k];
+        __syncthreads();
+        // ---------------- forward FFT along range (row r)
+#pragma unroll
+        for (int m = 0; m < H; m++)
+            x[m] = F[r * LD + 2 * m + h];
+        __syncthreads();
+        fwd_pair<W>(x, h);
+#pragma unroll
+        for (int k = 0; k < H; k++)
+            F[r * LD + h * H + k] = x[k];
+        __syncthreads();
+        // ---------------- cross spectrum X = F1 conj(F2), row r, half h
+        {
+            const float2 *Fo = lds + (size_t)(1 - g) * W * LD;
+#pragma unroll
+            for (int k = 0; k < H; k++) {
+#pragma clang fp contract(off)
+                const float2 o = Fo[r * LD + h * H + k];
+                x[k] = g == 0 ? cmulconj(x[k], o) : cmulconj(o, x[k]);
+            }
+        }
+        __syncthreads();
+        // ---------------- polyphase inverse: components (pp, q = g)
+        const int krow = ((r - az) % W + W) % W;
+        const int ks = krow < W / 2 ? krow : krow - W;
+        float2 *Z = lds + (size_t)g * W * LD;
+        Top2 best{0.0f, 0x7fffffff, 0.0f};
+#pragma unroll 1
+        for (int pp = 0; pp < 2; pp++) {
+            float2 y[H];
+            // exp(i pi (ks pp + ls q) / w) = row factor (per lane) x column factor
+            const float2 rowf = c_tw128[(ks * pp * (64 / W)) & 127];
+#pragma unroll
+            for (int k = 0; k < H; k++) {
+                const float2 c0 = c_tw128[(k * (64 / W)) & 127];
+                const float2 c1 = c_tw128[((k - H) * (64 / W)) & 127];
+                const float2 cf = h ? c1 : c0;
+                const float2 tw = g ? cmul_f(rowf, cf) : rowf;
+                y[k] = cmul_f(x[k], tw);
+            }
+            inv_pair<W>(y, h);
+            // y[n'] = Z[krow][2 n' + h]
+#pragma unroll
+            for (int n = 0; n < H; n++)
+                Z[krow * LD + 2 * n + h] = y[n];
+            __syncthreads();
+            // columns: lane half h holds k in [h H, h H + H)
+#pragma unroll
+            for (int k = 0; k < H; k++)
+                y[k] = Z[(h * H + k) * LD + r];
+            __syncthreads();
+            inv_pair<W>(y, h);
+            // y[j] = C_pp,g[2 j + h][r]  ->  surface row n = 2 (2j + h) + pp, col m = 2 r + g
+            const int m = 2 * r + g;
+            const int J = (m + W) & (WW - 1);
+#pragma unroll
+            for (int jj = 0; jj < H; jj++) {
+#pragma clang fp contract(off)
+                const int n = 2 * (2 * jj + h) + pp;
+                const int I = (n + W) & (WW - 1);
+                const float x2 = y[jj].x * y[jj].x + y[jj].y * y[jj].y;
+                const float v = (float)sqrt((double)x2);
+                M[I * WW + J] = v;
+                const int i0 = I < W ? I : I - W, j0 = J < W ? J : J - W;
+                const int q = (I < W) ? (J < W ? 0 : 3) : (J < W ? 2 : 1);
+                const int rank = (i0 * W + j0) * 4 + q;
+                if (v > 0.0f) {
+                    Top2 c{v, rank, 0.0f};
+                    best = top2_combine(best, c);
+                } else {
+                    best.v2 = fmaxf(best.v2, v);
+                }
+            }
+        }
+        best = wg_top2<NTH>(best, sht);
+        __syncthreads();
+        PeakRes *res = &C.res[it];
+        int I = -1000, J = -1000;
+        float raw = 0.f;
+        if (best.r1 != 0x7fffffff) {
+            const int q = best.r1 & 3, ij = best.r1 >> 2;
+            const int i0 = ij / W, j0 = ij - (ij / W) * W;
+            I = (q == 0 || q == 3) ? i0 : i0 + W;
+            J = (q == 0 || q == 2) ? j0 : j0 + W;
+            raw = best.v1;
+        }
+        if (t == 0) {
+            res->gap = raw > 0 ? (raw - best.v2) / raw : 0.f;
+            res->lmOk = 0;
+            res->fi = res->fj = -1;
+            res->fineMax = 0;
+            res->fineGap = 0;
+        }
+        if (I < NFAST / 2 || I > WW - NFAST / 2 || J < NFAST / 2 || J > WW - NFAST / 2) {
+            if (t == 0) {
+                res->I = res->J = -1000;
+                res->raw = -1.f;
+            }
+        } else {
+            if (t == 0) {
+                res->I = I;
+                res->J = J;
+                res->raw = raw;
+            }
+            double *Pm = (double *)lds;
+            double2 *T = (double2 *)(lds + 64);
+            if (t < NFAST * NFAST)
+                Pm[t] = (double)M[(I - NFAST / 2 + t / NFAST) * WW + (J - NFAST / 2 + t % NFAST)];
+            __syncthreads();
+#pragma unroll 1
+            for (int e = t; e < NFAST * kFine; e += NTH) {
+                const int mm = e / kFine, v = e - mm * kFine;
+                double re = 0, im = 0;
+                for (int nn = 0; nn < NFAST; nn++) {
+                    const double2 d = c_dir96[(v + 24) - 12 * nn + 96];
+                    const double pv = Pm[mm * NFAST + nn];
+                    re += pv * d.x;
+                    im += pv * d.y;
+                }
+                T[e] = make_double2(re, im);
+            }
+            __syncthreads();
+            double f1 = 0.0, f2 = 0.0;
+            int fr = 0x7fffffff;
+#pragma unroll 1
+            for (int e = t; e < kFine * kFine; e += NTH) {
+                const int u = e / kFine, v = e - u * kFine;
+                double re = 0, im = 0;
+                for (int mm = 0; mm < NFAST; mm++) {
+                    const double2 d = c_dir96[(u + 24) - 12 * mm + 96];
+                    const double2 tv = T[mm * kFine + v];
+                    re += d.x * tv.x - d.y * tv.y;
+                    im += d.x * tv.y + d.y * tv.x;
+                }
+                const double a = sqrt(re * re + im * im);
+                if (a > f1 || (a == f1 && e < fr)) {
+                    f2 = fmax(f2, f1);
+                    f1 = a;
+                    fr = e;
+                } else
+                    f2 = fmax(f2, a);
+            }
+            // workgroup top-2 in double
+#pragma unroll
+            for (int o = 1; o < (NTH < 64 ? NTH : 64); o <<= 1) {
+                const double b1 = __shfl_xor(f1, o), b2 = __shfl_xor(f2, o);
+                const int br = __shfl_xor(fr, o);
+                if (b1 > f1 || (b1 == f1 && br < fr)) {
+                    f2 = fmax(f1, b2);
+                    f1 = b1;
+                    fr = br;
+                } else
+                    f2 = fmax(f2, b1);
+            }
+            if constexpr (NTH > 64) {
+                double *s1 = shd;  // [0, 4): best, [4, 8): second
+                int *sr = (int *)(shd + 8);
+                __syncthreads();
+                if ((t & 63) == 0) {
+                    s1[t >> 6] = f1;
+                    s1[4 + (t >> 6)] = f2;
+                    sr[t >> 6] = fr;
+                }
+                __syncthreads();
+                f1 = s1[0];
+                f2 = s1[4];
+                fr = sr[0];
+                for (int k = 1; k < NTH / 64; k++) {
+                    const double b1 = s1[k], b2 = s1[4 + k];
+                    const int br = sr[k];
+                    if (b1 > f1 || (b1 == f1 && br < fr)) {
+                        f2 = fmax(f1, b2);
+                        f1 = b1;
+                        fr = br;
+                    } else
+                        f2 = fmax(f2, b1);
+                }
+                __syncthreads();
+            }
+            int fi = 0, fj = 0;
+            if (fr != 0x7fffffff) {
+                fi = fr / kFine;
+                fj = fr - fi * kFine;
+            }
+            if (t == 0) {
+                res->fi = fi;
+                res->fj = fj;
+                res->fineMax = fr != 0x7fffffff ? f1 : 0.0;
+                res->fineGap = fr != 0x7fffffff ? (float)((f1 - f2) / f1) : 0.f;
+            }
+            const int iF = I * NOVER + (fi + 24) - (NOVER * NFAST) / 2;
+            const int jF = J * NOVER + (fj + 24) - (NOVER * NFAST) / 2;
+            const int cI = iF / NOVER, cJ = jF / NOVER;
+            const bool ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < WW && (cJ - NGAUSS / 2) >= 0 &&
+                            (cJ + NGAUSS / 2) < WW;
+            if (t == 0) {
+                res->lmOk = ok ? 1 : 0;
+                res->lmX0[0] = (float)(cJ - NGAUSS / 2);
+                res->lmX0[1] = (float)(cI - NGAUSS / 2);
+            }
+            if (ok && t < 2 * NGAUSS) {
+                const int k = t % NGAUSS;
+                res->lmY[t / NGAUSS][k] = t < NGAUSS ? M[cI * WW + (cJ - NGAUSS / 2 + k)]
+                                                     : M[(cI - NGAUSS / 2 + k) * WW + cJ];
+            }
+        }
+        if (t == 0) {
+            C.have[it] = 1;
+            C.az[it] = az;
+            C.p[2 * it] = p1s / (double)(W * W);
+            C.p[2 * it + 1] = p2s / (double)(W * W);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------
 // host side
 
 void host_hanning(int wA, int wR, std::vector<float> &h)
@@ -900,6 +1384,13 @@ void init_dirichlet(st_ctx *ctx)
         tab[t + 96] = make_double2(re, im);
     }
     ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dir96), tab, sizeof(tab)));
+    float2 t64[64], t128[128];
+    for (int j = 0; j < 64; j++)
+        t64[j] = make_float2((float)cos(-2.0 * M_PI * j / 64.0), (float)sin(-2.0 * M_PI * j / 64.0));
+    for (int j = 0; j < 128; j++)
+        t128[j] = make_float2((float)cos(2.0 * M_PI * j / 128.0), (float)sin(2.0 * M_PI * j / 128.0));
+    ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fftreg::c_tw64), t64, sizeof(t64)));
+    ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fftreg::c_tw128), t128, sizeof(t128)));
     done_dev = ctx->device;
 }
 
@@ -909,6 +1400,43 @@ T *upload(st_ctx *ctx, const T *h, size_t n)
     T *d = ctx->arena.take<T>(n);
     ST_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
     return d;
+}
+
+template <int W, int STYPE>
+void launch_cpx_fused_t(st_ctx *ctx, const KPar *dP, const Pts *dS, const int *list, int count, const CpxBatch *dC)
+{
+    const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
+    int perCU = 0, ncu = 0;
+    ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, STYPE>, 4 * W, lds));
+    ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const int grid = std::max(1, std::min(count, std::max(1, perCU) * ncu));
+    float *mag = ctx->arena.take<float>((size_t)grid * 4 * W * W);
+    hipLaunchKernelGGL((k_cpx_fused<W, STYPE>), dim3(grid), dim3(4 * W), lds, ctx->stream, dP, dS, list, count, dC,
+                       mag);
+    ST_HIP(hipGetLastError());
+}
+
+template <int W>
+void launch_cpx_fused(st_ctx *ctx, const KPar &P, const Pts &S, const int *list, int count, const CpxBatch &C)
+{
+    // parameter blocks live in device memory: read through the scalar cache
+    // instead of occupying SGPRs for the whole kernel
+    KPar *dP = ctx->arena.take<KPar>(1);
+    Pts *dS = ctx->arena.take<Pts>(1);
+    CpxBatch *dC = ctx->arena.take<CpxBatch>(1);
+    ST_HIP(hipMemcpyAsync(dP, &P, sizeof(KPar), hipMemcpyHostToDevice, ctx->stream));
+    ST_HIP(hipMemcpyAsync(dS, &S, sizeof(Pts), hipMemcpyHostToDevice, ctx->stream));
+    ST_HIP(hipMemcpyAsync(dC, &C, sizeof(CpxBatch), hipMemcpyHostToDevice, ctx->stream));
+    if (P.im.stype == ST_SAMPLE_C64)
+        launch_cpx_fused_t<W, ST_SAMPLE_C64>(ctx, dP, dS, list, count, dC);
+    else
+        launch_cpx_fused_t<W, ST_SAMPLE_CI16>(ctx, dP, dS, list, count, dC);
+}
+
+bool staged_only()
+{
+    const char *e = getenv("SPECKLE_AMD_PATH");
+    return e && strcmp(e, "staged") == 0;
 }
 
 int read_count(st_ctx *ctx, int *dcounts, int idx)
@@ -938,8 +1466,10 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     ctx->stats_reset();
     hipEvent_t *sev = ctx->ev + 64;  // stage boundary events
     ST_HIP(hipEventRecord(sev[0], st));
+    const bool fusedPath = (par.wR == 16 || par.wR == 32 || par.wR == 64) && !staged_only();
+    const int kPeak = ctx->kernel_slot(fusedPath ? "k_cpx_fused" : "peak_refine");
     const int kGather = ctx->kernel_slot("cpx_gather"), kFft = ctx->kernel_slot("fft2d"),
-              kPeak = ctx->kernel_slot("peak_refine"), kAmpDet = ctx->kernel_slot("amp_detect");
+              kAmpDet = ctx->kernel_slot("amp_detect");
     const int npts = par.nR * par.nA;
     const int w = par.wR;
 
@@ -1055,7 +1585,27 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     const int nCpx = read_count(ctx, S.counts, CNT_LIST);
 
     // ---- complex attempts
-    if (nCpx > 0) {
+    const bool fused = (w == 16 || w == 32 || w == 64) && !staged_only();
+    if (nCpx > 0 && fused) {
+        CpxBatch C{};
+        C.res = ar.take<PeakRes>(nCpx);
+        C.sig = ar.take<float>(2 * (size_t)nCpx);
+        C.have = ar.take<uint8_t>(nCpx);
+        C.az = ar.take<int>(nCpx);
+        C.p = ar.take<double>(2 * (size_t)nCpx);
+        ctx->time_begin(kPeak);
+        if (w == 16)
+            launch_cpx_fused<16>(ctx, P, S, list, nCpx, C);
+        else if (w == 32)
+            launch_cpx_fused<32>(ctx, P, S, list, nCpx, C);
+        else
+            launch_cpx_fused<64>(ctx, P, S, list, nCpx, C);
+        ctx->time_end(kPeak);
+        hipLaunchKernelGGL(k_lm, dim3((2 * nCpx + 127) / 128), dim3(128), 0, st, C.res, nCpx, C.sig);
+        hipLaunchKernelGGL(k_cpx_finalize, dim3((nCpx + 127) / 128), dim3(128), 0, st, P, S, list, 0, nCpx, C);
+        ST_HIP(hipGetLastError());
+        ar.used = mark;
+    } else if (nCpx > 0) {
         const int B = std::min(Bc, nCpx);
         CpxBatch C;
         C.P = ar.take<float2>((size_t)2 * B * w * w);
