@@ -114,69 +114,76 @@ int orc_cull_st_data(st_cull_params *cp)
     return ST_OK;
 }
 
-/* Least squares y ~ A x, A = n x 3, by one-sided Jacobi SVD with the
- * svdfit truncation rule.  Returns x (3). */
-static void lsq3_svd(const double *A, const double *y, int n, double x[3])
+/* Least squares y ~ A x, A = n x 3 with rows (c0, xi[k], xj[k]), by
+ * one-sided Jacobi SVD with the svdfit truncation rule (singular values
+ * below 1e-5 * max dropped).  The rotated columns U = A V are never
+ * stored: every dot product recomputes u = a_k . v from the rows, in a
+ * fixed order, so the GPU restatement can follow the same arithmetic.
+ * Sweeps stop once every column pair is orthogonal to 1e-13 (cosine);
+ * the rounding floor of these 81-row dot products is ~1e-14, so a
+ * tighter limit only spins. */
+static void lsq3_svd(double c0, const int *xi, const int *xj, const double *y, int n, double x[3])
 {
-    double U[81 * 3], V[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    memcpy(U, A, sizeof(double) * (size_t)n * 3);
+    double V[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     for (int sweep = 0; sweep < 30; sweep++) {
         double off = 0.0;
         for (int p = 0; p < 2; p++)
             for (int q = p + 1; q < 3; q++) {
                 double alpha = 0, beta = 0, gamma = 0;
                 for (int i = 0; i < n; i++) {
-                    alpha += U[i * 3 + p] * U[i * 3 + p];
-                    beta += U[i * 3 + q] * U[i * 3 + q];
-                    gamma += U[i * 3 + p] * U[i * 3 + q];
+                    const double a0 = c0, a1 = (double)xi[i], a2 = (double)xj[i];
+                    const double up = a0 * V[0 * 3 + p] + a1 * V[1 * 3 + p] + a2 * V[2 * 3 + p];
+                    const double uq = a0 * V[0 * 3 + q] + a1 * V[1 * 3 + q] + a2 * V[2 * 3 + q];
+                    alpha += up * up;
+                    beta += uq * uq;
+                    gamma += up * uq;
                 }
                 if (gamma == 0.0)
                     continue;
                 double conv = fabs(gamma) / sqrt(alpha * beta);
                 if (conv > off)
                     off = conv;
-                if (conv < 1e-15)
+                if (conv < 1e-13)
                     continue;
                 double zeta = (beta - alpha) / (2.0 * gamma);
                 double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                 double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
-                for (int i = 0; i < n; i++) {
-                    double up = U[i * 3 + p], uq = U[i * 3 + q];
-                    U[i * 3 + p] = c * up - s * uq;
-                    U[i * 3 + q] = s * up + c * uq;
-                }
-                for (int i = 0; i < 3; i++) {
-                    double vp = V[i * 3 + p], vq = V[i * 3 + q];
-                    V[i * 3 + p] = c * vp - s * vq;
-                    V[i * 3 + q] = s * vp + c * vq;
+                for (int r = 0; r < 3; r++) {
+                    double vp = V[r * 3 + p], vq = V[r * 3 + q];
+                    V[r * 3 + p] = c * vp - s * vq;
+                    V[r * 3 + q] = s * vp + c * vq;
                 }
             }
-        if (off < 1e-15)
+        if (off < 1e-13)
             break;
+    }
+    double w2[3] = {0, 0, 0}, uy[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        const double a0 = c0, a1 = (double)xi[i], a2 = (double)xj[i];
+        for (int k = 0; k < 3; k++) {
+            const double u = a0 * V[0 * 3 + k] + a1 * V[1 * 3 + k] + a2 * V[2 * 3 + k];
+            w2[k] += u * u;
+            uy[k] += u * y[i];
+        }
     }
     double w[3], wmax = 0;
     for (int k = 0; k < 3; k++) {
-        double s2 = 0;
-        for (int i = 0; i < n; i++)
-            s2 += U[i * 3 + k] * U[i * 3 + k];
-        w[k] = sqrt(s2);
+        w[k] = sqrt(w2[k]);
         if (w[k] > wmax)
             wmax = w[k];
     }
-    double thresh = 1.0e-5 * wmax;
+    const double thresh = 1.0e-5 * wmax;
     double coef[3];
-    for (int k = 0; k < 3; k++) {
-        coef[k] = 0;
-        if (w[k] < thresh || w[k] == 0.0)
-            continue;
-        /* (u_k . y) / w_k with u_k = U[:,k] / w_k */
-        double s = 0;
-        for (int i = 0; i < n; i++)
-            s += U[i * 3 + k] * y[i];
-        coef[k] = s / (w[k] * w[k]);
-    }
+    for (int k = 0; k < 3; k++)
+        coef[k] = (w[k] < thresh || w[k] == 0.0) ? 0.0 : uy[k] / w2[k];
     for (int r = 0; r < 3; r++)
         x[r] = V[r * 3 + 0] * coef[0] + V[r * 3 + 1] * coef[1] + V[r * 3 + 2] * coef[2];
+}
+
+/* Test hook: the plane fit of cullStats on explicit points. */
+void orc_lsq3(double c0, const int *xi, const int *xj, const double *y, int n, double *x)
+{
+    lsq3_svd(c0, xi, xj, y, n, x);
 }
 
 int orc_cull_stats(st_cull_params *cp)
@@ -185,7 +192,7 @@ int orc_cull_stats(st_cull_params *cp)
     const int b = 9; /* cullStats.c:38-39 forces 9 x 9 */
     cp->bR = b;
     cp->bA = b;
-    double listR[81], listA[81], A[81 * 3];
+    double listR[81], listA[81];
     int xi[81], xj[81];
     double aR[3] = {0, 0, 0}, aA[3] = {0, 0, 0};
     for (int i = 0; i < nA; i++) {
@@ -213,15 +220,8 @@ int orc_cull_stats(st_cull_params *cp)
             meanR /= (double)ngood;
             if (ngood > 8) {
                 /* the basis value is stored in a float field (cullStats.c:80) */
-                for (int k = 0; k < ngood; k++) {
-                    A[k * 3 + 0] = (double)(float)meanR;
-                    A[k * 3 + 1] = (double)(float)xi[k];
-                    A[k * 3 + 2] = (double)(float)xj[k];
-                }
-                lsq3_svd(A, listR, ngood, aR);
-                for (int k = 0; k < ngood; k++)
-                    A[k * 3 + 0] = (double)(float)meanA;
-                lsq3_svd(A, listA, ngood, aA);
+                lsq3_svd((double)(float)meanR, xi, xj, listR, ngood, aR);
+                lsq3_svd((double)(float)meanA, xi, xj, listA, ngood, aA);
                 aR[0] *= meanR;
                 aA[0] *= meanA;
             }

@@ -1,8 +1,686 @@
-// cullst stages on the GPU (placeholder).
+// cullst stages on the GPU (Cullst/cullSTData.c, cullStats.c,
+// cullSmooth.c, cullIslands.c).
+//
+// cullSTData culls in place in raster order, so a cell's decision sees the
+// decisions already taken for earlier cells of the same pass.  The GPU
+// solves that recurrence exactly as a fixpoint: every iteration re-decides
+// all cells in parallel, earlier cells taking their decision from the
+// previous iteration; the sequential result is the unique fixpoint and is
+// reached when an iteration changes nothing (a dependency chain of length
+// k is settled after k iterations).  A single-workgroup wavefront kernel
+// (cell (i, j) runs at step i*(bR/2+1) + j) finishes the pass if the
+// fixpoint has not settled after kMaxIter iterations.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
 #include "st_ctx.h"
 
-extern "C" int st_cull_st_data(st_ctx *c, st_cull_params *cp) { (void)c; (void)cp; return ST_ERR_UNSUPPORTED; }
-extern "C" int st_cull_stats(st_ctx *c, st_cull_params *cp) { (void)c; (void)cp; return ST_ERR_UNSUPPORTED; }
-extern "C" int st_cull_smooth(st_ctx *c, st_cull_params *cp) { (void)c; (void)cp; return ST_ERR_UNSUPPORTED; }
-extern "C" int st_cull_islands(st_ctx *c, st_cull_params *cp) { (void)c; (void)cp; return ST_ERR_UNSUPPORTED; }
-extern "C" int st_cull_run(st_ctx *c, st_cull_params *cp) { (void)c; (void)cp; return ST_ERR_UNSUPPORTED; }
+// The cull arithmetic mirrors the reference operation by operation; no FMA
+// contraction anywhere in this file.
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr float kNoData = -2.0e9f;
+constexpr int kMaxIter = 48;
+constexpr int kMaxBox = 15;  // box sides up to 15 (225 cells per box)
+
+__device__ inline bool valid2(float r, float a)
+{
+    return r > (float)(1 - ST_LARGEINT) && a > (float)(1 - ST_LARGEINT);
+}
+
+struct CullDev {
+    int nR, nA;
+    float *offR, *offA, *corr;
+    uint8_t *type;
+    const uint8_t *mask;
+    float *offRS, *offAS, *sigmaR, *sigmaA;
+    float *R0, *A0;       // state at the start of a cullSTData pass
+    uint8_t *cPrev, *cNew; // cull decisions of the pass
+    int *counters;        // [0] changed, [1] nCulled, [2] max root, ...
+    int *parent, *bbox;   // islands
+};
+
+// Pre-pass of cullSTData (cullSTData.c:34-68).
+__global__ void k_cull_prepass(CullDev D, float corrThresh, int singleMT)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.nR * D.nA)
+        return;
+    if (D.corr[c] < corrThresh) {
+        D.offR[c] = kNoData;
+        D.offA[c] = kNoData;
+        D.type[c] = 0;
+    }
+    if (D.mask) {
+        const unsigned char mv = (unsigned char)D.mask[c] & 4;
+        if (mv > 0 && D.type[c] > 2) {
+            D.offR[c] = kNoData;
+            D.offA[c] = kNoData;
+            D.type[c] = 0;
+        }
+        if (singleMT > 0 && D.type[c] != singleMT) {
+            D.offR[c] = kNoData;
+            D.offA[c] = kNoData;
+            D.type[c] = 0;
+        }
+    }
+    D.R0[c] = D.offR[c];
+    D.A0[c] = D.offA[c];
+    D.cPrev[c] = 0;
+}
+
+// Median test of one cell by one wavefront (cullSTData.c:73-122): the box
+// values are ranked by counting, the floor(n/2)-th smallest is the median
+// (selectForCull's k = n/2 on a 1-based list).
+__device__ inline bool cull_decide(const CullDev &D, int c, int bR, int bA, int nGood, float maxA, float maxR,
+                                   const uint8_t *earlier, float *lr, float *la)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = c / D.nR, j = c - (c / D.nR) * D.nR;
+    const int i1 = imax(0, i - bA / 2), i2 = imin(D.nA - 1, i + bA / 2);
+    const int j1 = imax(0, j - bR / 2), j2 = imin(D.nR - 1, j + bR / 2);
+    const int bw = j2 - j1 + 1, cnt = bw * (i2 - i1 + 1);
+    int ngood = 0;
+    for (int base = 0; base < cnt; base += 64) {
+        const int s = base + lane;
+        bool ok = false;
+        float r = 0, a = 0;
+        if (s < cnt) {
+            const int q = (i1 + s / bw) * D.nR + j1 + s % bw;
+            const bool culled = q < c && earlier[q];
+            r = culled ? kNoData : D.R0[q];
+            a = culled ? kNoData : D.A0[q];
+            ok = valid2(r, a);
+        }
+        const unsigned long long m = __ballot(ok);
+        const int pos = ngood + __popcll(m & ((1ull << lane) - 1ull));
+        if (ok) {
+            lr[pos] = r;
+            la[pos] = a;
+        }
+        ngood += __popcll(m);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (ngood <= nGood)
+        return true;
+    int k = ngood / 2;
+    if (k == 0)
+        k = 1;
+    float medR = 0, medA = 0;
+    for (int base = 0; base < ngood; base += 64) {
+        const int s = base + lane;
+        bool isR = false, isA = false;
+        float vr = 0, va = 0;
+        if (s < ngood) {
+            vr = lr[s];
+            va = la[s];
+            int lessR = 0, eqR = 0, lessA = 0, eqA = 0;
+            for (int u = 0; u < ngood; u++) {
+                const float xr = lr[u], xa = la[u];
+                lessR += xr < vr;
+                eqR += xr == vr;
+                lessA += xa < va;
+                eqA += xa == va;
+            }
+            isR = lessR < k && k <= lessR + eqR;
+            isA = lessA < k && k <= lessA + eqA;
+        }
+        const unsigned long long mR = __ballot(isR), mA = __ballot(isA);
+        if (mR)
+            medR = __shfl(vr, __ffsll((long long)mR) - 1);
+        if (mA)
+            medA = __shfl(va, __ffsll((long long)mA) - 1);
+    }
+    const double diffA = (double)(D.A0[c] - medA);
+    const double diffR = (double)(D.R0[c] - medR);
+    return fabs(diffA) > maxA || fabs(diffR) > maxR;
+}
+
+__global__ void __launch_bounds__(256) k_cull_iter(CullDev D, int bR, int bA, int nGood, float maxA, float maxR)
+{
+    __shared__ float lr[4][kMaxBox * kMaxBox], la[4][kMaxBox * kMaxBox];
+    const int w = threadIdx.x >> 6;
+    const int c = blockIdx.x * 4 + w;
+    if (c >= D.nR * D.nA)
+        return;
+    const bool cull = cull_decide(D, c, bR, bA, nGood, maxA, maxR, D.cPrev, lr[w], la[w]);
+    if ((threadIdx.x & 63) == 0) {
+        D.cNew[c] = cull ? 1 : 0;
+        if ((D.cPrev[c] != 0) != cull)
+            atomicAdd(&D.counters[0], 1);
+    }
+}
+
+// Exact sequential order for the leftover case: one workgroup sweeps the
+// wavefront steps, one wave per cell of a step.
+__global__ void __launch_bounds__(1024) k_cull_wavefront(CullDev D, int bR, int bA, int nGood, float maxA, float maxR)
+{
+    __shared__ float lr[16][kMaxBox * kMaxBox], la[16][kMaxBox * kMaxBox];
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int s = bR / 2 + 1;
+    const int steps = (D.nA - 1) * s + D.nR;
+    for (int T = 0; T < steps; T++) {
+        // cells with i*s + j == T
+        const int iLo = imax(0, (T - (D.nR - 1) + s - 1) / s), iHi = imin(D.nA - 1, T / s);
+        for (int i = iLo + w; i <= iHi; i += nw) {
+            const int j = T - i * s;
+            const int c = i * D.nR + j;
+            const bool cull = cull_decide(D, c, bR, bA, nGood, maxA, maxR, D.cNew, lr[w], la[w]);
+            if ((threadIdx.x & 63) == 0)
+                D.cNew[c] = cull ? 1 : 0;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_cull_apply(CullDev D)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.nR * D.nA)
+        return;
+    if (D.cNew[c]) {
+        D.offR[c] = kNoData;
+        D.offA[c] = kNoData;
+        atomicAdd(&D.counters[1], 1);
+    } else {
+        D.offR[c] = D.R0[c];
+        D.offA[c] = D.A0[c];
+    }
+}
+
+// cullStats (cullStats.c:19-144) with the plane fit done by a one-sided
+// Jacobi SVD whose rotated columns are recomputed from the box rows (the
+// oracle follows the same arithmetic).  Sweeps stop at column-pair cosines
+// below 1e-13, just above the rounding floor of the 81-row sums.
+__device__ void lsq3(double c0, const CullDev &D, int i1, int i2, int j1, int j2, bool useR, double x[3])
+{
+    double V[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int sweep = 0; sweep < 30; sweep++) {
+        double off = 0.0;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                double alpha = 0, beta = 0, gamma = 0;
+                for (int ii = i1; ii <= i2; ii++)
+                    for (int jj = j1; jj <= j2; jj++) {
+                        const int o = ii * D.nR + jj;
+                        if (!valid2(D.offR[o], D.offA[o]))
+                            continue;
+                        const double a0 = c0, a1 = (double)ii, a2 = (double)jj;
+                        const double up = a0 * V[0 * 3 + p] + a1 * V[1 * 3 + p] + a2 * V[2 * 3 + p];
+                        const double uq = a0 * V[0 * 3 + q] + a1 * V[1 * 3 + q] + a2 * V[2 * 3 + q];
+                        alpha += up * up;
+                        beta += uq * uq;
+                        gamma += up * uq;
+                    }
+                if (gamma == 0.0)
+                    continue;
+                const double conv = fabs(gamma) / sqrt(alpha * beta);
+                if (conv > off)
+                    off = conv;
+                if (conv < 1e-13)
+                    continue;
+                const double zeta = (beta - alpha) / (2.0 * gamma);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                for (int r = 0; r < 3; r++) {
+                    const double vp = V[r * 3 + p], vq = V[r * 3 + q];
+                    V[r * 3 + p] = c * vp - s * vq;
+                    V[r * 3 + q] = s * vp + c * vq;
+                }
+            }
+        if (off < 1e-13)
+            break;
+    }
+    double w2[3] = {0, 0, 0}, uy[3] = {0, 0, 0};
+    for (int ii = i1; ii <= i2; ii++)
+        for (int jj = j1; jj <= j2; jj++) {
+            const int o = ii * D.nR + jj;
+            if (!valid2(D.offR[o], D.offA[o]))
+                continue;
+            const double yv = useR ? (double)D.offR[o] : (double)D.offA[o];
+            const double a0 = c0, a1 = (double)ii, a2 = (double)jj;
+            for (int k = 0; k < 3; k++) {
+                const double u = a0 * V[0 * 3 + k] + a1 * V[1 * 3 + k] + a2 * V[2 * 3 + k];
+                w2[k] += u * u;
+                uy[k] += u * yv;
+            }
+        }
+    double w[3], wmax = 0;
+    for (int k = 0; k < 3; k++) {
+        w[k] = sqrt(w2[k]);
+        if (w[k] > wmax)
+            wmax = w[k];
+    }
+    const double thresh = 1.0e-5 * wmax;
+    double coef[3];
+    for (int k = 0; k < 3; k++)
+        coef[k] = (w[k] < thresh || w[k] == 0.0) ? 0.0 : uy[k] / w2[k];
+    for (int r = 0; r < 3; r++)
+        x[r] = V[r * 3 + 0] * coef[0] + V[r * 3 + 1] * coef[1] + V[r * 3 + 2] * coef[2];
+}
+
+__global__ void k_cull_stats(CullDev D)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.nR * D.nA)
+        return;
+    const int b = 9;
+    const int i = c / D.nR, j = c - (c / D.nR) * D.nR;
+    const int i1 = imax(0, i - b / 2), i2 = imin(D.nA - 1, i + b / 2);
+    const int j1 = imax(0, j - b / 2), j2 = imin(D.nR - 1, j + b / 2);
+    int ngood = 0;
+    double meanA = 0.0, meanR = 0.0;
+    for (int ii = i1; ii <= i2; ii++)
+        for (int jj = j1; jj <= j2; jj++) {
+            const int o = ii * D.nR + jj;
+            if (valid2(D.offR[o], D.offA[o])) {
+                ngood++;
+                meanA += D.offA[o];
+                meanR += D.offR[o];
+            }
+        }
+    meanA /= (double)ngood;
+    meanR /= (double)ngood;
+    if (!(ngood > 8 && D.offR[c] > (float)(1 - ST_LARGEINT))) {
+        D.sigmaA[c] = kNoData;
+        D.sigmaR[c] = kNoData;
+        return;
+    }
+    double aR[3], aA[3];
+    lsq3((double)(float)meanR, D, i1, i2, j1, j2, true, aR);
+    lsq3((double)(float)meanA, D, i1, i2, j1, j2, false, aA);
+    aR[0] *= meanR;
+    aA[0] *= meanA;
+    int ng = 0;
+    double mA = 0.0, mR = 0.0, varR = 0.0, varA = 0.0;
+    for (int ii = i1; ii <= i2; ii++)
+        for (int jj = j1; jj <= j2; jj++) {
+            const int o = ii * D.nR + jj;
+            if (valid2(D.offR[o], D.offA[o])) {
+                ng++;
+                float oR = D.offR[o], oA = D.offA[o];
+                oR = (float)(oR - (aR[0] + aR[1] * (float)ii + aR[2] * (float)jj));
+                oA = (float)(oA - (aA[0] + aA[1] * (float)ii + aA[2] * (float)jj));
+                mA += oA;
+                mR += oR;
+                varA += oA * oA;
+                varR += oR * oR;
+            }
+        }
+    float sA = (float)(varA / (float)(ng - 3));
+    float sR = (float)(varR / (float)(ng - 3));
+    sA = (float)(sA - mA * mA / (float)((ng - 3) * ng));
+    sR = (float)(sR - mR * mR / (float)((ng - 3) * ng));
+    sA = (float)(0.00001 > sA ? 0.00001 : sA);
+    sR = (float)(0.00001 > sR ? 0.00001 : sR);
+    D.sigmaA[c] = (float)sqrt((double)sA);
+    D.sigmaR[c] = (float)sqrt((double)sR);
+}
+
+// cullSmooth (cullSmooth.c:29-103)
+__device__ inline float s_weight(int m, int s)
+{
+    const float w2 = (float)((double)s / 2.0 - 0.01);
+    return (m < -w2 || m > w2) ? 0.5f : 1.0f;
+}
+
+__global__ void k_cull_smooth(CullDev D, int sR, int sA)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.nR * D.nA)
+        return;
+    const float nlooks[4] = {0.0f, 1.0f, 0.25f, 0.015f};
+    const int i = c / D.nR, j = c - (c / D.nR) * D.nR;
+    const int i1 = imax(0, i - sA / 2), i2 = imin(D.nA - 1, i + sA / 2);
+    const int j1 = imax(0, j - sR / 2), j2 = imin(D.nR - 1, j + sR / 2);
+    double ngood = 0, meanA = 0.0, meanR = 0.0;
+    float nLooksEff = 0.0f;
+    for (int ii = i1; ii <= i2; ii++)
+        for (int jj = j1; jj <= j2; jj++) {
+            const int o = ii * D.nR + jj;
+            if (valid2(D.offR[o], D.offA[o])) {
+                const double w = (double)(s_weight(ii - i, sA) * s_weight(jj - j, sR));
+                ngood += w;
+                nLooksEff = (float)(nLooksEff + w * nlooks[(int)D.type[o] & 3]);
+                meanA += w * D.offA[o];
+                meanR += w * D.offR[o];
+            }
+        }
+    if (((ngood > 1) || (sA == 1) || (sR == 1)) && D.offR[c] > (float)(1 - ST_LARGEINT) &&
+        D.sigmaA[c] > (float)(1 - ST_LARGEINT)) {
+        nLooksEff = (float)(nLooksEff > 1. ? nLooksEff : 1.);
+        D.sigmaA[c] = (float)(D.sigmaA[c] / sqrt((double)nLooksEff));
+        D.sigmaR[c] = (float)(D.sigmaR[c] / sqrt((double)nLooksEff));
+        D.offRS[c] = (float)(meanR / (float)ngood);
+        D.offAS[c] = (float)(meanA / (float)ngood);
+    } else {
+        D.offRS[c] = kNoData;
+        D.offAS[c] = kNoData;
+    }
+}
+
+// cullIslands (cullIslands.c:13-198): 4-connected components of valid
+// smoothed cells by atomic union-find.  Roots are the smallest member
+// index, i.e. the component's first cell in raster order, which is the
+// order a raster-scan labeller numbers components in; the component with
+// the largest root therefore carries the last label, which the reference
+// leaves alone (the loops stop at maxLabel - 1).
+__device__ inline bool isl_valid(const CullDev &D, int c) { return D.offRS[c] >= (float)(-ST_LARGEINT + 1000); }
+
+__device__ inline int uf_find(int *par, int x)
+{
+    int p = par[x];
+    while (p != x) {
+        x = p;
+        p = par[x];
+    }
+    return x;
+}
+
+__device__ inline void uf_unite(int *par, int a, int b)
+{
+    while (true) {
+        a = uf_find(par, a);
+        b = uf_find(par, b);
+        if (a == b)
+            return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(&par[a], a, b) == a)
+            return;
+    }
+}
+
+__global__ void k_isl_init(CullDev D)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.nR * D.nA)
+        return;
+    D.parent[c] = c;
+}
+
+__global__ void k_isl_merge(CullDev D)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.nR * D.nA || !isl_valid(D, c))
+        return;
+    const int j = c % D.nR;
+    if (j + 1 < D.nR && isl_valid(D, c + 1))
+        uf_unite(D.parent, c, c + 1);
+    if (c + D.nR < D.nR * D.nA && isl_valid(D, c + D.nR))
+        uf_unite(D.parent, c, c + D.nR);
+}
+
+__global__ void k_isl_bbox(CullDev D)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.nR * D.nA || !isl_valid(D, c))
+        return;
+    const int r = uf_find(D.parent, c);
+    D.parent[c] = r;
+    const int i = c / D.nR, j = c - (c / D.nR) * D.nR;
+    atomicMin(&D.bbox[4 * r + 0], j);
+    atomicMax(&D.bbox[4 * r + 1], j);
+    atomicMin(&D.bbox[4 * r + 2], i);
+    atomicMax(&D.bbox[4 * r + 3], i);
+    atomicMax(&D.counters[2], r);
+}
+
+__global__ void k_isl_bbox_init(CullDev D)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.nR * D.nA)
+        return;
+    D.bbox[4 * c + 0] = 0x7fffffff;
+    D.bbox[4 * c + 1] = -0x7fffffff;
+    D.bbox[4 * c + 2] = 0x7fffffff;
+    D.bbox[4 * c + 3] = -0x7fffffff;
+}
+
+__global__ void k_isl_remove(CullDev D, int thresh)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.nR * D.nA || !isl_valid(D, c))
+        return;
+    const int r = D.parent[c];
+    if (r == D.counters[2])
+        return;
+    const int *bb = D.bbox + 4 * r;
+    if ((bb[1] - bb[0]) <= thresh && (bb[3] - bb[2]) <= thresh) {
+        D.offRS[c] = kNoData;
+        if (D.offAS)
+            D.offAS[c] = kNoData;
+    }
+}
+
+inline unsigned nb(int n) { return (unsigned)((n + 255) / 256); }
+
+// Fixpoint iterations before the wavefront kernel takes over;
+// SPECKLE_AMD_CULL_ITERS lowers it (0 runs every pass on the wavefront
+// kernel), which the tests use to cover both schedules.
+int max_iterations()
+{
+    const char *e = getenv("SPECKLE_AMD_CULL_ITERS");
+    if (!e || !*e)
+        return kMaxIter;
+    const int v = atoi(e);
+    return v < 0 ? 0 : (v > kMaxIter ? kMaxIter : v);
+}
+
+struct CullRun {
+    st_ctx *ctx;
+    st_cull_params *cp;
+    CullDev D{};
+    int n;
+
+    CullRun(st_ctx *c, st_cull_params *p) : ctx(c), cp(p)
+    {
+        ST_REQUIRE(cp && cp->nR > 0 && cp->nA > 0 && cp->offR && cp->offA, ST_ERR_ARG, "cull grids");
+        n = cp->nR * cp->nA;
+        ctx->arena_reserve((size_t)n * 64 + 65536);
+        Arena &ar = ctx->arena;
+        D.nR = cp->nR;
+        D.nA = cp->nA;
+        D.offR = ar.take<float>(n);
+        D.offA = ar.take<float>(n);
+        D.corr = ar.take<float>(n);
+        D.type = ar.take<uint8_t>(n);
+        D.mask = nullptr;
+        D.offRS = ar.take<float>(n);
+        D.offAS = ar.take<float>(n);
+        D.sigmaR = ar.take<float>(n);
+        D.sigmaA = ar.take<float>(n);
+        D.R0 = ar.take<float>(n);
+        D.A0 = ar.take<float>(n);
+        D.cPrev = ar.take<uint8_t>(n);
+        D.cNew = ar.take<uint8_t>(n);
+        D.counters = ar.take<int>(16);
+        D.parent = ar.take<int>(n);
+        D.bbox = ar.take<int>(4 * (size_t)n);
+        if (cp->mask) {
+            uint8_t *m = ar.take<uint8_t>(n);
+            put(m, cp->mask, n);
+            D.mask = m;
+        }
+    }
+    template <class T>
+    void put(T *d, const T *h, size_t cnt)
+    {
+        if (h)
+            ST_HIP(hipMemcpyAsync(d, h, cnt * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    }
+    template <class T>
+    void get(T *h, const T *d, size_t cnt)
+    {
+        if (h)
+            ST_HIP(hipMemcpyAsync(h, d, cnt * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    int counter(int k)
+    {
+        int v = 0;
+        ST_HIP(hipMemcpyAsync(&v, D.counters + k, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        ST_HIP(hipStreamSynchronize(ctx->stream));
+        return v;
+    }
+
+    void st_data()
+    {
+        ST_REQUIRE(cp->bR >= 1 && cp->bA >= 1 && (cp->bR / 2) * 2 + 1 <= kMaxBox && (cp->bA / 2) * 2 + 1 <= kMaxBox,
+                   ST_ERR_UNSUPPORTED, "box sizes above 15");
+        hipStream_t st = ctx->stream;
+        hipLaunchKernelGGL(k_cull_prepass, dim3(nb(n)), dim3(256), 0, st, D, cp->corrThresh, cp->singleMT);
+        int it = 0;
+        bool settled = false;
+        const int maxIter = max_iterations();
+        for (; it < maxIter; it++) {
+            ST_HIP(hipMemsetAsync(D.counters, 0, sizeof(int), st));
+            hipLaunchKernelGGL(k_cull_iter, dim3((n + 3) / 4), dim3(256), 0, st, D, cp->bR, cp->bA, cp->nGood,
+                               cp->maxA, cp->maxR);
+            ST_HIP(hipGetLastError());
+            std::swap(D.cPrev, D.cNew);
+            if (counter(0) == 0) {
+                settled = true;
+                break;
+            }
+        }
+        std::swap(D.cPrev, D.cNew);  // cNew holds the last decisions
+        if (!settled)
+            hipLaunchKernelGGL(k_cull_wavefront, dim3(1), dim3(1024), 0, st, D, cp->bR, cp->bA, cp->nGood, cp->maxA,
+                               cp->maxR);
+        ST_HIP(hipMemsetAsync(D.counters + 1, 0, sizeof(int), st));
+        hipLaunchKernelGGL(k_cull_apply, dim3(nb(n)), dim3(256), 0, st, D);
+        ST_HIP(hipGetLastError());
+        cp->nCulled = counter(1);
+    }
+    void stats()
+    {
+        cp->bR = cp->bA = 9;  // cullStats.c:38-39
+        hipLaunchKernelGGL(k_cull_stats, dim3(nb(n)), dim3(256), 0, ctx->stream, D);
+        ST_HIP(hipGetLastError());
+    }
+    void smooth()
+    {
+        ST_REQUIRE(cp->sR >= 1 && cp->sA >= 1, ST_ERR_ARG, "smoothing window");
+        hipLaunchKernelGGL(k_cull_smooth, dim3(nb(n)), dim3(256), 0, ctx->stream, D, cp->sR, cp->sA);
+        ST_HIP(hipGetLastError());
+    }
+    void islands()
+    {
+        hipStream_t st = ctx->stream;
+        ST_HIP(hipMemsetAsync(D.counters + 2, 0xff, sizeof(int), st));  // max root = -1
+        hipLaunchKernelGGL(k_isl_init, dim3(nb(n)), dim3(256), 0, st, D);
+        hipLaunchKernelGGL(k_isl_bbox_init, dim3(nb(n)), dim3(256), 0, st, D);
+        hipLaunchKernelGGL(k_isl_merge, dim3(nb(n)), dim3(256), 0, st, D);
+        hipLaunchKernelGGL(k_isl_bbox, dim3(nb(n)), dim3(256), 0, st, D);
+        hipLaunchKernelGGL(k_isl_remove, dim3(nb(n)), dim3(256), 0, st, D, cp->islandThresh);
+        ST_HIP(hipGetLastError());
+    }
+};
+
+}  // namespace
+
+extern "C" int st_cull_st_data(st_ctx *c, st_cull_params *cp)
+{
+    return st_guard(c, [&]() {
+        CullRun R(c, cp);
+        ST_REQUIRE(cp->corr && cp->type, ST_ERR_ARG, "corr/type");
+        R.put(R.D.offR, cp->offR, R.n);
+        R.put(R.D.offA, cp->offA, R.n);
+        R.put(R.D.corr, cp->corr, R.n);
+        R.put(R.D.type, cp->type, R.n);
+        R.st_data();
+        R.get(cp->offR, R.D.offR, R.n);
+        R.get(cp->offA, R.D.offA, R.n);
+        R.get(cp->type, R.D.type, R.n);
+        ST_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int st_cull_stats(st_ctx *c, st_cull_params *cp)
+{
+    return st_guard(c, [&]() {
+        CullRun R(c, cp);
+        ST_REQUIRE(cp->sigmaR && cp->sigmaA, ST_ERR_ARG, "sigma outputs");
+        R.put(R.D.offR, cp->offR, R.n);
+        R.put(R.D.offA, cp->offA, R.n);
+        R.stats();
+        R.get(cp->sigmaR, R.D.sigmaR, R.n);
+        R.get(cp->sigmaA, R.D.sigmaA, R.n);
+        ST_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int st_cull_smooth(st_ctx *c, st_cull_params *cp)
+{
+    return st_guard(c, [&]() {
+        CullRun R(c, cp);
+        ST_REQUIRE(cp->sigmaR && cp->sigmaA && cp->offRS && cp->offAS && cp->type, ST_ERR_ARG, "smooth arrays");
+        R.put(R.D.offR, cp->offR, R.n);
+        R.put(R.D.offA, cp->offA, R.n);
+        R.put(R.D.type, cp->type, R.n);
+        R.put(R.D.sigmaR, cp->sigmaR, R.n);
+        R.put(R.D.sigmaA, cp->sigmaA, R.n);
+        R.smooth();
+        R.get(cp->sigmaR, R.D.sigmaR, R.n);
+        R.get(cp->sigmaA, R.D.sigmaA, R.n);
+        R.get(cp->offRS, R.D.offRS, R.n);
+        R.get(cp->offAS, R.D.offAS, R.n);
+        ST_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int st_cull_islands(st_ctx *c, st_cull_params *cp)
+{
+    return st_guard(c, [&]() {
+        CullRun R(c, cp);
+        ST_REQUIRE(cp->offRS, ST_ERR_ARG, "offRS");
+        R.put(R.D.offRS, cp->offRS, R.n);
+        if (cp->offAS)
+            R.put(R.D.offAS, cp->offAS, R.n);
+        else
+            R.D.offAS = nullptr;
+        R.islands();
+        R.get(cp->offRS, R.D.offRS, R.n);
+        if (cp->offAS)
+            R.get(cp->offAS, R.D.offAS, R.n);
+        ST_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int st_cull_run(st_ctx *c, st_cull_params *cp)
+{
+    return st_guard(c, [&]() {
+        CullRun R(c, cp);
+        ST_REQUIRE(cp->corr && cp->type && cp->sigmaR && cp->sigmaA && cp->offRS && cp->offAS, ST_ERR_ARG,
+                   "cull arrays");
+        R.put(R.D.offR, cp->offR, R.n);
+        R.put(R.D.offA, cp->offA, R.n);
+        R.put(R.D.corr, cp->corr, R.n);
+        R.put(R.D.type, cp->type, R.n);
+        R.st_data();
+        R.st_data();
+        R.st_data();
+        R.stats();
+        R.smooth();
+        if (cp->islandThresh > 0)
+            R.islands();
+        R.get(cp->offR, R.D.offR, R.n);
+        R.get(cp->offA, R.D.offA, R.n);
+        R.get(cp->type, R.D.type, R.n);
+        R.get(cp->sigmaR, R.D.sigmaR, R.n);
+        R.get(cp->sigmaA, R.D.sigmaA, R.n);
+        R.get(cp->offRS, R.D.offRS, R.n);
+        R.get(cp->offAS, R.D.offAS, R.n);
+        ST_HIP(hipStreamSynchronize(c->stream));
+    });
+}
