@@ -50,6 +50,38 @@ def flops_amp_patch(wa):
     return (2 * 5 * n1 * math.log2(n1) + 5 * 5 * n2 * math.log2(n2) + 6 * n2 + 8 * n2 + f_os)
 
 
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "k_cpx_fused_fullscene_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused complex-match kernel, from the
+    committed FETCH_SIZE / WRITE_SIZE profile of this workload (None when
+    absent)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return int(float(ln.split(":")[1]))
+    except OSError:
+        pass
+    return None
+
+
+def calibrate_copy(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of known size, for scaling the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) of a profiled run."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "GB_per_s": 2 * nbytes / dt / 1e9}))
+
+
 def cpu_baseline(spec, par, dev, pid, budget_s=15.0):
     """Time the FP32 CPU restatement (single thread) on a strip of grid
     rows of the same resident scene."""
@@ -93,11 +125,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=1, help="divide the scene sides (testing only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 1 GiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(sharding design in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     from specklesource_amd import configs, track
     spec, par = configs.cfg2(scale=args.scale)
@@ -135,11 +172,20 @@ def main():
     value = ncorr / elapsed
     st = dev.last_stats()
     ncpx = st.nCorr[0]
-    # roofline of the complex-match stage: algorithmic flops of the complex
-    # patch correlations over the stage's device time
-    cpx_ms = stages[1] / args.steps
-    achieved = flops_complex_patch(par.wR) * ncpx / (cpx_ms * 1e-3) / 1e12 if cpx_ms > 0 else 0.0
+    # roofline of the dominant kernel: the fused complex-match kernel does
+    # every complex patch correlation of a job in one launch, so its
+    # algorithmic flops per launch are F_c(w) x complex correlations per job,
+    # over its average launch duration (HIP events on the library's stream)
     dom = max(kern.items(), key=lambda kv: kv[1][0]) if kern else ("none", (0, 0))
+    if "k_cpx_fused" in kern and kern["k_cpx_fused"][1] > 0:
+        kname = "k_cpx_fused"
+        launch_ms = kern[kname][0] / kern[kname][1]
+        per_launch = flops_complex_patch(par.wR) * ncpx * args.steps / kern[kname][1]
+    else:
+        kname = "complex-match stage"
+        launch_ms = stages[1] / args.steps
+        per_launch = flops_complex_patch(par.wR) * ncpx
+    achieved = per_launch / (launch_ms * 1e-3) / 1e12 if launch_ms > 0 else 0.0
     line = {
         "metric": "patch correlations/sec + p99 job latency, 1 MI355X, at <=0.02 px offset error",
         "value": value,
@@ -163,8 +209,9 @@ def main():
                      "amp_small": stages[2] / args.steps, "amp_large": stages[3] / args.steps},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "complex-match stage (per-patch flops F_c(w) over the stage device time)",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(),
+                     "kernel": kname, "launch_ms": launch_ms, "flops_per_launch": per_launch,
+                     "algorithmic_bytes_per_launch": 2 * 8 * par.wR * par.wR * ncpx,
                      "dominant_kernel": dom[0]},
     }
     if not args.no_cpu:

@@ -1403,7 +1403,8 @@ T *upload(st_ctx *ctx, const T *h, size_t n)
 }
 
 template <int W, int STYPE>
-void launch_cpx_fused_t(st_ctx *ctx, const KPar *dP, const Pts *dS, const int *list, int count, const CpxBatch *dC)
+void launch_cpx_fused_t(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
+                        const CpxBatch *dC)
 {
     const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
     int perCU = 0, ncu = 0;
@@ -1411,13 +1412,16 @@ void launch_cpx_fused_t(st_ctx *ctx, const KPar *dP, const Pts *dS, const int *l
     ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     const int grid = std::max(1, std::min(count, std::max(1, perCU) * ncu));
     float *mag = ctx->arena.take<float>((size_t)grid * 4 * W * W);
+    ctx->time_begin(slot);
     hipLaunchKernelGGL((k_cpx_fused<W, STYPE>), dim3(grid), dim3(4 * W), lds, ctx->stream, dP, dS, list, count, dC,
                        mag);
+    ctx->time_end(slot);
     ST_HIP(hipGetLastError());
 }
 
 template <int W>
-void launch_cpx_fused(st_ctx *ctx, const KPar &P, const Pts &S, const int *list, int count, const CpxBatch &C)
+void launch_cpx_fused(st_ctx *ctx, int slot, const KPar &P, const Pts &S, const int *list, int count,
+                      const CpxBatch &C)
 {
     // parameter blocks live in device memory: read through the scalar cache
     // instead of occupying SGPRs for the whole kernel
@@ -1428,9 +1432,9 @@ void launch_cpx_fused(st_ctx *ctx, const KPar &P, const Pts &S, const int *list,
     ST_HIP(hipMemcpyAsync(dS, &S, sizeof(Pts), hipMemcpyHostToDevice, ctx->stream));
     ST_HIP(hipMemcpyAsync(dC, &C, sizeof(CpxBatch), hipMemcpyHostToDevice, ctx->stream));
     if (P.im.stype == ST_SAMPLE_C64)
-        launch_cpx_fused_t<W, ST_SAMPLE_C64>(ctx, dP, dS, list, count, dC);
+        launch_cpx_fused_t<W, ST_SAMPLE_C64>(ctx, slot, dP, dS, list, count, dC);
     else
-        launch_cpx_fused_t<W, ST_SAMPLE_CI16>(ctx, dP, dS, list, count, dC);
+        launch_cpx_fused_t<W, ST_SAMPLE_CI16>(ctx, slot, dP, dS, list, count, dC);
 }
 
 bool staged_only()
@@ -1593,14 +1597,12 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         C.have = ar.take<uint8_t>(nCpx);
         C.az = ar.take<int>(nCpx);
         C.p = ar.take<double>(2 * (size_t)nCpx);
-        ctx->time_begin(kPeak);
         if (w == 16)
-            launch_cpx_fused<16>(ctx, P, S, list, nCpx, C);
+            launch_cpx_fused<16>(ctx, kPeak, P, S, list, nCpx, C);
         else if (w == 32)
-            launch_cpx_fused<32>(ctx, P, S, list, nCpx, C);
+            launch_cpx_fused<32>(ctx, kPeak, P, S, list, nCpx, C);
         else
-            launch_cpx_fused<64>(ctx, P, S, list, nCpx, C);
-        ctx->time_end(kPeak);
+            launch_cpx_fused<64>(ctx, kPeak, P, S, list, nCpx, C);
         hipLaunchKernelGGL(k_lm, dim3((2 * nCpx + 127) / 128), dim3(128), 0, st, C.res, nCpx, C.sig);
         hipLaunchKernelGGL(k_cpx_finalize, dim3((nCpx + 127) / 128), dim3(128), 0, st, P, S, list, 0, nCpx, C);
         ST_HIP(hipGetLastError());
