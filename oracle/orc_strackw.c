@@ -431,12 +431,10 @@ static float track_point(orc_stw *s, int r1, int a1, int r2, int a2, float *rShi
     return cMax;
 }
 
-int orc_corr_track_fast(const st_track_params *par, const void *img1, int nr1, int na1, const void *img2, int nr2,
-                        int na2, int stype, st_track_out *out, st_track_diag *dg)
+static int stw_setup(orc_stw *s, const st_track_params *par, const void *img1, int nr1, int na1, const void *img2,
+                     int nr2, int na2, int stype)
 {
-    orc_stw S;
-    memset(&S, 0, sizeof(S));
-    orc_stw *s = &S;
+    memset(s, 0, sizeof(*s));
     s->par = par;
     s->img1 = img1;
     s->img2 = img2;
@@ -472,6 +470,82 @@ int orc_corr_track_fast(const st_track_params *par, const void *img1, int nr1, i
     s->sigmaS = (double *)calloc(nlag, sizeof(double));
     s->corrResult = (double *)calloc(nlag, sizeof(double));
     s->cFastOver = (ocplx *)calloc((size_t)(NFAST + 1) * NOVER * (NFAST + 1) * NOVER, sizeof(ocplx));
+    return ST_OK;
+}
+
+static void stw_free(orc_stw *s)
+{
+    free(s->im1in);
+    free(s->im2in);
+    free(s->img1c);
+    free(s->img2c);
+    free(s->cbuf);
+    free(s->dataR);
+    free(s->dataS);
+    free(s->mag);
+    free(s->tmpS1);
+    free(s->tmpS2);
+    free(s->meanS);
+    free(s->sigmaS);
+    free(s->corrResult);
+    free(s->cFastOver);
+}
+
+/* Test hook: one patch, returning the coarse NCC surface (nLagA x nLagR),
+ * the detected search (2wAa x 2wRa) and chip (wA2 x wR2) images and
+ * res = {cMax, rShift1, aShift1, azShift}. */
+int orc_stw_point(const st_track_params *par, const void *img1, int nr1, int na1, const void *img2, int nr2, int na2,
+                  int stype, int r1, int a1, int r2, int a2, double *corrResult, float *dataS, float *dataR,
+                  float *res)
+{
+    orc_stw S;
+    int rc = stw_setup(&S, par, img1, nr1, na1, img2, nr2, na2, stype);
+    if (rc != ST_OK)
+        return rc;
+    float rs = 0, as = 0;
+    int pI, pJ, fI, fJ;
+    float c = track_point(&S, r1, a1, r2, a2, &rs, &as, &pI, &pJ, &fI, &fJ);
+    size_t nlag = (size_t)S.nLagA * S.nLagR;
+    for (size_t k = 0; corrResult && k < nlag; k++)
+        corrResult[k] = S.corrResult[k];
+    for (size_t k = 0; dataS && k < (size_t)S.W_A * S.W_R; k++)
+        dataS[k] = (float)S.dataS[k];
+    for (size_t k = 0; dataR && k < (size_t)S.wA2 * S.wR2; k++)
+        dataR[k] = (float)S.dataR[k];
+    res[0] = c;
+    res[1] = rs;
+    res[2] = as;
+    res[3] = (float)S.azShift;
+    stw_free(&S);
+    return ST_OK;
+}
+
+/* Test hook: overSampleC of strackw on a 21 x 21 real patch; pw receives
+ * |z|^2 of the 336 x 336 result. */
+void orc_stw_oversample(const float *c, float *pw)
+{
+    orc_stw S;
+    memset(&S, 0, sizeof(S));
+    const int n = NFAST + 1, N = n * NOVER;
+    S.cFastOver = (ocplx *)calloc((size_t)N * N, sizeof(ocplx));
+    for (int k = 0; k < n * n; k++) {
+        S.cFast[k].re = (oreal)c[k];
+        S.cFast[k].im = 0;
+    }
+    over_sample_w(&S);
+    for (int k = 0; k < N * N; k++)
+        pw[k] = (float)(S.cFastOver[k].re * S.cFastOver[k].re + S.cFastOver[k].im * S.cFastOver[k].im);
+    free(S.cFastOver);
+}
+
+int orc_corr_track_fast(const st_track_params *par, const void *img1, int nr1, int na1, const void *img2, int nr2,
+                        int na2, int stype, st_track_out *out, st_track_diag *dg)
+{
+    orc_stw S;
+    orc_stw *s = &S;
+    int rc = stw_setup(s, par, img1, nr1, na1, img2, nr2, na2, stype);
+    if (rc != ST_OK)
+        return rc;
     int nAmp = 0, nFail = 0;
     for (int i = 0; i < par->nA; i++) {
         int a1 = par->aStart + i * par->deltaA - par->wAa / 2;
@@ -535,19 +609,6 @@ int orc_corr_track_fast(const st_track_params *par, const void *img1, int nr1, i
     out->nAmp = nAmp;
     out->nAmpL = 0;
     out->nFail = nFail;
-    free(s->im1in);
-    free(s->im2in);
-    free(s->img1c);
-    free(s->img2c);
-    free(s->cbuf);
-    free(s->dataR);
-    free(s->dataS);
-    free(s->mag);
-    free(s->tmpS1);
-    free(s->tmpS2);
-    free(s->meanS);
-    free(s->sigmaS);
-    free(s->corrResult);
-    free(s->cFastOver);
+    stw_free(s);
     return ST_OK;
 }
