@@ -132,47 +132,6 @@ __global__ void k_geometry(KPar P, Pts S, const int *hostR2, const int *hostA2, 
     }
 }
 
-// Doppler estimate of a w x w patch pair (estDopCarrier1,
-// Strack/dopplerEstimation.c:25-88): lag-1 azimuth autocorrelation after
-// complex mean removal, in double.
-template <class Get>
-__device__ inline int doppler_shift(Get get, int wA, int wR, int nAz, int wAnorm, double *sh)
-{
-    const int N = wA * wR;
-    double m[4] = {0, 0, 0, 0};
-    for (int e = threadIdx.x; e < N; e += NT) {
-        float2 a, b;
-        get(e, a, b);
-        m[0] += a.x;
-        m[1] += a.y;
-        m[2] += b.x;
-        m[3] += b.y;
-    }
-    for (int k = 0; k < 4; k++)
-        m[k] = block_sum_d<NT>(m[k], sh) / ((double)wA * (double)wR);
-    double z[4] = {0, 0, 0, 0};
-    for (int e = threadIdx.x; e < N - wR; e += NT) {
-        float2 a0, b0, a1, b1;
-        get(e, a0, b0);
-        get(e + wR, a1, b1);
-        double ar = a1.x - m[0], ai = a1.y - m[1], br = a0.x - m[0], bi = a0.y - m[1];
-        z[0] += ar * br + ai * bi;
-        z[1] += ai * br - ar * bi;
-        ar = b1.x - m[2], ai = b1.y - m[3], br = b0.x - m[2], bi = b0.y - m[3];
-        z[2] += ar * br + ai * bi;
-        z[3] += ai * br - ar * bi;
-    }
-    for (int k = 0; k < 4; k++)
-        z[k] = block_sum_d<NT>(z[k], sh);
-    double ang1 = (z[0] == 0.0 && z[1] == 0.0) ? NAN : atan2(z[1], z[0]);
-    double ang2 = (z[2] == 0.0 && z[3] == 0.0) ? NAN : atan2(z[3], z[2]);
-    double v = (double)(nAz / 2) * (ang1 + ang2) * .5 / 3.14159265358979323846;
-    int shift = isnan(v) ? 0 : ((v >= 0.0) ? (int)(v + 0.5) : (int)(v - 0.5));
-    if (shift < 0)
-        shift += nAz;
-    return (int)((float)shift * (float)wAnorm / (float)nAz);
-}
-
 // ---------------------------------------------------------------------
 // complex stage: getPatches (speckleTrack.c:1194-1330) + Doppler +
 // phaseCorrectBaseline (speckleTrack.c:868-907).  One block per point.

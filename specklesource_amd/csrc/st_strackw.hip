@@ -1,13 +1,839 @@
-// strackw on the GPU (placeholder until the NCC tracker lands).
+// strackw on the GPU: the edge-padded normalised cross-correlation tracker
+// of Strackw/corrTrackFast.c (corrTrackFast :89-231, getCorrPatchesFast
+// :299-372, detectPatch :765-839, computeMeanSigma :407-477, ampMatchEdge
+// :903-947, getPeakCorr :862-899, correlateFast + overSampleC :482-551,
+// 951-990, corrMatch :376-404).
+//
+// Patches are processed in batches; every stage is a batched kernel or a
+// batched FFT on the context's engine:
+//   gather + Doppler -> FFT wa^2 (x2) -> rotated 2x spectral pad -> inverse
+//   FFT (2wa)^2 (x2) -> detection with boxcar into one packed complex image
+//   per patch (search + i chip) -> window statistics -> one forward FFT per
+//   patch -> cross spectra of two patches packed into one array -> one
+//   inverse FFT per two patches -> NCC over all lags + argmax -> 21 x 21 ->
+//   64 x 64 Dirichlet refinement in FP64 (the 336 x 336 inverse FFT of the
+//   reference evaluated only on the searched window).
+// The search and chip images are real, so one complex transform carries
+// both (spectra separated by Hermitian symmetry), and the real correlation
+// surfaces of two patches share one inverse transform.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
 #include "st_ctx.h"
+#include "st_track_common.h"
+
+using namespace sttrack;
+
+namespace {
+
+constexpr int WNOVER = 16;  // corrTrackFast.c:11
+constexpr int WNFAST = 20;  // corrTrackFast.c:12
+constexpr int WOS = 2;      // corrTrackFast.c:16
+constexpr int kN21 = WNFAST + 1;
+constexpr int kFineLo = (kN21 * WNOVER) / 2 - 2 * WNOVER;  // 136: searched oversampled window
+constexpr int kFineN = 4 * WNOVER;                         // 64
+constexpr int kTile = 32;
+constexpr int kMaxNavg = 8;
+constexpr int kAmpMatch = 2;
+// Dirichlet kernel of the 21 -> 336 zero pad, D(t), t = kDirLo .. kDirLo + kDirN - 1
+constexpr int kDirLo = kFineLo - WNOVER * WNFAST;  // -184
+constexpr int kDirN = kFineN + WNOVER * WNFAST;    // 384
+
+__constant__ double c_dirw[kDirN];
+
+struct WPar {
+    Images im;
+    int wRa, wAa, W_R, W_A, epR, epA, wR2, wA2, nLagR, nLagA;
+    int navgR, navgA;
+    float scale;
+    double scaleW;
+};
+
+struct WRes {
+    float cMax, rShift1, aShift1, gapCoarse, gapFine;
+    int pI, pJ, fI, fJ, az;
+};
+
+struct RowBest {
+    double v, second;
+    int rank, pad;
+};
+
+struct WBatch {
+    const int4 *pos;  // r1, a1, r2, a2 of every listed patch
+    float2 *P;        // [2][B][wAa * wRa]
+    float2 *U;        // [2][B][W_A * W_R]; later X: [ceil(B/2)][W_A * W_R]
+    float2 *Z;        // [B][W_A * W_R]: search + i chip
+    double *V1, *V2;  // [B][nLagA][W_R]: vertical window sums of s, s^2
+    double *chip;     // [B][2]: chip mean, variance
+    float *cr;        // [B][nLagA][nLagR]: NCC surface
+    RowBest *rows;    // [B][nLagA]
+    int *az;          // [B]
+    WRes *res;        // [B]
+};
+
+__device__ inline float2 sample0(const Images &im, bool second, int a, int r)
+{
+    const int nr = second ? im.nr2 : im.nr1, na = second ? im.na2 : im.na1;
+    if (a < 0 || a >= na || r < 0 || r >= nr)
+        return make_float2(0.f, 0.f);
+    return load_sample(second ? im.img2 : im.img1, im.stype, nr, a, r);
+}
+
+// getCorrPatchesFast (corrTrackFast.c:299-372): copy both windows (samples
+// outside an image read as zero) and estimate the Doppler shift.
+__global__ void __launch_bounds__(NT) k_w_gather(WPar p, int off, WBatch Bt, int B)
+{
+    __shared__ double sh[NT / 64];
+    const int b = blockIdx.x;
+    const int4 q = Bt.pos[off + b];
+    const int n = p.wAa * p.wRa;
+    float2 *P1 = Bt.P + (size_t)b * n, *P2 = Bt.P + ((size_t)B + b) * n;
+    for (int e = threadIdx.x; e < n; e += NT) {
+        const int i = e / p.wRa, j = e - (e / p.wRa) * p.wRa;
+        P1[e] = sample0(p.im, false, q.y + i, q.x + j);
+        P2[e] = sample0(p.im, true, q.w + i, q.z + j);
+    }
+    auto get = [&](int e, float2 &a, float2 &c) {
+        const int i = e / p.wRa, j = e - (e / p.wRa) * p.wRa;
+        a = sample0(p.im, false, q.y + i, q.x + j);
+        c = sample0(p.im, true, q.w + i, q.z + j);
+    };
+    int az = doppler_shift(get, p.wAa, p.wRa, p.wAa, p.wAa, sh);
+    if (az < 0 || az >= p.wAa)
+        az = 0;
+    if (threadIdx.x == 0)
+        Bt.az[b] = az;
+}
+
+// zeroPad (corrTrackFast.c:738-761): spectrum into the corners of the
+// (2wa)^2 array, azimuth rows rotated by azShift, Nyquist on the negative
+// side.
+__global__ void k_w_pad(WPar p, WBatch Bt, int B)
+{
+    const int z = blockIdx.y;  // 0 .. 2B-1
+    const int b = z % B;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int NN = p.W_A * p.W_R;
+    if (e >= NN)
+        return;
+    const int hA = p.wAa / 2, hR = p.wRa / 2;
+    const int n = e / p.W_R, m = e - (e / p.W_R) * p.W_R;
+    const int k = n < hA ? n : (n >= p.W_A - hA ? n - p.W_A : (1 << 20));
+    const int l = m < hR ? m : (m >= p.W_R - hR ? m - p.W_R : (1 << 20));
+    float2 v = make_float2(0.f, 0.f);
+    if (k != (1 << 20) && l != (1 << 20)) {
+        const int ka = (k + p.wAa + Bt.az[b]) % p.wAa, la = (l + p.wRa) % p.wRa;
+        v = Bt.P[(size_t)z * p.wAa * p.wRa + ka * p.wRa + la];
+    }
+    Bt.U[(size_t)z * NN + e] = v;
+}
+
+#pragma clang fp contract(off)
+// detectPatch (corrTrackFast.c:765-839) of the search (image 1, whole
+// array, boxcar clamped to the array) and the chip (image 2, inner region,
+// boxcar clamped as the reference does, with the azimuth bound written in
+// terms of edgePadR); sqrt; packed as search + i chip with the chip's
+// edge-pad frame zero.
+__global__ void __launch_bounds__(256) k_w_detect(WPar p, WBatch Bt, int B)
+{
+    __shared__ float d1[(kTile + 2 * kMaxNavg) * (kTile + 2 * kMaxNavg)];
+    __shared__ float d2[(kTile + 2 * kMaxNavg) * (kTile + 2 * kMaxNavg)];
+    const int b = blockIdx.z;
+    const int NN = p.W_A * p.W_R;
+    const float2 *U1 = Bt.U + (size_t)b * NN, *U2 = Bt.U + ((size_t)B + b) * NN;
+    const bool box = p.navgA > 1 || p.navgR > 1;
+    const int hA = box ? p.navgA : 0, hR = box ? p.navgR : 0;
+    const int y0 = blockIdx.y * kTile, x0 = blockIdx.x * kTile;
+    const int TH = kTile + 2 * hA, TW = kTile + 2 * hR;
+    const int c0 = p.epA * WOS, c1 = c0 + p.wA2;  // chip rows [c0, c1)
+    const int e0 = p.epR * WOS, e1 = e0 + p.wR2;  // chip cols [e0, e1)
+    const bool chipTile = y0 < c1 && y0 + kTile > c0 && x0 < e1 && x0 + kTile > e0;
+    const int kmaxC = p.epR * WOS + p.wA2 + p.navgA - 1;
+    const int lmaxC = p.epR * WOS + p.wR2 + p.navgR - 1;
+    for (int t = threadIdx.x; t < TH * TW; t += blockDim.x) {
+        const int ty = t / TW, tx = t - (t / TW) * TW;
+        const int y = y0 - hA + ty, x = x0 - hR + tx;
+        const int ky = imin(imax(y, 0), p.W_A - 1), kx = imin(imax(x, 0), p.W_R - 1);
+        const float2 z1 = U1[(size_t)ky * p.W_R + kx];
+        d1[t] = (z1.x * z1.x + z1.y * z1.y) * p.scale;
+        if (chipTile) {
+            const int ky2 = imin(imax(y, 0), kmaxC), kx2 = imin(imax(x, 0), lmaxC);
+            const float2 z2 = U2[(size_t)imin(ky2, p.W_A - 1) * p.W_R + imin(kx2, p.W_R - 1)];
+            d2[t] = (z2.x * z2.x + z2.y * z2.y) * p.scale;
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kTile * kTile; t += blockDim.x) {
+        const int oy = t / kTile, ox = t - (t / kTile) * kTile;
+        const int y = y0 + oy, x = x0 + ox;
+        if (y >= p.W_A || x >= p.W_R)
+            continue;
+        float v1 = 0.f;
+        for (int m = -hA; m <= hA; m++)
+            for (int n = -hR; n <= hR; n++)
+                v1 += d1[(oy + hA + m) * TW + ox + hR + n];
+        float s2 = 0.f;
+        if (chipTile && y >= c0 && y < c1 && x >= e0 && x < e1) {
+            float v2 = 0.f;
+            for (int m = -hA; m <= hA; m++)
+                for (int n = -hR; n <= hR; n++)
+                    v2 += d2[(oy + hA + m) * TW + ox + hR + n];
+            s2 = (float)sqrt((double)v2);
+        }
+        Bt.Z[(size_t)b * NN + (size_t)y * p.W_R + x] = make_float2((float)sqrt((double)v1), s2);
+    }
+}
+
+// computeMeanSigma (corrTrackFast.c:407-477), search part: running sums of
+// s and s^2 (FP64) down every column over windows of wA2 rows.
+__global__ void k_w_vsum(WPar p, WBatch Bt)
+{
+    const int b = blockIdx.y;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.W_R)
+        return;
+    const float2 *Zb = Bt.Z + (size_t)b * p.W_A * p.W_R;
+    double *V1 = Bt.V1 + (size_t)b * p.nLagA * p.W_R, *V2 = Bt.V2 + (size_t)b * p.nLagA * p.W_R;
+    double s1 = 0.0, s2 = 0.0;
+    for (int l = 0; l < p.W_A; l++) {
+        const float d = Zb[(size_t)l * p.W_R + c].x;
+        s1 += d;
+        s2 += d * d;
+        if (l >= p.wA2 - 1) {
+            const int i = l - (p.wA2 - 1);
+            V1[(size_t)i * p.W_R + c] = s1;
+            V2[(size_t)i * p.W_R + c] = s2;
+            const float e = Zb[(size_t)i * p.W_R + c].x;
+            s1 -= e;
+            s2 -= e * e;
+        }
+    }
+}
+
+// Chip mean and variance (computeMeanSigma, corrTrackFast.c:418-431):
+// sigma = sum(d^2)/(N-1) - mean^2.
+__global__ void __launch_bounds__(NT) k_w_chipstats(WPar p, WBatch Bt)
+{
+    __shared__ double sh[NT / 64];
+    const int b = blockIdx.x;
+    const float2 *Zb = Bt.Z + (size_t)b * p.W_A * p.W_R;
+    const int n = p.wA2 * p.wR2;
+    double s1 = 0.0, s2 = 0.0;
+    for (int e = threadIdx.x; e < n; e += NT) {
+        const int i = e / p.wR2, j = e - (e / p.wR2) * p.wR2;
+        const float d = Zb[(size_t)(p.epA * WOS + i) * p.W_R + p.epR * WOS + j].y;
+        s1 += d;
+        s2 += d * d;
+    }
+    s1 = block_sum_d<NT>(s1, sh);
+    s2 = block_sum_d<NT>(s2, sh);
+    if (threadIdx.x == 0) {
+        const double mean = s1 / (double)n;
+        double sig = s2 / (double)(n - 1.);
+        sig -= mean * mean;
+        Bt.chip[2 * b] = mean;
+        Bt.chip[2 * b + 1] = sig;
+    }
+}
+
+// ampMatchEdge (corrTrackFast.c:903-947) cross spectra.  Z = s + i c holds
+// the search and chip images of one patch: S(k) = (Z(k) + Z*(-k)) / 2,
+// C(k) = (Z(k) - Z*(-k)) / 2i, G = S conj(C).  Patches 2q and 2q+1 are
+// packed as G0 + i G1, whose inverse transform is g0 + i g1 (both real).
+__global__ void k_w_cross(WPar p, WBatch Bt, int B)
+{
+    const int q = blockIdx.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int NN = p.W_A * p.W_R;
+    if (e >= NN)
+        return;
+    const int ky = e / p.W_R, kx = e - (e / p.W_R) * p.W_R;
+    const int ny = ky == 0 ? 0 : p.W_A - ky, nx = kx == 0 ? 0 : p.W_R - kx;
+    const int en = ny * p.W_R + nx;
+    float2 G[2];
+    for (int h = 0; h < 2; h++) {
+        const int b = 2 * q + h;
+        if (b >= B) {
+            G[h] = make_float2(0.f, 0.f);
+            continue;
+        }
+        const float2 a = Bt.Z[(size_t)b * NN + e];
+        const float2 c = Bt.Z[(size_t)b * NN + en];  // Z(-k); conj below
+        const float2 S = make_float2(0.5f * (a.x + c.x), 0.5f * (a.y - c.y));
+        const float2 Cc = make_float2(0.5f * (a.y + c.y), 0.5f * (c.x - a.x));
+        G[h] = cmulconj(S, Cc);
+    }
+    Bt.U[(size_t)q * NN + e] = make_float2(G[0].x - G[1].y, G[0].y + G[1].x);
+}
+
+// Block-wide inclusive scan of doubles (256 threads).
+__device__ inline double block_scan_d(double v, double *sh)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        const double u = __shfl_up(v, o);
+        if (lane >= o)
+            v += u;
+    }
+    __syncthreads();
+    if (lane == 63)
+        sh[wid] = v;
+    __syncthreads();
+    double pre = 0.0;
+    for (int w = 0; w < wid; w++)
+        pre += sh[w];
+    __syncthreads();
+    return v + pre;
+}
+
+// getPeakCorr (corrTrackFast.c:862-899) for one lag row per block: window
+// sums from prefix sums of the vertical sums, NCC in FP64, best and
+// runner-up within the searched lags (rank = row-major lag index, the
+// reference's first-maximum rule).
+__global__ void __launch_bounds__(256) k_w_ncc(WPar p, WBatch Bt)
+{
+    extern __shared__ double lds[];
+    double *P1 = lds, *P2 = lds + (p.W_R + 1);
+    __shared__ double sh[4];
+    __shared__ RowBest rb[4];
+    const int i = blockIdx.x, b = blockIdx.y;
+    const double *V1 = Bt.V1 + ((size_t)b * p.nLagA + i) * p.W_R;
+    const double *V2 = Bt.V2 + ((size_t)b * p.nLagA + i) * p.W_R;
+    // prefix sums P[c] = sum_{t < c} V[t]
+    const int per = (p.W_R + 255) / 256;
+    double a1 = 0.0, a2 = 0.0;
+    for (int k = 0; k < per; k++) {
+        const int c = threadIdx.x * per + k;
+        if (c < p.W_R) {
+            a1 += V1[c];
+            a2 += V2[c];
+        }
+    }
+    const double x1 = block_scan_d(a1, sh) - a1;
+    const double x2 = block_scan_d(a2, sh) - a2;
+    double r1 = x1, r2 = x2;
+    for (int k = 0; k < per; k++) {
+        const int c = threadIdx.x * per + k;
+        if (c < p.W_R) {
+            P1[c] = r1;
+            P2[c] = r2;
+            r1 += V1[c];
+            r2 += V2[c];
+        }
+    }
+    if (threadIdx.x == 255) {
+        P1[p.W_R] = r1;
+        P2[p.W_R] = r2;
+    }
+    __syncthreads();
+    const double N = (double)p.wR2 * p.wA2;
+    const double meanR = Bt.chip[2 * b], sigmaR = Bt.chip[2 * b + 1];
+    const int l = i - WOS * p.epA;
+    const bool rowIn = l >= -WOS * (p.epA - WNFAST / 4) && l <= WOS * (p.epA - WNFAST / 4);
+    const int NN = p.W_A * p.W_R;
+    const float2 *X = Bt.U + (size_t)(b >> 1) * NN + (size_t)((l + p.W_A) % p.W_A) * p.W_R;
+    float *crRow = Bt.cr + ((size_t)b * p.nLagA + i) * p.nLagR;
+    double best = -1e30, second = -1e30;
+    int rank = 0x7fffffff;
+    for (int s = threadIdx.x; s < p.nLagR; s += blockDim.x) {
+        const double m = (P1[s + p.wR2] - P1[s]) / N;
+        const double sig = (P2[s + p.wR2] - P2[s]) / (N - 1.) - m * m;
+        const int c = s - WOS * p.epR;
+        const float2 xv = X[(c + p.W_R) % p.W_R];
+        const float v = (b & 1) ? xv.y : xv.x;
+        const float mag = (float)sqrt((double)(v * v));
+        double cr = (double)mag * p.scaleW;
+        cr = cr - meanR * m;
+        cr = cr / sqrt(sigmaR * sig);
+        crRow[s] = (float)cr;
+        if (rowIn && c >= -WOS * (p.epR - WNFAST / 4) && c <= WOS * (p.epR - WNFAST / 4)) {
+            if (cr > best) {
+                second = best;
+                best = cr;
+                rank = i * p.nLagR + s;
+            } else if (cr > second) {
+                second = cr;
+            }
+        }
+    }
+    // top-2 with first-rank tie-break across the block
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o), os = __shfl_xor(second, o);
+        const int orank = __shfl_xor(rank, o);
+        const bool takeOther = ob > best || (ob == best && orank < rank);
+        const double lo = takeOther ? best : ob;
+        const double hiSecond = takeOther ? os : second;
+        second = fmax(hiSecond, lo);
+        if (takeOther) {
+            best = ob;
+            rank = orank;
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        rb[wid] = RowBest{best, second, rank, 0};
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        RowBest r = rb[0];
+        for (int w = 1; w < 4; w++) {
+            const RowBest o = rb[w];
+            const bool takeOther = o.v > r.v || (o.v == r.v && o.rank < r.rank);
+            const double lo = takeOther ? r.v : o.v;
+            const double hs = takeOther ? o.second : r.second;
+            r.second = fmax(hs, lo);
+            if (takeOther) {
+                r.v = o.v;
+                r.rank = o.rank;
+            }
+        }
+        Bt.rows[(size_t)b * p.nLagA + i] = r;
+    }
+}
+
+// Peak over all lag rows, then correlateFast's refinement
+// (corrTrackFast.c:482-551, 951-990): the 21 x 21 NCC neighbourhood is
+// interpolated by its 16x spectral zero pad, evaluated in FP64 on the
+// searched 64 x 64 window only, Q = A C A^T with A[u][m] = D(136 + u - 16 m).
+__global__ void __launch_bounds__(256) k_w_refine(WPar p, WBatch Bt)
+{
+    __shared__ double dir[kDirN];
+    __shared__ float c[kN21 * kN21];
+    __shared__ double T[kN21 * kFineN];
+    __shared__ ArgMax am[4];
+    __shared__ double s2[4];
+    __shared__ RowBest best;
+    const int b = blockIdx.x;
+    for (int k = threadIdx.x; k < kDirN; k += blockDim.x)
+        dir[k] = c_dirw[k];
+    if (threadIdx.x == 0) {
+        RowBest r{-1e30, -1e30, 0x7fffffff, 0};
+        const RowBest *rows = Bt.rows + (size_t)b * p.nLagA;
+        for (int i = 0; i < p.nLagA; i++) {
+            const RowBest o = rows[i];
+            const bool takeOther = o.v > r.v || (o.v == r.v && o.rank < r.rank);
+            const double lo = takeOther ? r.v : o.v;
+            const double hs = takeOther ? o.second : r.second;
+            r.second = fmax(hs, lo);
+            if (takeOther) {
+                r.v = o.v;
+                r.rank = o.rank;
+            }
+        }
+        best = r;
+    }
+    __syncthreads();
+    const RowBest r = best;
+    const int iMax = r.rank == 0x7fffffff ? 0 : r.rank / p.nLagR;
+    const int jMax = r.rank == 0x7fffffff ? 0 : r.rank - (r.rank / p.nLagR) * p.nLagR;
+    WRes out{};
+    out.pI = iMax;
+    out.pJ = jMax;
+    out.fI = out.fJ = -2000;
+    out.az = Bt.az[b];
+    out.gapCoarse = r.v > 0 ? (float)((r.v - r.second) / r.v) : 0.f;
+    if (!(r.v >= 0.0)) {  // maxCorr < 0 (or nothing defined): no match
+        if (threadIdx.x == 0) {
+            out.cMax = 0.f;
+            Bt.res[b] = out;
+        }
+        return;
+    }
+    const float *cr = Bt.cr + (size_t)b * p.nLagA * p.nLagR;
+    for (int k = threadIdx.x; k < kN21 * kN21; k += blockDim.x) {
+        const int m = k / kN21, n = k - (k / kN21) * kN21;
+        c[k] = cr[(size_t)(iMax - WNFAST / 2 + m) * p.nLagR + jMax - WNFAST / 2 + n];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kN21 * kFineN; k += blockDim.x) {
+        const int m = k / kFineN, v = k - (k / kFineN) * kFineN;
+        double acc = 0.0;
+        for (int n = 0; n < kN21; n++)
+            acc += (double)c[m * kN21 + n] * dir[kFineLo + v - WNOVER * n - kDirLo];
+        T[k] = acc;
+    }
+    __syncthreads();
+    ArgMax a{-1.f, 0x7fffffff};
+    double sec = -1.0;
+    for (int k = threadIdx.x; k < kFineN * kFineN; k += blockDim.x) {
+        const int u = k / kFineN, v = k - (k / kFineN) * kFineN;
+        double acc = 0.0;
+        for (int m = 0; m < kN21; m++)
+            acc += dir[kFineLo + u - WNOVER * m - kDirLo] * T[m * kFineN + v];
+        const float q = (float)acc;
+        const float pw = q * q;
+        if (pw > a.v || (pw == a.v && k < a.rank)) {
+            if (a.v >= 0.f)
+                sec = fmax(sec, (double)a.v);
+            a.v = pw;
+            a.rank = k;
+        } else {
+            sec = fmax(sec, (double)pw);
+        }
+    }
+    // block argmax (first maximum) and runner-up
+    {
+        for (int o = 32; o > 0; o >>= 1) {
+            ArgMax o2;
+            o2.v = __shfl_xor(a.v, o);
+            o2.rank = __shfl_xor(a.rank, o);
+            const double os = __shfl_xor(sec, o);
+            const bool takeOther = o2.v > a.v || (o2.v == a.v && o2.rank < a.rank);
+            const double lo = takeOther ? (double)a.v : (double)o2.v;
+            const double hs = takeOther ? os : sec;
+            sec = fmax(hs, lo);
+            if (takeOther)
+                a = o2;
+        }
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        if (lane == 0) {
+            am[wid] = a;
+            s2[wid] = sec;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            ArgMax A = am[0];
+            double S = s2[0];
+            for (int w = 1; w < 4; w++) {
+                const bool takeOther = am[w].v > A.v || (am[w].v == A.v && am[w].rank < A.rank);
+                const double lo = takeOther ? (double)A.v : (double)am[w].v;
+                const double hs = takeOther ? s2[w] : S;
+                S = fmax(hs, lo);
+                if (takeOther)
+                    A = am[w];
+            }
+            const int u = A.rank / kFineN, v = A.rank - (A.rank / kFineN) * kFineN;
+            out.gapFine = A.v > 0.f ? (float)(((double)A.v - S) / (double)A.v) : 0.f;
+            out.cMax = (float)(sqrt((double)A.v) / (double)(kN21 * kN21));
+            const int iF = iMax * WNOVER + kFineLo + u - (WNOVER * WNFAST) / 2;
+            const int jF = jMax * WNOVER + kFineLo + v - (WNOVER * WNFAST) / 2;
+            out.fI = iF;
+            out.fJ = jF;
+            out.rShift1 = ((float)jF - (float)(p.epR * WOS * WNOVER)) / (float)(WNOVER * WOS);
+            out.aShift1 = ((float)iF - (float)(p.epA * WOS * WNOVER)) / (float)(WNOVER * WOS);
+            Bt.res[b] = out;
+        }
+    }
+}
+#pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------
+// host side
+
+// Plain bilinear interpolation of the initial-offset grid (the reference
+// calls mosaicSource's bilinearInterp, corrTrackFast.c:851-852).
+float bilinear(const float *g, double x, double y, int nx, int ny, float minvalue, float nodata)
+{
+    if (!(x >= 0.0 && y >= 0.0 && x <= nx - 1 && y <= ny - 1))
+        return nodata;
+    int ix = (int)x, iy = (int)y;
+    if (ix > nx - 2)
+        ix = nx - 2;
+    if (iy > ny - 2)
+        iy = ny - 2;
+    if (ix < 0 || iy < 0)
+        return nodata;
+    const double t = x - ix, u = y - iy;
+    const float p1 = g[(size_t)iy * nx + ix], p2 = g[(size_t)iy * nx + ix + 1];
+    const float p3 = g[(size_t)(iy + 1) * nx + ix + 1], p4 = g[(size_t)(iy + 1) * nx + ix];
+    if (p1 <= minvalue || p2 <= minvalue || p3 <= minvalue || p4 <= minvalue)
+        return nodata;
+    return (float)((1.0 - t) * (1.0 - u) * p1 + t * (1.0 - u) * p2 + t * u * p3 + (1.0 - t) * u * p4);
+}
+
+// findImage2Pos + getShifts of strackw (corrTrackFast.c:257-292, 843-858).
+void image2_pos(const st_track_params &par, int r1, int a1, int *r2, int *a2)
+{
+    double aShift, rShift;
+    const double sc = (double)par.scaleFactor;
+    auto poly = [&]() {
+        rShift = par.rShiftPoly[0] + (double)(r1 * sc) * par.rShiftPoly[1] + (double)(a1 * sc) * par.rShiftPoly[2];
+        aShift = par.aShiftPoly[0] + (double)(r1 * sc) * par.aShiftPoly[1] + (double)(a1 * sc) * par.aShiftPoly[2];
+    };
+    if (par.polyShift || par.initDr == nullptr) {
+        poly();
+    } else {
+        const float minvalue = (float)-ST_LARGEINT * 0.9f;
+        const double range = (r1 - par.initR0) / par.initDeltaR;
+        const double azimuth = (a1 - par.initA0) / par.initDeltaA;
+        rShift = bilinear(par.initDr, range, azimuth, par.initNr, par.initNa, minvalue, (float)-ST_LARGEINT);
+        aShift = bilinear(par.initDa, range, azimuth, par.initNr, par.initNa, minvalue, (float)-ST_LARGEINT);
+        if (rShift <= minvalue || aShift <= minvalue) {
+            rShift = minvalue;
+            aShift = minvalue;
+        }
+        if (rShift < (-ST_LARGEINT + 1))
+            poly();
+    }
+    rShift /= sc;
+    aShift /= sc;
+    *r2 = (rShift >= 0) ? r1 + (int)(rShift + 0.5) : r1 + (int)(rShift - 0.5);
+    *a2 = (aShift >= 0) ? a1 + (int)(aShift + 0.5) : a1 + (int)(aShift - 0.5);
+}
+
+// maskValue of strackw (corrTrackFast.c:1141-1160)
+int mask_value(const st_track_params &p, int r1, int a1)
+{
+    if (p.mask == nullptr)
+        return 1;
+    const int dr = (p.wRa / 2 - p.maskR0) * p.scaleFactor;
+    const int da = (p.wAa / 2 - p.maskA0) * p.scaleFactor;
+    const int nrl = p.maskNrl > 0 ? p.maskNrl : 1, nal = p.maskNal > 0 ? p.maskNal : 1;
+    const int ia1 = (a1 + da) / nal, ir1 = (r1 + dr) / nrl;
+    if (ia1 < 0 || ia1 >= p.maskNa || ir1 < 0 || ir1 >= p.maskNr)
+        return 0;
+    return (int)p.mask[(size_t)ia1 * p.maskNr + ir1];
+}
+
+void init_dirichlet_w(st_ctx *ctx)
+{
+    static bool done[64] = {};
+    if (ctx->device >= 0 && ctx->device < 64 && done[ctx->device])
+        return;
+    const int N = kN21 * WNOVER;
+    double h[kDirN];
+    for (int k = 0; k < kDirN; k++) {
+        const int t = kDirLo + k;
+        long double s = 1.0L;
+        for (int f = 1; f <= WNFAST / 2; f++)
+            s += 2.0L * cosl(2.0L * 3.141592653589793238462643383279502884L * (long double)f * (long double)t /
+                             (long double)N);
+        h[k] = (double)s;
+    }
+    ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dirw), h, sizeof(h)));
+    if (ctx->device >= 0 && ctx->device < 64)
+        done[ctx->device] = true;
+}
+
+}  // namespace
 
 void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, st_track_out *out,
                              st_track_diag *diag)
 {
-    (void)ctx;
-    (void)pr;
-    (void)par;
-    (void)out;
-    (void)diag;
-    throw StError{ST_ERR_UNSUPPORTED, "strackw is not implemented yet"};
+    ST_REQUIRE(par.nR > 0 && par.nA > 0 && par.scaleFactor > 0, ST_ERR_ARG, "grid");
+    ST_REQUIRE(par.wRa > 0 && par.wAa > 0 && !(par.wRa & 1) && !(par.wAa & 1), ST_ERR_ARG, "window sizes");
+    WPar p{};
+    p.wRa = par.wRa;
+    p.wAa = par.wAa;
+    p.W_R = 2 * par.wRa;
+    p.W_A = 2 * par.wAa;
+    p.epR = par.edgePadR + WNFAST / 4;  // corrTrackFast.c:116-117
+    p.epA = par.edgePadA + WNFAST / 4;
+    p.wR2 = (p.wRa - 2 * p.epR) * WOS;
+    p.wA2 = (p.wAa - 2 * p.epA) * WOS;
+    p.nLagR = 2 * p.epR * WOS + 1;
+    p.nLagA = 2 * p.epA * WOS + 1;
+    p.navgR = par.navgR;
+    p.navgA = par.navgA;
+    ST_REQUIRE(par.edgePadR >= 0 && par.edgePadA >= 0 && p.wR2 > 0 && p.wA2 > 0, ST_ERR_ARG,
+               "edge pad leaves no chip");
+    ST_REQUIRE(stfft::supported_size(p.wRa) && stfft::supported_size(p.wAa) && stfft::supported_size(p.W_R) &&
+                   stfft::supported_size(p.W_A),
+               ST_ERR_UNSUPPORTED, "window size not supported by the FFT engine");
+    ST_REQUIRE(p.navgR >= 0 && p.navgA >= 0 && p.navgR <= kMaxNavg && p.navgA <= kMaxNavg, ST_ERR_UNSUPPORTED,
+               "boxcar half-widths above 8");
+    ST_REQUIRE(par.polyShift || (par.initDr && par.initDa && par.initNr > 1 && par.initNa > 1), ST_ERR_ARG,
+               "initial offsets: polyShift=0 needs initDr/initDa");
+    // corrTrackFast.c:333-334
+    p.scale = (float)(1. / ((float)p.wRa * (float)p.wAa * (float)p.wRa * (float)p.wAa *
+                            (float)(WOS * WOS * WOS * WOS) * (WOS * WOS * (p.navgA + 1) * (p.navgR + 1))));
+    p.scaleW = 1.0 / ((WOS * WOS * WOS * WOS) * (double)p.wAa * (double)(p.wAa - 2 * p.epA) * (double)p.wRa *
+                      (double)(p.wRa - 2 * p.epR));
+    p.im = Images{pr.d1, pr.d2, pr.nr1, pr.na1, pr.nr2, pr.na2, pr.stype};
+
+    hipStream_t st = ctx->stream;
+    init_dirichlet_w(ctx);
+    ctx->stats_reset();
+    hipEvent_t *sev = ctx->ev + 64;
+    ST_HIP(hipEventRecord(sev[0], st));
+    const int kFftSlot = ctx->kernel_slot("fft2d"), kDet = ctx->kernel_slot("w_detect"),
+              kNcc = ctx->kernel_slot("w_ncc"), kRef = ctx->kernel_slot("w_refine"),
+              kStat = ctx->kernel_slot("w_stats");
+
+    // geometry on the host: grid walk, initial offsets, mask, corner-only
+    // bounds (corrTrackFast.c:160-231)
+    const int npts = par.nR * par.nA;
+    std::vector<int4> pos;
+    std::vector<int> listPt;
+    std::vector<int> r2v(npts), a2v(npts);
+    std::vector<uint8_t> typ(npts, 0);
+    for (int i = 0; i < par.nA; i++) {
+        const int a1 = par.aStart + i * par.deltaA - par.wAa / 2;
+        for (int j = 0; j < par.nR; j++) {
+            const int o = i * par.nR + j;
+            const int r1 = par.rStart + j * par.deltaR - par.wRa / 2;
+            int r2, a2;
+            image2_pos(par, r1, a1, &r2, &a2);
+            r2v[o] = r2;
+            a2v[o] = a2;
+            const int mv = mask_value(par, r1 * par.scaleFactor, a1 * par.scaleFactor);
+            if (mv == 1 && r1 > 0 && r1 < pr.nr1 && a1 > 0 && a1 < pr.na1 && r2 > 0 && r2 < pr.nr2 && a2 > 0 &&
+                a2 < pr.na2) {
+                pos.push_back(make_int4(r1, a1, r2, a2));
+                listPt.push_back(o);
+            }
+        }
+    }
+    const int nList = (int)pos.size();
+    std::vector<WRes> res(nList);
+
+    if (nList > 0) {
+        const size_t NN = (size_t)p.W_A * p.W_R, n1 = (size_t)p.wAa * p.wRa;
+        const size_t per = 2 * n1 * 8 + 2 * NN * 8 + 2 * NN * 8 + 2 * (size_t)p.nLagA * p.W_R * 8 +
+                           (size_t)p.nLagA * p.nLagR * 4 + (size_t)p.nLagA * sizeof(RowBest) + 4096;
+        const size_t budget = std::max(ctx->scratchBudget, per * 4);
+        const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nList, budget / per, (size_t)4096}));
+        const int npair = (B + 1) / 2;
+        ctx->arena_reserve((size_t)B * per + (size_t)nList * (16 + sizeof(WRes)) + 65536);
+        Arena &ar = ctx->arena;
+        int4 *dPos = ar.take<int4>(nList);
+        WRes *dRes = ar.take<WRes>(nList);
+        ST_HIP(hipMemcpyAsync(dPos, pos.data(), nList * sizeof(int4), hipMemcpyHostToDevice, st));
+        WBatch Bt{};
+        Bt.pos = dPos;
+        Bt.P = ar.take<float2>(2 * (size_t)B * n1);
+        Bt.U = ar.take<float2>(2 * (size_t)B * NN);
+        float2 *TMP = ar.take<float2>(2 * (size_t)B * NN);
+        Bt.Z = TMP;
+        Bt.V1 = ar.take<double>((size_t)B * p.nLagA * p.W_R);
+        Bt.V2 = ar.take<double>((size_t)B * p.nLagA * p.W_R);
+        Bt.chip = ar.take<double>(2 * (size_t)B);
+        Bt.cr = ar.take<float>((size_t)B * p.nLagA * p.nLagR);
+        Bt.rows = ar.take<RowBest>((size_t)B * p.nLagA);
+        Bt.az = ar.take<int>(B);
+        const size_t ldsNcc = 2 * (size_t)(p.W_R + 1) * sizeof(double);
+        ST_REQUIRE(ldsNcc <= 64 * 1024, ST_ERR_UNSUPPORTED, "window too large for the NCC kernel");
+        for (int off = 0; off < nList; off += B) {
+            const int nb = std::min(B, nList - off);
+            const int np = (nb + 1) / 2;
+            Bt.res = dRes + off;
+            hipLaunchKernelGGL(k_w_gather, dim3(nb), dim3(NT), 0, st, p, off, Bt, nb);
+            ctx->time_begin(kFftSlot);
+            ctx->fft.fft2d(Bt.P, Bt.U, 2 * nb, p.wAa, p.wRa, -1, st);
+            ctx->time_end(kFftSlot);
+            hipLaunchKernelGGL(k_w_pad, dim3((unsigned)((NN + 255) / 256), 2 * nb), dim3(256), 0, st, p, Bt, nb);
+            ctx->time_begin(kFftSlot);
+            ctx->fft.fft2d(Bt.U, TMP, 2 * nb, p.W_A, p.W_R, +1, st);
+            ctx->time_end(kFftSlot);
+            ctx->time_begin(kDet);
+            hipLaunchKernelGGL(k_w_detect, dim3((p.W_R + kTile - 1) / kTile, (p.W_A + kTile - 1) / kTile, nb),
+                               dim3(256), 0, st, p, Bt, nb);
+            ctx->time_end(kDet);
+            ctx->time_begin(kStat);
+            hipLaunchKernelGGL(k_w_vsum, dim3((p.W_R + 63) / 64, nb), dim3(64), 0, st, p, Bt);
+            hipLaunchKernelGGL(k_w_chipstats, dim3(nb), dim3(NT), 0, st, p, Bt);
+            ctx->time_end(kStat);
+            ctx->time_begin(kFftSlot);
+            ctx->fft.fft2d(Bt.Z, Bt.U, nb, p.W_A, p.W_R, -1, st);
+            ctx->time_end(kFftSlot);
+            hipLaunchKernelGGL(k_w_cross, dim3((unsigned)((NN + 255) / 256), np), dim3(256), 0, st, p, Bt, nb);
+            ctx->time_begin(kFftSlot);
+            ctx->fft.fft2d(Bt.U, TMP, np, p.W_A, p.W_R, +1, st);
+            ctx->time_end(kFftSlot);
+            ctx->time_begin(kNcc);
+            hipLaunchKernelGGL(k_w_ncc, dim3(p.nLagA, nb), dim3(256), ldsNcc, st, p, Bt);
+            ctx->time_end(kNcc);
+            ctx->time_begin(kRef);
+            hipLaunchKernelGGL(k_w_refine, dim3(nb), dim3(256), 0, st, p, Bt);
+            ctx->time_end(kRef);
+            ST_HIP(hipGetLastError());
+        }
+        ST_HIP(hipMemcpyAsync(res.data(), dRes, nList * sizeof(WRes), hipMemcpyDeviceToHost, st));
+    }
+    ST_HIP(hipEventRecord(sev[1], st));
+    ST_HIP(hipStreamSynchronize(st));
+    ctx->flush_timings();
+
+    // corrMatch (corrTrackFast.c:376-404) and the output grids
+    for (int k = 0; k < npts; k++) {
+        out->offR[k] = (float)-ST_LARGEINT;
+        out->offA[k] = (float)-ST_LARGEINT;
+        out->corr[k] = 0.f;
+        out->type[k] = 0;
+        if (out->azd)
+            out->azd[k] = 0.f;
+    }
+    if (diag) {
+        for (int k = 0; k < npts; k++) {
+            if (diag->r2)
+                diag->r2[k] = r2v[k];
+            if (diag->a2)
+                diag->a2[k] = a2v[k];
+            if (diag->azShift)
+                diag->azShift[k] = -1;
+            for (int t = 0; t < 3; t++) {
+                if (diag->peakI[t])
+                    diag->peakI[t][k] = -2000;
+                if (diag->peakJ[t])
+                    diag->peakJ[t][k] = -2000;
+                if (diag->fineI[t])
+                    diag->fineI[t][k] = -2000;
+                if (diag->fineJ[t])
+                    diag->fineJ[t][k] = -2000;
+                if (diag->cmax[t])
+                    diag->cmax[t][k] = 0;
+                if (diag->sigRg[t])
+                    diag->sigRg[t][k] = 0;
+                if (diag->sigAz[t])
+                    diag->sigAz[t][k] = 0;
+                if (diag->peakGap[t])
+                    diag->peakGap[t][k] = 0;
+                if (diag->fineGap[t])
+                    diag->fineGap[t][k] = 0;
+            }
+        }
+    }
+    int nAmp = 0, nFail = 0;
+    for (int q = 0; q < nList; q++) {
+        const int o = listPt[q];
+        const int4 ps = pos[q];
+        const WRes &r = res[q];
+        float rShift = (float)(ps.z - ps.x), aShift = (float)(ps.w - ps.y);
+        if (r.cMax > 0.0f) {
+            rShift -= r.rShift1;
+            aShift -= r.aShift1;
+            nAmp++;
+        } else {
+            rShift = (float)-ST_LARGEINT;
+            aShift = (float)-ST_LARGEINT;
+            nFail++;
+        }
+        out->offR[o] = rShift;
+        out->offA[o] = aShift;
+        out->corr[o] = r.cMax;
+        out->type[o] = kAmpMatch;
+        if (diag) {
+            if (diag->azShift)
+                diag->azShift[o] = r.az;
+            if (diag->peakI[0])
+                diag->peakI[0][o] = r.pI;
+            if (diag->peakJ[0])
+                diag->peakJ[0][o] = r.pJ;
+            if (diag->fineI[0])
+                diag->fineI[0][o] = r.fI;
+            if (diag->fineJ[0])
+                diag->fineJ[0][o] = r.fJ;
+            if (diag->cmax[0])
+                diag->cmax[0][o] = r.cMax;
+            if (diag->peakGap[0])
+                diag->peakGap[0][o] = r.gapCoarse;
+            if (diag->fineGap[0])
+                diag->fineGap[0][o] = r.gapFine;
+        }
+    }
+    out->nComplex = 0;
+    out->nAmp = nAmp;
+    out->nAmpL = 0;
+    out->nFail = nFail;
+    {
+        float ms;
+        ST_HIP(hipEventElapsedTime(&ms, sev[0], sev[1]));
+        ctx->stats.totalMs = ms;
+        ctx->stats.stageMs[2] = ms;
+        ctx->stats.nPoints = npts;
+        ctx->stats.nCorr[1] = nList;
+    }
 }
