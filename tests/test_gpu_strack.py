@@ -137,3 +137,28 @@ def test_parity_zero_filled_block(device, oracle32, oracle64):
     g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
     assert np.array_equal(g.type == 0, o32.type == 0)
     _check(rep, probs)
+
+
+def test_parity_baseline_phase(device, oracle32, oracle64):
+    """phaseCorrectBaseline (speckleTrack.c:868-907) with a non-zero
+    baseline and per-range look-angle tables: patch 1 is multiplied by
+    exp(i phi) before the complex match."""
+    s = configs._spec(400, 400, 25, dr0=0.9, da0=-0.6, gmin=0.7, gmax=0.8)
+    a, b = track.synth_host(s)
+    p = small_params(32, 400, 400, step=16, rShiftPoly=[1, 0, 0], aShiftPoly=[-1, 0, 0])
+    j = np.arange(400, dtype=np.float64)
+    theta = 0.6 + 4e-5 * j
+    sinT = np.sin(theta)
+    cosT = np.cos(theta)
+    overR = 1.0 / (8.0e5 + 2.3 * j)
+    p.baseFlag = 1
+    p.Bn, p.Bp, p.dBn, p.dBp, p.dBnQ, p.dBpQ = 50.0, 20.0, 0.02, -0.01, 0.001, 0.0005
+    p.lambda_ = 0.0555
+    p.nSlpA1 = 400
+    p.sinThetaD = abi.ptr(sinT, abi.C.c_double)
+    p.cosThetaD = abi.ptr(cosT, abi.C.c_double)
+    p.overRange = abi.ptr(overR, abi.C.c_double)
+    g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
+    # the phase ramp costs about half of the complex matches: both paths run
+    assert g.nComplex > 50 and g.nAmp > 50
+    _check(rep, probs)
