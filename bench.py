@@ -82,17 +82,172 @@ def calibrate_copy(nbytes=1 << 30, reps=5):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "GB_per_s": 2 * nbytes / dt / 1e9}))
 
 
+def flops_strackw_patch(wa):
+    """Algorithmic FP32 flops of one strackw patch, counted from the
+    reference's transform list: forward wa^2 x2, inverse (2wa)^2 x2,
+    forward (2wa)^2 x2, inverse (2wa)^2, the 336^2 inverse of the
+    oversampling, the cross multiply and the |z| / detection passes."""
+    import math
+    n1, n2, n3 = wa * wa, 4 * wa * wa, 336 * 336
+    return (2 * 5 * n1 * math.log2(n1) + 5 * 5 * n2 * math.log2(n2) + 5 * n3 * math.log2(n3)
+            + 6 * n2 + 3 * 4 * n2)
+
+
+def bench_strackw(args):
+    """Sheared-flow strackw workload: 8192 x 8192 pair, coherence 0.15-0.4,
+    shear to +/-120 px, 5x5 boxcar, chip 128 (wRa 384) or 256 (wRa 512),
+    edge pad 123, initial-offset grid at 1/8 resolution with clipped
+    2.5 px Gaussian error."""
+    import ctypes as C
+    from specklesource_amd import abi, configs, track
+    spec, par = configs.cfg4(chip=args.chip, scale=args.scale)
+    gr, ga, (r0, a0, dR, dA) = configs.initial_grid(spec, par)
+    par.polyShift = 0
+    par.initDr = abi.ptr(gr, C.c_float)
+    par.initDa = abi.ptr(ga, C.c_float)
+    par.initNa, par.initNr = gr.shape
+    par.initR0, par.initA0, par.initDeltaR, par.initDeltaA = r0, a0, dR, dA
+    dev = track.Device(0)
+    pid = dev.synth_pair(spec)
+    for k in range(args.warmup):
+        dev.corr_track_fast(pid, par)
+        print("warmup %d done" % k, file=sys.stderr, flush=True)
+    dev.set_profiling(True)
+    lat, kern, npatch, res = [], {}, 0, None
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        t = time.perf_counter()
+        res = dev.corr_track_fast(pid, par)
+        lat.append(time.perf_counter() - t)
+        st = dev.last_stats()
+        npatch += st.nCorr[1]
+        for i in range(8):
+            nm = st.kernelNames[i].value.decode()
+            if nm:
+                ms, n = kern.get(nm, (0.0, 0))
+                kern[nm] = (ms + st.kernelMs[i], n + st.kernelLaunches[i])
+        print("step %d: %.3f s" % (k, lat[-1]), file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    dev.set_profiling(False)
+    per = npatch // args.steps
+    fl = flops_strackw_patch(par.wRa)
+    busy_ms = sum(v[0] for v in kern.values()) / args.steps
+    dom = max(kern.items(), key=lambda kv: kv[1][0])
+    line = {
+        "metric": "strackw patch correlations/sec, 1 MI355X",
+        "value": npatch / elapsed, "unit": "patch correlations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (fp64 window sums, NCC and refinement)",
+        "data": "synthetic (generated in HBM)",
+        "config": {"workload": "strackw sheared flow, chip %d (wRa %d), edge pad 123, 5x5 boxcar"
+                               % (args.chip, par.wRa), "scene": [spec.nr, spec.na], "grid": [par.nR, par.nA]},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "work": {"patches_per_job": per, "matches": res.nAmp, "fail": res.nFail},
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
+        "roofline": {"bound": "mfma", "achieved": fl * per / (busy_ms * 1e-3) / 1e12 if busy_ms > 0 else 0.0,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "traffic": None,
+                     "kernel": "all strackw kernels (algorithmic flops of the reference transform list over "
+                               "the summed kernel time)", "dominant_kernel": dom[0]},
+    }
+    line["roofline"]["frac"] = line["roofline"]["achieved"] / PEAK_FP32_TFLOPS
+    dev.close()
+    print(json.dumps(line))
+
+
+def bench_batched(args):
+    """Batched uneven workload: 64 pairs with sides drawn from 1024-8192 px,
+    windows from {16, 32, 64, 128} on a w/2 grid, affine offsets within
+    +/-20 px, all resident in HBM and submitted at once to the serving
+    layer (4 slots); one step is the whole set of 64 jobs."""
+    from specklesource_amd import abi, configs, track
+    prec = abi.ST_PREC_MFMA if args.precision == "mfma" else abi.ST_PREC_FP32
+    pairs = configs.cfg3_pairs(n_pairs=args.pairs, scale=args.scale)
+    srv = track.Server(0, slots=args.slots, hbm_budget=200 << 30)
+    t0 = time.perf_counter()
+    pids = [srv.synth_pair(s) for s, _ in pairs]
+    print("generated %d pairs in HBM in %.1f s" % (len(pids), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    for _, p in pairs:
+        p.precision = prec
+
+    def run_all():
+        t = time.perf_counter()
+        ids = [srv.submit(p, pair=pid) for (_, p), pid in zip(pairs, pids)]
+        infos = [srv.wait(j)[1] for j in ids]
+        return time.perf_counter() - t, infos
+
+    for k in range(args.warmup):
+        run_all()
+        print("warmup %d done" % k, file=sys.stderr, flush=True)
+    walls, lats, patches = [], [], 0
+    for k in range(args.steps):
+        wall, infos = run_all()
+        walls.append(wall)
+        lats += [i.latencyMs for i in infos]
+        patches += sum(i.patches for i in infos)
+        print("step %d: %.3f s" % (k, wall), file=sys.stderr, flush=True)
+    srv.close()
+    ws = {}
+    for _, p in pairs:
+        ws[p.wR] = ws.get(p.wR, 0) + 1
+    line = {
+        "metric": "patch correlations/sec + p99 job latency, 1 MI355X (batched uneven pairs)",
+        "value": patches / sum(walls), "unit": "patch correlations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * sum(walls) / len(walls), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32" if prec == abi.ST_PREC_FP32 else "bf16 split hi+lo, fp32 accumulate",
+        "data": "synthetic (generated in HBM)",
+        "config": {"workload": "%d pairs 1024-8192 px, w in {16,32,64,128}, grid w/2, served on %d slots"
+                               % (len(pairs), args.slots), "windows": ws, "precision": args.precision},
+        "latency_ms": {"p50": float(np.percentile(lats, 50)), "p99": float(np.percentile(lats, 99))},
+        "work": {"patch_correlations_per_step": patches // args.steps},
+    }
+    print(json.dumps(line))
+
+
+def bench_cull(args):
+    """cullst on the full-scene offset grid (821 x 571 points) with 5%
+    uniform +/-30 px outliers: three median passes, plane-fit sigma,
+    smoothing, island removal (threshold 20)."""
+    from specklesource_amd import configs, track
+    offR, offA, corr, typ = configs.make_cull_grid(571, 821, seed=5, outlier_frac=0.05)
+    dev = track.Device(0)
+    lat = []
+    for k in range(args.warmup + args.steps):
+        cp, arrs = track.cull_params(offR, offA, corr, typ, islandThresh=20)
+        t = time.perf_counter()
+        dev.cull("st_cull_run", cp)
+        if k >= args.warmup:
+            lat.append(time.perf_counter() - t)
+    n = offR.size
+    line = {
+        "metric": "cullst grid points/sec, 1 MI355X", "value": n * len(lat) / sum(lat), "unit": "grid points/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * sum(lat) / len(lat),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 plane fits)",
+        "data": "synthetic", "config": {"workload": "cullst full-scene grid 821 x 571, 5% outliers, islands 20"},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "note": "host grids in and out (PCIe included)",
+    }
+    dev.close()
+    print(json.dumps(line))
+
+
 def cpu_baseline(spec, par, dev, pid, budget_s=15.0):
     """Time the FP32 CPU restatement (single thread) on a strip of grid
-    rows of the same resident scene."""
+    rows of the same resident scene, and use the first rows of that strip
+    as the accuracy sample: the GPU rerun on the same strip with
+    diagnostics is compared with the FP32 oracle (integer peaks, match
+    types, near-ties) and the FP64 oracle (sub-pixel offsets,
+    correlation)."""
     from oracle.pyoracle import Oracle
     from specklesource_amd import track
+    from tests.helpers import compare_strack
     o = Oracle("f32")
     rows = 1
     total_pts = 0
     total_t = 0.0
     total_corr = 0
     i0 = par.nA // 3
+    acc = None
     while total_t < budget_s and i0 + rows <= par.nA:
         sub = track.default_params()
         for f, _ in par._fields_:
@@ -111,11 +266,24 @@ def cpu_baseline(spec, par, dev, pid, budget_s=15.0):
         total_t += dt
         total_pts += sub.nR * sub.nA
         total_corr += int(sum((r.diag["peakI"][k] > -2000).sum() for k in range(3)))
+        if acc is None and rows >= 2:
+            spid = dev.upload_pair(img1, img2)
+            g = dev.speckle_track(spid, sub, diag=True)
+            dev.release_pair(spid)
+            r64 = Oracle("f64").speckle_track(sub, img1, img2, diag=True)
+            rep, probs = compare_strack(sub, g, r, r64)
+            acc = {"sample_points": sub.nR * sub.nA,
+                   "rms_offset_px_vs_fp64": [rep.get("rms_r_vs_f64"), rep.get("rms_a_vs_f64")],
+                   "integer_peak_agreement_vs_fp32": rep["peak_agree"], "near_ties": rep["near_ties"],
+                   "type_agreement": rep["type_agree"], "unexplained_mismatches": len(probs),
+                   "corr_relerr_median_vs_fp64": rep.get("corr_relerr_median"),
+                   "corr_relerr_max_vs_fp64": rep.get("corr_relerr_max")}
         i0 += rows
         rows = min(4, rows * 2)
-    return {"value": total_corr / total_t, "unit": "patch correlations/s", "cores": 1, "kind": "port",
+    base = {"value": total_corr / total_t, "unit": "patch correlations/s", "cores": 1, "kind": "port",
             "sample": "%d grid points (%d consecutive grid rows of the same scene), %.1f s single-thread"
                       % (total_pts, total_pts // par.nR, total_t)}
+    return base, acc
 
 
 def main():
@@ -125,6 +293,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=1, help="divide the scene sides (testing only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["fullscene", "strackw", "cull", "batched"], default="fullscene")
+    ap.add_argument("--precision", choices=["fp32", "mfma"], default="fp32")
+    ap.add_argument("--pairs", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--chip", type=int, default=128, help="strackw chip size (128 or 256)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
@@ -134,6 +307,15 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "strackw":
+        bench_strackw(args)
+        return
+    if args.workload == "cull":
+        bench_cull(args)
+        return
+    if args.workload == "batched":
+        bench_batched(args)
         return
 
     from specklesource_amd import configs, track
@@ -215,7 +397,7 @@ def main():
                      "dominant_kernel": dom[0]},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(spec, par, dev, pid)
+        line["cpu_baseline"], line["accuracy"] = cpu_baseline(spec, par, dev, pid)
     dev.close()
     print(json.dumps(line))
 

@@ -215,6 +215,67 @@ int st_cull_islands(st_ctx *ctx, st_cull_params *cp);
  * cullStats, cullSmooth, cullIslands when islandThresh > 0. */
 int st_cull_run(st_ctx *ctx, st_cull_params *cp);
 
+/* ---- serving --------------------------------------------------------- */
+/* A server runs tracking jobs of different scene and window sizes
+ * concurrently on one GPU (continuous batching).  Each of its slots owns a
+ * stream and scratch arena; a queued job is admitted when a slot is idle
+ * and its device memory (per-job image upload + slot scratch) fits in the
+ * HBM budget left by the running jobs and the resident pairs, so jobs that
+ * arrive while others run start as soon as a slot and memory free up and
+ * their kernels overlap on the device.  Results do not depend on what else
+ * runs.  No counterpart in the reference (single-threaded, one job per
+ * process: Strack/strack.c:119-135); it is the request loop the
+ * function-level entry points sit under. */
+typedef struct st_server st_server;
+
+#define ST_JOB_STRACK 0  /* st_speckle_track */
+#define ST_JOB_STRACKW 1 /* st_corr_track_fast */
+
+#define ST_JOB_QUEUED 0
+#define ST_JOB_RUNNING 1
+#define ST_JOB_DONE 2
+#define ST_JOB_FAILED 3
+
+typedef struct {
+    int32_t kind;            /* ST_JOB_* */
+    int32_t pair;            /* resident pair of the server, or -1: upload img1/img2 for this job */
+    const void *img1, *img2; /* host images (pair = -1); must stay valid until the job is done */
+    int32_t nr1, na1, nr2, na2, sampleType;
+    st_track_params par;     /* copied at submit; its pointer members must stay valid until done */
+    st_track_out out;        /* caller-owned output arrays, written when the job is done */
+    st_cull_params *cull;    /* optional: cullst (st_cull_run) on the job's output grids after
+                                tracking; its offR/offA/corr/type must be the out arrays */
+} st_job;
+
+typedef struct {
+    int32_t state;       /* ST_JOB_* */
+    int32_t status;      /* ST_OK or the error code of a failed job */
+    int32_t slot;        /* slot that ran the job (-1 while queued) */
+    int32_t pad;
+    double queueMs;      /* submit -> start (wall clock) */
+    double runMs;        /* start -> done */
+    double latencyMs;    /* submit -> done */
+    int64_t patches;     /* patch correlations the job computed */
+    int32_t nComplex, nAmp, nAmpL, nFail; /* match counts of the job (st_track_out) */
+    char error[128];     /* message of a failed job */
+} st_job_info;
+
+/* Create a server on `device` with `slots` concurrent job slots (1..8) and
+ * an HBM budget in bytes for resident pairs plus admitted jobs. */
+int st_server_create(int32_t device, int32_t slots, uint64_t hbmBudget, st_server **srv);
+/* Waits for every submitted job, then frees everything. */
+int st_server_destroy(st_server *srv);
+/* Resident pairs shared by all jobs of the server (counted in the budget). */
+int st_server_pair_upload(st_server *srv, const void *img1, int32_t nr1, int32_t na1, const void *img2,
+                          int32_t nr2, int32_t na2, int32_t sampleType, int32_t *pair);
+int st_server_pair_release(st_server *srv, int32_t pair);
+/* Queue a job; returns its id at once. */
+int st_submit(st_server *srv, const st_job *job, int64_t *id);
+/* State of a job without waiting (info may be NULL). */
+int st_poll(st_server *srv, int64_t id, st_job_info *info);
+/* Block until the job is done or failed. */
+int st_wait(st_server *srv, int64_t id, st_job_info *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,9 @@ int st_synth_host(const st_synth_spec *spec, float *img1, float *img2);
 int st_synth_device(st_ctx *ctx, const st_synth_spec *spec, int32_t *pair);
 /* The true offset field at (r, a). */
 void st_synth_offset(const st_synth_spec *spec, double r, double a, double *dr, double *da);
+/* Generate a pair in HBM as a resident pair of a server (counted in its
+ * budget). */
+int st_server_pair_synth(st_server *srv, const st_synth_spec *spec, int32_t *pair);
 
 #ifdef __cplusplus
 }

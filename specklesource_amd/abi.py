@@ -107,6 +107,33 @@ class CullParams(C.Structure):
     ]
 
 
+ST_JOB_STRACK, ST_JOB_STRACKW = 0, 1
+ST_JOB_QUEUED, ST_JOB_RUNNING, ST_JOB_DONE, ST_JOB_FAILED = 0, 1, 2, 3
+
+
+class Job(C.Structure):
+    """st_job: one tracking request of a server."""
+
+    _fields_ = [
+        ("kind", C.c_int32), ("pair", C.c_int32),
+        ("img1", C.c_void_p), ("img2", C.c_void_p),
+        ("nr1", C.c_int32), ("na1", C.c_int32), ("nr2", C.c_int32), ("na2", C.c_int32),
+        ("sampleType", C.c_int32),
+        ("par", TrackParams), ("out", TrackOut),
+        ("cull", C.POINTER(CullParams)),
+    ]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32), ("status", C.c_int32), ("slot", C.c_int32), ("pad", C.c_int32),
+        ("queueMs", C.c_double), ("runMs", C.c_double), ("latencyMs", C.c_double),
+        ("patches", C.c_int64),
+        ("nComplex", C.c_int32), ("nAmp", C.c_int32), ("nAmpL", C.c_int32), ("nFail", C.c_int32),
+        ("error", C.c_char * 128),
+    ]
+
+
 class SynthSpec(C.Structure):
     _fields_ = [
         ("nr", C.c_int32), ("na", C.c_int32), ("seed", C.c_uint64),

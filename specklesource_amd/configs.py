@@ -132,3 +132,28 @@ def initial_grid(spec, par, factor=8, err_sigma=2.5, err_clip=5.0, seed=44):
             gr[i, j] = dr + e[0]
             ga[i, j] = da + e[1]
     return gr, ga, (r0, a0, dR, dA)
+
+
+def make_cull_grid(nA, nR, seed=0, outlier_frac=0.05, hole_frac=0.08):
+    """Offset grids like strack output: smooth field + noise, a fraction of
+    uniform +/-30 px outliers, random holes (-2e9) and islands."""
+    rng = np.random.default_rng(seed)
+    ii, jj = np.mgrid[0:nA, 0:nR]
+    offR = (1.5 + 0.004 * jj + 0.5 * np.sin(ii / 17.0) + 0.05 * rng.standard_normal((nA, nR))).astype(np.float32)
+    offA = (-0.8 + 0.003 * ii + 0.4 * np.cos(jj / 23.0) + 0.05 * rng.standard_normal((nA, nR))).astype(np.float32)
+    out = rng.random((nA, nR)) < outlier_frac
+    offR[out] += rng.uniform(-30, 30, out.sum()).astype(np.float32)
+    offA[out] += rng.uniform(-30, 30, out.sum()).astype(np.float32)
+    holes = rng.random((nA, nR)) < hole_frac
+    # a block of missing data with a few small islands inside
+    holes[nA // 3: nA // 3 + nA // 5, nR // 4: nR // 4 + nR // 3] = True
+    for k in range(4):
+        a0 = nA // 3 + 2 + 4 * k
+        r0 = nR // 4 + 3 + 7 * k
+        holes[a0:a0 + 2 + k, r0:r0 + 3] = False
+    offR[holes] = -2e9
+    offA[holes] = -2e9
+    typ = rng.integers(1, 4, (nA, nR)).astype(np.uint8)
+    typ[holes] = 0
+    corr = rng.uniform(0.0, 1.0, (nA, nR)).astype(np.float32)
+    return offR, offA, corr, typ

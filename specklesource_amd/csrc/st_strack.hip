@@ -16,6 +16,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "st_ctx.h"
@@ -1329,8 +1330,14 @@ void host_positions(const st_track_params &p, std::vector<int> &r2v, std::vector
 
 void init_dirichlet(st_ctx *ctx)
 {
-    static int done_dev = -1;
-    if (done_dev == ctx->device)
+    // constant tables are per device and per process; contexts of several
+    // threads may arrive here together, and none may launch before the
+    // upload is complete
+    static std::mutex mu;
+    static bool done[64] = {};
+    std::lock_guard<std::mutex> lk(mu);
+    ST_REQUIRE(ctx->device >= 0 && ctx->device < 64, ST_ERR_ARG, "device index");
+    if (done[ctx->device])
         return;
     double2 tab[193];
     for (int t = -96; t <= 96; t++) {
@@ -1350,7 +1357,7 @@ void init_dirichlet(st_ctx *ctx)
         t128[j] = make_float2((float)cos(2.0 * M_PI * j / 128.0), (float)sin(2.0 * M_PI * j / 128.0));
     ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fftreg::c_tw64), t64, sizeof(t64)));
     ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fftreg::c_tw128), t128, sizeof(t128)));
-    done_dev = ctx->device;
+    done[ctx->device] = true;
 }
 
 template <class T>
@@ -1376,6 +1383,19 @@ void launch_cpx_fused_t(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, co
                        mag);
     ctx->time_end(slot);
     ST_HIP(hipGetLastError());
+}
+
+// Resident workgroups per CU of the fused kernel (sizes its |C| scratch).
+template <int W>
+int fused_per_cu(int stype)
+{
+    const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
+    int perCU = 0;
+    if (stype == ST_SAMPLE_C64)
+        ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, ST_SAMPLE_C64>, 4 * W, lds));
+    else
+        ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, ST_SAMPLE_CI16>, 4 * W, lds));
+    return std::max(1, perCU);
 }
 
 template <int W>
@@ -1456,6 +1476,18 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     int Bc = (int)std::max<size_t>(1, std::min<size_t>({(size_t)npts, budget / perCpx, kMaxBatch}));
     int Ba = (int)std::max<size_t>(1, std::min<size_t>({(size_t)npts, budget / perAmp, kMaxBatch}));
     size_t need = fixed + std::max((size_t)Bc * perCpx, (size_t)Ba * perAmp) + 65536;
+    if (fusedPath) {
+        // per-point results of the single fused launch + |C| scratch of
+        // every resident workgroup
+        int ncu = 0;
+        ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        const int perCU = w == 16 ? fused_per_cu<16>(pr.stype)
+                                  : (w == 32 ? fused_per_cu<32>(pr.stype) : fused_per_cu<64>(pr.stype));
+        const size_t perPt = sizeof(PeakRes) + 2 * sizeof(float) + 1 + sizeof(int) + 2 * sizeof(double);
+        const size_t fusedNeed = fixed + (size_t)npts * perPt + 8 * 256 +
+                                 (size_t)std::min(npts, perCU * ncu) * 4 * w * w * sizeof(float) + 65536;
+        need = std::max(need, fusedNeed);
+    }
     ctx->arena_reserve(need);
     Arena &ar = ctx->arena;
 

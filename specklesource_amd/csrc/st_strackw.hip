@@ -20,6 +20,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "st_ctx.h"
@@ -590,8 +591,11 @@ int mask_value(const st_track_params &p, int r1, int a1)
 
 void init_dirichlet_w(st_ctx *ctx)
 {
+    static std::mutex mu;
     static bool done[64] = {};
-    if (ctx->device >= 0 && ctx->device < 64 && done[ctx->device])
+    std::lock_guard<std::mutex> lk(mu);
+    ST_REQUIRE(ctx->device >= 0 && ctx->device < 64, ST_ERR_ARG, "device index");
+    if (done[ctx->device])
         return;
     const int N = kN21 * WNOVER;
     double h[kDirN];
@@ -604,8 +608,7 @@ void init_dirichlet_w(st_ctx *ctx)
         h[k] = (double)s;
     }
     ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dirw), h, sizeof(h)));
-    if (ctx->device >= 0 && ctx->device < 64)
-        done[ctx->device] = true;
+    done[ctx->device] = true;
 }
 
 }  // namespace

@@ -48,6 +48,15 @@ def load_library(path=LIB_PATH):
                  "st_cull_islands", "st_cull_run"):
         getattr(lib, name).argtypes = [vp, C.POINTER(abi.CullParams)]
     lib.st_last_stats.argtypes = [vp, C.POINTER(abi.TrackStats)]
+    lib.st_server_create.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.POINTER(vp)]
+    lib.st_server_destroy.argtypes = [vp]
+    lib.st_server_pair_upload.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int32, C.c_int32, C.c_int32,
+                                          C.POINTER(C.c_int32)]
+    lib.st_server_pair_release.argtypes = [vp, C.c_int32]
+    lib.st_server_pair_synth.argtypes = [vp, C.POINTER(abi.SynthSpec), C.POINTER(C.c_int32)]
+    lib.st_submit.argtypes = [vp, C.POINTER(abi.Job), C.POINTER(C.c_int64)]
+    lib.st_poll.argtypes = [vp, C.c_int64, C.POINTER(abi.JobInfo)]
+    lib.st_wait.argtypes = [vp, C.c_int64, C.POINTER(abi.JobInfo)]
     lib.st_set_profiling.argtypes = [vp, C.c_int32]
     lib.st_synth_host.argtypes = [C.POINTER(abi.SynthSpec), vp, vp]
     lib.st_synth_device.argtypes = [vp, C.POINTER(abi.SynthSpec), C.POINTER(C.c_int32)]
@@ -205,6 +214,134 @@ class Device:
         return cp
 
 
+class Server:
+    """Continuous-batching tracking server on one GPU (st_server): jobs of
+    any scene and window size are submitted at once and run concurrently
+    on `slots` streams, admitted by the HBM budget."""
+
+    def __init__(self, device=0, slots=4, hbm_budget=200 << 30):
+        self.lib = load_library()
+        self.srv = C.c_void_p()
+        rc = self.lib.st_server_create(device, slots, hbm_budget, C.byref(self.srv))
+        if rc != abi.ST_OK:
+            raise RuntimeError("st_server_create failed (%d)" % rc)
+        self._jobs = {}
+
+    def close(self):
+        if self.srv:
+            self.lib.st_server_destroy(self.srv)
+            self.srv = C.c_void_p()
+            self._jobs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_pair(self, img1, img2, sample_type=abi.ST_SAMPLE_C64):
+        img1 = np.ascontiguousarray(img1)
+        img2 = np.ascontiguousarray(img2)
+        pid = C.c_int32()
+        rc = self.lib.st_server_pair_upload(self.srv, img1.ctypes.data, img1.shape[1], img1.shape[0],
+                                            img2.ctypes.data, img2.shape[1], img2.shape[0], sample_type,
+                                            C.byref(pid))
+        if rc != abi.ST_OK:
+            raise RuntimeError("st_server_pair_upload failed (%d)" % rc)
+        return pid.value
+
+    def synth_pair(self, spec):
+        """Generate a synthetic pair directly in HBM as a resident pair."""
+        pid = C.c_int32()
+        rc = self.lib.st_server_pair_synth(self.srv, C.byref(spec), C.byref(pid))
+        if rc != abi.ST_OK:
+            raise RuntimeError("st_server_pair_synth failed (%d)" % rc)
+        return pid.value
+
+    def release_pair(self, pair):
+        rc = self.lib.st_server_pair_release(self.srv, pair)
+        if rc != abi.ST_OK:
+            raise RuntimeError("st_server_pair_release failed (%d)" % rc)
+
+    def submit(self, par, pair=None, images=None, kind=abi.ST_JOB_STRACK, sample_type=abi.ST_SAMPLE_C64,
+               cull=None):
+        """Queue a job on a resident pair or on host images (img1, img2);
+        cull = abi.CullParams over the job's output arrays (see
+        cull_for_result) runs cullst after tracking.  Returns the job id."""
+        res, out, _ = alloc_outputs(par)
+        job = abi.Job()
+        job.kind = kind
+        keep = [res, par]
+        if pair is not None:
+            job.pair = pair
+        else:
+            img1 = np.ascontiguousarray(images[0])
+            img2 = np.ascontiguousarray(images[1])
+            keep += [img1, img2]
+            job.pair = -1
+            job.img1, job.img2 = img1.ctypes.data, img2.ctypes.data
+            job.nr1, job.na1 = img1.shape[1], img1.shape[0]
+            job.nr2, job.na2 = img2.shape[1], img2.shape[0]
+            job.sampleType = sample_type
+        job.par = par
+        job.out = out
+        if cull is not None:
+            cp, arrs = cull(res)
+            keep += [cp, arrs]
+            job.cull = C.pointer(cp)
+        jid = C.c_int64()
+        rc = self.lib.st_submit(self.srv, C.byref(job), C.byref(jid))
+        if rc != abi.ST_OK:
+            raise RuntimeError("st_submit failed (%d)" % rc)
+        self._jobs[jid.value] = (res, out, job, keep)
+        return jid.value
+
+    def poll(self, jid):
+        info = abi.JobInfo()
+        if self.lib.st_poll(self.srv, jid, C.byref(info)) != abi.ST_OK:
+            raise RuntimeError("unknown job %d" % jid)
+        return info
+
+    def wait(self, jid):
+        """Block until the job ends; returns (TrackResult, JobInfo, extra)
+        where extra holds the cull arrays when the job culled."""
+        info = abi.JobInfo()
+        if self.lib.st_wait(self.srv, jid, C.byref(info)) != abi.ST_OK:
+            raise RuntimeError("unknown job %d" % jid)
+        res, out, job, keep = self._jobs.pop(jid)
+        if info.state != abi.ST_JOB_DONE:
+            raise RuntimeError("job %d failed (%d): %s" % (jid, info.status, info.error.decode()))
+        res.nComplex, res.nAmp, res.nAmpL, res.nFail = info.nComplex, info.nAmp, info.nAmpL, info.nFail
+        extra = keep[-1] if job.cull else None
+        return res, info, extra
+
+
+def cull_for_result(**kw):
+    """Cull callback for Server.submit: cullst runs in place on the job's
+    own output grids (reference defaults, overridden by kw)."""
+    def make(res):
+        nA, nR = res.offR.shape
+        arrs = dict(offRS=np.zeros((nA, nR), np.float32), offAS=np.zeros((nA, nR), np.float32),
+                    sigmaR=np.zeros((nA, nR), np.float32), sigmaA=np.zeros((nA, nR), np.float32))
+        cp = abi.CullParams()
+        cp.nR, cp.nA = nR, nA
+        cp.offR, cp.offA = abi.ptr(res.offR, C.c_float), abi.ptr(res.offA, C.c_float)
+        cp.corr, cp.type = abi.ptr(res.corr, C.c_float), abi.ptr(res.type, C.c_uint8)
+        for k, v in arrs.items():
+            setattr(cp, k, abi.ptr(v, C.c_float))
+        cp.bR = cp.bA = 9
+        cp.nGood = 17
+        cp.maxA, cp.maxR = 1.0, 0.75
+        cp.sR, cp.sA = 2, 8
+        cp.islandThresh = -1
+        cp.singleMT = -1
+        cp.corrThresh = 0.0
+        for k, v in kw.items():
+            setattr(cp, k, v)
+        return cp, arrs
+    return make
+
+
 def synth_host(spec):
     """Generate a synthetic pair on the host (complex64 arrays na x nr)."""
     lib = load_library()
@@ -221,3 +358,28 @@ def synth_offset(spec, r, a):
     dr, da = C.c_double(), C.c_double()
     lib.st_synth_offset(C.byref(spec), float(r), float(a), C.byref(dr), C.byref(da))
     return dr.value, da.value
+
+
+def cull_params(offR, offA, corr, typ, mask=None, **kw):
+    """abi.CullParams over copies of the input grids (reference defaults,
+    Cullst/cullst.c:140-150); returns (params, arrays) keeping buffers alive."""
+    nA, nR = offR.shape
+    arrs = dict(offR=offR.copy(), offA=offA.copy(), corr=corr.copy(), type=typ.copy(),
+                offRS=np.zeros_like(offR), offAS=np.zeros_like(offR), sigmaR=np.zeros_like(offR),
+                sigmaA=np.zeros_like(offR))
+    if mask is not None:
+        arrs["mask"] = mask.copy()
+    cp = abi.CullParams()
+    cp.nR, cp.nA = nR, nA
+    for k, v in arrs.items():
+        setattr(cp, k, abi.ptr(v, C.c_uint8 if v.dtype == np.uint8 else C.c_float))
+    cp.bR = cp.bA = 9
+    cp.nGood = 17
+    cp.maxA, cp.maxR = 1.0, 0.75
+    cp.sR, cp.sA = 2, 8
+    cp.islandThresh = -1
+    cp.singleMT = -1
+    cp.corrThresh = 0.0
+    for k, v in kw.items():
+        setattr(cp, k, v)
+    return cp, arrs

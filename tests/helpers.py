@@ -120,51 +120,6 @@ def compare_strack(par, g, o32, o64):
     return rep, probs
 
 
-def make_cull_grid(nA, nR, seed=0, outlier_frac=0.05, hole_frac=0.08):
-    """Offset grids like strack output: smooth field + noise, a fraction of
-    uniform +/-30 px outliers, random holes (-2e9) and islands."""
-    rng = np.random.default_rng(seed)
-    ii, jj = np.mgrid[0:nA, 0:nR]
-    offR = (1.5 + 0.004 * jj + 0.5 * np.sin(ii / 17.0) + 0.05 * rng.standard_normal((nA, nR))).astype(np.float32)
-    offA = (-0.8 + 0.003 * ii + 0.4 * np.cos(jj / 23.0) + 0.05 * rng.standard_normal((nA, nR))).astype(np.float32)
-    out = rng.random((nA, nR)) < outlier_frac
-    offR[out] += rng.uniform(-30, 30, out.sum()).astype(np.float32)
-    offA[out] += rng.uniform(-30, 30, out.sum()).astype(np.float32)
-    holes = rng.random((nA, nR)) < hole_frac
-    # a block of missing data with a few small islands inside
-    holes[nA // 3: nA // 3 + nA // 5, nR // 4: nR // 4 + nR // 3] = True
-    for k in range(4):
-        a0 = nA // 3 + 2 + 4 * k
-        r0 = nR // 4 + 3 + 7 * k
-        holes[a0:a0 + 2 + k, r0:r0 + 3] = False
-    offR[holes] = -2e9
-    offA[holes] = -2e9
-    typ = rng.integers(1, 4, (nA, nR)).astype(np.uint8)
-    typ[holes] = 0
-    corr = rng.uniform(0.0, 1.0, (nA, nR)).astype(np.float32)
-    return offR, offA, corr, typ
-
-
-def cull_params(offR, offA, corr, typ, mask=None, **kw):
-    """abi.CullParams over copies of the input grids (reference defaults,
-    Cullst/cullst.c:140-150); returns (params, arrays) keeping buffers alive."""
-    nA, nR = offR.shape
-    arrs = dict(offR=offR.copy(), offA=offA.copy(), corr=corr.copy(), type=typ.copy(),
-                offRS=np.zeros_like(offR), offAS=np.zeros_like(offR), sigmaR=np.zeros_like(offR),
-                sigmaA=np.zeros_like(offR))
-    if mask is not None:
-        arrs["mask"] = mask.copy()
-    cp = abi.CullParams()
-    cp.nR, cp.nA = nR, nA
-    for k, v in arrs.items():
-        setattr(cp, k, abi.ptr(v, C.c_uint8 if v.dtype == np.uint8 else C.c_float))
-    cp.bR = cp.bA = 9
-    cp.nGood = 17
-    cp.maxA, cp.maxR = 1.0, 0.75
-    cp.sR, cp.sA = 2, 8
-    cp.islandThresh = -1
-    cp.singleMT = -1
-    cp.corrThresh = 0.0
-    for k, v in kw.items():
-        setattr(cp, k, v)
-    return cp, arrs
+# the cull workload generator and the CullParams builder live in the package
+from specklesource_amd.configs import make_cull_grid  # noqa: E402,F401
+from specklesource_amd.track import cull_params  # noqa: E402,F401

@@ -37,8 +37,9 @@ def test_struct_layouts_match_header():
 #include "speckle_amd.h"
 #include "speckle_synth.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu\n", sizeof(st_track_params), sizeof(st_track_out),
-         sizeof(st_track_diag), sizeof(st_cull_params), sizeof(st_synth_spec));
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(st_track_params), sizeof(st_track_out),
+         sizeof(st_track_diag), sizeof(st_cull_params), sizeof(st_synth_spec), sizeof(st_job),
+         sizeof(st_job_info));
   return 0;
 }
 '''
@@ -50,7 +51,8 @@ int main(void) {
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     sizes = [int(x) for x in out]
     assert sizes == [C.sizeof(abi.TrackParams), C.sizeof(abi.TrackOut), C.sizeof(abi.TrackDiag),
-                     C.sizeof(abi.CullParams), C.sizeof(abi.SynthSpec)]
+                     C.sizeof(abi.CullParams), C.sizeof(abi.SynthSpec), C.sizeof(abi.Job),
+                     C.sizeof(abi.JobInfo)]
 
 
 def test_ctx_create_without_gpu_reports_nodev():
@@ -60,6 +62,16 @@ def test_ctx_create_without_gpu_reports_nodev():
     lib = track.load_library()
     ctx = C.c_void_p()
     assert lib.st_ctx_create(0, C.byref(ctx)) == abi.ST_ERR_NODEV
+
+
+def test_server_create_without_gpu_reports_nodev():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    lib = track.load_library()
+    srv = C.c_void_p()
+    assert lib.st_server_create(0, 2, 1 << 30, C.byref(srv)) == abi.ST_ERR_NODEV
+    assert lib.st_server_create(0, 0, 1 << 30, C.byref(srv)) == abi.ST_ERR_ARG
 
 
 def test_host_synth_is_deterministic():
