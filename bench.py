@@ -162,6 +162,9 @@ def bench_batched(args):
     from specklesource_amd import abi, configs, track
     prec = abi.ST_PREC_MFMA if args.precision == "mfma" else abi.ST_PREC_FP32
     pairs = configs.cfg3_pairs(n_pairs=args.pairs, scale=args.scale)
+    if args.windows:
+        keep = {int(x) for x in args.windows.split(",")}
+        pairs = [(s, p) for s, p in pairs if p.wR in keep]
     srv = track.Server(0, slots=args.slots, hbm_budget=200 << 30)
     t0 = time.perf_counter()
     pids = [srv.synth_pair(s) for s, _ in pairs]
@@ -297,6 +300,7 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "mfma"], default="fp32")
     ap.add_argument("--pairs", type=int, default=64)
     ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--windows", default="", help="batched workload: only these window sizes (e.g. 16,32)")
     ap.add_argument("--chip", type=int, default=128, help="strackw chip size (128 or 256)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB device copy (for scaling HBM byte counters)")

@@ -866,7 +866,144 @@ __device__ inline Top2 wg_top2(Top2 a, Top2 *sh)
     }
 }
 
+// Per-patch statistics of getPatches (speckleTrack.c:1194-1330): power,
+// zero-sample count, rejection and the Doppler estimate (estDopCarrier1);
+// one workgroup of 4w threads per patch pair.
 template <int W, int STYPE>
+__global__ void __launch_bounds__(4 * W) k_cpx_prep(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
+                                                    const int *list, int count, const CpxBatch *__restrict__ Cp)
+{
+    const KPar &P = *Pp;
+    const Pts &S = *Sp;
+    const CpxBatch &C = *Cp;
+    constexpr int H = W / 2, LD = W + 1;
+    extern __shared__ float2 lds[];  // [2][W][LD]
+    __shared__ double shd[16];
+    __shared__ unsigned shmask[2][2 * W];
+    __shared__ double shang[2][4];
+    __shared__ int shflag;
+    constexpr bool isf = STYPE == ST_SAMPLE_C64;
+    const float thr = isf ? 1e-15f : 1e-9f;
+    const float thrAll = thr * W * W;
+
+    const int it = blockIdx.x;
+    if (it >= count)
+        return;
+    const int t = threadIdx.x;
+    const int g = t / (2 * W);
+    const int r = (t >> 1) % W;
+    const int h = t & 1;
+    const int pt = list[it];
+    const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
+    // ---------------- load column r of patch g: samples a = 2m + h
+    float2 x[H];
+    const void *img = g ? P.im.img2 : P.im.img1;
+    const int nr = g ? P.im.nr2 : P.im.nr1;
+    const int ra = (g ? r2 : r1) + r, aa = (g ? a2 : a1) + h;
+    unsigned low = 0;
+    double psum = 0, mre = 0, mim = 0;
+    const bool useH = (g == 1) && P.hanning;
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+        const size_t o = (size_t)(aa + 2 * m) * nr + ra;
+        if constexpr (isf) {
+            x[m] = ((const float2 *)img)[o];
+        } else {
+            const short2 q = ((const short2 *)img)[o];
+            x[m] = make_float2((float)q.x, (float)q.y);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+#pragma clang fp contract(off)
+        float2 v = x[m];
+        if constexpr (isf) {
+            v.x *= 1000;
+            v.y *= 1000;
+        }
+        const float hw = P.hann[(2 * m + h) * W + r];
+        v.x = useH ? v.x * hw : v.x;
+        v.y = useH ? v.y * hw : v.y;
+        x[m] = v;
+        const float pw = v.x * v.x + v.y * v.y;
+        low |= (pw < thr ? 1u : 0u) << m;
+        psum += pw;
+        mre += v.x;
+        mim += v.y;
+    }
+    // the scaled patches also go to LDS: the Doppler sums and the
+    // baseline ramp run over them with rolled loops (unrolled over the
+    // register copies they would dominate the register allocation)
+    float2 *R = lds + (size_t)g * W * LD;
+#pragma unroll
+    for (int m = 0; m < H; m++)
+        R[(2 * m + h) * LD + r] = x[m];
+    shmask[g][2 * r + h] = low;
+    psum = group_sum<W>(psum, shd);
+    mre = group_sum<W>(mre, shd);
+    mim = group_sum<W>(mim, shd);
+    __syncthreads();
+    int nz = 0;
+    if (g == 0)
+        nz = __popc(shmask[0][2 * r + h] | shmask[1][2 * r + h]);
+    const double nzd = wg_sum<W>((double)nz, shd);
+    if (t == 0)
+        shang[0][0] = shang[1][0] = 0;
+    __syncthreads();
+    if ((t & (2 * W - 1)) == 0)
+        shang[g][0] = psum;
+    __syncthreads();
+    const double p1s = shang[0][0], p2s = shang[1][0];
+    if (nzd > 20.5 || p1s < thrAll || p2s < thrAll) {
+        if (t == 0)
+            C.have[it] = 0;
+        return;
+    }
+    // ---------------- Doppler (estDopCarrier1): lag-1 azimuth products
+    {
+        const double cnt = (double)W * (double)W;
+        const double mr = mre / cnt, mi = mim / cnt;
+        double zr = 0, zi = 0;
+#pragma unroll 1
+        for (int e = t & (2 * W - 1); e < (W - 1) * W; e += 2 * W) {
+            const int a = e / W, c = e - (e / W) * W;
+            const float2 p0 = R[a * LD + c], p1 = R[(a + 1) * LD + c];
+            const double ar = (double)p1.x - mr, ai = (double)p1.y - mi;
+            const double br = (double)p0.x - mr, bi = (double)p0.y - mi;
+            zr += ar * br + ai * bi;
+            zi += ai * br - ar * bi;
+        }
+        zr = group_sum<W>(zr, shd);
+        zi = group_sum<W>(zi, shd);
+        if ((t & (2 * W - 1)) == 0) {
+            shang[g][1] = zr;
+            shang[g][2] = zi;
+        }
+        __syncthreads();
+        if (t == 0) {
+            const double z0r = shang[0][1], z0i = shang[0][2], z1r = shang[1][1], z1i = shang[1][2];
+            const double ang1 = (z0r == 0.0 && z0i == 0.0) ? NAN : atan2(z0i, z0r);
+            const double ang2 = (z1r == 0.0 && z1i == 0.0) ? NAN : atan2(z1i, z1r);
+            const double v = (double)(W / 2) * (ang1 + ang2) * .5 / 3.14159265358979323846;
+            int shift = isnan(v) ? 0 : ((v >= 0.0) ? (int)(v + 0.5) : (int)(v - 0.5));
+            if (shift < 0)
+                shift += W;
+            shflag = (int)((float)shift * (float)P.wA / (float)W);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        C.have[it] = 1;
+        C.az[it] = shflag;
+        C.p[2 * it] = p1s / (double)(W * W);
+        C.p[2 * it + 1] = p2s / (double)(W * W);
+    }
+}
+
+// BASE = 1 adds the baseline phase ramp on patch 1 (a separate
+// instantiation: its FP64 sin/cos would otherwise raise the register
+// allocation of every launch).
+template <int W, int STYPE, int BASE>
 __global__ void __launch_bounds__(4 * W) k_cpx_fused(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
                                                      const int *list, int count, const CpxBatch *__restrict__ Cp,
                                                      float *magScratch)
@@ -879,12 +1016,7 @@ __global__ void __launch_bounds__(4 * W) k_cpx_fused(const KPar *__restrict__ Pp
     extern __shared__ float2 lds[];  // [2][W][LD]
     __shared__ double shd[16];
     __shared__ Top2 sht[16];
-    __shared__ unsigned shmask[2][2 * W];
-    __shared__ double shang[2][4];
-    __shared__ int shflag;
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
-    const float thr = isf ? 1e-15f : 1e-9f;
-    const float thrAll = thr * W * W;
 
     for (int it = blockIdx.x; it < count; it += gridDim.x) {
         // Lane indices are re-derived behind an opaque copy every item so
@@ -898,13 +1030,14 @@ __global__ void __launch_bounds__(4 * W) k_cpx_fused(const KPar *__restrict__ Pp
         float *M = magScratch + (size_t)blockIdx.x * WW * WW;
         const int pt = list[it];
         const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
+        if (!C.have[it])
+            continue;
+        const int az = C.az[it];
         // ---------------- load column r of patch g: samples a = 2m + h
         float2 x[H];
         const void *img = g ? P.im.img2 : P.im.img1;
         const int nr = g ? P.im.nr2 : P.im.nr1;
         const int ra = (g ? r2 : r1) + r, aa = (g ? a2 : a1) + h;
-        unsigned low = 0;
-        double psum = 0, mre = 0, mim = 0;
         const bool useH = (g == 1) && P.hanning;
 #pragma unroll
         for (int m = 0; m < H; m++) {
@@ -928,91 +1061,37 @@ __global__ void __launch_bounds__(4 * W) k_cpx_fused(const KPar *__restrict__ Pp
             v.x = useH ? v.x * hw : v.x;
             v.y = useH ? v.y * hw : v.y;
             x[m] = v;
-            const float pw = v.x * v.x + v.y * v.y;
-            low |= (pw < thr ? 1u : 0u) << m;
-            psum += pw;
-            mre += v.x;
-            mim += v.y;
         }
-        shmask[g][2 * r + h] = low;
-        psum = group_sum<W>(psum, shd);
-        mre = group_sum<W>(mre, shd);
-        mim = group_sum<W>(mim, shd);
-        __syncthreads();
-        int nz = 0;
-        if (g == 0)
-            nz = __popc(shmask[0][2 * r + h] | shmask[1][2 * r + h]);
-        const double nzd = wg_sum<W>((double)nz, shd);
-        if (t == 0)
-            shang[0][0] = shang[1][0] = 0;
-        __syncthreads();
-        if ((t & (2 * W - 1)) == 0)
-            shang[g][0] = psum;
-        __syncthreads();
-        const double p1s = shang[0][0], p2s = shang[1][0];
-        if (nzd > 20.5 || p1s < thrAll || p2s < thrAll) {
-            if (t == 0)
-                C.have[it] = 0;
-            __syncthreads();
-            continue;
-        }
-        // ---------------- Doppler (estDopCarrier1): lag-1 azimuth products
-        {
-            const double cnt = (double)W * (double)W;
-            const double mr = mre / cnt, mi = mim / cnt;
-            double zr = 0, zi = 0;
-            // a = 2m + h; next sample a + 1 lives in the partner lane:
-            // h = 0 -> partner m, h = 1 -> partner m + 1
+        // ---------------- baseline phase ramp on patch 1 (in LDS)
+        if constexpr (BASE) {
+            float2 *R = lds + (size_t)g * W * LD;
 #pragma unroll
-            for (int m = 0; m < H; m++) {
-                const float2 yp = shfl_pair(x[m]);            // partner's sample m
-                const float2 yn = shfl_pair(x[m + 1 < H ? m + 1 : m]);  // partner's sample m+1
-                const bool valid = (h == 0) || (m + 1 < H);
-                const float2 nxt = h ? yn : yp;
-                if (valid) {
-                    const double ar = (double)nxt.x - mr, ai = (double)nxt.y - mi;
-                    const double br = (double)x[m].x - mr, bi = (double)x[m].y - mi;
-                    zr += ar * br + ai * bi;
-                    zi += ai * br - ar * bi;
-                }
-            }
-            zr = group_sum<W>(zr, shd);
-            zi = group_sum<W>(zi, shd);
-            if ((t & (2 * W - 1)) == 0) {
-                shang[g][1] = zr;
-                shang[g][2] = zi;
-            }
+            for (int m = 0; m < H; m++)
+                R[(2 * m + h) * LD + r] = x[m];
             __syncthreads();
-            if (t == 0) {
-                const double z0r = shang[0][1], z0i = shang[0][2], z1r = shang[1][1], z1i = shang[1][2];
-                const double ang1 = (z0r == 0.0 && z0i == 0.0) ? NAN : atan2(z0i, z0r);
-                const double ang2 = (z1r == 0.0 && z1i == 0.0) ? NAN : atan2(z1i, z1r);
-                const double v = (double)(W / 2) * (ang1 + ang2) * .5 / 3.14159265358979323846;
-                int shift = isnan(v) ? 0 : ((v >= 0.0) ? (int)(v + 0.5) : (int)(v - 0.5));
-                if (shift < 0)
-                    shift += W;
-                shflag = (int)((float)shift * (float)P.wA / (float)W);
-            }
-            __syncthreads();
-        }
-        const int az = shflag;
-        // ---------------- baseline phase ramp on patch 1
-        if (P.baseFlag && g == 0) {
-#pragma unroll
-            for (int m = 0; m < H; m++) {
-                const int i = 2 * m + h;
-                const double x0 = (a1 - P.wA / 2 - (P.nSlpA1 / 2)) / (double)P.nSlpA1;
+            const double x0 = (a1 - P.wA / 2 - (P.nSlpA1 / 2)) / (double)P.nSlpA1;
+            const double bn = P.Bn + x0 * P.dBn + x0 * x0 * P.dBnQ;
+            const double ptod = -(4.0 * 3.14159265358979323846) / P.lambda;
+            const int j0 = r1 - P.wR / 2 > 0 ? r1 - P.wR / 2 : 0;
+            float2 *R0 = lds;
+#pragma unroll 1
+            for (int e = t; e < W * W; e += NTH) {
+                const int i = e / W, j = e - (e / W) * W;
                 const double xx = x0 + i * (1. / (double)P.nSlpA1);
-                const double bn = P.Bn + x0 * P.dBn + x0 * x0 * P.dBnQ;
                 const double bp = P.Bp + xx * P.dBp + xx * xx * P.dBpQ;
                 const double bsq = bn * bn + bp * bp;
-                const int j1 = (r1 - P.wR / 2 > 0 ? r1 - P.wR / 2 : 0) + r;
-                double ph = -bn * P.sinT[j1] - bp * P.cosT[j1] + (0.5 * bsq * P.overR[j1]);
-                ph *= -(4.0 * 3.14159265358979323846) / P.lambda;
+                double ph = -bn * P.sinT[j0 + j] - bp * P.cosT[j0 + j] + (0.5 * bsq * P.overR[j0 + j]);
+                ph *= ptod;
                 const double pr = cos(ph), pim = sin(ph);
-                const double cr = x[m].x, ci = x[m].y;
-                x[m] = make_float2((float)(cr * pr - ci * pim), (float)(cr * pim + ci * pr));
+                const float2 v = R0[i * LD + j];
+                const double cr = v.x, ci = v.y;
+                R0[i * LD + j] = make_float2((float)(cr * pr - ci * pim), (float)(cr * pim + ci * pr));
             }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < H; m++)
+                x[m] = R[(2 * m + h) * LD + r];
+            __syncthreads();
         }
         // ---------------- forward FFT along azimuth (column r), then LDS
         fwd_pair<W>(x, h);
@@ -1047,19 +1126,24 @@ __global__ void __launch_bounds__(4 * W) k_cpx_fused(const KPar *__restrict__ Pp
         const int ks = krow < W / 2 ? krow : krow - W;
         float2 *Z = lds + (size_t)g * W * LD;
         Top2 best{0.0f, 0x7fffffff, 0.0f};
-#pragma unroll 1
-        for (int pp = 0; pp < 2; pp++) {
-            float2 y[H];
-            // exp(i pi (ks pp + ls q) / w) = row factor (per lane) x column factor
-            const float2 rowf = c_tw128[(ks * pp * (64 / W)) & 127];
+        // exp(i pi (ks pp + ls q) / w) = row factor (per lane, per pp) x
+        // column factor (per element, q = g only): the column factor is
+        // applied once, before the component loop
+        if (g) {
 #pragma unroll
             for (int k = 0; k < H; k++) {
                 const float2 c0 = c_tw128[(k * (64 / W)) & 127];
                 const float2 c1 = c_tw128[((k - H) * (64 / W)) & 127];
-                const float2 cf = h ? c1 : c0;
-                const float2 tw = g ? cmul_f(rowf, cf) : rowf;
-                y[k] = cmul_f(x[k], tw);
+                x[k] = cmul_f(x[k], h ? c1 : c0);
             }
+        }
+#pragma unroll 1
+        for (int pp = 0; pp < 2; pp++) {
+            float2 y[H];
+            const float2 rowf = c_tw128[(ks * pp * (64 / W)) & 127];
+#pragma unroll
+            for (int k = 0; k < H; k++)
+                y[k] = cmul_f(x[k], rowf);
             inv_pair<W>(y, h);
             // y[n'] = Z[krow][2 n' + h]
 #pragma unroll
@@ -1226,12 +1310,6 @@ __global__ void __launch_bounds__(4 * W) k_cpx_fused(const KPar *__restrict__ Pp
                                                      : M[(cI - NGAUSS / 2 + k) * WW + cJ];
             }
         }
-        if (t == 0) {
-            C.have[it] = 1;
-            C.az[it] = az;
-            C.p[2 * it] = p1s / (double)(W * W);
-            C.p[2 * it + 1] = p2s / (double)(W * W);
-        }
         __syncthreads();
     }
 }
@@ -1368,21 +1446,35 @@ T *upload(st_ctx *ctx, const T *h, size_t n)
     return d;
 }
 
-template <int W, int STYPE>
-void launch_cpx_fused_t(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
-                        const CpxBatch *dC)
+template <int W, int STYPE, int BASE>
+void launch_fused_kernel(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
+                         const CpxBatch *dC, size_t lds)
 {
-    const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
     int perCU = 0, ncu = 0;
-    ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, STYPE>, 4 * W, lds));
+    ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, STYPE, BASE>, 4 * W, lds));
     ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     const int grid = std::max(1, std::min(count, std::max(1, perCU) * ncu));
     float *mag = ctx->arena.take<float>((size_t)grid * 4 * W * W);
     ctx->time_begin(slot);
-    hipLaunchKernelGGL((k_cpx_fused<W, STYPE>), dim3(grid), dim3(4 * W), lds, ctx->stream, dP, dS, list, count, dC,
-                       mag);
+    hipLaunchKernelGGL((k_cpx_fused<W, STYPE, BASE>), dim3(grid), dim3(4 * W), lds, ctx->stream, dP, dS, list, count,
+                       dC, mag);
     ctx->time_end(slot);
     ST_HIP(hipGetLastError());
+}
+
+template <int W, int STYPE>
+void launch_cpx_fused_t(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
+                        const CpxBatch *dC, bool base)
+{
+    const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
+    const int prepSlot = ctx->kernel_slot("k_cpx_prep");
+    ctx->time_begin(prepSlot);
+    hipLaunchKernelGGL((k_cpx_prep<W, STYPE>), dim3(count), dim3(4 * W), lds, ctx->stream, dP, dS, list, count, dC);
+    ctx->time_end(prepSlot);
+    if (base)
+        launch_fused_kernel<W, STYPE, 1>(ctx, slot, dP, dS, list, count, dC, lds);
+    else
+        launch_fused_kernel<W, STYPE, 0>(ctx, slot, dP, dS, list, count, dC, lds);
 }
 
 // Resident workgroups per CU of the fused kernel (sizes its |C| scratch).
@@ -1392,9 +1484,9 @@ int fused_per_cu(int stype)
     const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
     int perCU = 0;
     if (stype == ST_SAMPLE_C64)
-        ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, ST_SAMPLE_C64>, 4 * W, lds));
+        ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, ST_SAMPLE_C64, 0>, 4 * W, lds));
     else
-        ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, ST_SAMPLE_CI16>, 4 * W, lds));
+        ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, ST_SAMPLE_CI16, 0>, 4 * W, lds));
     return std::max(1, perCU);
 }
 
@@ -1411,9 +1503,9 @@ void launch_cpx_fused(st_ctx *ctx, int slot, const KPar &P, const Pts &S, const 
     ST_HIP(hipMemcpyAsync(dS, &S, sizeof(Pts), hipMemcpyHostToDevice, ctx->stream));
     ST_HIP(hipMemcpyAsync(dC, &C, sizeof(CpxBatch), hipMemcpyHostToDevice, ctx->stream));
     if (P.im.stype == ST_SAMPLE_C64)
-        launch_cpx_fused_t<W, ST_SAMPLE_C64>(ctx, slot, dP, dS, list, count, dC);
+        launch_cpx_fused_t<W, ST_SAMPLE_C64>(ctx, slot, dP, dS, list, count, dC, P.baseFlag != 0);
     else
-        launch_cpx_fused_t<W, ST_SAMPLE_CI16>(ctx, slot, dP, dS, list, count, dC);
+        launch_cpx_fused_t<W, ST_SAMPLE_CI16>(ctx, slot, dP, dS, list, count, dC, P.baseFlag != 0);
 }
 
 bool staged_only()
