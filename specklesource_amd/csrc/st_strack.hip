@@ -16,6 +16,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -1002,9 +1003,11 @@ __global__ void __launch_bounds__(4 * W) k_cpx_prep(const KPar *__restrict__ Pp,
 
 // BASE = 1 adds the baseline phase ramp on patch 1 (a separate
 // instantiation: its FP64 sin/cos would otherwise raise the register
-// allocation of every launch).
+// allocation of every launch).  At least two waves per SIMD: at w = 64
+// this caps the allocation at 256 VGPRs (a few spilled values) instead of
+// one wave per SIMD.
 template <int W, int STYPE, int BASE>
-__global__ void __launch_bounds__(4 * W) k_cpx_fused(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
+__global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8))) k_cpx_fused(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
                                                      const int *list, int count, const CpxBatch *__restrict__ Cp,
                                                      float *magScratch)
 {
@@ -1446,6 +1449,8 @@ T *upload(st_ctx *ctx, const T *h, size_t n)
     return d;
 }
 
+#include "st_cpx_mfma.h"
+
 template <int W, int STYPE, int BASE>
 void launch_fused_kernel(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
                          const CpxBatch *dC, size_t lds)
@@ -1508,6 +1513,130 @@ void launch_cpx_fused(st_ctx *ctx, int slot, const KPar &P, const Pts &S, const 
         launch_cpx_fused_t<W, ST_SAMPLE_CI16>(ctx, slot, dP, dS, list, count, dC, P.baseFlag != 0);
 }
 
+// bf16 of a float, round to nearest even (host side of the operand split)
+uint16_t bf16_bits(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+float bf16_value(uint16_t b)
+{
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// A-operand tables of the MFMA complex match for window W (built once per
+// device and window, kept for the life of the process).
+MfmaTab mfma_tables(st_ctx *ctx, int W)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, MfmaTab> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_pair(ctx->device, W);
+    auto f = cache.find(key);
+    if (f != cache.end())
+        return f->second;
+    const int K2 = 2 * W;
+    const double pi = 3.14159265358979323846;
+    // [matrix][re/im output][hi/lo] host planes
+    std::vector<uint16_t> h[2][2][2];
+    const int rows[2] = {W, 2 * W};
+    for (int mtx = 0; mtx < 2; mtx++)
+        for (int o = 0; o < 2; o++)
+            for (int hl = 0; hl < 2; hl++)
+                h[mtx][o][hl].assign((size_t)rows[mtx] * K2, 0);
+    auto put = [&](int mtx, int o, int row, int col, double v) {
+        const float hi = bf16_value(bf16_bits((float)v));
+        h[mtx][o][0][(size_t)row * K2 + col] = bf16_bits((float)v);
+        h[mtx][o][1][(size_t)row * K2 + col] = bf16_bits((float)(v - (double)hi));
+    };
+    for (int k = 0; k < W; k++)
+        for (int j = 0; j < W; j++) {
+            // A_W[k][j] = exp(-2 pi i k j / W)
+            const double ang = -2.0 * pi * (double)((long)k * j % W) / W;
+            const double ar = cos(ang), ai = sin(ang);
+            put(0, 0, k, j, ar);
+            put(0, 0, k, W + j, -ai);
+            put(0, 1, k, j, ai);
+            put(0, 1, k, W + j, ar);
+        }
+    for (int m = 0; m < 2 * W; m++)
+        for (int u = 0; u < W; u++) {
+            // B[m][u] = exp(+2 pi i m u_s / 2W), u_s signed (Nyquist negative)
+            const int us = u < W / 2 ? u : u - W;
+            const long e = (((long)m * us) % (2 * W) + 2 * W) % (2 * W);
+            const double ang = 2.0 * pi * (double)e / (2.0 * W);
+            const double br = cos(ang), bi = sin(ang);
+            put(1, 0, m, u, br);
+            put(1, 0, m, W + u, -bi);
+            put(1, 1, m, u, bi);
+            put(1, 1, m, W + u, br);
+        }
+    MfmaTab t{};
+    for (int mtx = 0; mtx < 2; mtx++)
+        for (int o = 0; o < 2; o++)
+            for (int hl = 0; hl < 2; hl++) {
+                void *d = nullptr;
+                const size_t bytes = h[mtx][o][hl].size() * 2;
+                ST_HIP(hipMalloc(&d, bytes));
+                ST_HIP(hipMemcpy(d, h[mtx][o][hl].data(), bytes, hipMemcpyHostToDevice));
+                (mtx == 0 ? t.fa : t.ib)[o][hl] = (const __bf16 *)d;
+            }
+    cache[key] = t;
+    return t;
+}
+
+template <int W, int STYPE>
+void launch_mfma_kernel(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
+                        const CpxBatch *dC, const MfmaTab &tab)
+{
+    using G = MfGeom<W>;
+    const size_t lds = (size_t)G::PPB * G::BYTES;
+    int perCU = 0, ncu = 0;
+    ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_mfma<W, STYPE>, 256, lds));
+    ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const int blocksNeeded = (count + G::PPB - 1) / G::PPB;
+    const int grid = std::max(1, std::min(blocksNeeded, std::max(1, perCU) * ncu));
+    ctx->time_begin(slot);
+    hipLaunchKernelGGL((k_cpx_mfma<W, STYPE>), dim3(grid), dim3(256), lds, ctx->stream, dP, dS, list, count, dC,
+                       tab);
+    ctx->time_end(slot);
+    ST_HIP(hipGetLastError());
+}
+
+// Complex attempts on the matrix cores (windows 16 and 32).
+template <int W>
+void launch_cpx_mfma(st_ctx *ctx, int slot, const KPar &P, const Pts &S, const int *list, int count,
+                     const CpxBatch &C)
+{
+    const MfmaTab tab = mfma_tables(ctx, W);
+    KPar *dP = ctx->arena.take<KPar>(1);
+    Pts *dS = ctx->arena.take<Pts>(1);
+    CpxBatch *dC = ctx->arena.take<CpxBatch>(1);
+    ST_HIP(hipMemcpyAsync(dP, &P, sizeof(KPar), hipMemcpyHostToDevice, ctx->stream));
+    ST_HIP(hipMemcpyAsync(dS, &S, sizeof(Pts), hipMemcpyHostToDevice, ctx->stream));
+    ST_HIP(hipMemcpyAsync(dC, &C, sizeof(CpxBatch), hipMemcpyHostToDevice, ctx->stream));
+    const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
+    const int prepSlot = ctx->kernel_slot("k_cpx_prep");
+    ctx->time_begin(prepSlot);
+    if (P.im.stype == ST_SAMPLE_C64)
+        hipLaunchKernelGGL((k_cpx_prep<W, ST_SAMPLE_C64>), dim3(count), dim3(4 * W), lds, ctx->stream, dP, dS, list,
+                           count, dC);
+    else
+        hipLaunchKernelGGL((k_cpx_prep<W, ST_SAMPLE_CI16>), dim3(count), dim3(4 * W), lds, ctx->stream, dP, dS, list,
+                           count, dC);
+    ctx->time_end(prepSlot);
+    if (P.im.stype == ST_SAMPLE_C64)
+        launch_mfma_kernel<W, ST_SAMPLE_C64>(ctx, slot, dP, dS, list, count, dC, tab);
+    else
+        launch_mfma_kernel<W, ST_SAMPLE_CI16>(ctx, slot, dP, dS, list, count, dC, tab);
+}
+
 bool staged_only()
 {
     const char *e = getenv("SPECKLE_AMD_PATH");
@@ -1541,8 +1670,12 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     ctx->stats_reset();
     hipEvent_t *sev = ctx->ev + 64;  // stage boundary events
     ST_HIP(hipEventRecord(sev[0], st));
+    // the matrix-core complex match covers windows 16 and 32 without the
+    // baseline ramp; other cases run the FP32 paths
+    const bool mfmaPath = par.precision == ST_PREC_MFMA && (par.wR == 16 || par.wR == 32) && !par.baseFlag &&
+                          !staged_only();
     const bool fusedPath = (par.wR == 16 || par.wR == 32 || par.wR == 64) && !staged_only();
-    const int kPeak = ctx->kernel_slot(fusedPath ? "k_cpx_fused" : "peak_refine");
+    const int kPeak = ctx->kernel_slot(mfmaPath ? "k_cpx_mfma" : (fusedPath ? "k_cpx_fused" : "peak_refine"));
     const int kGather = ctx->kernel_slot("cpx_gather"), kFft = ctx->kernel_slot("fft2d"),
               kAmpDet = ctx->kernel_slot("amp_detect");
     const int npts = par.nR * par.nA;
@@ -1673,7 +1806,22 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
 
     // ---- complex attempts
     const bool fused = (w == 16 || w == 32 || w == 64) && !staged_only();
-    if (nCpx > 0 && fused) {
+    if (nCpx > 0 && mfmaPath) {
+        CpxBatch C{};
+        C.res = ar.take<PeakRes>(nCpx);
+        C.sig = ar.take<float>(2 * (size_t)nCpx);
+        C.have = ar.take<uint8_t>(nCpx);
+        C.az = ar.take<int>(nCpx);
+        C.p = ar.take<double>(2 * (size_t)nCpx);
+        if (w == 16)
+            launch_cpx_mfma<16>(ctx, kPeak, P, S, list, nCpx, C);
+        else
+            launch_cpx_mfma<32>(ctx, kPeak, P, S, list, nCpx, C);
+        hipLaunchKernelGGL(k_lm, dim3((2 * nCpx + 127) / 128), dim3(128), 0, st, C.res, nCpx, C.sig);
+        hipLaunchKernelGGL(k_cpx_finalize, dim3((nCpx + 127) / 128), dim3(128), 0, st, P, S, list, 0, nCpx, C);
+        ST_HIP(hipGetLastError());
+        ar.used = mark;
+    } else if (nCpx > 0 && fused) {
         CpxBatch C{};
         C.res = ar.take<PeakRes>(nCpx);
         C.sig = ar.take<float>(2 * (size_t)nCpx);

@@ -56,11 +56,12 @@ def truth_grid(spec, par):
     return tr, ta
 
 
-def compare_strack(par, g, o32, o64):
+def compare_strack(par, g, o32, o64, tie=TIE, thresh_tol=THRESH_TOL, fine_tie=1e-3):
     """Parity report of a GPU run g against the FP32-faithful (o32) and FP64
     (o64) oracle runs (all with diagnostics).  Returns (report, problems):
     problems lists points whose disagreement is not explained by a near-tie
-    or a near-threshold value."""
+    (relative top-two gap below tie, fine-surface gap below fine_tie) or a
+    value within thresh_tol (relative) of an acceptance threshold."""
     rep = {}
     probs = []
     n = par.nR * par.nA
@@ -72,7 +73,7 @@ def compare_strack(par, g, o32, o64):
         ran = (d32["peakI"][t] > -2000) & (dg["peakI"][t] > -2000)
         same = ran & (d32["peakI"][t] == dg["peakI"][t]) & (d32["peakJ"][t] == dg["peakJ"][t])
         diff = ran & ~same
-        near = diff & ((d32["peakGap"][t] < TIE) | (dg["peakGap"][t] < TIE))
+        near = diff & ((d32["peakGap"][t] < tie) | (dg["peakGap"][t] < tie))
         tot += int(ran.sum())
         agree += int(same.sum())
         ties += int(near.sum())
@@ -94,10 +95,10 @@ def compare_strack(par, g, o32, o64):
             continue
         c = [d32["cmax"][t][idx] for t in range(3)]
         for t, thr in ((0, 0.185), (1, 0.07), (2, 0.028)):
-            if abs(c[t] - thr) < THRESH_TOL * thr:
+            if abs(c[t] - thr) < thresh_tol * thr:
                 expl[idx] = True
         # fine peak moved by one quantum near the acceptance limit
-        if any(d32["fineGap"][t][idx] < 1e-3 for t in range(3)):
+        if any(d32["fineGap"][t][idx] < fine_tie for t in range(3)):
             expl[idx] = True
     for idx in zip(*np.nonzero(tmis & ~expl)):
         probs.append(("type", idx, int(o32.type[idx]), int(g.type[idx])))

@@ -1,0 +1,94 @@
+"""The matrix-core complex match (precision = ST_PREC_MFMA, windows 16 and
+32): bf16 hi + lo split products with FP32 accumulation.
+
+Tolerances: the split keeps about 16 significant bits per operand, so
+surface values carry ~2e-5 relative error; a top-two gap below MF_TIE
+(5e-4) or a correlation within MF_THRESH (1e-3, relative) of an acceptance
+threshold counts as a near-tie.  Everything else must agree with the
+FP32-faithful oracle, and sub-pixel offsets must stay within 0.02 px RMS
+of the FP64 oracle."""
+import numpy as np
+import pytest
+
+from specklesource_amd import abi, configs, track
+from tests.helpers import compare_strack, periodic_pair, small_params
+
+pytestmark = pytest.mark.gpu
+
+MF_TIE = 5e-4
+MF_THRESH = 1e-3
+RMS_TOL = 0.02
+
+
+def _run(device, par, a, b, stype=abi.ST_SAMPLE_C64):
+    pid = device.upload_pair(a, b, stype)
+    try:
+        device.set_profiling(True)
+        g = device.speckle_track(pid, par, diag=True)
+        st = device.last_stats()
+        device.set_profiling(False)
+    finally:
+        device.release_pair(pid)
+    names = {st.kernelNames[i].value.decode(): st.kernelLaunches[i] for i in range(8)}
+    return g, names
+
+
+@pytest.mark.parametrize("w", [16, 32])
+def test_mfma_circular_shift_known_answer(device, w):
+    img1, img2 = periodic_pair(w, 8 * w, 8 * w, 3, -2, seed=w)
+    par = small_params(w, 8 * w, 8 * w, step=w // 2, hanningFlag=0, rShiftPoly=[3, 0, 0], aShiftPoly=[-2, 0, 0],
+                       precision=abi.ST_PREC_MFMA)
+    g, names = _run(device, par, img1, img2)
+    assert names.get("k_cpx_mfma", 0) > 0
+    assert (g.type == abi.ST_CMATCH).all()
+    assert np.all(g.offR == 3) and np.all(g.offA == -2)
+    assert np.allclose(g.corr, 1.0, atol=2e-3)
+
+
+@pytest.mark.parametrize("w", [16, 32])
+def test_mfma_parity_synthetic(device, oracle32, oracle64, w):
+    s = configs._spec(480, 480, 500 + w, dr0=0.8, drR=1e-3, da0=-1.3, daA=2e-3, gmin=0.35, gmax=0.85,
+                      gLr=300, gLa=250, maskR0=300, maskR1=380, maskA0=100, maskA1=220)
+    p = small_params(w, 480, 480, step=w // 2, rShiftPoly=[0.8, 1e-3, 0], aShiftPoly=[-1.3, 0, 2e-3],
+                     precision=abi.ST_PREC_MFMA)
+    a, b = track.synth_host(s)
+    g, names = _run(device, p, a, b)
+    assert names.get("k_cpx_mfma", 0) > 0
+    o32 = oracle32.speckle_track(p, a, b, diag=True)
+    o64 = oracle64.speckle_track(p, a, b, diag=True)
+    rep, probs = compare_strack(p, g, o32, o64, tie=MF_TIE, thresh_tol=MF_THRESH)
+    print("\n", rep)
+    assert not probs, probs[:10]
+    assert rep["peak_agree"] >= 0.999 or rep["attempts"] - rep["near_ties"] <= round(rep["peak_agree"] * rep["attempts"])
+    assert rep["rms_r_vs_f64"] <= RMS_TOL and rep["rms_a_vs_f64"] <= RMS_TOL
+    assert rep["corr_relerr_median"] < 1e-4
+
+
+def test_mfma_agrees_with_fp32_gpu_path(device):
+    """Same scene through both precisions: integer peaks and match types
+    agree at >= 99.9 % of the grid points, offsets within one 1/24 px
+    quantum there."""
+    s = configs._spec(640, 640, 77, dr0=1.4, da0=0.6, gmin=0.4, gmax=0.9, gLr=400, gLa=300)
+    a, b = track.synth_host(s)
+    for w in (16, 32):
+        p32 = small_params(w, 640, 640, step=w // 2, rShiftPoly=[1.4, 0, 0], aShiftPoly=[0.6, 0, 0])
+        pmf = small_params(w, 640, 640, step=w // 2, rShiftPoly=[1.4, 0, 0], aShiftPoly=[0.6, 0, 0],
+                           precision=abi.ST_PREC_MFMA)
+        g32, _ = _run(device, p32, a, b)
+        gmf, _ = _run(device, pmf, a, b)
+        same = (g32.diag["peakI"][0] == gmf.diag["peakI"][0]) & (g32.diag["peakJ"][0] == gmf.diag["peakJ"][0])
+        assert same.mean() >= 0.999, (w, same.mean())
+        both = (g32.type == gmf.type) & (g32.type > 0)
+        assert (g32.type == gmf.type).mean() >= 0.999
+        d = np.maximum(np.abs(g32.offR - gmf.offR), np.abs(g32.offA - gmf.offA))[both]
+        assert (d <= 1.0 / 24 + 1e-6).mean() >= 0.999
+
+
+def test_mfma_falls_back_for_other_windows(device):
+    """Windows outside {16, 32} run the FP32 path under the MFMA request."""
+    s = configs._spec(480, 480, 9, dr0=0.5, da0=0.5, gmin=0.8, gmax=0.8)
+    a, b = track.synth_host(s)
+    p = small_params(64, 480, 480, step=32, precision=abi.ST_PREC_MFMA)
+    g, names = _run(device, p, a, b)
+    assert names.get("k_cpx_mfma", 0) == 0 and names.get("k_cpx_fused", 0) > 0
+    assert g.nComplex > 0
