@@ -1,32 +1,36 @@
 // Complex match with the transforms as dense contractions on the matrix
 // cores (included by st_strack.hip inside its anonymous namespace, after
-// KPar / Pts / CpxBatch / PeakRes).
+// KPar / Pts / CpxBatch / PeakRes / Top2).
 //
-// Per patch pair (window W in {16, 32}):
-//   S1  L_g  = A_W P_g              (DFT along azimuth, both patches)
-//   S2  F_g^T = A_W L_g^T           (DFT along range)
-//   X~[u][v] = F_1 conj(F_2) at ((u + az) mod W, v)   (cmpPSWithPad's
-//             rotated corners, Nyquist on the negative side)
-//   S3  G^T  = B X~^T               (padded inverse along range)
-//   S4  C    = B G                  (padded inverse along azimuth)
-// with A_W[k][j] = exp(-2 pi i k j / W) and B[m][u] = exp(+2 pi i m u_s /
-// 2W), u_s the signed frequency of u.  Each complex product runs as real
-// MFMA products on the stacked [re; im] operands; every operand is split
-// into bf16 hi + lo parts and a product is Ah Bh + Ah Bl + Al Bh with FP32
-// accumulation (about 16 significant bits per operand).  |C| goes to LDS;
-// the peak search, the FP64 8x8 -> 48x48 refinement and the Gaussian-fit
-// samples follow as in the FP32 path.
+// Per patch pair (window W in {16, 32}), with A_W[k][j] = exp(-2 pi i k j
+// / W) and B[m][u] = exp(+2 pi i m u_s / 2W) (u_s the signed frequency of
+// u, Nyquist on the negative side):
+//   S1  L_g^T[r][k] = sum_a P_g^T[r][a] A_W[a][k]      (along azimuth)
+//   S2  F_g^T[l][k] = sum_r A_W[l][r] L_g^T[r][k]      (along range)
+//   X~[u][v] = F_1 conj(F_2) at ((u + az) mod W, v)    (cmpPSWithPad's
+//             rotated corners)
+//   S3  G[u][n]     = sum_v X~[u][v] B[n][v]           (padded inverse, range)
+//   S4  C[m][n]     = sum_u B[m][u] G[u][n]            (padded inverse, azimuth)
+// The operand roles alternate so that every stage contracts over the row
+// index of the previous stage's output tile: each lane's four accumulator
+// rows then land as one contiguous 16-byte run of the next operand.
+// Complex values are interleaved (re, im) along K; a complex product is two
+// real MFMA products against the [re, -im] / [im, re] table rows.  Every
+// operand is split into bf16 hi + lo and a product is Ah Bh + Ah Bl + Al Bh
+// with FP32 accumulation (about 16 significant bits per operand).  |C| goes
+// to LDS; the peak search, the FP64 8x8 -> 48x48 refinement and the
+// Gaussian-fit samples follow as in the FP32 path.
 #pragma once
 
 typedef __bf16 mf_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
 
-// Operand tables of one window size in global memory: for each of the two
-// complex matrices (A_W: W x W, B: 2W x W) the real-form A operands of the
-// real and imaginary outputs, [row][2W] bf16, hi and lo parts.
+// Table operands of one window size in global memory, [row][2W] bf16 with
+// (re, im) interleaved along K: fa for A_W (W rows), ib for B (2W rows);
+// [0] gives the real part of a product, [1] the imaginary part; hi and lo.
 struct MfmaTab {
-    const __bf16 *fa[2][2];  // [re/im output][hi/lo], W rows
-    const __bf16 *ib[2][2];  // [re/im output][hi/lo], 2W rows
+    const __bf16 *fa[2][2];
+    const __bf16 *ib[2][2];
 };
 
 __device__ inline mf_f4 mf_mma(mf_bf16x8 a, mf_bf16x8 b, mf_f4 c)
@@ -34,67 +38,39 @@ __device__ inline mf_f4 mf_mma(mf_bf16x8 a, mf_bf16x8 b, mf_f4 c)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ inline void mf_split(float x, __bf16 &hi, __bf16 &lo)
+// 8 consecutive K values of row `row` (chunk kc) for this lane.
+__device__ inline mf_bf16x8 mf_frag(const __bf16 *p, int pitch, int row, int kc)
 {
-    hi = (__bf16)x;
-    lo = (__bf16)(x - (float)hi);
+    return *(const mf_bf16x8 *)(p + row * pitch + kc * 32 + 8 * ((threadIdx.x & 63) >> 4));
+}
+
+// hi/lo split of four complex values, packed as (re, im) pairs
+__device__ inline void mf_split4(const mf_f4 &re, const mf_f4 &im, mf_bf16x8 &hi, mf_bf16x8 &lo)
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const __bf16 hr = (__bf16)re[q], hi_ = (__bf16)im[q];
+        hi[2 * q] = hr;
+        hi[2 * q + 1] = hi_;
+        lo[2 * q] = (__bf16)(re[q] - (float)hr);
+        lo[2 * q + 1] = (__bf16)(im[q] - (float)hi_);
+    }
 }
 
 template <int W>
 struct MfGeom {
-    static constexpr int LDK = 2 * W + 8;                   // row pitch (bf16) of a [n][2W] operand
-    static constexpr int PLANE = W * LDK;                   // one W-row operand plane
+    static constexpr int LDK = 2 * W + 8;  // row pitch (bf16) of an LDS operand
+    static constexpr int PLANE = W * LDK;  // W rows
     // bytes: P (2 patches, hi+lo), then G (2W rows, hi+lo), then the
     // refinement rows T (8 x 48 double2)
     static constexpr int R0 = 8 * W * LDK > 8 * 48 * 16 ? 8 * W * LDK : 8 * 48 * 16;
-    // L (2 patches, hi+lo), then |C| (2W x 2W float)
+    // L (2 patches, hi+lo), then |C| (2W x 2W float, transposed)
     static constexpr int R1 = 8 * W * LDK > 16 * W * W ? 8 * W * LDK : 16 * W * W;
-    static constexpr int R2 = 4 * W * LDK;                  // X~ (hi+lo), then the 8 x 8 doubles
+    static constexpr int R2 = 4 * W * LDK;  // X~ (hi+lo), then the 8 x 8 doubles
     static constexpr int BYTES = R0 + R1 + R2;
-    static constexpr int WPP = W == 16 ? 1 : 4;             // waves per patch
-    static constexpr int PPB = 4 / WPP;                     // patches per 256-thread block
+    static constexpr int WPP = W == 16 ? 1 : 4;  // waves per patch
+    static constexpr int PPB = 4 / WPP;          // patches per 256-thread block
 };
-
-// One 16 x 16 output tile pair (re, im) of  Y = A X  over KC chunks of 32
-// real K values: A from the global table (rows m0..m0+15), X from an LDS
-// operand [n][k] (n0..n0+15).
-template <int KC>
-__device__ inline void mf_tile(const __bf16 *__restrict__ ArH, const __bf16 *__restrict__ ArL,
-                               const __bf16 *__restrict__ AiH, const __bf16 *__restrict__ AiL, int lda, int m0,
-                               const __bf16 *XH, const __bf16 *XL, int ldx, int n0, mf_f4 &accR, mf_f4 &accI)
-{
-    const int lane = threadIdx.x & 63;
-    const int row = m0 + (lane & 15), kq = 8 * (lane >> 4);
-    const int col = n0 + (lane & 15);
-#pragma unroll
-    for (int kc = 0; kc < KC; kc++) {
-        const int ka = row * lda + kc * 32 + kq;
-        const mf_bf16x8 arh = *(const mf_bf16x8 *)(ArH + ka), arl = *(const mf_bf16x8 *)(ArL + ka);
-        const mf_bf16x8 aih = *(const mf_bf16x8 *)(AiH + ka), ail = *(const mf_bf16x8 *)(AiL + ka);
-        const int kx = col * ldx + kc * 32 + kq;
-        const mf_bf16x8 xh = *(const mf_bf16x8 *)(XH + kx), xl = *(const mf_bf16x8 *)(XL + kx);
-        accR = mf_mma(arh, xh, accR);
-        accR = mf_mma(arh, xl, accR);
-        accR = mf_mma(arl, xh, accR);
-        accI = mf_mma(aih, xh, accI);
-        accI = mf_mma(aih, xl, accI);
-        accI = mf_mma(ail, xh, accI);
-    }
-}
-
-// Store a complex value into an LDS operand [n][2W] (re at k, im at W + k).
-template <int W>
-__device__ inline void mf_put(__bf16 *H, __bf16 *L, int n, int k, float re, float im)
-{
-    constexpr int LDK = MfGeom<W>::LDK;
-    __bf16 h, l;
-    mf_split(re, h, l);
-    H[n * LDK + k] = h;
-    L[n * LDK + k] = l;
-    mf_split(im, h, l);
-    H[n * LDK + W + k] = h;
-    L[n * LDK + W + k] = l;
-}
 
 template <int WPP>
 __device__ inline void mf_sync()
@@ -105,6 +81,51 @@ __device__ inline void mf_sync()
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     } else {
         __syncthreads();
+    }
+}
+
+// Table fragments of one 16-row block: [chunk][re/im][hi/lo]
+template <int KC>
+struct MfTab {
+    mf_bf16x8 f[KC][2][2];
+};
+
+template <int KC>
+__device__ inline void mf_load_tab(MfTab<KC> &t, const __bf16 *const (&tab)[2][2], int pitch, int row)
+{
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++)
+#pragma unroll
+        for (int o = 0; o < 2; o++)
+#pragma unroll
+            for (int hl = 0; hl < 2; hl++)
+                t.f[kc][o][hl] = mf_frag(tab[o][hl], pitch, row, kc);
+}
+
+// acc{R,I} += (data x table) with the data as the A operand (rows of the
+// output), or (table x data) with the table as the A operand.
+template <int KC, bool DATA_A>
+__device__ inline void mf_mul(const MfTab<KC> &t, const __bf16 *dH, const __bf16 *dL, int pitch, int row,
+                              mf_f4 &accR, mf_f4 &accI)
+{
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) {
+        const mf_bf16x8 xh = mf_frag(dH, pitch, row, kc), xl = mf_frag(dL, pitch, row, kc);
+        if constexpr (DATA_A) {
+            accR = mf_mma(xh, t.f[kc][0][0], accR);
+            accR = mf_mma(xl, t.f[kc][0][0], accR);
+            accR = mf_mma(xh, t.f[kc][0][1], accR);
+            accI = mf_mma(xh, t.f[kc][1][0], accI);
+            accI = mf_mma(xl, t.f[kc][1][0], accI);
+            accI = mf_mma(xh, t.f[kc][1][1], accI);
+        } else {
+            accR = mf_mma(t.f[kc][0][0], xh, accR);
+            accR = mf_mma(t.f[kc][0][0], xl, accR);
+            accR = mf_mma(t.f[kc][0][1], xh, accR);
+            accI = mf_mma(t.f[kc][1][0], xh, accI);
+            accI = mf_mma(t.f[kc][1][0], xl, accI);
+            accI = mf_mma(t.f[kc][1][1], xh, accI);
+        }
     }
 }
 
@@ -145,7 +166,7 @@ __global__ void __launch_bounds__(256) k_cpx_mfma(const KPar *__restrict__ Pp, c
         }
         const int pt = list[it];
         const int az = C.az[it];
-        // ---------------- gather both windows into the S1 operand [r][a]
+        // ---------------- gather both windows into the S1 operand [r][2a + c]
         {
             const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
 #pragma unroll 1
@@ -168,89 +189,134 @@ __global__ void __launch_bounds__(256) k_cpx_mfma(const KPar *__restrict__ Pp, c
                     v.x *= hw;
                     v.y *= hw;
                 }
-                mf_put<W>(bP + (2 * g) * G::PLANE, bP + (2 * g + 1) * G::PLANE, r, a, v.x, v.y);
+                const __bf16 hr = (__bf16)v.x, hi_ = (__bf16)v.y;
+                __bf16 *H = bP + (2 * g) * G::PLANE + r * LDK + 2 * a;
+                __bf16 *L = bP + (2 * g + 1) * G::PLANE + r * LDK + 2 * a;
+                H[0] = hr;
+                H[1] = hi_;
+                L[0] = (__bf16)(v.x - (float)hr);
+                L[1] = (__bf16)(v.y - (float)hi_);
             }
         }
         mf_sync<WPP>();
-        // ---------------- S1: L_g = A_W P_g
+        const int lrow = lane & 15, lq = 4 * (lane >> 4);
+        // ---------------- S1: L_g^T = P_g^T A_W   (tiles: table k-block x (g, r-block))
+        {
+            constexpr int NTT = W / 16, ND = 2 * (W / 16), NU = NTT * ND, PER = NU / WPP;
+            MfTab<W / 16> tb;
+            int tcur = -1;
 #pragma unroll 1
-        for (int u = wv; u < 2 * (W / 16) * (W / 16); u += WPP) {
-            const int g = u / ((W / 16) * (W / 16)), mt = (u / (W / 16)) % (W / 16), nt = u % (W / 16);
-            mf_f4 ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
-            mf_tile<W / 16>(tab.fa[0][0], tab.fa[0][1], tab.fa[1][0], tab.fa[1][1], WW, mt * 16,
-                            bP + (2 * g) * G::PLANE, bP + (2 * g + 1) * G::PLANE, LDK, nt * 16, ar, ai);
-            const int col = nt * 16 + (lane & 15);
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int row = mt * 16 + 4 * (lane >> 4) + q;
-                mf_put<W>(bL + (2 * g) * G::PLANE, bL + (2 * g + 1) * G::PLANE, row, col, ar[q], ai[q]);
+            for (int u = wv * PER; u < (wv + 1) * PER; u++) {
+                const int tt = u / ND, dt = u % ND, g = dt / (W / 16), mt = dt % (W / 16);
+                if (tt != tcur) {
+                    mf_load_tab<W / 16>(tb, tab.fa, WW, tt * 16 + lrow);
+                    tcur = tt;
+                }
+                mf_f4 ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
+                mf_mul<W / 16, true>(tb, bP + (2 * g) * G::PLANE, bP + (2 * g + 1) * G::PLANE, LDK, mt * 16 + lrow,
+                                     ar, ai);
+                // rows r = mt*16 + lq + 0..3, column k = tt*16 + lrow -> L operand [k][2r + c]
+                mf_bf16x8 hv, lv;
+                mf_split4(ar, ai, hv, lv);
+                const int k = tt * 16 + lrow, r0 = mt * 16 + lq;
+                *(mf_bf16x8 *)(bL + (2 * g) * G::PLANE + k * LDK + 2 * r0) = hv;
+                *(mf_bf16x8 *)(bL + (2 * g + 1) * G::PLANE + k * LDK + 2 * r0) = lv;
             }
         }
         mf_sync<WPP>();
         // ---------------- S2: F_g^T = A_W L_g^T, then X~ = F_1 conj(F_2) rotated
+        {
+            constexpr int NTT = W / 16, ND = W / 16, NU = NTT * ND, PER = (NU + WPP - 1) / WPP;
+            MfTab<W / 16> tb;
+            int tcur = -1;
 #pragma unroll 1
-        for (int u = wv; u < (W / 16) * (W / 16); u += WPP) {
-            const int mt = u / (W / 16), nt = u % (W / 16);
-            mf_f4 r0 = {0, 0, 0, 0}, i0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
-            mf_tile<W / 16>(tab.fa[0][0], tab.fa[0][1], tab.fa[1][0], tab.fa[1][1], WW, mt * 16, bL, bL + G::PLANE,
-                            LDK, nt * 16, r0, i0);
-            mf_tile<W / 16>(tab.fa[0][0], tab.fa[0][1], tab.fa[1][0], tab.fa[1][1], WW, mt * 16,
-                            bL + 2 * G::PLANE, bL + 3 * G::PLANE, LDK, nt * 16, r1, i1);
-            const int k = nt * 16 + (lane & 15);  // azimuth frequency
-            const int un = (k - az + W) % W;
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int l = mt * 16 + 4 * (lane >> 4) + q;  // range frequency
-                // F1 conj(F2): re = ar br + ai bi, im = ai br - ar bi
-                const float xr = r0[q] * r1[q] + i0[q] * i1[q];
-                const float xi = i0[q] * r1[q] - r0[q] * i1[q];
-                mf_put<W>(bX, bX + G::PLANE, un, l, xr, xi);
-            }
-        }
-        mf_sync<WPP>();
-        // ---------------- S3: G^T = B X~^T   (2W x W)
-        __bf16 *bG = bP;
-#pragma unroll 1
-        for (int u = wv; u < (WW / 16) * (W / 16); u += WPP) {
-            const int mt = u / (W / 16), nt = u % (W / 16);
-            mf_f4 ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
-            mf_tile<W / 16>(tab.ib[0][0], tab.ib[0][1], tab.ib[1][0], tab.ib[1][1], WW, mt * 16, bX, bX + G::PLANE,
-                            LDK, nt * 16, ar, ai);
-            const int col = nt * 16 + (lane & 15);
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int row = mt * 16 + 4 * (lane >> 4) + q;
-                mf_put<W>(bG, bG + 2 * G::PLANE, row, col, ar[q], ai[q]);
-            }
-        }
-        mf_sync<WPP>();
-        // ---------------- S4: C = B G   (2W x 2W) -> fftshifted |C|, top-2
-        Top2 best{0.0f, 0x7fffffff, 0.0f};
-#pragma unroll 1
-        for (int u = wv; u < (WW / 16) * (WW / 16); u += WPP) {
-            const int mt = u / (WW / 16), nt = u % (WW / 16);
-            mf_f4 ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
-            mf_tile<W / 16>(tab.ib[0][0], tab.ib[0][1], tab.ib[1][0], tab.ib[1][1], WW, mt * 16, bG, bG + 2 * G::PLANE,
-                            LDK, nt * 16, ar, ai);
-            const int n = nt * 16 + (lane & 15);
-            const int J = (n + W) & (WW - 1);
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-#pragma clang fp contract(off)
-                const int m = mt * 16 + 4 * (lane >> 4) + q;
-                const int I = (m + W) & (WW - 1);
-                const float x2 = ar[q] * ar[q] + ai[q] * ai[q];
-                const float v = (float)sqrt((double)x2);
-                Mg[I * WW + J] = v;
-                const int i0 = I < W ? I : I - W, j0 = J < W ? J : J - W;
-                const int qd = (I < W) ? (J < W ? 0 : 3) : (J < W ? 2 : 1);
-                const int rank = (i0 * W + j0) * 4 + qd;
-                if (v > 0.0f) {
-                    Top2 c{v, rank, 0.0f};
-                    best = top2_combine(best, c);
-                } else {
-                    best.v2 = fmaxf(best.v2, v);
+            for (int u = wv * PER; u < (wv + 1) * PER && u < NU; u++) {
+                const int tt = u / ND, nt = u % ND;
+                if (tt != tcur) {
+                    mf_load_tab<W / 16>(tb, tab.fa, WW, tt * 16 + lrow);
+                    tcur = tt;
                 }
+                mf_f4 r0 = {0, 0, 0, 0}, i0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
+                mf_mul<W / 16, false>(tb, bL, bL + G::PLANE, LDK, nt * 16 + lrow, r0, i0);
+                mf_mul<W / 16, false>(tb, bL + 2 * G::PLANE, bL + 3 * G::PLANE, LDK, nt * 16 + lrow, r1, i1);
+                // rows l = tt*16 + lq + 0..3 (range frequency), column k = nt*16 + lrow
+                mf_f4 xr, xi;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    // F1 conj(F2): re = ar br + ai bi, im = ai br - ar bi
+                    xr[q] = r0[q] * r1[q] + i0[q] * i1[q];
+                    xi[q] = i0[q] * r1[q] - r0[q] * i1[q];
+                }
+                mf_bf16x8 hv, lv;
+                mf_split4(xr, xi, hv, lv);
+                const int un = (nt * 16 + lrow - az + W) % W, l0 = tt * 16 + lq;
+                *(mf_bf16x8 *)(bX + un * LDK + 2 * l0) = hv;
+                *(mf_bf16x8 *)(bX + G::PLANE + un * LDK + 2 * l0) = lv;
+            }
+        }
+        mf_sync<WPP>();
+        // ---------------- S3: G = X~ B^T   (2W columns: table n-block x u-block)
+        __bf16 *bG = bP;
+        {
+            constexpr int NTT = WW / 16, ND = W / 16, NU = NTT * ND, PER = NU / WPP;
+            MfTab<W / 16> tb;
+            int tcur = -1;
+#pragma unroll 1
+            for (int u = wv * PER; u < (wv + 1) * PER; u++) {
+                const int tt = u / ND, mt = u % ND;
+                if (tt != tcur) {
+                    mf_load_tab<W / 16>(tb, tab.ib, WW, tt * 16 + lrow);
+                    tcur = tt;
+                }
+                mf_f4 ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
+                mf_mul<W / 16, true>(tb, bX, bX + G::PLANE, LDK, mt * 16 + lrow, ar, ai);
+                // rows u = mt*16 + lq + 0..3, column n = tt*16 + lrow -> G operand [n][2u + c]
+                mf_bf16x8 hv, lv;
+                mf_split4(ar, ai, hv, lv);
+                const int n = tt * 16 + lrow, u0 = mt * 16 + lq;
+                *(mf_bf16x8 *)(bG + n * LDK + 2 * u0) = hv;
+                *(mf_bf16x8 *)(bG + 2 * G::PLANE + n * LDK + 2 * u0) = lv;
+            }
+        }
+        mf_sync<WPP>();
+        // ---------------- S4: C = B G   (table m-block x n-block) -> |C|, top-2
+        Top2 best{0.0f, 0x7fffffff, 0.0f};
+        {
+            constexpr int NTT = WW / 16, ND = WW / 16, NU = NTT * ND, PER = NU / WPP;
+            MfTab<W / 16> tb;
+            int tcur = -1;
+#pragma unroll 1
+            for (int u = wv * PER; u < (wv + 1) * PER; u++) {
+                const int tt = u / ND, nt = u % ND;
+                if (tt != tcur) {
+                    mf_load_tab<W / 16>(tb, tab.ib, WW, tt * 16 + lrow);
+                    tcur = tt;
+                }
+                mf_f4 ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
+                mf_mul<W / 16, false>(tb, bG, bG + 2 * G::PLANE, LDK, nt * 16 + lrow, ar, ai);
+                // rows m = tt*16 + lq + 0..3, column n = nt*16 + lrow
+                const int n = nt * 16 + lrow;
+                const int J = (n + W) & (WW - 1);
+                const int I0 = (tt * 16 + lq + W) & (WW - 1);
+                mf_f4 mv;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+#pragma clang fp contract(off)
+                    const int I = I0 + q;
+                    const float x2 = ar[q] * ar[q] + ai[q] * ai[q];
+                    const float v = sqrtf(x2);
+                    mv[q] = v;
+                    const int i0 = I < W ? I : I - W, j0 = J < W ? J : J - W;
+                    const int qd = (I < W) ? (J < W ? 0 : 3) : (J < W ? 2 : 1);
+                    const int rank = (i0 * W + j0) * 4 + qd;
+                    if (v > 0.0f) {
+                        Top2 c{v, rank, 0.0f};
+                        best = top2_combine(best, c);
+                    } else {
+                        best.v2 = fmaxf(best.v2, v);
+                    }
+                }
+                *(mf_f4 *)(Mg + J * WW + I0) = mv;  // transposed: Mg[J][I]
             }
         }
         // top-2 over the patch's threads
@@ -309,7 +375,7 @@ __global__ void __launch_bounds__(256) k_cpx_mfma(const KPar *__restrict__ Pp, c
         // (overSampleC, speckleTrack.c:852-863): rows first, T[mm][v]
         double *Pm = (double *)bX;
         if (tp < NFAST * NFAST)
-            Pm[tp] = (double)Mg[(I - NFAST / 2 + tp / NFAST) * WW + (J - NFAST / 2 + tp % NFAST)];
+            Pm[tp] = (double)Mg[(J - NFAST / 2 + tp % NFAST) * WW + (I - NFAST / 2 + tp / NFAST)];
         mf_sync<WPP>();
 #pragma unroll 1
         for (int e = tp; e < NFAST * kFine; e += TPP) {
@@ -336,7 +402,7 @@ __global__ void __launch_bounds__(256) k_cpx_mfma(const KPar *__restrict__ Pp, c
                 re += d.x * tv.x - d.y * tv.y;
                 im += d.x * tv.y + d.y * tv.x;
             }
-            const double a = sqrt(re * re + im * im);
+            const double a = re * re + im * im;  // |Q|^2: same order as |Q|
             if (a > f1 || (a == f1 && e < fr)) {
                 f2 = fmax(f2, f1);
                 f1 = a;
@@ -385,8 +451,8 @@ __global__ void __launch_bounds__(256) k_cpx_mfma(const KPar *__restrict__ Pp, c
         if (tp == 0) {
             res->fi = fi;
             res->fj = fj;
-            res->fineMax = fr != 0x7fffffff ? f1 : 0.0;
-            res->fineGap = fr != 0x7fffffff ? (float)((f1 - f2) / f1) : 0.f;
+            res->fineMax = fr != 0x7fffffff ? sqrt(f1) : 0.0;
+            res->fineGap = fr != 0x7fffffff ? (float)((sqrt(f1) - sqrt(f2)) / sqrt(f1)) : 0.f;
         }
         const int iF = I * NOVER + (fi + 24) - (NOVER * NFAST) / 2;
         const int jF = J * NOVER + (fj + 24) - (NOVER * NFAST) / 2;
@@ -400,8 +466,8 @@ __global__ void __launch_bounds__(256) k_cpx_mfma(const KPar *__restrict__ Pp, c
         }
         if (ok && tp < 2 * NGAUSS) {
             const int k = tp % NGAUSS;
-            res->lmY[tp / NGAUSS][k] = tp < NGAUSS ? Mg[cI * WW + (cJ - NGAUSS / 2 + k)]
-                                                   : Mg[(cI - NGAUSS / 2 + k) * WW + cJ];
+            res->lmY[tp / NGAUSS][k] = tp < NGAUSS ? Mg[(cJ - NGAUSS / 2 + k) * WW + cI]
+                                                   : Mg[cJ * WW + (cI - NGAUSS / 2 + k)];
         }
         mf_sync<WPP>();
     }

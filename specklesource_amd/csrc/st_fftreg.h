@@ -135,4 +135,14 @@ __device__ inline void inv_pair(float2 (&x)[W / 2], int h)
     fft<H, +1>(x);
 }
 
+// Lane-group FFTs (lane pairs): the block of outputs (forward) / inputs
+// (inverse) held by lane h, and the transforms.
+template <int W, int L>
+struct LaneFFT {
+    static_assert(L == 2, "lane pairs");
+    __device__ static inline int blk(int h) { return h; }
+    __device__ static inline void fwd(float2 (&x)[W / L], int h) { fwd_pair<W>(x, h); }
+    __device__ static inline void inv(float2 (&x)[W / L], int h) { inv_pair<W>(x, h); }
+};
+
 }  // namespace fftreg

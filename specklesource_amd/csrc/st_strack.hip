@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W,
         const int n = e / W, m = e - n * W;
         const float2 z = C[e];
         const float x2 = z.x * z.x + z.y * z.y;
-        const float v = (float)sqrt((double)x2);
+        const float v = sqrtf(x2);
         const int I = n < H ? n + H : n - H, J = m < H ? m + H : m - H;
         M[I * W + J] = v;
         const int i = I < H ? I : I - H, j = J < H ? J : J - H;
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W,
             re += d.x * tv.x - d.y * tv.y;
             im += d.x * tv.y + d.y * tv.x;
         }
-        const double a = sqrt(re * re + im * im);
+        const double a = re * re + im * im;  // |Q|^2: same order as |Q|
         qv[nq++] = a;
         if (a > 0.0) {
             ArgMaxD c{a, t};
@@ -357,8 +357,8 @@ __global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W,
     if (fb.rank != 0x7fffffff) {
         r.fi = fb.rank / kFine;
         r.fj = fb.rank - r.fi * kFine;
-        r.fineMax = fb.v;
-        r.fineGap = (float)((fb.v - fsec) / fb.v);
+        r.fineMax = sqrt(fb.v);
+        r.fineGap = (float)((sqrt(fb.v) - sqrt(fsec)) / sqrt(fb.v));
     } else {
         r.fi = r.fj = 0;
         r.fineMax = 0;
@@ -1001,13 +1001,24 @@ __global__ void __launch_bounds__(4 * W) k_cpx_prep(const KPar *__restrict__ Pp,
     }
 }
 
+// Lanes per transform of the fused kernel.  Lane pairs at every window
+// size: lane quads at w = 64 (16 points per lane, 512-thread blocks) were
+// measured slower on the full-scene job (55.6 ms vs 43.6 ms per launch) --
+// the second DPP exchange stage, the per-lane twiddle selection and the
+// transposes' bank conflicts cost more than the occupancy gained.
+template <int W>
+struct FusedLanes {
+    static constexpr int L = 2;
+};
+
 // BASE = 1 adds the baseline phase ramp on patch 1 (a separate
 // instantiation: its FP64 sin/cos would otherwise raise the register
-// allocation of every launch).  At least two waves per SIMD: at w = 64
-// this caps the allocation at 256 VGPRs (a few spilled values) instead of
-// one wave per SIMD.
-template <int W, int STYPE, int BASE>
-__global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8))) k_cpx_fused(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
+// allocation of every launch).  OCC is the minimum number of waves per
+// SIMD the register allocation must allow: 2 (at w = 64 this caps the
+// allocation at 256 VGPRs with a few spilled values instead of one wave
+// per SIMD).
+template <int W, int STYPE, int BASE, int OCC>
+__global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgpu_waves_per_eu(OCC, 8))) k_cpx_fused(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
                                                      const int *list, int count, const CpxBatch *__restrict__ Cp,
                                                      float *magScratch)
 {
@@ -1015,9 +1026,11 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
     const Pts &S = *Sp;
     const CpxBatch &C = *Cp;
     using namespace ::fftreg;
-    constexpr int H = W / 2, NTH = 4 * W, LD = W + 1, WW = 2 * W;
+    constexpr int L = FusedLanes<W>::L, H = W / L, NTH = 2 * L * W, LD = W + 1, WW = 2 * W;
+    using LF = LaneFFT<W, L>;
     extern __shared__ float2 lds[];  // [2][W][LD]
-    __shared__ double shd[16];
+    __shared__ double shd[32];
+    __shared__ int shr[16];
     __shared__ Top2 sht[16];
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
 
@@ -1027,9 +1040,10 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
         // loop (they would stay live in registers for the whole kernel).
         int t = threadIdx.x;
         asm volatile("" : "+v"(t));
-        const int g = t / (2 * W);
-        const int r = (t >> 1) % W;
-        const int h = t & 1;
+        const int g = t / (L * W);
+        const int r = (t / L) % W;
+        const int h = t & (L - 1);
+        const int blk = LF::blk(h);
         float *M = magScratch + (size_t)blockIdx.x * WW * WW;
         const int pt = list[it];
         const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
@@ -1044,7 +1058,7 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
         const bool useH = (g == 1) && P.hanning;
 #pragma unroll
         for (int m = 0; m < H; m++) {
-            const size_t o = (size_t)(aa + 2 * m) * nr + ra;
+            const size_t o = (size_t)(aa + L * m) * nr + ra;
             if constexpr (isf) {
                 x[m] = ((const float2 *)img)[o];
             } else {
@@ -1060,7 +1074,7 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
                 v.x *= 1000;
                 v.y *= 1000;
             }
-            const float hw = P.hann[(2 * m + h) * W + r];
+            const float hw = P.hann[(L * m + h) * W + r];
             v.x = useH ? v.x * hw : v.x;
             v.y = useH ? v.y * hw : v.y;
             x[m] = v;
@@ -1070,7 +1084,7 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
             float2 *R = lds + (size_t)g * W * LD;
 #pragma unroll
             for (int m = 0; m < H; m++)
-                R[(2 * m + h) * LD + r] = x[m];
+                R[(L * m + h) * LD + r] = x[m];
             __syncthreads();
             const double x0 = (a1 - P.wA / 2 - (P.nSlpA1 / 2)) / (double)P.nSlpA1;
             const double bn = P.Bn + x0 * P.dBn + x0 * x0 * P.dBnQ;
@@ -1093,25 +1107,25 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < H; m++)
-                x[m] = R[(2 * m + h) * LD + r];
+                x[m] = R[(L * m + h) * LD + r];
             __syncthreads();
         }
         // ---------------- forward FFT along azimuth (column r), then LDS
-        fwd_pair<W>(x, h);
+        LF::fwd(x, h);
         float2 *F = lds + (size_t)g * W * LD;
 #pragma unroll
         for (int k = 0; k < H; k++)
-            F[(h * H + k) * LD + r] = x[k];
+            F[(blk * H + k) * LD + r] = x[k];
         __syncthreads();
         // ---------------- forward FFT along range (row r)
 #pragma unroll
         for (int m = 0; m < H; m++)
-            x[m] = F[r * LD + 2 * m + h];
+            x[m] = F[r * LD + L * m + h];
         __syncthreads();
-        fwd_pair<W>(x, h);
+        LF::fwd(x, h);
 #pragma unroll
         for (int k = 0; k < H; k++)
-            F[r * LD + h * H + k] = x[k];
+            F[r * LD + blk * H + k] = x[k];
         __syncthreads();
         // ---------------- cross spectrum X = F1 conj(F2), row r, half h
         {
@@ -1119,7 +1133,7 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
 #pragma unroll
             for (int k = 0; k < H; k++) {
 #pragma clang fp contract(off)
-                const float2 o = Fo[r * LD + h * H + k];
+                const float2 o = Fo[r * LD + blk * H + k];
                 x[k] = g == 0 ? cmulconj(x[k], o) : cmulconj(o, x[k]);
             }
         }
@@ -1135,9 +1149,16 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
         if (g) {
 #pragma unroll
             for (int k = 0; k < H; k++) {
-                const float2 c0 = c_tw128[(k * (64 / W)) & 127];
-                const float2 c1 = c_tw128[((k - H) * (64 / W)) & 127];
-                x[k] = cmul_f(x[k], h ? c1 : c0);
+                // column l = blk H + k, signed frequency (Nyquist negative)
+                float2 cf = c_tw128[(k * (64 / W)) & 127];
+#pragma unroll
+                for (int b2 = 1; b2 < L; b2++) {
+                    const int l = b2 * H + k;
+                    const int ls = l < W / 2 ? l : l - W;
+                    const float2 cb = c_tw128[(ls * (64 / W)) & 127];
+                    cf = blk == b2 ? cb : cf;
+                }
+                x[k] = cmul_f(x[k], cf);
             }
         }
 #pragma unroll 1
@@ -1147,28 +1168,28 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
 #pragma unroll
             for (int k = 0; k < H; k++)
                 y[k] = cmul_f(x[k], rowf);
-            inv_pair<W>(y, h);
-            // y[n'] = Z[krow][2 n' + h]
+            LF::inv(y, h);
+            // y[n'] = Z[krow][L n' + h]
 #pragma unroll
             for (int n = 0; n < H; n++)
-                Z[krow * LD + 2 * n + h] = y[n];
+                Z[krow * LD + L * n + h] = y[n];
             __syncthreads();
-            // columns: lane half h holds k in [h H, h H + H)
+            // columns: lane h holds k in [blk H, blk H + H)
 #pragma unroll
             for (int k = 0; k < H; k++)
-                y[k] = Z[(h * H + k) * LD + r];
+                y[k] = Z[(blk * H + k) * LD + r];
             __syncthreads();
-            inv_pair<W>(y, h);
-            // y[j] = C_pp,g[2 j + h][r]  ->  surface row n = 2 (2j + h) + pp, col m = 2 r + g
+            LF::inv(y, h);
+            // y[j] = C_pp,g[L j + h][r]  ->  surface row n = 2 (L j + h) + pp, col m = 2 r + g
             const int m = 2 * r + g;
             const int J = (m + W) & (WW - 1);
 #pragma unroll
             for (int jj = 0; jj < H; jj++) {
 #pragma clang fp contract(off)
-                const int n = 2 * (2 * jj + h) + pp;
+                const int n = 2 * (L * jj + h) + pp;
                 const int I = (n + W) & (WW - 1);
                 const float x2 = y[jj].x * y[jj].x + y[jj].y * y[jj].y;
-                const float v = (float)sqrt((double)x2);
+                const float v = sqrtf(x2);
                 M[I * WW + J] = v;
                 const int i0 = I < W ? I : I - W, j0 = J < W ? J : J - W;
                 const int q = (I < W) ? (J < W ? 0 : 3) : (J < W ? 2 : 1);
@@ -1241,7 +1262,7 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
                     re += d.x * tv.x - d.y * tv.y;
                     im += d.x * tv.y + d.y * tv.x;
                 }
-                const double a = sqrt(re * re + im * im);
+                const double a = re * re + im * im;  // |Q|^2: same order as |Q|
                 if (a > f1 || (a == f1 && e < fr)) {
                     f2 = fmax(f2, f1);
                     f1 = a;
@@ -1262,20 +1283,21 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
                     f2 = fmax(f2, b1);
             }
             if constexpr (NTH > 64) {
-                double *s1 = shd;  // [0, 4): best, [4, 8): second
-                int *sr = (int *)(shd + 8);
+                constexpr int NW = NTH / 64;
+                double *s1 = shd;  // [0, NW): best, [NW, 2 NW): second
+                int *sr = shr;
                 __syncthreads();
                 if ((t & 63) == 0) {
                     s1[t >> 6] = f1;
-                    s1[4 + (t >> 6)] = f2;
+                    s1[NW + (t >> 6)] = f2;
                     sr[t >> 6] = fr;
                 }
                 __syncthreads();
                 f1 = s1[0];
-                f2 = s1[4];
+                f2 = s1[NW];
                 fr = sr[0];
-                for (int k = 1; k < NTH / 64; k++) {
-                    const double b1 = s1[k], b2 = s1[4 + k];
+                for (int k = 1; k < NW; k++) {
+                    const double b1 = s1[k], b2 = s1[NW + k];
                     const int br = sr[k];
                     if (b1 > f1 || (b1 == f1 && br < fr)) {
                         f2 = fmax(f1, b2);
@@ -1294,8 +1316,8 @@ __global__ void __launch_bounds__(4 * W) __attribute__((amdgpu_waves_per_eu(2, 8
             if (t == 0) {
                 res->fi = fi;
                 res->fj = fj;
-                res->fineMax = fr != 0x7fffffff ? f1 : 0.0;
-                res->fineGap = fr != 0x7fffffff ? (float)((f1 - f2) / f1) : 0.f;
+                res->fineMax = fr != 0x7fffffff ? sqrt(f1) : 0.0;
+                res->fineGap = fr != 0x7fffffff ? (float)((sqrt(f1) - sqrt(f2)) / sqrt(f1)) : 0.f;
             }
             const int iF = I * NOVER + (fi + 24) - (NOVER * NFAST) / 2;
             const int jF = J * NOVER + (fj + 24) - (NOVER * NFAST) / 2;
@@ -1451,20 +1473,28 @@ T *upload(st_ctx *ctx, const T *h, size_t n)
 
 #include "st_cpx_mfma.h"
 
-template <int W, int STYPE, int BASE>
-void launch_fused_kernel(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
-                         const CpxBatch *dC, size_t lds)
+template <int W, int STYPE, int BASE, int OCC>
+void launch_fused_kernel_o(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
+                           const CpxBatch *dC, size_t lds)
 {
     int perCU = 0, ncu = 0;
-    ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, STYPE, BASE>, 4 * W, lds));
+    constexpr int NTH = 2 * FusedLanes<W>::L * W;
+    ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, STYPE, BASE, OCC>, NTH, lds));
     ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     const int grid = std::max(1, std::min(count, std::max(1, perCU) * ncu));
     float *mag = ctx->arena.take<float>((size_t)grid * 4 * W * W);
     ctx->time_begin(slot);
-    hipLaunchKernelGGL((k_cpx_fused<W, STYPE, BASE>), dim3(grid), dim3(4 * W), lds, ctx->stream, dP, dS, list, count,
-                       dC, mag);
+    hipLaunchKernelGGL((k_cpx_fused<W, STYPE, BASE, OCC>), dim3(grid), dim3(NTH), lds, ctx->stream, dP, dS, list,
+                       count, dC, mag);
     ctx->time_end(slot);
     ST_HIP(hipGetLastError());
+}
+
+template <int W, int STYPE, int BASE>
+void launch_fused_kernel(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
+                         const CpxBatch *dC, size_t lds)
+{
+    launch_fused_kernel_o<W, STYPE, BASE, 2>(ctx, slot, dP, dS, list, count, dC, lds);
 }
 
 template <int W, int STYPE>
@@ -1482,17 +1512,24 @@ void launch_cpx_fused_t(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, co
         launch_fused_kernel<W, STYPE, 0>(ctx, slot, dP, dS, list, count, dC, lds);
 }
 
+template <int W, int OCC>
+int fused_per_cu_o(int stype)
+{
+    const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
+    constexpr int NTH = 2 * FusedLanes<W>::L * W;
+    int perCU = 0;
+    if (stype == ST_SAMPLE_C64)
+        ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, ST_SAMPLE_C64, 0, OCC>, NTH, lds));
+    else
+        ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, ST_SAMPLE_CI16, 0, OCC>, NTH, lds));
+    return std::max(1, perCU);
+}
+
 // Resident workgroups per CU of the fused kernel (sizes its |C| scratch).
 template <int W>
 int fused_per_cu(int stype)
 {
-    const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
-    int perCU = 0;
-    if (stype == ST_SAMPLE_C64)
-        ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, ST_SAMPLE_C64, 0>, 4 * W, lds));
-    else
-        ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, ST_SAMPLE_CI16, 0>, 4 * W, lds));
-    return std::max(1, perCU);
+    return fused_per_cu_o<W, 2>(stype);
 }
 
 template <int W>
@@ -1555,15 +1592,18 @@ MfmaTab mfma_tables(st_ctx *ctx, int W)
         h[mtx][o][0][(size_t)row * K2 + col] = bf16_bits((float)v);
         h[mtx][o][1][(size_t)row * K2 + col] = bf16_bits((float)(v - (double)hi));
     };
+    // (re, im) interleaved along K: row entries [2j] / [2j + 1] are
+    // (Mr, -Mi) for the real part of a product and (Mi, Mr) for the
+    // imaginary part
     for (int k = 0; k < W; k++)
         for (int j = 0; j < W; j++) {
             // A_W[k][j] = exp(-2 pi i k j / W)
             const double ang = -2.0 * pi * (double)((long)k * j % W) / W;
             const double ar = cos(ang), ai = sin(ang);
-            put(0, 0, k, j, ar);
-            put(0, 0, k, W + j, -ai);
-            put(0, 1, k, j, ai);
-            put(0, 1, k, W + j, ar);
+            put(0, 0, k, 2 * j, ar);
+            put(0, 0, k, 2 * j + 1, -ai);
+            put(0, 1, k, 2 * j, ai);
+            put(0, 1, k, 2 * j + 1, ar);
         }
     for (int m = 0; m < 2 * W; m++)
         for (int u = 0; u < W; u++) {
@@ -1572,10 +1612,10 @@ MfmaTab mfma_tables(st_ctx *ctx, int W)
             const long e = (((long)m * us) % (2 * W) + 2 * W) % (2 * W);
             const double ang = 2.0 * pi * (double)e / (2.0 * W);
             const double br = cos(ang), bi = sin(ang);
-            put(1, 0, m, u, br);
-            put(1, 0, m, W + u, -bi);
-            put(1, 1, m, u, bi);
-            put(1, 1, m, W + u, br);
+            put(1, 0, m, 2 * u, br);
+            put(1, 0, m, 2 * u + 1, -bi);
+            put(1, 1, m, 2 * u, bi);
+            put(1, 1, m, 2 * u + 1, br);
         }
     MfmaTab t{};
     for (int mtx = 0; mtx < 2; mtx++)
