@@ -760,9 +760,10 @@ __global__ void k_compose(KPar P, Pts S, float *offR, float *offA, int npts)
 
 // ---------------------------------------------------------------------
 // Fused complex match for w in {16, 32, 64}: one workgroup of 4w threads
-// per patch pair, persistent over the work list.  The whole
-// getPatches -> cmpPSWithPad -> 2w x 2w inverse -> findCPeak ->
-// overSampleC chain stays on chip:
+// per patch pair, persistent over the work list.  With the window
+// statistics and the Doppler shift from k_cpx_prep_cols, the chain
+// windows -> cmpPSWithPad -> 2w x 2w inverse -> findCPeak -> overSampleC
+// stays on chip:
 //   * the w x w forward FFTs run in registers, each length-w transform
 //     split over a lane pair (st_fftreg.h), with one LDS transpose;
 //   * the zero-padded 2w x 2w inverse is done as its four polyphase
@@ -773,52 +774,6 @@ __global__ void k_compose(KPar P, Pts S, float *offR, float *offA, int npts)
 //     samples.
 // Thread t: group g = t / 2w (patch g in the forward part, column parity q
 // = g in the inverse part), index r = (t / 2) mod w, lane half h = t & 1.
-template <int W>
-__device__ inline double group_sum(double v, double *sh)
-{
-    constexpr int G = 2 * W;
-#pragma unroll
-    for (int o = 1; o < (G < 64 ? G : 64); o <<= 1)
-        v += __shfl_xor(v, o);
-    if constexpr (G <= 64) {
-        return v;
-    } else {
-        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        __syncthreads();
-        if (lane == 0)
-            sh[wv] = v;
-        __syncthreads();
-        const int g = threadIdx.x / G;
-        const double r = sh[2 * g] + sh[2 * g + 1];
-        __syncthreads();
-        return r;
-    }
-}
-
-// Sum over the whole workgroup (4w threads).
-template <int W>
-__device__ inline double wg_sum(double v, double *sh)
-{
-    constexpr int NTH = 4 * W;
-#pragma unroll
-    for (int o = 1; o < (NTH < 64 ? NTH : 64); o <<= 1)
-        v += __shfl_xor(v, o);
-    if constexpr (NTH <= 64) {
-        return v;
-    } else {
-        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        __syncthreads();
-        if (lane == 0)
-            sh[wv] = v;
-        __syncthreads();
-        double r = 0;
-        for (int k = 0; k < NTH / 64; k++)
-            r += sh[k];
-        __syncthreads();
-        return r;
-    }
-}
-
 struct Top2 {
     float v1;
     int r1;
@@ -867,138 +822,122 @@ __device__ inline Top2 wg_top2(Top2 a, Top2 *sh)
     }
 }
 
-// Per-patch statistics of getPatches (speckleTrack.c:1194-1330): power,
-// zero-sample count, rejection and the Doppler estimate (estDopCarrier1);
-// one workgroup of 4w threads per patch pair.
+// Per-patch statistics of getPatches (speckleTrack.c:1194-1330) with one
+// lane per window column: 64 / W patch pairs per wave, each lane walks its
+// column down both windows (coalesced row reads), so every sum is a lane
+// sum followed by a segmented wave reduction -- no LDS, no barriers.  The
+// Doppler lag-1 sum of estDopCarrier1 (dopplerEstimation.c:42-88) is
+// expanded around the patch mean:
+//   sum (x1 - m) conj(x0 - m) = sum x1 conj(x0) - conj(m) sum x1
+//                               - m sum conj(x0) + (W - 1) W |m|^2
+// with the first / last row sums taken out of the totals.
 template <int W, int STYPE>
-__global__ void __launch_bounds__(4 * W) k_cpx_prep(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
-                                                    const int *list, int count, const CpxBatch *__restrict__ Cp)
+__global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
+                                                      const int *list, int count, const CpxBatch *__restrict__ Cp)
 {
     const KPar &P = *Pp;
     const Pts &S = *Sp;
     const CpxBatch &C = *Cp;
-    constexpr int H = W / 2, LD = W + 1;
-    extern __shared__ float2 lds[];  // [2][W][LD]
-    __shared__ double shd[16];
-    __shared__ unsigned shmask[2][2 * W];
-    __shared__ double shang[2][4];
-    __shared__ int shflag;
+    constexpr int PPW = 64 / W;
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
     const float thr = isf ? 1e-15f : 1e-9f;
     const float thrAll = thr * W * W;
-
-    const int it = blockIdx.x;
-    if (it >= count)
-        return;
-    const int t = threadIdx.x;
-    const int g = t / (2 * W);
-    const int r = (t >> 1) % W;
-    const int h = t & 1;
-    const int pt = list[it];
-    const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
-    // ---------------- load column r of patch g: samples a = 2m + h
-    float2 x[H];
-    const void *img = g ? P.im.img2 : P.im.img1;
-    const int nr = g ? P.im.nr2 : P.im.nr1;
-    const int ra = (g ? r2 : r1) + r, aa = (g ? a2 : a1) + h;
-    unsigned low = 0;
-    double psum = 0, mre = 0, mim = 0;
-    const bool useH = (g == 1) && P.hanning;
-#pragma unroll
-    for (int m = 0; m < H; m++) {
-        const size_t o = (size_t)(aa + 2 * m) * nr + ra;
-        if constexpr (isf) {
-            x[m] = ((const float2 *)img)[o];
-        } else {
-            const short2 q = ((const short2 *)img)[o];
-            x[m] = make_float2((float)q.x, (float)q.y);
-        }
-    }
-#pragma unroll
-    for (int m = 0; m < H; m++) {
-#pragma clang fp contract(off)
-        float2 v = x[m];
-        if constexpr (isf) {
-            v.x *= 1000;
-            v.y *= 1000;
-        }
-        const float hw = P.hann[(2 * m + h) * W + r];
-        v.x = useH ? v.x * hw : v.x;
-        v.y = useH ? v.y * hw : v.y;
-        x[m] = v;
-        const float pw = v.x * v.x + v.y * v.y;
-        low |= (pw < thr ? 1u : 0u) << m;
-        psum += pw;
-        mre += v.x;
-        mim += v.y;
-    }
-    // the scaled patches also go to LDS: the Doppler sums and the
-    // baseline ramp run over them with rolled loops (unrolled over the
-    // register copies they would dominate the register allocation)
-    float2 *R = lds + (size_t)g * W * LD;
-#pragma unroll
-    for (int m = 0; m < H; m++)
-        R[(2 * m + h) * LD + r] = x[m];
-    shmask[g][2 * r + h] = low;
-    psum = group_sum<W>(psum, shd);
-    mre = group_sum<W>(mre, shd);
-    mim = group_sum<W>(mim, shd);
-    __syncthreads();
+    const int lane = threadIdx.x;
+    const int it = blockIdx.x * PPW + lane / W, c = lane % W;
+    const bool act = it < count;
+    double ps[2] = {0, 0}, T[2][2] = {{0, 0}, {0, 0}}, Lg[2][2] = {{0, 0}, {0, 0}};
+    double F0[2][2] = {{0, 0}, {0, 0}}, FL[2][2] = {{0, 0}, {0, 0}};
     int nz = 0;
-    if (g == 0)
-        nz = __popc(shmask[0][2 * r + h] | shmask[1][2 * r + h]);
-    const double nzd = wg_sum<W>((double)nz, shd);
-    if (t == 0)
-        shang[0][0] = shang[1][0] = 0;
-    __syncthreads();
-    if ((t & (2 * W - 1)) == 0)
-        shang[g][0] = psum;
-    __syncthreads();
-    const double p1s = shang[0][0], p2s = shang[1][0];
-    if (nzd > 20.5 || p1s < thrAll || p2s < thrAll) {
-        if (t == 0)
-            C.have[it] = 0;
+    if (act) {
+        const int pt = list[it];
+        const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
+        const bool useH = P.hanning != 0;
+        float2 prev[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+#pragma unroll 4
+        for (int a = 0; a < W; a++) {
+#pragma clang fp contract(off)
+            float pw[2];
+#pragma unroll
+            for (int g = 0; g < 2; g++) {
+                const void *img = g ? P.im.img2 : P.im.img1;
+                const int nr = g ? P.im.nr2 : P.im.nr1;
+                const size_t o = (size_t)((g ? a2 : a1) + a) * nr + (g ? r2 : r1) + c;
+                float2 x;
+                if constexpr (isf) {
+                    x = ((const float2 *)img)[o];
+                    x.x *= 1000;
+                    x.y *= 1000;
+                } else {
+                    const short2 q = ((const short2 *)img)[o];
+                    x = make_float2((float)q.x, (float)q.y);
+                }
+                if (g == 1 && useH) {
+                    const float hw = P.hann[a * W + c];
+                    x.x *= hw;
+                    x.y *= hw;
+                }
+                pw[g] = x.x * x.x + x.y * x.y;
+                ps[g] += pw[g];
+                T[g][0] += x.x;
+                T[g][1] += x.y;
+                if (a == 0) {
+                    F0[g][0] = x.x;
+                    F0[g][1] = x.y;
+                } else {
+                    // x conj(prev): re = x.re p.re + x.im p.im, im = x.im p.re - x.re p.im
+                    Lg[g][0] += (double)x.x * prev[g].x + (double)x.y * prev[g].y;
+                    Lg[g][1] += (double)x.y * prev[g].x - (double)x.x * prev[g].y;
+                }
+                if (a == W - 1) {
+                    FL[g][0] = x.x;
+                    FL[g][1] = x.y;
+                }
+                prev[g] = x;
+            }
+            nz += (pw[0] < thr || pw[1] < thr) ? 1 : 0;
+        }
+    }
+    // segmented reduction over the W lanes of each patch pair
+#pragma unroll
+    for (int o = 1; o < W; o <<= 1) {
+        for (int g = 0; g < 2; g++) {
+            ps[g] += __shfl_xor(ps[g], o);
+            for (int k = 0; k < 2; k++) {
+                T[g][k] += __shfl_xor(T[g][k], o);
+                Lg[g][k] += __shfl_xor(Lg[g][k], o);
+                F0[g][k] += __shfl_xor(F0[g][k], o);
+                FL[g][k] += __shfl_xor(FL[g][k], o);
+            }
+        }
+        nz += __shfl_xor(nz, o);
+    }
+    if (!act || c != 0)
+        return;
+    if (nz > 20 || ps[0] < thrAll || ps[1] < thrAll) {
+        C.have[it] = 0;
         return;
     }
-    // ---------------- Doppler (estDopCarrier1): lag-1 azimuth products
-    {
-        const double cnt = (double)W * (double)W;
-        const double mr = mre / cnt, mi = mim / cnt;
-        double zr = 0, zi = 0;
-#pragma unroll 1
-        for (int e = t & (2 * W - 1); e < (W - 1) * W; e += 2 * W) {
-            const int a = e / W, c = e - (e / W) * W;
-            const float2 p0 = R[a * LD + c], p1 = R[(a + 1) * LD + c];
-            const double ar = (double)p1.x - mr, ai = (double)p1.y - mi;
-            const double br = (double)p0.x - mr, bi = (double)p0.y - mi;
-            zr += ar * br + ai * bi;
-            zi += ai * br - ar * bi;
-        }
-        zr = group_sum<W>(zr, shd);
-        zi = group_sum<W>(zi, shd);
-        if ((t & (2 * W - 1)) == 0) {
-            shang[g][1] = zr;
-            shang[g][2] = zi;
-        }
-        __syncthreads();
-        if (t == 0) {
-            const double z0r = shang[0][1], z0i = shang[0][2], z1r = shang[1][1], z1i = shang[1][2];
-            const double ang1 = (z0r == 0.0 && z0i == 0.0) ? NAN : atan2(z0i, z0r);
-            const double ang2 = (z1r == 0.0 && z1i == 0.0) ? NAN : atan2(z1i, z1r);
-            const double v = (double)(W / 2) * (ang1 + ang2) * .5 / 3.14159265358979323846;
-            int shift = isnan(v) ? 0 : ((v >= 0.0) ? (int)(v + 0.5) : (int)(v - 0.5));
-            if (shift < 0)
-                shift += W;
-            shflag = (int)((float)shift * (float)P.wA / (float)W);
-        }
-        __syncthreads();
+    const double cnt = (double)W * (double)W;
+    double ang[2];
+    for (int g = 0; g < 2; g++) {
+        const double mr = T[g][0] / cnt, mi = T[g][1] / cnt;
+        // sums over the lag pairs: x1 over rows 1..W-1, x0 over rows 0..W-2
+        const double s1r = T[g][0] - F0[g][0], s1i = T[g][1] - F0[g][1];
+        const double s0r = T[g][0] - FL[g][0], s0i = T[g][1] - FL[g][1];
+        // - conj(m) s1 - m conj(s0) + (W - 1) W |m|^2
+        const double zr = Lg[g][0] - (mr * s1r + mi * s1i) - (mr * s0r + mi * s0i) +
+                          (double)(W - 1) * (double)W * (mr * mr + mi * mi);
+        const double zi = Lg[g][1] - (mr * s1i - mi * s1r) - (mi * s0r - mr * s0i);
+        ang[g] = (zr == 0.0 && zi == 0.0) ? NAN : atan2(zi, zr);
     }
-    if (t == 0) {
-        C.have[it] = 1;
-        C.az[it] = shflag;
-        C.p[2 * it] = p1s / (double)(W * W);
-        C.p[2 * it + 1] = p2s / (double)(W * W);
-    }
+    const double v = (double)(W / 2) * (ang[0] + ang[1]) * .5 / 3.14159265358979323846;
+    int shift = isnan(v) ? 0 : ((v >= 0.0) ? (int)(v + 0.5) : (int)(v - 0.5));
+    if (shift < 0)
+        shift += W;
+    C.have[it] = 1;
+    C.az[it] = (int)((float)shift * (float)P.wA / (float)W);
+    C.p[2 * it] = ps[0] / cnt;
+    C.p[2 * it + 1] = ps[1] / cnt;
 }
 
 // Lanes per transform of the fused kernel.  Lane pairs at every window
@@ -1504,7 +1443,8 @@ void launch_cpx_fused_t(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, co
     const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
     const int prepSlot = ctx->kernel_slot("k_cpx_prep");
     ctx->time_begin(prepSlot);
-    hipLaunchKernelGGL((k_cpx_prep<W, STYPE>), dim3(count), dim3(4 * W), lds, ctx->stream, dP, dS, list, count, dC);
+    hipLaunchKernelGGL((k_cpx_prep_cols<W, STYPE>), dim3((count + 64 / W - 1) / (64 / W)), dim3(64), 0, ctx->stream, dP,
+                       dS, list, count, dC);
     ctx->time_end(prepSlot);
     if (base)
         launch_fused_kernel<W, STYPE, 1>(ctx, slot, dP, dS, list, count, dC, lds);
@@ -1664,11 +1604,12 @@ void launch_cpx_mfma(st_ctx *ctx, int slot, const KPar &P, const Pts &S, const i
     const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
     const int prepSlot = ctx->kernel_slot("k_cpx_prep");
     ctx->time_begin(prepSlot);
+    const unsigned pb = (unsigned)((count + 64 / W - 1) / (64 / W));
     if (P.im.stype == ST_SAMPLE_C64)
-        hipLaunchKernelGGL((k_cpx_prep<W, ST_SAMPLE_C64>), dim3(count), dim3(4 * W), lds, ctx->stream, dP, dS, list,
+        hipLaunchKernelGGL((k_cpx_prep_cols<W, ST_SAMPLE_C64>), dim3(pb), dim3(64), 0, ctx->stream, dP, dS, list,
                            count, dC);
     else
-        hipLaunchKernelGGL((k_cpx_prep<W, ST_SAMPLE_CI16>), dim3(count), dim3(4 * W), lds, ctx->stream, dP, dS, list,
+        hipLaunchKernelGGL((k_cpx_prep_cols<W, ST_SAMPLE_CI16>), dim3(pb), dim3(64), 0, ctx->stream, dP, dS, list,
                            count, dC);
     ctx->time_end(prepSlot);
     if (P.im.stype == ST_SAMPLE_C64)
