@@ -948,6 +948,16 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
 template <int W>
 struct FusedLanes {
     static constexpr int L = 2;
+    // LDS transpose buffers [W][LD] float2: LD = 2 (mod 32) spreads the 16
+    // rows x 2 lanes of a half-wave row access over all banks; for w >= 32
+    // the rows of the second half start 16 elements later so that the
+    // column access of a lane pair (rows k and w/2 + k, 256-byte aligned
+    // apart) falls into the other half of the banks (at w = 16 the pair is
+    // already 128 bytes apart).
+    static constexpr int LD = W <= 16 ? 34 : W + 2;
+    static constexpr int SH = W >= 32 ? 16 : 0;
+    static constexpr int BUF = W * LD + SH;
+    __device__ static inline int at(int row, int col) { return row * LD + col + (row >= W / 2 ? SH : 0); }
 };
 
 // BASE = 1 adds the baseline phase ramp on patch 1 (a separate
@@ -965,9 +975,10 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
     const Pts &S = *Sp;
     const CpxBatch &C = *Cp;
     using namespace ::fftreg;
-    constexpr int L = FusedLanes<W>::L, H = W / L, NTH = 2 * L * W, LD = W + 1, WW = 2 * W;
+    constexpr int L = FusedLanes<W>::L, H = W / L, NTH = 2 * L * W, WW = 2 * W, BUF = FusedLanes<W>::BUF;
+    using FL = FusedLanes<W>;
     using LF = LaneFFT<W, L>;
-    extern __shared__ float2 lds[];  // [2][W][LD]
+    extern __shared__ float2 lds[];  // [2][BUF]
     __shared__ double shd[32];
     __shared__ int shr[16];
     __shared__ Top2 sht[16];
@@ -1020,10 +1031,10 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         }
         // ---------------- baseline phase ramp on patch 1 (in LDS)
         if constexpr (BASE) {
-            float2 *R = lds + (size_t)g * W * LD;
+            float2 *R = lds + (size_t)g * BUF;
 #pragma unroll
             for (int m = 0; m < H; m++)
-                R[(L * m + h) * LD + r] = x[m];
+                R[FL::at(L * m + h, r)] = x[m];
             __syncthreads();
             const double x0 = (a1 - P.wA / 2 - (P.nSlpA1 / 2)) / (double)P.nSlpA1;
             const double bn = P.Bn + x0 * P.dBn + x0 * x0 * P.dBnQ;
@@ -1039,40 +1050,40 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
                 double ph = -bn * P.sinT[j0 + j] - bp * P.cosT[j0 + j] + (0.5 * bsq * P.overR[j0 + j]);
                 ph *= ptod;
                 const double pr = cos(ph), pim = sin(ph);
-                const float2 v = R0[i * LD + j];
+                const float2 v = R0[FL::at(i, j)];
                 const double cr = v.x, ci = v.y;
-                R0[i * LD + j] = make_float2((float)(cr * pr - ci * pim), (float)(cr * pim + ci * pr));
+                R0[FL::at(i, j)] = make_float2((float)(cr * pr - ci * pim), (float)(cr * pim + ci * pr));
             }
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < H; m++)
-                x[m] = R[(L * m + h) * LD + r];
+                x[m] = R[FL::at(L * m + h, r)];
             __syncthreads();
         }
         // ---------------- forward FFT along azimuth (column r), then LDS
         LF::fwd(x, h);
-        float2 *F = lds + (size_t)g * W * LD;
+        float2 *F = lds + (size_t)g * BUF;
 #pragma unroll
         for (int k = 0; k < H; k++)
-            F[(blk * H + k) * LD + r] = x[k];
+            F[FL::at(blk * H + k, r)] = x[k];
         __syncthreads();
         // ---------------- forward FFT along range (row r)
 #pragma unroll
         for (int m = 0; m < H; m++)
-            x[m] = F[r * LD + L * m + h];
+            x[m] = F[FL::at(r, L * m + h)];
         __syncthreads();
         LF::fwd(x, h);
 #pragma unroll
         for (int k = 0; k < H; k++)
-            F[r * LD + blk * H + k] = x[k];
+            F[FL::at(r, blk * H + k)] = x[k];
         __syncthreads();
         // ---------------- cross spectrum X = F1 conj(F2), row r, half h
         {
-            const float2 *Fo = lds + (size_t)(1 - g) * W * LD;
+            const float2 *Fo = lds + (size_t)(1 - g) * BUF;
 #pragma unroll
             for (int k = 0; k < H; k++) {
 #pragma clang fp contract(off)
-                const float2 o = Fo[r * LD + blk * H + k];
+                const float2 o = Fo[FL::at(r, blk * H + k)];
                 x[k] = g == 0 ? cmulconj(x[k], o) : cmulconj(o, x[k]);
             }
         }
@@ -1080,7 +1091,7 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         // ---------------- polyphase inverse: components (pp, q = g)
         const int krow = ((r - az) % W + W) % W;
         const int ks = krow < W / 2 ? krow : krow - W;
-        float2 *Z = lds + (size_t)g * W * LD;
+        float2 *Z = lds + (size_t)g * BUF;
         Top2 best{0.0f, 0x7fffffff, 0.0f};
         // exp(i pi (ks pp + ls q) / w) = row factor (per lane, per pp) x
         // column factor (per element, q = g only): the column factor is
@@ -1111,12 +1122,12 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
             // y[n'] = Z[krow][L n' + h]
 #pragma unroll
             for (int n = 0; n < H; n++)
-                Z[krow * LD + L * n + h] = y[n];
+                Z[FL::at(krow, L * n + h)] = y[n];
             __syncthreads();
             // columns: lane h holds k in [blk H, blk H + H)
 #pragma unroll
             for (int k = 0; k < H; k++)
-                y[k] = Z[(blk * H + k) * LD + r];
+                y[k] = Z[FL::at(blk * H + k, r)];
             __syncthreads();
             LF::inv(y, h);
             // y[j] = C_pp,g[L j + h][r]  ->  surface row n = 2 (L j + h) + pp, col m = 2 r + g
@@ -1440,7 +1451,7 @@ template <int W, int STYPE>
 void launch_cpx_fused_t(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
                         const CpxBatch *dC, bool base)
 {
-    const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
+    const size_t lds = std::max<size_t>((size_t)2 * FusedLanes<W>::BUF * sizeof(float2), 8192);
     const int prepSlot = ctx->kernel_slot("k_cpx_prep");
     ctx->time_begin(prepSlot);
     hipLaunchKernelGGL((k_cpx_prep_cols<W, STYPE>), dim3((count + 64 / W - 1) / (64 / W)), dim3(64), 0, ctx->stream, dP,
@@ -1455,7 +1466,7 @@ void launch_cpx_fused_t(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, co
 template <int W, int OCC>
 int fused_per_cu_o(int stype)
 {
-    const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
+    const size_t lds = std::max<size_t>((size_t)2 * FusedLanes<W>::BUF * sizeof(float2), 8192);
     constexpr int NTH = 2 * FusedLanes<W>::L * W;
     int perCU = 0;
     if (stype == ST_SAMPLE_C64)
@@ -1601,7 +1612,6 @@ void launch_cpx_mfma(st_ctx *ctx, int slot, const KPar &P, const Pts &S, const i
     ST_HIP(hipMemcpyAsync(dP, &P, sizeof(KPar), hipMemcpyHostToDevice, ctx->stream));
     ST_HIP(hipMemcpyAsync(dS, &S, sizeof(Pts), hipMemcpyHostToDevice, ctx->stream));
     ST_HIP(hipMemcpyAsync(dC, &C, sizeof(CpxBatch), hipMemcpyHostToDevice, ctx->stream));
-    const size_t lds = std::max<size_t>((size_t)2 * W * (W + 1) * sizeof(float2), 8192);
     const int prepSlot = ctx->kernel_slot("k_cpx_prep");
     ctx->time_begin(prepSlot);
     const unsigned pb = (unsigned)((count + 64 / W - 1) / (64 / W));
