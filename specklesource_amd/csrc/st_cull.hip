@@ -42,7 +42,7 @@ struct CullDev {
     float *R0, *A0;       // state at the start of a cullSTData pass
     uint8_t *cPrev, *cNew; // cull decisions of the pass
     int *counters;        // [0] changed, [1] nCulled, [2] max root, ...
-    int *parent, *bbox;   // islands
+    int *parent, *root, *bbox;   // islands
 };
 
 // Pre-pass of cullSTData (cullSTData.c:34-68).
@@ -374,14 +374,25 @@ __global__ void k_cull_smooth(CullDev D, int sR, int sA)
 // leaves alone (the loops stop at maxLabel - 1).
 __device__ inline bool isl_valid(const CullDev &D, int c) { return D.offRS[c] >= (float)(-ST_LARGEINT + 1000); }
 
+// Parent links only ever point to a smaller index, so every value a thread
+// reads for par[x] is an ancestor of x, and path halving (par[x] = par[par[x]])
+// may race with other finds and with the linking CAS without breaking a tree:
+// it writes only to nodes that are no longer roots.  Loads and stores are
+// agent-scope atomics so a retried find sees links made by other waves.
+__device__ inline int uf_ld(const int *par, int x) { return __hip_atomic_load(par + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
 __device__ inline int uf_find(int *par, int x)
 {
-    int p = par[x];
-    while (p != x) {
-        x = p;
-        p = par[x];
+    while (true) {
+        const int p = uf_ld(par, x);
+        if (p == x)
+            return x;
+        const int g = uf_ld(par, p);
+        if (g == p)
+            return p;
+        __hip_atomic_store(par + x, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = g;
     }
-    return x;
 }
 
 __device__ inline void uf_unite(int *par, int a, int b)
@@ -401,39 +412,76 @@ __device__ inline void uf_unite(int *par, int a, int b)
     }
 }
 
+// Each valid cell starts linked to its left neighbour when that is valid,
+// else to the one above, so a row run is one chain before any merge.
 __global__ void k_isl_init(CullDev D)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= D.nR * D.nA)
         return;
-    D.parent[c] = c;
+    int p = c;
+    if (isl_valid(D, c)) {
+        const int j = c % D.nR;
+        if (j > 0 && isl_valid(D, c - 1))
+            p = c - 1;
+        else if (c >= D.nR && isl_valid(D, c - D.nR))
+            p = c - D.nR;
+    }
+    D.parent[c] = p;
 }
 
+// The only edge the initial links leave out: up, when left was taken.
 __global__ void k_isl_merge(CullDev D)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= D.nR * D.nA || !isl_valid(D, c))
+    if (c >= D.nR * D.nA || c < D.nR || !isl_valid(D, c))
         return;
     const int j = c % D.nR;
-    if (j + 1 < D.nR && isl_valid(D, c + 1))
-        uf_unite(D.parent, c, c + 1);
-    if (c + D.nR < D.nR * D.nA && isl_valid(D, c + D.nR))
-        uf_unite(D.parent, c, c + D.nR);
+    if (j > 0 && isl_valid(D, c - 1) && isl_valid(D, c - D.nR))
+        uf_unite(D.parent, c, c - D.nR);
 }
 
+// Roots and bounding boxes.  Most cells of a wave share a root (one large
+// component covers most of a grid), so the min/max are first reduced over
+// the lanes with the same root and one lane per root issues the atomics.
 __global__ void k_isl_bbox(CullDev D)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= D.nR * D.nA || !isl_valid(D, c))
-        return;
-    const int r = uf_find(D.parent, c);
-    D.parent[c] = r;
-    const int i = c / D.nR, j = c - (c / D.nR) * D.nR;
-    atomicMin(&D.bbox[4 * r + 0], j);
-    atomicMax(&D.bbox[4 * r + 1], j);
-    atomicMin(&D.bbox[4 * r + 2], i);
-    atomicMax(&D.bbox[4 * r + 3], i);
-    atomicMax(&D.counters[2], r);
+    const bool v = c < D.nR * D.nA && isl_valid(D, c);
+    int r = -1, i = 0, j = 0;
+    if (v) {
+        r = uf_find(D.parent, c);
+        D.root[c] = r;
+        i = c / D.nR;
+        j = c - i * D.nR;
+    }
+    int rmax = r;
+    for (int o = 32; o > 0; o >>= 1)
+        rmax = max(rmax, __shfl_xor(rmax, o));
+    const int lane = threadIdx.x & 63;
+    if (lane == 0 && rmax >= 0)
+        atomicMax(&D.counters[2], rmax);
+    unsigned long long pending = __ballot(v);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int rl = __shfl(r, leader);
+        const bool mine = v && r == rl;
+        int jmin = mine ? j : 0x7fffffff, jmax = mine ? j : -0x7fffffff;
+        int imin = mine ? i : 0x7fffffff, imax = mine ? i : -0x7fffffff;
+        for (int o = 32; o > 0; o >>= 1) {
+            jmin = min(jmin, __shfl_xor(jmin, o));
+            jmax = max(jmax, __shfl_xor(jmax, o));
+            imin = min(imin, __shfl_xor(imin, o));
+            imax = max(imax, __shfl_xor(imax, o));
+        }
+        if (lane == leader) {
+            atomicMin(&D.bbox[4 * rl + 0], jmin);
+            atomicMax(&D.bbox[4 * rl + 1], jmax);
+            atomicMin(&D.bbox[4 * rl + 2], imin);
+            atomicMax(&D.bbox[4 * rl + 3], imax);
+        }
+        pending &= ~__ballot(mine);
+    }
 }
 
 __global__ void k_isl_bbox_init(CullDev D)
@@ -452,7 +500,7 @@ __global__ void k_isl_remove(CullDev D, int thresh)
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= D.nR * D.nA || !isl_valid(D, c))
         return;
-    const int r = D.parent[c];
+    const int r = D.root[c];
     if (r == D.counters[2])
         return;
     const int *bb = D.bbox + 4 * r;
@@ -487,7 +535,7 @@ struct CullRun {
     {
         ST_REQUIRE(cp && cp->nR > 0 && cp->nA > 0 && cp->offR && cp->offA, ST_ERR_ARG, "cull grids");
         n = cp->nR * cp->nA;
-        ctx->arena_reserve((size_t)n * 64 + 65536);
+        ctx->arena_reserve((size_t)n * 72 + 65536);
         Arena &ar = ctx->arena;
         D.nR = cp->nR;
         D.nA = cp->nA;
@@ -506,6 +554,7 @@ struct CullRun {
         D.cNew = ar.take<uint8_t>(n);
         D.counters = ar.take<int>(16);
         D.parent = ar.take<int>(n);
+        D.root = ar.take<int>(n);
         D.bbox = ar.take<int>(4 * (size_t)n);
         if (cp->mask) {
             uint8_t *m = ar.take<uint8_t>(n);
