@@ -2,7 +2,8 @@
 //
 // Rows of length n (n = 2^a 3^b 5^c 7^d <= 4096) are transformed in LDS by
 // a mixed-radix Stockham autosort (radices 8, 4, 2, 3, 5, 7), several rows
-// per workgroup.  2-D transforms are row passes plus tiled transposes.
+// per workgroup.  2-D transforms are a row pass plus a column pass that
+// transforms blocks of adjacent columns in LDS (no transposes).
 // Conventions follow FFTW 2, which the reference links: sign -1 is
 // exp(-2 pi i jk/n) ("forward"), +1 is exp(+2 pi i jk/n) ("backward"),
 // no normalisation.
@@ -31,8 +32,10 @@ class Engine {
     void rows(float2 *data, long nrows, int n, int sign, hipStream_t st);
     // out[b][c][r] = in[b][r][c] for batch b of rows x cols arrays.
     static void transpose(const float2 *in, float2 *out, int batch, int rows, int cols, hipStream_t st);
-    // In-place batched 2-D FFT of batch arrays na x nr (row-major); tmp must
-    // hold batch*na*nr elements.
+    // In-place column transforms (length na) of batch arrays na x nr.
+    void cols(float2 *data, int batch, int na, int nr, int sign, hipStream_t st);
+    // In-place batched 2-D FFT of batch arrays na x nr (row-major); tmp is
+    // not used (kept so callers need not change).
     void fft2d(float2 *data, float2 *tmp, int batch, int na, int nr, int sign, hipStream_t st);
 
   private:

@@ -3,7 +3,7 @@
 
 #include <vector>
 
-#include "st_fft.h"
+#include "st_fftlds.h"
 
 namespace stfft {
 
@@ -11,85 +11,7 @@ namespace {
 
 constexpr int kPointsPerBlock = 1024;  // LDS: 2 x 1024 x 8 B per block
 constexpr int kThreads = 256;
-
-__device__ inline float2 mul_si(float2 v, int sign)
-{
-    // v * (sign * i)
-    return sign < 0 ? make_float2(v.y, -v.x) : make_float2(-v.y, v.x);
-}
-
-__device__ inline void dft2(float2 *v)
-{
-    float2 a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-}
-
-__device__ inline void dft4(float2 &a, float2 &b, float2 &c, float2 &d, int sign)
-{
-    float2 s0 = cadd(a, c), d0 = csub(a, c), s1 = cadd(b, d), d1 = mul_si(csub(b, d), sign);
-    a = cadd(s0, s1);
-    b = cadd(d0, d1);
-    c = csub(s0, s1);
-    d = csub(d0, d1);
-}
-
-__device__ inline void dft8(float2 *v, int sign)
-{
-    const float h = 0.70710678118654752440f;
-    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
-    dft4(e0, e1, e2, e3, sign);
-    dft4(o0, o1, o2, o3, sign);
-    // W8^1 = (1 + s i)/sqrt2, W8^2 = s i, W8^3 = (-1 + s i)/sqrt2
-    float2 w1 = make_float2(h, sign * h), w3 = make_float2(-h, sign * h);
-    o1 = cmul(o1, w1);
-    o2 = mul_si(o2, sign);
-    o3 = cmul(o3, w3);
-    v[0] = cadd(e0, o0);
-    v[4] = csub(e0, o0);
-    v[1] = cadd(e1, o1);
-    v[5] = csub(e1, o1);
-    v[2] = cadd(e2, o2);
-    v[6] = csub(e2, o2);
-    v[3] = cadd(e3, o3);
-    v[7] = csub(e3, o3);
-}
-
-__device__ inline void dft3(float2 *v, int sign)
-{
-    const float c = -0.5f, s = 0.86602540378443864676f * sign;
-    float2 a = v[0], b = v[1], d = v[2];
-    float2 t1 = cadd(b, d), t2 = csub(b, d);
-    float2 m = make_float2(a.x + c * t1.x, a.y + c * t1.y);
-    float2 r = make_float2(-s * t2.y, s * t2.x);
-    v[0] = cadd(a, t1);
-    v[1] = cadd(m, r);
-    v[2] = csub(m, r);
-}
-
-template <int P>
-__device__ inline void dft_generic(float2 *v, int sign)
-{
-    float2 o[P];
-#pragma unroll
-    for (int q = 0; q < P; q++) {
-        float sr = 0, si = 0;
-#pragma unroll
-        for (int r = 0; r < P; r++) {
-            const int e = (q * r) % P;
-            float cs, sn;
-            sincospif(2.0f * (float)e / (float)P, &sn, &cs);
-            sn *= sign;
-            sr += v[r].x * cs - v[r].y * sn;
-            si += v[r].x * sn + v[r].y * cs;
-        }
-        o[q] = make_float2(sr, si);
-    }
-#pragma unroll
-    for (int q = 0; q < P; q++)
-        v[q] = o[q];
-}
+constexpr int kColElems = 4096;  // per block: LDS kColElems x 8 B (+ one pad element per column)
 
 template <int P>
 __device__ inline void stage_p(const float2 *A, float2 *B, int n, int ns, const float2 *tw, int sign, int t)
@@ -104,16 +26,7 @@ __device__ inline void stage_p(const float2 *A, float2 *B, int n, int ns, const 
 #pragma unroll
     for (int r = 1; r < P; r++)
         v[r] = cmul(src[j + r * stride], tw[k * P + r]);
-    if constexpr (P == 2)
-        dft2(v);
-    else if constexpr (P == 4)
-        dft4(v[0], v[1], v[2], v[3], sign);
-    else if constexpr (P == 8)
-        dft8(v, sign);
-    else if constexpr (P == 3)
-        dft3(v, sign);
-    else
-        dft_generic<P>(v, sign);
+    dft_any<P>(v, sign);
     const int base = (j / ns) * ns * P + k;
 #pragma unroll
     for (int q = 0; q < P; q++)
@@ -174,6 +87,49 @@ __global__ void k_transpose(const float2 *in, float2 *out, int rows, int cols)
     }
 }
 
+// Column transforms of a batch of na x nr row-major arrays: each block
+// loads C adjacent columns (rows of C * 8 contiguous bytes), transforms them
+// in LDS as C sequences of length na, and writes them back in place.  All
+// of a thread's loads are issued before the first LDS store, and blocks are
+// numbered so that neighbouring column blocks (which share cache lines) run
+// on the same XCD: dispatch deals consecutive workgroups round-robin over
+// the 8 XCDs.
+__global__ void __launch_bounds__(kThreads) k_fft_cols(float2 *data, int na, int nr, int lc, PlanArgs P, int sign)
+{
+    extern __shared__ float2 lds[];
+    constexpr int IT = kColElems / kThreads;
+    const int C = 1 << lc;
+    const int nbx = gridDim.x, total = nbx * gridDim.y, flat = blockIdx.y * nbx + blockIdx.x;
+    const int per = total / 8;
+    const int L = flat < per * 8 ? (flat % 8) * per + flat / 8 : flat;
+    const int z = L / nbx, cb = L - (L / nbx) * nbx;
+    const int pitch = na + 1;
+    const int c0 = cb * C;
+    const int nc = nr - c0 < C ? nr - c0 : C;
+    const int cnt = na * C;
+    float2 *base = data + (size_t)z * na * nr + c0;
+    float2 v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int t = threadIdx.x + it * kThreads, r = t >> lc, c = t & (C - 1);
+        v[it] = t < cnt && c < nc ? base[(size_t)r * nr + c] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int t = threadIdx.x + it * kThreads, r = t >> lc, c = t & (C - 1);
+        if (t < cnt)
+            lds[c * pitch + r] = v[it];
+    }
+    __syncthreads();
+    lds_fft<kColElems, kThreads>(lds, pitch, C, P, sign);
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int t = threadIdx.x + it * kThreads, r = t >> lc, c = t & (C - 1);
+        if (t < cnt && c < nc)
+            base[(size_t)r * nr + c] = lds[c * pitch + r];
+    }
+}
+
 }  // namespace
 
 bool supported_size(int n)
@@ -205,8 +161,22 @@ const PlanArgs &Engine::plan(int n, int sign)
     P.n = n;
     int m = n, ns = 1, k = 0;
     std::vector<float2> tw;
+    int twos = 0;
+    for (int t = n; t % 2 == 0; t /= 2)
+        twos++;
     while (m > 1) {
-        int r = (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : 7;
+        // powers of two as radix 8, with 4 x 4 in place of 8 x 2 when the
+        // count of twos leaves one over (a radix-2 stage only for n = 2 * odd)
+        int r;
+        if (twos >= 3 && twos != 4)
+            r = 8;
+        else if (twos >= 2)
+            r = 4;
+        else if (twos == 1)
+            r = 2;
+        else
+            r = (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : 7;
+        twos -= r == 8 ? 3 : r == 4 ? 2 : r == 2 ? 1 : 0;
         ST_REQUIRE(k < kMaxStages, ST_ERR_UNSUPPORTED, "FFT plan too deep");
         P.radix[k] = r;
         P.ns[k] = ns;
@@ -249,12 +219,32 @@ void Engine::transpose(const float2 *in, float2 *out, int batch, int rows, int c
     ST_HIP(hipGetLastError());
 }
 
+int col_block_log2(int na)
+{
+    int lc = 0;
+    while ((2 << lc) * na <= kColElems && (2 << lc) <= 32)
+        lc++;
+    return lc;
+}
+
+void Engine::cols(float2 *data, int batch, int na, int nr, int sign, hipStream_t st)
+{
+    if (batch <= 0 || na <= 1)
+        return;
+    const PlanArgs &P = plan(na, sign);
+    ST_REQUIRE(na <= kColElems, ST_ERR_UNSUPPORTED, "column FFT length above the LDS block");
+    const int lc = col_block_log2(na), C = 1 << lc;
+    const size_t lds = (size_t)C * (na + 1) * sizeof(float2);
+    hipLaunchKernelGGL(k_fft_cols, dim3((unsigned)((nr + C - 1) / C), (unsigned)batch), dim3(kThreads), lds, st, data,
+                       na, nr, lc, P, sign);
+    ST_HIP(hipGetLastError());
+}
+
 void Engine::fft2d(float2 *data, float2 *tmp, int batch, int na, int nr, int sign, hipStream_t st)
 {
+    (void)tmp;
     rows(data, (long)batch * na, nr, sign, st);
-    transpose(data, tmp, batch, na, nr, st);
-    rows(tmp, (long)batch * nr, na, sign, st);
-    transpose(tmp, data, batch, nr, na, st);
+    cols(data, batch, na, nr, sign, st);
 }
 
 }  // namespace stfft

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "st_ctx.h"
+#include "st_fftlds.h"
 #include "st_track_common.h"
 
 using namespace sttrack;
@@ -65,7 +66,9 @@ struct RowBest {
 
 struct WBatch {
     const int4 *pos;  // r1, a1, r2, a2 of every listed patch
-    float2 *P;        // [2][B][wAa * wRa]
+    float2 *T;        // [2][B][wAa * W_R]: range-interpolated rows
+    double *dop;      // [2][B][nblk][4]: Doppler partial sums per row block
+    double *dopE;     // [2][B][4]: first- and last-row sums
     float2 *U;        // [2][B][W_A * W_R]; later X: [ceil(B/2)][W_A * W_R]
     float2 *Z;        // [B][W_A * W_R]: search + i chip
     double *V1, *V2;  // [B][nLagA][W_R]: vertical window sums of s, s^2
@@ -84,53 +87,228 @@ __device__ inline float2 sample0(const Images &im, bool second, int a, int r)
     return load_sample(second ? im.img2 : im.img1, im.stype, nr, a, r);
 }
 
-// getCorrPatchesFast (corrTrackFast.c:299-372): copy both windows (samples
-// outside an image read as zero) and estimate the Doppler shift.
-__global__ void __launch_bounds__(NT) k_w_gather(WPar p, int off, WBatch Bt, int B)
+constexpr int kRowElems = 4096;  // register staging and LDS of k_w_rowinterp (32 KB)
+constexpr int kColElems = 4096;  // same for k_w_colinterp (32 KB + one pad element per column)
+
+// getCorrPatchesFast + zeroPad (corrTrackFast.c:299-372, 738-761), range
+// half.  The reference transforms each window (wAa x wRa), moves the
+// spectrum into the corners of a 2wAa x 2wRa array (azimuth rows rotated by
+// the Doppler shift, Nyquist bins on the negative side) and inverse
+// transforms it.  Transform, pad and inverse are separable, so each block
+// takes R rows of one window (samples outside the image read as zero),
+// transforms them (wRa), pads each row spectrum to 2wRa and inverse
+// transforms it (2wRa); k_w_colinterp does the azimuth half.  The block also
+// leaves the Doppler partial sums of its rows: sum of x, sum of
+// x[i+1] conj(x[i]) over its rows i with a successor, and the first and last
+// window rows' sums.
+__global__ void __launch_bounds__(NT) k_w_rowinterp(WPar p, int off, WBatch Bt, int B, int R, stfft::PlanArgs Pf,
+                                                     stfft::PlanArgs Pi)
 {
+    extern __shared__ float2 ldsf[];
+    float2 *lds = ldsf;
     __shared__ double sh[NT / 64];
-    const int b = blockIdx.x;
+    const int z = blockIdx.y, img = z >= B ? 1 : 0, b = z - img * B;
     const int4 q = Bt.pos[off + b];
-    const int n = p.wAa * p.wRa;
-    float2 *P1 = Bt.P + (size_t)b * n, *P2 = Bt.P + ((size_t)B + b) * n;
-    for (int e = threadIdx.x; e < n; e += NT) {
-        const int i = e / p.wRa, j = e - (e / p.wRa) * p.wRa;
-        P1[e] = sample0(p.im, false, q.y + i, q.x + j);
-        P2[e] = sample0(p.im, true, q.w + i, q.z + j);
+    const int rr = img ? q.z : q.x, ra = img ? q.w : q.y;
+    const int i0 = blockIdx.x * R, nrow = imin(R, p.wAa - i0);
+    const int pitch = p.W_R;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // S, L, F0, FL (re, im)
+    {
+        constexpr int IT = kRowElems / NT;
+        const int cnt = nrow * p.wRa;
+        float2 x[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT, i = t / p.wRa, j = t - (t / p.wRa) * p.wRa;
+            x[it] = t < cnt ? sample0(p.im, img, ra + i0 + i, rr + j) : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT, i = t / p.wRa, j = t - (t / p.wRa) * p.wRa;
+            if (t < cnt) {
+                lds[i * pitch + j] = x[it];
+                s[0] += x[it].x;
+                s[1] += x[it].y;
+                if (i0 + i == 0) {
+                    s[4] += x[it].x;
+                    s[5] += x[it].y;
+                }
+                if (i0 + i == p.wAa - 1) {
+                    s[6] += x[it].x;
+                    s[7] += x[it].y;
+                }
+            }
+        }
     }
-    auto get = [&](int e, float2 &a, float2 &c) {
-        const int i = e / p.wRa, j = e - (e / p.wRa) * p.wRa;
-        a = sample0(p.im, false, q.y + i, q.x + j);
-        c = sample0(p.im, true, q.w + i, q.z + j);
-    };
-    int az = doppler_shift(get, p.wAa, p.wRa, p.wAa, p.wAa, sh);
-    if (az < 0 || az >= p.wAa)
-        az = 0;
-    if (threadIdx.x == 0)
-        Bt.az[b] = az;
+    __syncthreads();
+    const int nlag = imin(nrow, p.wAa - 1 - i0);
+    for (int t = threadIdx.x; t < nlag * p.wRa; t += NT) {
+        const int i = t / p.wRa, j = t - (t / p.wRa) * p.wRa;
+        const float2 x0 = lds[i * pitch + j];
+        const float2 x1 = i + 1 < nrow ? lds[(i + 1) * pitch + j] : sample0(p.im, img, ra + i0 + i + 1, rr + j);
+        s[2] += (double)x1.x * x0.x + (double)x1.y * x0.y;
+        s[3] += (double)x1.y * x0.x - (double)x1.x * x0.y;
+    }
+    for (int k = 0; k < 8; k++)
+        s[k] = block_sum_d<NT>(s[k], sh);
+    if (threadIdx.x == 0) {
+        double *d = Bt.dop + ((size_t)z * gridDim.x + blockIdx.x) * 4;
+        for (int k = 0; k < 4; k++)
+            d[k] = s[k];
+        if (i0 == 0) {
+            Bt.dopE[4 * z + 0] = s[4];
+            Bt.dopE[4 * z + 1] = s[5];
+        }
+        if (i0 + nrow == p.wAa) {
+            Bt.dopE[4 * z + 2] = s[6];
+            Bt.dopE[4 * z + 3] = s[7];
+        }
+    }
+    __syncthreads();
+    stfft::lds_fft<kRowElems, NT>(lds, pitch, nrow, Pf, -1);
+    // pad each row spectrum wRa -> 2wRa: bins 0 .. wRa/2-1 stay, the Nyquist
+    // bin and the negative frequencies move to the top, zeros between
+    {
+        constexpr int IT = kRowElems / NT;
+        const int hR = p.wRa / 2;
+        float2 v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            if (t < nrow * p.W_R) {
+                const int i = t / p.W_R, m = t - (t / p.W_R) * p.W_R;
+                const int l = m < hR ? m : (m >= p.W_R - hR ? m - p.wRa : -1);
+                v[it] = l >= 0 ? lds[i * pitch + l] : make_float2(0.f, 0.f);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            if (t < nrow * p.W_R)
+                lds[t] = v[it];
+        }
+        __syncthreads();
+    }
+    stfft::lds_fft<kRowElems, NT>(lds, pitch, nrow, Pi, +1);
+    float2 *Tz = Bt.T + ((size_t)z * p.wAa + i0) * p.W_R;
+    for (int t = threadIdx.x; t < nrow * p.W_R; t += NT)
+        Tz[t] = lds[t];
 }
 
-// zeroPad (corrTrackFast.c:738-761): spectrum into the corners of the
-// (2wa)^2 array, azimuth rows rotated by azShift, Nyquist on the negative
-// side.
-__global__ void k_w_pad(WPar p, WBatch Bt, int B)
+// The Doppler shift of each patch (getCorrPatchesFast, corrTrackFast.c:
+// 340-350, estDopCarrier1, dopplerEstimation.c:25-88) from the row-block
+// partial sums: sum (x[i+1] - m) conj(x[i] - m) over the lag pairs is
+// L - conj(m) (S - F0) - m conj(S - FL) + (wAa - 1) wRa |m|^2.
+__global__ void k_w_az(WPar p, WBatch Bt, int B, int nblk)
 {
-    const int z = blockIdx.y;  // 0 .. 2B-1
-    const int b = z % B;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    const int NN = p.W_A * p.W_R;
-    if (e >= NN)
-        return;
-    const int hA = p.wAa / 2, hR = p.wRa / 2;
-    const int n = e / p.W_R, m = e - (e / p.W_R) * p.W_R;
-    const int k = n < hA ? n : (n >= p.W_A - hA ? n - p.W_A : (1 << 20));
-    const int l = m < hR ? m : (m >= p.W_R - hR ? m - p.W_R : (1 << 20));
-    float2 v = make_float2(0.f, 0.f);
-    if (k != (1 << 20) && l != (1 << 20)) {
-        const int ka = (k + p.wAa + Bt.az[b]) % p.wAa, la = (l + p.wRa) % p.wRa;
-        v = Bt.P[(size_t)z * p.wAa * p.wRa + ka * p.wRa + la];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    double ang[2];
+    for (int img = 0; img < 2; img++) {
+        const int z = img * B + b;
+        double a[4] = {0, 0, 0, 0};
+        for (int k = lane; k < nblk; k += 64)
+            for (int e = 0; e < 4; e++)
+                a[e] += Bt.dop[((size_t)z * nblk + k) * 4 + e];
+        for (int o = 32; o > 0; o >>= 1)
+            for (int e = 0; e < 4; e++)
+                a[e] += __shfl_xor(a[e], o);
+        const double cnt = (double)p.wAa * (double)p.wRa;
+        const double mr = a[0] / cnt, mi = a[1] / cnt;
+        const double *E = Bt.dopE + 4 * z;
+        const double s1r = a[0] - E[0], s1i = a[1] - E[1];  // x over rows 1 .. wAa-1
+        const double s0r = a[0] - E[2], s0i = a[1] - E[3];  // x over rows 0 .. wAa-2
+        const double zr = a[2] - (mr * s1r + mi * s1i) - (mr * s0r + mi * s0i) +
+                          (double)(p.wAa - 1) * (double)p.wRa * (mr * mr + mi * mi);
+        const double zi = a[3] - (mr * s1i - mi * s1r) - (mi * s0r - mr * s0i);
+        ang[img] = (zr == 0.0 && zi == 0.0) ? NAN : atan2(zi, zr);
     }
-    Bt.U[(size_t)z * NN + e] = v;
+    if (lane != 0)
+        return;
+    const double v = (double)(p.wAa / 2) * (ang[0] + ang[1]) * .5 / 3.14159265358979323846;
+    int shift = isnan(v) ? 0 : ((v >= 0.0) ? (int)(v + 0.5) : (int)(v - 0.5));
+    if (shift < 0)
+        shift += p.wAa;
+    int az = (int)((float)shift * (float)p.wAa / (float)p.wAa);
+    if (az < 0 || az >= p.wAa)
+        az = 0;
+    Bt.az[b] = az;
+}
+
+// Azimuth half of the interpolation: C adjacent columns of the
+// range-interpolated rows of one window, transformed (wAa), rotated by the
+// Doppler shift and zero-padded to 2wAa (Nyquist on the negative side),
+// inverse transformed (2wAa).  Work items are laid out so that the column
+// blocks sharing cache lines go to the same XCD (dispatch is round-robin
+// over the 8 XCDs).
+__global__ void __launch_bounds__(NT) k_w_colinterp(WPar p, WBatch Bt, int B, int lc, stfft::PlanArgs Pf,
+                                                     stfft::PlanArgs Pi)
+{
+    const int C = 1 << lc;
+    extern __shared__ float2 ldsf[];
+    float2 *lds = ldsf;
+    const int nbx = gridDim.x, total = nbx * gridDim.y, flat = blockIdx.y * nbx + blockIdx.x;
+    const int per = total / 8;
+    const int L = flat < per * 8 ? (flat % 8) * per + flat / 8 : flat;
+    const int z = L / nbx, cb = L - (L / nbx) * nbx;
+    const int b = z >= B ? z - B : z;
+    const int c0 = cb * C, nc = imin(C, p.W_R - c0);
+    const int pitch = p.W_A + 1;
+    const float2 *Tz = Bt.T + (size_t)z * p.wAa * p.W_R + c0;
+    {
+        constexpr int IT = kColElems / NT;
+        const int cnt = p.wAa << lc;
+        float2 x[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT, i = t >> lc, c = t & (C - 1);
+            x[it] = t < cnt && c < nc ? Tz[(size_t)i * p.W_R + c] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT, i = t >> lc, c = t & (C - 1);
+            if (t < cnt)
+                lds[c * pitch + i] = x[it];
+        }
+    }
+    __syncthreads();
+    stfft::lds_fft<kColElems, NT>(lds, pitch, C, Pf, -1);
+    {
+        constexpr int IT = kColElems / NT;
+        const int hA = p.wAa / 2, az = Bt.az[b];
+        float2 v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            if (t < C * p.W_A) {
+                const int c = t / p.W_A, n = t - (t / p.W_A) * p.W_A;
+                const int k = n < hA ? n : (n >= p.W_A - hA ? n - p.W_A : (1 << 20));
+                v[it] = k != (1 << 20) ? lds[c * pitch + (k + p.wAa + az) % p.wAa] : make_float2(0.f, 0.f);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            if (t < C * p.W_A) {
+                const int c = t / p.W_A, n = t - (t / p.W_A) * p.W_A;
+                lds[c * pitch + n] = v[it];
+            }
+        }
+        __syncthreads();
+    }
+    stfft::lds_fft<kColElems, NT>(lds, pitch, C, Pi, +1);
+    float2 *Uz = Bt.U + (size_t)z * p.W_A * p.W_R + c0;
+    {
+        constexpr int IT = kColElems / NT;
+        const int cnt = p.W_A << lc;
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT, n = t >> lc, c = t & (C - 1);
+            if (t < cnt && c < nc)
+                Uz[(size_t)n * p.W_R + c] = lds[c * pitch + n];
+        }
+    }
 }
 
 #pragma clang fp contract(off)
@@ -684,12 +862,20 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
     std::vector<WRes> res(nList);
 
     if (nList > 0) {
-        const size_t NN = (size_t)p.W_A * p.W_R, n1 = (size_t)p.wAa * p.wRa;
-        const size_t per = 2 * n1 * 8 + 2 * NN * 8 + 2 * NN * 8 + 2 * (size_t)p.nLagA * p.W_R * 8 +
-                           (size_t)p.nLagA * p.nLagR * 4 + (size_t)p.nLagA * sizeof(RowBest) + 4096;
+        const size_t NN = (size_t)p.W_A * p.W_R;
+        // interpolation blocking: R rows per k_w_rowinterp block, C columns
+        // per k_w_colinterp block
+        const int R = std::max(1, std::min(p.wAa, kRowElems / p.W_R));
+        ST_REQUIRE(p.W_R <= kRowElems && p.W_A <= kColElems, ST_ERR_UNSUPPORTED, "window too large for the FFT blocks");
+        int lc = 0;
+        while ((2 << lc) <= 16 && (2 << lc) * p.W_A <= kColElems)
+            lc++;
+        const int C = 1 << lc;
+        const int nblk = (p.wAa + R - 1) / R;
+        const size_t per = 2 * NN * 8 + 2 * NN * 8 + 2 * (size_t)p.nLagA * p.W_R * 8 + (size_t)p.nLagA * p.nLagR * 4 +
+                           (size_t)p.nLagA * sizeof(RowBest) + 2 * (size_t)(nblk + 1) * 32 + 4096;
         const size_t budget = std::max(ctx->scratchBudget, per * 4);
         const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nList, budget / per, (size_t)4096}));
-        const int npair = (B + 1) / 2;
         ctx->arena_reserve((size_t)B * per + (size_t)nList * (16 + sizeof(WRes)) + 65536);
         Arena &ar = ctx->arena;
         int4 *dPos = ar.take<int4>(nList);
@@ -697,10 +883,12 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
         ST_HIP(hipMemcpyAsync(dPos, pos.data(), nList * sizeof(int4), hipMemcpyHostToDevice, st));
         WBatch Bt{};
         Bt.pos = dPos;
-        Bt.P = ar.take<float2>(2 * (size_t)B * n1);
         Bt.U = ar.take<float2>(2 * (size_t)B * NN);
         float2 *TMP = ar.take<float2>(2 * (size_t)B * NN);
         Bt.Z = TMP;
+        Bt.T = TMP;  // [2][B][wAa][W_R] fits: consumed before k_w_detect writes Z
+        Bt.dop = ar.take<double>(2 * (size_t)B * nblk * 4);
+        Bt.dopE = ar.take<double>(2 * (size_t)B * 4);
         Bt.V1 = ar.take<double>((size_t)B * p.nLagA * p.W_R);
         Bt.V2 = ar.take<double>((size_t)B * p.nLagA * p.W_R);
         Bt.chip = ar.take<double>(2 * (size_t)B);
@@ -709,17 +897,18 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
         Bt.az = ar.take<int>(B);
         const size_t ldsNcc = 2 * (size_t)(p.W_R + 1) * sizeof(double);
         ST_REQUIRE(ldsNcc <= 64 * 1024, ST_ERR_UNSUPPORTED, "window too large for the NCC kernel");
+        const stfft::PlanArgs PfR = ctx->fft.plan(p.wRa, -1), PiR = ctx->fft.plan(p.W_R, +1);
+        const stfft::PlanArgs PfA = ctx->fft.plan(p.wAa, -1), PiA = ctx->fft.plan(p.W_A, +1);
+        const size_t ldsRow = (size_t)R * p.W_R * sizeof(float2), ldsCol = (size_t)C * (p.W_A + 1) * sizeof(float2);
         for (int off = 0; off < nList; off += B) {
             const int nb = std::min(B, nList - off);
             const int np = (nb + 1) / 2;
             Bt.res = dRes + off;
-            hipLaunchKernelGGL(k_w_gather, dim3(nb), dim3(NT), 0, st, p, off, Bt, nb);
             ctx->time_begin(kFftSlot);
-            ctx->fft.fft2d(Bt.P, Bt.U, 2 * nb, p.wAa, p.wRa, -1, st);
-            ctx->time_end(kFftSlot);
-            hipLaunchKernelGGL(k_w_pad, dim3((unsigned)((NN + 255) / 256), 2 * nb), dim3(256), 0, st, p, Bt, nb);
-            ctx->time_begin(kFftSlot);
-            ctx->fft.fft2d(Bt.U, TMP, 2 * nb, p.W_A, p.W_R, +1, st);
+            hipLaunchKernelGGL(k_w_rowinterp, dim3(nblk, 2 * nb), dim3(NT), ldsRow, st, p, off, Bt, nb, R, PfR, PiR);
+            hipLaunchKernelGGL(k_w_az, dim3(nb), dim3(64), 0, st, p, Bt, nb, nblk);
+            hipLaunchKernelGGL(k_w_colinterp, dim3((p.W_R + C - 1) / C, 2 * nb), dim3(NT), ldsCol, st, p, Bt, nb, lc,
+                               PfA, PiA);
             ctx->time_end(kFftSlot);
             ctx->time_begin(kDet);
             hipLaunchKernelGGL(k_w_detect, dim3((p.W_R + kTile - 1) / kTile, (p.W_A + kTile - 1) / kTile, nb),
