@@ -30,6 +30,8 @@ class Engine {
     const PlanArgs &plan(int n, int sign);
     // In-place batched 1-D FFT of nrows contiguous rows of length n.
     void rows(float2 *data, long nrows, int n, int sign, hipStream_t st);
+    // The same, reading src and writing data (src == data is in place).
+    void rows(const float2 *src, float2 *data, long nrows, int n, int sign, hipStream_t st);
     // out[b][c][r] = in[b][r][c] for batch b of rows x cols arrays.
     static void transpose(const float2 *in, float2 *out, int batch, int rows, int cols, hipStream_t st);
     // In-place column transforms (length na) of batch arrays na x nr.
@@ -37,6 +39,8 @@ class Engine {
     // In-place batched 2-D FFT of batch arrays na x nr (row-major); tmp is
     // not used (kept so callers need not change).
     void fft2d(float2 *data, float2 *tmp, int batch, int na, int nr, int sign, hipStream_t st);
+    // Out of place: in is left untouched.
+    void fft2d_to(const float2 *in, float2 *out, int batch, int na, int nr, int sign, hipStream_t st);
 
   private:
     std::map<std::pair<int, int>, PlanArgs> plans_;

@@ -33,7 +33,8 @@ __device__ inline void stage_p(const float2 *A, float2 *B, int n, int ns, const 
         dst[base + q * ns] = v[q];
 }
 
-__global__ void __launch_bounds__(kThreads) k_fft_rows(float2 *data, long nrows, PlanArgs P, int sign)
+__global__ void __launch_bounds__(kThreads) k_fft_rows(const float2 *src, float2 *data, long nrows, PlanArgs P,
+                                                     int sign)
 {
     extern __shared__ float2 lds[];
     const int n = P.n, R = P.rowsPerBlock;
@@ -42,7 +43,7 @@ __global__ void __launch_bounds__(kThreads) k_fft_rows(float2 *data, long nrows,
     float2 *A = lds, *B = lds + R * n;
     const long base = row0 * n;
     for (int t = threadIdx.x; t < nrow * n; t += blockDim.x)
-        A[t] = data[base + t];
+        A[t] = src[base + t];
     __syncthreads();
     for (int s = 0; s < P.nst; s++) {
         const int p = P.radix[s], ns = P.ns[s];
@@ -201,14 +202,16 @@ const PlanArgs &Engine::plan(int n, int sign)
     return plans_.emplace(key, P).first->second;
 }
 
-void Engine::rows(float2 *data, long nrows, int n, int sign, hipStream_t st)
+void Engine::rows(float2 *data, long nrows, int n, int sign, hipStream_t st) { rows(data, data, nrows, n, sign, st); }
+
+void Engine::rows(const float2 *src, float2 *data, long nrows, int n, int sign, hipStream_t st)
 {
     if (nrows <= 0 || n <= 1)
         return;
     const PlanArgs &P = plan(n, sign);
     long blocks = (nrows + P.rowsPerBlock - 1) / P.rowsPerBlock;
     size_t lds = 2 * (size_t)P.rowsPerBlock * n * sizeof(float2);
-    hipLaunchKernelGGL(k_fft_rows, dim3((unsigned)blocks), dim3(kThreads), lds, st, data, nrows, P, sign);
+    hipLaunchKernelGGL(k_fft_rows, dim3((unsigned)blocks), dim3(kThreads), lds, st, src, data, nrows, P, sign);
     ST_HIP(hipGetLastError());
 }
 
@@ -245,6 +248,12 @@ void Engine::fft2d(float2 *data, float2 *tmp, int batch, int na, int nr, int sig
     (void)tmp;
     rows(data, (long)batch * na, nr, sign, st);
     cols(data, batch, na, nr, sign, st);
+}
+
+void Engine::fft2d_to(const float2 *in, float2 *out, int batch, int na, int nr, int sign, hipStream_t st)
+{
+    rows(in, out, (long)batch * na, nr, sign, st);
+    cols(out, batch, na, nr, sign, st);
 }
 
 }  // namespace stfft

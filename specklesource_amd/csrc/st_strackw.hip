@@ -71,7 +71,7 @@ struct WBatch {
     double *dopE;     // [2][B][4]: first- and last-row sums
     float2 *U;        // [2][B][W_A * W_R]; later X: [ceil(B/2)][W_A * W_R]
     float2 *Z;        // [B][W_A * W_R]: search + i chip
-    double *V1, *V2;  // [B][nLagA][W_R]: vertical window sums of s, s^2
+    float2 *Zs;       // [B][W_A * W_R]: its spectrum
     double *chip;     // [B][2]: chip mean, variance
     float *cr;        // [B][nLagA][nLagR]: NCC surface
     RowBest *rows;    // [B][nLagA]
@@ -367,32 +367,6 @@ __global__ void __launch_bounds__(256) k_w_detect(WPar p, WBatch Bt, int B)
     }
 }
 
-// computeMeanSigma (corrTrackFast.c:407-477), search part: running sums of
-// s and s^2 (FP64) down every column over windows of wA2 rows.
-__global__ void k_w_vsum(WPar p, WBatch Bt)
-{
-    const int b = blockIdx.y;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= p.W_R)
-        return;
-    const float2 *Zb = Bt.Z + (size_t)b * p.W_A * p.W_R;
-    double *V1 = Bt.V1 + (size_t)b * p.nLagA * p.W_R, *V2 = Bt.V2 + (size_t)b * p.nLagA * p.W_R;
-    double s1 = 0.0, s2 = 0.0;
-    for (int l = 0; l < p.W_A; l++) {
-        const float d = Zb[(size_t)l * p.W_R + c].x;
-        s1 += d;
-        s2 += d * d;
-        if (l >= p.wA2 - 1) {
-            const int i = l - (p.wA2 - 1);
-            V1[(size_t)i * p.W_R + c] = s1;
-            V2[(size_t)i * p.W_R + c] = s2;
-            const float e = Zb[(size_t)i * p.W_R + c].x;
-            s1 -= e;
-            s2 -= e * e;
-        }
-    }
-}
-
 // Chip mean and variance (computeMeanSigma, corrTrackFast.c:418-431):
 // sigma = sum(d^2)/(N-1) - mean^2.
 __global__ void __launch_bounds__(NT) k_w_chipstats(WPar p, WBatch Bt)
@@ -440,8 +414,8 @@ __global__ void k_w_cross(WPar p, WBatch Bt, int B)
             G[h] = make_float2(0.f, 0.f);
             continue;
         }
-        const float2 a = Bt.Z[(size_t)b * NN + e];
-        const float2 c = Bt.Z[(size_t)b * NN + en];  // Z(-k); conj below
+        const float2 a = Bt.Zs[(size_t)b * NN + e];
+        const float2 c = Bt.Zs[(size_t)b * NN + en];  // Z(-k); conj below
         const float2 S = make_float2(0.5f * (a.x + c.x), 0.5f * (a.y - c.y));
         const float2 Cc = make_float2(0.5f * (a.y + c.y), 0.5f * (c.x - a.x));
         G[h] = cmulconj(S, Cc);
@@ -449,127 +423,175 @@ __global__ void k_w_cross(WPar p, WBatch Bt, int B)
     Bt.U[(size_t)q * NN + e] = make_float2(G[0].x - G[1].y, G[0].y + G[1].x);
 }
 
-// Block-wide inclusive scan of doubles (256 threads).
-__device__ inline double block_scan_d(double v, double *sh)
+// Block-wide inclusive scan of double pairs (256 threads).
+__device__ inline double2 block_scan_d2(double2 v, double2 *sh)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int o = 1; o < 64; o <<= 1) {
-        const double u = __shfl_up(v, o);
-        if (lane >= o)
-            v += u;
+        const double ux = __shfl_up(v.x, o), uy = __shfl_up(v.y, o);
+        if (lane >= o) {
+            v.x += ux;
+            v.y += uy;
+        }
     }
     __syncthreads();
     if (lane == 63)
         sh[wid] = v;
     __syncthreads();
-    double pre = 0.0;
-    for (int w = 0; w < wid; w++)
-        pre += sh[w];
-    __syncthreads();
-    return v + pre;
+    double2 pre = make_double2(0.0, 0.0);
+    for (int w = 0; w < wid; w++) {
+        pre.x += sh[w].x;
+        pre.y += sh[w].y;
+    }
+    return make_double2(v.x + pre.x, v.y + pre.y);
 }
 
-// getPeakCorr (corrTrackFast.c:862-899) for one lag row per block: window
-// sums from prefix sums of the vertical sums, NCC in FP64, best and
+constexpr int kNccBand = 64;    // lag rows per k_w_ncc block
+constexpr int kNccMaxPer = 16;  // columns per thread: W_R <= 4096 (instances PER = 4 and 16)
+
+// computeMeanSigma (corrTrackFast.c:407-477, search part) and getPeakCorr
+// (:862-899) for a band of lag rows of one patch.  The vertical window sums
+// of s and s^2 (FP64, wA2 rows) are formed once for the band's first row
+// and slid down one row at a time; each row's horizontal window sums come
+// from an exclusive prefix over the columns.  NCC in FP64; best and
 // runner-up within the searched lags (rank = row-major lag index, the
-// reference's first-maximum rule).
+// reference's first-maximum rule) per row.
+template <int PER>
 __global__ void __launch_bounds__(256) k_w_ncc(WPar p, WBatch Bt)
 {
     extern __shared__ double lds[];
     double *P1 = lds, *P2 = lds + (p.W_R + 1);
-    __shared__ double sh[4];
+    __shared__ double2 sh[4];
     __shared__ RowBest rb[4];
-    const int i = blockIdx.x, b = blockIdx.y;
-    const double *V1 = Bt.V1 + ((size_t)b * p.nLagA + i) * p.W_R;
-    const double *V2 = Bt.V2 + ((size_t)b * p.nLagA + i) * p.W_R;
-    // prefix sums P[c] = sum_{t < c} V[t]
+    const int b = blockIdx.y, i0 = blockIdx.x * kNccBand, i1 = imin(i0 + kNccBand, p.nLagA);
+    const float2 *Zb = Bt.Z + (size_t)b * p.W_A * p.W_R;
+    double v1[PER], v2[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        v1[k] = 0.0;
+        v2[k] = 0.0;
+    }
+    for (int l = i0; l < i0 + p.wA2; l++) {
+        const float2 *row = Zb + (size_t)l * p.W_R;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int c = threadIdx.x + 256 * k;
+            if (c < p.W_R) {
+                const float d = row[c].x;
+                v1[k] += d;
+                v2[k] += d * d;
+            }
+        }
+    }
     const int per = (p.W_R + 255) / 256;
-    double a1 = 0.0, a2 = 0.0;
-    for (int k = 0; k < per; k++) {
-        const int c = threadIdx.x * per + k;
-        if (c < p.W_R) {
-            a1 += V1[c];
-            a2 += V2[c];
-        }
-    }
-    const double x1 = block_scan_d(a1, sh) - a1;
-    const double x2 = block_scan_d(a2, sh) - a2;
-    double r1 = x1, r2 = x2;
-    for (int k = 0; k < per; k++) {
-        const int c = threadIdx.x * per + k;
-        if (c < p.W_R) {
-            P1[c] = r1;
-            P2[c] = r2;
-            r1 += V1[c];
-            r2 += V2[c];
-        }
-    }
-    if (threadIdx.x == 255) {
-        P1[p.W_R] = r1;
-        P2[p.W_R] = r2;
-    }
-    __syncthreads();
     const double N = (double)p.wR2 * p.wA2;
     const double meanR = Bt.chip[2 * b], sigmaR = Bt.chip[2 * b + 1];
-    const int l = i - WOS * p.epA;
-    const bool rowIn = l >= -WOS * (p.epA - WNFAST / 4) && l <= WOS * (p.epA - WNFAST / 4);
     const int NN = p.W_A * p.W_R;
-    const float2 *X = Bt.U + (size_t)(b >> 1) * NN + (size_t)((l + p.W_A) % p.W_A) * p.W_R;
-    float *crRow = Bt.cr + ((size_t)b * p.nLagA + i) * p.nLagR;
-    double best = -1e30, second = -1e30;
-    int rank = 0x7fffffff;
-    for (int s = threadIdx.x; s < p.nLagR; s += blockDim.x) {
-        const double m = (P1[s + p.wR2] - P1[s]) / N;
-        const double sig = (P2[s + p.wR2] - P2[s]) / (N - 1.) - m * m;
-        const int c = s - WOS * p.epR;
-        const float2 xv = X[(c + p.W_R) % p.W_R];
-        const float v = (b & 1) ? xv.y : xv.x;
-        const float mag = (float)sqrt((double)(v * v));
-        double cr = (double)mag * p.scaleW;
-        cr = cr - meanR * m;
-        cr = cr / sqrt(sigmaR * sig);
-        crRow[s] = (float)cr;
-        if (rowIn && c >= -WOS * (p.epR - WNFAST / 4) && c <= WOS * (p.epR - WNFAST / 4)) {
-            if (cr > best) {
-                second = best;
-                best = cr;
-                rank = i * p.nLagR + s;
-            } else if (cr > second) {
-                second = cr;
+    for (int i = i0; i < i1; i++) {
+        if (i > i0) {
+            const float2 *top = Zb + (size_t)(i - 1) * p.W_R, *bot = Zb + (size_t)(i - 1 + p.wA2) * p.W_R;
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const int c = threadIdx.x + 256 * k;
+                if (c < p.W_R) {
+                    const float e = top[c].x, d = bot[c].x;
+                    v1[k] -= e;
+                    v2[k] -= e * e;
+                    v1[k] += d;
+                    v2[k] += d * d;
+                }
             }
         }
-    }
-    // top-2 with first-rank tie-break across the block
-    for (int o = 32; o > 0; o >>= 1) {
-        const double ob = __shfl_xor(best, o), os = __shfl_xor(second, o);
-        const int orank = __shfl_xor(rank, o);
-        const bool takeOther = ob > best || (ob == best && orank < rank);
-        const double lo = takeOther ? best : ob;
-        const double hiSecond = takeOther ? os : second;
-        second = fmax(hiSecond, lo);
-        if (takeOther) {
-            best = ob;
-            rank = orank;
+        // exclusive prefix over the columns: P[c] = sum_{t < c} V[t]
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int c = threadIdx.x + 256 * k;
+            if (c < p.W_R) {
+                P1[c] = v1[k];
+                P2[c] = v2[k];
+            }
         }
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0)
-        rb[wid] = RowBest{best, second, rank, 0};
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        RowBest r = rb[0];
-        for (int w = 1; w < 4; w++) {
-            const RowBest o = rb[w];
-            const bool takeOther = o.v > r.v || (o.v == r.v && o.rank < r.rank);
-            const double lo = takeOther ? r.v : o.v;
-            const double hs = takeOther ? o.second : r.second;
-            r.second = fmax(hs, lo);
+        __syncthreads();
+        double2 a = make_double2(0.0, 0.0);
+        const int cb = threadIdx.x * per, ce = imin(cb + per, p.W_R);
+        for (int c = cb; c < ce; c++) {
+            a.x += P1[c];
+            a.y += P2[c];
+        }
+        const double2 x = block_scan_d2(a, sh);
+        double r1 = x.x - a.x, r2 = x.y - a.y;
+        for (int c = cb; c < ce; c++) {
+            const double u1 = P1[c], u2 = P2[c];
+            P1[c] = r1;
+            P2[c] = r2;
+            r1 += u1;
+            r2 += u2;
+        }
+        if (threadIdx.x == 255) {
+            P1[p.W_R] = r1;
+            P2[p.W_R] = r2;
+        }
+        __syncthreads();
+        const int l = i - WOS * p.epA;
+        const bool rowIn = l >= -WOS * (p.epA - WNFAST / 4) && l <= WOS * (p.epA - WNFAST / 4);
+        const float2 *X = Bt.U + (size_t)(b >> 1) * NN + (size_t)((l + p.W_A) % p.W_A) * p.W_R;
+        float *crRow = Bt.cr + ((size_t)b * p.nLagA + i) * p.nLagR;
+        double best = -1e30, second = -1e30;
+        int rank = 0x7fffffff;
+        for (int s = threadIdx.x; s < p.nLagR; s += blockDim.x) {
+            const double m = (P1[s + p.wR2] - P1[s]) / N;
+            const double sig = (P2[s + p.wR2] - P2[s]) / (N - 1.) - m * m;
+            const int c = s - WOS * p.epR;
+            const float2 xv = X[(c + p.W_R) % p.W_R];
+            const float v = (b & 1) ? xv.y : xv.x;
+            const float mag = (float)sqrt((double)(v * v));
+            double cr = (double)mag * p.scaleW;
+            cr = cr - meanR * m;
+            cr = cr / sqrt(sigmaR * sig);
+            crRow[s] = (float)cr;
+            if (rowIn && c >= -WOS * (p.epR - WNFAST / 4) && c <= WOS * (p.epR - WNFAST / 4)) {
+                if (cr > best) {
+                    second = best;
+                    best = cr;
+                    rank = i * p.nLagR + s;
+                } else if (cr > second) {
+                    second = cr;
+                }
+            }
+        }
+        // top-2 with first-rank tie-break across the block
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o), os = __shfl_xor(second, o);
+            const int orank = __shfl_xor(rank, o);
+            const bool takeOther = ob > best || (ob == best && orank < rank);
+            const double lo = takeOther ? best : ob;
+            const double hiSecond = takeOther ? os : second;
+            second = fmax(hiSecond, lo);
             if (takeOther) {
-                r.v = o.v;
-                r.rank = o.rank;
+                best = ob;
+                rank = orank;
             }
         }
-        Bt.rows[(size_t)b * p.nLagA + i] = r;
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        if (lane == 0)
+            rb[wid] = RowBest{best, second, rank, 0};
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            RowBest r = rb[0];
+            for (int w = 1; w < 4; w++) {
+                const RowBest o = rb[w];
+                const bool takeOther = o.v > r.v || (o.v == r.v && o.rank < r.rank);
+                const double lo = takeOther ? r.v : o.v;
+                const double hs = takeOther ? o.second : r.second;
+                r.second = fmax(hs, lo);
+                if (takeOther) {
+                    r.v = o.v;
+                    r.rank = o.rank;
+                }
+            }
+            Bt.rows[(size_t)b * p.nLagA + i] = r;
+        }
+        __syncthreads();
     }
 }
 
@@ -872,7 +894,7 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
             lc++;
         const int C = 1 << lc;
         const int nblk = (p.wAa + R - 1) / R;
-        const size_t per = 2 * NN * 8 + 2 * NN * 8 + 2 * (size_t)p.nLagA * p.W_R * 8 + (size_t)p.nLagA * p.nLagR * 4 +
+        const size_t per = 2 * NN * 8 + 2 * NN * 8 + (size_t)p.nLagA * p.nLagR * 4 +
                            (size_t)p.nLagA * sizeof(RowBest) + 2 * (size_t)(nblk + 1) * 32 + 4096;
         const size_t budget = std::max(ctx->scratchBudget, per * 4);
         const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nList, budget / per, (size_t)4096}));
@@ -886,17 +908,17 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
         Bt.U = ar.take<float2>(2 * (size_t)B * NN);
         float2 *TMP = ar.take<float2>(2 * (size_t)B * NN);
         Bt.Z = TMP;
+        Bt.Zs = TMP + (size_t)B * NN;  // Z stays intact for the window sums of k_w_ncc
         Bt.T = TMP;  // [2][B][wAa][W_R] fits: consumed before k_w_detect writes Z
         Bt.dop = ar.take<double>(2 * (size_t)B * nblk * 4);
         Bt.dopE = ar.take<double>(2 * (size_t)B * 4);
-        Bt.V1 = ar.take<double>((size_t)B * p.nLagA * p.W_R);
-        Bt.V2 = ar.take<double>((size_t)B * p.nLagA * p.W_R);
         Bt.chip = ar.take<double>(2 * (size_t)B);
         Bt.cr = ar.take<float>((size_t)B * p.nLagA * p.nLagR);
         Bt.rows = ar.take<RowBest>((size_t)B * p.nLagA);
         Bt.az = ar.take<int>(B);
         const size_t ldsNcc = 2 * (size_t)(p.W_R + 1) * sizeof(double);
-        ST_REQUIRE(ldsNcc <= 64 * 1024, ST_ERR_UNSUPPORTED, "window too large for the NCC kernel");
+        ST_REQUIRE(ldsNcc <= 64 * 1024 && p.W_R <= 256 * kNccMaxPer, ST_ERR_UNSUPPORTED,
+                   "window too large for the NCC kernel");
         const stfft::PlanArgs PfR = ctx->fft.plan(p.wRa, -1), PiR = ctx->fft.plan(p.W_R, +1);
         const stfft::PlanArgs PfA = ctx->fft.plan(p.wAa, -1), PiA = ctx->fft.plan(p.W_A, +1);
         const size_t ldsRow = (size_t)R * p.W_R * sizeof(float2), ldsCol = (size_t)C * (p.W_A + 1) * sizeof(float2);
@@ -915,18 +937,22 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
                                dim3(256), 0, st, p, Bt, nb);
             ctx->time_end(kDet);
             ctx->time_begin(kStat);
-            hipLaunchKernelGGL(k_w_vsum, dim3((p.W_R + 63) / 64, nb), dim3(64), 0, st, p, Bt);
             hipLaunchKernelGGL(k_w_chipstats, dim3(nb), dim3(NT), 0, st, p, Bt);
             ctx->time_end(kStat);
             ctx->time_begin(kFftSlot);
-            ctx->fft.fft2d(Bt.Z, Bt.U, nb, p.W_A, p.W_R, -1, st);
+            ctx->fft.fft2d_to(Bt.Z, Bt.Zs, nb, p.W_A, p.W_R, -1, st);
             ctx->time_end(kFftSlot);
             hipLaunchKernelGGL(k_w_cross, dim3((unsigned)((NN + 255) / 256), np), dim3(256), 0, st, p, Bt, nb);
             ctx->time_begin(kFftSlot);
             ctx->fft.fft2d(Bt.U, TMP, np, p.W_A, p.W_R, +1, st);
             ctx->time_end(kFftSlot);
             ctx->time_begin(kNcc);
-            hipLaunchKernelGGL(k_w_ncc, dim3(p.nLagA, nb), dim3(256), ldsNcc, st, p, Bt);
+            if (p.W_R <= 4 * 256)
+                hipLaunchKernelGGL(k_w_ncc<4>, dim3((p.nLagA + kNccBand - 1) / kNccBand, nb), dim3(256), ldsNcc, st, p,
+                                   Bt);
+            else
+                hipLaunchKernelGGL(k_w_ncc<kNccMaxPer>, dim3((p.nLagA + kNccBand - 1) / kNccBand, nb), dim3(256), ldsNcc,
+                                   st, p, Bt);
             ctx->time_end(kNcc);
             ctx->time_begin(kRef);
             hipLaunchKernelGGL(k_w_refine, dim3(nb), dim3(256), 0, st, p, Bt);
