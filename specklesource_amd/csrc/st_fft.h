@@ -18,9 +18,29 @@ namespace stfft {
 
 constexpr int kMaxStages = 14;
 
+// Exact q = n / d for 0 <= n < 2^13 and 1 <= d <= 4096 by one multiply and
+// a shift: s = 13 + ceil(log2 d), m = ceil(2^s / d).  With e = m d - 2^s < d,
+// n m / 2^s = n / d + n e / (d 2^s) and n e < 2^13 d <= 2^s, so the floor is
+// exact; n m < 2^13 (2^14 + 1) fits 32 bits.
+struct FastDiv {
+    uint32_t m, s;
+};
+
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    uint32_t l = 0;
+    while ((1u << l) < d)
+        l++;
+    const uint32_t s = 13 + l;
+    return FastDiv{(uint32_t)(((1ull << s) + d - 1) / d), s};
+}
+
+__device__ inline int fdiv(int n, FastDiv f) { return (int)(((uint32_t)n * f.m) >> f.s); }
+
 struct PlanArgs {
     int n, nst, rowsPerBlock;
     int radix[kMaxStages], ns[kMaxStages], twoff[kMaxStages];
+    FastDiv dstride[kMaxStages], dns[kMaxStages];  // n / radix and ns of each stage
     const float2 *tw;
 };
 

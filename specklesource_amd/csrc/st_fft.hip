@@ -9,25 +9,26 @@ namespace stfft {
 
 namespace {
 
-constexpr int kPointsPerBlock = 1024;  // LDS: 2 x 1024 x 8 B per block
+constexpr int kPointsPerBlock = 1024;  // LDS: 2 x 1024 x 8 B per block (indices stay below 2^13)
 constexpr int kThreads = 256;
 constexpr int kColElems = 4096;  // per block: LDS kColElems x 8 B (+ one pad element per column)
 
 template <int P>
-__device__ inline void stage_p(const float2 *A, float2 *B, int n, int ns, const float2 *tw, int sign, int t)
+__device__ inline void stage_p(const float2 *A, float2 *B, int n, int ns, FastDiv ds, FastDiv dn, const float2 *tw,
+                               int sign, int t)
 {
     const int stride = n / P;
-    const int row = t / stride, j = t - row * stride;
+    const int row = fdiv(t, ds), j = t - row * stride;
     const float2 *src = A + row * n;
     float2 *dst = B + row * n;
-    const int k = j % ns;
+    const int jq = fdiv(j, dn), k = j - jq * ns;
     float2 v[P];
     v[0] = src[j];
 #pragma unroll
     for (int r = 1; r < P; r++)
         v[r] = cmul(src[j + r * stride], tw[k * P + r]);
     dft_any<P>(v, sign);
-    const int base = (j / ns) * ns * P + k;
+    const int base = jq * ns * P + k;
 #pragma unroll
     for (int q = 0; q < P; q++)
         dst[base + q * ns] = v[q];
@@ -47,16 +48,17 @@ __global__ void __launch_bounds__(kThreads) k_fft_rows(const float2 *src, float2
     __syncthreads();
     for (int s = 0; s < P.nst; s++) {
         const int p = P.radix[s], ns = P.ns[s];
+        const FastDiv ds = P.dstride[s], dn = P.dns[s];
         const float2 *tw = P.tw + P.twoff[s];
         const int work = nrow * (n / p);
         for (int t = threadIdx.x; t < work; t += blockDim.x) {
             switch (p) {
-            case 8: stage_p<8>(A, B, n, ns, tw, sign, t); break;
-            case 4: stage_p<4>(A, B, n, ns, tw, sign, t); break;
-            case 2: stage_p<2>(A, B, n, ns, tw, sign, t); break;
-            case 3: stage_p<3>(A, B, n, ns, tw, sign, t); break;
-            case 5: stage_p<5>(A, B, n, ns, tw, sign, t); break;
-            default: stage_p<7>(A, B, n, ns, tw, sign, t); break;
+            case 8: stage_p<8>(A, B, n, ns, ds, dn, tw, sign, t); break;
+            case 4: stage_p<4>(A, B, n, ns, ds, dn, tw, sign, t); break;
+            case 2: stage_p<2>(A, B, n, ns, ds, dn, tw, sign, t); break;
+            case 3: stage_p<3>(A, B, n, ns, ds, dn, tw, sign, t); break;
+            case 5: stage_p<5>(A, B, n, ns, ds, dn, tw, sign, t); break;
+            default: stage_p<7>(A, B, n, ns, ds, dn, tw, sign, t); break;
             }
         }
         __syncthreads();
@@ -181,6 +183,8 @@ const PlanArgs &Engine::plan(int n, int sign)
         ST_REQUIRE(k < kMaxStages, ST_ERR_UNSUPPORTED, "FFT plan too deep");
         P.radix[k] = r;
         P.ns[k] = ns;
+        P.dstride[k] = make_fastdiv((uint32_t)(n / r));
+        P.dns[k] = make_fastdiv((uint32_t)ns);
         P.twoff[k] = (int)tw.size();
         for (int j = 0; j < ns; j++)
             for (int q = 0; q < r; q++) {

@@ -112,8 +112,10 @@ __device__ inline void dft_any(float2 *v, int sign)
 // lds_fft, the six radix cases did not share registers and the caller
 // needed twice the VGPRs of the largest case.
 template <int R, int MAXE, int NTH>
-__device__ __attribute__((noinline)) void lds_stage(float2 *buf, int pitch, int nseq, int n, int ns, const float2 *tw, int sign)
+__device__ __attribute__((noinline)) void lds_stage(float2 *buf, int pitch, int nseq, int n, int ns, FastDiv dstride,
+                                                    FastDiv dns, const float2 *tw, int sign)
 {
+    static_assert(MAXE <= 8192, "FastDiv needs indices below 2^13");
     constexpr int IT = (MAXE / R + NTH - 1) / NTH;
     const int stride = n / R, work = nseq * stride;
     float2 v[IT][R];
@@ -123,14 +125,14 @@ __device__ __attribute__((noinline)) void lds_stage(float2 *buf, int pitch, int 
         const int t = threadIdx.x + it * NTH;
         at[it] = -1;
         if (t < work) {
-            const int sq = t / stride, j = t - sq * stride, k = j % ns;
+            const int sq = fdiv(t, dstride), j = t - sq * stride, jq = fdiv(j, dns), k = j - jq * ns;
             const float2 *src = buf + sq * pitch;
             v[it][0] = src[j];
 #pragma unroll
             for (int r = 1; r < R; r++)
                 v[it][r] = cmul(src[j + r * stride], tw[k * R + r]);
             dft_any<R>(v[it], sign);
-            at[it] = sq * pitch + (j / ns) * ns * R + k;
+            at[it] = sq * pitch + jq * ns * R + k;
         }
     }
     __syncthreads();
@@ -149,14 +151,15 @@ __device__ inline void lds_fft(float2 *buf, int pitch, int nseq, const PlanArgs 
 {
     for (int s = 0; s < P.nst; s++) {
         const int ns = P.ns[s];
+        const FastDiv ds = P.dstride[s], dn = P.dns[s];
         const float2 *tw = P.tw + P.twoff[s];
         switch (P.radix[s]) {
-        case 8: lds_stage<8, MAXE, NTH>(buf, pitch, nseq, P.n, ns, tw, sign); break;
-        case 4: lds_stage<4, MAXE, NTH>(buf, pitch, nseq, P.n, ns, tw, sign); break;
-        case 2: lds_stage<2, MAXE, NTH>(buf, pitch, nseq, P.n, ns, tw, sign); break;
-        case 3: lds_stage<3, MAXE, NTH>(buf, pitch, nseq, P.n, ns, tw, sign); break;
-        case 5: lds_stage<5, MAXE, NTH>(buf, pitch, nseq, P.n, ns, tw, sign); break;
-        default: lds_stage<7, MAXE, NTH>(buf, pitch, nseq, P.n, ns, tw, sign); break;
+        case 8: lds_stage<8, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
+        case 4: lds_stage<4, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
+        case 2: lds_stage<2, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
+        case 3: lds_stage<3, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
+        case 5: lds_stage<5, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
+        default: lds_stage<7, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
         }
     }
 }

@@ -49,6 +49,7 @@ __constant__ double c_dirw[kDirN];
 struct WPar {
     Images im;
     int wRa, wAa, W_R, W_A, epR, epA, wR2, wA2, nLagR, nLagA;
+    stfft::FastDiv dwRa, dWR, dWA;  // index splits in the interpolation kernels
     int navgR, navgA;
     float scale;
     double scaleW;
@@ -73,6 +74,7 @@ struct WBatch {
     float2 *Z;        // [B][W_A * W_R]: search + i chip
     float2 *Zs;       // [B][W_A * W_R]: its spectrum
     double *chip;     // [B][2]: chip mean, variance
+    double *csum;     // [B][ceil(W_A/32)][2][W_R]: column sums of s, s^2 over 32-row tiles
     float *cr;        // [B][nLagA][nLagR]: NCC surface
     RowBest *rows;    // [B][nLagA]
     int *az;          // [B]
@@ -119,12 +121,12 @@ __global__ void __launch_bounds__(NT) k_w_rowinterp(WPar p, int off, WBatch Bt, 
         float2 x[IT];
 #pragma unroll
         for (int it = 0; it < IT; it++) {
-            const int t = threadIdx.x + it * NT, i = t / p.wRa, j = t - (t / p.wRa) * p.wRa;
+            const int t = threadIdx.x + it * NT, i = stfft::fdiv(t, p.dwRa), j = t - i * p.wRa;
             x[it] = t < cnt ? sample0(p.im, img, ra + i0 + i, rr + j) : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int it = 0; it < IT; it++) {
-            const int t = threadIdx.x + it * NT, i = t / p.wRa, j = t - (t / p.wRa) * p.wRa;
+            const int t = threadIdx.x + it * NT, i = stfft::fdiv(t, p.dwRa), j = t - i * p.wRa;
             if (t < cnt) {
                 lds[i * pitch + j] = x[it];
                 s[0] += x[it].x;
@@ -143,7 +145,7 @@ __global__ void __launch_bounds__(NT) k_w_rowinterp(WPar p, int off, WBatch Bt, 
     __syncthreads();
     const int nlag = imin(nrow, p.wAa - 1 - i0);
     for (int t = threadIdx.x; t < nlag * p.wRa; t += NT) {
-        const int i = t / p.wRa, j = t - (t / p.wRa) * p.wRa;
+        const int i = stfft::fdiv(t, p.dwRa), j = t - i * p.wRa;
         const float2 x0 = lds[i * pitch + j];
         const float2 x1 = i + 1 < nrow ? lds[(i + 1) * pitch + j] : sample0(p.im, img, ra + i0 + i + 1, rr + j);
         s[2] += (double)x1.x * x0.x + (double)x1.y * x0.y;
@@ -176,7 +178,7 @@ __global__ void __launch_bounds__(NT) k_w_rowinterp(WPar p, int off, WBatch Bt, 
         for (int it = 0; it < IT; it++) {
             const int t = threadIdx.x + it * NT;
             if (t < nrow * p.W_R) {
-                const int i = t / p.W_R, m = t - (t / p.W_R) * p.W_R;
+                const int i = stfft::fdiv(t, p.dWR), m = t - i * p.W_R;
                 const int l = m < hR ? m : (m >= p.W_R - hR ? m - p.wRa : -1);
                 v[it] = l >= 0 ? lds[i * pitch + l] : make_float2(0.f, 0.f);
             }
@@ -281,9 +283,12 @@ __global__ void __launch_bounds__(NT) k_w_colinterp(WPar p, WBatch Bt, int B, in
         for (int it = 0; it < IT; it++) {
             const int t = threadIdx.x + it * NT;
             if (t < C * p.W_A) {
-                const int c = t / p.W_A, n = t - (t / p.W_A) * p.W_A;
+                const int c = stfft::fdiv(t, p.dWA), n = t - c * p.W_A;
                 const int k = n < hA ? n : (n >= p.W_A - hA ? n - p.W_A : (1 << 20));
-                v[it] = k != (1 << 20) ? lds[c * pitch + (k + p.wAa + az) % p.wAa] : make_float2(0.f, 0.f);
+                int ka = k + p.wAa + az;  // in [0, 3 wAa)
+                ka -= ka >= p.wAa ? p.wAa : 0;
+                ka -= ka >= p.wAa ? p.wAa : 0;
+                v[it] = k != (1 << 20) ? lds[c * pitch + ka] : make_float2(0.f, 0.f);
             }
         }
         __syncthreads();
@@ -291,7 +296,7 @@ __global__ void __launch_bounds__(NT) k_w_colinterp(WPar p, WBatch Bt, int B, in
         for (int it = 0; it < IT; it++) {
             const int t = threadIdx.x + it * NT;
             if (t < C * p.W_A) {
-                const int c = t / p.W_A, n = t - (t / p.W_A) * p.W_A;
+                const int c = stfft::fdiv(t, p.dWA), n = t - c * p.W_A;
                 lds[c * pitch + n] = v[it];
             }
         }
@@ -321,6 +326,7 @@ __global__ void __launch_bounds__(256) k_w_detect(WPar p, WBatch Bt, int B)
 {
     __shared__ float d1[(kTile + 2 * kMaxNavg) * (kTile + 2 * kMaxNavg)];
     __shared__ float d2[(kTile + 2 * kMaxNavg) * (kTile + 2 * kMaxNavg)];
+    __shared__ float sv[kTile][kTile + 1];
     const int b = blockIdx.z;
     const int NN = p.W_A * p.W_R;
     const float2 *U1 = Bt.U + (size_t)b * NN, *U2 = Bt.U + ((size_t)B + b) * NN;
@@ -349,6 +355,7 @@ __global__ void __launch_bounds__(256) k_w_detect(WPar p, WBatch Bt, int B)
     for (int t = threadIdx.x; t < kTile * kTile; t += blockDim.x) {
         const int oy = t / kTile, ox = t - (t / kTile) * kTile;
         const int y = y0 + oy, x = x0 + ox;
+        sv[oy][ox] = 0.f;
         if (y >= p.W_A || x >= p.W_R)
             continue;
         float v1 = 0.f;
@@ -363,7 +370,23 @@ __global__ void __launch_bounds__(256) k_w_detect(WPar p, WBatch Bt, int B)
                     v2 += d2[(oy + hA + m) * TW + ox + hR + n];
             s2 = (float)sqrt((double)v2);
         }
-        Bt.Z[(size_t)b * NN + (size_t)y * p.W_R + x] = make_float2((float)sqrt((double)v1), s2);
+        const float sr = (float)sqrt((double)v1);
+        sv[oy][ox] = sr;
+        Bt.Z[(size_t)b * NN + (size_t)y * p.W_R + x] = make_float2(sr, s2);
+    }
+    // column sums of s and s^2 over this tile's rows (FP64), for the window
+    // sums of k_w_ncc
+    __syncthreads();
+    if (threadIdx.x < kTile && x0 + (int)threadIdx.x < p.W_R) {
+        double a1 = 0.0, a2 = 0.0;
+        for (int oy = 0; oy < kTile; oy++) {
+            const float v = sv[oy][threadIdx.x];
+            a1 += v;
+            a2 += v * v;
+        }
+        double *cs = Bt.csum + ((size_t)b * gridDim.y + blockIdx.y) * 2 * p.W_R + x0 + threadIdx.x;
+        cs[0] = a1;
+        cs[p.W_R] = a2;
     }
 }
 
@@ -423,175 +446,217 @@ __global__ void k_w_cross(WPar p, WBatch Bt, int B)
     Bt.U[(size_t)q * NN + e] = make_float2(G[0].x - G[1].y, G[0].y + G[1].x);
 }
 
-// Block-wide inclusive scan of double pairs (256 threads).
-__device__ inline double2 block_scan_d2(double2 v, double2 *sh)
+constexpr int kNccRows = 32;  // lag rows per k_w_ncc block (a multiple of the 32-row tiles)
+constexpr int kNccStep = 4;   // lag rows per round of block barriers
+static_assert(kNccRows % kTile == 0, "NCC runs start on tile rows");
+
+// Top-2 merge with the first-rank tie-break.
+__device__ inline void top2_merge(double &best, double &second, int &rank, double ob, double os, int orank)
 {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int o = 1; o < 64; o <<= 1) {
-        const double ux = __shfl_up(v.x, o), uy = __shfl_up(v.y, o);
-        if (lane >= o) {
-            v.x += ux;
-            v.y += uy;
-        }
+    const bool takeOther = ob > best || (ob == best && orank < rank);
+    const double lo = takeOther ? best : ob;
+    const double hs = takeOther ? os : second;
+    second = fmax(hs, lo);
+    if (takeOther) {
+        best = ob;
+        rank = orank;
     }
-    __syncthreads();
-    if (lane == 63)
-        sh[wid] = v;
-    __syncthreads();
-    double2 pre = make_double2(0.0, 0.0);
-    for (int w = 0; w < wid; w++) {
-        pre.x += sh[w].x;
-        pre.y += sh[w].y;
-    }
-    return make_double2(v.x + pre.x, v.y + pre.y);
 }
 
-constexpr int kNccBand = 64;    // lag rows per k_w_ncc block
-constexpr int kNccMaxPer = 16;  // columns per thread: W_R <= 4096 (instances PER = 4 and 16)
-
 // computeMeanSigma (corrTrackFast.c:407-477, search part) and getPeakCorr
-// (:862-899) for a band of lag rows of one patch.  The vertical window sums
-// of s and s^2 (FP64, wA2 rows) are formed once for the band's first row
-// and slid down one row at a time; each row's horizontal window sums come
-// from an exclusive prefix over the columns.  NCC in FP64; best and
-// runner-up within the searched lags (rank = row-major lag index, the
-// reference's first-maximum rule) per row.
+// (:862-899) for a run of kNccRows lag rows of one patch, nStep <= kNccStep
+// rows per round of block barriers.  Thread t owns columns t + 256 k,
+// k < PER.  The vertical window sums of s and s^2 (FP64, wA2 rows) start
+// from the 32-row tile column sums of k_w_detect (plus the rows of a last
+// partial tile) and slide down one row at a time; each row's horizontal
+// window sums come from an exclusive prefix over the columns (in
+// LDS, one slice per row of the round).  NCC in FP64; best and runner-up
+// within the searched lags (rank = row-major lag index, the reference's
+// first-maximum rule) per row.
 template <int PER>
-__global__ void __launch_bounds__(256) k_w_ncc(WPar p, WBatch Bt)
+__global__ void __launch_bounds__(256) k_w_ncc(WPar p, WBatch Bt, int nRun, int nStep)
 {
     extern __shared__ double lds[];
-    double *P1 = lds, *P2 = lds + (p.W_R + 1);
-    __shared__ double2 sh[4];
-    __shared__ RowBest rb[4];
-    const int b = blockIdx.y, i0 = blockIdx.x * kNccBand, i1 = imin(i0 + kNccBand, p.nLagA);
+    __shared__ double sh[4][2 * kNccStep];
+    __shared__ RowBest rb[4][kNccStep];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int b = blockIdx.y, run = blockIdx.x;
+    const int i0 = run * kNccRows, i1 = imin(i0 + kNccRows, p.nLagA);
+    const int slice = 2 * (p.W_R + 1);
     const float2 *Zb = Bt.Z + (size_t)b * p.W_A * p.W_R;
+    const int NN = p.W_A * p.W_R;
+    const float2 *Xb = Bt.U + (size_t)(b >> 1) * NN;
+    const bool odd = b & 1;
+    const int nTile = (p.W_A + kTile - 1) / kTile;
+    const double *cs = Bt.csum + (size_t)b * nTile * 2 * p.W_R;
+    const int jA = i0 / kTile, jB = (i0 + p.wA2) / kTile;
     double v1[PER], v2[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
+        const int c = threadIdx.x + 256 * k;
         v1[k] = 0.0;
         v2[k] = 0.0;
-    }
-    for (int l = i0; l < i0 + p.wA2; l++) {
-        const float2 *row = Zb + (size_t)l * p.W_R;
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int c = threadIdx.x + 256 * k;
-            if (c < p.W_R) {
-                const float d = row[c].x;
+        if (c < p.W_R) {
+            for (int j = jA; j < jB; j++) {
+                v1[k] += cs[(size_t)j * 2 * p.W_R + c];
+                v2[k] += cs[(size_t)j * 2 * p.W_R + p.W_R + c];
+            }
+            for (int l = jB * kTile; l < i0 + p.wA2; l++) {
+                const float d = Zb[(size_t)l * p.W_R + c].x;
                 v1[k] += d;
                 v2[k] += d * d;
             }
         }
     }
     const int per = (p.W_R + 255) / 256;
+    const int cb = threadIdx.x * per, ce = imin(cb + per, p.W_R);
     const double N = (double)p.wR2 * p.wA2;
     const double meanR = Bt.chip[2 * b], sigmaR = Bt.chip[2 * b + 1];
-    const int NN = p.W_A * p.W_R;
-    for (int i = i0; i < i1; i++) {
-        if (i > i0) {
-            const float2 *top = Zb + (size_t)(i - 1) * p.W_R, *bot = Zb + (size_t)(i - 1 + p.wA2) * p.W_R;
+    for (int ib = i0; ib < i1; ib += nStep) {
+        const int nr = imin(nStep, i1 - ib);
+        // slide the window sums to each row of the round and park them
 #pragma unroll
-            for (int k = 0; k < PER; k++) {
-                const int c = threadIdx.x + 256 * k;
-                if (c < p.W_R) {
-                    const float e = top[c].x, d = bot[c].x;
-                    v1[k] -= e;
-                    v2[k] -= e * e;
-                    v1[k] += d;
-                    v2[k] += d * d;
+        for (int r = 0; r < kNccStep; r++) {
+            const int i = ib + r;
+            if (r < nr) {
+                if (i > i0) {
+                    const float2 *top = Zb + (size_t)(i - 1) * p.W_R, *bot = Zb + (size_t)(i - 1 + p.wA2) * p.W_R;
+#pragma unroll
+                    for (int k = 0; k < PER; k++) {
+                        const int c = threadIdx.x + 256 * k;
+                        if (c < p.W_R) {
+                            const float e = top[c].x, d = bot[c].x;
+                            v1[k] -= e;
+                            v2[k] -= e * e;
+                            v1[k] += d;
+                            v2[k] += d * d;
+                        }
+                    }
+                }
+                double *P1 = lds + r * slice, *P2 = P1 + (p.W_R + 1);
+#pragma unroll
+                for (int k = 0; k < PER; k++) {
+                    const int c = threadIdx.x + 256 * k;
+                    if (c < p.W_R) {
+                        P1[c] = v1[k];
+                        P2[c] = v2[k];
+                    }
                 }
             }
         }
-        // exclusive prefix over the columns: P[c] = sum_{t < c} V[t]
+        __syncthreads();
+        // exclusive prefixes over the columns: P[c] = sum_{t < c} V[t]
+        double a[2 * kNccStep], x[2 * kNccStep];
 #pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int c = threadIdx.x + 256 * k;
-            if (c < p.W_R) {
-                P1[c] = v1[k];
-                P2[c] = v2[k];
+        for (int r = 0; r < kNccStep; r++) {
+            const double *P1 = lds + r * slice, *P2 = P1 + (p.W_R + 1);
+            a[2 * r] = 0.0;
+            a[2 * r + 1] = 0.0;
+            if (r < nr)
+                for (int c = cb; c < ce; c++) {
+                    a[2 * r] += P1[c];
+                    a[2 * r + 1] += P2[c];
+                }
+            x[2 * r] = a[2 * r];
+            x[2 * r + 1] = a[2 * r + 1];
+        }
+        for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+            for (int q = 0; q < 2 * kNccStep; q++) {
+                const double u = __shfl_up(x[q], o);
+                if (lane >= o)
+                    x[q] += u;
+            }
+        }
+        if (lane == 63)
+#pragma unroll
+            for (int q = 0; q < 2 * kNccStep; q++)
+                sh[wid][q] = x[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 2 * kNccStep; q++) {
+            double pre = 0.0;
+            for (int w = 0; w < wid; w++)
+                pre += sh[w][q];
+            x[q] += pre - a[q];  // exclusive prefix at the start of the chunk
+        }
+#pragma unroll
+        for (int r = 0; r < kNccStep; r++) {
+            if (r < nr) {
+                double *P1 = lds + r * slice, *P2 = P1 + (p.W_R + 1);
+                double r1 = x[2 * r], r2 = x[2 * r + 1];
+                for (int c = cb; c < ce; c++) {
+                    const double u1 = P1[c], u2 = P2[c];
+                    P1[c] = r1;
+                    P2[c] = r2;
+                    r1 += u1;
+                    r2 += u2;
+                }
+                if (threadIdx.x == 255) {
+                    P1[p.W_R] = r1;
+                    P2[p.W_R] = r2;
+                }
             }
         }
         __syncthreads();
-        double2 a = make_double2(0.0, 0.0);
-        const int cb = threadIdx.x * per, ce = imin(cb + per, p.W_R);
-        for (int c = cb; c < ce; c++) {
-            a.x += P1[c];
-            a.y += P2[c];
-        }
-        const double2 x = block_scan_d2(a, sh);
-        double r1 = x.x - a.x, r2 = x.y - a.y;
-        for (int c = cb; c < ce; c++) {
-            const double u1 = P1[c], u2 = P2[c];
-            P1[c] = r1;
-            P2[c] = r2;
-            r1 += u1;
-            r2 += u2;
-        }
-        if (threadIdx.x == 255) {
-            P1[p.W_R] = r1;
-            P2[p.W_R] = r2;
-        }
-        __syncthreads();
-        const int l = i - WOS * p.epA;
-        const bool rowIn = l >= -WOS * (p.epA - WNFAST / 4) && l <= WOS * (p.epA - WNFAST / 4);
-        const float2 *X = Bt.U + (size_t)(b >> 1) * NN + (size_t)((l + p.W_A) % p.W_A) * p.W_R;
-        float *crRow = Bt.cr + ((size_t)b * p.nLagA + i) * p.nLagR;
-        double best = -1e30, second = -1e30;
-        int rank = 0x7fffffff;
-        for (int s = threadIdx.x; s < p.nLagR; s += blockDim.x) {
-            const double m = (P1[s + p.wR2] - P1[s]) / N;
-            const double sig = (P2[s + p.wR2] - P2[s]) / (N - 1.) - m * m;
-            const int c = s - WOS * p.epR;
-            const float2 xv = X[(c + p.W_R) % p.W_R];
-            const float v = (b & 1) ? xv.y : xv.x;
-            const float mag = (float)sqrt((double)(v * v));
-            double cr = (double)mag * p.scaleW;
-            cr = cr - meanR * m;
-            cr = cr / sqrt(sigmaR * sig);
-            crRow[s] = (float)cr;
-            if (rowIn && c >= -WOS * (p.epR - WNFAST / 4) && c <= WOS * (p.epR - WNFAST / 4)) {
-                if (cr > best) {
-                    second = best;
-                    best = cr;
-                    rank = i * p.nLagR + s;
-                } else if (cr > second) {
-                    second = cr;
+        double best[kNccStep], second[kNccStep];
+        int rank[kNccStep];
+#pragma unroll
+        for (int r = 0; r < kNccStep; r++) {
+            best[r] = -1e30;
+            second[r] = -1e30;
+            rank[r] = 0x7fffffff;
+            const int i = ib + r;
+            if (r < nr) {
+                const double *P1 = lds + r * slice, *P2 = P1 + (p.W_R + 1);
+                const int l = i - WOS * p.epA;
+                const bool rowIn = l >= -WOS * (p.epA - WNFAST / 4) && l <= WOS * (p.epA - WNFAST / 4);
+                const float2 *X = Xb + (size_t)((l + p.W_A) % p.W_A) * p.W_R;
+                float *crRow = Bt.cr + ((size_t)b * p.nLagA + i) * p.nLagR;
+                for (int s = threadIdx.x; s < p.nLagR; s += 256) {
+                    const double m = (P1[s + p.wR2] - P1[s]) / N;
+                    const double sig = (P2[s + p.wR2] - P2[s]) / (N - 1.) - m * m;
+                    const int c = s - WOS * p.epR;
+                    const float2 xv = X[(c + p.W_R) % p.W_R];
+                    const float v = odd ? xv.y : xv.x;
+                    const float mag = (float)sqrt((double)(v * v));
+                    double cr = (double)mag * p.scaleW;
+                    cr = cr - meanR * m;
+                    cr = cr / sqrt(sigmaR * sig);
+                    crRow[s] = (float)cr;
+                    if (rowIn && c >= -WOS * (p.epR - WNFAST / 4) && c <= WOS * (p.epR - WNFAST / 4)) {
+                        if (cr > best[r]) {
+                            second[r] = best[r];
+                            best[r] = cr;
+                            rank[r] = i * p.nLagR + s;
+                        } else if (cr > second[r]) {
+                            second[r] = cr;
+                        }
+                    }
                 }
             }
         }
         // top-2 with first-rank tie-break across the block
         for (int o = 32; o > 0; o >>= 1) {
-            const double ob = __shfl_xor(best, o), os = __shfl_xor(second, o);
-            const int orank = __shfl_xor(rank, o);
-            const bool takeOther = ob > best || (ob == best && orank < rank);
-            const double lo = takeOther ? best : ob;
-            const double hiSecond = takeOther ? os : second;
-            second = fmax(hiSecond, lo);
-            if (takeOther) {
-                best = ob;
-                rank = orank;
+#pragma unroll
+            for (int r = 0; r < kNccStep; r++) {
+                const double ob = __shfl_xor(best[r], o), os = __shfl_xor(second[r], o);
+                const int orank = __shfl_xor(rank[r], o);
+                top2_merge(best[r], second[r], rank[r], ob, os, orank);
             }
         }
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
         if (lane == 0)
-            rb[wid] = RowBest{best, second, rank, 0};
+#pragma unroll
+            for (int r = 0; r < kNccStep; r++)
+                rb[wid][r] = RowBest{best[r], second[r], rank[r], 0};
         __syncthreads();
-        if (threadIdx.x == 0) {
-            RowBest r = rb[0];
-            for (int w = 1; w < 4; w++) {
-                const RowBest o = rb[w];
-                const bool takeOther = o.v > r.v || (o.v == r.v && o.rank < r.rank);
-                const double lo = takeOther ? r.v : o.v;
-                const double hs = takeOther ? o.second : r.second;
-                r.second = fmax(hs, lo);
-                if (takeOther) {
-                    r.v = o.v;
-                    r.rank = o.rank;
-                }
-            }
-            Bt.rows[(size_t)b * p.nLagA + i] = r;
+        if (threadIdx.x < nr) {
+            const int r = threadIdx.x;
+            RowBest q = rb[0][r];
+            for (int w = 1; w < 4; w++)
+                top2_merge(q.v, q.second, q.rank, rb[w][r].v, rb[w][r].second, rb[w][r].rank);
+            Bt.rows[(size_t)b * p.nLagA + ib + r] = q;
         }
-        __syncthreads();
     }
 }
 
@@ -831,6 +896,9 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
     p.nLagA = 2 * p.epA * WOS + 1;
     p.navgR = par.navgR;
     p.navgA = par.navgA;
+    p.dwRa = stfft::make_fastdiv((uint32_t)p.wRa);
+    p.dWR = stfft::make_fastdiv((uint32_t)p.W_R);
+    p.dWA = stfft::make_fastdiv((uint32_t)p.W_A);
     ST_REQUIRE(par.edgePadR >= 0 && par.edgePadA >= 0 && p.wR2 > 0 && p.wA2 > 0, ST_ERR_ARG,
                "edge pad leaves no chip");
     ST_REQUIRE(stfft::supported_size(p.wRa) && stfft::supported_size(p.wAa) && stfft::supported_size(p.W_R) &&
@@ -894,8 +962,10 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
             lc++;
         const int C = 1 << lc;
         const int nblk = (p.wAa + R - 1) / R;
+        const int nRun = (p.nLagA + kNccRows - 1) / kNccRows;
         const size_t per = 2 * NN * 8 + 2 * NN * 8 + (size_t)p.nLagA * p.nLagR * 4 +
-                           (size_t)p.nLagA * sizeof(RowBest) + 2 * (size_t)(nblk + 1) * 32 + 4096;
+                           (size_t)p.nLagA * sizeof(RowBest) + 2 * (size_t)(nblk + 1) * 32 +
+                           (size_t)((p.W_A + kTile - 1) / kTile) * 2 * p.W_R * 8 + 4096;
         const size_t budget = std::max(ctx->scratchBudget, per * 4);
         const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nList, budget / per, (size_t)4096}));
         ctx->arena_reserve((size_t)B * per + (size_t)nList * (16 + sizeof(WRes)) + 65536);
@@ -913,12 +983,13 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
         Bt.dop = ar.take<double>(2 * (size_t)B * nblk * 4);
         Bt.dopE = ar.take<double>(2 * (size_t)B * 4);
         Bt.chip = ar.take<double>(2 * (size_t)B);
+        Bt.csum = ar.take<double>((size_t)B * ((p.W_A + kTile - 1) / kTile) * 2 * p.W_R);
         Bt.cr = ar.take<float>((size_t)B * p.nLagA * p.nLagR);
         Bt.rows = ar.take<RowBest>((size_t)B * p.nLagA);
         Bt.az = ar.take<int>(B);
-        const size_t ldsNcc = 2 * (size_t)(p.W_R + 1) * sizeof(double);
-        ST_REQUIRE(ldsNcc <= 64 * 1024 && p.W_R <= 256 * kNccMaxPer, ST_ERR_UNSUPPORTED,
-                   "window too large for the NCC kernel");
+        const int nStep = (int)std::min<size_t>(kNccStep, 65536 / (2 * (size_t)(p.W_R + 1) * sizeof(double)));
+        ST_REQUIRE(nStep >= 1 && p.W_R <= 256 * 8, ST_ERR_UNSUPPORTED, "window too large for the NCC kernel");
+        const size_t ldsNcc = (size_t)nStep * 2 * (p.W_R + 1) * sizeof(double);
         const stfft::PlanArgs PfR = ctx->fft.plan(p.wRa, -1), PiR = ctx->fft.plan(p.W_R, +1);
         const stfft::PlanArgs PfA = ctx->fft.plan(p.wAa, -1), PiA = ctx->fft.plan(p.W_A, +1);
         const size_t ldsRow = (size_t)R * p.W_R * sizeof(float2), ldsCol = (size_t)C * (p.W_A + 1) * sizeof(float2);
@@ -947,12 +1018,13 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
             ctx->fft.fft2d(Bt.U, TMP, np, p.W_A, p.W_R, +1, st);
             ctx->time_end(kFftSlot);
             ctx->time_begin(kNcc);
-            if (p.W_R <= 4 * 256)
-                hipLaunchKernelGGL(k_w_ncc<4>, dim3((p.nLagA + kNccBand - 1) / kNccBand, nb), dim3(256), ldsNcc, st, p,
-                                   Bt);
-            else
-                hipLaunchKernelGGL(k_w_ncc<kNccMaxPer>, dim3((p.nLagA + kNccBand - 1) / kNccBand, nb), dim3(256), ldsNcc,
-                                   st, p, Bt);
+            {
+                const dim3 g(nRun, nb);
+                if (p.W_R <= 256 * 4)
+                    hipLaunchKernelGGL(k_w_ncc<4>, g, dim3(256), ldsNcc, st, p, Bt, nRun, nStep);
+                else
+                    hipLaunchKernelGGL(k_w_ncc<8>, g, dim3(256), ldsNcc, st, p, Bt, nRun, nStep);
+            }
             ctx->time_end(kNcc);
             ctx->time_begin(kRef);
             hipLaunchKernelGGL(k_w_refine, dim3(nb), dim3(256), 0, st, p, Bt);
