@@ -6,16 +6,18 @@
 // even / odd samples (decimation in time) and leave the first / second half
 // of the output in lanes 0 / 1; inverse transforms take the first / second
 // half of the input (decimation in frequency) and leave the even / odd
-// outputs.  Twiddles come from constant tables indexed by compile-time
-// constants, so they are scalar loads.
+// outputs.  Twiddles are compile-time literals.
 #pragma once
 
 #include "st_common.h"
 
 namespace fftreg {
 
-// exp(-2 pi i j / 64), j = 0..63
-extern __constant__ float2 c_tw64[64];
+// exp(-2 pi i j / 64), j = 0..63: (float)cos and (float)sin of the double
+// angle -2 pi j / 64, written as literals so the compiler folds them into
+// instructions instead of holding them in SGPRs.
+constexpr float kTw64Re[64] = {0x1p+0f, 0x1.fd88dap-1f, 0x1.f6297cp-1f, 0x1.e9f416p-1f, 0x1.d906bcp-1f, 0x1.c38b3p-1f, 0x1.a9b662p-1f, 0x1.8bc806p-1f, 0x1.6a09e6p-1f, 0x1.44cf32p-1f, 0x1.1c73b4p-1f, 0x1.e2b5d4p-2f, 0x1.87de2ap-2f, 0x1.294062p-2f, 0x1.8f8b84p-3f, 0x1.917a6cp-4f, 0x1.1a6264p-54f, -0x1.917a6cp-4f, -0x1.8f8b84p-3f, -0x1.294062p-2f, -0x1.87de2ap-2f, -0x1.e2b5d4p-2f, -0x1.1c73b4p-1f, -0x1.44cf32p-1f, -0x1.6a09e6p-1f, -0x1.8bc806p-1f, -0x1.a9b662p-1f, -0x1.c38b3p-1f, -0x1.d906bcp-1f, -0x1.e9f416p-1f, -0x1.f6297cp-1f, -0x1.fd88dap-1f, -0x1p+0f, -0x1.fd88dap-1f, -0x1.f6297cp-1f, -0x1.e9f416p-1f, -0x1.d906bcp-1f, -0x1.c38b3p-1f, -0x1.a9b662p-1f, -0x1.8bc806p-1f, -0x1.6a09e6p-1f, -0x1.44cf32p-1f, -0x1.1c73b4p-1f, -0x1.e2b5d4p-2f, -0x1.87de2ap-2f, -0x1.294062p-2f, -0x1.8f8b84p-3f, -0x1.917a6cp-4f, -0x1.a79394p-53f, 0x1.917a6cp-4f, 0x1.8f8b84p-3f, 0x1.294062p-2f, 0x1.87de2ap-2f, 0x1.e2b5d4p-2f, 0x1.1c73b4p-1f, 0x1.44cf32p-1f, 0x1.6a09e6p-1f, 0x1.8bc806p-1f, 0x1.a9b662p-1f, 0x1.c38b3p-1f, 0x1.d906bcp-1f, 0x1.e9f416p-1f, 0x1.f6297cp-1f, 0x1.fd88dap-1f};
+constexpr float kTw64Im[64] = {-0x0p+0f, -0x1.917a6cp-4f, -0x1.8f8b84p-3f, -0x1.294062p-2f, -0x1.87de2ap-2f, -0x1.e2b5d4p-2f, -0x1.1c73b4p-1f, -0x1.44cf32p-1f, -0x1.6a09e6p-1f, -0x1.8bc806p-1f, -0x1.a9b662p-1f, -0x1.c38b3p-1f, -0x1.d906bcp-1f, -0x1.e9f416p-1f, -0x1.f6297cp-1f, -0x1.fd88dap-1f, -0x1p+0f, -0x1.fd88dap-1f, -0x1.f6297cp-1f, -0x1.e9f416p-1f, -0x1.d906bcp-1f, -0x1.c38b3p-1f, -0x1.a9b662p-1f, -0x1.8bc806p-1f, -0x1.6a09e6p-1f, -0x1.44cf32p-1f, -0x1.1c73b4p-1f, -0x1.e2b5d4p-2f, -0x1.87de2ap-2f, -0x1.294062p-2f, -0x1.8f8b84p-3f, -0x1.917a6cp-4f, -0x1.1a6264p-53f, 0x1.917a6cp-4f, 0x1.8f8b84p-3f, 0x1.294062p-2f, 0x1.87de2ap-2f, 0x1.e2b5d4p-2f, 0x1.1c73b4p-1f, 0x1.44cf32p-1f, 0x1.6a09e6p-1f, 0x1.8bc806p-1f, 0x1.a9b662p-1f, 0x1.c38b3p-1f, 0x1.d906bcp-1f, 0x1.e9f416p-1f, 0x1.f6297cp-1f, 0x1.fd88dap-1f, 0x1p+0f, 0x1.fd88dap-1f, 0x1.f6297cp-1f, 0x1.e9f416p-1f, 0x1.d906bcp-1f, 0x1.c38b3p-1f, 0x1.a9b662p-1f, 0x1.8bc806p-1f, 0x1.6a09e6p-1f, 0x1.44cf32p-1f, 0x1.1c73b4p-1f, 0x1.e2b5d4p-2f, 0x1.87de2ap-2f, 0x1.294062p-2f, 0x1.8f8b84p-3f, 0x1.917a6cp-4f};
 // exp(+2 pi i j / 128), j = 0..127
 extern __constant__ float2 c_tw128[128];
 
@@ -28,7 +30,7 @@ __device__ inline float2 cmul_f(float2 a, float2 b)
 template <int SIGN>
 __device__ inline float2 tw(int k, int len)
 {
-    float2 w = c_tw64[k * (64 / len)];
+    float2 w = make_float2(kTw64Re[k * (64 / len)], kTw64Im[k * (64 / len)]);
     if (SIGN > 0)
         w.y = -w.y;
     return w;
@@ -90,30 +92,37 @@ __device__ inline float2 shfl_pair(float2 v)
 }
 
 // Forward W-point FFT over a lane pair.  Lane h holds samples 2m + h
-// (m = 0..W/2-1); on return it holds outputs h*W/2 + k.
+// (m = 0..W/2-1); on return it holds outputs h*W/2 + k.  The odd lane
+// applies the twiddle to its own half before the exchange (the even lane
+// multiplies by 1, exactly), so both lanes finish with one sign-selected
+// add: out = partner + (h ? -1 : 1) * own.  The results are bit-identical
+// to forming E + tw O and E - tw O after the exchange.
 template <int W>
 __device__ inline void fwd_pair(float2 (&x)[W / 2], int h)
 {
     constexpr int H = W / 2;
     fft<H, -1>(x);
+    const float hf = (float)h, sg = h ? -1.f : 1.f;
 #pragma unroll
     for (int k = 0; k < H; k++) {
-        const float2 y = shfl_pair(x[k]);
-        const float2 E = h ? y : x[k];
-        const float2 O = h ? x[k] : y;
         float2 t;
-        if (k == 0)
-            t = O;
-        else if (4 * k == W)
-            t = make_float2(O.y, -O.x);
-        else
-            t = cmul_f(O, tw<-1>(k, W));
-        x[k] = h ? make_float2(E.x - t.x, E.y - t.y) : make_float2(E.x + t.x, E.y + t.y);
+        if (k == 0) {
+            t = x[k];
+        } else if (4 * k == W) {
+            t = h ? make_float2(x[k].y, -x[k].x) : x[k];
+        } else {
+            const float2 w = tw<-1>(k, W);
+            t = cmul_f(x[k], make_float2(h ? w.x : 1.f, hf * w.y));
+        }
+        const float2 y = shfl_pair(t);
+        x[k] = make_float2(fmaf(sg, t.x, y.x), fmaf(sg, t.y, y.y));
     }
 }
 
 // Inverse (exp(+2 pi i jk/W)) W-point FFT over a lane pair.  Lane h holds
-// inputs h*W/2 + n; on return it holds outputs 2k + h.
+// inputs h*W/2 + n; on return it holds outputs 2k + h.  (Forming the
+// butterfly as in fwd_pair, one sign-selected add and a per-lane twiddle,
+// raised register pressure here: 72 VGPRs spilled at W = 64.)
 template <int W>
 __device__ inline void inv_pair(float2 (&x)[W / 2], int h)
 {

@@ -30,7 +30,6 @@ namespace sttrack {
 __constant__ double2 c_dir96[193];
 }
 namespace fftreg {
-__constant__ float2 c_tw64[64];
 __constant__ float2 c_tw128[128];
 }  // namespace fftreg
 
@@ -1403,12 +1402,9 @@ void init_dirichlet(st_ctx *ctx)
         tab[t + 96] = make_double2(re, im);
     }
     ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dir96), tab, sizeof(tab)));
-    float2 t64[64], t128[128];
-    for (int j = 0; j < 64; j++)
-        t64[j] = make_float2((float)cos(-2.0 * M_PI * j / 64.0), (float)sin(-2.0 * M_PI * j / 64.0));
+    float2 t128[128];
     for (int j = 0; j < 128; j++)
         t128[j] = make_float2((float)cos(2.0 * M_PI * j / 128.0), (float)sin(2.0 * M_PI * j / 128.0));
-    ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fftreg::c_tw64), t64, sizeof(t64)));
     ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fftreg::c_tw128), t128, sizeof(t128)));
     done[ctx->device] = true;
 }
