@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "st_ctx.h"
+#include "st_fftlds.h"
 #include "st_track_common.h"
 #include "st_fftreg.h"
 
@@ -220,81 +221,21 @@ __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *lis
     }
 }
 
-// cmpPSWithPad (speckleTrack.c:604-643): F1 conj(F2) into the four corners
-// of the 2w x 2w spectrum, azimuth rows rotated by azShift, Nyquist on the
-// negative side.
-__global__ void k_cpx_pad(const float2 *P, int B, int w, const int *az, float2 *S)
-{
-    const int W = 2 * w, h = w / 2;
-    const int b = blockIdx.y;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= W * W)
-        return;
-    const int n = e / W, m = e - n * W;
-    float2 v = make_float2(0.f, 0.f);
-    const int k = n < h ? n : (n >= W - h ? n - W : 1 << 20);
-    const int l = m < h ? m : (m >= W - h ? m - W : 1 << 20);
-    if (k != (1 << 20) && l != (1 << 20)) {
-        const int ka = (k + w + az[b]) % w, la = (l + w) % w;
-        const float2 f1 = P[(size_t)b * w * w + ka * w + la];
-        const float2 f2 = P[((size_t)B + b) * w * w + ka * w + la];
-        v = cmulconj(f1, f2);
-    }
-    S[(size_t)b * W * W + e] = v;
-}
 
 // ---------------------------------------------------------------------
 // Peak stage (findCPeak speckleTrack.c:744-805, overSampleC :852-863,
 // argmax over the central box :686-700, sample extraction for
 // gaussPeak1D :2158-2180).  mode 0 = complex (Gaussian fits at the
 // refined peak / NOVER), 1 = amplitude (Gaussian fits at the coarse peak).
-__global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W, int mode, PeakRes *res)
+// The part of the peak stage after the coarse scan: border check, FP64
+// refinement of the 8 x 8 neighbourhood on the searched 48 x 48 box, and the
+// samples of the two Gaussian fits, from the fftshifted magnitude surface M.
+__device__ void peak_finish(const float *M, int W, int mode, int I, int J, float raw, float sec, PeakRes *out)
 {
-    __shared__ ArgMax sha[NT / 64];
     __shared__ ArgMaxD shd[NT / 64];
-    __shared__ float shf[NT / 64];
     __shared__ double shdd[NT / 64];
     __shared__ double Pm[NFAST * NFAST];
     __shared__ double2 T[NFAST * kFine];
-    const int b = blockIdx.x;
-    const float2 *C = S + (size_t)b * W * W;
-    float *M = mag + (size_t)b * W * W;
-    const int H = W / 2;
-    ArgMax best{0.0f, 0x7fffffff};
-    for (int e = threadIdx.x; e < W * W; e += NT) {
-#pragma clang fp contract(off)
-        const int n = e / W, m = e - n * W;
-        const float2 z = C[e];
-        const float x2 = z.x * z.x + z.y * z.y;
-        const float v = sqrtf(x2);
-        const int I = n < H ? n + H : n - H, J = m < H ? m + H : m - H;
-        M[I * W + J] = v;
-        const int i = I < H ? I : I - H, j = J < H ? J : J - H;
-        const int q = (I < H) ? (J < H ? 0 : 3) : (J < H ? 2 : 1);
-        const int rank = (i * H + j) * 4 + q;
-        if (v > 0.0f) {
-            ArgMax c{v, rank};
-            best = argmax_combine(best, c);
-        }
-    }
-    best = block_argmax<NT>(best, sha);
-    __syncthreads();
-    int I = -1000, J = -1000;
-    float raw = 0.f;
-    if (best.rank != 0x7fffffff) {
-        const int q = best.rank & 3, ij = best.rank >> 2;
-        const int i = ij / H, j = ij - (ij / H) * H;
-        I = (q == 0 || q == 3) ? i : i + H;
-        J = (q == 0 || q == 2) ? j : j + H;
-        raw = best.v;
-    }
-    // runner-up on the coarse surface
-    float sec = 0.f;
-    const int bestIdx = (I >= 0) ? I * W + J : -1;
-    for (int e = threadIdx.x; e < W * W; e += NT)
-        if (e != bestIdx)
-            sec = fmaxf(sec, M[e]);
-    sec = block_max_f<NT>(sec, shf);
     PeakRes r;
     r.gap = raw > 0 ? (raw - sec) / raw : 0.f;
     r.lmOk = 0;
@@ -305,7 +246,7 @@ __global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W,
         r.I = r.J = -1000;
         r.raw = -1.f;
         if (threadIdx.x == 0)
-            res[b] = r;
+            *out = r;
         return;
     }
     r.I = I;
@@ -386,7 +327,55 @@ __global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W,
         r.lmX0[1] = (float)(cI - NGAUSS / 2);
     }
     if (threadIdx.x == 0)
-        res[b] = r;
+        *out = r;
+}
+
+
+__global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W, int mode, PeakRes *res)
+{
+    __shared__ ArgMax sha[NT / 64];
+    __shared__ float shf[NT / 64];
+    const int b = blockIdx.x;
+    const float2 *C = S + (size_t)b * W * W;
+    float *M = mag + (size_t)b * W * W;
+    const int H = W / 2;
+    ArgMax best{0.0f, 0x7fffffff};
+    for (int e = threadIdx.x; e < W * W; e += NT) {
+#pragma clang fp contract(off)
+        const int n = e / W, m = e - n * W;
+        const float2 z = C[e];
+        const float x2 = z.x * z.x + z.y * z.y;
+        const float v = sqrtf(x2);
+        const int I = n < H ? n + H : n - H, J = m < H ? m + H : m - H;
+        M[I * W + J] = v;
+        const int i = I < H ? I : I - H, j = J < H ? J : J - H;
+        const int q = (I < H) ? (J < H ? 0 : 3) : (J < H ? 2 : 1);
+        const int rank = (i * H + j) * 4 + q;
+        if (v > 0.0f) {
+            ArgMax c{v, rank};
+            best = argmax_combine(best, c);
+        }
+    }
+    best = block_argmax<NT>(best, sha);
+    __syncthreads();
+    int I = -1000, J = -1000;
+    float raw = 0.f;
+    if (best.rank != 0x7fffffff) {
+        const int q = best.rank & 3, ij = best.rank >> 2;
+        const int i = ij / H, j = ij - (ij / H) * H;
+        I = (q == 0 || q == 3) ? i : i + H;
+        J = (q == 0 || q == 2) ? j : j + H;
+        raw = best.v;
+    }
+    // runner-up on the coarse surface
+    float sec = 0.f;
+    const int bestIdx = (I >= 0) ? I * W + J : -1;
+    for (int e = threadIdx.x; e < W * W; e += NT)
+        if (e != bestIdx)
+            sec = fmaxf(sec, M[e]);
+    sec = block_max_f<NT>(sec, shf);
+    PeakRes *out = res + b;
+    peak_finish(M, W, mode, I, J, raw, sec, out);
 }
 
 // Two fits per attempt: [0] along range, [1] along azimuth.
@@ -937,6 +926,176 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
     C.az[it] = (int)((float)shift * (float)P.wA / (float)W);
     C.p[2 * it] = ps[0] / cnt;
     C.p[2 * it + 1] = ps[1] / cnt;
+}
+
+// ---------------------------------------------------------------------
+// Staged complex path in two passes (windows outside the fused kernels'
+// sizes): the forward range transforms of both patches are an engine row
+// pass; k_cpx_colx and k_cpx_rowmag then do the rest of cmpPSWithPad
+// (speckleTrack.c:604-643) and the coarse scan of cmpTrackFast (:648-709)
+// without forming the 2w x 2w padded spectrum: the zero pad is separable,
+// so the azimuth half (forward azimuth transforms, F1 conj(F2), rotation by
+// the Doppler shift, pad, inverse) runs on spectrum columns and the range
+// half (pad, inverse, |C|) on rows.
+constexpr int kStElems = 4096;  // elements per workgroup (LDS and register staging)
+
+// Azimuth half: C adjacent spectrum columns la of one patch per workgroup;
+// out T[b][n][la], n < 2w.  Workgroups are numbered so that the column
+// blocks sharing cache lines run on the same XCD.
+__global__ void __launch_bounds__(NT) k_cpx_colx(const float2 *P, int B, int w, const int *az, float2 *T, int lc,
+                                                  stfft::PlanArgs Pf, stfft::PlanArgs Pi, stfft::FastDiv dW)
+{
+    extern __shared__ float2 ldsx[];
+    constexpr int IT = kStElems / NT;
+    const int C = 1 << lc, W = 2 * w, h = w / 2;
+    const int nbx = gridDim.x, total = nbx * gridDim.y, flat = blockIdx.y * nbx + blockIdx.x;
+    const int per = total / 8;
+    const int Lb = flat < per * 8 ? (flat % 8) * per + flat / 8 : flat;
+    const int b = Lb / nbx, c0 = (Lb - (Lb / nbx) * nbx) * C;
+    const int nc = imin(C, w - c0);
+    const int px = W + 1, py = w + 1;
+    float2 *X = ldsx, *Y = ldsx + (size_t)C * px;  // [C][px] F1 then the padded column, [C][py] F2
+    const float2 *P1 = P + (size_t)b * w * w + c0, *P2 = P + ((size_t)B + b) * w * w + c0;
+    {
+        const int cnt = w << lc;
+        float2 u[IT], v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1);
+            const bool ok = t < cnt && c < nc;
+            u[it] = ok ? P1[(size_t)a * w + c] : make_float2(0.f, 0.f);
+            v[it] = ok ? P2[(size_t)a * w + c] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1);
+            if (t < cnt) {
+                X[c * px + a] = u[it];
+                Y[c * py + a] = v[it];
+            }
+        }
+    }
+    __syncthreads();
+    stfft::lds_fft<kStElems, NT>(X, px, C, Pf, -1);
+    stfft::lds_fft<kStElems, NT>(Y, py, C, Pf, -1);
+    {
+        const int azb = az[b], cnt = C * W;
+        float2 v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            v[it] = make_float2(0.f, 0.f);
+            if (t < cnt) {
+                const int c = stfft::fdiv(t, dW), n = t - c * W;
+                const int k = n < h ? n : (n >= W - h ? n - W : (1 << 20));
+                if (k != (1 << 20)) {
+                    int ka = k + w + azb;  // in [0, 3w)
+                    ka -= ka >= w ? w : 0;
+                    ka -= ka >= w ? w : 0;
+                    v[it] = cmulconj(X[c * px + ka], Y[c * py + ka]);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            if (t < cnt) {
+                const int c = stfft::fdiv(t, dW), n = t - c * W;
+                X[c * px + n] = v[it];
+            }
+        }
+        __syncthreads();
+    }
+    stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
+    float2 *Tb = T + (size_t)b * W * w + c0;
+    const int cnt = W << lc;
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int t = threadIdx.x + it * NT, n = t >> lc, c = t & (C - 1);
+        if (t < cnt && c < nc)
+            Tb[(size_t)n * w + c] = X[c * px + n];
+    }
+}
+
+// Range half: R rows of one patch per workgroup: the w range frequencies
+// of T spread over 2w (Nyquist on the negative side, zeros between),
+// inverse transform, |C| into the fftshifted surface M, and this block's
+// top two in findCPeak's scan-rank order (speckleTrack.c:744-805).
+__global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R, float *mag, Top2 *part,
+                                                    stfft::PlanArgs Pi, stfft::FastDiv dW)
+{
+    extern __shared__ float2 ldsx[];
+    __shared__ Top2 sht[NT / 64];
+    constexpr int IT = kStElems / NT;
+    const int W = 2 * w, h = w / 2, H = w;
+    const int b = blockIdx.y, n0 = blockIdx.x * R, nrow = imin(R, W - n0);
+    const float2 *Tb = T + ((size_t)b * W + n0) * w;
+    const int cnt = nrow * W;
+    {
+        float2 v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            v[it] = make_float2(0.f, 0.f);
+            if (t < cnt) {
+                const int i = stfft::fdiv(t, dW), m = t - i * W;
+                const int la = m < h ? m : (m >= W - h ? m - w : -1);
+                if (la >= 0)
+                    v[it] = Tb[(size_t)i * w + la];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            if (t < cnt)
+                ldsx[t] = v[it];
+        }
+    }
+    __syncthreads();
+    stfft::lds_fft<kStElems, NT>(ldsx, W, nrow, Pi, +1);
+    float *M = mag + (size_t)b * W * W;
+    Top2 best{0.0f, 0x7fffffff, 0.0f};
+    for (int t = threadIdx.x; t < cnt; t += NT) {
+#pragma clang fp contract(off)
+        const int i = stfft::fdiv(t, dW), m = t - i * W, n = n0 + i;
+        const float2 z = ldsx[t];
+        const float x2 = z.x * z.x + z.y * z.y;
+        const float v = sqrtf(x2);
+        const int I = n < H ? n + H : n - H, J = m < H ? m + H : m - H;
+        M[(size_t)I * W + J] = v;
+        const int i0 = I < H ? I : I - H, j0 = J < H ? J : J - H;
+        const int q = (I < H) ? (J < H ? 0 : 3) : (J < H ? 2 : 1);
+        const int rank = (i0 * H + j0) * 4 + q;
+        if (v > 0.0f)
+            best = top2_combine(best, Top2{v, rank, 0.0f});
+        else
+            best.v2 = fmaxf(best.v2, v);
+    }
+    best = wg_top2<NT>(best, sht);
+    if (threadIdx.x == 0)
+        part[(size_t)b * gridDim.x + blockIdx.x] = best;
+}
+
+// Top two over the row blocks of each patch (in block order), then the
+// refinement and the Gaussian-fit samples of k_peak.
+__global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, const float *mag, int W, int mode,
+                                                    PeakRes *res)
+{
+    const int b = blockIdx.x, H = W / 2;
+    Top2 best = part[(size_t)b * nPart];
+    for (int k = 1; k < nPart; k++)
+        best = top2_combine(best, part[(size_t)b * nPart + k]);
+    int I = -1000, J = -1000;
+    float raw = 0.f;
+    if (best.r1 != 0x7fffffff) {
+        const int q = best.r1 & 3, ij = best.r1 >> 2;
+        const int i = ij / H, j = ij - (ij / H) * H;
+        I = (q == 0 || q == 3) ? i : i + H;
+        J = (q == 0 || q == 2) ? j : j + H;
+        raw = best.v1;
+    }
+    peak_finish(mag + (size_t)b * W * W, W, mode, I, J, raw, best.v2, res + b);
 }
 
 // Lanes per transform of the fused kernel.  Lane pairs at every window
@@ -1837,20 +1996,31 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         C.have = ar.take<uint8_t>(B);
         C.az = ar.take<int>(B);
         C.p = ar.take<double>(2 * (size_t)B);
+        // blocking of the two passes after the range transforms: Cx spectrum
+        // columns per k_cpx_colx workgroup, Rr rows per k_cpx_rowmag workgroup
+        const int W2 = 2 * w;
+        ST_REQUIRE(W2 <= kStElems, ST_ERR_UNSUPPORTED, "window too large for the staged complex path");
+        int lcx = 0;
+        while ((2 << lcx) <= 16 && (2 << lcx) * W2 <= kStElems)
+            lcx++;
+        const int Cx = 1 << lcx, Rr = kStElems / W2, nPart = (W2 + Rr - 1) / Rr;
+        const size_t ldsX = (size_t)Cx * ((W2 + 1) + (w + 1)) * sizeof(float2), ldsR = (size_t)Rr * W2 * sizeof(float2);
+        const stfft::PlanArgs PfW = ctx->fft.plan(w, -1), PiW2 = ctx->fft.plan(W2, +1);
+        const stfft::FastDiv dW2 = stfft::make_fastdiv((uint32_t)W2);
+        Top2 *part = (Top2 *)C.tmp;  // [B][nPart]
         for (int off = 0; off < nCpx; off += B) {
             const int nb = std::min(B, nCpx - off);
             ctx->time_begin(kGather);
             hipLaunchKernelGGL(k_cpx_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, C);
             ctx->time_end(kGather);
             ctx->time_begin(kFft);
-            ctx->fft.fft2d(C.P, C.tmp, 2 * nb, w, w, -1, st);
-            ctx->time_end(kFft);
-            hipLaunchKernelGGL(k_cpx_pad, dim3((4 * w * w + 255) / 256, nb), dim3(256), 0, st, C.P, nb, w, C.az, C.S);
-            ctx->time_begin(kFft);
-            ctx->fft.fft2d(C.S, C.tmp, nb, 2 * w, 2 * w, +1, st);
+            ctx->fft.rows(C.P, (long)2 * nb * w, w, -1, st);
             ctx->time_end(kFft);
             ctx->time_begin(kPeak);
-            hipLaunchKernelGGL(k_peak, dim3(nb), dim3(NT), 0, st, C.S, C.mag, 2 * w, 0, C.res);
+            hipLaunchKernelGGL(k_cpx_colx, dim3((w + Cx - 1) / Cx, nb), dim3(NT), ldsX, st, C.P, nb, w, C.az, C.S, lcx,
+                               PfW, PiW2, dW2);
+            hipLaunchKernelGGL(k_cpx_rowmag, dim3(nPart, nb), dim3(NT), ldsR, st, C.S, w, Rr, C.mag, part, PiW2, dW2);
+            hipLaunchKernelGGL(k_peak_merge, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0, C.res);
             ctx->time_end(kPeak);
             hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, C.res, nb, C.sig);
             hipLaunchKernelGGL(k_cpx_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, C);
