@@ -144,6 +144,16 @@ const char *st_last_error(st_ctx *ctx);
 int st_pair_upload(st_ctx *ctx, const void *img1, int32_t nr1, int32_t na1,
                    const void *img2, int32_t nr2, int32_t na2,
                    int32_t sample_type, int32_t *pair);
+/* The same from samples as the reference's files store them: c64 samples
+ * byte-swapped on the GPU when byteOrder is ST_MSB, ci16 samples always
+ * (loadBuffer reads int16 SLCs big-endian whatever -LSB says,
+ * speckleTrack.c:1380-1390).  The swap is an HBM-bound pass over the
+ * uploaded pair. */
+#define ST_LSB 0
+#define ST_MSB 1
+int st_pair_upload_raw(st_ctx *ctx, const void *img1, int32_t nr1, int32_t na1,
+                       const void *img2, int32_t nr2, int32_t na2,
+                       int32_t sample_type, int32_t byteOrder, int32_t *pair);
 int st_pair_release(st_ctx *ctx, int32_t pair);
 /* Copy rows [row0, row0+nrows) of both images of a resident pair back to
  * the host (complex float32). */

@@ -1,0 +1,86 @@
+/*
+ * speckle_io.h -- the files either side of the tracking path: offset
+ * products written after strack / strackw, the cullst inputs read from
+ * them, and the cullst products.  Host code only (no GPU call); part of
+ * libspeckle_amd.so.
+ *
+ * Raw rasters are row-major nA x nR: float32 (.dr .da .cc .azd .sr .sa) in
+ * the requested byte order (ST_MSB, the reference default, or ST_LSB) and
+ * bytes (.mt).  Text headers (.dat) use the reference's printf formats.
+ * VRT files are GDAL VRTRawRasterBand descriptions of the same rasters.
+ *
+ * Every function returns ST_OK or a negative ST_ERR_* code;
+ * st_io_last_error() gives the message of the calling thread's last
+ * failure.
+ */
+#ifndef SPECKLE_IO_H
+#define SPECKLE_IO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "speckle_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ST_ERR_IO (-6) /* file could not be opened, read or written */
+
+/* byte orders: ST_LSB / ST_MSB of speckle_amd.h (Cullst/writeCullData.c:8-9) */
+
+/* What the offset headers record (TrackParams fields, Strack/strack.h). */
+typedef struct st_offsets_meta {
+    int32_t rStart, aStart, deltaR, deltaA, scaleFactor;
+    int32_t wR, wA, wRa, wAa;
+    const char *intGeodat;  /* geodat of image 1 */
+    const char *imageFile1, *imageFile2, *maskFile;
+} st_offsets_meta;
+
+const char *st_io_last_error(void);
+
+/* stem.dr .da .cc (.azd when azd != NULL) .mt, stem.dat, stem.vrt (bands
+ * RangeOffsets, AzimuthOffsets, Correlation, no-data -2e9, byteOrder) and
+ * stem.mt.vrt (MatchType).  Replaces sTrackOut / writeOffsets
+ * (Strack/sTrackOut.c:11-60), writeDatFile (Strack/speckleTrack.c:1453-1468,
+ * Strackw/corrTrackFast.c:35) and writeVrtFile (Strack/writeVrt.c:16-70).
+ * strackw passes azd = NULL (it writes no .azd). */
+int st_write_offsets(const char *stem, const st_offsets_meta *meta, const float *offR, const float *offA,
+                     const float *corr, const float *azd, const uint8_t *type, int32_t nR, int32_t nA,
+                     int32_t byteOrder);
+
+/* First line of a .dat header ("%i%i%i%i%i%i": r0 a0 nr na deltaR deltaA)
+ * into hdr[6]; when the second line names geodat files, its two words into
+ * geo1 / geo2 (each geoLen bytes, "" otherwise).  loadCullData's .dat path
+ * (Cullst/loadCullData.c:154-168). */
+int st_read_dat(const char *datFile, int32_t hdr[6], char *geo1, char *geo2, size_t geoLen);
+
+/* stem.dr .da .cc .mt (nA x nR, byteOrder) into caller arrays, with the
+ * loader's rule that a point whose azimuth offset is below -LARGEINT + 1
+ * has match type 0 (Cullst/loadCullData.c:184-197). */
+int st_read_offsets(const char *stem, int32_t nR, int32_t nA, int32_t byteOrder, float *offR, float *offA,
+                    float *corr, uint8_t *type);
+
+/* What the cullst product headers record (CullParams, Cullst/cullst.h). */
+typedef struct st_cull_meta {
+    int32_t rStart, aStart, deltaR, deltaA;
+    int32_t sR, sA, boxSize, nGood, singleMT;
+    float maxR, maxA, corrThresh;
+    const char *geo1, *geo2; /* NULL: not recorded */
+} st_cull_meta;
+
+/* writeCullData (Cullst/writeCullData.c:15-81) with the file names of
+ * cullst.c:240-300: outBase.dr .da .cc .mt, inBase.sr .sa (the reference
+ * names the sigma files after the input), outBase.vrt (RangeOffsets,
+ * AzimuthOffsets, RangeSigma, AzimuthSigma, Correlation) and
+ * outBase.mt.vrt.  Header metadata as populateMetaData
+ * (Cullst/loadCullData.c:95-118). */
+int st_write_cull(const char *outBase, const char *inBase, const st_cull_meta *meta, const float *offRS,
+                  const float *offAS, const float *sigmaR, const float *sigmaA, const float *corr,
+                  const uint8_t *type, int32_t nR, int32_t nA, int32_t byteOrder);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPECKLE_IO_H */

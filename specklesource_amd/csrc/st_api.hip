@@ -127,8 +127,39 @@ extern "C" const char *st_last_error(st_ctx *c)
     return c ? c->err.c_str() : "null context";
 }
 
-extern "C" int st_pair_upload(st_ctx *c, const void *img1, int32_t nr1, int32_t na1, const void *img2, int32_t nr2,
-                              int32_t na2, int32_t stype, int32_t *pair)
+namespace {
+
+// In-place byte swap of an uploaded image: 32-bit words (c64 samples) or
+// the two 16-bit halves of each word (ci16 samples); 16 bytes per lane.
+__global__ void k_bswap(uint4 *d, size_t n16, uint32_t *tail, size_t nTail, int half)
+{
+    auto sw = [half](uint32_t x) {
+        return half ? (((x & 0x00FF00FFu) << 8) | ((x >> 8) & 0x00FF00FFu)) : __builtin_bswap32(x);
+    };
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
+        uint4 v = d[i];
+        v.x = sw(v.x);
+        v.y = sw(v.y);
+        v.z = sw(v.z);
+        v.w = sw(v.w);
+        d[i] = v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < nTail)
+        tail[threadIdx.x] = sw(tail[threadIdx.x]);
+}
+
+void bswap_device(void *d, size_t bytes, int half, hipStream_t st)
+{
+    const size_t n16 = bytes / 16, nTail = (bytes - n16 * 16) / 4;
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>((n16 + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_bswap, dim3(blocks), dim3(256), 0, st, (uint4 *)d, n16, (uint32_t *)((char *)d + n16 * 16),
+                       nTail, half);
+    ST_HIP(hipGetLastError());
+}
+
+int upload_pair(st_ctx *c, const void *img1, int32_t nr1, int32_t na1, const void *img2, int32_t nr2, int32_t na2,
+                int32_t stype, int swap, int32_t *pair)
 {
     return st_guard(c, [&]() {
         ST_REQUIRE(img1 && img2 && pair && nr1 > 0 && na1 > 0 && nr2 > 0 && na2 > 0, ST_ERR_ARG, "bad pair arguments");
@@ -145,11 +176,38 @@ extern "C" int st_pair_upload(st_ctx *c, const void *img1, int32_t nr1, int32_t 
         ST_HIP(hipMalloc(&p.d2, b2));
         ST_HIP(hipMemcpyAsync(p.d1, img1, b1, hipMemcpyHostToDevice, c->stream));
         ST_HIP(hipMemcpyAsync(p.d2, img2, b2, hipMemcpyHostToDevice, c->stream));
+        if (swap) {
+            const int half = stype == ST_SAMPLE_CI16;
+            c->stats_reset();
+            const int slot = c->kernel_slot("k_bswap");
+            c->time_begin(slot);
+            bswap_device(p.d1, b1, half, c->stream);
+            bswap_device(p.d2, b2, half, c->stream);
+            c->time_end(slot);
+        }
         ST_HIP(hipStreamSynchronize(c->stream));
+        c->flush_timings();
         int id = c->nextPair++;
         c->pairs[id] = p;
         *pair = id;
     });
+}
+
+}  // namespace
+
+extern "C" int st_pair_upload(st_ctx *c, const void *img1, int32_t nr1, int32_t na1, const void *img2, int32_t nr2,
+                              int32_t na2, int32_t stype, int32_t *pair)
+{
+    return upload_pair(c, img1, nr1, na1, img2, nr2, na2, stype, 0, pair);
+}
+
+extern "C" int st_pair_upload_raw(st_ctx *c, const void *img1, int32_t nr1, int32_t na1, const void *img2,
+                                  int32_t nr2, int32_t na2, int32_t stype, int32_t byteOrder, int32_t *pair)
+{
+    if (byteOrder != ST_MSB && byteOrder != ST_LSB)
+        return st_guard(c, [&]() { ST_REQUIRE(false, ST_ERR_ARG, "byte order"); });
+    const int swap = stype == ST_SAMPLE_CI16 || byteOrder == ST_MSB;
+    return upload_pair(c, img1, nr1, na1, img2, nr2, na2, stype, swap, pair);
 }
 
 extern "C" int st_pair_release(st_ctx *c, int32_t id)

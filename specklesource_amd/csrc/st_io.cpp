@@ -1,0 +1,378 @@
+// Offset and cull products on disk (see speckle_io.h).  Host code only.
+//
+// Raw rasters are written in chunks through a byte-swapping staging
+// buffer; MSB float32 is the big-endian encoding of each value, which is
+// what the reference's fwriteBS produces on a little-endian host
+// (Strack/sTrackOut.c:11-15).  Text headers use the reference's printf
+// formats; the VRT files are GDAL VRTRawRasterBand descriptions with the
+// metadata items the reference records, in the order it inserts them.
+#include <libgen.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "speckle_io.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct IoError {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw IoError{code, msg}; }
+
+template <class F>
+int io_guard(F &&f)
+{
+    try {
+        f();
+        return ST_OK;
+    } catch (const IoError &e) {
+        g_err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return ST_ERR_IO;
+    }
+}
+
+// printf("%s", s) of the reference, where a missing string prints as glibc does
+std::string str_or_null(const char *s) { return s ? std::string(s) : std::string("(null)"); }
+
+std::string fmt(const char *f, ...) __attribute__((format(printf, 1, 2)));
+std::string fmt(const char *f, ...)
+{
+    char buf[4096];
+    va_list ap;
+    va_start(ap, f);
+    vsnprintf(buf, sizeof buf, f, ap);
+    va_end(ap);
+    return buf;
+}
+
+std::string dir_of(const std::string &path)
+{
+    std::vector<char> b(path.begin(), path.end());
+    b.push_back('\0');
+    return dirname(b.data());
+}
+
+std::string base_of(const std::string &path)
+{
+    std::vector<char> b(path.begin(), path.end());
+    b.push_back('\0');
+    return basename(b.data());
+}
+
+struct File {
+    FILE *fp = nullptr;
+    std::string name;
+    File(const std::string &n, const char *mode) : name(n)
+    {
+        fp = fopen(n.c_str(), mode);
+        if (!fp)
+            fail(ST_ERR_IO, "cannot open " + n);
+    }
+    ~File()
+    {
+        if (fp)
+            fclose(fp);
+    }
+    void close()
+    {
+        if (fp && fclose(fp) != 0) {
+            fp = nullptr;
+            fail(ST_ERR_IO, "error closing " + name);
+        }
+        fp = nullptr;
+    }
+};
+
+void write_f32(const std::string &name, const float *v, size_t n, int order)
+{
+    File f(name, "wb");
+    constexpr size_t kChunk = 1 << 16;
+    std::vector<uint32_t> buf(kChunk);
+    for (size_t o = 0; o < n; o += kChunk) {
+        const size_t m = std::min(kChunk, n - o);
+        memcpy(buf.data(), v + o, m * 4);
+        if (order == ST_MSB)
+            for (size_t k = 0; k < m; k++)
+                buf[k] = __builtin_bswap32(buf[k]);
+        if (fwrite(buf.data(), 4, m, f.fp) != m)
+            fail(ST_ERR_IO, "short write to " + name);
+    }
+    f.close();
+}
+
+void write_u8(const std::string &name, const uint8_t *v, size_t n)
+{
+    File f(name, "wb");
+    if (n && fwrite(v, 1, n, f.fp) != n)
+        fail(ST_ERR_IO, "short write to " + name);
+    f.close();
+}
+
+void read_f32(const std::string &name, float *v, size_t n, int order)
+{
+    File f(name, "rb");
+    if (fread(v, 4, n, f.fp) != n)
+        fail(ST_ERR_IO, "short read from " + name);
+    if (order == ST_MSB) {
+        uint32_t *u = reinterpret_cast<uint32_t *>(v);
+        for (size_t k = 0; k < n; k++)
+            u[k] = __builtin_bswap32(u[k]);
+    }
+}
+
+void read_u8(const std::string &name, uint8_t *v, size_t n)
+{
+    File f(name, "rb");
+    if (n && fread(v, 1, n, f.fp) != n)
+        fail(ST_ERR_IO, "short read from " + name);
+}
+
+std::string xml_escape(const std::string &s)
+{
+    std::string o;
+    for (char c : s) {
+        switch (c) {
+        case '&': o += "&amp;"; break;
+        case '<': o += "&lt;"; break;
+        case '>': o += "&gt;"; break;
+        case '"': o += "&quot;"; break;
+        default: o += c;
+        }
+    }
+    return o;
+}
+
+struct Band {
+    std::string file, name;
+    bool f32;
+};
+
+// A band file beside the VRT is named relative to it; one elsewhere keeps
+// the path it was given.
+std::string source_element(const std::string &vrtFile, const std::string &file)
+{
+    if (dir_of(file) == dir_of(vrtFile))
+        return "    <SourceFilename relativeToVRT=\"1\">" + xml_escape(base_of(file)) + "</SourceFilename>\n";
+    return "    <SourceFilename relativeToVRT=\"0\">" + xml_escape(file) + "</SourceFilename>\n";
+}
+
+typedef std::vector<std::pair<std::string, std::string>> Meta;
+
+// One GDAL VRT of raw rasters, all nA x nR, side by side with vrtFile.
+void write_vrt(const std::string &vrtFile, int32_t nR, int32_t nA, const Meta &meta, const std::vector<Band> &bands,
+               int order, bool noData)
+{
+    std::string x = fmt("<VRTDataset rasterXSize=\"%d\" rasterYSize=\"%d\">\n", nR, nA);
+    x += "  <Metadata>\n";
+    for (const auto &kv : meta)
+        x += "    <MDI key=\"" + xml_escape(kv.first) + "\">" + xml_escape(kv.second) + "</MDI>\n";
+    x += "  </Metadata>\n";
+    for (size_t b = 0; b < bands.size(); b++) {
+        const Band &B = bands[b];
+        x += fmt("  <VRTRasterBand dataType=\"%s\" band=\"%d\" subClass=\"VRTRawRasterBand\">\n",
+                 B.f32 ? "Float32" : "Byte", (int)b + 1);
+        x += "    <Description>" + xml_escape(B.name) + "</Description>\n";
+        if (noData)
+            x += "    <NoDataValue>-2000000000</NoDataValue>\n";
+        x += source_element(vrtFile, B.file);
+        x += "    <ImageOffset>0</ImageOffset>\n";
+        x += fmt("    <PixelOffset>%d</PixelOffset>\n", B.f32 ? 4 : 1);
+        x += fmt("    <LineOffset>%lld</LineOffset>\n", (long long)nR * (B.f32 ? 4 : 1));
+        if (B.f32)
+            x += std::string("    <ByteOrder>") + (order == ST_MSB ? "MSB" : "LSB") + "</ByteOrder>\n";
+        x += "  </VRTRasterBand>\n";
+    }
+    x += "</VRTDataset>\n";
+    File f(vrtFile, "w");
+    if (fwrite(x.data(), 1, x.size(), f.fp) != x.size())
+        fail(ST_ERR_IO, "short write to " + vrtFile);
+    f.close();
+}
+
+void check_grid(int32_t nR, int32_t nA, int32_t order)
+{
+    if (nR <= 0 || nA <= 0)
+        fail(ST_ERR_ARG, "grid size");
+    if (order != ST_MSB && order != ST_LSB)
+        fail(ST_ERR_ARG, "byte order");
+}
+
+}  // namespace
+
+extern "C" const char *st_io_last_error(void) { return g_err.c_str(); }
+
+extern "C" int st_write_offsets(const char *stem, const st_offsets_meta *m, const float *offR, const float *offA,
+                                const float *corr, const float *azd, const uint8_t *type, int32_t nR, int32_t nA,
+                                int32_t byteOrder)
+{
+    return io_guard([&]() {
+        if (!stem || !m || !offR || !offA || !corr || !type)
+            fail(ST_ERR_ARG, "null argument");
+        if (!m->imageFile2 || !m->intGeodat)
+            fail(ST_ERR_ARG, "imageFile2 and intGeodat are needed for the geodat names");
+        check_grid(nR, nA, byteOrder);
+        const std::string s(stem);
+        const size_t n = (size_t)nR * nA;
+        // writeOffsets (sTrackOut.c:44-60), one call per grid row in the
+        // reference; the rows are contiguous, so one stream per file here
+        write_f32(s + ".dr", offR, n, byteOrder);
+        write_f32(s + ".da", offA, n, byteOrder);
+        write_f32(s + ".cc", corr, n, byteOrder);
+        if (azd)
+            write_f32(s + ".azd", azd, n, byteOrder);
+        write_u8(s + ".mt", type, n);
+        const int sf = m->scaleFactor;
+        const std::string geodat(m->intGeodat), image2(m->imageFile2);
+        // writeDatFile (speckleTrack.c:1453-1468): geo2 = dirname(image 2) / geodat
+        {
+            const std::string dat = fmt("%i %i %i %i %i %i\n", m->rStart * sf, m->aStart * sf, nR, nA,
+                                        m->deltaR * sf, m->deltaA * sf) +
+                                    geodat + " " + dir_of(image2) + "/" + geodat + "\n";
+            File f(s + ".dat", "w");
+            if (fwrite(dat.data(), 1, dat.size(), f.fp) != dat.size())
+                fail(ST_ERR_IO, "short write to " + s + ".dat");
+            f.close();
+        }
+        // writeVrtFile (writeVrt.c:16-70): geo2 = dirname(image 2) / basename(geodat)
+        Meta meta = {
+            {"r0", fmt("%i", m->rStart * sf)},
+            {"a0", fmt("%i", m->aStart * sf)},
+            {"deltaA", fmt("%i", m->deltaA * sf)},
+            {"deltaR", fmt("%i", m->deltaR * sf)},
+            {"sigmaStreaks", fmt("%f", 0.)},
+            {"sigmaRange", fmt("%f", 0.)},
+            {"geo1", geodat},
+            {"geo2", dir_of(image2) + "/" + base_of(geodat)},
+            {"Image1", str_or_null(m->imageFile1)},
+            {"Image2", image2},
+            {"mask", str_or_null(m->maskFile)},
+            {"wR", fmt("%i", m->wR)},
+            {"wA", fmt("%i", m->wA)},
+            {"wRa", fmt("%i", m->wRa)},
+            {"wAa", fmt("%i", m->wAa)},
+            {"scaleFactor", fmt("%i", sf)},
+        };
+        write_vrt(s + ".mt.vrt", nR, nA, meta, {{s + ".mt", "MatchType", false}}, byteOrder, false);
+        meta.push_back({"ByteOrder", byteOrder == ST_LSB ? "LSB" : "MSB"});
+        write_vrt(s + ".vrt", nR, nA, meta,
+                  {{s + ".dr", "RangeOffsets", true}, {s + ".da", "AzimuthOffsets", true},
+                   {s + ".cc", "Correlation", true}},
+                  byteOrder, true);
+    });
+}
+
+extern "C" int st_read_dat(const char *datFile, int32_t hdr[6], char *geo1, char *geo2, size_t geoLen)
+{
+    return io_guard([&]() {
+        if (!datFile || !hdr)
+            fail(ST_ERR_ARG, "null argument");
+        File f(datFile, "r");
+        char line[1024];
+        // getDataString skips comment lines starting with ';'
+        auto next = [&](char *buf) -> bool {
+            while (fgets(buf, sizeof line, f.fp))
+                if (buf[0] != ';' && buf[0] != '\n')
+                    return true;
+            return false;
+        };
+        if (!next(line) || sscanf(line, "%i%i%i%i%i%i", &hdr[0], &hdr[1], &hdr[2], &hdr[3], &hdr[4], &hdr[5]) != 6)
+            fail(ST_ERR_IO, std::string("missing image parameters in ") + datFile);
+        if (geo1 && geoLen)
+            geo1[0] = '\0';
+        if (geo2 && geoLen)
+            geo2[0] = '\0';
+        if (next(line) && strstr(line, "geodat")) {
+            char a[1024] = {0}, b[1024] = {0};
+            const int k = sscanf(line, "%1023s %1023s", a, b);
+            if (geo1 && geoLen && k >= 1)
+                snprintf(geo1, geoLen, "%s", a);
+            if (geo2 && geoLen && k >= 2)
+                snprintf(geo2, geoLen, "%s", b);
+        }
+    });
+}
+
+extern "C" int st_read_offsets(const char *stem, int32_t nR, int32_t nA, int32_t byteOrder, float *offR, float *offA,
+                               float *corr, uint8_t *type)
+{
+    return io_guard([&]() {
+        if (!stem || !offR || !offA || !corr || !type)
+            fail(ST_ERR_ARG, "null argument");
+        check_grid(nR, nA, byteOrder);
+        const std::string s(stem);
+        const size_t n = (size_t)nR * nA;
+        read_f32(s + ".dr", offR, n, byteOrder);
+        read_f32(s + ".da", offA, n, byteOrder);
+        read_f32(s + ".cc", corr, n, byteOrder);
+        read_u8(s + ".mt", type, n);
+        // offA < (-LARGEINT + 1): LARGEINT comes from the un-vendored clib
+        // headers; read as the integer 2000000000, the bound converts to
+        // the float -2e9, so the -2e9 sentinel itself is not below it
+        const float cut = (float)(-2000000000 + 1);
+        for (size_t k = 0; k < n; k++)
+            if (offA[k] < cut)
+                type[k] = 0;
+    });
+}
+
+extern "C" int st_write_cull(const char *outBase, const char *inBase, const st_cull_meta *m, const float *offRS,
+                             const float *offAS, const float *sigmaR, const float *sigmaA, const float *corr,
+                             const uint8_t *type, int32_t nR, int32_t nA, int32_t byteOrder)
+{
+    return io_guard([&]() {
+        if (!outBase || !inBase || !m || !offRS || !offAS || !sigmaR || !sigmaA || !corr || !type)
+            fail(ST_ERR_ARG, "null argument");
+        check_grid(nR, nA, byteOrder);
+        const std::string o(outBase), in(inBase);
+        const size_t n = (size_t)nR * nA;
+        // file names of cullst.c:240-285: the sigma files take the input base
+        write_f32(o + ".dr", offRS, n, byteOrder);
+        write_f32(o + ".da", offAS, n, byteOrder);
+        write_f32(in + ".sr", sigmaR, n, byteOrder);
+        write_f32(in + ".sa", sigmaA, n, byteOrder);
+        write_f32(o + ".cc", corr, n, byteOrder);
+        write_u8(o + ".mt", type, n);
+        // populateMetaData (loadCullData.c:95-118) + insertCullParams (:38-49)
+        Meta meta = {
+            {"r0", fmt("%i", m->rStart)},
+            {"a0", fmt("%i", m->aStart)},
+            {"deltaA", fmt("%i", m->deltaA)},
+            {"deltaR", fmt("%i", m->deltaR)},
+            {"sigmaStreaks", fmt("%f", 0.)},
+            {"sigmaRange", fmt("%f", 0.)},
+            {"sR", fmt("%i", m->sR)},
+            {"sA", fmt("%i", m->sA)},
+            {"boxSize", fmt("%i", m->boxSize)},
+            {"nGood", fmt("%i", m->nGood)},
+            {"maxR", fmt("%f", (double)m->maxR)},
+            {"maxA", fmt("%f", (double)m->maxA)},
+            {"corrThresh", fmt("%f", (double)m->corrThresh)},
+            {"singleMT", fmt("%i", m->singleMT)},
+        };
+        if (m->geo2) {
+            meta.push_back({"geo1", str_or_null(m->geo1)});
+            meta.push_back({"geo2", m->geo2});
+        }
+        write_vrt(o + ".mt.vrt", nR, nA, meta, {{o + ".mt", "MatchType", false}}, byteOrder, false);
+        meta.push_back({"ByteOrder", byteOrder == ST_LSB ? "LSB" : "MSB"});
+        // writeCullVrt (writeCullData.c:15-47)
+        write_vrt(o + ".vrt", nR, nA, meta,
+                  {{o + ".dr", "RangeOffsets", true}, {o + ".da", "AzimuthOffsets", true},
+                   {in + ".sr", "RangeSigma", true}, {in + ".sa", "AzimuthSigma", true},
+                   {o + ".cc", "Correlation", true}},
+                  byteOrder, true);
+    });
+}

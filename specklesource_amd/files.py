@@ -1,0 +1,118 @@
+"""Offset and cull products on disk (include/speckle_io.h), through the C
+ABI: the files strack / strackw write after tracking (.dr .da .cc .azd .mt
+.dat .vrt .mt.vrt), the cullst inputs read from them, and the cullst
+products.  Host code; no GPU is needed."""
+import ctypes as C
+
+import numpy as np
+
+from . import track
+
+ST_LSB, ST_MSB = 0, 1
+ST_ERR_IO = -6
+
+
+class OffsetsMeta(C.Structure):
+    _fields_ = [("rStart", C.c_int32), ("aStart", C.c_int32), ("deltaR", C.c_int32), ("deltaA", C.c_int32),
+                ("scaleFactor", C.c_int32), ("wR", C.c_int32), ("wA", C.c_int32), ("wRa", C.c_int32),
+                ("wAa", C.c_int32), ("intGeodat", C.c_char_p), ("imageFile1", C.c_char_p),
+                ("imageFile2", C.c_char_p), ("maskFile", C.c_char_p)]
+
+
+class CullMeta(C.Structure):
+    _fields_ = [("rStart", C.c_int32), ("aStart", C.c_int32), ("deltaR", C.c_int32), ("deltaA", C.c_int32),
+                ("sR", C.c_int32), ("sA", C.c_int32), ("boxSize", C.c_int32), ("nGood", C.c_int32),
+                ("singleMT", C.c_int32), ("maxR", C.c_float), ("maxA", C.c_float), ("corrThresh", C.c_float),
+                ("geo1", C.c_char_p), ("geo2", C.c_char_p)]
+
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    lib = track.load_library()
+    if not _bound:
+        vp = C.c_void_p
+        lib.st_io_last_error.restype = C.c_char_p
+        lib.st_write_offsets.argtypes = [C.c_char_p, C.POINTER(OffsetsMeta), vp, vp, vp, vp, vp, C.c_int32,
+                                         C.c_int32, C.c_int32]
+        lib.st_read_dat.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.c_char_p, C.c_char_p, C.c_size_t]
+        lib.st_read_offsets.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp]
+        lib.st_write_cull.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(CullMeta), vp, vp, vp, vp, vp, vp,
+                                      C.c_int32, C.c_int32, C.c_int32]
+        _bound = True
+    return lib
+
+
+class IoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (status %d)" % (msg, code))
+        self.code = code
+
+
+def _check(st):
+    if st != 0:
+        raise IoError(st, _lib().st_io_last_error().decode())
+
+
+def _b(s):
+    return None if s is None else s.encode()
+
+
+def _f32(a, shape):
+    a = np.ascontiguousarray(a, np.float32)
+    if a.shape != shape:
+        raise ValueError("grid shape %s, expected %s" % (a.shape, shape))
+    return a
+
+
+def offsets_meta(par, intGeodat, imageFile2, imageFile1=None, maskFile=None):
+    """Header fields of an offsets product from the tracking parameters."""
+    return OffsetsMeta(par.rStart, par.aStart, par.deltaR, par.deltaA, par.scaleFactor, par.wR, par.wA, par.wRa,
+                       par.wAa, _b(intGeodat), _b(imageFile1), _b(imageFile2), _b(maskFile))
+
+
+def write_offsets(stem, meta, offR, offA, corr, type_, azd=None, byte_order=ST_MSB):
+    """stem.dr .da .cc [.azd] .mt .dat .vrt .mt.vrt (sTrackOut.c / writeVrt.c)."""
+    nA, nR = np.shape(offR)
+    offR, offA, corr = (_f32(a, (nA, nR)) for a in (offR, offA, corr))
+    t = np.ascontiguousarray(type_, np.uint8)
+    z = None if azd is None else _f32(azd, (nA, nR))
+    _check(_lib().st_write_offsets(stem.encode(), C.byref(meta), offR.ctypes.data, offA.ctypes.data,
+                                   corr.ctypes.data, None if z is None else z.ctypes.data, t.ctypes.data, nR, nA,
+                                   byte_order))
+
+
+def write_result(stem, meta, res, byte_order=ST_MSB, azd=True):
+    """A TrackResult of Device.speckle_track / corr_track_fast."""
+    write_offsets(stem, meta, res.offR, res.offA, res.corr, res.type, res.azd if azd else None, byte_order)
+
+
+def read_dat(path):
+    """(r0, a0, nr, na, deltaR, deltaA), geo1, geo2 of a .dat header."""
+    hdr = (C.c_int32 * 6)()
+    g1 = C.create_string_buffer(1024)
+    g2 = C.create_string_buffer(1024)
+    _check(_lib().st_read_dat(path.encode(), hdr, g1, g2, 1024))
+    return tuple(hdr), g1.value.decode(), g2.value.decode()
+
+
+def read_offsets(stem, nR, nA, byte_order=ST_MSB):
+    """offR, offA, corr, type grids of stem.dr .da .cc .mt (cullst's loader)."""
+    offR = np.empty((nA, nR), np.float32)
+    offA = np.empty_like(offR)
+    corr = np.empty_like(offR)
+    t = np.empty((nA, nR), np.uint8)
+    _check(_lib().st_read_offsets(stem.encode(), nR, nA, byte_order, offR.ctypes.data, offA.ctypes.data,
+                                  corr.ctypes.data, t.ctypes.data))
+    return offR, offA, corr, t
+
+
+def write_cull(out_base, in_base, meta, offRS, offAS, sigmaR, sigmaA, corr, type_, byte_order=ST_MSB):
+    """out_base.dr .da .cc .mt .vrt .mt.vrt and in_base.sr .sa (writeCullData.c)."""
+    nA, nR = np.shape(offRS)
+    arrs = [_f32(a, (nA, nR)) for a in (offRS, offAS, sigmaR, sigmaA, corr)]
+    t = np.ascontiguousarray(type_, np.uint8)
+    _check(_lib().st_write_cull(out_base.encode(), in_base.encode(), C.byref(meta), *[a.ctypes.data for a in arrs],
+                                t.ctypes.data, nR, nA, byte_order))

@@ -39,6 +39,8 @@ def load_library(path=LIB_PATH):
     lib.st_last_error.restype = C.c_char_p
     lib.st_pair_upload.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int32,
                                    C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
+    lib.st_pair_upload_raw.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_int32, C.POINTER(C.c_int32)]
     lib.st_pair_release.argtypes = [vp, C.c_int32]
     lib.st_pair_download.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, vp]
     lib.st_speckle_track.argtypes = [vp, C.c_int32, C.POINTER(abi.TrackParams),
@@ -169,6 +171,21 @@ class Device:
                                      img2.ctypes.data, img2.shape[1], img2.shape[0],
                                      sample_type, C.byref(pid))
         self._check(rc, "st_pair_upload")
+        return pid.value
+
+    def upload_pair_raw(self, raw1, shape1, raw2, shape2, sample_type=abi.ST_SAMPLE_C64, byte_order=1):
+        """Upload samples as the reference's files hold them (bytes objects or
+        uint8 arrays; shapes (na, nr)): c64 big-endian when byte_order is
+        MSB (1), ci16 always big-endian; the swap runs on the GPU."""
+        b1 = np.frombuffer(raw1, np.uint8) if isinstance(raw1, (bytes, bytearray)) else np.ascontiguousarray(raw1)
+        b2 = np.frombuffer(raw2, np.uint8) if isinstance(raw2, (bytes, bytearray)) else np.ascontiguousarray(raw2)
+        es = 8 if sample_type == abi.ST_SAMPLE_C64 else 4
+        if b1.nbytes != es * shape1[0] * shape1[1] or b2.nbytes != es * shape2[0] * shape2[1]:
+            raise ValueError("raw buffer sizes do not match the shapes")
+        pid = C.c_int32()
+        rc = self.lib.st_pair_upload_raw(self.ctx, b1.ctypes.data, shape1[1], shape1[0], b2.ctypes.data, shape2[1],
+                                         shape2[0], sample_type, byte_order, C.byref(pid))
+        self._check(rc, "st_pair_upload_raw")
         return pid.value
 
     def synth_pair(self, spec):

@@ -11,7 +11,7 @@ import pytest
 from specklesource_amd import abi, track
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("speckle_amd.h", "speckle_synth.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("speckle_amd.h", "speckle_synth.h", "speckle_io.h")]
 
 
 def declared_functions():
@@ -36,10 +36,11 @@ def test_struct_layouts_match_header():
 #include <stdio.h>
 #include "speckle_amd.h"
 #include "speckle_synth.h"
+#include "speckle_io.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(st_track_params), sizeof(st_track_out),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(st_track_params), sizeof(st_track_out),
          sizeof(st_track_diag), sizeof(st_cull_params), sizeof(st_synth_spec), sizeof(st_job),
-         sizeof(st_job_info));
+         sizeof(st_job_info), sizeof(st_offsets_meta), sizeof(st_cull_meta));
   return 0;
 }
 '''
@@ -50,9 +51,10 @@ int main(void) {
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     sizes = [int(x) for x in out]
+    from specklesource_amd import files
     assert sizes == [C.sizeof(abi.TrackParams), C.sizeof(abi.TrackOut), C.sizeof(abi.TrackDiag),
                      C.sizeof(abi.CullParams), C.sizeof(abi.SynthSpec), C.sizeof(abi.Job),
-                     C.sizeof(abi.JobInfo)]
+                     C.sizeof(abi.JobInfo), C.sizeof(files.OffsetsMeta), C.sizeof(files.CullMeta)]
 
 
 def test_ctx_create_without_gpu_reports_nodev():
