@@ -13,9 +13,9 @@ constexpr int kPointsPerBlock = 1024;  // LDS: 2 x 1024 x 8 B per block (indices
 constexpr int kThreads = 256;
 constexpr int kColElems = 4096;  // per block: LDS kColElems x 8 B (+ one pad element per column)
 
-template <int P>
+template <int P, int sign>
 __device__ inline void stage_p(const float2 *A, float2 *B, int n, int ns, FastDiv ds, FastDiv dn, const float2 *tw,
-                               int sign, int t)
+                               int t)
 {
     const int stride = n / P;
     const int row = fdiv(t, ds), j = t - row * stride;
@@ -34,8 +34,8 @@ __device__ inline void stage_p(const float2 *A, float2 *B, int n, int ns, FastDi
         dst[base + q * ns] = v[q];
 }
 
-__global__ void __launch_bounds__(kThreads) k_fft_rows(const float2 *src, float2 *data, long nrows, PlanArgs P,
-                                                     int sign)
+template <int SIGN>
+__global__ void __launch_bounds__(kThreads) k_fft_rows(const float2 *src, float2 *data, long nrows, PlanArgs P)
 {
     extern __shared__ float2 lds[];
     const int n = P.n, R = P.rowsPerBlock;
@@ -53,12 +53,12 @@ __global__ void __launch_bounds__(kThreads) k_fft_rows(const float2 *src, float2
         const int work = nrow * (n / p);
         for (int t = threadIdx.x; t < work; t += blockDim.x) {
             switch (p) {
-            case 8: stage_p<8>(A, B, n, ns, ds, dn, tw, sign, t); break;
-            case 4: stage_p<4>(A, B, n, ns, ds, dn, tw, sign, t); break;
-            case 2: stage_p<2>(A, B, n, ns, ds, dn, tw, sign, t); break;
-            case 3: stage_p<3>(A, B, n, ns, ds, dn, tw, sign, t); break;
-            case 5: stage_p<5>(A, B, n, ns, ds, dn, tw, sign, t); break;
-            default: stage_p<7>(A, B, n, ns, ds, dn, tw, sign, t); break;
+            case 8: stage_p<8, SIGN>(A, B, n, ns, ds, dn, tw, t); break;
+            case 4: stage_p<4, SIGN>(A, B, n, ns, ds, dn, tw, t); break;
+            case 2: stage_p<2, SIGN>(A, B, n, ns, ds, dn, tw, t); break;
+            case 3: stage_p<3, SIGN>(A, B, n, ns, ds, dn, tw, t); break;
+            case 5: stage_p<5, SIGN>(A, B, n, ns, ds, dn, tw, t); break;
+            default: stage_p<7, SIGN>(A, B, n, ns, ds, dn, tw, t); break;
             }
         }
         __syncthreads();
@@ -215,7 +215,10 @@ void Engine::rows(const float2 *src, float2 *data, long nrows, int n, int sign, 
     const PlanArgs &P = plan(n, sign);
     long blocks = (nrows + P.rowsPerBlock - 1) / P.rowsPerBlock;
     size_t lds = 2 * (size_t)P.rowsPerBlock * n * sizeof(float2);
-    hipLaunchKernelGGL(k_fft_rows, dim3((unsigned)blocks), dim3(kThreads), lds, st, src, data, nrows, P, sign);
+    if (sign < 0)
+        hipLaunchKernelGGL(k_fft_rows<-1>, dim3((unsigned)blocks), dim3(kThreads), lds, st, src, data, nrows, P);
+    else
+        hipLaunchKernelGGL(k_fft_rows<+1>, dim3((unsigned)blocks), dim3(kThreads), lds, st, src, data, nrows, P);
     ST_HIP(hipGetLastError());
 }
 

@@ -111,10 +111,11 @@ __device__ inline void dft_any(float2 *v, int sign)
 // One Stockham stage.  Kept out of line: inlined into the stage loop of
 // lds_fft, the six radix cases did not share registers and the caller
 // needed twice the VGPRs of the largest case.
-template <int R, int MAXE, int NTH>
+template <int R, int MAXE, int NTH, int SIGN>
 __device__ __attribute__((noinline)) void lds_stage(float2 *buf, int pitch, int nseq, int n, int ns, FastDiv dstride,
-                                                    FastDiv dns, const float2 *tw, int sign)
+                                                    FastDiv dns, const float2 *tw)
 {
+    constexpr int sign = SIGN;
     static_assert(MAXE <= 8192, "FastDiv needs indices below 2^13");
     constexpr int IT = (MAXE / R + NTH - 1) / NTH;
     const int stride = n / R, work = nseq * stride;
@@ -146,22 +147,33 @@ __device__ __attribute__((noinline)) void lds_stage(float2 *buf, int pitch, int 
     __syncthreads();
 }
 
-template <int MAXE, int NTH>
-__device__ inline void lds_fft(float2 *buf, int pitch, int nseq, const PlanArgs &P, int sign)
+template <int MAXE, int NTH, int SIGN>
+__device__ inline void lds_fft_s(float2 *buf, int pitch, int nseq, const PlanArgs &P)
 {
     for (int s = 0; s < P.nst; s++) {
         const int ns = P.ns[s];
         const FastDiv ds = P.dstride[s], dn = P.dns[s];
         const float2 *tw = P.tw + P.twoff[s];
         switch (P.radix[s]) {
-        case 8: lds_stage<8, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
-        case 4: lds_stage<4, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
-        case 2: lds_stage<2, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
-        case 3: lds_stage<3, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
-        case 5: lds_stage<5, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
-        default: lds_stage<7, MAXE, NTH>(buf, pitch, nseq, P.n, ns, ds, dn, tw, sign); break;
+        case 8: lds_stage<8, MAXE, NTH, SIGN>(buf, pitch, nseq, P.n, ns, ds, dn, tw); break;
+        case 4: lds_stage<4, MAXE, NTH, SIGN>(buf, pitch, nseq, P.n, ns, ds, dn, tw); break;
+        case 2: lds_stage<2, MAXE, NTH, SIGN>(buf, pitch, nseq, P.n, ns, ds, dn, tw); break;
+        case 3: lds_stage<3, MAXE, NTH, SIGN>(buf, pitch, nseq, P.n, ns, ds, dn, tw); break;
+        case 5: lds_stage<5, MAXE, NTH, SIGN>(buf, pitch, nseq, P.n, ns, ds, dn, tw); break;
+        default: lds_stage<7, MAXE, NTH, SIGN>(buf, pitch, nseq, P.n, ns, ds, dn, tw); break;
         }
     }
+}
+
+// The sign is a template parameter of the stages (the butterflies' +-i
+// rotations fold into adds and subtracts); one branch picks the instance.
+template <int MAXE, int NTH>
+__device__ inline void lds_fft(float2 *buf, int pitch, int nseq, const PlanArgs &P, int sign)
+{
+    if (sign < 0)
+        lds_fft_s<MAXE, NTH, -1>(buf, pitch, nseq, P);
+    else
+        lds_fft_s<MAXE, NTH, +1>(buf, pitch, nseq, P);
 }
 
 }  // namespace stfft

@@ -458,38 +458,75 @@ __global__ void k_cpx_finalize(KPar P, Pts S, const int *list, int off, int B, C
 
 // Carry the last estimated azShift forward in raster order (the
 // reference's trackPar->azShift persists between points).
-__global__ void k_carry_az(Pts S, int npts)
+// The persistent azimuth shift (speckleTrack.c:246-262): a point uses the
+// Doppler shift of the last point at or before it in raster order that
+// estimated one, 0 before the first.  With lastIdx(p) = max{q <= p : valid
+// q}, this is a max-scan: k_carry_blocks leaves each block's largest valid
+// index, k_carry_az scans its block and takes the carry from the blocks
+// before it.
+constexpr int kCarryBlk = 1024;
+
+__device__ inline int wave_max_scan(int v)
 {
-    __shared__ int last[1024];
-    __shared__ int has[1024];
-    const int t = threadIdx.x, nt = blockDim.x;
-    const int chunk = (npts + nt - 1) / nt;
-    const int lo = t * chunk, hi = imin(npts, lo + chunk);
-    int lv = 0, lh = 0;
-    for (int p = lo; p < hi; p++)
-        if (S.azValid[p]) {
-            lv = S.azShift[p];
-            lh = 1;
-        }
-    last[t] = lv;
-    has[t] = lh;
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o);
+        if (lane >= o)
+            v = max(v, u);
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(kCarryBlk) k_carry_blocks(Pts S, int npts, int *blkMax)
+{
+    __shared__ int sh[kCarryBlk / 64];
+    const int p = blockIdx.x * kCarryBlk + threadIdx.x;
+    int v = (p < npts && S.azValid[p]) ? p : -1;
+    for (int o = 32; o > 0; o >>= 1)
+        v = max(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0)
+        sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (t == 0) {
-        int cur = 0;
-        for (int k = 0; k < nt; k++) {
-            int v = last[k], h = has[k];
-            last[k] = cur;  // value entering chunk k
-            if (h)
-                cur = v;
-        }
+    if (threadIdx.x == 0) {
+        int m = -1;
+        for (int w = 0; w < kCarryBlk / 64; w++)
+            m = max(m, sh[w]);
+        blkMax[blockIdx.x] = m;
+    }
+}
+
+__global__ void __launch_bounds__(kCarryBlk) k_carry_az(Pts S, int npts, const int *blkMax)
+{
+    __shared__ int sh[kCarryBlk / 64];
+    __shared__ int carry;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // carry from the blocks before this one
+    int c = -1;
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += kCarryBlk)
+        c = max(c, blkMax[k]);
+    for (int o = 32; o > 0; o >>= 1)
+        c = max(c, __shfl_xor(c, o));
+    if (lane == 0)
+        sh[wv] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = -1;
+        for (int w = 0; w < kCarryBlk / 64; w++)
+            m = max(m, sh[w]);
+        carry = m;
     }
     __syncthreads();
-    int cur = last[t];
-    for (int p = lo; p < hi; p++) {
-        if (S.azValid[p])
-            cur = S.azShift[p];
-        S.azUse[p] = cur;
-    }
+    const int p = blockIdx.x * kCarryBlk + threadIdx.x;
+    int v = wave_max_scan((p < npts && S.azValid[p]) ? p : -1);
+    if (lane == 63)
+        sh[wv] = v;
+    __syncthreads();
+    int m = carry;
+    for (int w = 0; w < wv; w++)
+        m = max(m, sh[w]);
+    v = max(v, m);
+    if (p < npts)
+        S.azUse[p] = v >= 0 ? S.azShift[v] : 0;
 }
 
 // Work list of the amplitude tries (the while loop of speckleTrack.c:246).
@@ -1833,7 +1870,8 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     std::vector<int> hr2, ha2;
     if (!par.polyShift)
         host_positions(par, hr2, ha2);
-    size_t fixed = (size_t)npts * 160 + hann.size() * 4 + (hr2.size() * 8) + 4096;
+    size_t fixed = (size_t)npts * 160 + hann.size() * 4 + (hr2.size() * 8) + 4096 +
+                   ((size_t)npts / kCarryBlk + 1) * 4 + 256;
     if (par.mask)
         fixed += (size_t)par.maskNr * par.maskNa;
     if (par.baseFlag)
@@ -1874,6 +1912,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     S.azValid = ar.take<uint8_t>(npts);
     S.azShift = ar.take<int>(npts);
     S.azUse = ar.take<int>(npts);
+    int *carryMax = ar.take<int>((npts + kCarryBlk - 1) / kCarryBlk);
     S.corr = ar.take<float>(npts);
     S.rShift1 = ar.take<float>(npts);
     S.aShift1 = ar.take<float>(npts);
@@ -2028,7 +2067,11 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         }
         ar.used = mark;
     }
-    hipLaunchKernelGGL(k_carry_az, dim3(1), dim3(1024), 0, st, S, npts);
+    {
+        const int nCarry = (npts + kCarryBlk - 1) / kCarryBlk;
+        hipLaunchKernelGGL(k_carry_blocks, dim3(nCarry), dim3(kCarryBlk), 0, st, S, npts, carryMax);
+        hipLaunchKernelGGL(k_carry_az, dim3(nCarry), dim3(kCarryBlk), 0, st, S, npts, carryMax);
+    }
     ST_HIP(hipEventRecord(sev[2], st));
     ST_HIP(hipEventRecord(sev[3], st));
     ST_HIP(hipEventRecord(sev[4], st));
