@@ -130,7 +130,7 @@ __device__ inline void mf_mul(const MfTab<KC> &t, const __bf16 *dH, const __bf16
 }
 
 template <int W, int STYPE>
-__global__ void __launch_bounds__(256) k_cpx_mfma(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 16 ? 3 : 2, 8))) k_cpx_mfma(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
                                                   const int *list, int count, const CpxBatch *__restrict__ Cp,
                                                   MfmaTab tab)
 {
@@ -154,6 +154,11 @@ __global__ void __launch_bounds__(256) k_cpx_mfma(const KPar *__restrict__ Pp, c
     __bf16 *bX = (__bf16 *)(base + G::R0 + G::R1);  // X~: [hi/lo]
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
 
+    // Window samples of a patch: NG per thread (element e = tp + k TPP of
+    // [g][a][r]), all loads issued before the first conversion.
+    constexpr int NG = 2 * W * W / TPP;
+    static_assert(NG * TPP == 2 * W * W, "gather split");
+    using Raw = typename std::conditional<isf, float2, short2>::type;
     for (int it = blockIdx.x * PPB + slot; it < count + (PPB - 1); it += gridDim.x * PPB) {
         // every wave of a multi-wave patch takes the same branch
         const bool live = it < count && C.have[it];
@@ -166,23 +171,30 @@ __global__ void __launch_bounds__(256) k_cpx_mfma(const KPar *__restrict__ Pp, c
         }
         const int pt = list[it];
         const int az = C.az[it];
-        // ---------------- gather both windows into the S1 operand [r][2a + c]
+        // ---------------- both windows into the S1 operand [r][2a + c]
         {
             const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
-#pragma unroll 1
-            for (int e = tp; e < 2 * W * W; e += TPP) {
+            Raw pre[NG];
+#pragma unroll
+            for (int k = 0; k < NG; k++) {
+                const int e = tp + k * TPP;
                 const int g = e / (W * W), a = (e / W) % W, r = e % W;
                 const void *img = g ? P.im.img2 : P.im.img1;
                 const int nr = g ? P.im.nr2 : P.im.nr1;
                 const size_t o = (size_t)((g ? a2 : a1) + a) * nr + (g ? r2 : r1) + r;
+                pre[k] = ((const Raw *)img)[o];
+            }
+#pragma unroll
+            for (int k = 0; k < NG; k++) {
+                const int e = tp + k * TPP;
+                const int g = e / (W * W), a = (e / W) % W, r = e % W;
                 float2 v;
                 if constexpr (isf) {
-                    v = ((const float2 *)img)[o];
+                    v = pre[k];
                     v.x *= 1000;
                     v.y *= 1000;
                 } else {
-                    const short2 q = ((const short2 *)img)[o];
-                    v = make_float2((float)q.x, (float)q.y);
+                    v = make_float2((float)pre[k].x, (float)pre[k].y);
                 }
                 if (g == 1 && P.hanning) {
                     const float hw = P.hann[a * W + r];

@@ -18,6 +18,7 @@
 
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "st_ctx.h"
