@@ -61,13 +61,16 @@ template <int W>
 struct MfGeom {
     static constexpr int LDK = 2 * W + 8;  // row pitch (bf16) of an LDS operand
     static constexpr int PLANE = W * LDK;  // W rows
-    // bytes: P (2 patches, hi+lo), then G (2W rows, hi+lo), then the
-    // refinement rows T (8 x 48 double2)
-    static constexpr int R0 = 8 * W * LDK > 8 * 48 * 16 ? 8 * W * LDK : 8 * 48 * 16;
-    // L (2 patches, hi+lo), then |C| (2W x 2W float, transposed)
-    static constexpr int R1 = 8 * W * LDK > 16 * W * W ? 8 * W * LDK : 16 * W * W;
-    static constexpr int R2 = 4 * W * LDK;  // X~ (hi+lo), then the 8 x 8 doubles
-    static constexpr int BYTES = R0 + R1 + R2;
+    // bytes of one patch slot: P (2 patches, hi+lo), later G (2W rows,
+    // hi+lo); X~ (hi+lo); L (2 patches, hi+lo), later |C| (2W x 2W float,
+    // transposed).  The refinement rows T (8 x 48 double2) overlay P/G and
+    // the head of X~; the 8 x 8 doubles Pm sit at the end of X~.
+    static constexpr int RP = 8 * W * LDK;
+    static constexpr int RX = 4 * W * LDK;
+    static constexpr int RL = 8 * W * LDK > 16 * W * W ? 8 * W * LDK : 16 * W * W;
+    static constexpr int PM = RP + RX - 8 * 8 * 8;  // offset of Pm
+    static constexpr int BYTES = RP + RX + RL;
+    static_assert(8 * 48 * 16 <= PM, "T overlaps Pm");
     static constexpr int WPP = W == 16 ? 1 : 4;  // waves per patch
     static constexpr int PPB = 4 / WPP;          // patches per 256-thread block
 };
@@ -148,10 +151,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1
     const int tp = wv * 64 + lane;    // thread index within the patch
     unsigned char *base = mf_lds + (size_t)slot * G::BYTES;
     __bf16 *bP = (__bf16 *)base;                  // P: [g][hi/lo] planes; later G: [hi/lo] 2W rows
-    double2 *T = (double2 *)base;                 // refinement rows (8 x 48), after G is dead
-    __bf16 *bL = (__bf16 *)(base + G::R0);        // L: [g][hi/lo]
-    float *Mg = (float *)(base + G::R0);          // |C| (2W x 2W), after L is dead
-    __bf16 *bX = (__bf16 *)(base + G::R0 + G::R1);  // X~: [hi/lo]
+    double2 *T = (double2 *)base;                 // refinement rows (8 x 48), after G and X~ are dead
+    __bf16 *bX = (__bf16 *)(base + G::RP);        // X~: [hi/lo]
+    __bf16 *bL = (__bf16 *)(base + G::RP + G::RX);  // L: [g][hi/lo]
+    float *Mg = (float *)(base + G::RP + G::RX);    // |C| (2W x 2W), after L is dead
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
 
     // Window samples of a patch: NG per thread (element e = tp + k TPP of
@@ -385,7 +388,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1
         }
         // ---------------- FP64 refinement of the 8 x 8 neighbourhood
         // (overSampleC, speckleTrack.c:852-863): rows first, T[mm][v]
-        double *Pm = (double *)bX;
+        double *Pm = (double *)(base + G::PM);
         if (tp < NFAST * NFAST)
             Pm[tp] = (double)Mg[(J - NFAST / 2 + tp % NFAST) * WW + (I - NFAST / 2 + tp / NFAST)];
         mf_sync<WPP>();
