@@ -79,6 +79,29 @@ int st_write_cull(const char *outBase, const char *inBase, const st_cull_meta *m
                   const float *offAS, const float *sigmaR, const float *sigmaA, const float *corr,
                   const uint8_t *type, int32_t nR, int32_t nA, int32_t byteOrder);
 
+/* ---- inputs before tracking ------------------------------------------ */
+
+/* parseInitialOffsets (Strack/parseInitialOffsets.c:11-64): the
+ * "range_offset_polynomial:" and "azimuth_offset_polynomial:" lines of an
+ * initial-offset file, three coefficients each (st_track_params
+ * rShiftPoly / aShiftPoly with polyShift = 1). */
+int st_read_offset_polys(const char *file, double rShiftPoly[3], double aShiftPoly[3]);
+
+/* readBothOffsetsStrack (Strack/readBothOffsetsStrack.c:76-120): the header
+ * of daFile.dat ("%i%i%i%i%f%f": r0 a0 nr na deltaR deltaA) ... */
+int st_read_initial_offsets_header(const char *daFile, int32_t *r0, int32_t *a0, int32_t *nr, int32_t *na,
+                                   float *deltaR, float *deltaA);
+/* ... and the two grids (na x nr big-endian float32: azimuth offsets from
+ * daFile, range offsets from drFile) for st_track_params initDa / initDr. */
+int st_read_initial_offsets(const char *daFile, const char *drFile, int32_t nr, int32_t na, float *da, float *dr);
+
+/* getMask, offset-mask branch (Strack/getMask.c:75-118): the header of the
+ * mask's .dat file into hdr[6] = r0 a0 nr na nrl nal (nrl, nal = the
+ * integer parts of deltaR, deltaA) ... */
+int st_read_mask_header(const char *datFile, int32_t hdr[6]);
+/* ... and the na x nr mask bytes (st_track_params mask). */
+int st_read_mask(const char *maskFile, int32_t nr, int32_t na, uint8_t *mask);
+
 #ifdef __cplusplus
 }
 #endif

@@ -376,3 +376,98 @@ extern "C" int st_write_cull(const char *outBase, const char *inBase, const st_c
                   byteOrder, true);
     });
 }
+
+// ---- inputs before tracking ----------------------------------------------
+
+namespace {
+
+// getDataString: the next line that is neither empty nor a ';' comment
+bool next_data_line(FILE *fp, char *buf, int len)
+{
+    while (fgets(buf, len, fp))
+        if (buf[0] != ';' && buf[0] != '\n' && buf[0] != '\r')
+            return true;
+    return false;
+}
+
+void read_header6(const std::string &name, const char *fmt6, void *a, void *b, void *c, void *d, void *e, void *f)
+{
+    File fp(name, "r");
+    char line[1024];
+    if (!next_data_line(fp.fp, line, sizeof line) || sscanf(line, fmt6, a, b, c, d, e, f) != 6)
+        fail(ST_ERR_IO, "missing image parameters in " + name);
+}
+
+}  // namespace
+
+extern "C" int st_read_offset_polys(const char *file, double rShiftPoly[3], double aShiftPoly[3])
+{
+    return io_guard([&]() {
+        if (!file || !rShiftPoly || !aShiftPoly)
+            fail(ST_ERR_ARG, "null argument");
+        File fp(file, "r");
+        char line[1024];
+        bool rDone = false, aDone = false;
+        while (!rDone || !aDone) {
+            if (!next_data_line(fp.fp, line, sizeof line))
+                fail(ST_ERR_IO, std::string("missing polynomial in ") + file);
+            if (const char *t = strstr(line, "range_offset_polynomial:")) {
+                t = strchr(t, ':') + 1;
+                if (sscanf(t, "%lf%lf%lf", &rShiftPoly[0], &rShiftPoly[1], &rShiftPoly[2]) != 3)
+                    fail(ST_ERR_IO, std::string("missing range parameter in ") + file);
+                rDone = true;
+            } else if (const char *t2 = strstr(line, "azimuth_offset_polynomial:")) {
+                t2 = strchr(t2, ':') + 1;
+                if (sscanf(t2, "%lf%lf%lf", &aShiftPoly[0], &aShiftPoly[1], &aShiftPoly[2]) != 3)
+                    fail(ST_ERR_IO, std::string("missing azimuth parameter in ") + file);
+                aDone = true;
+            }
+        }
+    });
+}
+
+extern "C" int st_read_initial_offsets_header(const char *daFile, int32_t *r0, int32_t *a0, int32_t *nr, int32_t *na,
+                                              float *deltaR, float *deltaA)
+{
+    return io_guard([&]() {
+        if (!daFile || !r0 || !a0 || !nr || !na || !deltaR || !deltaA)
+            fail(ST_ERR_ARG, "null argument");
+        read_header6(std::string(daFile) + ".dat", "%i%i%i%i%f%f", r0, a0, nr, na, deltaR, deltaA);
+    });
+}
+
+extern "C" int st_read_initial_offsets(const char *daFile, const char *drFile, int32_t nr, int32_t na, float *da,
+                                       float *dr)
+{
+    return io_guard([&]() {
+        if (!daFile || !drFile || !da || !dr)
+            fail(ST_ERR_ARG, "null argument");
+        if (nr <= 0 || na <= 0)
+            fail(ST_ERR_ARG, "grid size");
+        read_f32(daFile, da, (size_t)nr * na, ST_MSB);
+        read_f32(drFile, dr, (size_t)nr * na, ST_MSB);
+    });
+}
+
+extern "C" int st_read_mask_header(const char *datFile, int32_t hdr[6])
+{
+    return io_guard([&]() {
+        if (!datFile || !hdr)
+            fail(ST_ERR_ARG, "null argument");
+        float dR = 0, dA = 0;
+        read_header6(datFile, "%i%i%i%i%f%f", &hdr[0], &hdr[1], &hdr[2], &hdr[3], &dR, &dA);
+        hdr[4] = (int)dR;
+        hdr[5] = (int)dA;
+    });
+}
+
+extern "C" int st_read_mask(const char *maskFile, int32_t nr, int32_t na, uint8_t *mask)
+{
+    return io_guard([&]() {
+        if (!maskFile || !mask)
+            fail(ST_ERR_ARG, "null argument");
+        if (nr <= 0 || na <= 0)
+            fail(ST_ERR_ARG, "grid size");
+        read_u8(maskFile, mask, (size_t)nr * na);
+    });
+}

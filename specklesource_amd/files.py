@@ -116,3 +116,77 @@ def write_cull(out_base, in_base, meta, offRS, offAS, sigmaR, sigmaA, corr, type
     t = np.ascontiguousarray(type_, np.uint8)
     _check(_lib().st_write_cull(out_base.encode(), in_base.encode(), C.byref(meta), *[a.ctypes.data for a in arrs],
                                 t.ctypes.data, nR, nA, byte_order))
+
+
+# ---- inputs before tracking -------------------------------------------------
+
+def _bind_inputs(lib):
+    if getattr(lib, "_st_inputs_bound", False):
+        return
+    lib.st_read_offset_polys.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    i32p, f32p = C.POINTER(C.c_int32), C.POINTER(C.c_float)
+    lib.st_read_initial_offsets_header.argtypes = [C.c_char_p, i32p, i32p, i32p, i32p, f32p, f32p]
+    lib.st_read_initial_offsets.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
+    lib.st_read_mask_header.argtypes = [C.c_char_p, i32p]
+    lib.st_read_mask.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_void_p]
+    lib._st_inputs_bound = True
+
+
+def read_offset_polys(path):
+    """(rShiftPoly, aShiftPoly) of an initial-offset polynomial file."""
+    lib = _lib()
+    _bind_inputs(lib)
+    r = (C.c_double * 3)()
+    a = (C.c_double * 3)()
+    _check(lib.st_read_offset_polys(path.encode(), r, a))
+    return list(r), list(a)
+
+
+def read_initial_offsets(da_file, dr_file):
+    """Initial-offset grids of readBothOffsetsStrack: returns (r0, a0,
+    deltaR, deltaA, da, dr) with da / dr as (na, nr) float32 arrays."""
+    lib = _lib()
+    _bind_inputs(lib)
+    r0, a0, nr, na = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    dR, dA = C.c_float(), C.c_float()
+    _check(lib.st_read_initial_offsets_header(da_file.encode(), C.byref(r0), C.byref(a0), C.byref(nr), C.byref(na),
+                                              C.byref(dR), C.byref(dA)))
+    da = np.empty((na.value, nr.value), np.float32)
+    dr = np.empty_like(da)
+    _check(lib.st_read_initial_offsets(da_file.encode(), dr_file.encode(), nr.value, na.value, da.ctypes.data,
+                                       dr.ctypes.data))
+    return r0.value, a0.value, dR.value, dA.value, da, dr
+
+
+def apply_initial_offsets(par, grids):
+    """Point st_track_params at initial-offset grids (polyShift = 0).  The
+    arrays must outlive the tracking call; they are returned."""
+    from . import abi
+    r0, a0, dR, dA, da, dr = grids
+    par.polyShift = 0
+    par.initDr = abi.ptr(dr, C.c_float)
+    par.initDa = abi.ptr(da, C.c_float)
+    par.initNa, par.initNr = da.shape
+    par.initR0, par.initA0, par.initDeltaR, par.initDeltaA = float(r0), float(a0), float(dR), float(dA)
+    return da, dr
+
+
+def read_mask(dat_file, mask_file):
+    """Offset mask (getMask's offset branch): ((r0, a0, nr, na, nrl, nal), mask (na, nr) uint8)."""
+    lib = _lib()
+    _bind_inputs(lib)
+    hdr = (C.c_int32 * 6)()
+    _check(lib.st_read_mask_header(dat_file.encode(), hdr))
+    r0, a0, nr, na, nrl, nal = tuple(hdr)
+    m = np.empty((na, nr), np.uint8)
+    _check(lib.st_read_mask(mask_file.encode(), nr, na, m.ctypes.data))
+    return (r0, a0, nr, na, nrl, nal), m
+
+
+def apply_mask(par, hdr, mask):
+    """Point st_track_params at a mask read by read_mask; returns the array."""
+    from . import abi
+    r0, a0, nr, na, nrl, nal = hdr
+    par.mask = abi.ptr(mask, C.c_uint8)
+    par.maskNr, par.maskNa, par.maskR0, par.maskA0, par.maskNrl, par.maskNal = nr, na, r0, a0, nrl, nal
+    return mask

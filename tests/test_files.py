@@ -165,3 +165,52 @@ def test_bad_arguments():
     with pytest.raises(files.IoError) as e:
         files.write_offsets("/tmp/x", _meta(), offR, offA, corr, t, byte_order=7)
     assert e.value.code == -1
+
+
+# ---- inputs before tracking -------------------------------------------------
+# Formats from the reference's own scanf/fread calls: parseInitialOffsets.c
+# (keyword lines, "%lf%lf%lf"), readBothOffsetsStrack.c:94-120 (file.dat
+# "%i%i%i%i%f%f", big-endian float32 grids), getMask.c:75-118 (the same
+# header, nrl / nal = (int) deltaR / deltaA, bytes).
+
+def test_offset_polys(tmp_path):
+    p = tmp_path / "init.off"
+    p.write_text("; initial offsets\nsome_other_key: 1 2 3\nazimuth_offset_polynomial: -1.25 2e-4 -3.5e-6\n"
+                 "range_offset_polynomial:  4.5 -1e-3 7e-7\n")
+    r, a = files.read_offset_polys(str(p))
+    assert r == [4.5, -1e-3, 7e-7] and a == [-1.25, 2e-4, -3.5e-6]
+    p.write_text("range_offset_polynomial: 1 2 3\n")
+    with pytest.raises(files.IoError):
+        files.read_offset_polys(str(p))
+    p.write_text("range_offset_polynomial: 1 2\nazimuth_offset_polynomial: 1 2 3\n")
+    with pytest.raises(files.IoError):
+        files.read_offset_polys(str(p))
+
+
+def test_initial_offset_grids(tmp_path):
+    rng = np.random.default_rng(5)
+    da = rng.normal(0, 3, (9, 13)).astype(np.float32)
+    dr = rng.normal(0, 3, (9, 13)).astype(np.float32)
+    base = tmp_path / "prior.da"
+    (tmp_path / "prior.da.dat").write_text("120 240 13 9 40.0 80.0\n")
+    base.write_bytes(da.astype(">f4").tobytes())
+    (tmp_path / "prior.dr").write_bytes(dr.astype(">f4").tobytes())
+    r0, a0, dR, dA, gda, gdr = files.read_initial_offsets(str(base), str(tmp_path / "prior.dr"))
+    assert (r0, a0, dR, dA) == (120, 240, 40.0, 80.0)
+    assert gda.tobytes() == da.tobytes() and gdr.tobytes() == dr.tobytes()
+    from specklesource_amd import track
+    p = track.default_params(nR=4, nA=4, wR=32, wA=32, wRa=32, wAa=32)
+    keep = files.apply_initial_offsets(p, (r0, a0, dR, dA, gda, gdr))
+    assert p.polyShift == 0 and (p.initNr, p.initNa) == (13, 9) and p.initDeltaA == 80.0 and keep[0] is gda
+
+
+def test_offset_mask(tmp_path):
+    rng = np.random.default_rng(6)
+    m = rng.integers(0, 2, (7, 11)).astype(np.uint8)
+    (tmp_path / "mask.dat").write_text("; mask\n10 20 11 7 12.0 3.0\n")
+    (tmp_path / "mask.bytes").write_bytes(m.tobytes())
+    hdr, gm = files.read_mask(str(tmp_path / "mask.dat"), str(tmp_path / "mask.bytes"))
+    assert hdr == (10, 20, 11, 7, 12, 3) and np.array_equal(gm, m)
+    (tmp_path / "short.bytes").write_bytes(m.tobytes()[:-1])
+    with pytest.raises(files.IoError):
+        files.read_mask(str(tmp_path / "mask.dat"), str(tmp_path / "short.bytes"))
