@@ -101,6 +101,12 @@ typedef struct {
     const double *sinThetaD;    /* per-range tables, length nr of image 1 */
     const double *cosThetaD;
     const double *overRange;
+    /* interferogram phase correction on patch 1 (phaseCorrectInt,
+     * speckleTrack.c:909-1000; getInt.c:11-77): intNa x intNr complex
+     * samples as (re, im) float pairs, intNrl x intNal looks; NULL = off.
+     * Complex path only; not on the matrix-core path. */
+    const float *intf;
+    int32_t intNr, intNa, intNrl, intNal;
 } st_track_params;
 
 /* Output grids, caller-owned host arrays of nA*nR elements (row-major). */

@@ -102,6 +102,13 @@ int st_read_mask_header(const char *datFile, int32_t hdr[6]);
 /* ... and the na x nr mask bytes (st_track_params mask). */
 int st_read_mask(const char *maskFile, int32_t nr, int32_t na, uint8_t *mask);
 
+/* getInt (Strack/getInt.c:11-77): the complex interferogram, big-endian
+ * int16 (re, im) pairs, na lines of nr samples, into intf (2 * nr * na
+ * floats, st_track_params intf).  A file with fewer lines leaves the rest
+ * at 1 + 1i, as the reference's fill does; nr, na and the looks come from
+ * the interferogram's geodat file, which the caller reads. */
+int st_read_interferogram(const char *intFile, int32_t nr, int32_t na, float *intf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -229,6 +229,8 @@ static void find_image2_pos(const st_track_params *par, int r1, int a1, int *r2,
 }
 
 /* inBounds / boundsCheck / maskValue, speckleTrack.c:1822-1828,1943-1965 */
+static void zero_pad_fft(ocplx *pad, int pSA, int pSR, const ocplx *orig, int oSA, int oSR, int dA, int dR);
+
 static int in_bounds(int r1, int a1, int wR, int wA, int sR, int sA)
 {
     return (r1 > 0 && (r1 + wR) < sR && a1 > 0 && (a1 + wA) < sA);
@@ -276,6 +278,77 @@ static void phase_correct_baseline(orc_st *s, int r1, int a1)
         }
         x += deltaX;
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* phaseCorrectInt, speckleTrack.c:909-1000, with the patch size of
+ * speckleTrack.c:1576-1580 (osF = 1, speckleTrack.c:176).  The spectrum
+ * of the ps x ps interferogram patch keeps the real part of the bins whose
+ * power is at least 0.99 of the maximum; the imaginary part of every bin
+ * is cleared (the loop body at :960-962 sets .im unconditionally).  The
+ * zero-padded inverse (unnormalised) gives y; patch 1 is multiplied by
+ * conj(y) / |y|, or left alone where |Re y| and |Im y| are both below
+ * 1e-6. */
+int orc_int_patch_size(const st_track_params *p)
+{
+    int ps = p->wA / p->intNal;
+    if (p->wR / p->intNrl > ps)
+        ps = p->wR / p->intNrl;
+    return ps > 8 ? ps : 8;
+}
+
+static void phase_correct_int(orc_st *s, int r1, int a1)
+{
+    const st_track_params *p = s->par;
+    if (!p->intf)
+        return;
+    const int ps = orc_int_patch_size(p);
+    const int ia1 = (a1 + s->wA / 2) / p->intNal - ps / 2;
+    const int ir1 = (r1 + s->wR / 2) / p->intNrl - ps / 2;
+    if (!in_bounds(ir1, ia1, ps, ps, p->intNr, p->intNa))
+        return;
+    const int Pa = ps * p->intNal, Pr = ps * p->intNrl;
+    ocplx *X = (ocplx *)malloc(sizeof(ocplx) * (size_t)ps * ps);
+    ocplx *Y = (ocplx *)malloc(sizeof(ocplx) * (size_t)Pa * Pr);
+    for (int i = 0; i < ps; i++)
+        for (int j = 0; j < ps; j++) {
+            const float *v = p->intf + 2 * ((size_t)(ia1 + i) * p->intNr + (ir1 + j));
+            X[i * ps + j].re = (oreal)v[0];
+            X[i * ps + j].im = (oreal)v[1];
+        }
+    orc_fft2d(X, ps, ps, ORC_FORWARD);
+    double maxf = 0.;
+    for (int k = 0; k < ps * ps; k++) {
+        const double pw = (double)(X[k].re * X[k].re + X[k].im * X[k].im);
+        maxf = pw > maxf ? pw : maxf;
+    }
+    for (int k = 0; k < ps * ps; k++) {
+        const double pw = (double)(X[k].re * X[k].re + X[k].im * X[k].im);
+        if (pw < 0.99 * maxf)
+            X[k].re = 0;
+        X[k].im = 0;
+    }
+    zero_pad_fft(Y, Pa, Pr, X, ps, ps, 0, 0);
+    orc_fft2d(Y, Pa, Pr, ORC_BACKWARD);
+    int i1 = Pa / 2 - s->wA / 2;
+    for (int i = 0; i < s->wA; i++, i1++) {
+        int j1 = Pr / 2 - s->wR / 2;
+        for (int j = 0; j < s->wR; j++, j1++) {
+            double pr = Y[i1 * Pr + j1].re, pi = -Y[i1 * Pr + j1].im, scale;
+            if (fabs(pr) < 1.0e-6 && fabs(pi) < 1.0e-6) {
+                pr = 1.0;
+                pi = 0.0;
+                scale = 1.0;
+            } else
+                scale = 1.0 / sqrt(pr * pr + pi * pi);
+            ocplx *c = &s->patch1[i * s->wR + j];
+            const double cr = c->re, ci = c->im;
+            c->re = (oreal)((cr * pr - ci * pi) * scale);
+            c->im = (oreal)((cr * pi + ci * pr) * scale);
+        }
+    }
+    free(X);
+    free(Y);
 }
 
 /* ------------------------------------------------------------------ */
@@ -619,6 +692,7 @@ static int complex_match(orc_st *s, int r1, int a1, int iOut, int jOut, int *pI,
     int wA = s->wA, wR = s->wR;
     int WA = wA * OSA, WR = wR * OSA;
     phase_correct_baseline(s, r1, a1);
+    phase_correct_int(s, r1, a1);
     /* cmpPSWithPad */
     orc_fft2d(s->patch1, wA, wR, ORC_FORWARD);
     orc_fft2d(s->patch2, wA, wR, ORC_FORWARD);
@@ -819,6 +893,15 @@ int orc_speckle_track(const st_track_params *par, const void *img1, int nr1, int
         return ST_ERR_UNSUPPORTED;
     if (par->wR != par->wA || par->wRa != par->wAa || (par->wR & 1) || (par->wRa & 1))
         return ST_ERR_UNSUPPORTED;
+    if (par->intf && !par->noComplex) {
+        /* the corrected window must lie inside the oversampled patch (the
+         * reference indexes outside it otherwise) */
+        if (par->intNr < 1 || par->intNa < 1 || par->intNrl < 1 || par->intNal < 1)
+            return ST_ERR_ARG;
+        const int ps = orc_int_patch_size(par);
+        if ((ps * par->intNal) / 2 < par->wA / 2 || (ps * par->intNrl) / 2 < par->wR / 2)
+            return ST_ERR_ARG;
+    }
     orc_st S;
     memset(&S, 0, sizeof(S));
     orc_st *s = &S;
@@ -1035,6 +1118,19 @@ void orc_hanning(int w, float *out)
     for (int k = 0; k < w * w; k++)
         out[k] = (float)s.hanning[k];
     free(s.hanning);
+}
+
+/* phaseCorrectInt alone: patch (wA x wR, in place) as patch 1 of the
+ * point whose window starts at (r1, a1). */
+void orc_int_correct(const st_track_params *par, int r1, int a1, ocplx *patch)
+{
+    orc_st s;
+    memset(&s, 0, sizeof(s));
+    s.par = par;
+    s.wA = par->wA;
+    s.wR = par->wR;
+    s.patch1 = patch;
+    phase_correct_int(&s, r1, a1);
 }
 
 /* cmpPSWithPad + the 2w x 2w inverse + findCPeak's fftshifted magnitude,

@@ -61,6 +61,8 @@ class TrackParams(C.Structure):
         ("lambda_", C.c_double),
         ("nSlpA1", C.c_int32),
         ("sinThetaD", _d), ("cosThetaD", _d), ("overRange", _d),
+        ("intf", _f),
+        ("intNr", C.c_int32), ("intNa", C.c_int32), ("intNrl", C.c_int32), ("intNal", C.c_int32),
     ]
 
 

@@ -471,3 +471,30 @@ extern "C" int st_read_mask(const char *maskFile, int32_t nr, int32_t na, uint8_
         read_u8(maskFile, mask, (size_t)nr * na);
     });
 }
+
+extern "C" int st_read_interferogram(const char *intFile, int32_t nr, int32_t na, float *intf)
+{
+    return io_guard([&]() {
+        if (!intFile || !intf)
+            fail(ST_ERR_ARG, "null argument");
+        if (nr <= 0 || na <= 0)
+            fail(ST_ERR_ARG, "interferogram size");
+        const size_t n = (size_t)nr * na;
+        for (size_t k = 0; k < 2 * n; k++)
+            intf[k] = 1.0f;
+        File f(intFile, "rb");
+        if (fseek(f.fp, 0, SEEK_END) != 0)
+            fail(ST_ERR_IO, std::string("cannot size ") + intFile);
+        const long bytes = ftell(f.fp);
+        rewind(f.fp);
+        const long lines = std::min<long>(bytes / ((long)nr * 4), na);
+        std::vector<uint16_t> row(2 * (size_t)nr);
+        for (long i = 0; i < lines; i++) {
+            if (fread(row.data(), 4, (size_t)nr, f.fp) != (size_t)nr)
+                fail(ST_ERR_IO, std::string("short read from ") + intFile);
+            float *dst = intf + 2 * (size_t)i * nr;
+            for (size_t k = 0; k < 2 * (size_t)nr; k++)
+                dst[k] = (float)(int16_t)__builtin_bswap16(row[k]);
+        }
+    });
+}

@@ -37,6 +37,20 @@ __constant__ float2 c_tw128[128];
 
 namespace {
 
+// Kept bins of one point's interferogram spectrum (phaseCorrectInt): the
+// first kIntInline sit in the per-point record, a point with more keeps all
+// of them in a slot of ps^2 entries of the overflow area.
+struct IntBin {
+    int kp, lp;  // position in the Pa x Pr zero pad (azimuth, range)
+    double v;    // kept value: the real part of the bin
+};
+constexpr int kIntInline = 16;
+struct IntSet {
+    int n;    // kept non-zero bins (0: no correction)
+    int ovf;  // overflow slot when n > kIntInline
+    IntBin b[kIntInline];
+};
+
 struct KPar {
     int nR, nA, rStart, aStart, deltaR, deltaA, wR, wA, wRa, wAa, sf;
     int noComplex, hanning, maxTries, checkAz;
@@ -48,6 +62,11 @@ struct KPar {
     double Bn, Bp, dBn, dBp, dBnQ, dBpQ, lambda;
     int nSlpA1;
     const double *sinT, *cosT, *overR;
+    // interferogram phase correction: intSet is indexed by list position
+    int intOn, intNr, intNa, intNrl, intNal, intPs, intPa, intPr;
+    const float *intf;
+    const IntSet *intSet;
+    const IntBin *intOvf;
     const float *hann;
     Images im;
 };
@@ -70,6 +89,152 @@ __device__ inline bool in_bounds(int r1, int a1, int wR, int wA, int sR, int sA)
 {
     return r1 > 0 && (r1 + wR) < sR && a1 > 0 && (a1 + wA) < sA;
 }
+
+// ---------------------------------------------------------------------
+// Interferogram phase correction (phaseCorrectInt, speckleTrack.c:909-1000;
+// patch size speckleTrack.c:1576-1580, osF = 1).  The ps x ps interferogram
+// patch at ((a1 + wA/2)/nal - ps/2, (r1 + wR/2)/nrl - ps/2) has spectrum X;
+// the bins whose power reaches 0.99 of the maximum keep v = Re X (the
+// reference clears the imaginary part of every bin) and move to their place
+// in an Pa x Pr zero pad (Pa = ps nal, Pr = ps nrl; with odd ps the last row
+// and column are not copied).  y, the unnormalised inverse of that pad, is a
+// sum over the kept bins and is evaluated only on the w x w window that
+// starts at (Pa/2 - wA/2, Pr/2 - wR/2); patch 1 is multiplied there by
+// conj(y) / |y|, or left as it is where |Re y| and |Im y| are both below
+// 1e-6.  Spectrum and correction run in FP64.
+constexpr int kIntThreads = 256;
+constexpr int kIntMaxPs = 64;
+constexpr int kIntPer = kIntMaxPs * kIntMaxPs / kIntThreads;
+
+__device__ inline void int_correct(const KPar &P, int it, int i, int j, double &cr, double &ci)
+{
+    const IntSet *I = P.intSet + it;
+    const int nb = I->n;
+    if (nb == 0)
+        return;
+    const IntBin *B = nb <= kIntInline ? I->b : P.intOvf + (size_t)I->ovf * P.intPs * P.intPs;
+    const int m = P.intPa / 2 - P.wA / 2 + i, n = P.intPr / 2 - P.wR / 2 + j;
+    double yr = 0, yi = 0;
+    for (int q = 0; q < nb; q++) {
+        const IntBin b = B[q];
+        const double f = (double)((b.kp * m) % P.intPa) / P.intPa + (double)((b.lp * n) % P.intPr) / P.intPr;
+        double sn, cs;
+        sincospi(2.0 * f, &sn, &cs);
+        yr += b.v * cs;
+        yi += b.v * sn;
+    }
+    const double pr = yr, pim = -yi;
+    if (fabs(pr) < 1.0e-6 && fabs(pim) < 1.0e-6)
+        return;
+    const double scale = 1.0 / sqrt(pr * pr + pim * pim);
+    const double nr = (cr * pr - ci * pim) * scale, ni = (cr * pim + ci * pr) * scale;
+    cr = nr;
+    ci = ni;
+}
+
+// One workgroup per listed point.  Pass 0 writes the record (n, the first
+// kIntInline bins, an overflow slot); pass 1 runs only for points with more
+// bins and writes all of them to their slot.
+__global__ void __launch_bounds__(kIntThreads) k_int_bins(KPar P, Pts S, const int *list, IntSet *set, IntBin *ovf,
+                                                          int *ovfCount, int pass)
+{
+    extern __shared__ double2 ldsd[];
+    __shared__ double red[kIntThreads / 64];
+    __shared__ int nKept;
+    const int it = blockIdx.x;
+    IntSet *me = set + it;
+    if (pass == 1 && me->n <= kIntInline)
+        return;
+    const int ps = P.intPs, ps2 = ps * ps, h = ps / 2;
+    const int pt = list[it];
+    const int ia1 = (S.a1[pt] + P.wA / 2) / P.intNal - ps / 2;
+    const int ir1 = (S.r1[pt] + P.wR / 2) / P.intNrl - ps / 2;
+    if (!in_bounds(ir1, ia1, ps, ps, P.intNr, P.intNa)) {
+        if (threadIdx.x == 0) {
+            me->n = 0;
+            me->ovf = 0;
+        }
+        return;
+    }
+    double2 *T = ldsd, *tw = ldsd + ps2;
+    float2 *xin = (float2 *)(ldsd + ps2 + ps);
+    const float2 *src = (const float2 *)P.intf;
+    for (int e = threadIdx.x; e < ps2; e += kIntThreads) {
+        const int i = e / ps, j = e - (e / ps) * ps;
+        xin[e] = src[(size_t)(ia1 + i) * P.intNr + ir1 + j];
+    }
+    for (int t = threadIdx.x; t < ps; t += kIntThreads) {
+        double sn, cs;
+        sincospi(-2.0 * t / ps, &sn, &cs);
+        tw[t] = make_double2(cs, sn);
+    }
+    if (threadIdx.x == 0)
+        nKept = 0;
+    __syncthreads();
+    // forward DFT along range: T[i][l] = sum_j x[i][j] exp(-2 pi i j l / ps)
+    for (int e = threadIdx.x; e < ps2; e += kIntThreads) {
+        const int i = e / ps, l = e - (e / ps) * ps;
+        double sr = 0, si = 0;
+        int idx = 0;
+        for (int j = 0; j < ps; j++) {
+            const float2 x = xin[i * ps + j];
+            const double2 w = tw[idx];
+            sr += (double)x.x * w.x - (double)x.y * w.y;
+            si += (double)x.x * w.y + (double)x.y * w.x;
+            idx += l;
+            idx -= idx >= ps ? ps : 0;
+        }
+        T[e] = make_double2(sr, si);
+    }
+    __syncthreads();
+    // along azimuth, kept in registers: X[k][l] = sum_i T[i][l] exp(-2 pi i i k / ps)
+    double xr[kIntPer], pw[kIntPer];
+    double mx = 0;
+#pragma unroll
+    for (int q = 0; q < kIntPer; q++) {
+        const int e = threadIdx.x + q * kIntThreads;
+        xr[q] = 0;
+        pw[q] = -1;
+        if (e < ps2) {
+            const int k = e / ps, l = e - (e / ps) * ps;
+            double sr = 0, si = 0;
+            int idx = 0;
+            for (int i = 0; i < ps; i++) {
+                const double2 t = T[i * ps + l], w = tw[idx];
+                sr += t.x * w.x - t.y * w.y;
+                si += t.x * w.y + t.y * w.x;
+                idx += k;
+                idx -= idx >= ps ? ps : 0;
+            }
+            xr[q] = sr;
+            pw[q] = sr * sr + si * si;
+            mx = fmax(mx, pw[q]);
+        }
+    }
+    mx = block_max_d<kIntThreads>(mx, red);
+    const double thr = 0.99 * mx;
+    IntBin *dst = pass == 0 ? me->b : ovf + (size_t)me->ovf * ps2;
+    const int cap = pass == 0 ? kIntInline : ps2;
+#pragma unroll
+    for (int q = 0; q < kIntPer; q++) {
+        const int e = threadIdx.x + q * kIntThreads;
+        if (e < ps2 && !(pw[q] < thr) && xr[q] != 0.0) {
+            const int k = e / ps, l = e - (e / ps) * ps;
+            if (k < 2 * h && l < 2 * h) {
+                const int pos = atomicAdd(&nKept, 1);
+                if (pos < cap)
+                    dst[pos] = IntBin{k < h ? k : P.intPa - 2 * h + k, l < h ? l : P.intPr - 2 * h + l, xr[q]};
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && pass == 0) {
+        me->n = nKept;
+        me->ovf = nKept > kIntInline ? atomicAdd(ovfCount, 1) : 0;
+    }
+}
+
+size_t int_lds_bytes(int ps) { return (size_t)ps * ps * sizeof(double2) + ps * sizeof(double2) + (size_t)ps * ps * sizeof(float2); }
 
 // ---------------------------------------------------------------------
 // geometry: findImage2Pos (polynomial) or host-computed positions,
@@ -202,6 +367,12 @@ __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *lis
             double pr = cos(ph), pim = sin(ph);
             double cr = x.x, ci = x.y;
             x = make_float2((float)(cr * pr - ci * pim), (float)(cr * pim + ci * pr));
+        }
+        if (P.intOn) {
+            const int i = e / w, j = e - (e / w) * w;
+            double cr = x.x, ci = x.y;
+            int_correct(P, off + b, i, j, cr, ci);
+            x = make_float2((float)cr, (float)ci);
         }
         P1[e] = x;
     }
@@ -1156,9 +1327,9 @@ struct FusedLanes {
     __device__ static inline int at(int row, int col) { return row * LD + col + (row >= W / 2 ? SH : 0); }
 };
 
-// BASE = 1 adds the baseline phase ramp on patch 1 (a separate
-// instantiation: its FP64 sin/cos would otherwise raise the register
-// allocation of every launch).  OCC is the minimum number of waves per
+// BASE bit 0 adds the baseline phase ramp on patch 1, bit 1 the
+// interferogram correction (separate instantiations: their FP64 sin/cos
+// would otherwise raise the register allocation of every launch).  OCC is the minimum number of waves per
 // SIMD the register allocation must allow: 2 (at w = 64 this caps the
 // allocation at 256 VGPRs with a few spilled values instead of one wave
 // per SIMD).
@@ -1225,18 +1396,19 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
             v.y = useH ? v.y * hw : v.y;
             x[m] = v;
         }
-        // ---------------- baseline phase ramp on patch 1 (in LDS)
-        if constexpr (BASE) {
+        // ---------------- phase corrections on patch 1 (in LDS)
+        if constexpr (BASE != 0) {
             float2 *R = lds + (size_t)g * BUF;
 #pragma unroll
             for (int m = 0; m < H; m++)
                 R[FL::at(L * m + h, r)] = x[m];
             __syncthreads();
+            float2 *R0 = lds;
+            if constexpr ((BASE & 1) != 0) {
             const double x0 = (a1 - P.wA / 2 - (P.nSlpA1 / 2)) / (double)P.nSlpA1;
             const double bn = P.Bn + x0 * P.dBn + x0 * x0 * P.dBnQ;
             const double ptod = -(4.0 * 3.14159265358979323846) / P.lambda;
             const int j0 = r1 - P.wR / 2 > 0 ? r1 - P.wR / 2 : 0;
-            float2 *R0 = lds;
 #pragma unroll 1
             for (int e = t; e < W * W; e += NTH) {
                 const int i = e / W, j = e - (e / W) * W;
@@ -1249,6 +1421,17 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
                 const float2 v = R0[FL::at(i, j)];
                 const double cr = v.x, ci = v.y;
                 R0[FL::at(i, j)] = make_float2((float)(cr * pr - ci * pim), (float)(cr * pim + ci * pr));
+            }
+            }
+            if constexpr ((BASE & 2) != 0) {
+#pragma unroll 1
+                for (int e = t; e < W * W; e += NTH) {
+                    const int i = e / W, j = e - (e / W) * W;
+                    const float2 v = R0[FL::at(i, j)];
+                    double cr = v.x, ci = v.y;
+                    int_correct(P, it, i, j, cr, ci);
+                    R0[FL::at(i, j)] = make_float2((float)cr, (float)ci);
+                }
             }
             __syncthreads();
 #pragma unroll
@@ -1642,7 +1825,7 @@ void launch_fused_kernel(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, c
 
 template <int W, int STYPE>
 void launch_cpx_fused_t(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
-                        const CpxBatch *dC, bool base)
+                        const CpxBatch *dC, int base)
 {
     const size_t lds = std::max<size_t>((size_t)2 * FusedLanes<W>::BUF * sizeof(float2), 8192);
     const int prepSlot = ctx->kernel_slot("k_cpx_prep");
@@ -1650,10 +1833,12 @@ void launch_cpx_fused_t(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, co
     hipLaunchKernelGGL((k_cpx_prep_cols<W, STYPE>), dim3((count + 64 / W - 1) / (64 / W)), dim3(64), 0, ctx->stream, dP,
                        dS, list, count, dC);
     ctx->time_end(prepSlot);
-    if (base)
-        launch_fused_kernel<W, STYPE, 1>(ctx, slot, dP, dS, list, count, dC, lds);
-    else
-        launch_fused_kernel<W, STYPE, 0>(ctx, slot, dP, dS, list, count, dC, lds);
+    switch (base) {
+    case 0: launch_fused_kernel<W, STYPE, 0>(ctx, slot, dP, dS, list, count, dC, lds); break;
+    case 1: launch_fused_kernel<W, STYPE, 1>(ctx, slot, dP, dS, list, count, dC, lds); break;
+    case 2: launch_fused_kernel<W, STYPE, 2>(ctx, slot, dP, dS, list, count, dC, lds); break;
+    default: launch_fused_kernel<W, STYPE, 3>(ctx, slot, dP, dS, list, count, dC, lds); break;
+    }
 }
 
 template <int W, int OCC>
@@ -1688,10 +1873,11 @@ void launch_cpx_fused(st_ctx *ctx, int slot, const KPar &P, const Pts &S, const 
     ST_HIP(hipMemcpyAsync(dP, &P, sizeof(KPar), hipMemcpyHostToDevice, ctx->stream));
     ST_HIP(hipMemcpyAsync(dS, &S, sizeof(Pts), hipMemcpyHostToDevice, ctx->stream));
     ST_HIP(hipMemcpyAsync(dC, &C, sizeof(CpxBatch), hipMemcpyHostToDevice, ctx->stream));
+    const int base = (P.baseFlag != 0 ? 1 : 0) | (P.intOn ? 2 : 0);
     if (P.im.stype == ST_SAMPLE_C64)
-        launch_cpx_fused_t<W, ST_SAMPLE_C64>(ctx, slot, dP, dS, list, count, dC, P.baseFlag != 0);
+        launch_cpx_fused_t<W, ST_SAMPLE_C64>(ctx, slot, dP, dS, list, count, dC, base);
     else
-        launch_cpx_fused_t<W, ST_SAMPLE_CI16>(ctx, slot, dP, dS, list, count, dC, P.baseFlag != 0);
+        launch_cpx_fused_t<W, ST_SAMPLE_CI16>(ctx, slot, dP, dS, list, count, dC, base);
 }
 
 // bf16 of a float, round to nearest even (host side of the operand split)
@@ -1856,8 +2042,9 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     ST_HIP(hipEventRecord(sev[0], st));
     // the matrix-core complex match covers windows 16 and 32 without the
     // baseline ramp; other cases run the FP32 paths
+    const bool intOn = par.intf != nullptr && !par.noComplex;
     const bool mfmaPath = par.precision == ST_PREC_MFMA && (par.wR == 16 || par.wR == 32) && !par.baseFlag &&
-                          !staged_only();
+                          !intOn && !staged_only();
     const bool fusedPath = (par.wR == 16 || par.wR == 32 || par.wR == 64) && !staged_only();
     const int kPeak = ctx->kernel_slot(mfmaPath ? "k_cpx_mfma" : (fusedPath ? "k_cpx_fused" : "peak_refine"));
     const int kGather = ctx->kernel_slot("cpx_gather"), kFft = ctx->kernel_slot("fft2d"),
@@ -1877,6 +2064,19 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         fixed += (size_t)par.maskNr * par.maskNa;
     if (par.baseFlag)
         fixed += (size_t)pr.nr1 * 24;
+    int intPs = 0;
+    if (intOn) {
+        ST_REQUIRE(par.intNr > 0 && par.intNa > 0 && par.intNrl > 0 && par.intNal > 0, ST_ERR_ARG,
+                   "interferogram size and looks");
+        intPs = std::max(std::max(par.wA / par.intNal, par.wR / par.intNrl), 8);
+        ST_REQUIRE(intPs <= kIntMaxPs, ST_ERR_UNSUPPORTED,
+                   "interferogram patch above 64 x 64 samples (window / looks)");
+        ST_REQUIRE((intPs * par.intNal) / 2 >= par.wA / 2 && (intPs * par.intNrl) / 2 >= par.wR / 2, ST_ERR_ARG,
+                   "window not covered by the oversampled interferogram patch (window not a multiple of the looks)");
+        ST_REQUIRE((long)intPs * par.intNal <= 32768 && (long)intPs * par.intNrl <= 32768, ST_ERR_ARG,
+                   "interferogram looks");
+        fixed += (size_t)par.intNr * par.intNa * 8 + (size_t)npts * sizeof(IntSet) + 512;
+    }
     // batch sizes
     const size_t perCpx = (size_t)w * w * (16 + 32 + 32 + 16) + 2048;
     int ws_max = par.wAa * (par.maxTries >= 2 ? LA : 1);
@@ -1975,6 +2175,17 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         P.cosT = upload(ctx, par.cosThetaD, (size_t)pr.nr1);
         P.overR = upload(ctx, par.overRange, (size_t)pr.nr1);
     }
+    P.intOn = intOn ? 1 : 0;
+    if (intOn) {
+        P.intNr = par.intNr;
+        P.intNa = par.intNa;
+        P.intNrl = par.intNrl;
+        P.intNal = par.intNal;
+        P.intPs = intPs;
+        P.intPa = intPs * par.intNal;
+        P.intPr = intPs * par.intNrl;
+        P.intf = upload(ctx, par.intf, (size_t)2 * par.intNr * par.intNa);
+    }
     P.hann = upload(ctx, hann.data(), hann.size());
     P.im = Images{pr.d1, pr.d2, pr.nr1, pr.na1, pr.nr2, pr.na2, pr.stype};
     const int *dR2 = nullptr, *dA2 = nullptr;
@@ -1989,6 +2200,40 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     ST_HIP(hipGetLastError());
     ST_HIP(hipEventRecord(sev[1], st));
     const int nCpx = read_count(ctx, S.counts, CNT_LIST);
+
+    // ---- interferogram spectra of the complex attempts
+    struct OvfBuf {
+        void *p = nullptr;
+        ~OvfBuf()
+        {
+            if (p)
+                (void)hipFree(p);
+        }
+    } intOvfBuf;
+    if (intOn && nCpx > 0) {
+        IntSet *set = ar.take<IntSet>(nCpx);
+        int *nOvfDev = ar.take<int>(1);
+        ST_HIP(hipMemsetAsync(nOvfDev, 0, sizeof(int), st));
+        P.intSet = set;
+        const size_t lds = int_lds_bytes(intPs);
+        const int kInt = ctx->kernel_slot("k_int_bins");
+        ctx->time_begin(kInt);
+        hipLaunchKernelGGL(k_int_bins, dim3(nCpx), dim3(kIntThreads), lds, st, P, S, list, set, (IntBin *)nullptr,
+                           nOvfDev, 0);
+        ctx->time_end(kInt);
+        ST_HIP(hipGetLastError());
+        const int nOvf = read_count(ctx, nOvfDev, 0);
+        if (nOvf > 0) {
+            // spectra with more than kIntInline bins within 1% of their peak
+            ST_HIP(hipMalloc(&intOvfBuf.p, (size_t)nOvf * intPs * intPs * sizeof(IntBin)));
+            P.intOvf = (const IntBin *)intOvfBuf.p;
+            ctx->time_begin(kInt);
+            hipLaunchKernelGGL(k_int_bins, dim3(nCpx), dim3(kIntThreads), lds, st, P, S, list, set,
+                               (IntBin *)intOvfBuf.p, nOvfDev, 1);
+            ctx->time_end(kInt);
+            ST_HIP(hipGetLastError());
+        }
+    }
 
     // ---- complex attempts
     const bool fused = (w == 16 || w == 32 || w == 64) && !staged_only();

@@ -129,6 +129,7 @@ def _bind_inputs(lib):
     lib.st_read_initial_offsets.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
     lib.st_read_mask_header.argtypes = [C.c_char_p, i32p]
     lib.st_read_mask.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_void_p]
+    lib.st_read_interferogram.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_void_p]
     lib._st_inputs_bound = True
 
 
@@ -190,3 +191,24 @@ def apply_mask(par, hdr, mask):
     par.mask = abi.ptr(mask, C.c_uint8)
     par.maskNr, par.maskNa, par.maskR0, par.maskA0, par.maskNrl, par.maskNal = nr, na, r0, a0, nrl, nal
     return mask
+
+
+def read_interferogram(path, nr, na):
+    """getInt: the (na, 2 nr) float32 (re, im) interferogram of a big-endian
+    int16 file; lines past the end of a short file stay 1 + 1i."""
+    lib = _lib()
+    _bind_inputs(lib)
+    intf = np.empty((na, 2 * nr), np.float32)
+    _check(lib.st_read_interferogram(path.encode(), nr, na, intf.ctypes.data))
+    return intf
+
+
+def apply_interferogram(par, intf, nrl, nal):
+    """Point st_track_params at an interferogram ((na, 2 nr) float32 pairs)
+    with nrl x nal looks; returns the array, which must outlive the call."""
+    from . import abi
+    intf = np.ascontiguousarray(intf, dtype=np.float32)
+    par.intf = abi.ptr(intf, C.c_float)
+    par.intNa, par.intNr = intf.shape[0], intf.shape[1] // 2
+    par.intNrl, par.intNal = nrl, nal
+    return intf

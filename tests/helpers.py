@@ -124,3 +124,43 @@ def compare_strack(par, g, o32, o64, tie=TIE, thresh_tol=THRESH_TOL, fine_tie=1e
 # the cull workload generator and the CullParams builder live in the package
 from specklesource_amd.configs import make_cull_grid  # noqa: E402,F401
 from specklesource_amd.track import cull_params  # noqa: E402,F401
+
+
+def fringe_scene(n, w, nrl, nal, kfa, kfr, seed, noise=0.2, step=None, second=None, chirp=False):
+    """Synthetic pair whose image 1 carries the fringe of an interferogram
+    with nrl x nal looks, and that interferogram ((na_i, 2 nr_i) float32
+    pairs, amplitude 100).  The fringe has kfa / kfr cycles per ps
+    interferogram samples, ps = max(w / nal, w / nrl, 8) (the patch of the
+    correction), plus complex noise of relative amplitude `noise`.
+    second = (kfa2, kfr2, amp2) adds a second fringe over the right half of
+    the interferogram; chirp=True replaces the fringe by a quadratic-phase
+    (flat-spectrum) pattern over the lower half."""
+    ps = max(w // nal, w // nrl, 8)
+    s = configs._spec(n, n, seed, dr0=0.6, da0=-0.4, gmin=0.8, gmax=0.9)
+    a, b = track.synth_host(s)
+    na_i, nr_i = n // nal + 2, n // nrl + 2
+    ia = np.arange(na_i, dtype=np.float64)[:, None]
+    ir = np.arange(nr_i, dtype=np.float64)[None, :]
+    z = np.exp(2j * np.pi * (kfa * ia + kfr * ir) / ps)
+    if second is not None:
+        k2a, k2r, amp2 = second
+        half = (ir >= nr_i // 2) * np.ones_like(ia)
+        z = z + half * amp2 * np.exp(2j * np.pi * (k2a * ia + k2r * ir) / ps + 0.7j)
+    if chirp:
+        q = np.exp(1j * (np.pi * ((ia % ps) ** 2 + (ir % ps) ** 2) / ps + 0.3))
+        low = (ia >= na_i // 2) * np.ones_like(ir)
+        z = np.where(low > 0, q, z)
+    rng = np.random.default_rng(seed)
+    z = z + noise * (rng.standard_normal(z.shape) + 1j * rng.standard_normal(z.shape)) / np.sqrt(2)
+    intf = np.empty((na_i, 2 * nr_i), np.float32)
+    intf[:, 0::2] = 100 * z.real
+    intf[:, 1::2] = 100 * z.imag
+    # image 1 carries the interferogram's phase (image 1 x conj(image 2))
+    aa = np.arange(n, dtype=np.float64)[:, None] / nal
+    rr = np.arange(n, dtype=np.float64)[None, :] / nrl
+    ph = np.angle(np.exp(2j * np.pi * (kfa * aa + kfr * rr) / ps))
+    a = (a * np.exp(1j * ph)).astype(np.complex64)
+    par = small_params(w, n, n, step=step or w // 2, rShiftPoly=[0.6, 0, 0], aShiftPoly=[-0.4, 0, 0])
+    par.intf = abi.ptr(intf, C.c_float)
+    par.intNa, par.intNr, par.intNrl, par.intNal = na_i, nr_i, nrl, nal
+    return par, a, b, intf, ps

@@ -214,3 +214,23 @@ def test_offset_mask(tmp_path):
     (tmp_path / "short.bytes").write_bytes(m.tobytes()[:-1])
     with pytest.raises(files.IoError):
         files.read_mask(str(tmp_path / "mask.dat"), str(tmp_path / "short.bytes"))
+
+
+def test_interferogram(tmp_path):
+    """getInt (Strack/getInt.c:46-75): big-endian int16 (re, im) pairs; a
+    short file leaves the missing lines at 1 + 1i."""
+    rng = np.random.default_rng(7)
+    nr, na = 9, 6
+    z = rng.integers(-30000, 30000, (na, 2 * nr)).astype(">i2")
+    (tmp_path / "full.int").write_bytes(z.tobytes())
+    got = files.read_interferogram(str(tmp_path / "full.int"), nr, na)
+    assert got.dtype == np.float32 and np.array_equal(got, z.astype(np.float32))
+    (tmp_path / "short.int").write_bytes(z[:4].tobytes() + z[4, :5].tobytes())
+    got = files.read_interferogram(str(tmp_path / "short.int"), nr, na)
+    assert np.array_equal(got[:4], z[:4].astype(np.float32)) and np.all(got[4:] == 1.0)
+    from specklesource_amd import track
+    p = track.default_params(nR=4, nA=4, wR=32, wA=32, wRa=32, wAa=32)
+    keep = files.apply_interferogram(p, got, 2, 8)
+    assert (p.intNr, p.intNa, p.intNrl, p.intNal) == (nr, na, 2, 8) and keep is got
+    with pytest.raises(files.IoError):
+        files.read_interferogram(str(tmp_path / "missing.int"), nr, na)
