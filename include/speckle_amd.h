@@ -78,7 +78,7 @@ typedef struct {
     /* flags, defaults in Strack/strack.c:143-154 */
     int32_t noComplex;          /* -noComplex */
     int32_t hanningFlag;        /* default 1; -noHanning clears it */
-    int32_t gaussFlag;          /* -gauss: not supported (ST_ERR_UNSUPPORTED) */
+    int32_t gaussFlag;          /* -gauss: 2-D Gaussian peak fit (cmpTrackGauss) on the staged path */
     int32_t maxTries;           /* 2, -singleAmp gives 1 */
     int32_t checkAzFocus;       /* -checkAzFocus */
     float azDefocusThresh;      /* 1.8 (strack.c:45) */

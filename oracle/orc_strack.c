@@ -644,6 +644,176 @@ static int lm_step(lm_state *st, const oreal *x, const oreal *y, const oreal *sg
     return 1;
 }
 
+/* ------------------------------------------------------------------ */
+/* 2-D Gaussian Levenberg-Marquardt fit of the -gauss peak mode
+ * (cmpTrackGauss, mrqminMod, mrqcofMod, myGauss: speckleTrack.c:2205-2303,
+ * 2309-2416, 2421-2443).  Parameters a = {amplitude, mu_i, sigma_i, mu_j,
+ * sigma_j}, all five fitted; data weights 1.  Gauss-Jordan with full
+ * pivoting as NR's gaussj; a singular system ends the iterations (the
+ * reference's gaussj aborts the program there). */
+#define NG2 (NGAUSS * NGAUSS)
+
+static void gauss2d(oreal xx, oreal yy, const oreal a[5], oreal *y, oreal dyda[5])
+{
+    oreal dx = xx - a[1], dy = yy - a[3];
+    oreal argx = (oreal)0.5 * (dx * dx) / (a[2] * a[2]);
+    oreal argy = (oreal)0.5 * (dy * dy) / (a[4] * a[4]);
+    oreal ex = (oreal)exp((double)(-argx - argy));
+    *y = a[0] * ex;
+    dyda[0] = ex;
+    dyda[1] = (*y) * (xx - a[1]) / (a[2] * a[2]);
+    dyda[2] = *y * (xx - a[1]) * (xx - a[1]) / (a[2] * a[2] * a[2]);
+    dyda[3] = *y * (yy - a[3]) / (a[4] * a[4]);
+    dyda[4] = *y * (yy - a[3]) * (yy - a[3]) / (a[4] * a[4] * a[4]);
+}
+
+static void lm5_coef(const oreal *x, const oreal *yv, const oreal *g, const oreal a[5], oreal alpha[5][5],
+                     oreal beta[5], oreal *chisq)
+{
+    for (int j = 0; j < 5; j++) {
+        for (int k = 0; k <= j; k++)
+            alpha[j][k] = 0.0;
+        beta[j] = 0.0;
+    }
+    *chisq = 0.0;
+    const oreal sig2i = (oreal)(1.0 / (1.0f * 1.0f));
+    for (int i = 0; i < NG2; i++) {
+        oreal ymod, dyda[5];
+        gauss2d(x[i], yv[i], a, &ymod, dyda);
+        oreal dy = g[i] - ymod;
+        for (int l = 0; l < 5; l++) {
+            oreal wt = dyda[l] * sig2i;
+            for (int m = 0; m <= l; m++)
+                alpha[l][m] += wt * dyda[m];
+            beta[l] += dy * wt;
+        }
+        *chisq += dy * dy * sig2i;
+    }
+    for (int j = 1; j < 5; j++)
+        for (int k = 0; k < j; k++)
+            alpha[k][j] = alpha[j][k];
+}
+
+static int solve5(oreal m[5][5], oreal b[5])
+{
+    int used[5] = {0, 0, 0, 0, 0};
+    for (int it = 0; it < 5; it++) {
+        oreal big = 0;
+        int r = -1, c = -1;
+        for (int j = 0; j < 5; j++) {
+            if (used[j])
+                continue;
+            for (int k = 0; k < 5; k++)
+                if (!used[k] && fabs((double)m[j][k]) >= big) {
+                    big = (oreal)fabs((double)m[j][k]);
+                    r = j;
+                    c = k;
+                }
+        }
+        if (r < 0)
+            return 0;
+        used[c] = 1;
+        if (r != c) {
+            for (int l = 0; l < 5; l++) {
+                oreal t = m[r][l];
+                m[r][l] = m[c][l];
+                m[c][l] = t;
+            }
+            oreal t = b[r];
+            b[r] = b[c];
+            b[c] = t;
+        }
+        if (m[c][c] == 0)
+            return 0;
+        oreal pinv = (oreal)(1.0 / m[c][c]);
+        m[c][c] = 1.0;
+        for (int l = 0; l < 5; l++)
+            m[c][l] *= pinv;
+        b[c] *= pinv;
+        for (int ll = 0; ll < 5; ll++)
+            if (ll != c) {
+                oreal d = m[ll][c];
+                m[ll][c] = 0.0;
+                for (int l = 0; l < 5; l++)
+                    m[ll][l] -= m[c][l] * d;
+                b[ll] -= b[c] * d;
+            }
+    }
+    return 1;
+}
+
+/* cmpTrackGauss: fit the 18 x 18 samples of the fftshifted magnitude mag
+ * (row stride W_R) around the coarse peak (iMax, jMax); cMax is findCPeak's
+ * raw maximum, wR / wA the window the surface was made from.  Returns the
+ * peak in 1/NOVER units and the fitted amplitude (capped at 1), or
+ * -LARGEINT / 0 when the peak is too close to the edge or a width exceeds
+ * 2000. */
+static void gauss_fit_2d(const oreal *mag, int W_R, int iMax, int jMax, double cMax, int wR, int wA, double p1,
+                         double p2, double hanningCorrection, int *oI, int *oJ, double *oC)
+{
+    const int ng = NGAUSS;
+    if ((iMax - ng / 2) < 0 || (iMax + ng / 2) >= (wA * OS - 1) || (jMax - ng / 2) < 0 ||
+        (jMax + ng / 2) >= (wR * OS - 1)) {
+        *oI = *oJ = (int)-ORC_LARGEINT;
+        *oC = 0;
+        return;
+    }
+    const double scale = hanningCorrection / (sqrt(p1 * p2) * pow(wR * wA, 2));
+    oreal x[NG2], yv[NG2], g[NG2];
+    int k = 0;
+    for (int i = 0; i < ng; i++) {
+        int i1 = iMax - ng / 2 + i, j1 = jMax - ng / 2;
+        for (int j = 0; j < ng; j++, j1++, k++) {
+            x[k] = (oreal)(float)i1;
+            yv[k] = (oreal)(float)j1;
+            g[k] = (oreal)(mag[i1 * W_R + j1] * scale);
+        }
+    }
+    oreal a[5] = {(oreal)(cMax * scale), (oreal)iMax, 3, (oreal)jMax, 3};
+    oreal alpha[5][5], beta[5], ochisq, chisq, lambda = (oreal)0.001;
+    lm5_coef(x, yv, g, a, alpha, beta, &chisq);
+    ochisq = chisq;
+    oreal lasti = (oreal)2e9, lastj = (oreal)2.0e9;
+    for (int it = 0; it < 8; it++) {
+        oreal covar[5][5], da[5];
+        for (int j = 0; j < 5; j++) {
+            for (int kk = 0; kk < 5; kk++)
+                covar[j][kk] = alpha[j][kk];
+            covar[j][j] = (oreal)(alpha[j][j] * (1.0 + lambda));
+            da[j] = beta[j];
+        }
+        if (!solve5(covar, da))
+            break;
+        oreal atry[5];
+        for (int l = 0; l < 5; l++)
+            atry[l] = a[l] + da[l];
+        oreal nalpha[5][5], nbeta[5];
+        lm5_coef(x, yv, g, atry, nalpha, nbeta, &chisq);
+        if (chisq < ochisq) {
+            lambda = (oreal)(lambda * 0.1);
+            ochisq = chisq;
+            memcpy(alpha, nalpha, sizeof(nalpha));
+            memcpy(beta, nbeta, sizeof(nbeta));
+            for (int l = 0; l < 5; l++)
+                a[l] = atry[l];
+        } else {
+            lambda = (oreal)(lambda * 10.0);
+        }
+        if (fabs((double)(a[1] - lasti)) < 0.001 && fabs((double)(a[3] - lastj)) < 0.001)
+            break;
+        lasti = a[1];
+        lastj = a[3];
+    }
+    *oI = (int)round((double)(a[1] * NOVER));
+    *oJ = (int)round((double)(a[3] * NOVER));
+    *oC = a[0] < 1.0 ? (double)a[0] : 1.0;
+    const float sigThresh = 1000;
+    if ((0.5 * a[2]) > sigThresh || (0.5 * a[4]) > sigThresh) {
+        *oI = *oJ = (int)-ORC_LARGEINT;
+        *oC = 0;
+    }
+}
+
 /* gaussPeak1D: fit 18 samples through (iMax, jMax) along azimuth or range;
  * returns sigma = width / 2. */
 static float gauss_peak_1d(const oreal *cMag, int W_R, int iMax, int jMax, int azOrRg)
@@ -731,7 +901,10 @@ static int complex_match(orc_st *s, int r1, int a1, int iOut, int jOut, int *pI,
     *pI = iMax;
     *pJ = jMax;
     s->gapFine = 0;
-    if (iMax >= -999) {
+    if (par->gaussFlag) {
+        /* complexMatch's -gauss branch (speckleTrack.c:367-372) */
+        gauss_fit_2d(s->mag, WR, iMax, jMax, cMax, wR, wA, s->p1, s->p2, hanningCorrection, &iMax, &jMax, &cMax);
+    } else if (iMax >= -999) {
         int iMax1, jMax1;
         cMax = oversample_peak(s, s->mag, WR, iMax, jMax, &iMax1, &jMax1);
         cMax = cMax / (NFAST * NFAST);
@@ -875,6 +1048,13 @@ static void amp_match(orc_st *s, int large, int iOut, int jOut, int *pI, int *pJ
         *cMaxO = cMax;
         return;
     }
+    if (par->gaussFlag) {
+        /* ampTrackFast's -gauss branch (speckleTrack.c:550-557): powers
+         * scaled by pScale = 16, no Hanning correction */
+        const double pScale = 16.;
+        gauss_fit_2d(s->mag, W_R, iMax, jMax, cMax, wR, wA, p1 * pScale, p2 * pScale, 1.0, iMaxO, jMaxO, cMaxO);
+        return;
+    }
     int iMax1, jMax1;
     cMax = oversample_peak(s, s->mag, W_R, iMax, jMax, &iMax1, &jMax1);
     cMax = cMax / (pow((double)(W_A), 2) * pow((double)(W_R), 2) * pow((double)NFAST, 2));
@@ -889,8 +1069,6 @@ static void amp_match(orc_st *s, int large, int iOut, int jOut, int *pI, int *pJ
 int orc_speckle_track(const st_track_params *par, const void *img1, int nr1, int na1, const void *img2, int nr2,
                       int na2, int stype, st_track_out *out, st_track_diag *dg)
 {
-    if (par->gaussFlag)
-        return ST_ERR_UNSUPPORTED;
     if (par->wR != par->wA || par->wRa != par->wAa || (par->wR & 1) || (par->wRa & 1))
         return ST_ERR_UNSUPPORTED;
     if (par->intf && !par->noComplex) {
@@ -1118,6 +1296,17 @@ void orc_hanning(int w, float *out)
     for (int k = 0; k < w * w; k++)
         out[k] = (float)s.hanning[k];
     free(s.hanning);
+}
+
+/* cmpTrackGauss alone, on a float magnitude surface of row stride W_R. */
+void orc_gauss_fit_2d(const float *magIn, int W_R, int W_A, int iMax, int jMax, double cMax, int wR, int wA,
+                      double p1, double p2, double hc, int *oI, int *oJ, double *oC)
+{
+    oreal *mag = (oreal *)malloc(sizeof(oreal) * (size_t)W_R * W_A);
+    for (int k = 0; k < W_R * W_A; k++)
+        mag[k] = (oreal)magIn[k];
+    gauss_fit_2d(mag, W_R, iMax, jMax, cMax, wR, wA, p1, p2, hc, oI, oJ, oC);
+    free(mag);
 }
 
 /* phaseCorrectInt alone: patch (wA x wR, in place) as patch 1 of the

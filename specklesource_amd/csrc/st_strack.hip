@@ -53,7 +53,7 @@ struct IntSet {
 
 struct KPar {
     int nR, nA, rStart, aStart, deltaR, deltaA, wR, wA, wRa, wAa, sf;
-    int noComplex, hanning, maxTries, checkAz;
+    int noComplex, hanning, maxTries, checkAz, gauss;
     float azThresh;
     double rPoly[3], aPoly[3];
     const uint8_t *mask;
@@ -399,10 +399,22 @@ __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *lis
 // argmax over the central box :686-700, sample extraction for
 // gaussPeak1D :2158-2180).  mode 0 = complex (Gaussian fits at the
 // refined peak / NOVER), 1 = amplitude (Gaussian fits at the coarse peak).
+// Gaussian-mode arguments of the peak stage: the window wWin of the surface,
+// the per-item powers p (p1, p2) and their factor pScale, the Hanning
+// correction hc (cmpTrackGauss's scale, speckleTrack.c:2249-2254).
+struct GaussArg {
+    int on, wWin;
+    double hc, pScale;
+    const double *p;
+};
+
 // The part of the peak stage after the coarse scan: border check, FP64
-// refinement of the 8 x 8 neighbourhood on the searched 48 x 48 box, and the
-// samples of the two Gaussian fits, from the fftshifted magnitude surface M.
-__device__ void peak_finish(const float *M, int W, int mode, int I, int J, float raw, float sec, PeakRes *out)
+// refinement of the 8 x 8 neighbourhood on the searched 48 x 48 box (or,
+// in -gauss mode, the 2-D Gaussian fit), and the samples of the two 1-D
+// Gaussian fits, from the fftshifted magnitude surface M.
+template <bool GAUSS>
+__device__ void peak_finish(const float *M, int W, int mode, int I, int J, float raw, float sec, PeakRes *out,
+                            const GaussArg &G, int item)
 {
     __shared__ ArgMaxD shd[NT / 64];
     __shared__ double shdd[NT / 64];
@@ -414,6 +426,8 @@ __device__ void peak_finish(const float *M, int W, int mode, int I, int J, float
     r.fi = r.fj = -1;
     r.fineMax = 0;
     r.fineGap = 0;
+    r.gI = r.gJ = -2000000000;
+    r.gC = 0;
     if (I < NFAST / 2 || I > W - NFAST / 2 || J < NFAST / 2 || J > W - NFAST / 2) {
         r.I = r.J = -1000;
         r.raw = -1.f;
@@ -424,6 +438,36 @@ __device__ void peak_finish(const float *M, int W, int mode, int I, int J, float
     r.I = I;
     r.J = J;
     r.raw = raw;
+    if constexpr (GAUSS) {
+        if (threadIdx.x == 0) {
+            const double p1 = G.p[2 * item] * G.pScale, p2 = G.p[2 * item + 1] * G.pScale;
+            const double scale = G.hc / (sqrt(p1 * p2) * pow((double)(G.wWin * G.wWin), 2));
+            gauss_fit_2d(M, W, I, J, (double)raw, G.wWin, G.wWin, scale, &r.gI, &r.gJ, &r.gC);
+            int cI, cJ;
+            bool ok;
+            if (mode == 0) {
+                cI = r.gI / NOVER;
+                cJ = r.gJ / NOVER;
+                ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < W && (cJ - NGAUSS / 2) >= 0 &&
+                     (cJ + NGAUSS / 2) < W;
+            } else {
+                cI = I;
+                cJ = J;
+                ok = I > NGAUSS / 2 && I < W - NGAUSS / 2 && J > NGAUSS / 2 && J < W - NGAUSS / 2;
+            }
+            r.lmOk = ok ? 1 : 0;
+            if (ok) {
+                for (int t = 0; t < NGAUSS; t++) {
+                    r.lmY[0][t] = M[cI * W + (cJ - NGAUSS / 2 + t)];
+                    r.lmY[1][t] = M[(cI - NGAUSS / 2 + t) * W + cJ];
+                }
+                r.lmX0[0] = (float)(cJ - NGAUSS / 2);
+                r.lmX0[1] = (float)(cI - NGAUSS / 2);
+            }
+            *out = r;
+        }
+        return;
+    }
     // FP64 refinement: Q = A P A^T on the searched 48 x 48 box
     if (threadIdx.x < NFAST * NFAST)
         Pm[threadIdx.x] = (double)M[(I - NFAST / 2 + threadIdx.x / NFAST) * W + (J - NFAST / 2 + threadIdx.x % NFAST)];
@@ -503,7 +547,10 @@ __device__ void peak_finish(const float *M, int W, int mode, int I, int J, float
 }
 
 
-__global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W, int mode, PeakRes *res)
+// GAUSS selects the -gauss fit (a separate instantiation: the fit's
+// registers would lower the occupancy of the default refinement).
+template <bool GAUSS>
+__global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W, int mode, PeakRes *res, GaussArg G)
 {
     __shared__ ArgMax sha[NT / 64];
     __shared__ float shf[NT / 64];
@@ -547,7 +594,7 @@ __global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W,
             sec = fmaxf(sec, M[e]);
     sec = block_max_f<NT>(sec, shf);
     PeakRes *out = res + b;
-    peak_finish(M, W, mode, I, J, raw, sec, out);
+    peak_finish<GAUSS>(M, W, mode, I, J, raw, sec, out, G, b);
 }
 
 // Two fits per attempt: [0] along range, [1] along azimuth.
@@ -580,7 +627,12 @@ __global__ void k_cpx_finalize(KPar P, Pts S, const int *list, int off, int B, C
     const double p1 = C.p[2 * b], p2 = C.p[2 * b + 1];
     double cMax;
     int iMax, jMax;
-    if (r.I < -999) {
+    if (P.gauss) {
+        // complexMatch's -gauss branch (speckleTrack.c:367-372)
+        cMax = r.gC;
+        iMax = r.gI;
+        jMax = r.gJ;
+    } else if (r.I < -999) {
         cMax = -1.0;
         iMax = jMax = -1000;
     } else {
@@ -900,6 +952,11 @@ __global__ void k_amp_finalize(KPar P, Pts S, const int *list, int off, int B, i
         cMax = r.I < -999 ? -1.0 : (double)r.raw;
         iMax = r.I;
         jMax = r.J;
+    } else if (P.gauss) {
+        // ampTrackFast's -gauss branch (speckleTrack.c:553-557)
+        cMax = r.gC;
+        iMax = r.gI;
+        jMax = r.gJ;
     } else {
         const double p1 = A.p[2 * b], p2 = A.p[2 * b + 1];
         cMax = r.fineMax / (pow((double)W, 2) * pow((double)W, 2) * pow((double)NFAST, 2));
@@ -1288,8 +1345,9 @@ __global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R
 
 // Top two over the row blocks of each patch (in block order), then the
 // refinement and the Gaussian-fit samples of k_peak.
+template <bool GAUSS>
 __global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, const float *mag, int W, int mode,
-                                                    PeakRes *res)
+                                                    PeakRes *res, GaussArg G)
 {
     const int b = blockIdx.x, H = W / 2;
     Top2 best = part[(size_t)b * nPart];
@@ -1304,7 +1362,7 @@ __global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, 
         J = (q == 0 || q == 2) ? j : j + H;
         raw = best.v1;
     }
-    peak_finish(mag + (size_t)b * W * W, W, mode, I, J, raw, best.v2, res + b);
+    peak_finish<GAUSS>(mag + (size_t)b * W * W, W, mode, I, J, raw, best.v2, res + b, G, b);
 }
 
 // Lanes per transform of the fused kernel.  Lane pairs at every window
@@ -2026,7 +2084,6 @@ int read_count(st_ctx *ctx, int *dcounts, int idx)
 void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, st_track_out *out,
                            st_track_diag *diag)
 {
-    ST_REQUIRE(!par.gaussFlag, ST_ERR_UNSUPPORTED, "-gauss peak mode is not implemented");
     ST_REQUIRE(par.wR == par.wA && par.wRa == par.wAa, ST_ERR_UNSUPPORTED, "non-square windows");
     ST_REQUIRE(par.wR >= 8 && par.wRa >= 8 && !(par.wR & 1) && !(par.wRa & 1), ST_ERR_ARG, "window sizes");
     ST_REQUIRE(stfft::supported_size(par.wR) && stfft::supported_size(2 * par.wR) &&
@@ -2043,9 +2100,11 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     // the matrix-core complex match covers windows 16 and 32 without the
     // baseline ramp; other cases run the FP32 paths
     const bool intOn = par.intf != nullptr && !par.noComplex;
+    // -gauss fits its peak on the staged path's magnitude surfaces
+    const bool gauss = par.gaussFlag != 0;
     const bool mfmaPath = par.precision == ST_PREC_MFMA && (par.wR == 16 || par.wR == 32) && !par.baseFlag &&
-                          !intOn && !staged_only();
-    const bool fusedPath = (par.wR == 16 || par.wR == 32 || par.wR == 64) && !staged_only();
+                          !intOn && !gauss && !staged_only();
+    const bool fusedPath = (par.wR == 16 || par.wR == 32 || par.wR == 64) && !gauss && !staged_only();
     const int kPeak = ctx->kernel_slot(mfmaPath ? "k_cpx_mfma" : (fusedPath ? "k_cpx_fused" : "peak_refine"));
     const int kGather = ctx->kernel_slot("cpx_gather"), kFft = ctx->kernel_slot("fft2d"),
               kAmpDet = ctx->kernel_slot("amp_detect");
@@ -2147,6 +2206,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     P.hanning = par.hanningFlag;
     P.maxTries = par.maxTries;
     P.checkAz = par.checkAzFocus;
+    P.gauss = gauss ? 1 : 0;
     P.azThresh = par.azDefocusThresh;
     for (int k = 0; k < 3; k++) {
         P.rPoly[k] = par.rShiftPoly[k];
@@ -2236,7 +2296,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     }
 
     // ---- complex attempts
-    const bool fused = (w == 16 || w == 32 || w == 64) && !staged_only();
+    const bool fused = fusedPath;
     if (nCpx > 0 && mfmaPath) {
         CpxBatch C{};
         C.res = ar.take<PeakRes>(nCpx);
@@ -2305,7 +2365,13 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
             hipLaunchKernelGGL(k_cpx_colx, dim3((w + Cx - 1) / Cx, nb), dim3(NT), ldsX, st, C.P, nb, w, C.az, C.S, lcx,
                                PfW, PiW2, dW2);
             hipLaunchKernelGGL(k_cpx_rowmag, dim3(nPart, nb), dim3(NT), ldsR, st, C.S, w, Rr, C.mag, part, PiW2, dW2);
-            hipLaunchKernelGGL(k_peak_merge, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0, C.res);
+            const GaussArg G{gauss ? 1 : 0, w, par.hanningFlag ? 1.5 : 1.0, 1.0, C.p};
+            if (gauss)
+                hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0, C.res,
+                                   G);
+            else
+                hipLaunchKernelGGL(k_peak_merge<false>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0,
+                                   C.res, G);
             ctx->time_end(kPeak);
             hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, C.res, nb, C.sig);
             hipLaunchKernelGGL(k_cpx_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, C);
@@ -2359,7 +2425,11 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
                                A.S);
             ctx->fft.fft2d(A.S, A.tmp, nb, W, W, +1, st);
             ctx->time_begin(kPeak);
-            hipLaunchKernelGGL(k_peak, dim3(nb), dim3(NT), 0, st, A.S, A.mag, W, 1, A.res);
+            const GaussArg G{gauss ? 1 : 0, W / 2, 1.0, 16.0, A.p};
+            if (gauss)
+                hipLaunchKernelGGL(k_peak<true>, dim3(nb), dim3(NT), 0, st, A.S, A.mag, W, 1, A.res, G);
+            else
+                hipLaunchKernelGGL(k_peak<false>, dim3(nb), dim3(NT), 0, st, A.S, A.mag, W, 1, A.res, G);
             ctx->time_end(kPeak);
             hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, A.res, nb, A.sig);
             hipLaunchKernelGGL(k_amp_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, large, A);
