@@ -231,6 +231,44 @@ int st_cull_islands(st_ctx *ctx, st_cull_params *cp);
  * cullStats, cullSmooth, cullIslands when islandThresh > 0. */
 int st_cull_run(st_ctx *ctx, st_cull_params *cp);
 
+/* ---- Landsat culling (Cullls/cullls.h:2-35) ---------------------------- */
+/* CullLSParams with the matchResult / lsFit fields the stages read.  The
+ * type bits, the no-data value and the oversampling factor come from
+ * landsatSource64 / mosaicSource headers that are not part of the
+ * reference tree, so the caller passes them. */
+typedef struct {
+    int32_t nx, ny;            /* matches.nx / ny */
+    float *X, *Y;              /* in/out: matches.X / Y, culled in place */
+    const uint8_t *type;       /* matches.type */
+    float *XS, *YS;            /* out: smoothed offsets */
+    float *sigmaX, *sigmaY;    /* out: matches.sigmaX / sigmaY */
+    int32_t bR, bA;            /* median box (boxSize, default 9) */
+    int32_t nGood;             /* default 17 */
+    float maxX, maxY;          /* default 0.75, 1.0 */
+    int32_t sX, sY;            /* smoothing window (default 3, 3) */
+    int32_t islandThresh;      /* -1 = off */
+    double deltaT;             /* fitDat.deltaT, days */
+    double dx, dy;             /* matches.dx / dy */
+    uint8_t slowVelMask;       /* SLOWVEL bit of matches.type */
+    uint8_t noDataToUseMask;   /* NODATATOUSE bit of matches.type */
+    float noData;              /* NODATA */
+    int32_t nover;             /* NOVER of the Landsat matcher (sigma quantisation term) */
+    int32_t nMatch, nAttempt;  /* in/out: set by the first data pass when nMatch < 0 */
+    int32_t nCulled;           /* out: points culled by the last data pass */
+} st_cull_ls_params;
+
+/* One pass of cullLSData (Cullls/cullLSData.c:13-134). */
+int st_cull_ls_data(st_ctx *ctx, st_cull_ls_params *cp);
+/* cullLSStats (Cullls/cullLSStats.c:21-160). */
+int st_cull_ls_stats(st_ctx *ctx, st_cull_ls_params *cp);
+/* cullLSSmooth (Cullls/cullLSSmooth.c:11-73). */
+int st_cull_ls_smooth(st_ctx *ctx, st_cull_ls_params *cp);
+/* cullLSIslands (Cullls/cullLSIslands.c:17-177) on XS / YS. */
+int st_cull_ls_islands(st_ctx *ctx, st_cull_ls_params *cp);
+/* The cullls call order (Cullls/cullls.c:23-58): nMatch = -1, cullLSData
+ * x3, cullLSStats, cullLSSmooth, cullLSIslands when islandThresh > 0. */
+int st_cull_ls_run(st_ctx *ctx, st_cull_ls_params *cp);
+
 /* ---- serving --------------------------------------------------------- */
 /* A server runs tracking jobs of different scene and window sizes
  * concurrently on one GPU (continuous batching).  Each of its slots owns a

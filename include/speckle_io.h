@@ -79,6 +79,26 @@ int st_write_cull(const char *outBase, const char *inBase, const st_cull_meta *m
                   const float *offAS, const float *sigmaR, const float *sigmaA, const float *corr,
                   const uint8_t *type, int32_t nR, int32_t nA, int32_t byteOrder);
 
+/* What the Cullls header records (matchResult / lsFit fields of the
+ * un-vendored landsatSource64). */
+typedef struct st_cull_ls_meta {
+    const char *fileEarly, *fileLate;
+    double x0, y0, dx, dy;
+    uint32_t stepX, stepY;
+    int32_t slowFlag, proj;
+    double jdEarly, jdLate;
+    float noData; /* NODATA: a point counts as good when XS and YS exceed NODATA + 1 */
+} st_cull_ls_meta;
+
+/* writeLSCulledOffsets (Cullls/cullls.c:63-175): matchFile.cull.dx .dy .sx
+ * .sy (float32, byteOrder; the reference writes MSB) and .cull.mtype, and
+ * matchFile.cull.dat with the reference's printf formats, the success and
+ * culled rates from nMatch / nAttempt (st_cull_ls_params) and the RMS
+ * sigmas over the good points. */
+int st_write_cull_ls(const char *matchFile, const st_cull_ls_meta *meta, const float *XS, const float *YS,
+                     const float *sigmaX, const float *sigmaY, const uint8_t *type, int32_t nx, int32_t ny,
+                     int32_t nMatch, int32_t nAttempt, int32_t byteOrder);
+
 /* ---- inputs before tracking ------------------------------------------ */
 
 /* parseInitialOffsets (Strack/parseInitialOffsets.c:11-64): the

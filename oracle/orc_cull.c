@@ -186,12 +186,23 @@ void orc_lsq3(double c0, const int *xi, const int *xj, const double *y, int n, d
     lsq3_svd(c0, xi, xj, y, n, x);
 }
 
+/* cullStats / cullLSStats body: sigmaR/A of offR/A; addQuant adds the
+ * quantisation variance `quant` before the square root (cullLSStats.c:144-146). */
+static void cull_stats_core(int nA, int nR, const float *offR, const float *offA, float *sigmaR, float *sigmaA,
+                            int addQuant, double quant);
+
 int orc_cull_stats(st_cull_params *cp)
 {
-    int nA = cp->nA, nR = cp->nR;
-    const int b = 9; /* cullStats.c:38-39 forces 9 x 9 */
-    cp->bR = b;
-    cp->bA = b;
+    cp->bR = 9; /* cullStats.c:38-39 forces 9 x 9 */
+    cp->bA = 9;
+    cull_stats_core(cp->nA, cp->nR, cp->offR, cp->offA, cp->sigmaR, cp->sigmaA, 0, 0.0);
+    return ST_OK;
+}
+
+static void cull_stats_core(int nA, int nR, const float *offR, const float *offA, float *sigmaR, float *sigmaA,
+                            int addQuant, double quant)
+{
+    const int b = 9;
     double listR[81], listA[81];
     int xi[81], xj[81];
     double aR[3] = {0, 0, 0}, aA[3] = {0, 0, 0};
@@ -206,11 +217,11 @@ int orc_cull_stats(st_cull_params *cp)
             for (int ii = i1; ii <= i2; ii++)
                 for (int jj = j1; jj <= j2; jj++) {
                     size_t q = (size_t)ii * nR + jj;
-                    if (valid2(cp->offR[q], cp->offA[q])) {
-                        listR[ngood] = cp->offR[q];
-                        listA[ngood] = cp->offA[q];
-                        meanA += cp->offA[q];
-                        meanR += cp->offR[q];
+                    if (valid2(offR[q], offA[q])) {
+                        listR[ngood] = offR[q];
+                        listA[ngood] = offA[q];
+                        meanA += offA[q];
+                        meanR += offR[q];
                         xi[ngood] = ii;
                         xj[ngood] = jj;
                         ngood++;
@@ -230,9 +241,9 @@ int orc_cull_stats(st_cull_params *cp)
             for (int ii = i1; ii <= i2; ii++)
                 for (int jj = j1; jj <= j2; jj++) {
                     size_t q = (size_t)ii * nR + jj;
-                    if (valid2(cp->offR[q], cp->offA[q])) {
+                    if (valid2(offR[q], offA[q])) {
                         ng++;
-                        float oR = cp->offR[q], oA = cp->offA[q];
+                        float oR = offR[q], oA = offA[q];
                         oR = (float)(oR - (aR[0] + aR[1] * (float)ii + aR[2] * (float)jj));
                         oA = (float)(oA - (aA[0] + aA[1] * (float)ii + aA[2] * (float)jj));
                         mA += oA;
@@ -242,22 +253,25 @@ int orc_cull_stats(st_cull_params *cp)
                     }
                 }
             size_t o = (size_t)i * nR + j;
-            if (ng > 8 && cp->offR[o] > (1 - ORC_LARGEINT)) {
+            if (ng > 8 && offR[o] > (1 - ORC_LARGEINT)) {
                 float sA = (float)(varA / (float)(ng - 3));
                 float sR = (float)(varR / (float)(ng - 3));
                 sA = (float)(sA - mA * mA / (float)((ng - 3) * ng));
                 sR = (float)(sR - mR * mR / (float)((ng - 3) * ng));
                 sA = (float)(0.00001 > sA ? 0.00001 : sA);
                 sR = (float)(0.00001 > sR ? 0.00001 : sR);
-                cp->sigmaA[o] = (float)sqrt((double)sA);
-                cp->sigmaR[o] = (float)sqrt((double)sR);
+                if (addQuant) {
+                    sR = (float)(sR + quant);
+                    sA = (float)(sA + quant);
+                }
+                sigmaA[o] = (float)sqrt((double)sA);
+                sigmaR[o] = (float)sqrt((double)sR);
             } else {
-                cp->sigmaA[o] = NEG_LARGE;
-                cp->sigmaR[o] = NEG_LARGE;
+                sigmaA[o] = NEG_LARGE;
+                sigmaR[o] = NEG_LARGE;
             }
         }
     }
-    return ST_OK;
 }
 
 /* sWeights, cullSmooth.c:10-27 (allocated at its full size here) */
@@ -385,5 +399,172 @@ int orc_cull_run(st_cull_params *cp)
     orc_cull_smooth(cp);
     if (cp->islandThresh > 0)
         orc_cull_islands(cp);
+    return ST_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Cullls (Landsat offsets): Cullls/cullLSData.c, cullLSStats.c,
+ * cullLSSmooth.c, cullLSIslands.c.  X / Y take the places of offR / offA.
+ * The SLOWVEL / NODATATOUSE bits, NODATA and the Landsat NOVER come from
+ * un-vendored headers and are parameters. */
+
+/* cullLSData.c:13-134 */
+int orc_cull_ls_data(st_cull_ls_params *cp)
+{
+    int ny = cp->ny, nx = cp->nx;
+    float *X = cp->X, *Y = cp->Y;
+    int nGoodPts = 0, nWithData = 0;
+    for (size_t o = 0; o < (size_t)nx * ny; o++) {
+        if (valid2(X[o], Y[o]))
+            nGoodPts++;
+        if ((cp->type[o] & cp->noDataToUseMask) == 0)
+            nWithData++;
+    }
+    if (cp->nMatch < 0) {
+        cp->nAttempt = nWithData;
+        cp->nMatch = nGoodPts;
+    }
+    size_t nlist = 1000 + (size_t)(cp->bR + 1) * (cp->bA + 1);
+    float *listX = (float *)malloc(sizeof(float) * nlist), *listY = (float *)malloc(sizeof(float) * nlist);
+    int nCulled = 0;
+    for (int i = 0; i < ny; i++) {
+        int i1 = i - cp->bA / 2 > 0 ? i - cp->bA / 2 : 0;
+        int i2 = i + cp->bA / 2 < ny - 1 ? i + cp->bA / 2 : ny - 1;
+        for (int j = 0; j < nx; j++) {
+            int j1 = j - cp->bR / 2 > 0 ? j - cp->bR / 2 : 0;
+            int j2 = j + cp->bR / 2 < nx - 1 ? j + cp->bR / 2 : nx - 1;
+            int slowFlag = 1;
+            unsigned ngood = 0;
+            for (int ii = i1; ii <= i2; ii++)
+                for (int jj = j1; jj <= j2; jj++) {
+                    size_t q = (size_t)ii * nx + jj;
+                    if (valid2(X[q], Y[q])) {
+                        listX[ngood] = X[q];
+                        listY[ngood] = Y[q];
+                        ngood++;
+                        if (!(cp->type[q] & cp->slowVelMask))
+                            slowFlag = 0;
+                    }
+                }
+            size_t o = (size_t)i * nx + j;
+            if (ngood > (unsigned)cp->nGood && (double)Y[o] > (double)cp->noData + 1.0) {
+                unsigned k = ngood / 2;
+                if (k == 0)
+                    k = 1;
+                qsort(listX, ngood, sizeof(float), cmpf);
+                qsort(listY, ngood, sizeof(float), cmpf);
+                float medianX = listX[k - 1], medianY = listY[k - 1];
+                double diffY = (double)(Y[o] - medianY);
+                double diffX = (double)(X[o] - medianX);
+                double maxX, maxY;
+                if (slowFlag) {
+                    /* 40 m/yr over deltaT days, in pixels, at least 0.2 */
+                    maxX = 40.0 / 365. * cp->deltaT;
+                    maxY = maxX;
+                    maxX /= cp->dx;
+                    maxY /= cp->dy;
+                    maxX = maxX > 0.2 ? maxX : 0.2;
+                    maxY = maxY > 0.2 ? maxY : 0.2;
+                } else {
+                    maxX = cp->maxX;
+                    maxY = cp->maxY;
+                }
+                if (fabs(diffY) > maxY || fabs(diffX) > maxX) {
+                    X[o] = NEG_LARGE;
+                    Y[o] = NEG_LARGE;
+                    nCulled++;
+                }
+            } else {
+                X[o] = NEG_LARGE;
+                Y[o] = NEG_LARGE;
+                nCulled++;
+            }
+        }
+    }
+    free(listX);
+    free(listY);
+    cp->nCulled = nCulled;
+    return ST_OK;
+}
+
+/* cullLSStats.c:21-160: cullStats plus the quantisation variance
+ * (1/NOVER)^2 / 12 */
+int orc_cull_ls_stats(st_cull_ls_params *cp)
+{
+    if (cp->nover <= 0)
+        return ST_ERR_ARG;
+    cp->bR = 9;
+    cp->bA = 9;
+    const double q = ((1.0 / cp->nover) * (1.0 / cp->nover)) / 12.;
+    cull_stats_core(cp->ny, cp->nx, cp->X, cp->Y, cp->sigmaX, cp->sigmaY, 1, q);
+    return ST_OK;
+}
+
+/* cullLSSmooth.c:11-73: unweighted box mean, effective looks = count */
+int orc_cull_ls_smooth(st_cull_ls_params *cp)
+{
+    int ny = cp->ny, nx = cp->nx;
+    for (int i = 0; i < ny; i++) {
+        int i1 = i - cp->sY / 2 > 0 ? i - cp->sY / 2 : 0;
+        int i2 = i + cp->sY / 2 < ny - 1 ? i + cp->sY / 2 : ny - 1;
+        for (int j = 0; j < nx; j++) {
+            int j1 = j - cp->sX / 2 > 0 ? j - cp->sX / 2 : 0;
+            int j2 = j + cp->sX / 2 < nx - 1 ? j + cp->sX / 2 : nx - 1;
+            unsigned ngood = 0;
+            float nLooksEff = 0.0f;
+            double meanY = 0.0, meanX = 0.0;
+            for (int ii = i1; ii <= i2; ii++)
+                for (int jj = j1; jj <= j2; jj++) {
+                    size_t q = (size_t)ii * nx + jj;
+                    if (valid2(cp->X[q], cp->Y[q])) {
+                        ngood++;
+                        nLooksEff++;
+                        meanY += cp->Y[q];
+                        meanX += cp->X[q];
+                    }
+                }
+            size_t o = (size_t)i * nx + j;
+            if (((ngood > 1) || (cp->sY == 1) || (cp->sX == 1)) && cp->X[o] > (1 - ORC_LARGEINT) &&
+                cp->sigmaY[o] > (1 - ORC_LARGEINT)) {
+                nLooksEff = (float)(nLooksEff > 1. ? nLooksEff : 1.);
+                cp->sigmaY[o] = (float)(cp->sigmaY[o] / sqrt((double)nLooksEff));
+                cp->sigmaX[o] = (float)(cp->sigmaX[o] / sqrt((double)nLooksEff));
+                cp->XS[o] = (float)(meanX / (float)ngood);
+                cp->YS[o] = (float)(meanY / (float)ngood);
+            } else {
+                cp->XS[o] = NEG_LARGE;
+                cp->YS[o] = NEG_LARGE;
+            }
+        }
+    }
+    return ST_OK;
+}
+
+/* cullLSIslands.c:17-177: cullIslands on XS / YS */
+int orc_cull_ls_islands(st_cull_ls_params *cp)
+{
+    st_cull_params c;
+    memset(&c, 0, sizeof(c));
+    c.nR = cp->nx;
+    c.nA = cp->ny;
+    c.offRS = cp->XS;
+    c.offAS = cp->YS;
+    c.islandThresh = cp->islandThresh;
+    return orc_cull_islands(&c);
+}
+
+/* cullls.c:23-58 */
+int orc_cull_ls_run(st_cull_ls_params *cp)
+{
+    cp->nMatch = -1;
+    orc_cull_ls_data(cp);
+    orc_cull_ls_data(cp);
+    orc_cull_ls_data(cp);
+    int rc = orc_cull_ls_stats(cp);
+    if (rc != ST_OK)
+        return rc;
+    orc_cull_ls_smooth(cp);
+    if (cp->islandThresh > 0)
+        orc_cull_ls_islands(cp);
     return ST_OK;
 }

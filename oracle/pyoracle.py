@@ -41,6 +41,9 @@ def _load(prec):
     for nm in ("orc_cull_st_data", "orc_cull_stats", "orc_cull_smooth", "orc_cull_islands",
                "orc_cull_run"):
         getattr(lib, nm).argtypes = [C.POINTER(abi.CullParams)]
+    for nm in ("orc_cull_ls_data", "orc_cull_ls_stats", "orc_cull_ls_smooth", "orc_cull_ls_islands",
+               "orc_cull_ls_run"):
+        getattr(lib, nm).argtypes = [C.POINTER(abi.CullLSParams)]
     lib.orc_fft1d.argtypes = [vp, C.c_int, C.c_int]
     lib.orc_fft2d.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     _libs[prec] = lib

@@ -109,6 +109,24 @@ class CullParams(C.Structure):
     ]
 
 
+class CullLSParams(C.Structure):
+    """st_cull_ls_params: CullLSParams fields of Cullls/cullls.h:2-28 plus
+    the matchResult / lsFit fields the stages read."""
+
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32),
+        ("X", _f), ("Y", _f), ("type", _u8),
+        ("XS", _f), ("YS", _f), ("sigmaX", _f), ("sigmaY", _f),
+        ("bR", C.c_int32), ("bA", C.c_int32), ("nGood", C.c_int32),
+        ("maxX", C.c_float), ("maxY", C.c_float),
+        ("sX", C.c_int32), ("sY", C.c_int32), ("islandThresh", C.c_int32),
+        ("deltaT", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+        ("slowVelMask", C.c_uint8), ("noDataToUseMask", C.c_uint8),
+        ("noData", C.c_float), ("nover", C.c_int32),
+        ("nMatch", C.c_int32), ("nAttempt", C.c_int32), ("nCulled", C.c_int32),
+    ]
+
+
 ST_JOB_STRACK, ST_JOB_STRACKW = 0, 1
 ST_JOB_QUEUED, ST_JOB_RUNNING, ST_JOB_DONE, ST_JOB_FAILED = 0, 1, 2, 3
 

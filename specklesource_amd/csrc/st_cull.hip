@@ -74,11 +74,47 @@ __global__ void k_cull_prepass(CullDev D, float corrThresh, int singleMT)
     D.cPrev[c] = 0;
 }
 
+// Start of a cullLSData pass (cullLSData.c:37-58): the pass state, and the
+// counts of good cells ([3]) and cells with data ([4]).
+__global__ void k_cull_ls_begin(CullDev D, const uint8_t *type, uint8_t noDataToUse)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = c < D.nR * D.nA;
+    bool good = false, data = false;
+    if (in) {
+        D.R0[c] = D.offR[c];
+        D.A0[c] = D.offA[c];
+        D.cPrev[c] = 0;
+        good = valid2(D.offR[c], D.offA[c]);
+        data = (type[c] & noDataToUse) == 0;
+    }
+    const unsigned long long g = __ballot(good), d = __ballot(data);
+    if ((threadIdx.x & 63) == 0) {
+        if (g)
+            atomicAdd(&D.counters[3], __popcll(g));
+        if (d)
+            atomicAdd(&D.counters[4], __popcll(d));
+    }
+}
+
+// Landsat variant of the median test (cullLSData.c:66-121): the box's
+// good points all carrying the slow-velocity bit switch the limits to
+// 40 m/yr over deltaT in pixels (at least 0.2), and a cell whose Y is not
+// above NODATA + 1 is culled.
+struct LsArg {
+    const uint8_t *type;
+    uint8_t slow;
+    double noData1;  // NODATA + 1
+    double deltaT, dx, dy;
+};
+
 // Median test of one cell by one wavefront (cullSTData.c:73-122): the box
 // values are ranked by counting, the floor(n/2)-th smallest is the median
-// (selectForCull's k = n/2 on a 1-based list).
+// (selectForCull's k = n/2 on a 1-based list).  LS selects the Landsat
+// limits (cullLSData.c); maxA / maxR are then maxY / maxX.
+template <bool LS>
 __device__ inline bool cull_decide(const CullDev &D, int c, int bR, int bA, int nGood, float maxA, float maxR,
-                                   const uint8_t *earlier, float *lr, float *la)
+                                   const uint8_t *earlier, float *lr, float *la, const LsArg &L)
 {
     const int lane = threadIdx.x & 63;
     const int i = c / D.nR, j = c - (c / D.nR) * D.nR;
@@ -86,9 +122,10 @@ __device__ inline bool cull_decide(const CullDev &D, int c, int bR, int bA, int 
     const int j1 = imax(0, j - bR / 2), j2 = imin(D.nR - 1, j + bR / 2);
     const int bw = j2 - j1 + 1, cnt = bw * (i2 - i1 + 1);
     int ngood = 0;
+    bool anyFast = false;
     for (int base = 0; base < cnt; base += 64) {
         const int s = base + lane;
-        bool ok = false;
+        bool ok = false, fast = false;
         float r = 0, a = 0;
         if (s < cnt) {
             const int q = (i1 + s / bw) * D.nR + j1 + s % bw;
@@ -96,7 +133,11 @@ __device__ inline bool cull_decide(const CullDev &D, int c, int bR, int bA, int 
             r = culled ? kNoData : D.R0[q];
             a = culled ? kNoData : D.A0[q];
             ok = valid2(r, a);
+            if constexpr (LS)
+                fast = ok && !(L.type[q] & L.slow);
         }
+        if constexpr (LS)
+            anyFast = anyFast || __ballot(fast) != 0ull;
         const unsigned long long m = __ballot(ok);
         const int pos = ngood + __popcll(m & ((1ull << lane) - 1ull));
         if (ok) {
@@ -110,6 +151,10 @@ __device__ inline bool cull_decide(const CullDev &D, int c, int bR, int bA, int 
     __builtin_amdgcn_wave_barrier();
     if (ngood <= nGood)
         return true;
+    if constexpr (LS) {
+        if (!((double)D.A0[c] > L.noData1))
+            return true;
+    }
     int k = ngood / 2;
     if (k == 0)
         k = 1;
@@ -140,17 +185,31 @@ __device__ inline bool cull_decide(const CullDev &D, int c, int bR, int bA, int 
     }
     const double diffA = (double)(D.A0[c] - medA);
     const double diffR = (double)(D.R0[c] - medR);
+    if constexpr (LS) {
+        double mX = maxR, mY = maxA;
+        if (!anyFast) {
+            mX = 40.0 / 365. * L.deltaT;
+            mY = mX;
+            mX /= L.dx;
+            mY /= L.dy;
+            mX = mX > 0.2 ? mX : 0.2;
+            mY = mY > 0.2 ? mY : 0.2;
+        }
+        return fabs(diffA) > mY || fabs(diffR) > mX;
+    }
     return fabs(diffA) > maxA || fabs(diffR) > maxR;
 }
 
-__global__ void __launch_bounds__(256) k_cull_iter(CullDev D, int bR, int bA, int nGood, float maxA, float maxR)
+template <bool LS>
+__global__ void __launch_bounds__(256) k_cull_iter(CullDev D, int bR, int bA, int nGood, float maxA, float maxR,
+                                                   LsArg L)
 {
     __shared__ float lr[4][kMaxBox * kMaxBox], la[4][kMaxBox * kMaxBox];
     const int w = threadIdx.x >> 6;
     const int c = blockIdx.x * 4 + w;
     if (c >= D.nR * D.nA)
         return;
-    const bool cull = cull_decide(D, c, bR, bA, nGood, maxA, maxR, D.cPrev, lr[w], la[w]);
+    const bool cull = cull_decide<LS>(D, c, bR, bA, nGood, maxA, maxR, D.cPrev, lr[w], la[w], L);
     if ((threadIdx.x & 63) == 0) {
         D.cNew[c] = cull ? 1 : 0;
         if ((D.cPrev[c] != 0) != cull)
@@ -160,7 +219,9 @@ __global__ void __launch_bounds__(256) k_cull_iter(CullDev D, int bR, int bA, in
 
 // Exact sequential order for the leftover case: one workgroup sweeps the
 // wavefront steps, one wave per cell of a step.
-__global__ void __launch_bounds__(1024) k_cull_wavefront(CullDev D, int bR, int bA, int nGood, float maxA, float maxR)
+template <bool LS>
+__global__ void __launch_bounds__(1024) k_cull_wavefront(CullDev D, int bR, int bA, int nGood, float maxA, float maxR,
+                                                         LsArg L)
 {
     __shared__ float lr[16][kMaxBox * kMaxBox], la[16][kMaxBox * kMaxBox];
     const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -172,7 +233,7 @@ __global__ void __launch_bounds__(1024) k_cull_wavefront(CullDev D, int bR, int 
         for (int i = iLo + w; i <= iHi; i += nw) {
             const int j = T - i * s;
             const int c = i * D.nR + j;
-            const bool cull = cull_decide(D, c, bR, bA, nGood, maxA, maxR, D.cNew, lr[w], la[w]);
+            const bool cull = cull_decide<LS>(D, c, bR, bA, nGood, maxA, maxR, D.cNew, lr[w], la[w], L);
             if ((threadIdx.x & 63) == 0)
                 D.cNew[c] = cull ? 1 : 0;
         }
@@ -266,7 +327,9 @@ __device__ void lsq3(double c0, const CullDev &D, int i1, int i2, int j1, int j2
         x[r] = V[r * 3 + 0] * coef[0] + V[r * 3 + 1] * coef[1] + V[r * 3 + 2] * coef[2];
 }
 
-__global__ void k_cull_stats(CullDev D)
+// addQuant: cullLSStats adds the quantisation variance `quant` before the
+// square root (cullLSStats.c:144-146).
+__global__ void k_cull_stats(CullDev D, int addQuant, double quant)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= D.nR * D.nA)
@@ -320,6 +383,10 @@ __global__ void k_cull_stats(CullDev D)
     sR = (float)(sR - mR * mR / (float)((ng - 3) * ng));
     sA = (float)(0.00001 > sA ? 0.00001 : sA);
     sR = (float)(0.00001 > sR ? 0.00001 : sR);
+    if (addQuant) {
+        sR = (float)(sR + quant);
+        sA = (float)(sA + quant);
+    }
     D.sigmaA[c] = (float)sqrt((double)sA);
     D.sigmaR[c] = (float)sqrt((double)sR);
 }
@@ -360,6 +427,42 @@ __global__ void k_cull_smooth(CullDev D, int sR, int sA)
         D.sigmaR[c] = (float)(D.sigmaR[c] / sqrt((double)nLooksEff));
         D.offRS[c] = (float)(meanR / (float)ngood);
         D.offAS[c] = (float)(meanA / (float)ngood);
+    } else {
+        D.offRS[c] = kNoData;
+        D.offAS[c] = kNoData;
+    }
+}
+
+// cullLSSmooth (cullLSSmooth.c:11-73): unweighted box mean, effective
+// looks = the count of good cells.
+__global__ void k_cull_ls_smooth(CullDev D, int sX, int sY)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D.nR * D.nA)
+        return;
+    const int i = c / D.nR, j = c - (c / D.nR) * D.nR;
+    const int i1 = imax(0, i - sY / 2), i2 = imin(D.nA - 1, i + sY / 2);
+    const int j1 = imax(0, j - sX / 2), j2 = imin(D.nR - 1, j + sX / 2);
+    unsigned ngood = 0;
+    float nLooksEff = 0.0f;
+    double meanY = 0.0, meanX = 0.0;
+    for (int ii = i1; ii <= i2; ii++)
+        for (int jj = j1; jj <= j2; jj++) {
+            const int o = ii * D.nR + jj;
+            if (valid2(D.offR[o], D.offA[o])) {
+                ngood++;
+                nLooksEff++;
+                meanY += D.offA[o];
+                meanX += D.offR[o];
+            }
+        }
+    if (((ngood > 1) || (sY == 1) || (sX == 1)) && D.offR[c] > (float)(1 - ST_LARGEINT) &&
+        D.sigmaA[c] > (float)(1 - ST_LARGEINT)) {
+        nLooksEff = (float)(nLooksEff > 1. ? nLooksEff : 1.);
+        D.sigmaA[c] = (float)(D.sigmaA[c] / sqrt((double)nLooksEff));
+        D.sigmaR[c] = (float)(D.sigmaR[c] / sqrt((double)nLooksEff));
+        D.offRS[c] = (float)(meanX / (float)ngood);
+        D.offAS[c] = (float)(meanY / (float)ngood);
     } else {
         D.offRS[c] = kNoData;
         D.offAS[c] = kNoData;
@@ -535,10 +638,20 @@ struct CullRun {
     {
         ST_REQUIRE(cp && cp->nR > 0 && cp->nA > 0 && cp->offR && cp->offA, ST_ERR_ARG, "cull grids");
         n = cp->nR * cp->nA;
+        alloc(cp->nR, cp->nA);
+        if (cp->mask) {
+            uint8_t *m = ctx->arena.take<uint8_t>(n);
+            put(m, cp->mask, n);
+            D.mask = m;
+        }
+    }
+    // Device copies of every grid of the stages (n x 72 bytes).
+    void alloc(int nR, int nA)
+    {
         ctx->arena_reserve((size_t)n * 72 + 65536);
         Arena &ar = ctx->arena;
-        D.nR = cp->nR;
-        D.nA = cp->nA;
+        D.nR = nR;
+        D.nA = nA;
         D.offR = ar.take<float>(n);
         D.offA = ar.take<float>(n);
         D.corr = ar.take<float>(n);
@@ -556,11 +669,11 @@ struct CullRun {
         D.parent = ar.take<int>(n);
         D.root = ar.take<int>(n);
         D.bbox = ar.take<int>(4 * (size_t)n);
-        if (cp->mask) {
-            uint8_t *m = ar.take<uint8_t>(n);
-            put(m, cp->mask, n);
-            D.mask = m;
-        }
+    }
+    CullRun(st_ctx *c, int nR, int nA) : ctx(c), cp(nullptr)
+    {
+        n = nR * nA;
+        alloc(nR, nA);
     }
     template <class T>
     void put(T *d, const T *h, size_t cnt)
@@ -584,17 +697,23 @@ struct CullRun {
 
     void st_data()
     {
-        ST_REQUIRE(cp->bR >= 1 && cp->bA >= 1 && (cp->bR / 2) * 2 + 1 <= kMaxBox && (cp->bA / 2) * 2 + 1 <= kMaxBox,
+        hipLaunchKernelGGL(k_cull_prepass, dim3(nb(n)), dim3(256), 0, ctx->stream, D, cp->corrThresh, cp->singleMT);
+        cp->nCulled = median_pass<false>(cp->bR, cp->bA, cp->nGood, cp->maxA, cp->maxR, LsArg{});
+    }
+    // The in-place raster-order median pass on the state left in R0 / A0:
+    // fixpoint iterations, the wavefront kernel if they have not settled,
+    // then the culls are applied.  Returns the number of culled cells.
+    template <bool LS>
+    int median_pass(int bR, int bA, int nGood, float maxA, float maxR, const LsArg &L)
+    {
+        ST_REQUIRE(bR >= 1 && bA >= 1 && (bR / 2) * 2 + 1 <= kMaxBox && (bA / 2) * 2 + 1 <= kMaxBox,
                    ST_ERR_UNSUPPORTED, "box sizes above 15");
         hipStream_t st = ctx->stream;
-        hipLaunchKernelGGL(k_cull_prepass, dim3(nb(n)), dim3(256), 0, st, D, cp->corrThresh, cp->singleMT);
-        int it = 0;
         bool settled = false;
         const int maxIter = max_iterations();
-        for (; it < maxIter; it++) {
+        for (int it = 0; it < maxIter; it++) {
             ST_HIP(hipMemsetAsync(D.counters, 0, sizeof(int), st));
-            hipLaunchKernelGGL(k_cull_iter, dim3((n + 3) / 4), dim3(256), 0, st, D, cp->bR, cp->bA, cp->nGood,
-                               cp->maxA, cp->maxR);
+            hipLaunchKernelGGL(k_cull_iter<LS>, dim3((n + 3) / 4), dim3(256), 0, st, D, bR, bA, nGood, maxA, maxR, L);
             ST_HIP(hipGetLastError());
             std::swap(D.cPrev, D.cNew);
             if (counter(0) == 0) {
@@ -604,17 +723,16 @@ struct CullRun {
         }
         std::swap(D.cPrev, D.cNew);  // cNew holds the last decisions
         if (!settled)
-            hipLaunchKernelGGL(k_cull_wavefront, dim3(1), dim3(1024), 0, st, D, cp->bR, cp->bA, cp->nGood, cp->maxA,
-                               cp->maxR);
+            hipLaunchKernelGGL(k_cull_wavefront<LS>, dim3(1), dim3(1024), 0, st, D, bR, bA, nGood, maxA, maxR, L);
         ST_HIP(hipMemsetAsync(D.counters + 1, 0, sizeof(int), st));
         hipLaunchKernelGGL(k_cull_apply, dim3(nb(n)), dim3(256), 0, st, D);
         ST_HIP(hipGetLastError());
-        cp->nCulled = counter(1);
+        return counter(1);
     }
     void stats()
     {
         cp->bR = cp->bA = 9;  // cullStats.c:38-39
-        hipLaunchKernelGGL(k_cull_stats, dim3(nb(n)), dim3(256), 0, ctx->stream, D);
+        hipLaunchKernelGGL(k_cull_stats, dim3(nb(n)), dim3(256), 0, ctx->stream, D, 0, 0.0);
         ST_HIP(hipGetLastError());
     }
     void smooth()
@@ -623,7 +741,8 @@ struct CullRun {
         hipLaunchKernelGGL(k_cull_smooth, dim3(nb(n)), dim3(256), 0, ctx->stream, D, cp->sR, cp->sA);
         ST_HIP(hipGetLastError());
     }
-    void islands()
+    void islands() { islands_t(cp->islandThresh); }
+    void islands_t(int thresh)
     {
         hipStream_t st = ctx->stream;
         ST_HIP(hipMemsetAsync(D.counters + 2, 0xff, sizeof(int), st));  // max root = -1
@@ -631,7 +750,7 @@ struct CullRun {
         hipLaunchKernelGGL(k_isl_bbox_init, dim3(nb(n)), dim3(256), 0, st, D);
         hipLaunchKernelGGL(k_isl_merge, dim3(nb(n)), dim3(256), 0, st, D);
         hipLaunchKernelGGL(k_isl_bbox, dim3(nb(n)), dim3(256), 0, st, D);
-        hipLaunchKernelGGL(k_isl_remove, dim3(nb(n)), dim3(256), 0, st, D, cp->islandThresh);
+        hipLaunchKernelGGL(k_isl_remove, dim3(nb(n)), dim3(256), 0, st, D, thresh);
         ST_HIP(hipGetLastError());
     }
 };
@@ -730,6 +849,151 @@ extern "C" int st_cull_run(st_ctx *c, st_cull_params *cp)
         R.get(cp->sigmaA, R.D.sigmaA, R.n);
         R.get(cp->offRS, R.D.offRS, R.n);
         R.get(cp->offAS, R.D.offAS, R.n);
+        ST_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+namespace {
+
+// Cullls on the cullst machinery: X / Y in the offR / offA slots, XS / YS
+// in offRS / offAS, sigmaX / sigmaY in sigmaR / sigmaA.
+struct CullLSRun {
+    st_cull_ls_params *cp;
+    CullRun R;
+    LsArg L{};
+
+    static st_cull_ls_params *check(st_cull_ls_params *p)
+    {
+        ST_REQUIRE(p && p->nx > 0 && p->ny > 0 && p->X && p->Y, ST_ERR_ARG, "cull grids");
+        return p;
+    }
+    CullLSRun(st_ctx *c, st_cull_ls_params *p) : cp(check(p)), R(c, p->nx, p->ny) {}
+
+    void ls_data()
+    {
+        ST_REQUIRE(cp->dx != 0 && cp->dy != 0, ST_ERR_ARG, "pixel sizes");
+        hipStream_t st = R.ctx->stream;
+        ST_HIP(hipMemsetAsync(R.D.counters + 3, 0, 2 * sizeof(int), st));
+        hipLaunchKernelGGL(k_cull_ls_begin, dim3(nb(R.n)), dim3(256), 0, st, R.D, R.D.type, cp->noDataToUseMask);
+        ST_HIP(hipGetLastError());
+        const int nGoodPts = R.counter(3), nWithData = R.counter(4);
+        if (cp->nMatch < 0) {
+            cp->nAttempt = nWithData;
+            cp->nMatch = nGoodPts;
+        }
+        L.type = R.D.type;
+        L.slow = cp->slowVelMask;
+        L.noData1 = (double)cp->noData + 1.0;
+        L.deltaT = cp->deltaT;
+        L.dx = cp->dx;
+        L.dy = cp->dy;
+        cp->nCulled = R.median_pass<true>(cp->bR, cp->bA, cp->nGood, cp->maxY, cp->maxX, L);
+    }
+    void ls_stats()
+    {
+        ST_REQUIRE(cp->nover > 0, ST_ERR_ARG, "nover");
+        cp->bR = cp->bA = 9;  // cullLSStats.c:39-40
+        const double q = ((1.0 / cp->nover) * (1.0 / cp->nover)) / 12.;
+        hipLaunchKernelGGL(k_cull_stats, dim3(nb(R.n)), dim3(256), 0, R.ctx->stream, R.D, 1, q);
+        ST_HIP(hipGetLastError());
+    }
+    void ls_smooth()
+    {
+        ST_REQUIRE(cp->sX >= 1 && cp->sY >= 1, ST_ERR_ARG, "smoothing window");
+        hipLaunchKernelGGL(k_cull_ls_smooth, dim3(nb(R.n)), dim3(256), 0, R.ctx->stream, R.D, cp->sX, cp->sY);
+        ST_HIP(hipGetLastError());
+    }
+};
+
+}  // namespace
+
+extern "C" int st_cull_ls_data(st_ctx *c, st_cull_ls_params *cp)
+{
+    return st_guard(c, [&]() {
+        CullLSRun L(c, cp);
+        ST_REQUIRE(cp->type, ST_ERR_ARG, "type");
+        L.R.put(L.R.D.offR, cp->X, L.R.n);
+        L.R.put(L.R.D.offA, cp->Y, L.R.n);
+        L.R.put(L.R.D.type, cp->type, L.R.n);
+        L.ls_data();
+        L.R.get(cp->X, L.R.D.offR, L.R.n);
+        L.R.get(cp->Y, L.R.D.offA, L.R.n);
+        ST_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int st_cull_ls_stats(st_ctx *c, st_cull_ls_params *cp)
+{
+    return st_guard(c, [&]() {
+        CullLSRun L(c, cp);
+        ST_REQUIRE(cp->sigmaX && cp->sigmaY, ST_ERR_ARG, "sigma outputs");
+        L.R.put(L.R.D.offR, cp->X, L.R.n);
+        L.R.put(L.R.D.offA, cp->Y, L.R.n);
+        L.ls_stats();
+        L.R.get(cp->sigmaX, L.R.D.sigmaR, L.R.n);
+        L.R.get(cp->sigmaY, L.R.D.sigmaA, L.R.n);
+        ST_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int st_cull_ls_smooth(st_ctx *c, st_cull_ls_params *cp)
+{
+    return st_guard(c, [&]() {
+        CullLSRun L(c, cp);
+        ST_REQUIRE(cp->sigmaX && cp->sigmaY && cp->XS && cp->YS, ST_ERR_ARG, "smooth arrays");
+        L.R.put(L.R.D.offR, cp->X, L.R.n);
+        L.R.put(L.R.D.offA, cp->Y, L.R.n);
+        L.R.put(L.R.D.sigmaR, cp->sigmaX, L.R.n);
+        L.R.put(L.R.D.sigmaA, cp->sigmaY, L.R.n);
+        L.ls_smooth();
+        L.R.get(cp->sigmaX, L.R.D.sigmaR, L.R.n);
+        L.R.get(cp->sigmaY, L.R.D.sigmaA, L.R.n);
+        L.R.get(cp->XS, L.R.D.offRS, L.R.n);
+        L.R.get(cp->YS, L.R.D.offAS, L.R.n);
+        ST_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int st_cull_ls_islands(st_ctx *c, st_cull_ls_params *cp)
+{
+    return st_guard(c, [&]() {
+        CullLSRun L(c, cp);
+        ST_REQUIRE(cp->XS, ST_ERR_ARG, "XS");
+        L.R.put(L.R.D.offRS, cp->XS, L.R.n);
+        if (cp->YS)
+            L.R.put(L.R.D.offAS, cp->YS, L.R.n);
+        else
+            L.R.D.offAS = nullptr;
+        L.R.islands_t(cp->islandThresh);
+        L.R.get(cp->XS, L.R.D.offRS, L.R.n);
+        if (cp->YS)
+            L.R.get(cp->YS, L.R.D.offAS, L.R.n);
+        ST_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int st_cull_ls_run(st_ctx *c, st_cull_ls_params *cp)
+{
+    return st_guard(c, [&]() {
+        CullLSRun L(c, cp);
+        ST_REQUIRE(cp->type && cp->sigmaX && cp->sigmaY && cp->XS && cp->YS, ST_ERR_ARG, "cull arrays");
+        L.R.put(L.R.D.offR, cp->X, L.R.n);
+        L.R.put(L.R.D.offA, cp->Y, L.R.n);
+        L.R.put(L.R.D.type, cp->type, L.R.n);
+        cp->nMatch = -1;
+        L.ls_data();
+        L.ls_data();
+        L.ls_data();
+        L.ls_stats();
+        L.ls_smooth();
+        if (cp->islandThresh > 0)
+            L.R.islands_t(cp->islandThresh);
+        L.R.get(cp->X, L.R.D.offR, L.R.n);
+        L.R.get(cp->Y, L.R.D.offA, L.R.n);
+        L.R.get(cp->sigmaX, L.R.D.sigmaR, L.R.n);
+        L.R.get(cp->sigmaY, L.R.D.sigmaA, L.R.n);
+        L.R.get(cp->XS, L.R.D.offRS, L.R.n);
+        L.R.get(cp->YS, L.R.D.offAS, L.R.n);
         ST_HIP(hipStreamSynchronize(c->stream));
     });
 }

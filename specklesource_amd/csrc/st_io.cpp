@@ -7,6 +7,7 @@
 // formats; the VRT files are GDAL VRTRawRasterBand descriptions with the
 // metadata items the reference records, in the order it inserts them.
 #include <libgen.h>
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -496,5 +497,66 @@ extern "C" int st_read_interferogram(const char *intFile, int32_t nr, int32_t na
             for (size_t k = 0; k < 2 * (size_t)nr; k++)
                 dst[k] = (float)(int16_t)__builtin_bswap16(row[k]);
         }
+    });
+}
+
+// ---- Cullls products ------------------------------------------------------
+
+extern "C" int st_write_cull_ls(const char *matchFile, const st_cull_ls_meta *m, const float *XS, const float *YS,
+                                const float *sigmaX, const float *sigmaY, const uint8_t *type, int32_t nx, int32_t ny,
+                                int32_t nMatch, int32_t nAttempt, int32_t byteOrder)
+{
+    return io_guard([&]() {
+        if (!matchFile || !m || !XS || !YS || !sigmaX || !sigmaY || !type)
+            fail(ST_ERR_ARG, "null argument");
+        check_grid(nx, ny, byteOrder);
+        const std::string b(matchFile);
+        const size_t n = (size_t)nx * ny;
+        // final count and RMS sigmas (cullls.c:84-101)
+        int nGoodPts = 0;
+        double sx = 0, sy = 0;
+        const double lim = (double)m->noData + 1.0;
+        for (size_t k = 0; k < n; k++)
+            if ((double)XS[k] > lim && (double)YS[k] > lim) {
+                sx += sigmaX[k] * sigmaX[k];
+                sy += sigmaY[k] * sigmaY[k];
+                nGoodPts++;
+            }
+        sx = sqrt(sx / (double)std::max(nGoodPts, 1));
+        sy = sqrt(sy / (double)std::max(nGoodPts, 1));
+        write_f32(b + ".cull.dx", XS, n, byteOrder);
+        write_f32(b + ".cull.dy", YS, n, byteOrder);
+        write_f32(b + ".cull.sx", sigmaX, n, byteOrder);
+        write_f32(b + ".cull.sy", sigmaY, n, byteOrder);
+        write_u8(b + ".cull.mtype", type, n);
+        // the header of cullls.c:150-174 (its stepY line prints stepX and
+        // its stepX line stepY)
+        File f(b + ".cull.dat", "w");
+        std::string t;
+        t += "fileEarly = " + str_or_null(m->fileEarly) + "\n";
+        t += "fileLate = " + str_or_null(m->fileLate) + "\n";
+        t += fmt("x0 = %11.2lf\n", m->x0);
+        t += fmt("y0 = %11.2lf\n", m->y0);
+        t += fmt("dx = %10.5lf\n", m->dx);
+        t += fmt("dy = %10.5lf\n", m->dy);
+        t += fmt("stepY = %u\n", m->stepX);
+        t += fmt("stepX = %u\n", m->stepY);
+        t += fmt("nx = %9i\n", nx);
+        t += fmt("ny = %9i\n", ny);
+        t += fmt("slowFlag = %9i\n", m->slowFlag);
+        t += fmt("EPSG = %9i\n", m->proj);
+        t += fmt("earlyImageJD = %10lf\n", m->jdEarly);
+        t += fmt("lateImageJD = %10lf\n", m->jdLate);
+        t += fmt("IntervalBetweenImages = %5i\n", (int)(m->jdLate - m->jdEarly));
+        t += fmt("Success_rate_for_attempted_matches(%%) =  %7.2f \n",
+                 ((float)nMatch / (float)std::max(nAttempt, 1)) * 100.);
+        t += fmt("Culled_rate_for_attempted_matches(%%) =  %7.2f \n",
+                 ((float)nGoodPts / (float)std::max(nAttempt, 1)) * 100.);
+        t += fmt("Mean_sigmaX = %11.3lf\n", sx);
+        t += fmt("Mean_sigmaY = %11.3lf\n", sy);
+        t += "& \n";
+        if (fwrite(t.data(), 1, t.size(), f.fp) != t.size())
+            fail(ST_ERR_IO, "short write to " + b + ".cull.dat");
+        f.close();
     });
 }

@@ -26,6 +26,13 @@ class CullMeta(C.Structure):
                 ("geo1", C.c_char_p), ("geo2", C.c_char_p)]
 
 
+class CullLSMeta(C.Structure):
+    _fields_ = [("fileEarly", C.c_char_p), ("fileLate", C.c_char_p), ("x0", C.c_double), ("y0", C.c_double),
+                ("dx", C.c_double), ("dy", C.c_double), ("stepX", C.c_uint32), ("stepY", C.c_uint32),
+                ("slowFlag", C.c_int32), ("proj", C.c_int32), ("jdEarly", C.c_double), ("jdLate", C.c_double),
+                ("noData", C.c_float)]
+
+
 _bound = False
 
 
@@ -41,6 +48,8 @@ def _lib():
         lib.st_read_offsets.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp]
         lib.st_write_cull.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(CullMeta), vp, vp, vp, vp, vp, vp,
                                       C.c_int32, C.c_int32, C.c_int32]
+        lib.st_write_cull_ls.argtypes = [C.c_char_p, C.POINTER(CullLSMeta), vp, vp, vp, vp, vp, C.c_int32,
+                                         C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         _bound = True
     return lib
 
@@ -116,6 +125,15 @@ def write_cull(out_base, in_base, meta, offRS, offAS, sigmaR, sigmaA, corr, type
     t = np.ascontiguousarray(type_, np.uint8)
     _check(_lib().st_write_cull(out_base.encode(), in_base.encode(), C.byref(meta), *[a.ctypes.data for a in arrs],
                                 t.ctypes.data, nR, nA, byte_order))
+
+
+def write_cull_ls(match_file, meta, XS, YS, sigmaX, sigmaY, type_, n_match, n_attempt, byte_order=ST_MSB):
+    """match_file.cull.dx .dy .sx .sy .mtype .dat (Cullls/cullls.c:63-175)."""
+    ny, nx = np.shape(XS)
+    arrs = [_f32(a, (ny, nx)) for a in (XS, YS, sigmaX, sigmaY)]
+    t = np.ascontiguousarray(type_, np.uint8)
+    _check(_lib().st_write_cull_ls(match_file.encode(), C.byref(meta), *[a.ctypes.data for a in arrs], t.ctypes.data,
+                                   nx, ny, n_match, n_attempt, byte_order))
 
 
 # ---- inputs before tracking -------------------------------------------------

@@ -49,6 +49,9 @@ def load_library(path=LIB_PATH):
     for name in ("st_cull_st_data", "st_cull_stats", "st_cull_smooth",
                  "st_cull_islands", "st_cull_run"):
         getattr(lib, name).argtypes = [vp, C.POINTER(abi.CullParams)]
+    for name in ("st_cull_ls_data", "st_cull_ls_stats", "st_cull_ls_smooth",
+                 "st_cull_ls_islands", "st_cull_ls_run"):
+        getattr(lib, name).argtypes = [vp, C.POINTER(abi.CullLSParams)]
     lib.st_last_stats.argtypes = [vp, C.POINTER(abi.TrackStats)]
     lib.st_server_create.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.POINTER(vp)]
     lib.st_server_destroy.argtypes = [vp]
@@ -397,6 +400,36 @@ def cull_params(offR, offA, corr, typ, mask=None, **kw):
     cp.islandThresh = -1
     cp.singleMT = -1
     cp.corrThresh = 0.0
+    for k, v in kw.items():
+        setattr(cp, k, v)
+    return cp, arrs
+
+
+def cull_ls_params(X, Y, typ, *, slow_vel_mask, no_data_to_use_mask, no_data, nover, deltaT, dx, dy, **kw):
+    """abi.CullLSParams over copies of Landsat offset grids (the cullls
+    defaults of Cullls/cullls.c:146-153 for the rest, overridden by kw);
+    returns (params, arrays) keeping the buffers alive.  The type bits,
+    no-data value and oversampling factor are those of the caller's Landsat
+    matcher (they live in headers outside the reference tree)."""
+    ny, nx = X.shape
+    if Y.shape != X.shape or typ.shape != X.shape:
+        raise ValueError("X, Y and type must have the same shape")
+    arrs = dict(X=X.astype(np.float32).copy(), Y=Y.astype(np.float32).copy(), type=typ.astype(np.uint8).copy(),
+                XS=np.zeros((ny, nx), np.float32), YS=np.zeros((ny, nx), np.float32),
+                sigmaX=np.zeros((ny, nx), np.float32), sigmaY=np.zeros((ny, nx), np.float32))
+    cp = abi.CullLSParams()
+    cp.nx, cp.ny = nx, ny
+    for k, v in arrs.items():
+        setattr(cp, k, abi.ptr(v, C.c_uint8 if v.dtype == np.uint8 else C.c_float))
+    cp.bR = cp.bA = 9
+    cp.nGood = 17
+    cp.maxX, cp.maxY = 0.75, 1.0
+    cp.sX, cp.sY = 3, 3
+    cp.islandThresh = -1
+    cp.deltaT, cp.dx, cp.dy = deltaT, dx, dy
+    cp.slowVelMask, cp.noDataToUseMask = slow_vel_mask, no_data_to_use_mask
+    cp.noData, cp.nover = no_data, nover
+    cp.nMatch = -1
     for k, v in kw.items():
         setattr(cp, k, v)
     return cp, arrs

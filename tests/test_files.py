@@ -234,3 +234,38 @@ def test_interferogram(tmp_path):
     assert (p.intNr, p.intNa, p.intNrl, p.intNal) == (nr, na, 2, 8) and keep is got
     with pytest.raises(files.IoError):
         files.read_interferogram(str(tmp_path / "missing.int"), nr, na)
+
+
+def test_cull_ls_products(tmp_path):
+    """writeLSCulledOffsets (Cullls/cullls.c:63-175): four big-endian float32
+    rasters, the type bytes, and the .cull.dat header with the reference's
+    formats (its stepY line carries stepX and the other way round)."""
+    rng = np.random.default_rng(8)
+    ny, nx = 7, 9
+    XS = rng.normal(0, 2, (ny, nx)).astype(np.float32)
+    YS = rng.normal(0, 2, (ny, nx)).astype(np.float32)
+    sx = rng.uniform(0.1, 0.5, (ny, nx)).astype(np.float32)
+    sy = rng.uniform(0.1, 0.5, (ny, nx)).astype(np.float32)
+    XS[2, 3] = YS[2, 3] = np.float32(-2e9)
+    t = rng.integers(0, 255, (ny, nx)).astype(np.uint8)
+    m = files.CullLSMeta(fileEarly=b"LC08_early.TIF", fileLate=b"LC08_late.TIF", x0=-3001234.5, y0=123456.25,
+                         dx=15.0, dy=15.0, stepX=24, stepY=12, slowFlag=1, proj=3413, jdEarly=2459000.5,
+                         jdLate=2459016.5, noData=-2e9)
+    base = str(tmp_path / "pair.match")
+    files.write_cull_ls(base, m, XS, YS, sx, sy, t, 60, 63)
+    for ext, a in ((".cull.dx", XS), (".cull.dy", YS), (".cull.sx", sx), (".cull.sy", sy)):
+        assert (tmp_path / ("pair.match" + ext)).read_bytes() == a.astype(">f4").tobytes()
+    assert (tmp_path / "pair.match.cull.mtype").read_bytes() == t.tobytes()
+    good = np.ones((ny, nx), bool)
+    good[2, 3] = False
+    rx = np.sqrt(np.sum((sx[good] * sx[good]).astype(np.float64)) / good.sum())
+    ry = np.sqrt(np.sum((sy[good] * sy[good]).astype(np.float64)) / good.sum())
+    want = ("fileEarly = LC08_early.TIF\nfileLate = LC08_late.TIF\nx0 = %11.2f\ny0 = %11.2f\ndx = %10.5f\n"
+            "dy = %10.5f\nstepY = 24\nstepX = 12\nnx = %9d\nny = %9d\nslowFlag = %9d\nEPSG = %9d\n"
+            "earlyImageJD = %10f\nlateImageJD = %10f\nIntervalBetweenImages = %5d\n"
+            "Success_rate_for_attempted_matches(%%) =  %7.2f \nCulled_rate_for_attempted_matches(%%) =  %7.2f \n"
+            "Mean_sigmaX = %11.3f\nMean_sigmaY = %11.3f\n& \n") % (
+        -3001234.5, 123456.25, 15.0, 15.0, nx, ny, 1, 3413, 2459000.5, 2459016.5, 16,
+        float(np.float32(np.float32(60) / np.float32(63))) * 100.0,
+        float(np.float32(np.float32(good.sum()) / np.float32(63))) * 100.0, rx, ry)
+    assert (tmp_path / "pair.match.cull.dat").read_text() == want
