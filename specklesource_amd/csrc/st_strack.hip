@@ -1097,7 +1097,11 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
     const float thr = isf ? 1e-15f : 1e-9f;
     const float thrAll = thr * W * W;
     const int lane = threadIdx.x;
-    const int it = blockIdx.x * PPW + lane / W, c = lane % W;
+    // consecutive items (overlapping patches) on one XCD, so their shared
+    // rows hit that XCD's L2: workgroups are dealt round-robin over the 8 XCDs
+    const int nb = gridDim.x, per = nb / 8, b = blockIdx.x;
+    const int blk = b < per * 8 ? (b % 8) * per + b / 8 : b;
+    const int it = blk * PPW + lane / W, c = lane % W;
     const bool act = it < count;
     double ps[2] = {0, 0}, T[2][2] = {{0, 0}, {0, 0}}, Lg[2][2] = {{0, 0}, {0, 0}};
     double F0[2][2] = {{0, 0}, {0, 0}}, FL[2][2] = {{0, 0}, {0, 0}};
@@ -1409,7 +1413,15 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
     __shared__ Top2 sht[16];
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
 
-    for (int it = blockIdx.x; it < count; it += gridDim.x) {
+    // Items in XCD-contiguous order: at every round the workgroups of one
+    // XCD (dealt round-robin, b % 8) take a contiguous run of items, so the
+    // overlapping patches of neighbouring points share that XCD's L2.
+    const int G = gridDim.x, Q = G / 8;
+    const int base = (G % 8 == 0) ? (blockIdx.x % 8) * Q + blockIdx.x / 8 : blockIdx.x;
+    for (int k0 = 0; k0 < count; k0 += G) {
+        const int it = k0 + base;
+        if (it >= count)
+            break;
         // Lane indices are re-derived behind an opaque copy every item so
         // the compiler does not hoist per-lane addresses out of the work
         // loop (they would stay live in registers for the whole kernel).
