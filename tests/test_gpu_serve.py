@@ -115,10 +115,10 @@ def test_poll_reports_states(device):
         for j in ids:
             srv.wait(j)
         assert all(srv.poll(j).state == abi.ST_JOB_DONE for j in ids)
-        bad = track.default_params(rStart=0, aStart=0, deltaR=8, deltaA=8, nR=4, nA=4, wR=24, wA=24, wRa=24,
-                                   wAa=24, gaussFlag=1)
+        bad = track.default_params(rStart=0, aStart=0, deltaR=8, deltaA=8, nR=4, nA=4, wR=24, wA=32, wRa=24,
+                                   wAa=32)
         jid = srv.submit(bad, images=(a, b), kind=kind)
-        with pytest.raises(RuntimeError, match="gauss"):
+        with pytest.raises(RuntimeError, match="non-square"):
             srv.wait(jid)
         assert srv.poll(jid).state == abi.ST_JOB_FAILED
     finally:
