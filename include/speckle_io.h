@@ -129,6 +129,31 @@ int st_read_mask(const char *maskFile, int32_t nr, int32_t na, uint8_t *mask);
  * the interferogram's geodat file, which the caller reads. */
 int st_read_interferogram(const char *intFile, int32_t nr, int32_t na, float *intf);
 
+/* ---- GDAL VRT inputs (raw bands) -------------------------------------- */
+/* The VRT files GrIMP reads through GDAL (GDALOpen / GDALRasterIO /
+ * readDataSetMetaData) where the bands are VRTRawRasterBand, as its own
+ * writers produce them: readBothOffsetsStrackVrt
+ * (Strack/readBothOffsetsStrack.c:46-76), getMaskVRT (Strack/getMask.c:22-48),
+ * loadFromVRT (Cullst/loadCullData.c:46-92).  Other band kinds return
+ * ST_ERR_UNSUPPORTED.  Errors: st_vrt_last_error(). */
+typedef struct st_vrt st_vrt;
+
+const char *st_vrt_last_error(void);
+int st_vrt_open(const char *vrtFile, st_vrt **vrt);
+void st_vrt_close(st_vrt *vrt);
+/* raster size (GDALGetRasterXSize / YSize) and band count */
+int st_vrt_size(const st_vrt *vrt, int32_t *nx, int32_t *ny, int32_t *nBands);
+/* a default-domain dataset metadata item (readDataSetMetaData + get_value) */
+int st_vrt_metadata(const st_vrt *vrt, const char *key, char *value, size_t len);
+/* band (1-based) description, or its "Description" metadata item */
+int st_vrt_band_description(const st_vrt *vrt, int32_t band, char *desc, size_t len);
+/* band converted to float32 (GDALRasterIO with GDT_Float32), ny x nx */
+int st_vrt_read_f32(const st_vrt *vrt, int32_t band, float *out);
+/* Byte band (GDT_Byte) */
+int st_vrt_read_u8(const st_vrt *vrt, int32_t band, uint8_t *out);
+/* CInt16 / CFloat32 band as (re, im) float pairs (SLC samples) */
+int st_vrt_read_c64(const st_vrt *vrt, int32_t band, float *out);
+
 #ifdef __cplusplus
 }
 #endif

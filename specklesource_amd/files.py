@@ -230,3 +230,126 @@ def apply_interferogram(par, intf, nrl, nal):
     par.intNa, par.intNr = intf.shape[0], intf.shape[1] // 2
     par.intNrl, par.intNal = nrl, nal
     return intf
+
+
+# ---- GDAL VRT inputs (raw bands) ---------------------------------------------
+
+def _bind_vrt(lib):
+    if getattr(lib, "_st_vrt_bound", False):
+        return
+    vp = C.c_void_p
+    lib.st_vrt_last_error.restype = C.c_char_p
+    lib.st_vrt_open.argtypes = [C.c_char_p, C.POINTER(vp)]
+    lib.st_vrt_close.argtypes = [vp]
+    lib.st_vrt_close.restype = None
+    i32p = C.POINTER(C.c_int32)
+    lib.st_vrt_size.argtypes = [vp, i32p, i32p, i32p]
+    lib.st_vrt_metadata.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_size_t]
+    lib.st_vrt_band_description.argtypes = [vp, C.c_int32, C.c_char_p, C.c_size_t]
+    for nm in ("st_vrt_read_f32", "st_vrt_read_u8", "st_vrt_read_c64"):
+        getattr(lib, nm).argtypes = [vp, C.c_int32, vp]
+    lib._st_vrt_bound = True
+
+
+class Vrt:
+    """A VRT with raw bands: .nx, .ny, .n_bands, .metadata(key),
+    .description(band), .read(band, 'f32' | 'u8' | 'c64')."""
+
+    def __init__(self, path):
+        self._lib = _lib()
+        _bind_vrt(self._lib)
+        self._h = C.c_void_p()
+        self._ck(self._lib.st_vrt_open(path.encode(), C.byref(self._h)))
+        nx, ny, nb = C.c_int32(), C.c_int32(), C.c_int32()
+        self._ck(self._lib.st_vrt_size(self._h, C.byref(nx), C.byref(ny), C.byref(nb)))
+        self.nx, self.ny, self.n_bands = nx.value, ny.value, nb.value
+
+    def _ck(self, st):
+        if st != 0:
+            raise IoError(st, self._lib.st_vrt_last_error().decode())
+
+    def metadata(self, key):
+        buf = C.create_string_buffer(4096)
+        self._ck(self._lib.st_vrt_metadata(self._h, key.encode(), buf, len(buf)))
+        return buf.value.decode()
+
+    def description(self, band):
+        buf = C.create_string_buffer(4096)
+        self._ck(self._lib.st_vrt_band_description(self._h, band, buf, len(buf)))
+        return buf.value.decode()
+
+    def read(self, band, kind="f32"):
+        if kind == "f32":
+            out = np.empty((self.ny, self.nx), np.float32)
+        elif kind == "u8":
+            out = np.empty((self.ny, self.nx), np.uint8)
+        elif kind == "c64":
+            out = np.empty((self.ny, self.nx), np.complex64)
+        else:
+            raise ValueError(kind)
+        self._ck(getattr(self._lib, "st_vrt_read_" + kind)(self._h, band, out.ctypes.data))
+        return out
+
+    def close(self):
+        if self._h:
+            self._lib.st_vrt_close(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _atoi(s):
+    """C atoi: the leading integer of s, 0 when there is none."""
+    import re
+    m = re.match(r"\s*([+-]?\d+)", s)
+    return int(m.group(1)) if m else 0
+
+
+def _atof(s):
+    """C atof: the leading floating-point number of s, 0 when there is none."""
+    import re
+    m = re.match(r"\s*([+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?)", s)
+    return float(m.group(1)) if m else 0.0
+
+
+def read_initial_offsets_vrt(vrt_path):
+    """readBothOffsetsStrackVrt (Strack/readBothOffsetsStrack.c:29-76): the
+    (r0, a0, deltaR, deltaA, da, dr) tuple of read_initial_offsets from an
+    offsets VRT (band 1 range, band 2 azimuth offsets, checked by their
+    descriptions)."""
+    with Vrt(vrt_path) as v:
+        r0, a0 = _atoi(v.metadata("r0")), _atoi(v.metadata("a0"))
+        dR, dA = _atof(v.metadata("deltaR")), _atof(v.metadata("deltaA"))
+        if "Range" not in v.description(1):
+            raise IoError(-6, "Problem reading range offsets from %s" % vrt_path)
+        dr = v.read(1)
+        if "Azimuth" not in v.description(2):
+            raise IoError(-6, "Problem reading azimuth offsets from %s" % vrt_path)
+        da = v.read(2)
+    return r0, a0, dR, dA, da, dr
+
+
+def read_mask_vrt(vrt_path):
+    """getMaskVRT (Strack/getMask.c:22-48): ((r0, a0, nr, na, nrl, nal), mask)
+    with nrl / nal the integer deltaR / deltaA of the metadata."""
+    with Vrt(vrt_path) as v:
+        hdr = (_atoi(v.metadata("r0")), _atoi(v.metadata("a0")), v.nx, v.ny, _atoi(v.metadata("deltaR")),
+               _atoi(v.metadata("deltaA")))
+        return hdr, v.read(1, "u8")
+
+
+def read_cull_vrt(vrt_path, mt_vrt_path):
+    """loadFromVRT (Cullst/loadCullData.c:46-92): metadata r0 a0 deltaR deltaA
+    and the offR, offA, corr bands of the offsets VRT, the match types of
+    the .mt VRT."""
+    with Vrt(vrt_path) as v:
+        meta = dict(r0=_atoi(v.metadata("r0")), a0=_atoi(v.metadata("a0")), deltaR=_atof(v.metadata("deltaR")),
+                    deltaA=_atof(v.metadata("deltaA")))
+        offR, offA, corr = v.read(1), v.read(2), v.read(3)
+    with Vrt(mt_vrt_path) as m:
+        typ = m.read(1, "u8")
+    return meta, offR, offA, corr, typ
