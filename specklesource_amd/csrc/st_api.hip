@@ -2,7 +2,54 @@
 // (include/speckle_amd.h).
 #include <string.h>
 
+#include <algorithm>
+#include <thread>
+
 #include "st_ctx.h"
+
+void *st_ctx::pinned_reserve(size_t bytes)
+{
+    if (pinnedCap < bytes) {
+        if (pinned)
+            ST_HIP(hipHostFree(pinned));
+        pinned = nullptr;
+        pinnedCap = 0;
+        const size_t cap = bytes + (bytes >> 3);
+        ST_HIP(hipHostMalloc(&pinned, cap, hipHostMallocDefault));
+        pinnedCap = cap;
+    }
+    return pinned;
+}
+
+void parallel_copy(const std::vector<std::tuple<void *, const void *, size_t>> &jobs)
+{
+    constexpr size_t kPiece = (size_t)1 << 20;
+    std::vector<std::tuple<char *, const char *, size_t>> pieces;
+    size_t total = 0;
+    for (const auto &j : jobs) {
+        char *d = (char *)std::get<0>(j);
+        const char *s = (const char *)std::get<1>(j);
+        const size_t n = std::get<2>(j);
+        for (size_t o = 0; o < n; o += kPiece)
+            pieces.emplace_back(d + o, s + o, std::min(kPiece, n - o));
+        total += n;
+    }
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>({(size_t)4, (size_t)hw, pieces.size()});
+    if (nt <= 1 || total < ((size_t)2 << 20)) {
+        for (const auto &p : pieces)
+            memcpy(std::get<0>(p), std::get<1>(p), std::get<2>(p));
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; t++)
+        th.emplace_back([&, t]() {
+            for (size_t k = t; k < pieces.size(); k += nt)
+                memcpy(std::get<0>(pieces[k]), std::get<1>(pieces[k]), std::get<2>(pieces[k]));
+        });
+    for (auto &x : th)
+        x.join();
+}
 
 void st_ctx::arena_reserve(size_t bytes)
 {
@@ -114,6 +161,8 @@ extern "C" int st_ctx_destroy(st_ctx *c)
     }
     if (c->arena.base)
         hipFree(c->arena.base);
+    if (c->pinned)
+        hipHostFree(c->pinned);
     if (c->evInit)
         for (int k = 0; k < 72; k++)
             hipEventDestroy(c->ev[k]);

@@ -5,6 +5,8 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
+#include <vector>
 
 #include "st_common.h"
 #include "st_fft.h"
@@ -42,6 +44,10 @@ struct st_ctx {
 
     // Make the arena at least `bytes` large and empty it.
     void arena_reserve(size_t bytes);
+    // Pinned host staging for result copies, at least `bytes` large.
+    void *pinned = nullptr;
+    size_t pinnedCap = 0;
+    void *pinned_reserve(size_t bytes);
     StPair &pair(int id);
 };
 
@@ -65,6 +71,10 @@ int st_guard(st_ctx *ctx, F fn)
         return ST_ERR_NOMEM;
     }
 }
+
+// Host copies (dst, src, bytes) split over a few threads: the step from a
+// pinned staging buffer into the caller's arrays.
+void parallel_copy(const std::vector<std::tuple<void *, const void *, size_t>> &jobs);
 
 // Entry points implemented in the per-tool translation units.
 void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, st_track_out *out,

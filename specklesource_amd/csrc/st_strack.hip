@@ -2455,12 +2455,22 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     ST_HIP(hipGetLastError());
     int counts[CNT_N];
     ST_HIP(hipMemcpyAsync(counts, S.counts, sizeof(counts), hipMemcpyDeviceToHost, st));
-    ST_HIP(hipMemcpyAsync(out->offR, dOffR, npts * sizeof(float), hipMemcpyDeviceToHost, st));
-    ST_HIP(hipMemcpyAsync(out->offA, dOffA, npts * sizeof(float), hipMemcpyDeviceToHost, st));
-    ST_HIP(hipMemcpyAsync(out->corr, S.corr, npts * sizeof(float), hipMemcpyDeviceToHost, st));
-    ST_HIP(hipMemcpyAsync(out->type, S.type, npts, hipMemcpyDeviceToHost, st));
-    if (out->azd)
-        ST_HIP(hipMemcpyAsync(out->azd, S.azd, npts * sizeof(float), hipMemcpyDeviceToHost, st));
+    // results go through a pinned staging buffer (a device-to-pageable copy
+    // takes the driver's slower staged path) and are spread into the
+    // caller's arrays after the stream has drained
+    const size_t f4 = (size_t)npts * sizeof(float);
+    char *stg = (char *)ctx->pinned_reserve(4 * f4 + (size_t)npts);
+    std::vector<std::tuple<void *, const void *, size_t>> spread = {
+        {out->offR, stg, f4}, {out->offA, stg + f4, f4}, {out->corr, stg + 2 * f4, f4},
+        {out->type, stg + 3 * f4, (size_t)npts}};
+    ST_HIP(hipMemcpyAsync(stg, dOffR, f4, hipMemcpyDeviceToHost, st));
+    ST_HIP(hipMemcpyAsync(stg + f4, dOffA, f4, hipMemcpyDeviceToHost, st));
+    ST_HIP(hipMemcpyAsync(stg + 2 * f4, S.corr, f4, hipMemcpyDeviceToHost, st));
+    ST_HIP(hipMemcpyAsync(stg + 3 * f4, S.type, npts, hipMemcpyDeviceToHost, st));
+    if (out->azd) {
+        ST_HIP(hipMemcpyAsync(stg + 3 * f4 + npts, S.azd, f4, hipMemcpyDeviceToHost, st));
+        spread.emplace_back(out->azd, stg + 3 * f4 + npts, f4);
+    }
     if (diag) {
         auto cp = [&](void *h, const void *d, size_t bytes) {
             if (h)
@@ -2491,6 +2501,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     }
     ST_HIP(hipEventRecord(sev[5], st));
     ST_HIP(hipStreamSynchronize(st));
+    parallel_copy(spread);
     ctx->flush_timings();
     {
         float ms;
