@@ -1142,9 +1142,11 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
                     F0[g][0] = x.x;
                     F0[g][1] = x.y;
                 } else {
-                    // x conj(prev): re = x.re p.re + x.im p.im, im = x.im p.re - x.re p.im
-                    Lg[g][0] += (double)x.x * prev[g].x + (double)x.y * prev[g].y;
-                    Lg[g][1] += (double)x.y * prev[g].x - (double)x.x * prev[g].y;
+                    // x conj(prev): re = x.re p.re + x.im p.im, im = x.im p.re - x.re p.im.
+                    // A product of two floats is exact in double, so each fma
+                    // rounds once, like the add it replaces.
+                    Lg[g][0] = fma((double)x.y, (double)prev[g].y, fma((double)x.x, (double)prev[g].x, Lg[g][0]));
+                    Lg[g][1] = fma(-(double)x.x, (double)prev[g].y, fma((double)x.y, (double)prev[g].x, Lg[g][1]));
                 }
                 if (a == W - 1) {
                     FL[g][0] = x.x;
