@@ -132,6 +132,71 @@ __device__ inline void int_correct(const KPar &P, int it, int i, int j, double &
     ci = ni;
 }
 
+// Points with at most kIntTab kept bins (nearly all: a fringe has one
+// spectral peak) use the separable form of each bin's term,
+// exp(2 pi i (kp m / Pa + lp n / Pr)) = A[m] B[n]: the workgroup tabulates A
+// over the window's rows and B over its columns once per point, and an
+// element costs a few FP64 multiplies instead of a sincos.  Other points
+// (and windows wider than TW) evaluate every term directly.
+constexpr int kIntTab = 2;
+
+template <int TW>
+struct IntTab {
+    double2 a[kIntTab][TW], b[kIntTab][TW];
+    double v[kIntTab];
+    int nb;  // kept bins, or -1: direct evaluation
+};
+
+// All threads of the workgroup call it (one barrier inside).
+template <int TW>
+__device__ inline void int_tables(const KPar &P, int it, int w, IntTab<TW> &T, int tid, int nth)
+{
+    const IntSet *I = P.intSet + it;
+    const int nb = I->n;
+    if (nb <= kIntTab && w <= TW) {
+        const int m0 = P.intPa / 2 - P.wA / 2, n0 = P.intPr / 2 - P.wR / 2;
+        for (int e = tid; e < nb * w; e += nth) {
+            const int q = e / w, k = e - (e / w) * w;
+            const IntBin b = I->b[q];
+            double sn, cs;
+            sincospi(2.0 * (double)((b.kp * (m0 + k)) % P.intPa) / P.intPa, &sn, &cs);
+            T.a[q][k] = make_double2(cs, sn);
+            sincospi(2.0 * (double)((b.lp * (n0 + k)) % P.intPr) / P.intPr, &sn, &cs);
+            T.b[q][k] = make_double2(cs, sn);
+        }
+        if (tid < nb)
+            T.v[tid] = I->b[tid].v;
+    }
+    if (tid == 0)
+        T.nb = (nb <= kIntTab && w <= TW) ? nb : -1;
+    __syncthreads();
+}
+
+template <int TW>
+__device__ inline void int_apply(const KPar &P, const IntTab<TW> &T, int it, int i, int j, double &cr, double &ci)
+{
+    const int nb = T.nb;
+    if (nb == 0)
+        return;
+    if (nb < 0) {
+        int_correct(P, it, i, j, cr, ci);
+        return;
+    }
+    double yr = 0, yi = 0;
+    for (int q = 0; q < nb; q++) {
+        const double2 a = T.a[q][i], b = T.b[q][j];
+        yr += T.v[q] * (a.x * b.x - a.y * b.y);
+        yi += T.v[q] * (a.x * b.y + a.y * b.x);
+    }
+    const double pr = yr, pim = -yi;
+    if (fabs(pr) < 1.0e-6 && fabs(pim) < 1.0e-6)
+        return;
+    const double scale = 1.0 / sqrt(pr * pr + pim * pim);
+    const double nr = (cr * pr - ci * pim) * scale, ni = (cr * pim + ci * pr) * scale;
+    cr = nr;
+    ci = ni;
+}
+
 // One workgroup per listed point.  Pass 0 writes the record (n, the first
 // kIntInline bins, an overflow slot); pass 1 runs only for points with more
 // bins and writes all of them to their slot.
@@ -342,6 +407,9 @@ __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *lis
         }
     };
     float2 *P1 = C.P + (size_t)b * N, *P2 = C.P + ((size_t)B + b) * N;
+    __shared__ IntTab<128> itab;
+    if (P.intOn)
+        int_tables<128>(P, off + b, w, itab, threadIdx.x, NT);
     int nZero = 0;
     double p1 = 0, p2 = 0;
     for (int e = threadIdx.x; e < N; e += NT) {
@@ -371,7 +439,7 @@ __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *lis
         if (P.intOn) {
             const int i = e / w, j = e - (e / w) * w;
             double cr = x.x, ci = x.y;
-            int_correct(P, off + b, i, j, cr, ci);
+            int_apply<128>(P, itab, off + b, i, j, cr, ci);
             x = make_float2((float)cr, (float)ci);
         }
         P1[e] = x;
@@ -439,33 +507,22 @@ __device__ void peak_finish(const float *M, int W, int mode, int I, int J, float
     r.J = J;
     r.raw = raw;
     if constexpr (GAUSS) {
+        // the fit itself runs in k_gauss_fit, one thread per attempt; the
+        // amplitude mode's 1-D fits sit at the coarse peak and are known now
         if (threadIdx.x == 0) {
-            const double p1 = G.p[2 * item] * G.pScale, p2 = G.p[2 * item + 1] * G.pScale;
-            const double scale = G.hc / (sqrt(p1 * p2) * pow((double)(G.wWin * G.wWin), 2));
-            gauss_fit_2d(M, W, I, J, (double)raw, G.wWin, G.wWin, scale, &r.gI, &r.gJ, &r.gC);
-            int cI, cJ;
-            bool ok;
-            if (mode == 0) {
-                cI = r.gI / NOVER;
-                cJ = r.gJ / NOVER;
-                ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < W && (cJ - NGAUSS / 2) >= 0 &&
-                     (cJ + NGAUSS / 2) < W;
-            } else {
-                cI = I;
-                cJ = J;
-                ok = I > NGAUSS / 2 && I < W - NGAUSS / 2 && J > NGAUSS / 2 && J < W - NGAUSS / 2;
-            }
-            r.lmOk = ok ? 1 : 0;
-            if (ok) {
+            if (mode == 1 && I > NGAUSS / 2 && I < W - NGAUSS / 2 && J > NGAUSS / 2 && J < W - NGAUSS / 2) {
+                r.lmOk = 1;
                 for (int t = 0; t < NGAUSS; t++) {
-                    r.lmY[0][t] = M[cI * W + (cJ - NGAUSS / 2 + t)];
-                    r.lmY[1][t] = M[(cI - NGAUSS / 2 + t) * W + cJ];
+                    r.lmY[0][t] = M[I * W + (J - NGAUSS / 2 + t)];
+                    r.lmY[1][t] = M[(I - NGAUSS / 2 + t) * W + J];
                 }
-                r.lmX0[0] = (float)(cJ - NGAUSS / 2);
-                r.lmX0[1] = (float)(cI - NGAUSS / 2);
+                r.lmX0[0] = (float)(J - NGAUSS / 2);
+                r.lmX0[1] = (float)(I - NGAUSS / 2);
             }
             *out = r;
         }
+        (void)G;
+        (void)item;
         return;
     }
     // FP64 refinement: Q = A P A^T on the searched 48 x 48 box
@@ -595,6 +652,44 @@ __global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W,
     sec = block_max_f<NT>(sec, shf);
     PeakRes *out = res + b;
     peak_finish<GAUSS>(M, W, mode, I, J, raw, sec, out, G, b);
+}
+
+// -gauss: the 2-D fit of each attempt on one thread (cmpTrackGauss,
+// speckleTrack.c:2205-2303), so a wave runs 64 fits at once in the
+// reference's sequential operation order.  In the complex mode the 1-D fit
+// samples follow at the fitted peak / NOVER (speckleTrack.c:377-386).
+__global__ void __launch_bounds__(128) k_gauss_fit(const float *mag, int W, int mode, PeakRes *res, int B,
+                                                   GaussArg G)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B)
+        return;
+    PeakRes &r = res[b];
+    if (r.I < -999)
+        return;  // rejected coarse peak: the fit's edge test fails (defaults stand)
+    const float *M = mag + (size_t)b * W * W;
+    const double p1 = G.p[2 * b] * G.pScale, p2 = G.p[2 * b + 1] * G.pScale;
+    const double scale = G.hc / (sqrt(p1 * p2) * pow((double)(G.wWin * G.wWin), 2));
+    int gI, gJ;
+    double gC;
+    gauss_fit_2d(M, W, r.I, r.J, (double)r.raw, G.wWin, G.wWin, scale, &gI, &gJ, &gC);
+    r.gI = gI;
+    r.gJ = gJ;
+    r.gC = gC;
+    if (mode != 0)
+        return;
+    const int cI = gI / NOVER, cJ = gJ / NOVER;
+    const bool ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < W && (cJ - NGAUSS / 2) >= 0 &&
+                    (cJ + NGAUSS / 2) < W;
+    r.lmOk = ok ? 1 : 0;
+    if (ok) {
+        for (int t = 0; t < NGAUSS; t++) {
+            r.lmY[0][t] = M[cI * W + (cJ - NGAUSS / 2 + t)];
+            r.lmY[1][t] = M[(cI - NGAUSS / 2 + t) * W + cJ];
+        }
+        r.lmX0[0] = (float)(cJ - NGAUSS / 2);
+        r.lmX0[1] = (float)(cI - NGAUSS / 2);
+    }
 }
 
 // Two fits per attempt: [0] along range, [1] along azimuth.
@@ -1496,12 +1591,14 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
             }
             }
             if constexpr ((BASE & 2) != 0) {
+                __shared__ IntTab<W> itab;
+                int_tables<W>(P, it, W, itab, t, NTH);
 #pragma unroll 1
                 for (int e = t; e < W * W; e += NTH) {
                     const int i = e / W, j = e - (e / W) * W;
                     const float2 v = R0[FL::at(i, j)];
                     double cr = v.x, ci = v.y;
-                    int_correct(P, it, i, j, cr, ci);
+                    int_apply<W>(P, itab, it, i, j, cr, ci);
                     R0[FL::at(i, j)] = make_float2((float)cr, (float)ci);
                 }
             }
@@ -2380,10 +2477,12 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
                                PfW, PiW2, dW2);
             hipLaunchKernelGGL(k_cpx_rowmag, dim3(nPart, nb), dim3(NT), ldsR, st, C.S, w, Rr, C.mag, part, PiW2, dW2);
             const GaussArg G{gauss ? 1 : 0, w, par.hanningFlag ? 1.5 : 1.0, 1.0, C.p};
-            if (gauss)
+            if (gauss) {
                 hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0, C.res,
                                    G);
-            else
+                hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, C.mag, 2 * w, 0, C.res, nb,
+                                   G);
+            } else
                 hipLaunchKernelGGL(k_peak_merge<false>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0,
                                    C.res, G);
             ctx->time_end(kPeak);
@@ -2440,9 +2539,10 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
             ctx->fft.fft2d(A.S, A.tmp, nb, W, W, +1, st);
             ctx->time_begin(kPeak);
             const GaussArg G{gauss ? 1 : 0, W / 2, 1.0, 16.0, A.p};
-            if (gauss)
+            if (gauss) {
                 hipLaunchKernelGGL(k_peak<true>, dim3(nb), dim3(NT), 0, st, A.S, A.mag, W, 1, A.res, G);
-            else
+                hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, A.mag, W, 1, A.res, nb, G);
+            } else
                 hipLaunchKernelGGL(k_peak<false>, dim3(nb), dim3(NT), 0, st, A.S, A.mag, W, 1, A.res, G);
             ctx->time_end(kPeak);
             hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, A.res, nb, A.sig);
