@@ -2217,6 +2217,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
                           !intOn && !gauss && !staged_only();
     const bool fusedPath = (par.wR == 16 || par.wR == 32 || par.wR == 64) && !gauss && !staged_only();
     const int kPeak = ctx->kernel_slot(mfmaPath ? "k_cpx_mfma" : (fusedPath ? "k_cpx_fused" : "peak_refine"));
+    const int kAmpPeak = ctx->kernel_slot("amp_peak");
     const int kGather = ctx->kernel_slot("cpx_gather"), kFft = ctx->kernel_slot("fft2d"),
               kAmpDet = ctx->kernel_slot("amp_detect");
     const int npts = par.nR * par.nA;
@@ -2537,14 +2538,14 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
             hipLaunchKernelGGL(k_amp_cross, dim3((W * W + 255) / 256, nb), dim3(256), 0, st, A.U, nb, W * W, A.state,
                                A.S);
             ctx->fft.fft2d(A.S, A.tmp, nb, W, W, +1, st);
-            ctx->time_begin(kPeak);
+            ctx->time_begin(kAmpPeak);
             const GaussArg G{gauss ? 1 : 0, W / 2, 1.0, 16.0, A.p};
             if (gauss) {
                 hipLaunchKernelGGL(k_peak<true>, dim3(nb), dim3(NT), 0, st, A.S, A.mag, W, 1, A.res, G);
                 hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, A.mag, W, 1, A.res, nb, G);
             } else
                 hipLaunchKernelGGL(k_peak<false>, dim3(nb), dim3(NT), 0, st, A.S, A.mag, W, 1, A.res, G);
-            ctx->time_end(kPeak);
+            ctx->time_end(kAmpPeak);
             hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, A.res, nb, A.sig);
             hipLaunchKernelGGL(k_amp_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, large, A);
             ST_HIP(hipGetLastError());
