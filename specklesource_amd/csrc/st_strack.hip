@@ -604,56 +604,6 @@ __device__ void peak_finish(const float *M, int W, int mode, int I, int J, float
 }
 
 
-// GAUSS selects the -gauss fit (a separate instantiation: the fit's
-// registers would lower the occupancy of the default refinement).
-template <bool GAUSS>
-__global__ void __launch_bounds__(NT) k_peak(const float2 *S, float *mag, int W, int mode, PeakRes *res, GaussArg G)
-{
-    __shared__ ArgMax sha[NT / 64];
-    __shared__ float shf[NT / 64];
-    const int b = blockIdx.x;
-    const float2 *C = S + (size_t)b * W * W;
-    float *M = mag + (size_t)b * W * W;
-    const int H = W / 2;
-    ArgMax best{0.0f, 0x7fffffff};
-    for (int e = threadIdx.x; e < W * W; e += NT) {
-#pragma clang fp contract(off)
-        const int n = e / W, m = e - n * W;
-        const float2 z = C[e];
-        const float x2 = z.x * z.x + z.y * z.y;
-        const float v = sqrtf(x2);
-        const int I = n < H ? n + H : n - H, J = m < H ? m + H : m - H;
-        M[I * W + J] = v;
-        const int i = I < H ? I : I - H, j = J < H ? J : J - H;
-        const int q = (I < H) ? (J < H ? 0 : 3) : (J < H ? 2 : 1);
-        const int rank = (i * H + j) * 4 + q;
-        if (v > 0.0f) {
-            ArgMax c{v, rank};
-            best = argmax_combine(best, c);
-        }
-    }
-    best = block_argmax<NT>(best, sha);
-    __syncthreads();
-    int I = -1000, J = -1000;
-    float raw = 0.f;
-    if (best.rank != 0x7fffffff) {
-        const int q = best.rank & 3, ij = best.rank >> 2;
-        const int i = ij / H, j = ij - (ij / H) * H;
-        I = (q == 0 || q == 3) ? i : i + H;
-        J = (q == 0 || q == 2) ? j : j + H;
-        raw = best.v;
-    }
-    // runner-up on the coarse surface
-    float sec = 0.f;
-    const int bestIdx = (I >= 0) ? I * W + J : -1;
-    for (int e = threadIdx.x; e < W * W; e += NT)
-        if (e != bestIdx)
-            sec = fmaxf(sec, M[e]);
-    sec = block_max_f<NT>(sec, shf);
-    PeakRes *out = res + b;
-    peak_finish<GAUSS>(M, W, mode, I, J, raw, sec, out, G, b);
-}
-
 // -gauss: the 2-D fit of each attempt on one thread (cmpTrackGauss,
 // speckleTrack.c:2205-2303), so a wave runs 64 fits at once in the
 // reference's sequential operation order.  In the complex mode the 1-D fit
@@ -863,17 +813,21 @@ __global__ void k_amp_list(KPar P, Pts S, int npts, int *list)
 // ---------------------------------------------------------------------
 // amplitude stages (getAmpPatches speckleTrack.c:1006-1189, ampMatch
 // :398-484, ampTrackFast :489-597)
+struct AmpPart;
+struct Top2;
 struct AmpBatch {
-    float2 *I;     // [2][B][ws^2]
-    float2 *U;     // [2][B][4 ws^2]
-    float2 *S;     // [B][4 ws^2]
-    float2 *tmp;
-    float *mag;
+    float2 *I;       // [2][B][ws^2] patches, then their range spectra
+    float2 *F;       // [2][B][W^2]: interpolation columns [2][B][W][ws], then forward range spectra
+    float2 *S;       // [B][W^2]: amplitudes [2][B][W^2] (float), then the correlation's azimuth half
+    float *mag;      // [B][W^2]
     PeakRes *res;
     float *sig;
     uint8_t *state;  // 0 = bounds failed, 1 = no data, 2 = matched
     int *az;
     double *p;
+    AmpPart *dpart;  // detection partials [B][nPd]
+    double2 *ppart;  // power partials [B][nPd]
+    Top2 *part;      // coarse top two [B][nPm]
 };
 
 __global__ void __launch_bounds__(NT) k_amp_gather(KPar P, Pts S, const int *list, int off, int B, int large,
@@ -918,99 +872,6 @@ __global__ void __launch_bounds__(NT) k_amp_gather(KPar P, Pts S, const int *lis
         A.state[b] = 2;
         A.az[b] = (int)((float)azUse * (float)ws / (float)P.wA);
     }
-}
-
-// zeroPadFFT (speckleTrack.c:2014-2038) of both spectra into 2ws x 2ws.
-__global__ void k_amp_pad(const float2 *I, int B, int ws, const int *az, const uint8_t *state, float2 *U)
-{
-    const int W = 2 * ws, h = ws / 2;
-    const int b = blockIdx.y % B, which = blockIdx.y / B;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= W * W || state[b] != 2)
-        return;
-    const int n = e / W, m = e - n * W;
-    float2 v = make_float2(0.f, 0.f);
-    const int k = n < h ? n : (n >= W - h ? n - W : 1 << 20);
-    const int l = m < h ? m : (m >= W - h ? m - W : 1 << 20);
-    if (k != (1 << 20) && l != (1 << 20)) {
-        const int ka = (k + ws + az[b]) % ws, la = (l + ws) % ws;
-        v = I[((size_t)which * B + b) * ws * ws + ka * ws + la];
-    }
-    U[((size_t)which * B + b) * W * W + e] = v;
-}
-
-// Detection, zero test, amplitude, mean removal and power
-// (speckleTrack.c:1166-1188, 437-467).
-__global__ void __launch_bounds__(NT) k_amp_detect(KPar P, int B, int ws, int large, AmpBatch A)
-{
-    __shared__ double sh[NT / 64];
-    __shared__ int shi[NT / 64];
-    const int b = blockIdx.x;
-    if (A.state[b] != 2)
-        return;
-    const int W = 2 * ws, N = W * W;
-    float2 *U1 = A.U + (size_t)b * N, *U2 = A.U + ((size_t)B + b) * N;
-    const size_t sws = (size_t)ws;
-    const float scale = (float)(1. / (double)(sws * sws * sws * sws * 8));
-    const float thr = P.im.stype == ST_SAMPLE_C64 ? 1.0e-15f : 1.0e-6f;
-    int nZero = 0;
-    double s1 = 0, s2 = 0;
-    for (int e = threadIdx.x; e < N; e += NT) {
-#pragma clang fp contract(off)
-        const float2 u = U1[e], v = U2[e];
-        const float x1 = (u.x * u.x + u.y * u.y) * scale, x2 = (v.x * v.x + v.y * v.y) * scale;
-        nZero += (x1 < thr || x2 < thr);
-        s1 += x1;
-        s2 += x2;
-        U1[e] = make_float2(x1, 0.f);
-        U2[e] = make_float2(x2, 0.f);
-    }
-    nZero = block_sum_i<NT>(nZero, shi);
-    s1 = block_sum_d<NT>(s1, sh);
-    s2 = block_sum_d<NT>(s2, sh);
-    if (nZero > 20 || s1 < 1.0e-6 || s2 < 1.0e-6) {
-        if (threadIdx.x == 0)
-            A.state[b] = 1;
-        return;
-    }
-    double m1 = 0, m2 = 0;
-    for (int e = threadIdx.x; e < N; e += NT) {
-        const float v1 = (float)sqrt((double)U1[e].x), v2 = (float)sqrt((double)U2[e].x);
-        U1[e].x = v1;
-        U2[e].x = v2;
-        m1 += v1;
-        m2 += v2;
-    }
-    m1 = block_sum_d<NT>(m1, sh) / (double)N;
-    m2 = block_sum_d<NT>(m2, sh) / (double)N;
-    double p1 = 0, p2 = 0;
-    for (int e = threadIdx.x; e < N; e += NT) {
-#pragma clang fp contract(off)
-        float v1 = U1[e].x, v2 = U2[e].x;
-        if (v1 > 1.0e-20)
-            v1 = (float)(v1 - m1);
-        v2 = (float)(v2 - m2);
-        U1[e].x = v1;
-        U2[e].x = v2;
-        p1 += v1 * v1;
-        p2 += v2 * v2;
-    }
-    p1 = block_sum_d<NT>(p1, sh);
-    p2 = block_sum_d<NT>(p2, sh);
-    if (threadIdx.x == 0) {
-        A.p[2 * b] = p1 / (double)N;
-        A.p[2 * b + 1] = p2 / (double)N;
-    }
-}
-
-__global__ void k_amp_cross(const float2 *U, int B, int N, const uint8_t *state, float2 *S)
-{
-    const int b = blockIdx.y;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= N || state[b] != 2)
-        return;
-    const float2 a = U[(size_t)b * N + e], c = U[((size_t)B + b) * N + e];
-    S[(size_t)b * N + e] = cmulconj(a, c);
 }
 
 // ampTrackFast tail and the driver's acceptance test
@@ -1389,15 +1250,24 @@ __global__ void __launch_bounds__(NT) k_cpx_colx(const float2 *P, int B, int w, 
 // of T spread over 2w (Nyquist on the negative side, zeros between),
 // inverse transform, |C| into the fftshifted surface M, and this block's
 // top two in findCPeak's scan-rank order (speckleTrack.c:744-805).
+// PAD = false: T already holds all W = 2w columns (the amplitude
+// correlation, which is not zero padded), and patches whose state is not
+// "matched" report no peak.
+template <bool PAD>
 __global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R, float *mag, Top2 *part,
-                                                    stfft::PlanArgs Pi, stfft::FastDiv dW)
+                                                    stfft::PlanArgs Pi, stfft::FastDiv dW, const uint8_t *state)
 {
     extern __shared__ float2 ldsx[];
     __shared__ Top2 sht[NT / 64];
     constexpr int IT = kStElems / NT;
-    const int W = 2 * w, h = w / 2, H = w;
+    const int W = 2 * w, h = w / 2, H = w, tc = PAD ? w : W;
     const int b = blockIdx.y, n0 = blockIdx.x * R, nrow = imin(R, W - n0);
-    const float2 *Tb = T + ((size_t)b * W + n0) * w;
+    if (!PAD && state[b] != 2) {
+        if (threadIdx.x == 0)
+            part[(size_t)b * gridDim.x + blockIdx.x] = Top2{0.0f, 0x7fffffff, 0.0f};
+        return;
+    }
+    const float2 *Tb = T + ((size_t)b * W + n0) * tc;
     const int cnt = nrow * W;
     {
         float2 v[IT];
@@ -1407,9 +1277,9 @@ __global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R
             v[it] = make_float2(0.f, 0.f);
             if (t < cnt) {
                 const int i = stfft::fdiv(t, dW), m = t - i * W;
-                const int la = m < h ? m : (m >= W - h ? m - w : -1);
+                const int la = !PAD ? m : (m < h ? m : (m >= W - h ? m - w : -1));
                 if (la >= 0)
-                    v[it] = Tb[(size_t)i * w + la];
+                    v[it] = Tb[(size_t)i * tc + la];
             }
         }
 #pragma unroll
@@ -1445,7 +1315,7 @@ __global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R
 }
 
 // Top two over the row blocks of each patch (in block order), then the
-// refinement and the Gaussian-fit samples of k_peak.
+// refinement and the Gaussian-fit samples (peak_finish).
 template <bool GAUSS>
 __global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, const float *mag, int W, int mode,
                                                     PeakRes *res, GaussArg G)
@@ -1464,6 +1334,318 @@ __global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, 
         raw = best.v1;
     }
     peak_finish<GAUSS>(mag + (size_t)b * W * W, W, mode, I, J, raw, best.v2, res + b, G, b);
+}
+
+// ---------------------------------------------------------------------
+// Amplitude correlation (getAmpPatches speckleTrack.c:1006-1189, ampMatch
+// :398-484) in five passes that never form a padded array: after the range
+// transforms of the two ws x ws patches,
+//   k_amp_colx    azimuth transforms, Doppler rotation, pad to W = 2 ws,
+//                 inverse azimuth transforms               -> T [2][B][W][ws]
+//   k_amp_rowdet  range pad, inverse range transforms, detection |z|^2,
+//                 amplitude, partial sums of the zero test and the means
+//                                                          -> amplitudes [2][B][W][W]
+//   k_amp_rowfwd  zero test, mean removal, partial powers, forward range
+//                 transforms                               -> F [2][B][W][W]
+//   k_amp_colx2   forward azimuth transforms, F1 conj(F2), inverse azimuth
+//                 transforms                               -> S [B][W][W]
+//   k_cpx_rowmag<false>  inverse range transforms, |C|, coarse top two
+// so each patch moves about a third of the bytes of the padded form.
+
+// partial sums of one k_amp_rowdet workgroup
+struct AmpPart {
+    double s1, s2, m1, m2;
+    int nz, pad;
+};
+
+// workgroup -> (item, block) with neighbouring blocks of one item on one
+// XCD (dispatch deals consecutive workgroups round-robin over 8 XCDs)
+__device__ inline void xcd_item_block(int nbx, int &item, int &blk)
+{
+    const int total = nbx * gridDim.y, flat = blockIdx.y * nbx + blockIdx.x;
+    const int per = total / 8;
+    const int L = flat < per * 8 ? (flat % 8) * per + flat / 8 : flat;
+    item = L / nbx;
+    blk = L - item * nbx;
+}
+
+// zeroPadFFT's azimuth half (speckleTrack.c:2014-2038): C adjacent range
+// frequencies la of one image's row spectra per workgroup; item z = image
+// * B + patch.
+__global__ void __launch_bounds__(NT) k_amp_colx(const float2 *I, int B, int ws, const int *az, const uint8_t *state,
+                                                  float2 *T, int lc, stfft::PlanArgs Pf, stfft::PlanArgs Pi,
+                                                  stfft::FastDiv dW)
+{
+    extern __shared__ float2 ldsx[];
+    constexpr int IT = kStElems / NT;
+    const int C = 1 << lc, W = 2 * ws, h = ws / 2;
+    int z, cb;
+    xcd_item_block(gridDim.x, z, cb);
+    const int b = z % B, c0 = cb * C;
+    if (state[b] != 2)
+        return;
+    const int nc = imin(C, ws - c0), px = C > 1 ? W + 1 : W;
+    float2 *X = ldsx;  // [C][px]
+    const float2 *Ib = I + (size_t)z * ws * ws + c0;
+    {
+        const int cnt = ws << lc;
+        float2 u[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1);
+            u[it] = t < cnt && c < nc ? Ib[(size_t)a * ws + c] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1);
+            if (t < cnt)
+                X[c * px + a] = u[it];
+        }
+    }
+    __syncthreads();
+    stfft::lds_fft<kStElems, NT>(X, px, C, Pf, -1);
+    {
+        const int azb = az[b], cnt = C * W;
+        float2 v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            v[it] = make_float2(0.f, 0.f);
+            if (t < cnt) {
+                const int c = stfft::fdiv(t, dW), n = t - c * W;
+                const int k = n < h ? n : (n >= W - h ? n - W : (1 << 20));
+                if (k != (1 << 20))
+                    v[it] = X[c * px + (k + ws + azb) % ws];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            if (t < cnt) {
+                const int c = stfft::fdiv(t, dW), n = t - c * W;
+                X[c * px + n] = v[it];
+            }
+        }
+        __syncthreads();
+    }
+    stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
+    float2 *Tb = T + (size_t)z * W * ws + c0;
+    const int cnt = W << lc;
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int t = threadIdx.x + it * NT, n = t >> lc, c = t & (C - 1);
+        if (t < cnt && c < nc)
+            Tb[(size_t)n * ws + c] = X[c * px + n];
+    }
+}
+
+// zeroPadFFT's range half, then detection and amplitude
+// (speckleTrack.c:1166-1188): R rows of both images of one patch per
+// workgroup (the zero test counts cells where either image is below the
+// threshold).
+__global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int B, int ws, int R,
+                                                    const uint8_t *state, float *amp, AmpPart *part,
+                                                    stfft::PlanArgs Pi, stfft::FastDiv dW)
+{
+    extern __shared__ float2 ldsx[];
+    __shared__ double sh[NT / 64];
+    __shared__ int shi[NT / 64];
+    constexpr int IT = kStElems / NT;
+    const int W = 2 * ws, h = ws / 2;
+    int b, blk;
+    xcd_item_block(gridDim.x, b, blk);
+    if (state[b] != 2)
+        return;
+    const int n0 = blk * R, nrow = imin(R, W - n0), cnt = nrow * W;
+    float2 *X1 = ldsx, *X2 = ldsx + (size_t)R * W;
+    const float2 *T1 = T + ((size_t)b * W + n0) * ws, *T2 = T + (((size_t)B + b) * W + n0) * ws;
+    for (int t0 = 0; t0 < cnt; t0 += kStElems) {
+        float2 u[IT], v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = t0 + threadIdx.x + it * NT;
+            u[it] = v[it] = make_float2(0.f, 0.f);
+            if (t < cnt) {
+                const int i = stfft::fdiv(t, dW), m = t - i * W;
+                const int la = m < h ? m : (m >= W - h ? m - ws : -1);
+                if (la >= 0) {
+                    u[it] = T1[(size_t)i * ws + la];
+                    v[it] = T2[(size_t)i * ws + la];
+                }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = t0 + threadIdx.x + it * NT;
+            if (t < cnt) {
+                X1[t] = u[it];
+                X2[t] = v[it];
+            }
+        }
+    }
+    __syncthreads();
+    stfft::lds_fft<kStElems, NT>(X1, W, nrow, Pi, +1);
+    stfft::lds_fft<kStElems, NT>(X2, W, nrow, Pi, +1);
+    const size_t sws = (size_t)ws;
+    const float scale = (float)(1. / (double)(sws * sws * sws * sws * 8));
+    const float thr = P.im.stype == ST_SAMPLE_C64 ? 1.0e-15f : 1.0e-6f;
+    float *A1 = amp + ((size_t)b * W + n0) * W, *A2 = amp + (((size_t)B + b) * W + n0) * W;
+    int nz = 0;
+    double s1 = 0, s2 = 0, m1 = 0, m2 = 0;
+    for (int t = threadIdx.x; t < cnt; t += NT) {
+#pragma clang fp contract(off)
+        const float2 u = X1[t], v = X2[t];
+        const float x1 = (u.x * u.x + u.y * u.y) * scale, x2 = (v.x * v.x + v.y * v.y) * scale;
+        nz += (x1 < thr || x2 < thr);
+        s1 += x1;
+        s2 += x2;
+        const float a1 = (float)sqrt((double)x1), a2 = (float)sqrt((double)x2);
+        m1 += a1;
+        m2 += a2;
+        A1[t] = a1;
+        A2[t] = a2;
+    }
+    nz = block_sum_i<NT>(nz, shi);
+    s1 = block_sum_d<NT>(s1, sh);
+    s2 = block_sum_d<NT>(s2, sh);
+    m1 = block_sum_d<NT>(m1, sh);
+    m2 = block_sum_d<NT>(m2, sh);
+    if (threadIdx.x == 0)
+        part[(size_t)b * gridDim.x + blk] = AmpPart{s1, s2, m1, m2, nz, 0};
+}
+
+// The zero test and mean removal of ampMatch (speckleTrack.c:437-467) and
+// the forward range transforms of the correlation: every workgroup of a
+// patch sums the detection partials in the same order; a rejected patch
+// is marked "no data" (state 1) by its first workgroup.
+__global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int W, int R, int nPd,
+                                                    const AmpPart *dpart, uint8_t *state, float2 *F, double2 *ppart,
+                                                    stfft::PlanArgs Pf)
+{
+    extern __shared__ float2 ldsx[];
+    __shared__ double sh[NT / 64];
+    __shared__ double shm[2];
+    __shared__ int shrej;
+    int b, blk;
+    xcd_item_block(gridDim.x, b, blk);
+    if (state[b] != 2)
+        return;
+    if (threadIdx.x == 0) {
+        const AmpPart *q = dpart + (size_t)b * nPd;
+        AmpPart a = q[0];
+        for (int k = 1; k < nPd; k++) {
+            a.s1 += q[k].s1;
+            a.s2 += q[k].s2;
+            a.m1 += q[k].m1;
+            a.m2 += q[k].m2;
+            a.nz += q[k].nz;
+        }
+        shrej = a.nz > 20 || a.s1 < 1.0e-6 || a.s2 < 1.0e-6;
+        shm[0] = a.m1 / (double)((size_t)W * W);
+        shm[1] = a.m2 / (double)((size_t)W * W);
+    }
+    __syncthreads();
+    if (shrej) {
+        if (blk == 0 && threadIdx.x == 0)
+            state[b] = 1;
+        return;
+    }
+    const double m1 = shm[0], m2 = shm[1];
+    const int n0 = blk * R, nrow = imin(R, W - n0), cnt = nrow * W;
+    float2 *X1 = ldsx, *X2 = ldsx + (size_t)R * W;
+    const float *A1 = amp + ((size_t)b * W + n0) * W, *A2 = amp + (((size_t)B + b) * W + n0) * W;
+    double p1 = 0, p2 = 0;
+    for (int t = threadIdx.x; t < cnt; t += NT) {
+#pragma clang fp contract(off)
+        float v1 = A1[t], v2 = A2[t];
+        if (v1 > 1.0e-20)
+            v1 = (float)(v1 - m1);
+        v2 = (float)(v2 - m2);
+        p1 += v1 * v1;
+        p2 += v2 * v2;
+        X1[t] = make_float2(v1, 0.f);
+        X2[t] = make_float2(v2, 0.f);
+    }
+    p1 = block_sum_d<NT>(p1, sh);
+    p2 = block_sum_d<NT>(p2, sh);
+    if (threadIdx.x == 0)
+        ppart[(size_t)b * gridDim.x + blk] = make_double2(p1, p2);
+    __syncthreads();
+    stfft::lds_fft<kStElems, NT>(X1, W, nrow, Pf, -1);
+    stfft::lds_fft<kStElems, NT>(X2, W, nrow, Pf, -1);
+    float2 *F1 = F + ((size_t)b * W + n0) * W, *F2 = F + (((size_t)B + b) * W + n0) * W;
+    for (int t = threadIdx.x; t < cnt; t += NT) {
+        F1[t] = X1[t];
+        F2[t] = X2[t];
+    }
+}
+
+// The correlation's azimuth half: C adjacent columns of both range spectra
+// per workgroup, forward transforms, F1 conj(F2), inverse transform.
+__global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *F, int B, int W, const uint8_t *state, float2 *S,
+                                                   int lc, stfft::PlanArgs Pf, stfft::PlanArgs Pi, stfft::FastDiv dW)
+{
+    extern __shared__ float2 ldsx[];
+    constexpr int IT = kStElems / NT;
+    const int C = 1 << lc;
+    int b, cb;
+    xcd_item_block(gridDim.x, b, cb);
+    if (state[b] != 2)
+        return;
+    const int c0 = cb * C, nc = imin(C, W - c0), px = C > 1 ? W + 1 : W;
+    float2 *X = ldsx, *Y = ldsx + (size_t)C * px;
+    const float2 *F1 = F + (size_t)b * W * W + c0, *F2 = F + ((size_t)B + b) * W * W + c0;
+    const int cnt = W << lc;
+    for (int t0 = 0; t0 < cnt; t0 += kStElems) {
+        float2 u[IT], v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = t0 + threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1);
+            const bool ok = t < cnt && c < nc;
+            u[it] = ok ? F1[(size_t)a * W + c] : make_float2(0.f, 0.f);
+            v[it] = ok ? F2[(size_t)a * W + c] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = t0 + threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1);
+            if (t < cnt) {
+                X[c * px + a] = u[it];
+                Y[c * px + a] = v[it];
+            }
+        }
+    }
+    __syncthreads();
+    stfft::lds_fft<kStElems, NT>(X, px, C, Pf, -1);
+    stfft::lds_fft<kStElems, NT>(Y, px, C, Pf, -1);
+    for (int t = threadIdx.x; t < cnt; t += NT) {
+        const int c = stfft::fdiv(t, dW), n = t - c * W;
+        X[c * px + n] = cmulconj(X[c * px + n], Y[c * px + n]);
+    }
+    __syncthreads();
+    stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
+    float2 *Sb = S + (size_t)b * W * W + c0;
+    for (int t = threadIdx.x; t < cnt; t += NT) {
+        const int n = t >> lc, c = t & (C - 1);
+        if (c < nc)
+            Sb[(size_t)n * W + c] = X[c * px + n];
+    }
+}
+
+// The mean powers of ampMatch from the partials, in block order.
+__global__ void k_amp_psum(const double2 *ppart, int nPf, int B, int W, const uint8_t *state, double *p)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B || state[b] != 2)
+        return;
+    double p1 = 0, p2 = 0;
+    for (int k = 0; k < nPf; k++) {
+        p1 += ppart[(size_t)b * nPf + k].x;
+        p2 += ppart[(size_t)b * nPf + k].y;
+    }
+    const double N = (double)((size_t)W * W);
+    p[2 * b] = p1 / N;
+    p[2 * b + 1] = p2 / N;
 }
 
 // Lanes per transform of the fused kernel.  Lane pairs at every window
@@ -2251,7 +2433,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
     // batch sizes
     const size_t perCpx = (size_t)w * w * (16 + 32 + 32 + 16) + 2048;
     int ws_max = par.wAa * (par.maxTries >= 2 ? LA : 1);
-    const size_t perAmp = (size_t)ws_max * ws_max * 192 + 2048;
+    const size_t perAmp = (size_t)ws_max * ws_max * 128 + (size_t)2 * ws_max * 96 + 4096;
     size_t budget = ctx->scratchBudget;
     const size_t kMaxBatch = 8192;  // keeps 2 x batch inside the 65535 grid-z limit of the transposes
     int Bc = (int)std::max<size_t>(1, std::min<size_t>({(size_t)npts, budget / perCpx, kMaxBatch}));
@@ -2476,7 +2658,8 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
             ctx->time_begin(kPeak);
             hipLaunchKernelGGL(k_cpx_colx, dim3((w + Cx - 1) / Cx, nb), dim3(NT), ldsX, st, C.P, nb, w, C.az, C.S, lcx,
                                PfW, PiW2, dW2);
-            hipLaunchKernelGGL(k_cpx_rowmag, dim3(nPart, nb), dim3(NT), ldsR, st, C.S, w, Rr, C.mag, part, PiW2, dW2);
+            hipLaunchKernelGGL(k_cpx_rowmag<true>, dim3(nPart, nb), dim3(NT), ldsR, st, C.S, w, Rr, C.mag, part, PiW2, dW2,
+                               nullptr);
             const GaussArg G{gauss ? 1 : 0, w, par.hanningFlag ? 1.5 : 1.0, 1.0, C.p};
             if (gauss) {
                 hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0, C.res,
@@ -2511,40 +2694,65 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         if (nA == 0)
             break;
         const int ws = par.wAa * (large ? LA : 1), W = 2 * ws;
-        const size_t per = (size_t)ws * ws * 192 + 2048;
+        // blocking: Cx interpolation columns per k_amp_colx workgroup, Rd
+        // rows per k_amp_rowdet / k_amp_rowfwd workgroup, Cc correlation
+        // columns per k_amp_colx2 workgroup, Rm rows per k_cpx_rowmag one
+        int lcx = 0, lcc = 0;
+        while ((2 << lcx) <= 16 && (2 << lcx) <= ws && (2 << lcx) * W <= kStElems)
+            lcx++;
+        while ((2 << lcc) <= 16 && (2 << lcc) * W * 2 <= kStElems)
+            lcc++;
+        const int Cx = 1 << lcx, Cc = 1 << lcc;
+        const int Rd = std::max(1, kStElems / (2 * W)), nPd = (W + Rd - 1) / Rd;
+        const int Rm = kStElems / W, nPm = (W + Rm - 1) / Rm;
+        const size_t ldsX = (size_t)Cx * (Cx > 1 ? W + 1 : W) * sizeof(float2);
+        const size_t ldsD = (size_t)2 * Rd * W * sizeof(float2);
+        const size_t ldsC = (size_t)2 * Cc * (Cc > 1 ? W + 1 : W) * sizeof(float2);
+        const size_t ldsM = (size_t)Rm * W * sizeof(float2);
+        const size_t per = (size_t)ws * ws * 128 + (size_t)nPd * (sizeof(AmpPart) + sizeof(double2)) +
+                           (size_t)nPm * sizeof(Top2) + 2048;
         const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nA, (ar.cap - ar.used) / per, kMaxBatch}));
         AmpBatch A;
         A.I = ar.take<float2>((size_t)2 * B * ws * ws);
-        A.U = ar.take<float2>((size_t)2 * B * W * W);
+        A.F = ar.take<float2>((size_t)2 * B * W * W);
         A.S = ar.take<float2>((size_t)B * W * W);
-        A.tmp = ar.take<float2>((size_t)2 * B * W * W);
         A.mag = ar.take<float>((size_t)B * W * W);
         A.res = ar.take<PeakRes>(B);
         A.sig = ar.take<float>(2 * (size_t)B);
         A.state = ar.take<uint8_t>(B);
         A.az = ar.take<int>(B);
         A.p = ar.take<double>(2 * (size_t)B);
+        A.dpart = ar.take<AmpPart>((size_t)B * nPd);
+        A.ppart = ar.take<double2>((size_t)B * nPd);
+        A.part = ar.take<Top2>((size_t)B * nPm);
+        const stfft::PlanArgs Pfs = ctx->fft.plan(ws, -1), PfW = ctx->fft.plan(W, -1), PiW = ctx->fft.plan(W, +1);
+        const stfft::FastDiv dW = stfft::make_fastdiv((uint32_t)W);
         for (int off = 0; off < nA; off += B) {
             const int nb = std::min(B, nA - off);
             hipLaunchKernelGGL(k_amp_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, large, A);
-            ctx->fft.fft2d(A.I, A.tmp, 2 * nb, ws, ws, -1, st);
-            hipLaunchKernelGGL(k_amp_pad, dim3((W * W + 255) / 256, 2 * nb), dim3(256), 0, st, A.I, nb, ws, A.az,
-                               A.state, A.U);
-            ctx->fft.fft2d(A.U, A.tmp, 2 * nb, W, W, +1, st);
+            ctx->fft.rows(A.I, (long)2 * nb * ws, ws, -1, st);
+            hipLaunchKernelGGL(k_amp_colx, dim3((ws + Cx - 1) / Cx, 2 * nb), dim3(NT), ldsX, st, A.I, nb, ws, A.az,
+                               A.state, A.F, lcx, Pfs, PiW, dW);
             ctx->time_begin(kAmpDet);
-            hipLaunchKernelGGL(k_amp_detect, dim3(nb), dim3(NT), 0, st, P, nb, ws, large, A);
+            hipLaunchKernelGGL(k_amp_rowdet, dim3(nPd, nb), dim3(NT), ldsD, st, P, A.F, nb, ws, Rd, A.state,
+                               (float *)A.S, A.dpart, PiW, dW);
             ctx->time_end(kAmpDet);
-            ctx->fft.fft2d(A.U, A.tmp, 2 * nb, W, W, -1, st);
-            hipLaunchKernelGGL(k_amp_cross, dim3((W * W + 255) / 256, nb), dim3(256), 0, st, A.U, nb, W * W, A.state,
-                               A.S);
-            ctx->fft.fft2d(A.S, A.tmp, nb, W, W, +1, st);
+            hipLaunchKernelGGL(k_amp_rowfwd, dim3(nPd, nb), dim3(NT), ldsD, st, (const float *)A.S, nb, W, Rd, nPd,
+                               A.dpart, A.state, A.F, A.ppart, PfW);
+            hipLaunchKernelGGL(k_amp_colx2, dim3((W + Cc - 1) / Cc, nb), dim3(NT), ldsC, st, A.F, nb, W, A.state, A.S,
+                               lcc, PfW, PiW, dW);
+            hipLaunchKernelGGL(k_amp_psum, dim3((nb + 127) / 128), dim3(128), 0, st, A.ppart, nPd, nb, W, A.state,
+                               A.p);
             ctx->time_begin(kAmpPeak);
+            hipLaunchKernelGGL(k_cpx_rowmag<false>, dim3(nPm, nb), dim3(NT), ldsM, st, A.S, ws, Rm, A.mag, A.part, PiW,
+                               dW, A.state);
             const GaussArg G{gauss ? 1 : 0, W / 2, 1.0, 16.0, A.p};
             if (gauss) {
-                hipLaunchKernelGGL(k_peak<true>, dim3(nb), dim3(NT), 0, st, A.S, A.mag, W, 1, A.res, G);
+                hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, A.part, nPm, A.mag, W, 1, A.res, G);
                 hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, A.mag, W, 1, A.res, nb, G);
             } else
-                hipLaunchKernelGGL(k_peak<false>, dim3(nb), dim3(NT), 0, st, A.S, A.mag, W, 1, A.res, G);
+                hipLaunchKernelGGL(k_peak_merge<false>, dim3(nb), dim3(NT), 0, st, A.part, nPm, A.mag, W, 1, A.res,
+                                   G);
             ctx->time_end(kAmpPeak);
             hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, A.res, nb, A.sig);
             hipLaunchKernelGGL(k_amp_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, large, A);

@@ -69,6 +69,19 @@ def test_parity_amplitude_only(device, oracle32, oracle64):
     _check(rep, probs)
 
 
+@pytest.mark.parametrize("w", [24, 48])
+def test_parity_amplitude_radix3_windows(device, oracle32, oracle64, w):
+    """-noComplex with windows of 24 and 48 (transforms of 48 / 96 / 192
+    points, radix 3 stages) and the doubled-window retries."""
+    s = configs._spec(480, 480, 200 + w, dr0=0.6, drR=1e-3, da0=-1.1, daA=2e-3, gmin=0.3, gmax=0.85, gLr=300,
+                      gLa=250)
+    p = small_params(w, 480, 480, step=w // 2, noComplex=1, rShiftPoly=[0.6, 1e-3, 0], aShiftPoly=[-1.1, 0, 2e-3])
+    a, b = track.synth_host(s)
+    g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
+    assert g.nAmp > 0.3 * g.type.size
+    _check(rep, probs)
+
+
 @pytest.mark.parametrize("w", [16, 32])
 def test_parity_window_sizes(device, oracle32, oracle64, w):
     s = configs._spec(480, 480, 100 + w, dr0=0.8, drR=1e-3, da0=-1.3, daA=2e-3, gmin=0.35, gmax=0.85,
