@@ -817,7 +817,7 @@ struct AmpPart;
 struct Top2;
 struct AmpBatch {
     float2 *I;       // [2][B][ws^2] patches, then their range spectra
-    float2 *F;       // [2][B][W^2]: interpolation columns [2][B][W][ws], then forward range spectra
+    float2 *F;       // [2][B][W^2]: interpolation columns [2][B][W][ws], then range spectra Z [B][W^2]
     float2 *S;       // [B][W^2]: amplitudes [2][B][W^2] (float), then the correlation's azimuth half
     float *mag;      // [B][W^2]
     PeakRes *res;
@@ -1346,11 +1346,12 @@ __global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, 
 //                 amplitude, partial sums of the zero test and the means
 //                                                          -> amplitudes [2][B][W][W]
 //   k_amp_rowfwd  zero test, mean removal, partial powers, forward range
-//                 transforms                               -> F [2][B][W][W]
-//   k_amp_colx2   forward azimuth transforms, F1 conj(F2), inverse azimuth
-//                 transforms                               -> S [B][W][W]
+//                 transforms of v1 + i v2                  -> Z [B][W][W]
+//   k_amp_colx2   forward azimuth transforms, separation of the two
+//                 spectra, F1 conj(F2), inverse azimuth transforms
+//                                                          -> S [B][W][W]
 //   k_cpx_rowmag<false>  inverse range transforms, |C|, coarse top two
-// so each patch moves about a third of the bytes of the padded form.
+// so each patch moves about a quarter of the bytes of the padded form.
 
 // partial sums of one k_amp_rowdet workgroup
 struct AmpPart {
@@ -1518,9 +1519,11 @@ __global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int 
 // The zero test and mean removal of ampMatch (speckleTrack.c:437-467) and
 // the forward range transforms of the correlation: every workgroup of a
 // patch sums the detection partials in the same order; a rejected patch
-// is marked "no data" (state 1) by its first workgroup.
+// is marked "no data" (state 1) by its first workgroup.  Both images are
+// real, so one complex transform of v1 + i v2 carries both spectra
+// (k_amp_colx2 separates them); out Z [B][W][W].
 __global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int W, int R, int nPd,
-                                                    const AmpPart *dpart, uint8_t *state, float2 *F, double2 *ppart,
+                                                    const AmpPart *dpart, uint8_t *state, float2 *Z, double2 *ppart,
                                                     stfft::PlanArgs Pf)
 {
     extern __shared__ float2 ldsx[];
@@ -1553,7 +1556,7 @@ __global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int 
     }
     const double m1 = shm[0], m2 = shm[1];
     const int n0 = blk * R, nrow = imin(R, W - n0), cnt = nrow * W;
-    float2 *X1 = ldsx, *X2 = ldsx + (size_t)R * W;
+    float2 *X = ldsx;
     const float *A1 = amp + ((size_t)b * W + n0) * W, *A2 = amp + (((size_t)B + b) * W + n0) * W;
     double p1 = 0, p2 = 0;
     for (int t = threadIdx.x; t < cnt; t += NT) {
@@ -1564,47 +1567,54 @@ __global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int 
         v2 = (float)(v2 - m2);
         p1 += v1 * v1;
         p2 += v2 * v2;
-        X1[t] = make_float2(v1, 0.f);
-        X2[t] = make_float2(v2, 0.f);
+        X[t] = make_float2(v1, v2);
     }
     p1 = block_sum_d<NT>(p1, sh);
     p2 = block_sum_d<NT>(p2, sh);
     if (threadIdx.x == 0)
         ppart[(size_t)b * gridDim.x + blk] = make_double2(p1, p2);
     __syncthreads();
-    stfft::lds_fft<kStElems, NT>(X1, W, nrow, Pf, -1);
-    stfft::lds_fft<kStElems, NT>(X2, W, nrow, Pf, -1);
-    float2 *F1 = F + ((size_t)b * W + n0) * W, *F2 = F + (((size_t)B + b) * W + n0) * W;
-    for (int t = threadIdx.x; t < cnt; t += NT) {
-        F1[t] = X1[t];
-        F2[t] = X2[t];
-    }
+    stfft::lds_fft<kStElems, NT>(X, W, nrow, Pf, -1);
+    float2 *Zb = Z + ((size_t)b * W + n0) * W;
+    for (int t = threadIdx.x; t < cnt; t += NT)
+        Zb[t] = X[t];
 }
 
-// The correlation's azimuth half: C adjacent columns of both range spectra
-// per workgroup, forward transforms, F1 conj(F2), inverse transform.
-__global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *F, int B, int W, const uint8_t *state, float2 *S,
-                                                   int lc, stfft::PlanArgs Pf, stfft::PlanArgs Pi, stfft::FastDiv dW)
+// The two spectra of Z = F(v1) + i F(v2) at (k, l) from Z(k, l) and
+// Z(-k, -l), and their cross spectrum F1 conj(F2) (ampMatch's product).
+__device__ inline float2 amp_cross_sep(float2 za, float2 zb)
+{
+    const float2 x1 = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));  // (za + conj zb) / 2
+    const float2 x2 = make_float2(0.5f * (za.y + zb.y), 0.5f * (zb.x - za.x));  // (za - conj zb) / 2i
+    return cmulconj(x1, x2);
+}
+
+// The correlation's azimuth half: slot c of a workgroup holds column
+// l = c0 + c (l < W/2) in X and its mirror -l in Y (column W/2 for l = 0);
+// forward transforms, separation and cross spectra of both columns,
+// inverse transforms; out S [B][W][W].
+__global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *Z, int W, const uint8_t *state, float2 *S, int lc,
+                                                   stfft::PlanArgs Pf, stfft::PlanArgs Pi, stfft::FastDiv dW)
 {
     extern __shared__ float2 ldsx[];
     constexpr int IT = kStElems / NT;
-    const int C = 1 << lc;
+    const int C = 1 << lc, H = W / 2;
     int b, cb;
     xcd_item_block(gridDim.x, b, cb);
     if (state[b] != 2)
         return;
-    const int c0 = cb * C, nc = imin(C, W - c0), px = C > 1 ? W + 1 : W;
+    const int c0 = cb * C, nc = imin(C, H - c0), px = C > 1 ? W + 1 : W;
     float2 *X = ldsx, *Y = ldsx + (size_t)C * px;
-    const float2 *F1 = F + (size_t)b * W * W + c0, *F2 = F + ((size_t)B + b) * W * W + c0;
+    const float2 *Zb = Z + (size_t)b * W * W;
     const int cnt = W << lc;
     for (int t0 = 0; t0 < cnt; t0 += kStElems) {
         float2 u[IT], v[IT];
 #pragma unroll
         for (int it = 0; it < IT; it++) {
-            const int t = t0 + threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1);
+            const int t = t0 + threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1), l = c0 + c;
             const bool ok = t < cnt && c < nc;
-            u[it] = ok ? F1[(size_t)a * W + c] : make_float2(0.f, 0.f);
-            v[it] = ok ? F2[(size_t)a * W + c] : make_float2(0.f, 0.f);
+            u[it] = ok ? Zb[(size_t)a * W + l] : make_float2(0.f, 0.f);
+            v[it] = ok ? Zb[(size_t)a * W + (l == 0 ? H : W - l)] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int it = 0; it < IT; it++) {
@@ -1618,17 +1628,44 @@ __global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *F, int B, int W,
     __syncthreads();
     stfft::lds_fft<kStElems, NT>(X, px, C, Pf, -1);
     stfft::lds_fft<kStElems, NT>(Y, px, C, Pf, -1);
-    for (int t = threadIdx.x; t < cnt; t += NT) {
-        const int c = stfft::fdiv(t, dW), n = t - c * W;
-        X[c * px + n] = cmulconj(X[c * px + n], Y[c * px + n]);
+    for (int t0 = 0; t0 < cnt; t0 += kStElems) {
+        float2 u[IT], v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = t0 + threadIdx.x + it * NT;
+            u[it] = v[it] = make_float2(0.f, 0.f);
+            if (t < cnt) {
+                const int c = stfft::fdiv(t, dW), k = t - c * W, km = k == 0 ? 0 : W - k;
+                if (c0 + c == 0) {  // columns 0 and W/2 are their own mirrors
+                    u[it] = amp_cross_sep(X[k], X[km]);
+                    v[it] = amp_cross_sep(Y[k], Y[km]);
+                } else {
+                    u[it] = amp_cross_sep(X[c * px + k], Y[c * px + km]);
+                    v[it] = amp_cross_sep(Y[c * px + k], X[c * px + km]);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = t0 + threadIdx.x + it * NT;
+            if (t < cnt) {
+                const int c = stfft::fdiv(t, dW), k = t - c * W;
+                X[c * px + k] = u[it];
+                Y[c * px + k] = v[it];
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
     stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
-    float2 *Sb = S + (size_t)b * W * W + c0;
+    stfft::lds_fft<kStElems, NT>(Y, px, C, Pi, +1);
+    float2 *Sb = S + (size_t)b * W * W;
     for (int t = threadIdx.x; t < cnt; t += NT) {
-        const int n = t >> lc, c = t & (C - 1);
-        if (c < nc)
-            Sb[(size_t)n * W + c] = X[c * px + n];
+        const int n = t >> lc, c = t & (C - 1), l = c0 + c;
+        if (c < nc) {
+            Sb[(size_t)n * W + l] = X[c * px + n];
+            Sb[(size_t)n * W + (l == 0 ? H : W - l)] = Y[c * px + n];
+        }
     }
 }
 
@@ -2700,13 +2737,14 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         int lcx = 0, lcc = 0;
         while ((2 << lcx) <= 16 && (2 << lcx) <= ws && (2 << lcx) * W <= kStElems)
             lcx++;
-        while ((2 << lcc) <= 16 && (2 << lcc) * W * 2 <= kStElems)
+        while ((2 << lcc) <= 16 && (2 << lcc) <= ws && (2 << lcc) * W * 2 <= kStElems)
             lcc++;
         const int Cx = 1 << lcx, Cc = 1 << lcc;
         const int Rd = std::max(1, kStElems / (2 * W)), nPd = (W + Rd - 1) / Rd;
+        const int Rf = kStElems / W, nPf = (W + Rf - 1) / Rf;
         const int Rm = kStElems / W, nPm = (W + Rm - 1) / Rm;
         const size_t ldsX = (size_t)Cx * (Cx > 1 ? W + 1 : W) * sizeof(float2);
-        const size_t ldsD = (size_t)2 * Rd * W * sizeof(float2);
+        const size_t ldsD = (size_t)2 * Rd * W * sizeof(float2), ldsF = (size_t)Rf * W * sizeof(float2);
         const size_t ldsC = (size_t)2 * Cc * (Cc > 1 ? W + 1 : W) * sizeof(float2);
         const size_t ldsM = (size_t)Rm * W * sizeof(float2);
         const size_t per = (size_t)ws * ws * 128 + (size_t)nPd * (sizeof(AmpPart) + sizeof(double2)) +
@@ -2737,11 +2775,11 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
             hipLaunchKernelGGL(k_amp_rowdet, dim3(nPd, nb), dim3(NT), ldsD, st, P, A.F, nb, ws, Rd, A.state,
                                (float *)A.S, A.dpart, PiW, dW);
             ctx->time_end(kAmpDet);
-            hipLaunchKernelGGL(k_amp_rowfwd, dim3(nPd, nb), dim3(NT), ldsD, st, (const float *)A.S, nb, W, Rd, nPd,
+            hipLaunchKernelGGL(k_amp_rowfwd, dim3(nPf, nb), dim3(NT), ldsF, st, (const float *)A.S, nb, W, Rf, nPd,
                                A.dpart, A.state, A.F, A.ppart, PfW);
-            hipLaunchKernelGGL(k_amp_colx2, dim3((W + Cc - 1) / Cc, nb), dim3(NT), ldsC, st, A.F, nb, W, A.state, A.S,
+            hipLaunchKernelGGL(k_amp_colx2, dim3((ws + Cc - 1) / Cc, nb), dim3(NT), ldsC, st, A.F, W, A.state, A.S,
                                lcc, PfW, PiW, dW);
-            hipLaunchKernelGGL(k_amp_psum, dim3((nb + 127) / 128), dim3(128), 0, st, A.ppart, nPd, nb, W, A.state,
+            hipLaunchKernelGGL(k_amp_psum, dim3((nb + 127) / 128), dim3(128), 0, st, A.ppart, nPf, nb, W, A.state,
                                A.p);
             ctx->time_begin(kAmpPeak);
             hipLaunchKernelGGL(k_cpx_rowmag<false>, dim3(nPm, nb), dim3(NT), ldsM, st, A.S, ws, Rm, A.mag, A.part, PiW,
