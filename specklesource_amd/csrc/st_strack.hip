@@ -1250,24 +1250,15 @@ __global__ void __launch_bounds__(NT) k_cpx_colx(const float2 *P, int B, int w, 
 // of T spread over 2w (Nyquist on the negative side, zeros between),
 // inverse transform, |C| into the fftshifted surface M, and this block's
 // top two in findCPeak's scan-rank order (speckleTrack.c:744-805).
-// PAD = false: T already holds all W = 2w columns (the amplitude
-// correlation, which is not zero padded), and patches whose state is not
-// "matched" report no peak.
-template <bool PAD>
 __global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R, float *mag, Top2 *part,
-                                                    stfft::PlanArgs Pi, stfft::FastDiv dW, const uint8_t *state)
+                                                    stfft::PlanArgs Pi, stfft::FastDiv dW)
 {
     extern __shared__ float2 ldsx[];
     __shared__ Top2 sht[NT / 64];
     constexpr int IT = kStElems / NT;
-    const int W = 2 * w, h = w / 2, H = w, tc = PAD ? w : W;
+    const int W = 2 * w, h = w / 2, H = w;
     const int b = blockIdx.y, n0 = blockIdx.x * R, nrow = imin(R, W - n0);
-    if (!PAD && state[b] != 2) {
-        if (threadIdx.x == 0)
-            part[(size_t)b * gridDim.x + blockIdx.x] = Top2{0.0f, 0x7fffffff, 0.0f};
-        return;
-    }
-    const float2 *Tb = T + ((size_t)b * W + n0) * tc;
+    const float2 *Tb = T + ((size_t)b * W + n0) * w;
     const int cnt = nrow * W;
     {
         float2 v[IT];
@@ -1277,9 +1268,9 @@ __global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R
             v[it] = make_float2(0.f, 0.f);
             if (t < cnt) {
                 const int i = stfft::fdiv(t, dW), m = t - i * W;
-                const int la = !PAD ? m : (m < h ? m : (m >= W - h ? m - w : -1));
+                const int la = m < h ? m : (m >= W - h ? m - w : -1);
                 if (la >= 0)
-                    v[it] = Tb[(size_t)i * tc + la];
+                    v[it] = Tb[(size_t)i * w + la];
             }
         }
 #pragma unroll
@@ -1348,9 +1339,11 @@ __global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, 
 //   k_amp_rowfwd  zero test, mean removal, partial powers, forward range
 //                 transforms of v1 + i v2                  -> Z [B][W][W]
 //   k_amp_colx2   forward azimuth transforms, separation of the two
-//                 spectra, F1 conj(F2), inverse azimuth transforms
-//                                                          -> S [B][W][W]
-//   k_cpx_rowmag<false>  inverse range transforms, |C|, coarse top two
+//                 spectra, F1 conj(F2), inverse azimuth transforms of the
+//                 columns l <= W/2 (the correlation is real)
+//                                                          -> T [B][W][W/2 + 1 of W]
+//   k_amp_rowmag  inverse range transforms two rows at a time, |s|,
+//                 coarse top two
 // so each patch moves about a quarter of the bytes of the padded form.
 
 // partial sums of one k_amp_rowdet workgroup
@@ -1657,16 +1650,86 @@ __global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *Z, int W, const 
         }
         __syncthreads();
     }
+    // s = F^-1(F1 conj(F2)) is real, so T(n, -l) = conj T(n, l): only the
+    // columns l <= W/2 are transformed back and stored
     stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
-    stfft::lds_fft<kStElems, NT>(Y, px, C, Pi, +1);
+    if (c0 == 0)
+        stfft::lds_fft<kStElems, NT>(Y, px, 1, Pi, +1);
     float2 *Sb = S + (size_t)b * W * W;
     for (int t = threadIdx.x; t < cnt; t += NT) {
-        const int n = t >> lc, c = t & (C - 1), l = c0 + c;
-        if (c < nc) {
-            Sb[(size_t)n * W + l] = X[c * px + n];
-            Sb[(size_t)n * W + (l == 0 ? H : W - l)] = Y[c * px + n];
+        const int n = t >> lc, c = t & (C - 1);
+        if (c < nc)
+            Sb[(size_t)n * W + c0 + c] = X[c * px + n];
+    }
+    if (c0 == 0)
+        for (int n = threadIdx.x; n < W; n += NT)
+            Sb[(size_t)n * W + H] = Y[n];
+}
+
+// The correlation's range half: from the stored columns l <= W/2 of T,
+// two rows per complex inverse transform (row n0 + 2q in the real part,
+// n0 + 2q + 1 in the imaginary part, each transform real), then |s| into
+// the fftshifted surface and this block's top two, as k_cpx_rowmag.
+__global__ void __launch_bounds__(NT) k_amp_rowmag(const float2 *T, int W, int R, float *mag, Top2 *part,
+                                                    stfft::PlanArgs Pi, stfft::FastDiv dW, const uint8_t *state)
+{
+    extern __shared__ float2 ldsx[];
+    __shared__ Top2 sht[NT / 64];
+    constexpr int IT = kStElems / NT;
+    const int H = W / 2;
+    const int b = blockIdx.y, n0 = blockIdx.x * R, nrow = imin(R, W - n0), npair = (nrow + 1) / 2;
+    if (state[b] != 2) {
+        if (threadIdx.x == 0)
+            part[(size_t)b * gridDim.x + blockIdx.x] = Top2{0.0f, 0x7fffffff, 0.0f};
+        return;
+    }
+    const float2 *Tb = T + ((size_t)b * W + n0) * W;
+    const int cnt = npair * W;
+    {
+        float2 v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            v[it] = make_float2(0.f, 0.f);
+            if (t < cnt) {
+                const int q = stfft::fdiv(t, dW), m = t - q * W, la = m <= H ? m : W - m;
+                float2 a = Tb[(size_t)(2 * q) * W + la];
+                float2 c = 2 * q + 1 < nrow ? Tb[(size_t)(2 * q + 1) * W + la] : make_float2(0.f, 0.f);
+                if (m > H) {
+                    a.y = -a.y;
+                    c.y = -c.y;
+                }
+                v[it] = make_float2(a.x - c.y, a.y + c.x);
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = threadIdx.x + it * NT;
+            if (t < cnt)
+                ldsx[t] = v[it];
         }
     }
+    __syncthreads();
+    stfft::lds_fft<kStElems, NT>(ldsx, W, npair, Pi, +1);
+    float *M = mag + (size_t)b * W * W;
+    Top2 best{0.0f, 0x7fffffff, 0.0f};
+    for (int t = threadIdx.x; t < nrow * W; t += NT) {
+        const int i = stfft::fdiv(t, dW), m = t - i * W, n = n0 + i;
+        const float2 z = ldsx[(i >> 1) * W + m];
+        const float v = fabsf((i & 1) ? z.y : z.x);
+        const int I = n < H ? n + H : n - H, J = m < H ? m + H : m - H;
+        M[(size_t)I * W + J] = v;
+        const int i0 = I < H ? I : I - H, j0 = J < H ? J : J - H;
+        const int q = (I < H) ? (J < H ? 0 : 3) : (J < H ? 2 : 1);
+        const int rank = (i0 * H + j0) * 4 + q;
+        if (v > 0.0f)
+            best = top2_combine(best, Top2{v, rank, 0.0f});
+        else
+            best.v2 = fmaxf(best.v2, v);
+    }
+    best = wg_top2<NT>(best, sht);
+    if (threadIdx.x == 0)
+        part[(size_t)b * gridDim.x + blockIdx.x] = best;
 }
 
 // The mean powers of ampMatch from the partials, in block order.
@@ -2695,8 +2758,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
             ctx->time_begin(kPeak);
             hipLaunchKernelGGL(k_cpx_colx, dim3((w + Cx - 1) / Cx, nb), dim3(NT), ldsX, st, C.P, nb, w, C.az, C.S, lcx,
                                PfW, PiW2, dW2);
-            hipLaunchKernelGGL(k_cpx_rowmag<true>, dim3(nPart, nb), dim3(NT), ldsR, st, C.S, w, Rr, C.mag, part, PiW2, dW2,
-                               nullptr);
+            hipLaunchKernelGGL(k_cpx_rowmag, dim3(nPart, nb), dim3(NT), ldsR, st, C.S, w, Rr, C.mag, part, PiW2, dW2);
             const GaussArg G{gauss ? 1 : 0, w, par.hanningFlag ? 1.5 : 1.0, 1.0, C.p};
             if (gauss) {
                 hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0, C.res,
@@ -2733,7 +2795,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         const int ws = par.wAa * (large ? LA : 1), W = 2 * ws;
         // blocking: Cx interpolation columns per k_amp_colx workgroup, Rd
         // rows per k_amp_rowdet / k_amp_rowfwd workgroup, Cc correlation
-        // columns per k_amp_colx2 workgroup, Rm rows per k_cpx_rowmag one
+        // columns per k_amp_colx2 workgroup, Rm rows per k_amp_rowmag one
         int lcx = 0, lcc = 0;
         while ((2 << lcx) <= 16 && (2 << lcx) <= ws && (2 << lcx) * W <= kStElems)
             lcx++;
@@ -2742,11 +2804,11 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         const int Cx = 1 << lcx, Cc = 1 << lcc;
         const int Rd = std::max(1, kStElems / (2 * W)), nPd = (W + Rd - 1) / Rd;
         const int Rf = kStElems / W, nPf = (W + Rf - 1) / Rf;
-        const int Rm = kStElems / W, nPm = (W + Rm - 1) / Rm;
+        const int Rm = 2 * (kStElems / W), nPm = (W + Rm - 1) / Rm;
         const size_t ldsX = (size_t)Cx * (Cx > 1 ? W + 1 : W) * sizeof(float2);
         const size_t ldsD = (size_t)2 * Rd * W * sizeof(float2), ldsF = (size_t)Rf * W * sizeof(float2);
         const size_t ldsC = (size_t)2 * Cc * (Cc > 1 ? W + 1 : W) * sizeof(float2);
-        const size_t ldsM = (size_t)Rm * W * sizeof(float2);
+        const size_t ldsM = (size_t)(Rm / 2) * W * sizeof(float2);
         const size_t per = (size_t)ws * ws * 128 + (size_t)nPd * (sizeof(AmpPart) + sizeof(double2)) +
                            (size_t)nPm * sizeof(Top2) + 2048;
         const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nA, (ar.cap - ar.used) / per, kMaxBatch}));
@@ -2782,8 +2844,8 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
             hipLaunchKernelGGL(k_amp_psum, dim3((nb + 127) / 128), dim3(128), 0, st, A.ppart, nPf, nb, W, A.state,
                                A.p);
             ctx->time_begin(kAmpPeak);
-            hipLaunchKernelGGL(k_cpx_rowmag<false>, dim3(nPm, nb), dim3(NT), ldsM, st, A.S, ws, Rm, A.mag, A.part, PiW,
-                               dW, A.state);
+            hipLaunchKernelGGL(k_amp_rowmag, dim3(nPm, nb), dim3(NT), ldsM, st, A.S, W, Rm, A.mag, A.part, PiW, dW,
+                               A.state);
             const GaussArg G{gauss ? 1 : 0, W / 2, 1.0, 16.0, A.p};
             if (gauss) {
                 hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, A.part, nPm, A.mag, W, 1, A.res, G);
