@@ -302,6 +302,9 @@ def main():
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--windows", default="", help="batched workload: only these window sizes (e.g. 16,32)")
     ap.add_argument("--chip", type=int, default=128, help="strackw chip size (128 or 256)")
+    ap.add_argument("--coherence", type=float, default=None,
+                    help="full scene with this uniform coherence instead of 0.3-0.9 (e.g. 0.05: every point "
+                         "falls back to amplitude matching)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
@@ -322,8 +325,11 @@ def main():
         bench_batched(args)
         return
 
-    from specklesource_amd import configs, track
+    from specklesource_amd import abi, configs, track
     spec, par = configs.cfg2(scale=args.scale)
+    if args.coherence is not None:
+        spec = abi.SynthSpec.from_buffer_copy(spec)
+        spec.gmin = spec.gmax = args.coherence
     dev = track.Device(0)
     t0 = time.perf_counter()
     pid = dev.synth_pair(spec)
@@ -386,7 +392,8 @@ def main():
         "dtype": "fp32 (fp64 sub-pixel refinement)",
         "data": "synthetic (generated in HBM)",
         "config": {"workload": "full-scene single pair, strack complex + amplitude, 64x64 windows, 24 px grid",
-                   "scene": [spec.nr, spec.na], "grid": [par.nR, par.nA], "window": par.wR},
+                   "scene": [spec.nr, spec.na], "grid": [par.nR, par.nA], "window": par.wR,
+                   "coherence": [spec.gmin, spec.gmax]},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "work": {"patch_correlations_per_job": ncorr // args.steps, "complex": int(ncpx),
                  "amp_small": int(st.nCorr[1]), "amp_large": int(st.nCorr[2]),
