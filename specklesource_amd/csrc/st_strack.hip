@@ -827,6 +827,7 @@ struct AmpBatch {
     double *p;
     AmpPart *dpart;  // detection partials [B][nPd]
     double2 *ppart;  // power partials [B][nPd]
+    double2 *mean;   // amplitude means [B]
     Top2 *part;      // coarse top two [B][nPm]
 };
 
@@ -1336,8 +1337,9 @@ __global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, 
 //   k_amp_rowdet  range pad, inverse range transforms, detection |z|^2,
 //                 amplitude, partial sums of the zero test and the means
 //                                                          -> amplitudes [2][B][W][W]
-//   k_amp_rowfwd  zero test, mean removal, partial powers, forward range
-//                 transforms of v1 + i v2                  -> Z [B][W][W]
+//   k_amp_stat    zero test and means from the partials (one wave per patch)
+//   k_amp_rowfwd  mean removal, partial powers, forward range transforms
+//                 of v1 + i v2                             -> Z [B][W][W]
 //   k_amp_colx2   forward azimuth transforms, separation of the two
 //                 spectra, F1 conj(F2), inverse azimuth transforms of the
 //                 columns l <= W/2 (the correlation is real)
@@ -1509,45 +1511,54 @@ __global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int 
         part[(size_t)b * gridDim.x + blk] = AmpPart{s1, s2, m1, m2, nz, 0};
 }
 
-// The zero test and mean removal of ampMatch (speckleTrack.c:437-467) and
-// the forward range transforms of the correlation: every workgroup of a
-// patch sums the detection partials in the same order; a rejected patch
-// is marked "no data" (state 1) by its first workgroup.  Both images are
-// real, so one complex transform of v1 + i v2 carries both spectra
-// (k_amp_colx2 separates them); out Z [B][W][W].
-__global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int W, int R, int nPd,
-                                                    const AmpPart *dpart, uint8_t *state, float2 *Z, double2 *ppart,
+// The zero test and the means of ampMatch (speckleTrack.c:437-467) from
+// the detection partials: one wave per patch, lanes over the partials, then
+// a fixed-order wave sum; a rejected patch is marked "no data" (state 1).
+__global__ void __launch_bounds__(64) k_amp_stat(const AmpPart *dpart, int nPd, int W, uint8_t *state, double2 *mean)
+{
+    const int b = blockIdx.x, l = threadIdx.x;
+    if (state[b] != 2)
+        return;
+    double s1 = 0, s2 = 0, m1 = 0, m2 = 0;
+    int nz = 0;
+    for (int k = l; k < nPd; k += 64) {
+        const AmpPart q = dpart[(size_t)b * nPd + k];
+        s1 += q.s1;
+        s2 += q.s2;
+        m1 += q.m1;
+        m2 += q.m2;
+        nz += q.nz;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
+        m1 += __shfl_xor(m1, o);
+        m2 += __shfl_xor(m2, o);
+        nz += __shfl_xor(nz, o);
+    }
+    if (l == 0) {
+        if (nz > 20 || s1 < 1.0e-6 || s2 < 1.0e-6)
+            state[b] = 1;
+        else
+            mean[b] = make_double2(m1 / (double)((size_t)W * W), m2 / (double)((size_t)W * W));
+    }
+}
+
+// Mean removal (ampMatch's rules) and the forward range transforms of the
+// correlation.  Both images are real, so one complex transform of
+// v1 + i v2 carries both spectra (k_amp_colx2 separates them); out
+// Z [B][W][W] and this block's partial powers.
+__global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int W, int R, const double2 *mean,
+                                                    const uint8_t *state, float2 *Z, double2 *ppart,
                                                     stfft::PlanArgs Pf)
 {
     extern __shared__ float2 ldsx[];
     __shared__ double sh[NT / 64];
-    __shared__ double shm[2];
-    __shared__ int shrej;
     int b, blk;
     xcd_item_block(gridDim.x, b, blk);
     if (state[b] != 2)
         return;
-    if (threadIdx.x == 0) {
-        const AmpPart *q = dpart + (size_t)b * nPd;
-        AmpPart a = q[0];
-        for (int k = 1; k < nPd; k++) {
-            a.s1 += q[k].s1;
-            a.s2 += q[k].s2;
-            a.m1 += q[k].m1;
-            a.m2 += q[k].m2;
-            a.nz += q[k].nz;
-        }
-        shrej = a.nz > 20 || a.s1 < 1.0e-6 || a.s2 < 1.0e-6;
-        shm[0] = a.m1 / (double)((size_t)W * W);
-        shm[1] = a.m2 / (double)((size_t)W * W);
-    }
-    __syncthreads();
-    if (shrej) {
-        if (blk == 0 && threadIdx.x == 0)
-            state[b] = 1;
-        return;
-    }
-    const double m1 = shm[0], m2 = shm[1];
+    const double m1 = mean[b].x, m2 = mean[b].y;
     const int n0 = blk * R, nrow = imin(R, W - n0), cnt = nrow * W;
     float2 *X = ldsx;
     const float *A1 = amp + ((size_t)b * W + n0) * W, *A2 = amp + (((size_t)B + b) * W + n0) * W;
@@ -2824,6 +2835,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         A.p = ar.take<double>(2 * (size_t)B);
         A.dpart = ar.take<AmpPart>((size_t)B * nPd);
         A.ppart = ar.take<double2>((size_t)B * nPd);
+        A.mean = ar.take<double2>(B);
         A.part = ar.take<Top2>((size_t)B * nPm);
         const stfft::PlanArgs Pfs = ctx->fft.plan(ws, -1), PfW = ctx->fft.plan(W, -1), PiW = ctx->fft.plan(W, +1);
         const stfft::FastDiv dW = stfft::make_fastdiv((uint32_t)W);
@@ -2837,8 +2849,9 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
             hipLaunchKernelGGL(k_amp_rowdet, dim3(nPd, nb), dim3(NT), ldsD, st, P, A.F, nb, ws, Rd, A.state,
                                (float *)A.S, A.dpart, PiW, dW);
             ctx->time_end(kAmpDet);
-            hipLaunchKernelGGL(k_amp_rowfwd, dim3(nPf, nb), dim3(NT), ldsF, st, (const float *)A.S, nb, W, Rf, nPd,
-                               A.dpart, A.state, A.F, A.ppart, PfW);
+            hipLaunchKernelGGL(k_amp_stat, dim3(nb), dim3(64), 0, st, A.dpart, nPd, W, A.state, A.mean);
+            hipLaunchKernelGGL(k_amp_rowfwd, dim3(nPf, nb), dim3(NT), ldsF, st, (const float *)A.S, nb, W, Rf, A.mean,
+                               A.state, A.F, A.ppart, PfW);
             hipLaunchKernelGGL(k_amp_colx2, dim3((ws + Cc - 1) / Cc, nb), dim3(NT), ldsC, st, A.F, W, A.state, A.S,
                                lcc, PfW, PiW, dW);
             hipLaunchKernelGGL(k_amp_psum, dim3((nb + 127) / 128), dim3(128), 0, st, A.ppart, nPf, nb, W, A.state,
