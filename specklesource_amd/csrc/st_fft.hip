@@ -43,8 +43,24 @@ __global__ void __launch_bounds__(kThreads) k_fft_rows(const float2 *src, float2
     const int nrow = (int)((nrows - row0) < R ? (nrows - row0) : R);
     float2 *A = lds, *B = lds + R * n;
     const long base = row0 * n;
-    for (int t = threadIdx.x; t < nrow * n; t += blockDim.x)
-        A[t] = src[base + t];
+    // per chunk of kPointsPerBlock points, all of a thread's loads are
+    // issued before its first LDS store
+    constexpr int IT = kPointsPerBlock / kThreads;
+    const int cnt = nrow * n;
+    for (int t0 = 0; t0 < cnt; t0 += kPointsPerBlock) {
+        float2 u[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = t0 + threadIdx.x + it * kThreads;
+            u[it] = t < cnt ? src[base + t] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = t0 + threadIdx.x + it * kThreads;
+            if (t < cnt)
+                A[t] = u[it];
+        }
+    }
     __syncthreads();
     for (int s = 0; s < P.nst; s++) {
         const int p = P.radix[s], ns = P.ns[s];

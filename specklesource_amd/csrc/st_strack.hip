@@ -1562,16 +1562,30 @@ __global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int 
     const int n0 = blk * R, nrow = imin(R, W - n0), cnt = nrow * W;
     float2 *X = ldsx;
     const float *A1 = amp + ((size_t)b * W + n0) * W, *A2 = amp + (((size_t)B + b) * W + n0) * W;
+    constexpr int IT = kStElems / NT;
     double p1 = 0, p2 = 0;
-    for (int t = threadIdx.x; t < cnt; t += NT) {
+    for (int t0 = 0; t0 < cnt; t0 += kStElems) {
+        float u[IT], v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int t = t0 + threadIdx.x + it * NT;
+            u[it] = t < cnt ? A1[t] : 0.f;
+            v[it] = t < cnt ? A2[t] : 0.f;
+        }
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
 #pragma clang fp contract(off)
-        float v1 = A1[t], v2 = A2[t];
-        if (v1 > 1.0e-20)
-            v1 = (float)(v1 - m1);
-        v2 = (float)(v2 - m2);
-        p1 += v1 * v1;
-        p2 += v2 * v2;
-        X[t] = make_float2(v1, v2);
+            const int t = t0 + threadIdx.x + it * NT;
+            if (t < cnt) {
+                float v1 = u[it], v2 = v[it];
+                if (v1 > 1.0e-20)
+                    v1 = (float)(v1 - m1);
+                v2 = (float)(v2 - m2);
+                p1 += v1 * v1;
+                p2 += v2 * v2;
+                X[t] = make_float2(v1, v2);
+            }
+        }
     }
     p1 = block_sum_d<NT>(p1, sh);
     p2 = block_sum_d<NT>(p2, sh);
