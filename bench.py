@@ -296,7 +296,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=1, help="divide the scene sides (testing only)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["fullscene", "strackw", "cull", "batched"], default="fullscene")
+    ap.add_argument("--workload", choices=["fullscene", "amponly", "strackw", "cull", "batched"], default="fullscene")
     ap.add_argument("--precision", choices=["fp32", "mfma"], default="fp32")
     ap.add_argument("--pairs", type=int, default=64)
     ap.add_argument("--slots", type=int, default=4)
@@ -326,7 +326,8 @@ def main():
         return
 
     from specklesource_amd import abi, configs, track
-    spec, par = configs.cfg2(scale=args.scale)
+    amponly = args.workload == "amponly"
+    spec, par = configs.cfg1(scale=args.scale) if amponly else configs.cfg2(scale=args.scale)
     if args.coherence is not None:
         spec = abi.SynthSpec.from_buffer_copy(spec)
         spec.gmin = spec.gmax = args.coherence
@@ -369,7 +370,13 @@ def main():
     # algorithmic flops per launch are F_c(w) x complex correlations per job,
     # over its average launch duration (HIP events on the library's stream)
     dom = max(kern.items(), key=lambda kv: kv[1][0]) if kern else ("none", (0, 0))
-    if "k_cpx_fused" in kern and kern["k_cpx_fused"][1] > 0:
+    if amponly:
+        # amplitude-only job: the whole amplitude stage (k_amp_* and the
+        # transforms) against the algorithmic flops of its tries
+        kname = "amplitude stage (k_amp_*)"
+        launch_ms = (stages[2] + stages[3]) / args.steps
+        per_launch = flops_amp_patch(par.wAa) * st.nCorr[1] + flops_amp_patch(2 * par.wAa) * st.nCorr[2]
+    elif "k_cpx_fused" in kern and kern["k_cpx_fused"][1] > 0:
         kname = "k_cpx_fused"
         launch_ms = kern[kname][0] / kern[kname][1]
         per_launch = flops_complex_patch(par.wR) * ncpx * args.steps / kern[kname][1]
@@ -391,7 +398,9 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 (fp64 sub-pixel refinement)",
         "data": "synthetic (generated in HBM)",
-        "config": {"workload": "full-scene single pair, strack complex + amplitude, 64x64 windows, 24 px grid",
+        "config": {"workload": ("amplitude-only single pair (-noComplex), 32x32 windows, 64x64 retries, 16 px grid"
+                                if amponly else
+                                "full-scene single pair, strack complex + amplitude, 64x64 windows, 24 px grid"),
                    "scene": [spec.nr, spec.na], "grid": [par.nR, par.nA], "window": par.wR,
                    "coherence": [spec.gmin, spec.gmax]},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
@@ -402,9 +411,10 @@ def main():
                      "amp_small": stages[2] / args.steps, "amp_large": stages[3] / args.steps},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(),
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None if amponly else measured_traffic(),
                      "kernel": kname, "launch_ms": launch_ms, "flops_per_launch": per_launch,
-                     "algorithmic_bytes_per_launch": 2 * 8 * par.wR * par.wR * ncpx,
+                     "algorithmic_bytes_per_launch": (2 * 8 * par.wAa * par.wAa * (st.nCorr[1] + 4 * st.nCorr[2])
+                                                      if amponly else 2 * 8 * par.wR * par.wR * ncpx),
                      "dominant_kernel": dom[0]},
     }
     if not args.no_cpu:

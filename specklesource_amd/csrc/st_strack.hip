@@ -2886,8 +2886,12 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
             ST_HIP(hipGetLastError());
         }
         ar.used = mark;
-        ST_HIP(hipEventRecord(sev[3 + large], st));
+        if (!large)
+            ST_HIP(hipEventRecord(sev[3], st));
     }
+    // the doubled-window stage ends here also when it did not run (no
+    // failed small tries, or one try allowed), so its time reads zero
+    ST_HIP(hipEventRecord(sev[4], st));
 
     hipLaunchKernelGGL(k_compose, dim3((npts + 255) / 256), dim3(256), 0, st, P, S, dOffR, dOffA, npts);
     ST_HIP(hipGetLastError());
