@@ -2822,9 +2822,9 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         // rows per k_amp_rowdet / k_amp_rowfwd workgroup, Cc correlation
         // columns per k_amp_colx2 workgroup, Rm rows per k_amp_rowmag one
         int lcx = 0, lcc = 0;
-        while ((2 << lcx) <= 16 && (2 << lcx) <= ws && (2 << lcx) * W <= kStElems)
+        while ((2 << lcx) <= ws && (2 << lcx) * W <= kStElems)
             lcx++;
-        while ((2 << lcc) <= 16 && (2 << lcc) <= ws && (2 << lcc) * W * 2 <= kStElems)
+        while ((2 << lcc) <= ws && (2 << lcc) * W * 2 <= kStElems)
             lcc++;
         const int Cx = 1 << lcx, Cc = 1 << lcc;
         const int Rd = std::max(1, kStElems / (2 * W)), nPd = (W + Rd - 1) / Rd;
