@@ -1445,7 +1445,7 @@ __global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int 
                                                     stfft::PlanArgs Pi, stfft::FastDiv dW)
 {
     extern __shared__ float2 ldsx[];
-    __shared__ double sh[NT / 64];
+    __shared__ double sh[4 * (NT / 64)];
     __shared__ int shi[NT / 64];
     constexpr int IT = kStElems / NT;
     const int W = 2 * ws, h = ws / 2;
@@ -1502,13 +1502,35 @@ __global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int 
         A1[t] = a1;
         A2[t] = a2;
     }
-    nz = block_sum_i<NT>(nz, shi);
-    s1 = block_sum_d<NT>(s1, sh);
-    s2 = block_sum_d<NT>(s2, sh);
-    m1 = block_sum_d<NT>(m1, sh);
-    m2 = block_sum_d<NT>(m2, sh);
-    if (threadIdx.x == 0)
-        part[(size_t)b * gridDim.x + blk] = AmpPart{s1, s2, m1, m2, nz, 0};
+    // the five sums in one reduction: wave sums, then the first thread over
+    // the waves (one barrier)
+    for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
+        m1 += __shfl_xor(m1, o);
+        m2 += __shfl_xor(m2, o);
+        nz += __shfl_xor(nz, o);
+    }
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[4 * wid] = s1;
+        sh[4 * wid + 1] = s2;
+        sh[4 * wid + 2] = m1;
+        sh[4 * wid + 3] = m2;
+        shi[wid] = nz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        AmpPart a{sh[0], sh[1], sh[2], sh[3], shi[0], 0};
+        for (int w = 1; w < NT / 64; w++) {
+            a.s1 += sh[4 * w];
+            a.s2 += sh[4 * w + 1];
+            a.m1 += sh[4 * w + 2];
+            a.m2 += sh[4 * w + 3];
+            a.nz += shi[w];
+        }
+        part[(size_t)b * gridDim.x + blk] = a;
+    }
 }
 
 // The zero test and the means of ampMatch (speckleTrack.c:437-467) from
@@ -1553,7 +1575,7 @@ __global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int 
                                                     stfft::PlanArgs Pf)
 {
     extern __shared__ float2 ldsx[];
-    __shared__ double sh[NT / 64];
+    __shared__ double sh[2 * (NT / 64)];
     int b, blk;
     xcd_item_block(gridDim.x, b, blk);
     if (state[b] != 2)
@@ -1587,11 +1609,26 @@ __global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int 
             }
         }
     }
-    p1 = block_sum_d<NT>(p1, sh);
-    p2 = block_sum_d<NT>(p2, sh);
-    if (threadIdx.x == 0)
-        ppart[(size_t)b * gridDim.x + blk] = make_double2(p1, p2);
+    // partial powers: wave sums, then the first thread over the waves; the
+    // one barrier also orders the LDS stores before the transform
+    for (int o = 32; o > 0; o >>= 1) {
+        p1 += __shfl_xor(p1, o);
+        p2 += __shfl_xor(p2, o);
+    }
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[2 * wid] = p1;
+        sh[2 * wid + 1] = p2;
+    }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        double q1 = sh[0], q2 = sh[1];
+        for (int w = 1; w < NT / 64; w++) {
+            q1 += sh[2 * w];
+            q2 += sh[2 * w + 1];
+        }
+        ppart[(size_t)b * gridDim.x + blk] = make_double2(q1, q2);
+    }
     stfft::lds_fft<kStElems, NT>(X, W, nrow, Pf, -1);
     float2 *Zb = Z + ((size_t)b * W + n0) * W;
     for (int t = threadIdx.x; t < cnt; t += NT)
