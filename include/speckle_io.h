@@ -129,6 +129,43 @@ int st_read_mask(const char *maskFile, int32_t nr, int32_t na, uint8_t *mask);
  * the interferogram's geodat file, which the caller reads. */
 int st_read_interferogram(const char *intFile, int32_t nr, int32_t na, float *intf);
 
+/* parseTrack (Strack/parseTrack.c:13-190): the strack parameter file, an
+ * RDF keyword file with one "keyword (units) = value ! comment" per line
+ * (units and comment optional).  The reference reads it through GrIMP's
+ * rdfSource, which is not in the reference tree, so the line rules here
+ * are parity unpinned: keywords compare without regard to case, the first
+ * occurrence of a keyword wins, and blank or comment-only lines are
+ * skipped.  The keywords, their %i formats, which are required, the
+ * defaults and the quirks follow parseTrack.c:
+ *   - edgeR / edgeA override edgepad; an unreadable edgeA resets edgePadR
+ *     (parseTrack.c:155-159);
+ *   - rStart, aStart, deltaR, deltaA are divided by scalefactor (C integer
+ *     division, parseTrack.c:165-168);
+ *   - offsetmaskfile without offsetmaskdat leaves maskGeodat empty
+ *     (parseTrack.c:73-77);
+ *   - intfile and baseparams each require intgeodat; noComplex (the
+ *     command-line flag) clears baseFlag (parseTrack.c:89-109).
+ * An image with a VRT beside it ("<image>.vrt", checkForVrt in the
+ * un-vendored clib: parity unpinned) needs no image1par / image2par.
+ * Missing required keywords give ST_ERR_ARG with parseTrack's message. */
+#define ST_PAR_PATH 1024
+#define ST_GEODATMASK 0 /* strack.h:5 */
+#define ST_OFFSETMASK 1 /* strack.h:6 */
+typedef struct st_track_par_file {
+    char image1[ST_PAR_PATH], image2[ST_PAR_PATH];
+    char par1[ST_PAR_PATH], par2[ST_PAR_PATH]; /* image1par / image2par ("" with a VRT) */
+    char vrt1[ST_PAR_PATH], vrt2[ST_PAR_PATH]; /* "" without a VRT */
+    char initShift[ST_PAR_PATH];               /* initial-offset grids (polyShift = 0) */
+    char initialOffsetFile[ST_PAR_PATH];       /* offset polynomials (polyShift = 1) */
+    char outputFile[ST_PAR_PATH];              /* base of the .dr .da .cc .mt .dat .azd .vrt .mt.vrt products */
+    char maskFile[ST_PAR_PATH], maskGeodat[ST_PAR_PATH], maskVrt[ST_PAR_PATH];
+    char intFile[ST_PAR_PATH], intGeodat[ST_PAR_PATH], baseParams[ST_PAR_PATH];
+    int32_t polyShift, maskFlag, maskType, intFlag, baseFlag;
+    int32_t rStart, aStart, deltaR, deltaA, nR, nA, wR, wA, wRa, wAa;
+    int32_t edgePad, edgePadR, edgePadA, scaleFactor, navgR, navgA;
+} st_track_par_file;
+int st_read_track_par(const char *parFile, int32_t noComplex, st_track_par_file *out);
+
 /* ---- GDAL VRT inputs (raw bands) -------------------------------------- */
 /* The VRT files GrIMP reads through GDAL (GDALOpen / GDALRasterIO /
  * readDataSetMetaData) where the bands are VRTRawRasterBand, as its own

@@ -6,6 +6,7 @@
 // (Strack/sTrackOut.c:11-15).  Text headers use the reference's printf
 // formats; the VRT files are GDAL VRTRawRasterBand descriptions with the
 // metadata items the reference records, in the order it inserts them.
+#include <ctype.h>
 #include <libgen.h>
 #include <math.h>
 #include <stdarg.h>
@@ -212,6 +213,249 @@ void check_grid(int32_t nR, int32_t nA, int32_t order)
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------
+// strack parameter file (parseTrack, Strack/parseTrack.c:13-190)
+
+namespace {
+
+struct RdfKeys {
+    std::vector<std::pair<std::string, std::string>> kv;  // lower-case keyword, value
+    const char *value(const char *key) const
+    {
+        std::string k(key);
+        for (auto &c : k)
+            c = (char)tolower((unsigned char)c);
+        for (const auto &p : kv)
+            if (p.first == k)
+                return p.second.c_str();
+        return nullptr;
+    }
+};
+
+std::string trim(const std::string &x)
+{
+    size_t a = 0, b = x.size();
+    while (a < b && isspace((unsigned char)x[a]))
+        a++;
+    while (b > a && isspace((unsigned char)x[b - 1]))
+        b--;
+    return x.substr(a, b - a);
+}
+
+// "keyword (units) = value ! comment" lines; others are skipped
+RdfKeys rdf_read(const char *path)
+{
+    FILE *f = fopen(path, "r");
+    if (!f)
+        fail(ST_ERR_IO, fmt("parseTrack: cannot open %s", path));
+    RdfKeys r;
+    char line[8192];
+    while (fgets(line, sizeof line, f)) {
+        std::string l(line);
+        const size_t bang = l.find('!');
+        if (bang != std::string::npos)
+            l.resize(bang);
+        const size_t eq = l.find('=');
+        if (eq == std::string::npos)
+            continue;
+        std::string key = l.substr(0, eq);
+        const size_t par = key.find('(');
+        if (par != std::string::npos)
+            key.resize(par);
+        key = trim(key);
+        if (key.empty())
+            continue;
+        for (auto &c : key)
+            c = (char)tolower((unsigned char)c);
+        r.kv.emplace_back(key, trim(l.substr(eq + 1)));
+    }
+    fclose(f);
+    return r;
+}
+
+void put(char *dst, const char *src, const char *what)
+{
+    if (!src) {
+        dst[0] = '\0';
+        return;
+    }
+    if (strlen(src) >= ST_PAR_PATH)
+        fail(ST_ERR_ARG, fmt("parseTrack: %s longer than %d characters", what, ST_PAR_PATH - 1));
+    strcpy(dst, src);
+}
+
+bool file_exists(const std::string &p)
+{
+    FILE *f = fopen(p.c_str(), "rb");
+    if (!f)
+        return false;
+    fclose(f);
+    return true;
+}
+
+// the VRT of an image: the file itself when it is one, else "<image>.vrt"
+bool image_vrt(const char *image, std::string &vrt)
+{
+    const std::string im(image);
+    if (im.size() > 4 && im.compare(im.size() - 4, 4, ".vrt") == 0 && file_exists(im)) {
+        vrt = im;
+        return true;
+    }
+    if (file_exists(im + ".vrt")) {
+        vrt = im + ".vrt";
+        return true;
+    }
+    return false;
+}
+
+// parseValues (parseTrack.c:245-254) for one %i value
+void int_value(const RdfKeys &k, const char *key, int32_t *dst)
+{
+    const char *v = k.value(key);
+    if (!v)
+        fail(ST_ERR_ARG, fmt("%s not specified", key));
+    int x;
+    if (sscanf(v, "%i", &x) != 1)
+        fail(ST_ERR_ARG, fmt("Invalid %s", key));
+    *dst = x;
+}
+
+// an optional %i value: kept when absent, `fallback` when unreadable
+void opt_int(const RdfKeys &k, const char *key, int32_t *dst, int32_t fallback)
+{
+    const char *v = k.value(key);
+    int x;
+    if (v && sscanf(v, "%i", &x) == 1)
+        *dst = x;
+    else if (v)
+        *dst = fallback;
+}
+
+}  // namespace
+
+extern "C" int st_read_track_par(const char *parFile, int32_t noComplex, st_track_par_file *out)
+{
+    return io_guard([&] {
+        if (!parFile || !out)
+            fail(ST_ERR_ARG, "st_read_track_par: null argument");
+        const RdfKeys k = rdf_read(parFile);
+        st_track_par_file o;
+        memset(&o, 0, sizeof o);
+        std::string vrt;
+        // images, their par files or VRTs (parseTrack.c:26-52)
+        const char *im1 = k.value("image1");
+        if (!im1)
+            fail(ST_ERR_ARG, "parseTrack: Missing image1");
+        put(o.image1, im1, "image1");
+        if (image_vrt(im1, vrt)) {
+            put(o.vrt1, vrt.c_str(), "image1 vrt");
+        } else {
+            const char *p1 = k.value("image1par");
+            if (!p1)
+                fail(ST_ERR_ARG, "parseTrack: Missing image1par and no vrt");
+            put(o.par1, p1, "image1par");
+        }
+        const char *im2 = k.value("image2");
+        if (!im2)
+            fail(ST_ERR_ARG, "parseTrack: Missing image2");
+        put(o.image2, im2, "image2");
+        if (image_vrt(im2, vrt)) {
+            put(o.vrt2, vrt.c_str(), "image2 vrt");
+        } else {
+            const char *p2 = k.value("image2par");
+            if (!p2)
+                fail(ST_ERR_ARG, "parseTrack: Missing image2par and no vrt");
+            put(o.par2, p2, "image2par");
+        }
+        // mask (parseTrack.c:55-86)
+        if (const char *m = k.value("maskfile")) {
+            o.maskFlag = 1;
+            o.maskType = ST_GEODATMASK;
+            put(o.maskFile, m, "maskfile");
+            const char *g = k.value("maskgeodat");
+            if (!g)
+                fail(ST_ERR_ARG, "parseTrack: Missing maskgeodat");
+            put(o.maskGeodat, g, "maskgeodat");
+        } else {
+            if (const char *om = k.value("offsetmaskfile")) {
+                o.maskFlag = 1;
+                o.maskType = ST_OFFSETMASK;
+                put(o.maskFile, om, "offsetmaskfile");
+                put(o.maskGeodat, k.value("offsetmaskdat"), "offsetmaskdat");
+            }
+            put(o.maskVrt, k.value("offsetmaskvrt"), "offsetmaskvrt");
+        }
+        // interferogram and baseline (parseTrack.c:89-109)
+        if (const char *in = k.value("intfile")) {
+            o.intFlag = 1;
+            put(o.intFile, in, "intfile");
+            const char *g = k.value("intgeodat");
+            if (!g)
+                fail(ST_ERR_ARG, "parseTrack: Missing intgeodat");
+            put(o.intGeodat, g, "intgeodat");
+        }
+        if (const char *bp = k.value("baseparams")) {
+            o.baseFlag = 1;
+            put(o.baseParams, bp, "baseparams");
+            const char *g = k.value("intgeodat");
+            if (!g)
+                fail(ST_ERR_ARG, "parseTrack: Missing intgeodat");
+            put(o.intGeodat, g, "intgeodat");
+        }
+        if (noComplex)
+            o.baseFlag = 0;
+        // initial offsets (parseTrack.c:111-124)
+        if (const char *is = k.value("initShift")) {
+            o.polyShift = 0;
+            put(o.initShift, is, "initShift");
+        } else {
+            o.polyShift = 1;
+            const char *io = k.value("initialoffsetfile");
+            if (!io)
+                fail(ST_ERR_ARG, "parseTrack: Missing initialoffsetfile");
+            put(o.initialOffsetFile, io, "initialoffsetfile");
+        }
+        const char *of = k.value("outputfile");
+        if (!of)
+            fail(ST_ERR_ARG, "parseTrack: missing ouput file");
+        put(o.outputFile, of, "outputfile");
+        // grid and windows (parseTrack.c:160-199)
+        int_value(k, "rstart", &o.rStart);
+        int_value(k, "astart", &o.aStart);
+        int_value(k, "deltar", &o.deltaR);
+        int_value(k, "deltaa", &o.deltaA);
+        int_value(k, "nR", &o.nR);
+        int_value(k, "nA", &o.nA);
+        int_value(k, "wr", &o.wR);
+        int_value(k, "wa", &o.wA);
+        int_value(k, "wra", &o.wRa);
+        int_value(k, "waa", &o.wAa);
+        o.edgePad = 0;
+        opt_int(k, "edgepad", &o.edgePad, 0);
+        o.edgePadR = o.edgePad;
+        o.edgePadA = o.edgePad;
+        o.scaleFactor = 1;
+        opt_int(k, "scalefactor", &o.scaleFactor, 1);
+        opt_int(k, "edgeR", &o.edgePadR, o.edgePad);
+        if (const char *ea = k.value("edgeA")) {
+            int x;
+            if (sscanf(ea, "%i", &x) == 1)
+                o.edgePadA = x;
+            else
+                o.edgePadR = o.edgePad;  // the reference resets the range pad here
+        }
+        int_value(k, "navgR", &o.navgR);
+        int_value(k, "navgA", &o.navgA);
+        if (o.scaleFactor == 0)
+            fail(ST_ERR_ARG, "parseTrack: scalefactor 0");
+        o.rStart /= o.scaleFactor;
+        o.aStart /= o.scaleFactor;
+        o.deltaR /= o.scaleFactor;
+        o.deltaA /= o.scaleFactor;
+        *out = o;
+    });
+}
 
 extern "C" const char *st_io_last_error(void) { return g_err.c_str(); }
 

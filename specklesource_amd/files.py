@@ -33,6 +33,19 @@ class CullLSMeta(C.Structure):
                 ("noData", C.c_float)]
 
 
+_PAR_PATH = 1024
+_PAR_STRINGS = ("image1", "image2", "par1", "par2", "vrt1", "vrt2", "initShift", "initialOffsetFile", "outputFile",
+                "maskFile", "maskGeodat", "maskVrt", "intFile", "intGeodat", "baseParams")
+_PAR_INTS = ("polyShift", "maskFlag", "maskType", "intFlag", "baseFlag", "rStart", "aStart", "deltaR", "deltaA", "nR",
+             "nA", "wR", "wA", "wRa", "wAa", "edgePad", "edgePadR", "edgePadA", "scaleFactor", "navgR", "navgA")
+
+
+class TrackParFile(C.Structure):
+    """st_track_par_file: what parseTrack (Strack/parseTrack.c) takes from
+    the strack parameter file."""
+    _fields_ = [(n, C.c_char * _PAR_PATH) for n in _PAR_STRINGS] + [(n, C.c_int32) for n in _PAR_INTS]
+
+
 _bound = False
 
 
@@ -148,7 +161,33 @@ def _bind_inputs(lib):
     lib.st_read_mask_header.argtypes = [C.c_char_p, i32p]
     lib.st_read_mask.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_void_p]
     lib.st_read_interferogram.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_void_p]
+    lib.st_read_track_par.argtypes = [C.c_char_p, C.c_int32, C.POINTER(TrackParFile)]
     lib._st_inputs_bound = True
+
+
+def read_track_par(path, no_complex=False):
+    """The strack parameter file (parseTrack, Strack/parseTrack.c:13-190) as
+    a dict: file names as str ("" when absent), flags and grid / window
+    values as int (rStart, aStart, deltaR, deltaA already divided by
+    scaleFactor).  Raises IoError with parseTrack's message when a required
+    keyword is missing or a value does not parse."""
+    lib = _lib()
+    _bind_inputs(lib)
+    o = TrackParFile()
+    _check(lib.st_read_track_par(path.encode(), 1 if no_complex else 0, C.byref(o)))
+    d = {n: getattr(o, n).decode() for n in _PAR_STRINGS}
+    d.update({n: getattr(o, n) for n in _PAR_INTS})
+    return d
+
+
+def track_params_from_par(d, **kw):
+    """st_track_params from read_track_par's dict (grid, windows, edge pads,
+    scale factor, polyShift); the rest from track.default_params / kw."""
+    keys = ("rStart", "aStart", "deltaR", "deltaA", "nR", "nA", "wR", "wA", "wRa", "wAa", "edgePadR", "edgePadA",
+            "scaleFactor", "polyShift")
+    base = {k: d[k] for k in keys}
+    base.update(kw)
+    return track.default_params(**base)
 
 
 def read_offset_polys(path):
