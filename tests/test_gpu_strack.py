@@ -226,3 +226,16 @@ def test_parity_window_128(device, oracle32, oracle64):
     g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
     assert (g.type == abi.ST_CMATCH).mean() > 0.5
     _check(rep, probs)
+
+
+def test_parity_decorrelated_amplitude_retries(device, oracle32, oracle64):
+    """Low coherence at w = 64: most complex matches fail, so the amplitude
+    tries at W = 128 and the doubled-window tries at W = 256 carry the job
+    (the separable interpolation, packed forward transform, conjugate
+    separation and paired-row inverse of the amplitude path)."""
+    s = configs._spec(720, 720, 91, dr0=0.7, drR=1e-3, da0=-0.4, daA=1e-3, gmin=0.12, gmax=0.2)
+    p = small_params(64, 720, 720, step=32, rShiftPoly=[0.7, 1e-3, 0], aShiftPoly=[-0.4, 0, 1e-3])
+    a, b = track.synth_host(s)
+    g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
+    assert g.nAmp > 30 and g.nAmpL > 60
+    _check(rep, probs)
