@@ -1496,7 +1496,10 @@ __global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int 
         nz += (x1 < thr || x2 < thr);
         s1 += x1;
         s2 += x2;
-        const float a1 = (float)sqrt((double)x1), a2 = (float)sqrt((double)x2);
+        // the reference's sqrt in double, rounded to float: sqrtf gives the same
+        // bits for every non-negative finite float (all of them checked on
+        // gfx950), without the FP64 square-root sequence
+        const float a1 = sqrtf(x1), a2 = sqrtf(x2);
         m1 += a1;
         m2 += a2;
         A1[t] = a1;

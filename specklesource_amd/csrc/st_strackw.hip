@@ -368,9 +368,11 @@ __global__ void __launch_bounds__(256) k_w_detect(WPar p, WBatch Bt, int B)
             for (int m = -hA; m <= hA; m++)
                 for (int n = -hR; n <= hR; n++)
                     v2 += d2[(oy + hA + m) * TW + ox + hR + n];
-            s2 = (float)sqrt((double)v2);
+            s2 = sqrtf(v2);
         }
-        const float sr = (float)sqrt((double)v1);
+        // sqrtf: the double square root rounded to float, bit for bit, for
+        // every non-negative finite float (all of them checked on gfx950)
+        const float sr = sqrtf(v1);
         sv[oy][ox] = sr;
         Bt.Z[(size_t)b * NN + (size_t)y * p.W_R + x] = make_float2(sr, s2);
     }
@@ -619,7 +621,7 @@ __global__ void __launch_bounds__(256) k_w_ncc(WPar p, WBatch Bt, int nRun, int 
                     const int c = s - WOS * p.epR;
                     const float2 xv = X[(c + p.W_R) % p.W_R];
                     const float v = odd ? xv.y : xv.x;
-                    const float mag = (float)sqrt((double)(v * v));
+                    const float mag = sqrtf(v * v);  // = (float)sqrt((double)(v * v)), bit for bit
                     double cr = (double)mag * p.scaleW;
                     cr = cr - meanR * m;
                     cr = cr / sqrt(sigmaR * sig);
