@@ -1454,7 +1454,9 @@ __global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int 
     if (state[b] != 2)
         return;
     const int n0 = blk * R, nrow = imin(R, W - n0), cnt = nrow * W;
-    float2 *X1 = ldsx, *X2 = ldsx + (size_t)R * W;
+    // the second image's rows follow the first's, so one transform can take
+    // both (2 nrow sequences) where they fit the register staging
+    float2 *X1 = ldsx, *X2 = ldsx + (size_t)nrow * W;
     const float2 *T1 = T + ((size_t)b * W + n0) * ws, *T2 = T + (((size_t)B + b) * W + n0) * ws;
     for (int t0 = 0; t0 < cnt; t0 += kStElems) {
         float2 u[IT], v[IT];
@@ -1481,8 +1483,12 @@ __global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int 
         }
     }
     __syncthreads();
-    stfft::lds_fft<kStElems, NT>(X1, W, nrow, Pi, +1);
-    stfft::lds_fft<kStElems, NT>(X2, W, nrow, Pi, +1);
+    if (2 * cnt <= kStElems) {
+        stfft::lds_fft<kStElems, NT>(X1, W, 2 * nrow, Pi, +1);
+    } else {
+        stfft::lds_fft<kStElems, NT>(X1, W, nrow, Pi, +1);
+        stfft::lds_fft<kStElems, NT>(X2, W, nrow, Pi, +1);
+    }
     const size_t sws = (size_t)ws;
     const float scale = (float)(1. / (double)(sws * sws * sws * sws * 8));
     const float thr = P.im.stype == ST_SAMPLE_C64 ? 1.0e-15f : 1.0e-6f;
@@ -1684,8 +1690,14 @@ __global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *Z, int W, const 
         }
     }
     __syncthreads();
-    stfft::lds_fft<kStElems, NT>(X, px, C, Pf, -1);
-    stfft::lds_fft<kStElems, NT>(Y, px, C, Pf, -1);
+    // Y follows X at the same pitch: one transform of 2C sequences where
+    // they fit the register staging
+    if (2 * C * W <= kStElems) {
+        stfft::lds_fft<kStElems, NT>(X, px, 2 * C, Pf, -1);
+    } else {
+        stfft::lds_fft<kStElems, NT>(X, px, C, Pf, -1);
+        stfft::lds_fft<kStElems, NT>(Y, px, C, Pf, -1);
+    }
     for (int t0 = 0; t0 < cnt; t0 += kStElems) {
         float2 u[IT], v[IT];
 #pragma unroll
@@ -1717,9 +1729,14 @@ __global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *Z, int W, const 
     }
     // s = F^-1(F1 conj(F2)) is real, so T(n, -l) = conj T(n, l): only the
     // columns l <= W/2 are transformed back and stored
-    stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
-    if (c0 == 0)
+    if (c0 != 0) {
+        stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
+    } else if ((C + 1) * W <= kStElems) {
+        stfft::lds_fft<kStElems, NT>(X, px, C + 1, Pi, +1);  // Y's slot 0 (column W/2) is sequence C
+    } else {
+        stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
         stfft::lds_fft<kStElems, NT>(Y, px, 1, Pi, +1);
+    }
     float2 *Sb = S + (size_t)b * W * W;
     for (int t = threadIdx.x; t < cnt; t += NT) {
         const int n = t >> lc, c = t & (C - 1);
