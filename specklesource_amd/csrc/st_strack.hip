@@ -1183,7 +1183,10 @@ __global__ void __launch_bounds__(NT) k_cpx_colx(const float2 *P, int B, int w, 
     const int b = Lb / nbx, c0 = (Lb - (Lb / nbx) * nbx) * C;
     const int nc = imin(C, w - c0);
     const int px = W + 1, py = w + 1;
-    float2 *X = ldsx, *Y = ldsx + (size_t)C * px;  // [C][px] F1 then the padded column, [C][py] F2
+    // F1 [C][py] and F2 [C][py] back to back, so one transform takes both;
+    // the padded columns [C][px] then overwrite them from the start of the
+    // buffer (every read of the pad pass precedes its first write)
+    float2 *X = ldsx, *Y = ldsx + (size_t)C * py;
     const float2 *P1 = P + (size_t)b * w * w + c0, *P2 = P + ((size_t)B + b) * w * w + c0;
     {
         const int cnt = w << lc;
@@ -1199,14 +1202,13 @@ __global__ void __launch_bounds__(NT) k_cpx_colx(const float2 *P, int B, int w, 
         for (int it = 0; it < IT; it++) {
             const int t = threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1);
             if (t < cnt) {
-                X[c * px + a] = u[it];
+                X[c * py + a] = u[it];
                 Y[c * py + a] = v[it];
             }
         }
     }
     __syncthreads();
-    stfft::lds_fft<kStElems, NT>(X, px, C, Pf, -1);
-    stfft::lds_fft<kStElems, NT>(Y, py, C, Pf, -1);
+    stfft::lds_fft<kStElems, NT>(X, py, 2 * C, Pf, -1);
     {
         const int azb = az[b], cnt = C * W;
         float2 v[IT];
@@ -1221,7 +1223,7 @@ __global__ void __launch_bounds__(NT) k_cpx_colx(const float2 *P, int B, int w, 
                     int ka = k + w + azb;  // in [0, 3w)
                     ka -= ka >= w ? w : 0;
                     ka -= ka >= w ? w : 0;
-                    v[it] = cmulconj(X[c * px + ka], Y[c * py + ka]);
+                    v[it] = cmulconj(X[c * py + ka], Y[c * py + ka]);
                 }
             }
         }
@@ -2825,7 +2827,8 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         while ((2 << lcx) <= 16 && (2 << lcx) * W2 <= kStElems)
             lcx++;
         const int Cx = 1 << lcx, Rr = kStElems / W2, nPart = (W2 + Rr - 1) / Rr;
-        const size_t ldsX = (size_t)Cx * ((W2 + 1) + (w + 1)) * sizeof(float2), ldsR = (size_t)Rr * W2 * sizeof(float2);
+        const size_t ldsX = (size_t)Cx * std::max(2 * (w + 1), W2 + 1) * sizeof(float2),
+                     ldsR = (size_t)Rr * W2 * sizeof(float2);
         const stfft::PlanArgs PfW = ctx->fft.plan(w, -1), PiW2 = ctx->fft.plan(W2, +1);
         const stfft::FastDiv dW2 = stfft::make_fastdiv((uint32_t)W2);
         Top2 *part = (Top2 *)C.tmp;  // [B][nPart]
