@@ -25,7 +25,7 @@ struct st_ctx {
     std::map<int, StPair> pairs;
     int nextPair = 1;
     std::mutex mu;
-    size_t scratchBudget = (size_t)3 << 30;  // bytes of scratch per tracking batch
+    size_t scratchBudget = (size_t)12 << 30;  // bytes of scratch per tracking batch
     st_track_stats stats{};
     bool profiling = false;
     hipEvent_t ev[72];  // [0, 64) kernel timing pairs, [64, 72) stage boundaries
