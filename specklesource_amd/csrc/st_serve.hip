@@ -180,7 +180,7 @@ extern "C" int st_server_create(int32_t device, int32_t slots, uint64_t hbmBudge
     st_server *s = new st_server();
     s->device = device;
     s->budget = hbmBudget;
-    s->slotScratch = std::min<size_t>((size_t)3 << 30, hbmBudget / (2 * (size_t)slots));
+    s->slotScratch = std::min<size_t>((size_t)12 << 30, hbmBudget / (2 * (size_t)slots));
     for (int k = 0; k < slots; k++) {
         st_ctx *c = nullptr;
         const int rc = st_ctx_create(device, &c);
