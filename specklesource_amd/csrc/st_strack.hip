@@ -2896,7 +2896,9 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         const size_t ldsM = (size_t)(Rm / 2) * W * sizeof(float2);
         const size_t per = (size_t)ws * ws * 128 + (size_t)nPd * (sizeof(AmpPart) + sizeof(double2)) +
                            (size_t)nPm * sizeof(Top2) + 2048;
-        const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nA, (ar.cap - ar.used) / per, kMaxBatch}));
+        // the amplitude kernels put 2 x batch on grid y (at most 65535)
+        const size_t kMaxAmpBatch = 16384;
+        const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nA, (ar.cap - ar.used) / per, kMaxAmpBatch}));
         AmpBatch A;
         A.I = ar.take<float2>((size_t)2 * B * ws * ws);
         A.F = ar.take<float2>((size_t)2 * B * W * W);
