@@ -39,6 +39,7 @@ extern "C" {
 #define ST_ERR_UNSUPPORTED (-3) /* option not implemented on this path */
 #define ST_ERR_NOMEM (-4)       /* device allocation failed / over budget */
 #define ST_ERR_NODEV (-5)       /* no usable GPU */
+#define ST_ERR_CANCELLED (-6)   /* the job was cancelled (st_cancel) */
 
 /* SLC sample types (Strack/strack.h:10-16, speckleTrack.c:1221-1232) */
 #define ST_SAMPLE_C64 0  /* complex float32, fftw_complex {re, im} */
@@ -272,23 +273,30 @@ int st_cull_ls_run(st_ctx *ctx, st_cull_ls_params *cp);
 /* ---- serving --------------------------------------------------------- */
 /* A server runs tracking jobs of different scene and window sizes
  * concurrently on one GPU (continuous batching).  Each of its slots owns a
- * stream and scratch arena; a queued job is admitted when a slot is idle
- * and its device memory (per-job image upload + slot scratch) fits in the
- * HBM budget left by the running jobs and the resident pairs, so jobs that
- * arrive while others run start as soon as a slot and memory free up and
- * their kernels overlap on the device.  Results do not depend on what else
- * runs.  No counterpart in the reference (single-threaded, one job per
- * process: Strack/strack.c:119-135); it is the request loop the
- * function-level entry points sit under. */
+ * stream and a scratch arena capped at the scratch the server charged for
+ * it.  A job that names a resident pair is ready at once; a job that
+ * brings host images is staged first: a stager thread uploads its images
+ * on its own stream, as soon as they fit in the HBM budget, while the
+ * slots compute other jobs.  Idle slots take ready jobs in arrival order,
+ * so jobs that arrive while others run start as soon as a slot frees up
+ * and their kernels overlap on the device.  Admission charges every slot's
+ * arena (which holds a job's per-point state and device output grids),
+ * the resident pairs and the staged images.  Results do not depend on what
+ * else runs.  For strack jobs with a cull chain the offset grids stay on
+ * the device from tracking into cullst.  No counterpart in the reference
+ * (single-threaded, one job per process: Strack/strack.c:119-135); it is
+ * the request loop the function-level entry points sit under. */
 typedef struct st_server st_server;
 
 #define ST_JOB_STRACK 0  /* st_speckle_track */
 #define ST_JOB_STRACKW 1 /* st_corr_track_fast */
+#define ST_JOB_CULL 2    /* cullst alone (st_cull_run) on the host grids of `cull`: no pair, no images */
 
 #define ST_JOB_QUEUED 0
 #define ST_JOB_RUNNING 1
 #define ST_JOB_DONE 2
 #define ST_JOB_FAILED 3
+#define ST_JOB_CANCELLED 4
 
 typedef struct {
     int32_t kind;            /* ST_JOB_* */
@@ -297,8 +305,10 @@ typedef struct {
     int32_t nr1, na1, nr2, na2, sampleType;
     st_track_params par;     /* copied at submit; its pointer members must stay valid until done */
     st_track_out out;        /* caller-owned output arrays, written when the job is done */
-    st_cull_params *cull;    /* optional: cullst (st_cull_run) on the job's output grids after
-                                tracking; its offR/offA/corr/type must be the out arrays */
+    st_cull_params *cull;    /* optional: the cullst chain (st_cull_run's order) on the job's output
+                                grids after tracking; its offR/offA/corr/type must be the out
+                                arrays, which then hold the culled grids.  strack jobs cull the
+                                grids on the device before they are copied out. */
 } st_job;
 
 typedef struct {
@@ -322,13 +332,25 @@ int st_server_destroy(st_server *srv);
 /* Resident pairs shared by all jobs of the server (counted in the budget). */
 int st_server_pair_upload(st_server *srv, const void *img1, int32_t nr1, int32_t na1, const void *img2,
                           int32_t nr2, int32_t na2, int32_t sampleType, int32_t *pair);
+/* Marks the pair as leaving (new jobs naming it are refused), waits until
+ * every submitted job that names it has ended, then frees it.  A second
+ * release of the same pair fails with ST_ERR_ARG. */
 int st_server_pair_release(st_server *srv, int32_t pair);
 /* Queue a job; returns its id at once. */
 int st_submit(st_server *srv, const st_job *job, int64_t *id);
 /* State of a job without waiting (info may be NULL). */
 int st_poll(st_server *srv, int64_t id, st_job_info *info);
-/* Block until the job is done or failed. */
+/* Block until the job is done, failed or cancelled. */
 int st_wait(st_server *srv, int64_t id, st_job_info *info);
+/* Cancel a job: a queued job ends at once, a running one at its next stage
+ * or batch boundary (its outputs are then unspecified); either way its
+ * state becomes ST_JOB_CANCELLED with status ST_ERR_CANCELLED.  ST_ERR_ARG
+ * for a job that has already ended. */
+int st_cancel(st_server *srv, int64_t id);
+/* Forget an ended job (done, failed or cancelled) so the server's job
+ * table does not grow without bound; its id is invalid afterwards.
+ * ST_ERR_ARG while the job is queued or running. */
+int st_job_release(st_server *srv, int64_t id);
 
 #ifdef __cplusplus
 }

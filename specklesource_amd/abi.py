@@ -12,6 +12,7 @@ ST_ERR_HIP = -2
 ST_ERR_UNSUPPORTED = -3
 ST_ERR_NOMEM = -4
 ST_ERR_NODEV = -5
+ST_ERR_CANCELLED = -6
 
 ST_SAMPLE_C64 = 0
 ST_SAMPLE_CI16 = 1
@@ -127,8 +128,8 @@ class CullLSParams(C.Structure):
     ]
 
 
-ST_JOB_STRACK, ST_JOB_STRACKW = 0, 1
-ST_JOB_QUEUED, ST_JOB_RUNNING, ST_JOB_DONE, ST_JOB_FAILED = 0, 1, 2, 3
+ST_JOB_STRACK, ST_JOB_STRACKW, ST_JOB_CULL = 0, 1, 2
+ST_JOB_QUEUED, ST_JOB_RUNNING, ST_JOB_DONE, ST_JOB_FAILED, ST_JOB_CANCELLED = 0, 1, 2, 3, 4
 
 
 class Job(C.Structure):

@@ -53,13 +53,23 @@ void parallel_copy(const std::vector<std::tuple<void *, const void *, size_t>> &
 
 void st_ctx::arena_reserve(size_t bytes)
 {
+    ST_REQUIRE(!arenaLimit || bytes <= arenaLimit, ST_ERR_NOMEM,
+               "scratch need of " + std::to_string(bytes >> 20) + " MB above the context's arena limit");
     if (arena.cap < bytes) {
         if (arena.base)
             ST_HIP(hipFree(arena.base));
         arena.base = nullptr;
         arena.cap = 0;
         size_t cap = bytes + (bytes >> 3);
-        ST_HIP(hipMalloc(&arena.base, cap));
+        if (arenaLimit)
+            cap = std::min(cap, arenaLimit);
+        const hipError_t e = hipMalloc(&arena.base, cap);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            arena.base = nullptr;
+            throw StError{ST_ERR_NOMEM, "device memory: no room for " + std::to_string(cap >> 20) + " MB of scratch"};
+        }
+        ST_HIP(e);
         arena.cap = cap;
     }
     arena.reset();
@@ -145,6 +155,9 @@ extern "C" int st_ctx_create(int32_t device, st_ctx **out)
         delete c;
         return ST_ERR_HIP;
     }
+    size_t freeB = 0, totalB = 0;
+    if (hipMemGetInfo(&freeB, &totalB) == hipSuccess && freeB > 0)
+        c->scratchBudget = std::min(c->scratchBudget, std::max<size_t>(freeB / 4, (size_t)64 << 20));
     *out = c;
     return ST_OK;
 }

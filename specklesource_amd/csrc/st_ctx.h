@@ -2,6 +2,7 @@
 // scratch arena and the resident SLC pairs.
 #pragma once
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -25,7 +26,16 @@ struct st_ctx {
     std::map<int, StPair> pairs;
     int nextPair = 1;
     std::mutex mu;
-    size_t scratchBudget = (size_t)12 << 30;  // bytes of scratch per tracking batch
+    // bytes of scratch per tracking batch: min(12 GB, a quarter of the free
+    // device memory at creation); a batch that cannot get its arena is
+    // retried with half the budget
+    size_t scratchBudget = (size_t)12 << 30;
+    // hard cap on the arena (0 = none): a server slot's arena never grows
+    // past the scratch the server charged for it
+    size_t arenaLimit = 0;
+    // set by st_cancel for the job running on this context; the tracking
+    // calls poll it between stages and batches
+    std::atomic<bool> cancel{false};
     st_track_stats stats{};
     bool profiling = false;
     hipEvent_t ev[72];  // [0, 64) kernel timing pairs, [64, 72) stage boundaries
@@ -42,8 +52,15 @@ struct st_ctx {
     int pendSlot[32];
     int npend = 0;
 
-    // Make the arena at least `bytes` large and empty it.
+    // Make the arena at least `bytes` large and empty it.  Throws
+    // ST_ERR_NOMEM when the device (or arenaLimit) cannot give that much.
     void arena_reserve(size_t bytes);
+    // Throws ST_ERR_CANCELLED once st_cancel has asked for it.
+    void check_cancel() const
+    {
+        if (cancel.load(std::memory_order_relaxed))
+            throw StError{ST_ERR_CANCELLED, "job cancelled"};
+    }
     // Pinned host staging for result copies, at least `bytes` large.
     void *pinned = nullptr;
     size_t pinnedCap = 0;
@@ -76,8 +93,19 @@ int st_guard(st_ctx *ctx, F fn)
 // pinned staging buffer into the caller's arrays.
 void parallel_copy(const std::vector<std::tuple<void *, const void *, size_t>> &jobs);
 
-// Entry points implemented in the per-tool translation units.
+// Entry points implemented in the per-tool translation units.  With
+// `cull` set, strack runs the cullst chain on its output grids while they
+// are still on the device (st_cull_chain_device) before copying them out.
 void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, st_track_out *out,
-                           st_track_diag *diag);
+                           st_track_diag *diag, st_cull_params *cull = nullptr);
 void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, st_track_out *out,
                              st_track_diag *diag);
+
+// The cullst call order (Cullst/cullst.c:75-100) on grids already on the
+// device: offR / offA / type are culled in place, corr is read; the other
+// grids of cp (sigma, smoothed offsets) are copied to its host arrays.
+// Takes its scratch from the context arena after what is in use (the
+// caller reserved cull_device_bytes(n) for it).
+size_t cull_device_bytes(size_t n);
+void st_cull_chain_device(st_ctx *ctx, st_cull_params *cp, float *dOffR, float *dOffA, float *dCorr,
+                          uint8_t *dType);

@@ -648,14 +648,20 @@ struct CullRun {
     // Device copies of every grid of the stages (n x 72 bytes).
     void alloc(int nR, int nA)
     {
-        ctx->arena_reserve((size_t)n * 72 + 65536);
+        ctx->arena_reserve(cull_device_bytes((size_t)n) + (size_t)4 * 4 * n + 4 * 256);
+        D.offR = ctx->arena.take<float>(n);
+        D.offA = ctx->arena.take<float>(n);
+        D.corr = ctx->arena.take<float>(n);
+        D.type = ctx->arena.take<uint8_t>(n);
+        carve(nR, nA);
+    }
+    // The stage grids other than offR / offA / corr / type, from the arena
+    // after what is in use.
+    void carve(int nR, int nA)
+    {
         Arena &ar = ctx->arena;
         D.nR = nR;
         D.nA = nA;
-        D.offR = ar.take<float>(n);
-        D.offA = ar.take<float>(n);
-        D.corr = ar.take<float>(n);
-        D.type = ar.take<uint8_t>(n);
         D.mask = nullptr;
         D.offRS = ar.take<float>(n);
         D.offAS = ar.take<float>(n);
@@ -674,6 +680,23 @@ struct CullRun {
     {
         n = nR * nA;
         alloc(nR, nA);
+    }
+    // On grids already on the device (tracking outputs): no arena reset.
+    CullRun(st_ctx *c, st_cull_params *p, float *dOffR, float *dOffA, float *dCorr, uint8_t *dType)
+        : ctx(c), cp(p)
+    {
+        ST_REQUIRE(cp && cp->nR > 0 && cp->nA > 0, ST_ERR_ARG, "cull grids");
+        n = cp->nR * cp->nA;
+        D.offR = dOffR;
+        D.offA = dOffA;
+        D.corr = dCorr;
+        D.type = dType;
+        carve(cp->nR, cp->nA);
+        if (cp->mask) {
+            uint8_t *m = ctx->arena.take<uint8_t>(n);
+            put(m, cp->mask, n);
+            D.mask = m;
+        }
     }
     template <class T>
     void put(T *d, const T *h, size_t cnt)
@@ -756,6 +779,27 @@ struct CullRun {
 };
 
 }  // namespace
+
+// Arena bytes the stage grids take beyond offR / offA / corr / type,
+// including the optional mask and the 256-byte alignment of each grid.
+size_t cull_device_bytes(size_t n) { return n * 56 + n + 16 * 256 + 65536; }
+
+void st_cull_chain_device(st_ctx *c, st_cull_params *cp, float *dOffR, float *dOffA, float *dCorr, uint8_t *dType)
+{
+    CullRun R(c, cp, dOffR, dOffA, dCorr, dType);
+    ST_REQUIRE(cp->sigmaR && cp->sigmaA && cp->offRS && cp->offAS, ST_ERR_ARG, "cull arrays");
+    R.st_data();
+    R.st_data();
+    R.st_data();
+    R.stats();
+    R.smooth();
+    if (cp->islandThresh > 0)
+        R.islands();
+    R.get(cp->sigmaR, R.D.sigmaR, R.n);
+    R.get(cp->sigmaA, R.D.sigmaA, R.n);
+    R.get(cp->offRS, R.D.offRS, R.n);
+    R.get(cp->offAS, R.D.offAS, R.n);
+}
 
 extern "C" int st_cull_st_data(st_ctx *c, st_cull_params *cp)
 {

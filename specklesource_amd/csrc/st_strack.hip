@@ -2559,7 +2559,7 @@ int read_count(st_ctx *ctx, int *dcounts, int idx)
 }  // namespace
 
 void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, st_track_out *out,
-                           st_track_diag *diag)
+                           st_track_diag *diag, st_cull_params *cull)
 {
     ST_REQUIRE(par.wR == par.wA && par.wRa == par.wAa, ST_ERR_UNSUPPORTED, "non-square windows");
     ST_REQUIRE(par.wR >= 8 && par.wRa >= 8 && !(par.wR & 1) && !(par.wRa & 1), ST_ERR_ARG, "window sizes");
@@ -2614,28 +2614,45 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
                    "interferogram looks");
         fixed += (size_t)par.intNr * par.intNa * 8 + (size_t)npts * sizeof(IntSet) + 512;
     }
-    // batch sizes
+    // the cull chain works on the output grids in the same arena
+    if (cull) {
+        ST_REQUIRE(cull->nR == par.nR && cull->nA == par.nA && cull->offR == out->offR && cull->offA == out->offA &&
+                       cull->corr == out->corr && cull->type == out->type,
+                   ST_ERR_ARG, "cull grids must be the tracking outputs");
+        fixed += cull_device_bytes((size_t)npts);
+    }
+    // batch sizes: a budget the device cannot hold is halved until it fits
     const size_t perCpx = (size_t)w * w * (16 + 32 + 32 + 16) + 2048;
     int ws_max = par.wAa * (par.maxTries >= 2 ? LA : 1);
     const size_t perAmp = (size_t)ws_max * ws_max * 128 + (size_t)2 * ws_max * 96 + 4096;
     size_t budget = ctx->scratchBudget;
     const size_t kMaxBatch = 8192;  // keeps 2 x batch inside the 65535 grid-z limit of the transposes
-    int Bc = (int)std::max<size_t>(1, std::min<size_t>({(size_t)npts, budget / perCpx, kMaxBatch}));
-    int Ba = (int)std::max<size_t>(1, std::min<size_t>({(size_t)npts, budget / perAmp, kMaxBatch}));
-    size_t need = fixed + std::max((size_t)Bc * perCpx, (size_t)Ba * perAmp) + 65536;
-    if (fusedPath) {
-        // per-point results of the single fused launch + |C| scratch of
-        // every resident workgroup
-        int ncu = 0;
-        ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
-        const int perCU = w == 16 ? fused_per_cu<16>(pr.stype)
-                                  : (w == 32 ? fused_per_cu<32>(pr.stype) : fused_per_cu<64>(pr.stype));
-        const size_t perPt = sizeof(PeakRes) + 2 * sizeof(float) + 1 + sizeof(int) + 2 * sizeof(double);
-        const size_t fusedNeed = fixed + (size_t)npts * perPt + 8 * 256 +
-                                 (size_t)std::min(npts, perCU * ncu) * 4 * w * w * sizeof(float) + 65536;
-        need = std::max(need, fusedNeed);
+    int Bc = 1, Ba = 1;
+    for (;;) {
+        Bc = (int)std::max<size_t>(1, std::min<size_t>({(size_t)npts, budget / perCpx, kMaxBatch}));
+        Ba = (int)std::max<size_t>(1, std::min<size_t>({(size_t)npts, budget / perAmp, kMaxBatch}));
+        size_t need = fixed + std::max((size_t)Bc * perCpx, (size_t)Ba * perAmp) + 65536;
+        if (fusedPath) {
+            // per-point results of the single fused launch + |C| scratch of
+            // every resident workgroup
+            int ncu = 0;
+            ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+            const int perCU = w == 16 ? fused_per_cu<16>(pr.stype)
+                                      : (w == 32 ? fused_per_cu<32>(pr.stype) : fused_per_cu<64>(pr.stype));
+            const size_t perPt = sizeof(PeakRes) + 2 * sizeof(float) + 1 + sizeof(int) + 2 * sizeof(double);
+            const size_t fusedNeed = fixed + (size_t)npts * perPt + 8 * 256 +
+                                     (size_t)std::min(npts, perCU * ncu) * 4 * w * w * sizeof(float) + 65536;
+            need = std::max(need, fusedNeed);
+        }
+        try {
+            ctx->arena_reserve(need);
+            break;
+        } catch (const StError &e) {
+            if (e.code != ST_ERR_NOMEM || (Bc == 1 && Ba == 1))
+                throw;
+            budget /= 2;
+        }
     }
-    ctx->arena_reserve(need);
     Arena &ar = ctx->arena;
 
     Pts S;
@@ -2833,6 +2850,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         const stfft::FastDiv dW2 = stfft::make_fastdiv((uint32_t)W2);
         Top2 *part = (Top2 *)C.tmp;  // [B][nPart]
         for (int off = 0; off < nCpx; off += B) {
+            ctx->check_cancel();
             const int nb = std::min(B, nCpx - off);
             ctx->time_begin(kGather);
             hipLaunchKernelGGL(k_cpx_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, C);
@@ -2860,6 +2878,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         }
         ar.used = mark;
     }
+    ctx->check_cancel();
     {
         const int nCarry = (npts + kCarryBlk - 1) / kCarryBlk;
         hipLaunchKernelGGL(k_carry_blocks, dim3(nCarry), dim3(kCarryBlk), 0, st, S, npts, carryMax);
@@ -2916,6 +2935,7 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
         const stfft::PlanArgs Pfs = ctx->fft.plan(ws, -1), PfW = ctx->fft.plan(W, -1), PiW = ctx->fft.plan(W, +1);
         const stfft::FastDiv dW = stfft::make_fastdiv((uint32_t)W);
         for (int off = 0; off < nA; off += B) {
+            ctx->check_cancel();
             const int nb = std::min(B, nA - off);
             hipLaunchKernelGGL(k_amp_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, large, A);
             ctx->fft.rows(A.I, (long)2 * nb * ws, ws, -1, st);
@@ -2957,6 +2977,10 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
 
     hipLaunchKernelGGL(k_compose, dim3((npts + 255) / 256), dim3(256), 0, st, P, S, dOffR, dOffA, npts);
     ST_HIP(hipGetLastError());
+    if (cull) {
+        ctx->check_cancel();
+        st_cull_chain_device(ctx, cull, dOffR, dOffA, S.corr, S.type);
+    }
     int counts[CNT_N];
     ST_HIP(hipMemcpyAsync(counts, S.counts, sizeof(counts), hipMemcpyDeviceToHost, st));
     // results go through a pinned staging buffer (a device-to-pageable copy

@@ -968,9 +968,20 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
         const size_t per = 2 * NN * 8 + 2 * NN * 8 + (size_t)p.nLagA * p.nLagR * 4 +
                            (size_t)p.nLagA * sizeof(RowBest) + 2 * (size_t)(nblk + 1) * 32 +
                            (size_t)((p.W_A + kTile - 1) / kTile) * 2 * p.W_R * 8 + 4096;
-        const size_t budget = std::max(ctx->scratchBudget, per * 4);
-        const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nList, budget / per, (size_t)4096}));
-        ctx->arena_reserve((size_t)B * per + (size_t)nList * (16 + sizeof(WRes)) + 65536);
+        // a batch the device cannot hold is halved until it fits
+        size_t budget = std::max(ctx->scratchBudget, per * 4);
+        int B = 1;
+        for (;;) {
+            B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nList, budget / per, (size_t)4096}));
+            try {
+                ctx->arena_reserve((size_t)B * per + (size_t)nList * (16 + sizeof(WRes)) + 65536);
+                break;
+            } catch (const StError &e) {
+                if (e.code != ST_ERR_NOMEM || B == 1)
+                    throw;
+                budget = (size_t)B * per / 2;
+            }
+        }
         Arena &ar = ctx->arena;
         int4 *dPos = ar.take<int4>(nList);
         WRes *dRes = ar.take<WRes>(nList);
@@ -996,6 +1007,7 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
         const stfft::PlanArgs PfA = ctx->fft.plan(p.wAa, -1), PiA = ctx->fft.plan(p.W_A, +1);
         const size_t ldsRow = (size_t)R * p.W_R * sizeof(float2), ldsCol = (size_t)C * (p.W_A + 1) * sizeof(float2);
         for (int off = 0; off < nList; off += B) {
+            ctx->check_cancel();
             const int nb = std::min(B, nList - off);
             const int np = (nb + 1) / 2;
             Bt.res = dRes + off;

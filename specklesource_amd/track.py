@@ -62,6 +62,8 @@ def load_library(path=LIB_PATH):
     lib.st_submit.argtypes = [vp, C.POINTER(abi.Job), C.POINTER(C.c_int64)]
     lib.st_poll.argtypes = [vp, C.c_int64, C.POINTER(abi.JobInfo)]
     lib.st_wait.argtypes = [vp, C.c_int64, C.POINTER(abi.JobInfo)]
+    lib.st_cancel.argtypes = [vp, C.c_int64]
+    lib.st_job_release.argtypes = [vp, C.c_int64]
     lib.st_set_profiling.argtypes = [vp, C.c_int32]
     lib.st_synth_host.argtypes = [C.POINTER(abi.SynthSpec), vp, vp]
     lib.st_synth_device.argtypes = [vp, C.POINTER(abi.SynthSpec), C.POINTER(C.c_int32)]
@@ -316,22 +318,47 @@ class Server:
         self._jobs[jid.value] = (res, out, job, keep)
         return jid.value
 
+    def submit_cull(self, cp, arrs):
+        """Queue a cullst job on host grids (abi.CullParams from
+        cull_params, with its arrays); wait() returns them culled as
+        `extra`."""
+        job = abi.Job()
+        job.kind = abi.ST_JOB_CULL
+        job.pair = -1
+        job.cull = C.pointer(cp)
+        jid = C.c_int64()
+        rc = self.lib.st_submit(self.srv, C.byref(job), C.byref(jid))
+        if rc != abi.ST_OK:
+            raise RuntimeError("st_submit failed (%d)" % rc)
+        self._jobs[jid.value] = (None, None, job, [cp, arrs])
+        return jid.value
+
     def poll(self, jid):
         info = abi.JobInfo()
         if self.lib.st_poll(self.srv, jid, C.byref(info)) != abi.ST_OK:
             raise RuntimeError("unknown job %d" % jid)
         return info
 
-    def wait(self, jid):
-        """Block until the job ends; returns (TrackResult, JobInfo, extra)
-        where extra holds the cull arrays when the job culled."""
+    def cancel(self, jid):
+        """Ask the server to cancel a job (st_cancel); False when the job
+        had already ended."""
+        return self.lib.st_cancel(self.srv, jid) == abi.ST_OK
+
+    def wait(self, jid, check=True):
+        """Block until the job ends and forget it on the server
+        (st_job_release); returns (TrackResult, JobInfo, extra) where extra
+        holds the cull arrays when the job culled.  With check, a job that
+        did not finish raises."""
         info = abi.JobInfo()
         if self.lib.st_wait(self.srv, jid, C.byref(info)) != abi.ST_OK:
             raise RuntimeError("unknown job %d" % jid)
+        if self.lib.st_job_release(self.srv, jid) != abi.ST_OK:
+            raise RuntimeError("st_job_release failed for job %d" % jid)
         res, out, job, keep = self._jobs.pop(jid)
-        if info.state != abi.ST_JOB_DONE:
+        if check and info.state != abi.ST_JOB_DONE:
             raise RuntimeError("job %d failed (%d): %s" % (jid, info.status, info.error.decode()))
-        res.nComplex, res.nAmp, res.nAmpL, res.nFail = info.nComplex, info.nAmp, info.nAmpL, info.nFail
+        if res is not None:
+            res.nComplex, res.nAmp, res.nAmpL, res.nFail = info.nComplex, info.nAmp, info.nAmpL, info.nFail
         extra = keep[-1] if job.cull else None
         return res, info, extra
 

@@ -80,11 +80,20 @@ def test_results_in_the_mix_equal_results_alone(device):
         srv.close()
 
 
+def _admission_job():
+    """A strack job on a 2048 x 1024 pair with a sparse grid: its images
+    (32 MB) are large next to its scratch needs."""
+    s = configs._spec(2048, 1024, 321, dr0=0.4, da0=-0.3, gmin=0.6, gmax=0.9)
+    a, b = track.synth_host(s)
+    p = small_params(32, 2048, 1024, step=96, rShiftPoly=[0.4, 0, 0], aShiftPoly=[-0.3, 0, 0])
+    return abi.ST_JOB_STRACK, p, a, b
+
+
 def test_admission_by_budget(device):
     """A budget that leaves room for one job's images at a time runs the
     jobs one after another (the last one's latency covers every run); a
     job that can never fit is refused at submit."""
-    kind, p, a, b, _ = _jobs()[1]
+    kind, p, a, b = _admission_job()
     img = a.nbytes + b.nbytes
     # slot scratch is budget / (2 slots) each, so half the budget is left
     # for images: 1.5 jobs' worth
@@ -112,14 +121,68 @@ def test_poll_reports_states(device):
     try:
         ids = [srv.submit(p, images=(a, b), kind=kind) for _ in range(3)]
         assert srv.poll(ids[-1]).state in (abi.ST_JOB_QUEUED, abi.ST_JOB_RUNNING)
-        for j in ids:
-            srv.wait(j)
-        assert all(srv.poll(j).state == abi.ST_JOB_DONE for j in ids)
+        srv.wait(ids[-1])
+        assert all(srv.poll(j).state == abi.ST_JOB_DONE for j in ids[:-1])
+        assert all(srv.wait(j)[1].state == abi.ST_JOB_DONE for j in ids[:-1])
         bad = track.default_params(rStart=0, aStart=0, deltaR=8, deltaA=8, nR=4, nA=4, wR=24, wA=32, wRa=24,
                                    wAa=32)
         jid = srv.submit(bad, images=(a, b), kind=kind)
+        assert srv.poll(jid).state in (abi.ST_JOB_QUEUED, abi.ST_JOB_RUNNING, abi.ST_JOB_FAILED)
         with pytest.raises(RuntimeError, match="non-square"):
             srv.wait(jid)
-        assert srv.poll(jid).state == abi.ST_JOB_FAILED
+        # wait() released the job: its id is no longer known
+        with pytest.raises(RuntimeError, match="unknown job"):
+            srv.poll(jid)
+    finally:
+        srv.close()
+
+
+def test_cancel_queued_and_running_jobs(device):
+    """On one slot: cancelling a queued job ends it at once without running
+    it; cancelling the running job ends it at a batch boundary (or it
+    finishes first); the jobs after them run and give the results they give
+    alone."""
+    kind, p, a, b, _ = _jobs()[1]
+    ref, _ = _alone(device, kind, p, a, b, None)
+    srv = track.Server(0, slots=1, hbm_budget=16 << 30)
+    try:
+        ids = [srv.submit(p, images=(a, b), kind=kind) for _ in range(5)]
+        assert srv.cancel(ids[3])
+        assert srv.cancel(ids[0])
+        states = {}
+        for j in ids:
+            r, info, _ = srv.wait(j, check=False)
+            states[j] = info.state
+            if info.state == abi.ST_JOB_DONE:
+                _same(r, ref)
+            else:
+                assert info.state == abi.ST_JOB_CANCELLED and info.status == abi.ST_ERR_CANCELLED
+        assert states[ids[3]] == abi.ST_JOB_CANCELLED
+        assert states[ids[0]] in (abi.ST_JOB_CANCELLED, abi.ST_JOB_DONE)
+        assert all(states[j] == abi.ST_JOB_DONE for j in (ids[1], ids[2], ids[4]))
+        # an ended job cannot be cancelled; the server still runs new work
+        jid = srv.submit(p, images=(a, b), kind=kind)
+        r, info, _ = srv.wait(jid)
+        _same(r, ref)
+        assert not srv.cancel(jid)
+    finally:
+        srv.close()
+
+
+def test_pair_release_waits_for_its_jobs_and_refuses_new_ones(device):
+    kind, p, a, b, _ = _jobs()[0]
+    ref, _ = _alone(device, kind, p, a, b, None)
+    srv = track.Server(0, slots=2, hbm_budget=16 << 30)
+    try:
+        rp = srv.upload_pair(a, b)
+        ids = [srv.submit(p, pair=rp, kind=kind) for _ in range(4)]
+        srv.release_pair(rp)  # returns once the four jobs have ended
+        assert all(srv.poll(j).state == abi.ST_JOB_DONE for j in ids)
+        for j in ids:
+            _same(srv.wait(j)[0], ref)
+        with pytest.raises(RuntimeError):
+            srv.submit(p, pair=rp, kind=kind)
+        with pytest.raises(RuntimeError):
+            srv.release_pair(rp)
     finally:
         srv.close()
