@@ -1356,3 +1356,19 @@ void orc_corr_surface(const ocplx *p1, const ocplx *p2, int w, int azShift, orea
     free(s.cbuf);
     free(s.cFastOver);
 }
+
+/* findCPeak alone on a W_A x W_R complex surface in the inverse
+ * transform's (unshifted) layout; mag receives the fftshifted |C|. */
+void orc_find_cpeak(const ocplx *C, int W_R, int W_A, oreal *mag, int *iMax, int *jMax, double *cMax)
+{
+    orc_st s;
+    memset(&s, 0, sizeof(s));
+    find_cpeak(&s, mag, C, W_R, W_A, iMax, jMax, cMax);
+}
+
+/* getShifts alone: the initial offset at (range, azimuth) from the grids
+ * of par (initDr / initDa). */
+void orc_get_shifts(const st_track_params *par, double range, double azimuth, double *dr, double *da)
+{
+    get_shifts(par, range, azimuth, dr, da);
+}

@@ -56,7 +56,9 @@ def test_mfma_parity_synthetic(device, oracle32, oracle64, w):
     assert names.get("k_cpx_mfma", 0) > 0
     o32 = oracle32.speckle_track(p, a, b, diag=True)
     o64 = oracle64.speckle_track(p, a, b, diag=True)
-    rep, probs = compare_strack(p, g, o32, o64, tie=MF_TIE, thresh_tol=MF_THRESH)
+    # the split-bf16 surfaces differ from FP32 ones by ~1e-5 relative, which
+    # the five-iteration width fits carry into the .azd widths
+    rep, probs = compare_strack(p, g, o32, o64, tie=MF_TIE, thresh_tol=MF_THRESH, lm_rtol=2e-2)
     print("\n", rep)
     assert not probs, probs[:10]
     assert rep["peak_agree"] >= 0.999 or rep["attempts"] - rep["near_ties"] <= round(rep["peak_agree"] * rep["attempts"])
