@@ -157,3 +157,55 @@ def make_cull_grid(nA, nR, seed=0, outlier_frac=0.05, hole_frac=0.08):
     typ[holes] = 0
     corr = rng.uniform(0.0, 1.0, (nA, nR)).astype(np.float32)
     return offR, offA, corr, typ
+
+
+def inject_outliers(offR, offA, frac=0.05, seed=0, amp=30.0):
+    """Copies of the offset grids with `frac` of the valid points moved by
+    uniform +/-amp px in both components (the served stream's injected
+    outliers)."""
+    rng = np.random.default_rng(seed)
+    r, a = offR.copy(), offA.copy()
+    pick = (rng.random(r.shape) < frac) & (r > -1e9) & (a > -1e9)
+    r[pick] += rng.uniform(-amp, amp, int(pick.sum())).astype(np.float32)
+    a[pick] += rng.uniform(-amp, amp, int(pick.sum())).astype(np.float32)
+    return r, a
+
+
+def cfg5_stream(n_jobs=40, scale=1, seed=5, rate=20.0):
+    """The served stream: Poisson arrivals at `rate` jobs/s of tracking jobs
+    drawn from the other four configurations by size -- amplitude-only
+    scenes (cfg1), complex + amplitude scenes up to 20000 px with 64 px
+    windows (cfg2), uneven pairs with windows 16-128 (cfg3) and strackw
+    scenes with chips 128 / 256 (cfg4) -- each with a cullst pass (box 5 or
+    9, islands below 20).  Returns a list of dicts with keys kind ('strack'
+    / 'strackw'), spec, par, arrival (s), cull (keyword overrides of the
+    cull parameters), src (the configuration it was drawn from)."""
+    rng = np.random.default_rng(seed)
+    jobs, t = [], 0.0
+    for k in range(n_jobs):
+        t += float(rng.exponential(1.0 / rate))
+        src = int(rng.integers(1, 5))
+        sub = seed * 100 + k
+        if src == 1:
+            s, p = cfg1(scale=scale, seed=sub)
+            kind = "strack"
+        elif src == 2:
+            side = int(rng.integers(4096, 20001))
+            nr, na = side // scale, int(side * 0.7) // scale
+            s = _spec(nr, na, sub, dr0=0.3, da0=0.2, ridgeR=4.0, ridgeA=12.0, ridgeC=0.55 * nr,
+                      ridgeW=0.08 * nr, gmin=0.3, gmax=0.9, gLr=nr / 1.5, gLa=na / 1.25)
+            r0, a0, nR, nA = grid_for(nr, na, 128, 24, 24)
+            rp, ap = affine_poly(s)
+            p = default_params(rStart=r0, aStart=a0, deltaR=24, deltaA=24, nR=nR, nA=nA, wR=64, wA=64,
+                               wRa=64, wAa=64, rShiftPoly=rp, aShiftPoly=ap)
+            kind = "strack"
+        elif src == 3:
+            (s, p), = cfg3_pairs(n_pairs=1, scale=scale, seed=sub)
+            kind = "strack"
+        else:
+            s, p = cfg4(chip=int(rng.choice([128, 256])), scale=scale, seed=sub)
+            kind = "strackw"
+        box = 5 if k % 2 else 9
+        jobs.append(dict(kind=kind, spec=s, par=p, arrival=t, src=src,
+                         cull=dict(bR=box, bA=box, islandThresh=20)))
+    return jobs

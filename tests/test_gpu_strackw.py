@@ -121,6 +121,29 @@ def test_parity_cfg4_window(device, oracle32, oracle64):
     assert np.abs(g.offR - tr[..., 0]).max() < 0.35 and np.abs(g.offA - tr[..., 1]).max() < 0.35
 
 
+def test_parity_cfg4_chip256_window(device, oracle32, oracle64):
+    """The chip-256 strackw window of the sheared-flow workload: wRa = 512
+    (chip 256 + 2 x (123 + 5)), 1024^2 transforms, 513^2 lags, 5 x 5
+    boxcar (corrTrackFast.c:116-124), on a 1536 x 1536 scene with offsets
+    of tens of pixels found from an initial polynomial a few pixels off,
+    coherence 0.2-0.4, 2 x 2 grid points."""
+    s = configs._spec(1536, 1536, 6, dr0=20.2, drR=2e-3, da0=-35.6, daA=-1e-3, gmin=0.2, gmax=0.4,
+                      gLr=1200, gLa=1000)
+    p = track.default_params(rStart=640, aStart=640, deltaR=160, deltaA=160, nR=2, nA=2, wR=512, wA=512,
+                             wRa=512, wAa=512, edgePadR=123, edgePadA=123, navgR=2, navgA=2, noComplex=1,
+                             rShiftPoly=[18.0, 0, 0], aShiftPoly=[-40.0, 0, 0])
+    a, b = track.synth_host(s)
+    g = _run(device, p, a, b)
+    o32 = oracle32.corr_track_fast(p, a, b, diag=True)
+    o64 = oracle64.corr_track_fast(p, a, b, diag=True)
+    rep, probs = compare(g, o32, o64)
+    print("\n", rep)
+    _check(rep, probs)
+    assert g.nAmp == 4
+    tr = np.array([[track.synth_offset(s, 640 + 160 * j, 640 + 160 * i) for j in range(2)] for i in range(2)])
+    assert np.abs(g.offR - tr[..., 0]).max() < 0.35 and np.abs(g.offA - tr[..., 1]).max() < 0.35
+
+
 def test_ci16_samples(device, oracle32, oracle64):
     s, par = _scene(400, 400, 77, 64, 8, 2, step=40)
     a, b = track.synth_host(s)
