@@ -231,6 +231,15 @@ int st_cull_islands(st_ctx *ctx, st_cull_params *cp);
 /* The cullst call order (Cullst/cullst.c:75-100): cullSTData x3,
  * cullStats, cullSmooth, cullIslands when islandThresh > 0. */
 int st_cull_run(st_ctx *ctx, st_cull_params *cp);
+/* st_cull_run without any device allocation: no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every device grid of the chain lives in one fixed device
+ * array of about 37 MB per GPU, held from library load until the process
+ * ends; grids above 524 288 points (nR * nA) fail with ST_ERR_ARG.  The
+ * array is shared by every context on a GPU: the library serialises such
+ * calls within a process, and each call initialises what it reads. */
+int st_cull_run_noalloc(st_ctx *ctx, st_cull_params *cp);
 
 /* ---- Landsat culling (Cullls/cullls.h:2-35) ---------------------------- */
 /* CullLSParams with the matchResult / lsFit fields the stages read.  The

@@ -158,6 +158,13 @@ extern "C" int st_ctx_create(int32_t device, st_ctx **out)
     size_t freeB = 0, totalB = 0;
     if (hipMemGetInfo(&freeB, &totalB) == hipSuccess && freeB > 0)
         c->scratchBudget = std::min(c->scratchBudget, std::max<size_t>(freeB / 4, (size_t)64 << 20));
+    try {
+        cull_noalloc_init(device);
+    } catch (const StError &e) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return e.code;
+    }
     *out = c;
     return ST_OK;
 }

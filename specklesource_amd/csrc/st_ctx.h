@@ -109,3 +109,7 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
 size_t cull_device_bytes(size_t n);
 void st_cull_chain_device(st_ctx *ctx, st_cull_params *cp, float *dOffR, float *dOffA, float *dCorr,
                           uint8_t *dType);
+
+// Resolves the device address of the no-allocation cull's scratch array on
+// `device` (once per device; st_ctx_create calls it).
+void cull_noalloc_init(int device);

@@ -14,6 +14,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "st_ctx.h"
@@ -633,14 +635,33 @@ struct CullRun {
     st_cull_params *cp;
     CullDev D{};
     int n;
+    Arena *ar = nullptr;  // where the grids are carved from
+    Arena fixedBuf;       // a caller-given buffer (the no-allocation entry point)
 
     CullRun(st_ctx *c, st_cull_params *p) : ctx(c), cp(p)
     {
         ST_REQUIRE(cp && cp->nR > 0 && cp->nA > 0 && cp->offR && cp->offA, ST_ERR_ARG, "cull grids");
         n = cp->nR * cp->nA;
+        ctx->arena_reserve(cull_device_bytes((size_t)n) + (size_t)4 * 4 * n + 4 * 256);
+        ar = &ctx->arena;
         alloc(cp->nR, cp->nA);
+        take_mask();
+    }
+    // Every grid in `bytes` of device memory at `base`: allocates nothing.
+    CullRun(st_ctx *c, st_cull_params *p, void *base, size_t bytes) : ctx(c), cp(p)
+    {
+        ST_REQUIRE(cp && cp->nR > 0 && cp->nA > 0 && cp->offR && cp->offA, ST_ERR_ARG, "cull grids");
+        n = cp->nR * cp->nA;
+        fixedBuf.base = (char *)base;
+        fixedBuf.cap = bytes;
+        ar = &fixedBuf;
+        alloc(cp->nR, cp->nA);
+        take_mask();
+    }
+    void take_mask()
+    {
         if (cp->mask) {
-            uint8_t *m = ctx->arena.take<uint8_t>(n);
+            uint8_t *m = ar->take<uint8_t>(n);
             put(m, cp->mask, n);
             D.mask = m;
         }
@@ -648,37 +669,38 @@ struct CullRun {
     // Device copies of every grid of the stages (n x 72 bytes).
     void alloc(int nR, int nA)
     {
-        ctx->arena_reserve(cull_device_bytes((size_t)n) + (size_t)4 * 4 * n + 4 * 256);
-        D.offR = ctx->arena.take<float>(n);
-        D.offA = ctx->arena.take<float>(n);
-        D.corr = ctx->arena.take<float>(n);
-        D.type = ctx->arena.take<uint8_t>(n);
+        D.offR = ar->take<float>(n);
+        D.offA = ar->take<float>(n);
+        D.corr = ar->take<float>(n);
+        D.type = ar->take<uint8_t>(n);
         carve(nR, nA);
     }
-    // The stage grids other than offR / offA / corr / type, from the arena
-    // after what is in use.
+    // The stage grids other than offR / offA / corr / type, after what is
+    // in use.
     void carve(int nR, int nA)
     {
-        Arena &ar = ctx->arena;
+        Arena &a = *ar;
         D.nR = nR;
         D.nA = nA;
         D.mask = nullptr;
-        D.offRS = ar.take<float>(n);
-        D.offAS = ar.take<float>(n);
-        D.sigmaR = ar.take<float>(n);
-        D.sigmaA = ar.take<float>(n);
-        D.R0 = ar.take<float>(n);
-        D.A0 = ar.take<float>(n);
-        D.cPrev = ar.take<uint8_t>(n);
-        D.cNew = ar.take<uint8_t>(n);
-        D.counters = ar.take<int>(16);
-        D.parent = ar.take<int>(n);
-        D.root = ar.take<int>(n);
-        D.bbox = ar.take<int>(4 * (size_t)n);
+        D.offRS = a.take<float>(n);
+        D.offAS = a.take<float>(n);
+        D.sigmaR = a.take<float>(n);
+        D.sigmaA = a.take<float>(n);
+        D.R0 = a.take<float>(n);
+        D.A0 = a.take<float>(n);
+        D.cPrev = a.take<uint8_t>(n);
+        D.cNew = a.take<uint8_t>(n);
+        D.counters = a.take<int>(16);
+        D.parent = a.take<int>(n);
+        D.root = a.take<int>(n);
+        D.bbox = a.take<int>(4 * (size_t)n);
     }
     CullRun(st_ctx *c, int nR, int nA) : ctx(c), cp(nullptr)
     {
         n = nR * nA;
+        ctx->arena_reserve(cull_device_bytes((size_t)n) + (size_t)4 * 4 * n + 4 * 256);
+        ar = &ctx->arena;
         alloc(nR, nA);
     }
     // On grids already on the device (tracking outputs): no arena reset.
@@ -687,16 +709,13 @@ struct CullRun {
     {
         ST_REQUIRE(cp && cp->nR > 0 && cp->nA > 0, ST_ERR_ARG, "cull grids");
         n = cp->nR * cp->nA;
+        ar = &ctx->arena;
         D.offR = dOffR;
         D.offA = dOffA;
         D.corr = dCorr;
         D.type = dType;
         carve(cp->nR, cp->nA);
-        if (cp->mask) {
-            uint8_t *m = ctx->arena.take<uint8_t>(n);
-            put(m, cp->mask, n);
-            D.mask = m;
-        }
+        take_mask();
     }
     template <class T>
     void put(T *d, const T *h, size_t cnt)
@@ -869,31 +888,98 @@ extern "C" int st_cull_islands(st_ctx *c, st_cull_params *cp)
     });
 }
 
+namespace {
+
+// cullst's order on host grids: upload, cullSTData x3, cullStats,
+// cullSmooth, cullIslands, download.
+void cull_chain_host(st_ctx *c, st_cull_params *cp, CullRun &R)
+{
+    R.put(R.D.offR, cp->offR, R.n);
+    R.put(R.D.offA, cp->offA, R.n);
+    R.put(R.D.corr, cp->corr, R.n);
+    R.put(R.D.type, cp->type, R.n);
+    R.st_data();
+    R.st_data();
+    R.st_data();
+    R.stats();
+    R.smooth();
+    if (cp->islandThresh > 0)
+        R.islands();
+    R.get(cp->offR, R.D.offR, R.n);
+    R.get(cp->offA, R.D.offA, R.n);
+    R.get(cp->type, R.D.type, R.n);
+    R.get(cp->sigmaR, R.D.sigmaR, R.n);
+    R.get(cp->sigmaA, R.D.sigmaA, R.n);
+    R.get(cp->offRS, R.D.offRS, R.n);
+    R.get(cp->offAS, R.D.offAS, R.n);
+    ST_HIP(hipStreamSynchronize(c->stream));
+}
+
+bool cull_arrays(const st_cull_params *cp)
+{
+    return cp->corr && cp->type && cp->sigmaR && cp->sigmaA && cp->offRS && cp->offAS;
+}
+
+constexpr size_t kNoAllocCells = (size_t)1 << 19;  // 524 288 grid points
+// the stage grids (50 B per point), the four input grids (13 B), the mask
+// (1 B), with room to spare, and 256-byte alignment of each of the 18 grids
+constexpr size_t kNoAllocBytes = kNoAllocCells * (56 + 1 + 13 + 1) + 18 * 256 + 4096;
+
+}  // namespace
+
+// ---- the entry point that allocates nothing ------------------------------
+// Every grid of the chain lives in this file-scope device array.  In host
+// code `scratch` is only a stand-in name: its device address on each GPU
+// comes from hipGetSymbolAddress, resolved once per device (st_ctx_create
+// calls cull_noalloc_init), and only that address is handed to the
+// kernels and copies.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMu;               // guards gScratch and serialises the calls
+void *gScratch[kMaxDevices] = {};    // device address of `scratch` per device
+
+}  // namespace
+
+void cull_noalloc_init(int device)
+{
+    std::lock_guard<std::mutex> lk(gScratchMu);
+    ST_REQUIRE(device >= 0 && device < kMaxDevices, ST_ERR_ARG, "device index");
+    if (gScratch[device])
+        return;
+    void *p = nullptr;
+    ST_HIP(hipSetDevice(device));
+    ST_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ST_REQUIRE(p != nullptr, ST_ERR_HIP, "no device address for the cull scratch array");
+    gScratch[device] = p;
+}
+
 extern "C" int st_cull_run(st_ctx *c, st_cull_params *cp)
 {
     return st_guard(c, [&]() {
         CullRun R(c, cp);
-        ST_REQUIRE(cp->corr && cp->type && cp->sigmaR && cp->sigmaA && cp->offRS && cp->offAS, ST_ERR_ARG,
-                   "cull arrays");
-        R.put(R.D.offR, cp->offR, R.n);
-        R.put(R.D.offA, cp->offA, R.n);
-        R.put(R.D.corr, cp->corr, R.n);
-        R.put(R.D.type, cp->type, R.n);
-        R.st_data();
-        R.st_data();
-        R.st_data();
-        R.stats();
-        R.smooth();
-        if (cp->islandThresh > 0)
-            R.islands();
-        R.get(cp->offR, R.D.offR, R.n);
-        R.get(cp->offA, R.D.offA, R.n);
-        R.get(cp->type, R.D.type, R.n);
-        R.get(cp->sigmaR, R.D.sigmaR, R.n);
-        R.get(cp->sigmaA, R.D.sigmaA, R.n);
-        R.get(cp->offRS, R.D.offRS, R.n);
-        R.get(cp->offAS, R.D.offAS, R.n);
-        ST_HIP(hipStreamSynchronize(c->stream));
+        ST_REQUIRE(cull_arrays(cp), ST_ERR_ARG, "cull arrays");
+        cull_chain_host(c, cp, R);
+    });
+}
+
+extern "C" int st_cull_run_noalloc(st_ctx *c, st_cull_params *cp)
+{
+    return st_guard(c, [&]() {
+        ST_REQUIRE(cp && cp->nR > 0 && cp->nA > 0 && cp->offR && cp->offA && cull_arrays(cp), ST_ERR_ARG,
+                   "cull grids");
+        ST_REQUIRE((size_t)cp->nR * cp->nA <= kNoAllocCells, ST_ERR_ARG,
+                   "grid of " + std::to_string((size_t)cp->nR * cp->nA) + " points above the " +
+                       std::to_string(kNoAllocCells) + " the no-allocation cull holds");
+        // one call at a time per process: every context on a device shares
+        // the device's array
+        std::lock_guard<std::mutex> lk(gScratchMu);
+        void *base = (c->device >= 0 && c->device < kMaxDevices) ? gScratch[c->device] : nullptr;
+        ST_REQUIRE(base, ST_ERR_HIP, "cull scratch array not set up on this device");
+        CullRun R(c, cp, base, kNoAllocBytes);
+        cull_chain_host(c, cp, R);
     });
 }
 

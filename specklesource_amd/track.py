@@ -47,7 +47,7 @@ def load_library(path=LIB_PATH):
                                      C.POINTER(abi.TrackOut), C.POINTER(abi.TrackDiag)]
     lib.st_corr_track_fast.argtypes = lib.st_speckle_track.argtypes
     for name in ("st_cull_st_data", "st_cull_stats", "st_cull_smooth",
-                 "st_cull_islands", "st_cull_run"):
+                 "st_cull_islands", "st_cull_run", "st_cull_run_noalloc"):
         getattr(lib, name).argtypes = [vp, C.POINTER(abi.CullParams)]
     for name in ("st_cull_ls_data", "st_cull_ls_stats", "st_cull_ls_smooth",
                  "st_cull_ls_islands", "st_cull_ls_run"):
