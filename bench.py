@@ -207,6 +207,264 @@ def bench_batched(args):
     print(json.dumps(line))
 
 
+def flops_mfma_patch(w):
+    """Matrix-core work of one complex patch on the split-bf16 DFT path: six
+    w x w complex 2-D DFT contractions (two forward, four polyphase
+    inverses), 16 w^3 flops each as two complex w x w x w products, three
+    products for the hi/lo split."""
+    return 6 * 16 * w ** 3 * 3
+
+
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def _kernel_table(dev, table):
+    st = dev.last_stats()
+    for i in range(8):
+        nm = st.kernelNames[i].value.decode()
+        if nm:
+            ms, n = table.get(nm, (0.0, 0))
+            table[nm] = (ms + st.kernelMs[i], n + st.kernelLaunches[i])
+    return st
+
+
+def served_roofline(pairs, precision):
+    """Dominant-kernel roofline of the served workload, measured on one
+    context (no other slot running, so each launch's HIP-event duration is
+    its own): the same jobs run one after another with kernel timing on,
+    and the complex-match kernel's algorithmic work over all of them is
+    divided by its summed launch time."""
+    from specklesource_amd import abi, track
+    dev = track.Device(0)
+    pids = [dev.synth_pair(s) for s, _ in pairs]
+    dev.set_profiling(True)
+    table, work = {}, {}
+    for (s, p), pid in zip(pairs, pids):
+        q = abi.TrackParams.from_buffer_copy(p)
+        q.precision = precision
+        dev.speckle_track(pid, q)
+        st = _kernel_table(dev, table)
+        names = [st.kernelNames[i].value.decode() for i in range(8)]
+        kern = "k_cpx_mfma" if "k_cpx_mfma" in names else ("k_cpx_fused" if "k_cpx_fused" in names else "staged")
+        fl = flops_mfma_patch(p.wR) if kern == "k_cpx_mfma" else flops_complex_patch(p.wR)
+        f, n, b = work.get(kern, (0.0, 0, 0))
+        work[kern] = (f + fl * st.nCorr[0], n + st.nCorr[0], b + 16 * p.wR * p.wR * st.nCorr[0])
+    dev.set_profiling(False)
+    for pid in pids:
+        dev.release_pair(pid)
+    dev.close()
+    dom = max((k for k in table if k in ("k_cpx_fused", "k_cpx_mfma")), key=lambda k: table[k][0])
+    ms, launches = table[dom]
+    flops, patches, byts = work[dom]
+    mfma = dom == "k_cpx_mfma"
+    achieved = flops / (ms * 1e-3) / 1e12
+    peak = PEAK_BF16_TFLOPS if mfma else PEAK_FP32_TFLOPS
+    return {"bound": "mfma" if mfma else "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": None, "kernel": dom, "launches": launches,
+            "launch_ms_avg": ms / launches, "patches": patches,
+            "flops_total": flops, "algorithmic_bytes_total": byts,
+            "kernel_ms_per_pass": {k: v[0] for k, v in sorted(table.items(), key=lambda kv: -kv[1][0])},
+            "measured": "one context, the same %d jobs in sequence, HIP events on the library stream" % len(pairs)}
+
+
+def _oracle_strip(job):
+    """CPU baseline worker: the FP32 oracle on a strip of grid rows (the
+    parameter block comes as bytes: it holds no pointers for this
+    workload)."""
+    from oracle.pyoracle import Oracle
+    from specklesource_amd import abi
+    par, img1, img2 = job
+    if isinstance(par, (bytes, bytearray)):
+        par = abi.TrackParams.from_buffer_copy(par)
+    t = time.perf_counter()
+    r = Oracle("f32").speckle_track(par, img1, img2, diag=True)
+    dt = time.perf_counter() - t
+    return dt, int(sum((r.diag["peakI"][k] > -2000).sum() for k in range(3)))
+
+
+def _strips(pairs, dev, pids, rows_per_strip=2, per_pair=2):
+    """Bounded sample of the served workload: `per_pair` strips of grid rows
+    from every pair, with their images read back from HBM."""
+    from specklesource_amd import track
+    out = []
+    for (spec, par), pid in zip(pairs, pids):
+        for k in range(per_pair):
+            i0 = (par.nA * (k + 1)) // (per_pair + 1)
+            rows = min(rows_per_strip, par.nA - i0)
+            if rows <= 0:
+                continue
+            sub = track.default_params()
+            for f, _ in par._fields_:
+                setattr(sub, f, getattr(par, f))
+            sub.nA = rows
+            sub.aStart = par.aStart + i0 * par.deltaA
+            a0 = max(0, sub.aStart - 2 * par.wA - 64)
+            a1 = min(spec.na, sub.aStart + (rows - 1) * par.deltaA + 2 * par.wA + 64)
+            img1, img2 = dev.download_pair(pid, a0, a1 - a0, spec.nr)
+            sub.aStart -= a0
+            sub.aShiftPoly[0] = par.aShiftPoly[0] + par.aShiftPoly[2] * a0
+            sub.rShiftPoly[0] = par.rShiftPoly[0] + par.rShiftPoly[2] * a0
+            out.append((sub, img1, img2))
+    return out
+
+
+def cpu_cores():
+    """Host cores this process may use, at most 16 (one GPU's share of the
+    box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def served_cpu_and_accuracy(pairs, budget_s=12.0):
+    """CPU baseline (the FP32 oracle restatement, one core and all cores)
+    and accuracy of both GPU precisions on a bounded sample of the served
+    workload: two strips of two grid rows from every pair."""
+    import concurrent.futures as cf
+    from oracle.pyoracle import Oracle
+    from specklesource_amd import abi, track
+    from tests.helpers import compare_strack
+    dev = track.Device(0)
+    pids = [dev.synth_pair(s) for s, _ in pairs]
+    jobs = _strips(pairs, dev, pids)
+    for pid in pids:
+        dev.release_pair(pid)
+    # one core, in order, until the budget is spent
+    t1 = n1 = 0
+    for j in jobs:
+        dt, n = _oracle_strip(j)
+        t1 += dt
+        n1 += n
+        if t1 > budget_s:
+            break
+    cores = cpu_cores()
+    t = time.perf_counter()
+    nall = 0
+    with cf.ProcessPoolExecutor(max_workers=cores) as ex:
+        for dt, n in ex.map(_oracle_strip, [(bytes(p), a, b) for p, a, b in jobs]):
+            nall += n
+    tall = time.perf_counter() - t
+    base1 = {"value": n1 / t1, "unit": "patch correlations/s", "cores": 1, "kind": "port",
+             "sample": "%d patch correlations from strips of 2 grid rows of the 64 pairs, %.1f s single-thread"
+                       % (n1, t1)}
+    baseN = {"value": nall / tall, "unit": "patch correlations/s", "cores": cores, "kind": "port",
+             "sample": "%d patch correlations (2 strips of 2 grid rows from each of the 64 pairs) in %d "
+                       "processes, %.1f s wall" % (nall, cores, tall)}
+    # accuracy of both precisions on the same strips
+    o32, o64 = Oracle("f32"), Oracle("f64")
+    acc = {}
+    for prec_name, prec in (("fp32", abi.ST_PREC_FP32), ("mfma", abi.ST_PREC_MFMA)):
+        tot = agree = ties = unexpl = 0
+        er, ea, cr, sr = [], [], [], []
+        for sub, img1, img2 in jobs:
+            q = abi.TrackParams.from_buffer_copy(sub)
+            q.precision = prec
+            pid = dev.upload_pair(img1, img2)
+            g = dev.speckle_track(pid, q, diag=True)
+            dev.release_pair(pid)
+            r32 = o32.speckle_track(sub, img1, img2, diag=True)
+            r64 = o64.speckle_track(sub, img1, img2, diag=True)
+            rep, probs = compare_strack(q, g, r32, r64, lm_rtol=2e-2 if prec else 2e-3,
+                                        tie=5e-4 if prec else 2e-5, thresh_tol=1e-3 if prec else 1e-4)
+            tot += rep["attempts"]
+            agree += round(rep["peak_agree"] * rep["attempts"])
+            ties += rep["near_ties"]
+            unexpl += len(probs)
+            both = (g.type > 0) & (r64.type == g.type)
+            er += list((g.offR - r64.offR)[both].astype(np.float64))
+            ea += list((g.offA - r64.offA)[both].astype(np.float64))
+            cr += list((np.abs(g.corr.astype(np.float64) - r64.corr) / np.maximum(np.abs(r64.corr), 1e-12))[both])
+            for t in range(3):
+                ran = (g.diag["peakI"][t] > -1000) & (r64.diag["peakI"][t] == g.diag["peakI"][t])
+                if ran.any():
+                    a, b = g.diag["snr"][t][ran].astype(np.float64), r64.diag["snr"][t][ran].astype(np.float64)
+                    sr += list(np.abs(a - b) / np.maximum(np.abs(b), 1e-12))
+        acc[prec_name] = {
+            "sample_points": int(sum(j[0].nR * j[0].nA for j in jobs)), "attempts": tot,
+            "integer_peak_agreement_vs_fp32_oracle": agree / max(tot, 1), "near_ties": ties,
+            "unexplained_mismatches": unexpl,
+            "rms_offset_px_vs_fp64": [float(np.sqrt(np.mean(np.square(er)))) if er else None,
+                                      float(np.sqrt(np.mean(np.square(ea)))) if ea else None],
+            "corr_relerr_median_vs_fp64": float(np.median(cr)) if cr else None,
+            "corr_relerr_max_vs_fp64": float(np.max(cr)) if cr else None,
+            "snr_relerr_median_vs_fp64": float(np.median(sr)) if sr else None,
+            "snr_relerr_max_vs_fp64": float(np.max(sr)) if sr else None}
+    dev.close()
+    return base1, baseN, acc
+
+
+def served_passes(srv, pairs, pids, precision, warmup, steps, label):
+    from specklesource_amd import abi
+
+    def run_all():
+        t = time.perf_counter()
+        ids = []
+        for (_, p), pid in zip(pairs, pids):
+            q = abi.TrackParams.from_buffer_copy(p)
+            q.precision = precision
+            ids.append(srv.submit(q, pair=pid))
+        infos = [srv.wait(j)[1] for j in ids]
+        return time.perf_counter() - t, infos
+
+    for k in range(warmup):
+        run_all()
+        print("%s warmup %d done" % (label, k), file=sys.stderr, flush=True)
+    walls, lats, patches = [], [], 0
+    for k in range(steps):
+        wall, infos = run_all()
+        walls.append(wall)
+        lats += [i.latencyMs for i in infos]
+        patches += sum(i.patches for i in infos)
+        print("%s step %d: %.3f s" % (label, k, wall), file=sys.stderr, flush=True)
+    return patches / sum(walls), walls, lats
+
+
+def bench_served(args):
+    """The batched uneven workload through the serving layer: 64 pairs with
+    sides drawn from 1024-8192 px, windows from {16, 32, 64, 128} on a w/2
+    grid, affine offsets within +/-20 px, resident in HBM; one step submits
+    the 64 jobs to a 4-slot server and waits for all of them.  Both the FP32
+    path and the split-bf16 matrix-core path are timed; `value` is the
+    FP32 path (the faster one)."""
+    from specklesource_amd import abi, configs, track
+    pairs = configs.cfg3_pairs(n_pairs=args.pairs, scale=args.scale)
+    srv = track.Server(0, slots=args.slots, hbm_budget=200 << 30)
+    t0 = time.perf_counter()
+    pids = [srv.synth_pair(s) for s, _ in pairs]
+    print("generated %d pairs in HBM in %.1f s" % (len(pids), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    v32, w32, l32 = served_passes(srv, pairs, pids, abi.ST_PREC_FP32, args.warmup, args.steps, "fp32")
+    vmf, wmf, lmf = served_passes(srv, pairs, pids, abi.ST_PREC_MFMA, args.warmup, args.steps, "mfma")
+    srv.close()
+    ws = {}
+    for _, p in pairs:
+        ws[p.wR] = ws.get(p.wR, 0) + 1
+    line = {
+        "metric": "patch correlations/sec + p99 job latency, 1 MI355X, at <=0.02 px offset error",
+        "value": v32, "unit": "patch correlations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * sum(w32) / len(w32), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 (fp64 sub-pixel refinement and normalisation sums)",
+        "data": "synthetic (generated in HBM)",
+        "config": {"workload": "%d uneven pairs 1024-8192 px, w in {16,32,64,128}, grid w/2, search w/4, "
+                               "served on %d slots" % (len(pairs), args.slots), "windows": ws},
+        "latency_ms": {"p50": float(np.percentile(l32, 50)), "p99": float(np.percentile(l32, 99))},
+        "precisions": {
+            "fp32": {"value": v32, "latency_ms": {"p50": float(np.percentile(l32, 50)),
+                                                  "p99": float(np.percentile(l32, 99))}},
+            "mfma": {"value": vmf, "latency_ms": {"p50": float(np.percentile(lmf, 50)),
+                                                  "p99": float(np.percentile(lmf, 99))},
+                     "dtype": "bf16 hi+lo split products, fp32 accumulate (w = 16, 32; larger windows fp32)"},
+            "mfma_over_fp32": vmf / v32, "target_ratio": 3.0},
+    }
+    if not args.no_roofline:
+        line["roofline"] = served_roofline(pairs, abi.ST_PREC_FP32)
+        line["roofline_mfma"] = served_roofline(pairs, abi.ST_PREC_MFMA)
+    if not args.no_cpu:
+        line["cpu_baseline"], line["cpu_baseline_all_cores"], line["accuracy"] = served_cpu_and_accuracy(pairs)
+    print(json.dumps(line))
+
+
 def bench_cull(args):
     """cullst on the full-scene offset grid (821 x 571 points) with 5%
     uniform +/-30 px outliers: three median passes, plane-fit sigma,
@@ -296,7 +554,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=1, help="divide the scene sides (testing only)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["fullscene", "amponly", "strackw", "cull", "batched"], default="fullscene")
+    ap.add_argument("--workload", choices=["served", "fullscene", "amponly", "strackw", "cull", "batched"],
+                    default="served")
+    ap.add_argument("--no-roofline", action="store_true", help="served workload: skip the one-context roofline pass")
     ap.add_argument("--precision", choices=["fp32", "mfma"], default="fp32")
     ap.add_argument("--pairs", type=int, default=64)
     ap.add_argument("--slots", type=int, default=4)
@@ -323,6 +583,9 @@ def main():
         return
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "served":
+        bench_served(args)
         return
 
     from specklesource_amd import abi, configs, track

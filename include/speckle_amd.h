@@ -39,7 +39,7 @@ extern "C" {
 #define ST_ERR_UNSUPPORTED (-3) /* option not implemented on this path */
 #define ST_ERR_NOMEM (-4)       /* device allocation failed / over budget */
 #define ST_ERR_NODEV (-5)       /* no usable GPU */
-#define ST_ERR_CANCELLED (-6)   /* the job was cancelled (st_cancel) */
+#define ST_ERR_CANCELLED (-7)   /* the job was cancelled (st_cancel); -6 is speckle_io.h's ST_ERR_IO */
 
 /* SLC sample types (Strack/strack.h:10-16, speckleTrack.c:1221-1232) */
 #define ST_SAMPLE_C64 0  /* complex float32, fftw_complex {re, im} */
@@ -136,6 +136,9 @@ typedef struct {
      * oversampled search box: small values mark near-tied peaks whose
      * argmax may legitimately differ between FP32 implementations. */
     float *peakGap[3], *fineGap[3];
+    /* coarse peak over the mean |C| of the correlation surface (a new
+     * diagnostic: the reference computes none) */
+    float *snr[3];
 } st_track_diag;
 
 /* ---- contexts and image pairs ---------------------------------------- */
@@ -173,6 +176,15 @@ int st_pair_download(st_ctx *ctx, int32_t pair, int32_t row0, int32_t nrows,
  * Replaces speckleTrack (Strack/strack.h:170, speckleTrack.c:159-346). */
 int st_speckle_track(st_ctx *ctx, int32_t pair, const st_track_params *par,
                      st_track_out *out, st_track_diag *diag);
+
+/* Several strack calls (pairs, grids and parameters of their own) as one
+ * batch: the calls whose complex matches run on the fused FP32 kernel
+ * (windows 16, 32, 64) share one persistent launch per window class, the
+ * rest of each call runs as st_speckle_track does.  Each call's outputs are
+ * the ones it gives alone.  pars[k] and outs[k] describe call k; no
+ * diagnostics.  No counterpart in the reference (one call per process). */
+int st_speckle_track_batch(st_ctx *ctx, int32_t n, const int32_t *pairs, const st_track_params *pars,
+                           st_track_out *outs);
 
 /* strackw: edge-padded normalised cross-correlation tracker.
  * Replaces corrTrackFast (Strackw/strackw.h:1, corrTrackFast.c:89-231). */

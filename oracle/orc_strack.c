@@ -57,6 +57,7 @@ typedef struct {
     ocplx *cFastOver;        /* 96 x 96 */
     float *azd;              /* output aZDefocus grid */
     float gapCoarse, gapFine;
+    double snr;              /* coarse peak over the mean surface magnitude */
     float sigRgLast, sigAzLast;
 } orc_st;
 
@@ -417,12 +418,15 @@ static void find_cpeak(orc_st *s, oreal *mag, const ocplx *C, int W_R, int W_A, 
             j1++;
         }
     }
-    /* runner-up for the near-tie diagnostic */
-    double second = 0.0;
-    for (int k = 0; k < W_R * W_A; k++)
+    /* runner-up for the near-tie diagnostic; SNR = peak / mean |C| */
+    double second = 0.0, sum = 0.0;
+    for (int k = 0; k < W_R * W_A; k++) {
+        sum += (double)mag[k];
         if (k != bi * W_R + bj && mag[k] > second)
             second = mag[k];
+    }
     s->gapCoarse = best > 0 ? (float)((best - second) / best) : 0.f;
+    s->snr = sum > 0 ? best * (double)(W_R * W_A) / sum : 0.0;
     *cMax = best;
     *iMax = bi;
     *jMax = bj;
@@ -1148,6 +1152,7 @@ int orc_speckle_track(const st_track_params *par, const void *img1, int nr1, int
                     if (dg->sigAz[t]) dg->sigAz[t][o] = 0;
                     if (dg->peakGap[t]) dg->peakGap[t][o] = 0;
                     if (dg->fineGap[t]) dg->fineGap[t][o] = 0;
+                    if (dg->snr[t]) dg->snr[t][o] = 0;
                 }
             }
             if (bounds_check(s, r1, a1, r2, a2, par->wR, par->wA) && maskVal > 0 && par->noComplex == 0) {
@@ -1167,6 +1172,7 @@ int orc_speckle_track(const st_track_params *par, const void *img1, int nr1, int
                         if (dg->sigRg[0]) dg->sigRg[0][o] = s->sigRgLast;
                         if (dg->sigAz[0]) dg->sigAz[0][o] = s->sigAzLast;
                         if (dg->peakGap[0]) dg->peakGap[0][o] = s->gapCoarse;
+                        if (dg->snr[0]) dg->snr[0][o] = (float)s->snr;
                         if (dg->fineGap[0]) dg->fineGap[0][o] = s->gapFine;
                     }
                 }
@@ -1196,6 +1202,7 @@ int orc_speckle_track(const st_track_params *par, const void *img1, int nr1, int
                             if (dg->sigRg[t]) dg->sigRg[t][o] = s->sigRgLast;
                             if (dg->sigAz[t]) dg->sigAz[t][o] = s->sigAzLast;
                             if (dg->peakGap[t]) dg->peakGap[t][o] = s->gapCoarse;
+                            if (dg->snr[t]) dg->snr[t][o] = (float)s->snr;
                             if (dg->fineGap[t]) dg->fineGap[t][o] = s->gapFine;
                         }
                     } else
@@ -1359,11 +1366,12 @@ void orc_corr_surface(const ocplx *p1, const ocplx *p2, int w, int azShift, orea
 
 /* findCPeak alone on a W_A x W_R complex surface in the inverse
  * transform's (unshifted) layout; mag receives the fftshifted |C|. */
-void orc_find_cpeak(const ocplx *C, int W_R, int W_A, oreal *mag, int *iMax, int *jMax, double *cMax)
+void orc_find_cpeak(const ocplx *C, int W_R, int W_A, oreal *mag, int *iMax, int *jMax, double *cMax, double *snr)
 {
     orc_st s;
     memset(&s, 0, sizeof(s));
     find_cpeak(&s, mag, C, W_R, W_A, iMax, jMax, cMax);
+    *snr = s.snr;
 }
 
 /* getShifts alone: the initial offset at (range, azimuth) from the grids

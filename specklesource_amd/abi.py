@@ -12,7 +12,7 @@ ST_ERR_HIP = -2
 ST_ERR_UNSUPPORTED = -3
 ST_ERR_NOMEM = -4
 ST_ERR_NODEV = -5
-ST_ERR_CANCELLED = -6
+ST_ERR_CANCELLED = -7
 
 ST_SAMPLE_C64 = 0
 ST_SAMPLE_CI16 = 1
@@ -83,6 +83,7 @@ class TrackDiag(C.Structure):
         ("cmax", _d * 3),
         ("sigRg", _f * 3), ("sigAz", _f * 3),
         ("peakGap", _f * 3), ("fineGap", _f * 3),
+        ("snr", _f * 3),
     ]
 
 

@@ -316,6 +316,24 @@ extern "C" int st_speckle_track(st_ctx *c, int32_t pair, const st_track_params *
     });
 }
 
+extern "C" int st_speckle_track_batch(st_ctx *c, int32_t n, const int32_t *pairs, const st_track_params *pars,
+                                      st_track_out *outs)
+{
+    return st_guard(c, [&]() {
+        ST_REQUIRE(n >= 1 && pairs && pars && outs, ST_ERR_ARG, "null argument");
+        std::vector<StPair *> prs(n);
+        std::vector<const st_track_params *> ps(n);
+        std::vector<st_track_out *> os(n);
+        for (int k = 0; k < n; k++) {
+            ST_REQUIRE(outs[k].offR && outs[k].offA && outs[k].corr && outs[k].type, ST_ERR_ARG, "null output");
+            prs[k] = &c->pair(pairs[k]);
+            ps[k] = &pars[k];
+            os[k] = &outs[k];
+        }
+        st_speckle_track_batch_impl(c, n, prs.data(), ps.data(), os.data());
+    });
+}
+
 extern "C" int st_corr_track_fast(st_ctx *c, int32_t pair, const st_track_params *par, st_track_out *out,
                                   st_track_diag *diag)
 {

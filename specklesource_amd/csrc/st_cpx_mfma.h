@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1
         }
         mf_sync<WPP>();
         // ---------------- S4: C = B G   (table m-block x n-block) -> |C|, top-2
-        Top2 best{0.0f, 0x7fffffff, 0.0f};
+        Top2 best{0.0f, 0x7fffffff, 0.0f, 0.0f};
         {
             constexpr int NTT = WW / 16, ND = WW / 16, NU = NTT * ND, PER = NU / WPP;
             MfTab<W / 16> tb;
@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1
                     const int qd = (I < W) ? (J < W ? 0 : 3) : (J < W ? 2 : 1);
                     const int rank = (i0 * W + j0) * 4 + qd;
                     if (v > 0.0f) {
-                        Top2 c{v, rank, 0.0f};
+                        Top2 c{v, rank, 0.0f, v};
                         best = top2_combine(best, c);
                     } else {
                         best.v2 = fmaxf(best.v2, v);
@@ -341,6 +341,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1
             b.v1 = __shfl_xor(best.v1, o);
             b.r1 = __shfl_xor(best.r1, o);
             b.v2 = __shfl_xor(best.v2, o);
+            b.sum = __shfl_xor(best.sum, o);
             best = top2_combine(best, b);
         }
         if constexpr (WPP > 1) {
@@ -348,12 +349,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1
             if (lane == 0) {
                 shd[wv][0] = best.v1;
                 shd[wv][1] = best.v2;
+                shd[wv][4] = best.sum;
                 shi[wv][0] = best.r1;
             }
             __syncthreads();
-            best = Top2{(float)shd[0][0], shi[0][0], (float)shd[0][1]};
+            best = Top2{(float)shd[0][0], shi[0][0], (float)shd[0][1], (float)shd[0][4]};
             for (int k = 1; k < WPP; k++)
-                best = top2_combine(best, Top2{(float)shd[k][0], shi[k][0], (float)shd[k][1]});
+                best = top2_combine(best, Top2{(float)shd[k][0], shi[k][0], (float)shd[k][1], (float)shd[k][4]});
         }
         mf_sync<WPP>();
         PeakRes *res = &C.res[it];
@@ -368,6 +370,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1
         }
         if (tp == 0) {
             res->gap = raw > 0 ? (raw - best.v2) / raw : 0.f;
+            res->snr = best.sum > 0.0f ? best.v1 * (float)(WW * WW) / best.sum : 0.0f;
             res->lmOk = 0;
             res->fi = res->fj = -1;
             res->fineMax = 0;

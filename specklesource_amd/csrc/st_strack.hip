@@ -17,7 +17,9 @@
 #include <string.h>
 
 #include <map>
+#include <memory>
 #include <mutex>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -79,7 +81,7 @@ struct Pts {
     float *corr, *rShift1, *aShift1, *azd;
     int *peakI, *peakJ, *fineI, *fineJ; // 3 x npts
     double *cmax;
-    float *sigRg, *sigAz, *peakGap, *fineGap;
+    float *sigRg, *sigAz, *peakGap, *fineGap, *snr;
     int *counts;  // nComplex, nAmp, nAmpL, nFail, list sizes...
 };
 
@@ -356,7 +358,7 @@ __global__ void k_geometry(KPar P, Pts S, const int *hostR2, const int *hostA2, 
         const size_t o = (size_t)t * npts + pt;
         S.peakI[o] = S.peakJ[o] = S.fineI[o] = S.fineJ[o] = -2000;
         S.cmax[o] = 0;
-        S.sigRg[o] = S.sigAz[o] = S.peakGap[o] = S.fineGap[o] = 0.f;
+        S.sigRg[o] = S.sigAz[o] = S.peakGap[o] = S.fineGap[o] = S.snr[o] = 0.f;
     }
     const bool ok = in_bounds(r1, a1, P.wR, P.wA, P.im.nr1, P.im.na1) && in_bounds(r2, a2, P.wR, P.wA, P.im.nr2, P.im.na2);
     if (ok && mv > 0 && !P.noComplex) {
@@ -378,6 +380,7 @@ struct CpxBatch {
     uint8_t *have;    // [B]
     int *az;          // [B]
     double *p;        // [B][2]
+    const int *list;  // fused and matrix-core paths: item -> grid point
 };
 
 __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *list, int off, int B, CpxBatch C)
@@ -481,8 +484,8 @@ struct GaussArg {
 // in -gauss mode, the 2-D Gaussian fit), and the samples of the two 1-D
 // Gaussian fits, from the fftshifted magnitude surface M.
 template <bool GAUSS>
-__device__ void peak_finish(const float *M, int W, int mode, int I, int J, float raw, float sec, PeakRes *out,
-                            const GaussArg &G, int item)
+__device__ void peak_finish(const float *M, int W, int mode, int I, int J, float raw, float sec, float snr,
+                            PeakRes *out, const GaussArg &G, int item)
 {
     __shared__ ArgMaxD shd[NT / 64];
     __shared__ double shdd[NT / 64];
@@ -490,6 +493,7 @@ __device__ void peak_finish(const float *M, int W, int mode, int I, int J, float
     __shared__ double2 T[NFAST * kFine];
     PeakRes r;
     r.gap = raw > 0 ? (raw - sec) / raw : 0.f;
+    r.snr = snr;
     r.lmOk = 0;
     r.fi = r.fj = -1;
     r.fineMax = 0;
@@ -722,6 +726,7 @@ __global__ void k_cpx_finalize(KPar P, Pts S, const int *list, int off, int B, C
     S.sigAz[pt] = r.lmOk ? sAz : 0.f;
     S.peakGap[pt] = r.gap;
     S.fineGap[pt] = r.fineGap;
+    S.snr[pt] = r.snr;
     (void)npts;
 }
 
@@ -942,6 +947,7 @@ __global__ void k_amp_finalize(KPar P, Pts S, const int *list, int off, int B, i
     S.sigAz[o] = r.lmOk ? sAz : 0.f;
     S.peakGap[o] = r.gap;
     S.fineGap[o] = r.fineGap;
+    S.snr[o] = r.snr;
     (void)W;
 }
 
@@ -989,11 +995,13 @@ struct Top2 {
     float v1;
     int r1;
     float v2;
+    float sum;  // sum of |C| over the values seen (the SNR diagnostic's mean)
 };
 
 __device__ inline Top2 top2_combine(Top2 a, Top2 b)
 {
     Top2 o;
+    o.sum = a.sum + b.sum;
     if (b.v1 > a.v1 || (b.v1 == a.v1 && b.r1 < a.r1)) {
         o.v1 = b.v1;
         o.r1 = b.r1;
@@ -1015,6 +1023,7 @@ __device__ inline Top2 wg_top2(Top2 a, Top2 *sh)
         b.v1 = __shfl_xor(a.v1, o);
         b.r1 = __shfl_xor(a.r1, o);
         b.v2 = __shfl_xor(a.v2, o);
+        b.sum = __shfl_xor(a.sum, o);
         a = top2_combine(a, b);
     }
     if constexpr (NTH <= 64) {
@@ -1286,7 +1295,7 @@ __global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R
     __syncthreads();
     stfft::lds_fft<kStElems, NT>(ldsx, W, nrow, Pi, +1);
     float *M = mag + (size_t)b * W * W;
-    Top2 best{0.0f, 0x7fffffff, 0.0f};
+    Top2 best{0.0f, 0x7fffffff, 0.0f, 0.0f};
     for (int t = threadIdx.x; t < cnt; t += NT) {
 #pragma clang fp contract(off)
         const int i = stfft::fdiv(t, dW), m = t - i * W, n = n0 + i;
@@ -1299,7 +1308,7 @@ __global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R
         const int q = (I < H) ? (J < H ? 0 : 3) : (J < H ? 2 : 1);
         const int rank = (i0 * H + j0) * 4 + q;
         if (v > 0.0f)
-            best = top2_combine(best, Top2{v, rank, 0.0f});
+            best = top2_combine(best, Top2{v, rank, 0.0f, v});
         else
             best.v2 = fmaxf(best.v2, v);
     }
@@ -1327,7 +1336,8 @@ __global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, 
         J = (q == 0 || q == 2) ? j : j + H;
         raw = best.v1;
     }
-    peak_finish<GAUSS>(mag + (size_t)b * W * W, W, mode, I, J, raw, best.v2, res + b, G, b);
+    const float snr = best.sum > 0.0f ? best.v1 * (float)(W * W) / best.sum : 0.0f;
+    peak_finish<GAUSS>(mag + (size_t)b * W * W, W, mode, I, J, raw, best.v2, snr, res + b, G, b);
 }
 
 // ---------------------------------------------------------------------
@@ -1764,7 +1774,7 @@ __global__ void __launch_bounds__(NT) k_amp_rowmag(const float2 *T, int W, int R
     const int b = blockIdx.y, n0 = blockIdx.x * R, nrow = imin(R, W - n0), npair = (nrow + 1) / 2;
     if (state[b] != 2) {
         if (threadIdx.x == 0)
-            part[(size_t)b * gridDim.x + blockIdx.x] = Top2{0.0f, 0x7fffffff, 0.0f};
+            part[(size_t)b * gridDim.x + blockIdx.x] = Top2{0.0f, 0x7fffffff, 0.0f, 0.0f};
         return;
     }
     const float2 *Tb = T + ((size_t)b * W + n0) * W;
@@ -1796,7 +1806,7 @@ __global__ void __launch_bounds__(NT) k_amp_rowmag(const float2 *T, int W, int R
     __syncthreads();
     stfft::lds_fft<kStElems, NT>(ldsx, W, npair, Pi, +1);
     float *M = mag + (size_t)b * W * W;
-    Top2 best{0.0f, 0x7fffffff, 0.0f};
+    Top2 best{0.0f, 0x7fffffff, 0.0f, 0.0f};
     for (int t = threadIdx.x; t < nrow * W; t += NT) {
         const int i = stfft::fdiv(t, dW), m = t - i * W, n = n0 + i;
         const float2 z = ldsx[(i >> 1) * W + m];
@@ -1807,7 +1817,7 @@ __global__ void __launch_bounds__(NT) k_amp_rowmag(const float2 *T, int W, int R
         const int q = (I < H) ? (J < H ? 0 : 3) : (J < H ? 2 : 1);
         const int rank = (i0 * H + j0) * 4 + q;
         if (v > 0.0f)
-            best = top2_combine(best, Top2{v, rank, 0.0f});
+            best = top2_combine(best, Top2{v, rank, 0.0f, v});
         else
             best.v2 = fmaxf(best.v2, v);
     }
@@ -1858,14 +1868,17 @@ struct FusedLanes {
 // SIMD the register allocation must allow: 2 (at w = 64 this caps the
 // allocation at 256 VGPRs with a few spilled values instead of one wave
 // per SIMD).
+//
+// One launch serves the complex attempts of njobs tracking calls (pairs
+// and grids of their own, one window class): items [jobStart[j],
+// jobStart[j + 1]) are job j's, and every item reads its job's parameter,
+// point-state and batch blocks.
 template <int W, int STYPE, int BASE, int OCC>
 __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgpu_waves_per_eu(OCC, 8))) k_cpx_fused(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
-                                                     const int *list, int count, const CpxBatch *__restrict__ Cp,
-                                                     float *magScratch)
+                                                     const int *__restrict__ jobStart, int njobs,
+                                                     const CpxBatch *__restrict__ Cp, float *magScratch)
 {
-    const KPar &P = *Pp;
-    const Pts &S = *Sp;
-    const CpxBatch &C = *Cp;
+    const int count = jobStart[njobs];
     using namespace ::fftreg;
     constexpr int L = FusedLanes<W>::L, H = W / L, NTH = 2 * L * W, WW = 2 * W, BUF = FusedLanes<W>::BUF;
     using FL = FusedLanes<W>;
@@ -1882,9 +1895,22 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
     const int G = gridDim.x, Q = G / 8;
     const int base = (G % 8 == 0) ? (blockIdx.x % 8) * Q + blockIdx.x / 8 : blockIdx.x;
     for (int k0 = 0; k0 < count; k0 += G) {
-        const int it = k0 + base;
-        if (it >= count)
+        const int itg = k0 + base;
+        if (itg >= count)
             break;
+        // the item's job (workgroup-uniform: a scalar search), its index there
+        int jb = 0;
+        for (int lo = 0, hi = njobs - 1; lo < hi;) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobStart[mid] <= itg)
+                lo = jb = mid;
+            else
+                hi = mid - 1;
+        }
+        const int it = itg - jobStart[jb];
+        const KPar &P = Pp[jb];
+        const Pts &S = Sp[jb];
+        const CpxBatch &C = Cp[jb];
         // Lane indices are re-derived behind an opaque copy every item so
         // the compiler does not hoist per-lane addresses out of the work
         // loop (they would stay live in registers for the whole kernel).
@@ -1895,7 +1921,7 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         const int h = t & (L - 1);
         const int blk = LF::blk(h);
         float *M = magScratch + (size_t)blockIdx.x * WW * WW;
-        const int pt = list[it];
+        const int pt = C.list[it];
         const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
         if (!C.have[it])
             continue;
@@ -2006,7 +2032,7 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         const int krow = ((r - az) % W + W) % W;
         const int ks = krow < W / 2 ? krow : krow - W;
         float2 *Z = lds + (size_t)g * BUF;
-        Top2 best{0.0f, 0x7fffffff, 0.0f};
+        Top2 best{0.0f, 0x7fffffff, 0.0f, 0.0f};
         // exp(i pi (ks pp + ls q) / w) = row factor (per lane, per pp) x
         // column factor (per element, q = g only): the column factor is
         // applied once, before the component loop
@@ -2059,7 +2085,7 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
                 const int q = (I < W) ? (J < W ? 0 : 3) : (J < W ? 2 : 1);
                 const int rank = (i0 * W + j0) * 4 + q;
                 if (v > 0.0f) {
-                    Top2 c{v, rank, 0.0f};
+                    Top2 c{v, rank, 0.0f, v};
                     best = top2_combine(best, c);
                 } else {
                     best.v2 = fmaxf(best.v2, v);
@@ -2080,6 +2106,7 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         }
         if (t == 0) {
             res->gap = raw > 0 ? (raw - best.v2) / raw : 0.f;
+            res->snr = best.sum > 0.0f ? best.v1 * (float)(WW * WW) / best.sum : 0.0f;
             res->lmOk = 0;
             res->fi = res->fj = -1;
             res->fineMax = 0;
@@ -2335,44 +2362,56 @@ T *upload(st_ctx *ctx, const T *h, size_t n)
 #include "st_cpx_mfma.h"
 
 template <int W, int STYPE, int BASE, int OCC>
-void launch_fused_kernel_o(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
-                           const CpxBatch *dC, size_t lds)
+void launch_fused_kernel_o(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *dStart, int njobs,
+                           int total, const CpxBatch *dC, size_t lds)
 {
     int perCU = 0, ncu = 0;
     constexpr int NTH = 2 * FusedLanes<W>::L * W;
     ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, STYPE, BASE, OCC>, NTH, lds));
     ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    const int grid = std::max(1, std::min(count, std::max(1, perCU) * ncu));
+    const int grid = std::max(1, std::min(total, std::max(1, perCU) * ncu));
     float *mag = ctx->arena.take<float>((size_t)grid * 4 * W * W);
     ctx->time_begin(slot);
-    hipLaunchKernelGGL((k_cpx_fused<W, STYPE, BASE, OCC>), dim3(grid), dim3(NTH), lds, ctx->stream, dP, dS, list,
-                       count, dC, mag);
+    hipLaunchKernelGGL((k_cpx_fused<W, STYPE, BASE, OCC>), dim3(grid), dim3(NTH), lds, ctx->stream, dP, dS, dStart,
+                       njobs, dC, mag);
     ctx->time_end(slot);
     ST_HIP(hipGetLastError());
 }
 
 template <int W, int STYPE, int BASE>
-void launch_fused_kernel(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
-                         const CpxBatch *dC, size_t lds)
+void launch_fused_kernel(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *dStart, int njobs,
+                         int total, const CpxBatch *dC, size_t lds)
 {
-    launch_fused_kernel_o<W, STYPE, BASE, 2>(ctx, slot, dP, dS, list, count, dC, lds);
+    launch_fused_kernel_o<W, STYPE, BASE, 2>(ctx, slot, dP, dS, dStart, njobs, total, dC, lds);
 }
 
+// The complex attempts of one or more tracking calls of window class W.
+struct FusedJob {
+    KPar P;
+    Pts S;
+    CpxBatch C;  // C.list: the call's items
+    int count;   // its complex attempts
+};
+
 template <int W, int STYPE>
-void launch_cpx_fused_t(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *list, int count,
-                        const CpxBatch *dC, int base)
+void launch_cpx_fused_t(st_ctx *ctx, int slot, const std::vector<FusedJob> &jobs, const KPar *dP, const Pts *dS,
+                        const CpxBatch *dC, const int *dStart, int total, int base)
 {
     const size_t lds = std::max<size_t>((size_t)2 * FusedLanes<W>::BUF * sizeof(float2), 8192);
     const int prepSlot = ctx->kernel_slot("k_cpx_prep");
     ctx->time_begin(prepSlot);
-    hipLaunchKernelGGL((k_cpx_prep_cols<W, STYPE>), dim3((count + 64 / W - 1) / (64 / W)), dim3(64), 0, ctx->stream, dP,
-                       dS, list, count, dC);
+    for (size_t j = 0; j < jobs.size(); j++) {
+        const int count = jobs[j].count;
+        hipLaunchKernelGGL((k_cpx_prep_cols<W, STYPE>), dim3((count + 64 / W - 1) / (64 / W)), dim3(64), 0,
+                           ctx->stream, dP + j, dS + j, jobs[j].C.list, count, dC + j);
+    }
     ctx->time_end(prepSlot);
+    const int n = (int)jobs.size();
     switch (base) {
-    case 0: launch_fused_kernel<W, STYPE, 0>(ctx, slot, dP, dS, list, count, dC, lds); break;
-    case 1: launch_fused_kernel<W, STYPE, 1>(ctx, slot, dP, dS, list, count, dC, lds); break;
-    case 2: launch_fused_kernel<W, STYPE, 2>(ctx, slot, dP, dS, list, count, dC, lds); break;
-    default: launch_fused_kernel<W, STYPE, 3>(ctx, slot, dP, dS, list, count, dC, lds); break;
+    case 0: launch_fused_kernel<W, STYPE, 0>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
+    case 1: launch_fused_kernel<W, STYPE, 1>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
+    case 2: launch_fused_kernel<W, STYPE, 2>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
+    default: launch_fused_kernel<W, STYPE, 3>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
     }
 }
 
@@ -2396,23 +2435,43 @@ int fused_per_cu(int stype)
     return fused_per_cu_o<W, 2>(stype);
 }
 
-template <int W>
-void launch_cpx_fused(st_ctx *ctx, int slot, const KPar &P, const Pts &S, const int *list, int count,
-                      const CpxBatch &C)
+int fused_per_cu_w(int w, int stype)
 {
-    // parameter blocks live in device memory: read through the scalar cache
-    // instead of occupying SGPRs for the whole kernel
-    KPar *dP = ctx->arena.take<KPar>(1);
-    Pts *dS = ctx->arena.take<Pts>(1);
-    CpxBatch *dC = ctx->arena.take<CpxBatch>(1);
-    ST_HIP(hipMemcpyAsync(dP, &P, sizeof(KPar), hipMemcpyHostToDevice, ctx->stream));
-    ST_HIP(hipMemcpyAsync(dS, &S, sizeof(Pts), hipMemcpyHostToDevice, ctx->stream));
-    ST_HIP(hipMemcpyAsync(dC, &C, sizeof(CpxBatch), hipMemcpyHostToDevice, ctx->stream));
-    const int base = (P.baseFlag != 0 ? 1 : 0) | (P.intOn ? 2 : 0);
-    if (P.im.stype == ST_SAMPLE_C64)
-        launch_cpx_fused_t<W, ST_SAMPLE_C64>(ctx, slot, dP, dS, list, count, dC, base);
+    return w == 16 ? fused_per_cu<16>(stype) : (w == 32 ? fused_per_cu<32>(stype) : fused_per_cu<64>(stype));
+}
+
+// One persistent launch over the complex attempts of every job (all of
+// window class W, one sample type and correction set).  The parameter
+// blocks live in device memory, read through the scalar cache instead of
+// occupying SGPRs for the whole kernel.
+template <int W>
+void launch_cpx_fused(st_ctx *ctx, int slot, const std::vector<FusedJob> &jobs)
+{
+    const int n = (int)jobs.size();
+    std::vector<KPar> hp(n);
+    std::vector<Pts> hs(n);
+    std::vector<CpxBatch> hc(n);
+    std::vector<int> start(n + 1, 0);
+    for (int j = 0; j < n; j++) {
+        hp[j] = jobs[j].P;
+        hs[j] = jobs[j].S;
+        hc[j] = jobs[j].C;
+        start[j + 1] = start[j] + jobs[j].count;
+    }
+    KPar *dP = ctx->arena.take<KPar>(n);
+    Pts *dS = ctx->arena.take<Pts>(n);
+    CpxBatch *dC = ctx->arena.take<CpxBatch>(n);
+    int *dStart = ctx->arena.take<int>(n + 1);
+    ST_HIP(hipMemcpyAsync(dP, hp.data(), sizeof(KPar) * n, hipMemcpyHostToDevice, ctx->stream));
+    ST_HIP(hipMemcpyAsync(dS, hs.data(), sizeof(Pts) * n, hipMemcpyHostToDevice, ctx->stream));
+    ST_HIP(hipMemcpyAsync(dC, hc.data(), sizeof(CpxBatch) * n, hipMemcpyHostToDevice, ctx->stream));
+    ST_HIP(hipMemcpyAsync(dStart, start.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, ctx->stream));
+    const KPar &P0 = jobs[0].P;
+    const int base = (P0.baseFlag != 0 ? 1 : 0) | (P0.intOn ? 2 : 0);
+    if (P0.im.stype == ST_SAMPLE_C64)
+        launch_cpx_fused_t<W, ST_SAMPLE_C64>(ctx, slot, jobs, dP, dS, dC, dStart, start[n], base);
     else
-        launch_cpx_fused_t<W, ST_SAMPLE_CI16>(ctx, slot, dP, dS, list, count, dC, base);
+        launch_cpx_fused_t<W, ST_SAMPLE_CI16>(ctx, slot, jobs, dP, dS, dC, dStart, start[n], base);
 }
 
 // bf16 of a float, round to nearest even (host side of the operand split)
@@ -2558,500 +2617,652 @@ int read_count(st_ctx *ctx, int *dcounts, int idx)
 
 }  // namespace
 
-void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, st_track_out *out,
-                           st_track_diag *diag, st_cull_params *cull)
-{
-    ST_REQUIRE(par.wR == par.wA && par.wRa == par.wAa, ST_ERR_UNSUPPORTED, "non-square windows");
-    ST_REQUIRE(par.wR >= 8 && par.wRa >= 8 && !(par.wR & 1) && !(par.wRa & 1), ST_ERR_ARG, "window sizes");
-    ST_REQUIRE(stfft::supported_size(par.wR) && stfft::supported_size(2 * par.wR) &&
-                   stfft::supported_size(par.wRa * LA) && stfft::supported_size(4 * par.wRa),
-               ST_ERR_UNSUPPORTED, "window size not supported by the FFT engine");
-    ST_REQUIRE(par.nR > 0 && par.nA > 0 && par.scaleFactor > 0, ST_ERR_ARG, "grid");
-    ST_REQUIRE(par.polyShift || (par.initDr && par.initDa && par.initNr > 0 && par.initNa > 0), ST_ERR_ARG,
-               "initial offsets: polyShift=0 needs initDr/initDa");
-    hipStream_t st = ctx->stream;
-    init_dirichlet(ctx);
-    ctx->stats_reset();
-    hipEvent_t *sev = ctx->ev + 64;  // stage boundary events
-    ST_HIP(hipEventRecord(sev[0], st));
-    // the matrix-core complex match covers windows 16 and 32 without the
-    // baseline ramp; other cases run the FP32 paths
-    const bool intOn = par.intf != nullptr && !par.noComplex;
-    // -gauss fits its peak on the staged path's magnitude surfaces
-    const bool gauss = par.gaussFlag != 0;
-    const bool mfmaPath = par.precision == ST_PREC_MFMA && (par.wR == 16 || par.wR == 32) && !par.baseFlag &&
-                          !intOn && !gauss && !staged_only();
-    const bool fusedPath = (par.wR == 16 || par.wR == 32 || par.wR == 64) && !gauss && !staged_only();
-    const int kPeak = ctx->kernel_slot(mfmaPath ? "k_cpx_mfma" : (fusedPath ? "k_cpx_fused" : "peak_refine"));
-    const int kAmpPeak = ctx->kernel_slot("amp_peak");
-    const int kGather = ctx->kernel_slot("cpx_gather"), kFft = ctx->kernel_slot("fft2d"),
-              kAmpDet = ctx->kernel_slot("amp_detect");
-    const int npts = par.nR * par.nA;
-    const int w = par.wR;
+namespace {
 
-    // per-point state + inputs
+// One strack call on the device: its parameters, the per-point state it
+// carves from the context arena and the stages that run for it alone.  A
+// batch of calls carves all of them first; the calls whose complex
+// matches run on the fused kernel then share one launch per window class.
+struct StrackJob {
+    st_ctx *ctx = nullptr;
+    StPair pr;
+    const st_track_params *pp = nullptr;
+    st_track_out *out = nullptr;
+    st_track_diag *diag = nullptr;
+    st_cull_params *cull = nullptr;
+    bool intOn = false, gauss = false, mfmaPath = false, fusedPath = false;
+    int kPeak = 0, kAmpPeak = 0, kGather = 0, kFft = 0, kAmpDet = 0;
+    int npts = 0, w = 0, intPs = 0;
     std::vector<float> hann;
-    host_hanning(par.wA, par.wR, hann);
     std::vector<int> hr2, ha2;
-    if (!par.polyShift)
-        host_positions(par, hr2, ha2);
-    size_t fixed = (size_t)npts * 160 + hann.size() * 4 + (hr2.size() * 8) + 4096 +
-                   ((size_t)npts / kCarryBlk + 1) * 4 + 256;
-    if (par.mask)
-        fixed += (size_t)par.maskNr * par.maskNa;
-    if (par.baseFlag)
-        fixed += (size_t)pr.nr1 * 24;
-    int intPs = 0;
-    if (intOn) {
-        ST_REQUIRE(par.intNr > 0 && par.intNa > 0 && par.intNrl > 0 && par.intNal > 0, ST_ERR_ARG,
-                   "interferogram size and looks");
-        intPs = std::max(std::max(par.wA / par.intNal, par.wR / par.intNrl), 8);
-        ST_REQUIRE(intPs <= kIntMaxPs, ST_ERR_UNSUPPORTED,
-                   "interferogram patch above 64 x 64 samples (window / looks)");
-        ST_REQUIRE((intPs * par.intNal) / 2 >= par.wA / 2 && (intPs * par.intNrl) / 2 >= par.wR / 2, ST_ERR_ARG,
-                   "window not covered by the oversampled interferogram patch (window not a multiple of the looks)");
-        ST_REQUIRE((long)intPs * par.intNal <= 32768 && (long)intPs * par.intNrl <= 32768, ST_ERR_ARG,
-                   "interferogram looks");
-        fixed += (size_t)par.intNr * par.intNa * 8 + (size_t)npts * sizeof(IntSet) + 512;
-    }
-    // the cull chain works on the output grids in the same arena
-    if (cull) {
-        ST_REQUIRE(cull->nR == par.nR && cull->nA == par.nA && cull->offR == out->offR && cull->offA == out->offA &&
-                       cull->corr == out->corr && cull->type == out->type,
-                   ST_ERR_ARG, "cull grids must be the tracking outputs");
-        fixed += cull_device_bytes((size_t)npts);
-    }
-    // batch sizes: a budget the device cannot hold is halved until it fits
-    const size_t perCpx = (size_t)w * w * (16 + 32 + 32 + 16) + 2048;
-    int ws_max = par.wAa * (par.maxTries >= 2 ? LA : 1);
-    const size_t perAmp = (size_t)ws_max * ws_max * 128 + (size_t)2 * ws_max * 96 + 4096;
-    size_t budget = ctx->scratchBudget;
-    const size_t kMaxBatch = 8192;  // keeps 2 x batch inside the 65535 grid-z limit of the transposes
+    size_t fixed = 0;
+    size_t perCpx = 0, perAmp = 0;
     int Bc = 1, Ba = 1;
-    for (;;) {
-        Bc = (int)std::max<size_t>(1, std::min<size_t>({(size_t)npts, budget / perCpx, kMaxBatch}));
-        Ba = (int)std::max<size_t>(1, std::min<size_t>({(size_t)npts, budget / perAmp, kMaxBatch}));
-        size_t need = fixed + std::max((size_t)Bc * perCpx, (size_t)Ba * perAmp) + 65536;
-        if (fusedPath) {
-            // per-point results of the single fused launch + |C| scratch of
-            // every resident workgroup
-            int ncu = 0;
-            ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
-            const int perCU = w == 16 ? fused_per_cu<16>(pr.stype)
-                                      : (w == 32 ? fused_per_cu<32>(pr.stype) : fused_per_cu<64>(pr.stype));
-            const size_t perPt = sizeof(PeakRes) + 2 * sizeof(float) + 1 + sizeof(int) + 2 * sizeof(double);
-            const size_t fusedNeed = fixed + (size_t)npts * perPt + 8 * 256 +
-                                     (size_t)std::min(npts, perCU * ncu) * 4 * w * w * sizeof(float) + 65536;
-            need = std::max(need, fusedNeed);
-        }
-        try {
-            ctx->arena_reserve(need);
-            break;
-        } catch (const StError &e) {
-            if (e.code != ST_ERR_NOMEM || (Bc == 1 && Ba == 1))
-                throw;
-            budget /= 2;
-        }
-    }
-    Arena &ar = ctx->arena;
-
-    Pts S;
-    S.r1 = ar.take<int>(npts);
-    S.a1 = ar.take<int>(npts);
-    S.r2 = ar.take<int>(npts);
-    S.a2 = ar.take<int>(npts);
-    S.maskv = ar.take<uint8_t>(npts);
-    S.haveData = ar.take<uint8_t>(npts);
-    S.good = ar.take<uint8_t>(npts);
-    S.type = ar.take<uint8_t>(npts);
-    S.azValid = ar.take<uint8_t>(npts);
-    S.azShift = ar.take<int>(npts);
-    S.azUse = ar.take<int>(npts);
-    int *carryMax = ar.take<int>((npts + kCarryBlk - 1) / kCarryBlk);
-    S.corr = ar.take<float>(npts);
-    S.rShift1 = ar.take<float>(npts);
-    S.aShift1 = ar.take<float>(npts);
-    S.azd = ar.take<float>(npts);
-    S.peakI = ar.take<int>(3 * (size_t)npts);
-    S.peakJ = ar.take<int>(3 * (size_t)npts);
-    S.fineI = ar.take<int>(3 * (size_t)npts);
-    S.fineJ = ar.take<int>(3 * (size_t)npts);
-    S.cmax = ar.take<double>(3 * (size_t)npts);
-    S.sigRg = ar.take<float>(3 * (size_t)npts);
-    S.sigAz = ar.take<float>(3 * (size_t)npts);
-    S.peakGap = ar.take<float>(3 * (size_t)npts);
-    S.fineGap = ar.take<float>(3 * (size_t)npts);
-    S.counts = ar.take<int>(CNT_N);
-    int *list = ar.take<int>(npts);
-    float *dOffR = ar.take<float>(npts), *dOffA = ar.take<float>(npts);
-
+    Pts S{};
     KPar P{};
-    P.nR = par.nR;
-    P.nA = par.nA;
-    P.rStart = par.rStart;
-    P.aStart = par.aStart;
-    P.deltaR = par.deltaR;
-    P.deltaA = par.deltaA;
-    P.wR = par.wR;
-    P.wA = par.wA;
-    P.wRa = par.wRa;
-    P.wAa = par.wAa;
-    P.sf = par.scaleFactor;
-    P.noComplex = par.noComplex;
-    P.hanning = par.hanningFlag;
-    P.maxTries = par.maxTries;
-    P.checkAz = par.checkAzFocus;
-    P.gauss = gauss ? 1 : 0;
-    P.azThresh = par.azDefocusThresh;
-    for (int k = 0; k < 3; k++) {
-        P.rPoly[k] = par.rShiftPoly[k];
-        P.aPoly[k] = par.aShiftPoly[k];
-    }
-    P.mask = par.mask ? upload(ctx, par.mask, (size_t)par.maskNr * par.maskNa) : nullptr;
-    P.maskNr = par.maskNr;
-    P.maskNa = par.maskNa;
-    P.maskR0 = par.maskR0;
-    P.maskA0 = par.maskA0;
-    P.maskNrl = par.maskNrl > 0 ? par.maskNrl : 1;
-    P.maskNal = par.maskNal > 0 ? par.maskNal : 1;
-    P.baseFlag = par.baseFlag;
-    if (par.baseFlag) {
-        ST_REQUIRE(par.sinThetaD && par.cosThetaD && par.overRange && par.nSlpA1 > 0 && par.lambda != 0, ST_ERR_ARG,
-                   "baseline correction needs the range tables");
-        P.Bn = par.Bn;
-        P.Bp = par.Bp;
-        P.dBn = par.dBn;
-        P.dBp = par.dBp;
-        P.dBnQ = par.dBnQ;
-        P.dBpQ = par.dBpQ;
-        P.lambda = par.lambda;
-        P.nSlpA1 = par.nSlpA1;
-        P.sinT = upload(ctx, par.sinThetaD, (size_t)pr.nr1);
-        P.cosT = upload(ctx, par.cosThetaD, (size_t)pr.nr1);
-        P.overR = upload(ctx, par.overRange, (size_t)pr.nr1);
-    }
-    P.intOn = intOn ? 1 : 0;
-    if (intOn) {
-        P.intNr = par.intNr;
-        P.intNa = par.intNa;
-        P.intNrl = par.intNrl;
-        P.intNal = par.intNal;
-        P.intPs = intPs;
-        P.intPa = intPs * par.intNal;
-        P.intPr = intPs * par.intNrl;
-        P.intf = upload(ctx, par.intf, (size_t)2 * par.intNr * par.intNa);
-    }
-    P.hann = upload(ctx, hann.data(), hann.size());
-    P.im = Images{pr.d1, pr.d2, pr.nr1, pr.na1, pr.nr2, pr.na2, pr.stype};
+    int *carryMax = nullptr, *list = nullptr;
+    float *dOffR = nullptr, *dOffA = nullptr;
     const int *dR2 = nullptr, *dA2 = nullptr;
-    if (!par.polyShift) {
-        dR2 = upload(ctx, hr2.data(), hr2.size());
-        dA2 = upload(ctx, ha2.data(), ha2.size());
-    }
-    const size_t mark = ar.used;
-
-    ST_HIP(hipMemsetAsync(S.counts, 0, CNT_N * sizeof(int), st));
-    hipLaunchKernelGGL(k_geometry, dim3((npts + 255) / 256), dim3(256), 0, st, P, S, dR2, dA2, list);
-    ST_HIP(hipGetLastError());
-    ST_HIP(hipEventRecord(sev[1], st));
-    const int nCpx = read_count(ctx, S.counts, CNT_LIST);
-
-    // ---- interferogram spectra of the complex attempts
+    int nCpx = 0;
+    size_t mark = 0;  // arena position shared by the batch's stage scratch
+    CpxBatch C{};     // fused path: the per-item results of this call
     struct OvfBuf {
         void *p = nullptr;
+        OvfBuf() = default;
+        OvfBuf(const OvfBuf &) = delete;
+        OvfBuf &operator=(const OvfBuf &) = delete;
         ~OvfBuf()
         {
             if (p)
                 (void)hipFree(p);
         }
     } intOvfBuf;
-    if (intOn && nCpx > 0) {
-        IntSet *set = ar.take<IntSet>(nCpx);
-        int *nOvfDev = ar.take<int>(1);
-        ST_HIP(hipMemsetAsync(nOvfDev, 0, sizeof(int), st));
-        P.intSet = set;
-        const size_t lds = int_lds_bytes(intPs);
-        const int kInt = ctx->kernel_slot("k_int_bins");
-        ctx->time_begin(kInt);
-        hipLaunchKernelGGL(k_int_bins, dim3(nCpx), dim3(kIntThreads), lds, st, P, S, list, set, (IntBin *)nullptr,
-                           nOvfDev, 0);
-        ctx->time_end(kInt);
+
+    StrackJob(st_ctx *c, StPair &p, const st_track_params &par, st_track_out *o, st_track_diag *d,
+              st_cull_params *cu)
+        : ctx(c), pr(p), pp(&par), out(o), diag(d), cull(cu)
+    {
+    }
+    StrackJob(const StrackJob &) = delete;
+    StrackJob &operator=(const StrackJob &) = delete;
+
+    // Validation, host tables and the bytes this call needs.
+    void plan()
+    {
+        const st_track_params &par = *pp;
+        ST_REQUIRE(par.wR == par.wA && par.wRa == par.wAa, ST_ERR_UNSUPPORTED, "non-square windows");
+        ST_REQUIRE(par.wR >= 8 && par.wRa >= 8 && !(par.wR & 1) && !(par.wRa & 1), ST_ERR_ARG, "window sizes");
+        ST_REQUIRE(stfft::supported_size(par.wR) && stfft::supported_size(2 * par.wR) &&
+                       stfft::supported_size(par.wRa * LA) && stfft::supported_size(4 * par.wRa),
+                   ST_ERR_UNSUPPORTED, "window size not supported by the FFT engine");
+        ST_REQUIRE(par.nR > 0 && par.nA > 0 && par.scaleFactor > 0, ST_ERR_ARG, "grid");
+        ST_REQUIRE(par.polyShift || (par.initDr && par.initDa && par.initNr > 0 && par.initNa > 0), ST_ERR_ARG,
+                   "initial offsets: polyShift=0 needs initDr/initDa");
+        // the matrix-core complex match covers windows 16 and 32 without the
+        // baseline ramp; other cases run the FP32 paths
+        intOn = par.intf != nullptr && !par.noComplex;
+        // -gauss fits its peak on the staged path's magnitude surfaces
+        gauss = par.gaussFlag != 0;
+        mfmaPath = par.precision == ST_PREC_MFMA && (par.wR == 16 || par.wR == 32) && !par.baseFlag &&
+                       !intOn && !gauss && !staged_only();
+        fusedPath = (par.wR == 16 || par.wR == 32 || par.wR == 64) && !gauss && !staged_only();
+        kPeak = ctx->kernel_slot(mfmaPath ? "k_cpx_mfma" : (fusedPath ? "k_cpx_fused" : "peak_refine"));
+        kAmpPeak = ctx->kernel_slot("amp_peak");
+        kGather = ctx->kernel_slot("cpx_gather");
+        kFft = ctx->kernel_slot("fft2d");
+        kAmpDet = ctx->kernel_slot("amp_detect");
+        npts = par.nR * par.nA;
+        w = par.wR;
+
+        // per-point state + inputs
+        host_hanning(par.wA, par.wR, hann);
+        if (!par.polyShift)
+            host_positions(par, hr2, ha2);
+        fixed = (size_t)npts * 200 + hann.size() * 4 + (hr2.size() * 8) + 4096 +
+                       ((size_t)npts / kCarryBlk + 1) * 4 + 256;
+        if (par.mask)
+            fixed += (size_t)par.maskNr * par.maskNa;
+        if (par.baseFlag)
+            fixed += (size_t)pr.nr1 * 24;
+        intPs = 0;
+        if (intOn) {
+            ST_REQUIRE(par.intNr > 0 && par.intNa > 0 && par.intNrl > 0 && par.intNal > 0, ST_ERR_ARG,
+                       "interferogram size and looks");
+            intPs = std::max(std::max(par.wA / par.intNal, par.wR / par.intNrl), 8);
+            ST_REQUIRE(intPs <= kIntMaxPs, ST_ERR_UNSUPPORTED,
+                       "interferogram patch above 64 x 64 samples (window / looks)");
+            ST_REQUIRE((intPs * par.intNal) / 2 >= par.wA / 2 && (intPs * par.intNrl) / 2 >= par.wR / 2, ST_ERR_ARG,
+                       "window not covered by the oversampled interferogram patch (window not a multiple of the looks)");
+            ST_REQUIRE((long)intPs * par.intNal <= 32768 && (long)intPs * par.intNrl <= 32768, ST_ERR_ARG,
+                       "interferogram looks");
+            fixed += (size_t)par.intNr * par.intNa * 8 + (size_t)npts * sizeof(IntSet) + 512;
+        }
+        // the cull chain works on the output grids in the same arena
+        if (cull) {
+            ST_REQUIRE(cull->nR == par.nR && cull->nA == par.nA && cull->offR == out->offR && cull->offA == out->offA &&
+                           cull->corr == out->corr && cull->type == out->type,
+                       ST_ERR_ARG, "cull grids must be the tracking outputs");
+            fixed += cull_device_bytes((size_t)npts);
+        }
+        const int wsMax = par.wAa * (par.maxTries >= 2 ? LA : 1);
+        perCpx = (size_t)w * w * (16 + 32 + 32 + 16) + 2048;
+        perAmp = (size_t)wsMax * wsMax * 128 + (size_t)2 * wsMax * 96 + 4096;
+    }
+
+    // Bytes of the fused path's per-item results (an upper bound: every
+    // point an attempt) and of the |C| scratch of its resident workgroups.
+    size_t fused_bytes(int ncu) const
+    {
+        if (!fusedPath)
+            return 0;
+        const size_t perPt = sizeof(PeakRes) + 2 * sizeof(float) + 1 + sizeof(int) + 2 * sizeof(double);
+        const int perCU = fused_per_cu_w(w, pr.stype);
+        return (size_t)npts * perPt + 8 * 256 + (size_t)std::min(npts, perCU * ncu) * 4 * w * w * sizeof(float);
+    }
+
+    // Per-point state, parameter block, uploads, geometry, the count of
+    // complex attempts and the interferogram spectra.
+    void setup()
+    {
+        const st_track_params &par = *pp;
+        hipStream_t st = ctx->stream;
+        hipEvent_t *sev = ctx->ev + 64;
+        Arena &ar = ctx->arena;
+        S = Pts{};
+        S.r1 = ar.take<int>(npts);
+        S.a1 = ar.take<int>(npts);
+        S.r2 = ar.take<int>(npts);
+        S.a2 = ar.take<int>(npts);
+        S.maskv = ar.take<uint8_t>(npts);
+        S.haveData = ar.take<uint8_t>(npts);
+        S.good = ar.take<uint8_t>(npts);
+        S.type = ar.take<uint8_t>(npts);
+        S.azValid = ar.take<uint8_t>(npts);
+        S.azShift = ar.take<int>(npts);
+        S.azUse = ar.take<int>(npts);
+        carryMax = ar.take<int>((npts + kCarryBlk - 1) / kCarryBlk);
+        S.corr = ar.take<float>(npts);
+        S.rShift1 = ar.take<float>(npts);
+        S.aShift1 = ar.take<float>(npts);
+        S.azd = ar.take<float>(npts);
+        S.peakI = ar.take<int>(3 * (size_t)npts);
+        S.peakJ = ar.take<int>(3 * (size_t)npts);
+        S.fineI = ar.take<int>(3 * (size_t)npts);
+        S.fineJ = ar.take<int>(3 * (size_t)npts);
+        S.cmax = ar.take<double>(3 * (size_t)npts);
+        S.sigRg = ar.take<float>(3 * (size_t)npts);
+        S.sigAz = ar.take<float>(3 * (size_t)npts);
+        S.peakGap = ar.take<float>(3 * (size_t)npts);
+        S.fineGap = ar.take<float>(3 * (size_t)npts);
+        S.snr = ar.take<float>(3 * (size_t)npts);
+        S.counts = ar.take<int>(CNT_N);
+        list = ar.take<int>(npts);
+        dOffR = ar.take<float>(npts);
+        dOffA = ar.take<float>(npts);
+
+        P = KPar{};
+        P.nR = par.nR;
+        P.nA = par.nA;
+        P.rStart = par.rStart;
+        P.aStart = par.aStart;
+        P.deltaR = par.deltaR;
+        P.deltaA = par.deltaA;
+        P.wR = par.wR;
+        P.wA = par.wA;
+        P.wRa = par.wRa;
+        P.wAa = par.wAa;
+        P.sf = par.scaleFactor;
+        P.noComplex = par.noComplex;
+        P.hanning = par.hanningFlag;
+        P.maxTries = par.maxTries;
+        P.checkAz = par.checkAzFocus;
+        P.gauss = gauss ? 1 : 0;
+        P.azThresh = par.azDefocusThresh;
+        for (int k = 0; k < 3; k++) {
+            P.rPoly[k] = par.rShiftPoly[k];
+            P.aPoly[k] = par.aShiftPoly[k];
+        }
+        P.mask = par.mask ? upload(ctx, par.mask, (size_t)par.maskNr * par.maskNa) : nullptr;
+        P.maskNr = par.maskNr;
+        P.maskNa = par.maskNa;
+        P.maskR0 = par.maskR0;
+        P.maskA0 = par.maskA0;
+        P.maskNrl = par.maskNrl > 0 ? par.maskNrl : 1;
+        P.maskNal = par.maskNal > 0 ? par.maskNal : 1;
+        P.baseFlag = par.baseFlag;
+        if (par.baseFlag) {
+            ST_REQUIRE(par.sinThetaD && par.cosThetaD && par.overRange && par.nSlpA1 > 0 && par.lambda != 0, ST_ERR_ARG,
+                       "baseline correction needs the range tables");
+            P.Bn = par.Bn;
+            P.Bp = par.Bp;
+            P.dBn = par.dBn;
+            P.dBp = par.dBp;
+            P.dBnQ = par.dBnQ;
+            P.dBpQ = par.dBpQ;
+            P.lambda = par.lambda;
+            P.nSlpA1 = par.nSlpA1;
+            P.sinT = upload(ctx, par.sinThetaD, (size_t)pr.nr1);
+            P.cosT = upload(ctx, par.cosThetaD, (size_t)pr.nr1);
+            P.overR = upload(ctx, par.overRange, (size_t)pr.nr1);
+        }
+        P.intOn = intOn ? 1 : 0;
+        if (intOn) {
+            P.intNr = par.intNr;
+            P.intNa = par.intNa;
+            P.intNrl = par.intNrl;
+            P.intNal = par.intNal;
+            P.intPs = intPs;
+            P.intPa = intPs * par.intNal;
+            P.intPr = intPs * par.intNrl;
+            P.intf = upload(ctx, par.intf, (size_t)2 * par.intNr * par.intNa);
+        }
+        P.hann = upload(ctx, hann.data(), hann.size());
+        P.im = Images{pr.d1, pr.d2, pr.nr1, pr.na1, pr.nr2, pr.na2, pr.stype};
+        dR2 = nullptr;
+        dA2 = nullptr;
+        if (!par.polyShift) {
+            dR2 = upload(ctx, hr2.data(), hr2.size());
+            dA2 = upload(ctx, ha2.data(), ha2.size());
+        }
+        ST_HIP(hipMemsetAsync(S.counts, 0, CNT_N * sizeof(int), st));
+        hipLaunchKernelGGL(k_geometry, dim3((npts + 255) / 256), dim3(256), 0, st, P, S, dR2, dA2, list);
         ST_HIP(hipGetLastError());
-        const int nOvf = read_count(ctx, nOvfDev, 0);
-        if (nOvf > 0) {
-            // spectra with more than kIntInline bins within 1% of their peak
-            ST_HIP(hipMalloc(&intOvfBuf.p, (size_t)nOvf * intPs * intPs * sizeof(IntBin)));
-            P.intOvf = (const IntBin *)intOvfBuf.p;
+        ST_HIP(hipEventRecord(sev[1], st));
+        nCpx = read_count(ctx, S.counts, CNT_LIST);
+
+        // ---- interferogram spectra of the complex attempts
+        if (intOn && nCpx > 0) {
+            IntSet *set = ar.take<IntSet>(nCpx);
+            int *nOvfDev = ar.take<int>(1);
+            ST_HIP(hipMemsetAsync(nOvfDev, 0, sizeof(int), st));
+            P.intSet = set;
+            const size_t lds = int_lds_bytes(intPs);
+            const int kInt = ctx->kernel_slot("k_int_bins");
             ctx->time_begin(kInt);
-            hipLaunchKernelGGL(k_int_bins, dim3(nCpx), dim3(kIntThreads), lds, st, P, S, list, set,
-                               (IntBin *)intOvfBuf.p, nOvfDev, 1);
+            hipLaunchKernelGGL(k_int_bins, dim3(nCpx), dim3(kIntThreads), lds, st, P, S, list, set, (IntBin *)nullptr,
+                               nOvfDev, 0);
             ctx->time_end(kInt);
             ST_HIP(hipGetLastError());
+            const int nOvf = read_count(ctx, nOvfDev, 0);
+            if (nOvf > 0) {
+                // spectra with more than kIntInline bins within 1% of their peak
+                ST_HIP(hipMalloc(&intOvfBuf.p, (size_t)nOvf * intPs * intPs * sizeof(IntBin)));
+                P.intOvf = (const IntBin *)intOvfBuf.p;
+                ctx->time_begin(kInt);
+                hipLaunchKernelGGL(k_int_bins, dim3(nCpx), dim3(kIntThreads), lds, st, P, S, list, set,
+                                   (IntBin *)intOvfBuf.p, nOvfDev, 1);
+                ctx->time_end(kInt);
+                ST_HIP(hipGetLastError());
+            }
         }
+
     }
 
-    // ---- complex attempts
-    const bool fused = fusedPath;
-    if (nCpx > 0 && mfmaPath) {
-        CpxBatch C{};
-        C.res = ar.take<PeakRes>(nCpx);
-        C.sig = ar.take<float>(2 * (size_t)nCpx);
-        C.have = ar.take<uint8_t>(nCpx);
-        C.az = ar.take<int>(nCpx);
-        C.p = ar.take<double>(2 * (size_t)nCpx);
-        if (w == 16)
-            launch_cpx_mfma<16>(ctx, kPeak, P, S, list, nCpx, C);
-        else
-            launch_cpx_mfma<32>(ctx, kPeak, P, S, list, nCpx, C);
-        hipLaunchKernelGGL(k_lm, dim3((2 * nCpx + 127) / 128), dim3(128), 0, st, C.res, nCpx, C.sig);
-        hipLaunchKernelGGL(k_cpx_finalize, dim3((nCpx + 127) / 128), dim3(128), 0, st, P, S, list, 0, nCpx, C);
-        ST_HIP(hipGetLastError());
-        ar.used = mark;
-    } else if (nCpx > 0 && fused) {
-        CpxBatch C{};
-        C.res = ar.take<PeakRes>(nCpx);
-        C.sig = ar.take<float>(2 * (size_t)nCpx);
-        C.have = ar.take<uint8_t>(nCpx);
-        C.az = ar.take<int>(nCpx);
-        C.p = ar.take<double>(2 * (size_t)nCpx);
-        if (w == 16)
-            launch_cpx_fused<16>(ctx, kPeak, P, S, list, nCpx, C);
-        else if (w == 32)
-            launch_cpx_fused<32>(ctx, kPeak, P, S, list, nCpx, C);
-        else
-            launch_cpx_fused<64>(ctx, kPeak, P, S, list, nCpx, C);
-        hipLaunchKernelGGL(k_lm, dim3((2 * nCpx + 127) / 128), dim3(128), 0, st, C.res, nCpx, C.sig);
-        hipLaunchKernelGGL(k_cpx_finalize, dim3((nCpx + 127) / 128), dim3(128), 0, st, P, S, list, 0, nCpx, C);
-        ST_HIP(hipGetLastError());
-        ar.used = mark;
-    } else if (nCpx > 0) {
-        const int B = std::min(Bc, nCpx);
-        CpxBatch C;
-        C.P = ar.take<float2>((size_t)2 * B * w * w);
-        C.S = ar.take<float2>((size_t)B * 4 * w * w);
-        C.tmp = ar.take<float2>((size_t)B * 4 * w * w);
-        C.mag = ar.take<float>((size_t)B * 4 * w * w);
-        C.res = ar.take<PeakRes>(B);
-        C.sig = ar.take<float>(2 * (size_t)B);
-        C.have = ar.take<uint8_t>(B);
-        C.az = ar.take<int>(B);
-        C.p = ar.take<double>(2 * (size_t)B);
-        // blocking of the two passes after the range transforms: Cx spectrum
-        // columns per k_cpx_colx workgroup, Rr rows per k_cpx_rowmag workgroup
-        const int W2 = 2 * w;
-        ST_REQUIRE(W2 <= kStElems, ST_ERR_UNSUPPORTED, "window too large for the staged complex path");
-        int lcx = 0;
-        while ((2 << lcx) <= 16 && (2 << lcx) * W2 <= kStElems)
-            lcx++;
-        const int Cx = 1 << lcx, Rr = kStElems / W2, nPart = (W2 + Rr - 1) / Rr;
-        const size_t ldsX = (size_t)Cx * std::max(2 * (w + 1), W2 + 1) * sizeof(float2),
-                     ldsR = (size_t)Rr * W2 * sizeof(float2);
-        const stfft::PlanArgs PfW = ctx->fft.plan(w, -1), PiW2 = ctx->fft.plan(W2, +1);
-        const stfft::FastDiv dW2 = stfft::make_fastdiv((uint32_t)W2);
-        Top2 *part = (Top2 *)C.tmp;  // [B][nPart]
-        for (int off = 0; off < nCpx; off += B) {
-            ctx->check_cancel();
-            const int nb = std::min(B, nCpx - off);
-            ctx->time_begin(kGather);
-            hipLaunchKernelGGL(k_cpx_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, C);
-            ctx->time_end(kGather);
-            ctx->time_begin(kFft);
-            ctx->fft.rows(C.P, (long)2 * nb * w, w, -1, st);
-            ctx->time_end(kFft);
-            ctx->time_begin(kPeak);
-            hipLaunchKernelGGL(k_cpx_colx, dim3((w + Cx - 1) / Cx, nb), dim3(NT), ldsX, st, C.P, nb, w, C.az, C.S, lcx,
-                               PfW, PiW2, dW2);
-            hipLaunchKernelGGL(k_cpx_rowmag, dim3(nPart, nb), dim3(NT), ldsR, st, C.S, w, Rr, C.mag, part, PiW2, dW2);
-            const GaussArg G{gauss ? 1 : 0, w, par.hanningFlag ? 1.5 : 1.0, 1.0, C.p};
-            if (gauss) {
-                hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0, C.res,
-                                   G);
-                hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, C.mag, 2 * w, 0, C.res, nb,
-                                   G);
-            } else
-                hipLaunchKernelGGL(k_peak_merge<false>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0,
-                                   C.res, G);
-            ctx->time_end(kPeak);
-            hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, C.res, nb, C.sig);
-            hipLaunchKernelGGL(k_cpx_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, C);
-            ST_HIP(hipGetLastError());
-        }
-        ar.used = mark;
-    }
-    ctx->check_cancel();
+    // This call's complex attempts on the matrix-core or the staged path
+    // (the fused path runs batched, run_fused_group).
+    void complex_alone()
     {
-        const int nCarry = (npts + kCarryBlk - 1) / kCarryBlk;
-        hipLaunchKernelGGL(k_carry_blocks, dim3(nCarry), dim3(kCarryBlk), 0, st, S, npts, carryMax);
-        hipLaunchKernelGGL(k_carry_az, dim3(nCarry), dim3(kCarryBlk), 0, st, S, npts, carryMax);
-    }
-    ST_HIP(hipEventRecord(sev[2], st));
-    ST_HIP(hipEventRecord(sev[3], st));
-    ST_HIP(hipEventRecord(sev[4], st));
-
-    // ---- amplitude attempts
-    for (int t = 0; t < par.maxTries && t < 2; t++) {
-        const int large = t;
-        ST_HIP(hipMemsetAsync(S.counts + CNT_LIST, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_amp_list, dim3((npts + 255) / 256), dim3(256), 0, st, P, S, npts, list);
-        const int nA = read_count(ctx, S.counts, CNT_LIST);
-        if (nA == 0)
-            break;
-        const int ws = par.wAa * (large ? LA : 1), W = 2 * ws;
-        // blocking: Cx interpolation columns per k_amp_colx workgroup, Rd
-        // rows per k_amp_rowdet / k_amp_rowfwd workgroup, Cc correlation
-        // columns per k_amp_colx2 workgroup, Rm rows per k_amp_rowmag one
-        int lcx = 0, lcc = 0;
-        while ((2 << lcx) <= ws && (2 << lcx) * W <= kStElems)
-            lcx++;
-        while ((2 << lcc) <= ws && (2 << lcc) * W * 2 <= kStElems)
-            lcc++;
-        const int Cx = 1 << lcx, Cc = 1 << lcc;
-        const int Rd = std::max(1, kStElems / (2 * W)), nPd = (W + Rd - 1) / Rd;
-        const int Rf = kStElems / W, nPf = (W + Rf - 1) / Rf;
-        const int Rm = 2 * (kStElems / W), nPm = (W + Rm - 1) / Rm;
-        const size_t ldsX = (size_t)Cx * (Cx > 1 ? W + 1 : W) * sizeof(float2);
-        const size_t ldsD = (size_t)2 * Rd * W * sizeof(float2), ldsF = (size_t)Rf * W * sizeof(float2);
-        const size_t ldsC = (size_t)2 * Cc * (Cc > 1 ? W + 1 : W) * sizeof(float2);
-        const size_t ldsM = (size_t)(Rm / 2) * W * sizeof(float2);
-        const size_t per = (size_t)ws * ws * 128 + (size_t)nPd * (sizeof(AmpPart) + sizeof(double2)) +
-                           (size_t)nPm * sizeof(Top2) + 2048;
-        // the amplitude kernels put 2 x batch on grid y (at most 65535)
-        const size_t kMaxAmpBatch = 16384;
-        const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nA, (ar.cap - ar.used) / per, kMaxAmpBatch}));
-        AmpBatch A;
-        A.I = ar.take<float2>((size_t)2 * B * ws * ws);
-        A.F = ar.take<float2>((size_t)2 * B * W * W);
-        A.S = ar.take<float2>((size_t)B * W * W);
-        A.mag = ar.take<float>((size_t)B * W * W);
-        A.res = ar.take<PeakRes>(B);
-        A.sig = ar.take<float>(2 * (size_t)B);
-        A.state = ar.take<uint8_t>(B);
-        A.az = ar.take<int>(B);
-        A.p = ar.take<double>(2 * (size_t)B);
-        A.dpart = ar.take<AmpPart>((size_t)B * nPd);
-        A.ppart = ar.take<double2>((size_t)B * nPd);
-        A.mean = ar.take<double2>(B);
-        A.part = ar.take<Top2>((size_t)B * nPm);
-        const stfft::PlanArgs Pfs = ctx->fft.plan(ws, -1), PfW = ctx->fft.plan(W, -1), PiW = ctx->fft.plan(W, +1);
-        const stfft::FastDiv dW = stfft::make_fastdiv((uint32_t)W);
-        for (int off = 0; off < nA; off += B) {
-            ctx->check_cancel();
-            const int nb = std::min(B, nA - off);
-            hipLaunchKernelGGL(k_amp_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, large, A);
-            ctx->fft.rows(A.I, (long)2 * nb * ws, ws, -1, st);
-            hipLaunchKernelGGL(k_amp_colx, dim3((ws + Cx - 1) / Cx, 2 * nb), dim3(NT), ldsX, st, A.I, nb, ws, A.az,
-                               A.state, A.F, lcx, Pfs, PiW, dW);
-            ctx->time_begin(kAmpDet);
-            hipLaunchKernelGGL(k_amp_rowdet, dim3(nPd, nb), dim3(NT), ldsD, st, P, A.F, nb, ws, Rd, A.state,
-                               (float *)A.S, A.dpart, PiW, dW);
-            ctx->time_end(kAmpDet);
-            hipLaunchKernelGGL(k_amp_stat, dim3(nb), dim3(64), 0, st, A.dpart, nPd, W, A.state, A.mean);
-            hipLaunchKernelGGL(k_amp_rowfwd, dim3(nPf, nb), dim3(NT), ldsF, st, (const float *)A.S, nb, W, Rf, A.mean,
-                               A.state, A.F, A.ppart, PfW);
-            hipLaunchKernelGGL(k_amp_colx2, dim3((ws + Cc - 1) / Cc, nb), dim3(NT), ldsC, st, A.F, W, A.state, A.S,
-                               lcc, PfW, PiW, dW);
-            hipLaunchKernelGGL(k_amp_psum, dim3((nb + 127) / 128), dim3(128), 0, st, A.ppart, nPf, nb, W, A.state,
-                               A.p);
-            ctx->time_begin(kAmpPeak);
-            hipLaunchKernelGGL(k_amp_rowmag, dim3(nPm, nb), dim3(NT), ldsM, st, A.S, W, Rm, A.mag, A.part, PiW, dW,
-                               A.state);
-            const GaussArg G{gauss ? 1 : 0, W / 2, 1.0, 16.0, A.p};
-            if (gauss) {
-                hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, A.part, nPm, A.mag, W, 1, A.res, G);
-                hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, A.mag, W, 1, A.res, nb, G);
-            } else
-                hipLaunchKernelGGL(k_peak_merge<false>, dim3(nb), dim3(NT), 0, st, A.part, nPm, A.mag, W, 1, A.res,
-                                   G);
-            ctx->time_end(kAmpPeak);
-            hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, A.res, nb, A.sig);
-            hipLaunchKernelGGL(k_amp_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, large, A);
+        const st_track_params &par = *pp;
+        hipStream_t st = ctx->stream;
+        Arena &ar = ctx->arena;
+        if (nCpx > 0 && mfmaPath) {
+            CpxBatch C{};
+            C.res = ar.take<PeakRes>(nCpx);
+            C.sig = ar.take<float>(2 * (size_t)nCpx);
+            C.have = ar.take<uint8_t>(nCpx);
+            C.az = ar.take<int>(nCpx);
+            C.p = ar.take<double>(2 * (size_t)nCpx);
+            if (w == 16)
+                launch_cpx_mfma<16>(ctx, kPeak, P, S, list, nCpx, C);
+            else
+                launch_cpx_mfma<32>(ctx, kPeak, P, S, list, nCpx, C);
+            hipLaunchKernelGGL(k_lm, dim3((2 * nCpx + 127) / 128), dim3(128), 0, st, C.res, nCpx, C.sig);
+            hipLaunchKernelGGL(k_cpx_finalize, dim3((nCpx + 127) / 128), dim3(128), 0, st, P, S, list, 0, nCpx, C);
             ST_HIP(hipGetLastError());
+            ar.used = mark;
+        } else if (nCpx > 0 && !fusedPath) {
+            const int B = std::min(Bc, nCpx);
+            CpxBatch C;
+            C.P = ar.take<float2>((size_t)2 * B * w * w);
+            C.S = ar.take<float2>((size_t)B * 4 * w * w);
+            C.tmp = ar.take<float2>((size_t)B * 4 * w * w);
+            C.mag = ar.take<float>((size_t)B * 4 * w * w);
+            C.res = ar.take<PeakRes>(B);
+            C.sig = ar.take<float>(2 * (size_t)B);
+            C.have = ar.take<uint8_t>(B);
+            C.az = ar.take<int>(B);
+            C.p = ar.take<double>(2 * (size_t)B);
+            // blocking of the two passes after the range transforms: Cx spectrum
+            // columns per k_cpx_colx workgroup, Rr rows per k_cpx_rowmag workgroup
+            const int W2 = 2 * w;
+            ST_REQUIRE(W2 <= kStElems, ST_ERR_UNSUPPORTED, "window too large for the staged complex path");
+            int lcx = 0;
+            while ((2 << lcx) <= 16 && (2 << lcx) * W2 <= kStElems)
+                lcx++;
+            const int Cx = 1 << lcx, Rr = kStElems / W2, nPart = (W2 + Rr - 1) / Rr;
+            const size_t ldsX = (size_t)Cx * std::max(2 * (w + 1), W2 + 1) * sizeof(float2),
+                         ldsR = (size_t)Rr * W2 * sizeof(float2);
+            const stfft::PlanArgs PfW = ctx->fft.plan(w, -1), PiW2 = ctx->fft.plan(W2, +1);
+            const stfft::FastDiv dW2 = stfft::make_fastdiv((uint32_t)W2);
+            Top2 *part = (Top2 *)C.tmp;  // [B][nPart]
+            for (int off = 0; off < nCpx; off += B) {
+                ctx->check_cancel();
+                const int nb = std::min(B, nCpx - off);
+                ctx->time_begin(kGather);
+                hipLaunchKernelGGL(k_cpx_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, C);
+                ctx->time_end(kGather);
+                ctx->time_begin(kFft);
+                ctx->fft.rows(C.P, (long)2 * nb * w, w, -1, st);
+                ctx->time_end(kFft);
+                ctx->time_begin(kPeak);
+                hipLaunchKernelGGL(k_cpx_colx, dim3((w + Cx - 1) / Cx, nb), dim3(NT), ldsX, st, C.P, nb, w, C.az, C.S, lcx,
+                                   PfW, PiW2, dW2);
+                hipLaunchKernelGGL(k_cpx_rowmag, dim3(nPart, nb), dim3(NT), ldsR, st, C.S, w, Rr, C.mag, part, PiW2, dW2);
+                const GaussArg G{gauss ? 1 : 0, w, par.hanningFlag ? 1.5 : 1.0, 1.0, C.p};
+                if (gauss) {
+                    hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0, C.res,
+                                       G);
+                    hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, C.mag, 2 * w, 0, C.res, nb,
+                                       G);
+                } else
+                    hipLaunchKernelGGL(k_peak_merge<false>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0,
+                                       C.res, G);
+                ctx->time_end(kPeak);
+                hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, C.res, nb, C.sig);
+                hipLaunchKernelGGL(k_cpx_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, C);
+                ST_HIP(hipGetLastError());
+            }
+            ar.used = mark;
         }
-        ar.used = mark;
-        if (!large)
-            ST_HIP(hipEventRecord(sev[3], st));
     }
-    // the doubled-window stage ends here also when it did not run (no
-    // failed small tries, or one try allowed), so its time reads zero
-    ST_HIP(hipEventRecord(sev[4], st));
 
-    hipLaunchKernelGGL(k_compose, dim3((npts + 255) / 256), dim3(256), 0, st, P, S, dOffR, dOffA, npts);
-    ST_HIP(hipGetLastError());
-    if (cull) {
+    // The fused path: this call's share of a batched launch.
+    FusedJob fused_part()
+    {
+        Arena &ar = ctx->arena;
+        C = CpxBatch{};
+        C.res = ar.take<PeakRes>(nCpx);
+        C.sig = ar.take<float>(2 * (size_t)nCpx);
+        C.have = ar.take<uint8_t>(nCpx);
+        C.az = ar.take<int>(nCpx);
+        C.p = ar.take<double>(2 * (size_t)nCpx);
+        C.list = list;
+        return FusedJob{P, S, C, nCpx};
+    }
+
+    // Width fits and acceptance of the fused path's attempts.
+    void fused_finish()
+    {
+        hipStream_t st = ctx->stream;
+        hipLaunchKernelGGL(k_lm, dim3((2 * nCpx + 127) / 128), dim3(128), 0, st, C.res, nCpx, C.sig);
+        hipLaunchKernelGGL(k_cpx_finalize, dim3((nCpx + 127) / 128), dim3(128), 0, st, P, S, list, 0, nCpx, C);
+        ST_HIP(hipGetLastError());
+    }
+
+    // Carry of the azimuth shift, amplitude retries, composition, the
+    // optional cull chain and the copies out.
+    void finish()
+    {
+        const st_track_params &par = *pp;
+        hipStream_t st = ctx->stream;
+        hipEvent_t *sev = ctx->ev + 64;
+        Arena &ar = ctx->arena;
         ctx->check_cancel();
-        st_cull_chain_device(ctx, cull, dOffR, dOffA, S.corr, S.type);
+        {
+            const int nCarry = (npts + kCarryBlk - 1) / kCarryBlk;
+            hipLaunchKernelGGL(k_carry_blocks, dim3(nCarry), dim3(kCarryBlk), 0, st, S, npts, carryMax);
+            hipLaunchKernelGGL(k_carry_az, dim3(nCarry), dim3(kCarryBlk), 0, st, S, npts, carryMax);
+        }
+        ST_HIP(hipEventRecord(sev[2], st));
+        ST_HIP(hipEventRecord(sev[3], st));
+        ST_HIP(hipEventRecord(sev[4], st));
+
+        // ---- amplitude attempts
+        for (int t = 0; t < par.maxTries && t < 2; t++) {
+            const int large = t;
+            ST_HIP(hipMemsetAsync(S.counts + CNT_LIST, 0, sizeof(int), st));
+            hipLaunchKernelGGL(k_amp_list, dim3((npts + 255) / 256), dim3(256), 0, st, P, S, npts, list);
+            const int nA = read_count(ctx, S.counts, CNT_LIST);
+            if (nA == 0)
+                break;
+            const int ws = par.wAa * (large ? LA : 1), W = 2 * ws;
+            // blocking: Cx interpolation columns per k_amp_colx workgroup, Rd
+            // rows per k_amp_rowdet / k_amp_rowfwd workgroup, Cc correlation
+            // columns per k_amp_colx2 workgroup, Rm rows per k_amp_rowmag one
+            int lcx = 0, lcc = 0;
+            while ((2 << lcx) <= ws && (2 << lcx) * W <= kStElems)
+                lcx++;
+            while ((2 << lcc) <= ws && (2 << lcc) * W * 2 <= kStElems)
+                lcc++;
+            const int Cx = 1 << lcx, Cc = 1 << lcc;
+            const int Rd = std::max(1, kStElems / (2 * W)), nPd = (W + Rd - 1) / Rd;
+            const int Rf = kStElems / W, nPf = (W + Rf - 1) / Rf;
+            const int Rm = 2 * (kStElems / W), nPm = (W + Rm - 1) / Rm;
+            const size_t ldsX = (size_t)Cx * (Cx > 1 ? W + 1 : W) * sizeof(float2);
+            const size_t ldsD = (size_t)2 * Rd * W * sizeof(float2), ldsF = (size_t)Rf * W * sizeof(float2);
+            const size_t ldsC = (size_t)2 * Cc * (Cc > 1 ? W + 1 : W) * sizeof(float2);
+            const size_t ldsM = (size_t)(Rm / 2) * W * sizeof(float2);
+            const size_t per = (size_t)ws * ws * 128 + (size_t)nPd * (sizeof(AmpPart) + sizeof(double2)) +
+                               (size_t)nPm * sizeof(Top2) + 2048;
+            // the amplitude kernels put 2 x batch on grid y (at most 65535)
+            const size_t kMaxAmpBatch = 16384;
+            const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nA, (ar.cap - ar.used) / per, kMaxAmpBatch}));
+            AmpBatch A;
+            A.I = ar.take<float2>((size_t)2 * B * ws * ws);
+            A.F = ar.take<float2>((size_t)2 * B * W * W);
+            A.S = ar.take<float2>((size_t)B * W * W);
+            A.mag = ar.take<float>((size_t)B * W * W);
+            A.res = ar.take<PeakRes>(B);
+            A.sig = ar.take<float>(2 * (size_t)B);
+            A.state = ar.take<uint8_t>(B);
+            A.az = ar.take<int>(B);
+            A.p = ar.take<double>(2 * (size_t)B);
+            A.dpart = ar.take<AmpPart>((size_t)B * nPd);
+            A.ppart = ar.take<double2>((size_t)B * nPd);
+            A.mean = ar.take<double2>(B);
+            A.part = ar.take<Top2>((size_t)B * nPm);
+            const stfft::PlanArgs Pfs = ctx->fft.plan(ws, -1), PfW = ctx->fft.plan(W, -1), PiW = ctx->fft.plan(W, +1);
+            const stfft::FastDiv dW = stfft::make_fastdiv((uint32_t)W);
+            for (int off = 0; off < nA; off += B) {
+                ctx->check_cancel();
+                const int nb = std::min(B, nA - off);
+                hipLaunchKernelGGL(k_amp_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, large, A);
+                ctx->fft.rows(A.I, (long)2 * nb * ws, ws, -1, st);
+                hipLaunchKernelGGL(k_amp_colx, dim3((ws + Cx - 1) / Cx, 2 * nb), dim3(NT), ldsX, st, A.I, nb, ws, A.az,
+                                   A.state, A.F, lcx, Pfs, PiW, dW);
+                ctx->time_begin(kAmpDet);
+                hipLaunchKernelGGL(k_amp_rowdet, dim3(nPd, nb), dim3(NT), ldsD, st, P, A.F, nb, ws, Rd, A.state,
+                                   (float *)A.S, A.dpart, PiW, dW);
+                ctx->time_end(kAmpDet);
+                hipLaunchKernelGGL(k_amp_stat, dim3(nb), dim3(64), 0, st, A.dpart, nPd, W, A.state, A.mean);
+                hipLaunchKernelGGL(k_amp_rowfwd, dim3(nPf, nb), dim3(NT), ldsF, st, (const float *)A.S, nb, W, Rf, A.mean,
+                                   A.state, A.F, A.ppart, PfW);
+                hipLaunchKernelGGL(k_amp_colx2, dim3((ws + Cc - 1) / Cc, nb), dim3(NT), ldsC, st, A.F, W, A.state, A.S,
+                                   lcc, PfW, PiW, dW);
+                hipLaunchKernelGGL(k_amp_psum, dim3((nb + 127) / 128), dim3(128), 0, st, A.ppart, nPf, nb, W, A.state,
+                                   A.p);
+                ctx->time_begin(kAmpPeak);
+                hipLaunchKernelGGL(k_amp_rowmag, dim3(nPm, nb), dim3(NT), ldsM, st, A.S, W, Rm, A.mag, A.part, PiW, dW,
+                                   A.state);
+                const GaussArg G{gauss ? 1 : 0, W / 2, 1.0, 16.0, A.p};
+                if (gauss) {
+                    hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, A.part, nPm, A.mag, W, 1, A.res, G);
+                    hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, A.mag, W, 1, A.res, nb, G);
+                } else
+                    hipLaunchKernelGGL(k_peak_merge<false>, dim3(nb), dim3(NT), 0, st, A.part, nPm, A.mag, W, 1, A.res,
+                                       G);
+                ctx->time_end(kAmpPeak);
+                hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, A.res, nb, A.sig);
+                hipLaunchKernelGGL(k_amp_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, large, A);
+                ST_HIP(hipGetLastError());
+            }
+            ar.used = mark;
+            if (!large)
+                ST_HIP(hipEventRecord(sev[3], st));
+        }
+        // the doubled-window stage ends here also when it did not run (no
+        // failed small tries, or one try allowed), so its time reads zero
+        ST_HIP(hipEventRecord(sev[4], st));
+
+        hipLaunchKernelGGL(k_compose, dim3((npts + 255) / 256), dim3(256), 0, st, P, S, dOffR, dOffA, npts);
+        ST_HIP(hipGetLastError());
+        if (cull) {
+            ctx->check_cancel();
+            st_cull_chain_device(ctx, cull, dOffR, dOffA, S.corr, S.type);
+        }
+        int counts[CNT_N];
+        ST_HIP(hipMemcpyAsync(counts, S.counts, sizeof(counts), hipMemcpyDeviceToHost, st));
+        // results go through a pinned staging buffer (a device-to-pageable copy
+        // takes the driver's slower staged path) and are spread into the
+        // caller's arrays after the stream has drained
+        const size_t f4 = (size_t)npts * sizeof(float);
+        char *stg = (char *)ctx->pinned_reserve(4 * f4 + (size_t)npts);
+        std::vector<std::tuple<void *, const void *, size_t>> spread = {
+            {out->offR, stg, f4}, {out->offA, stg + f4, f4}, {out->corr, stg + 2 * f4, f4},
+            {out->type, stg + 3 * f4, (size_t)npts}};
+        ST_HIP(hipMemcpyAsync(stg, dOffR, f4, hipMemcpyDeviceToHost, st));
+        ST_HIP(hipMemcpyAsync(stg + f4, dOffA, f4, hipMemcpyDeviceToHost, st));
+        ST_HIP(hipMemcpyAsync(stg + 2 * f4, S.corr, f4, hipMemcpyDeviceToHost, st));
+        ST_HIP(hipMemcpyAsync(stg + 3 * f4, S.type, npts, hipMemcpyDeviceToHost, st));
+        if (out->azd) {
+            ST_HIP(hipMemcpyAsync(stg + 3 * f4 + npts, S.azd, f4, hipMemcpyDeviceToHost, st));
+            spread.emplace_back(out->azd, stg + 3 * f4 + npts, f4);
+        }
+        if (diag) {
+            auto cp = [&](void *h, const void *d, size_t bytes) {
+                if (h)
+                    ST_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
+            };
+            cp(diag->r2, S.r2, npts * 4);
+            cp(diag->a2, S.a2, npts * 4);
+            for (int t = 0; t < 3; t++) {
+                size_t o = (size_t)t * npts;
+                cp(diag->peakI[t], S.peakI + o, npts * 4);
+                cp(diag->peakJ[t], S.peakJ + o, npts * 4);
+                cp(diag->fineI[t], S.fineI + o, npts * 4);
+                cp(diag->fineJ[t], S.fineJ + o, npts * 4);
+                cp(diag->cmax[t], S.cmax + o, npts * 8);
+                cp(diag->sigRg[t], S.sigRg + o, npts * 4);
+                cp(diag->sigAz[t], S.sigAz + o, npts * 4);
+                cp(diag->peakGap[t], S.peakGap + o, npts * 4);
+                cp(diag->fineGap[t], S.fineGap + o, npts * 4);
+                cp(diag->snr[t], S.snr + o, npts * 4);
+            }
+        }
+        std::vector<uint8_t> azv;
+        std::vector<int> azs;
+        if (diag && diag->azShift) {
+            azv.resize(npts);
+            azs.resize(npts);
+            ST_HIP(hipMemcpyAsync(azv.data(), S.azValid, npts, hipMemcpyDeviceToHost, st));
+            ST_HIP(hipMemcpyAsync(azs.data(), S.azShift, npts * 4, hipMemcpyDeviceToHost, st));
+        }
+        ST_HIP(hipEventRecord(sev[5], st));
+        ST_HIP(hipStreamSynchronize(st));
+        parallel_copy(spread);
+        ctx->flush_timings();
+        {
+            float ms;
+            ST_HIP(hipEventElapsedTime(&ms, sev[0], sev[1]));
+            ctx->stats.stageMs[0] = ms;
+            ST_HIP(hipEventElapsedTime(&ms, sev[1], sev[2]));
+            ctx->stats.stageMs[1] = ms;
+            ST_HIP(hipEventElapsedTime(&ms, sev[2], sev[3]));
+            ctx->stats.stageMs[2] = ms;
+            ST_HIP(hipEventElapsedTime(&ms, sev[3], sev[4]));
+            ctx->stats.stageMs[3] = ms;
+            ST_HIP(hipEventElapsedTime(&ms, sev[0], sev[5]));
+            ctx->stats.totalMs = ms;
+            ctx->stats.nPoints += npts;
+            for (int t = 0; t < 3; t++)
+                ctx->stats.nCorr[t] += counts[CNT_CORR0 + t];
+        }
+        if (diag && diag->azShift)
+            for (int k = 0; k < npts; k++)
+                diag->azShift[k] = azv[k] ? azs[k] : -1;
+        out->nComplex = counts[CNT_COMPLEX];
+        out->nAmp = counts[CNT_AMP];
+        out->nAmpL = counts[CNT_AMPL];
+        out->nFail = counts[CNT_FAIL];
     }
-    int counts[CNT_N];
-    ST_HIP(hipMemcpyAsync(counts, S.counts, sizeof(counts), hipMemcpyDeviceToHost, st));
-    // results go through a pinned staging buffer (a device-to-pageable copy
-    // takes the driver's slower staged path) and are spread into the
-    // caller's arrays after the stream has drained
-    const size_t f4 = (size_t)npts * sizeof(float);
-    char *stg = (char *)ctx->pinned_reserve(4 * f4 + (size_t)npts);
-    std::vector<std::tuple<void *, const void *, size_t>> spread = {
-        {out->offR, stg, f4}, {out->offA, stg + f4, f4}, {out->corr, stg + 2 * f4, f4},
-        {out->type, stg + 3 * f4, (size_t)npts}};
-    ST_HIP(hipMemcpyAsync(stg, dOffR, f4, hipMemcpyDeviceToHost, st));
-    ST_HIP(hipMemcpyAsync(stg + f4, dOffA, f4, hipMemcpyDeviceToHost, st));
-    ST_HIP(hipMemcpyAsync(stg + 2 * f4, S.corr, f4, hipMemcpyDeviceToHost, st));
-    ST_HIP(hipMemcpyAsync(stg + 3 * f4, S.type, npts, hipMemcpyDeviceToHost, st));
-    if (out->azd) {
-        ST_HIP(hipMemcpyAsync(stg + 3 * f4 + npts, S.azd, f4, hipMemcpyDeviceToHost, st));
-        spread.emplace_back(out->azd, stg + 3 * f4 + npts, f4);
-    }
-    if (diag) {
-        auto cp = [&](void *h, const void *d, size_t bytes) {
-            if (h)
-                ST_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
-        };
-        cp(diag->r2, S.r2, npts * 4);
-        cp(diag->a2, S.a2, npts * 4);
-        for (int t = 0; t < 3; t++) {
-            size_t o = (size_t)t * npts;
-            cp(diag->peakI[t], S.peakI + o, npts * 4);
-            cp(diag->peakJ[t], S.peakJ + o, npts * 4);
-            cp(diag->fineI[t], S.fineI + o, npts * 4);
-            cp(diag->fineJ[t], S.fineJ + o, npts * 4);
-            cp(diag->cmax[t], S.cmax + o, npts * 8);
-            cp(diag->sigRg[t], S.sigRg + o, npts * 4);
-            cp(diag->sigAz[t], S.sigAz + o, npts * 4);
-            cp(diag->peakGap[t], S.peakGap + o, npts * 4);
-            cp(diag->fineGap[t], S.fineGap + o, npts * 4);
+};
+
+// The fused launches of a batch: one per window class (and sample type,
+// correction set), over every call of that class.
+void run_fused_group(st_ctx *ctx, std::vector<StrackJob *> &group)
+{
+    std::vector<FusedJob> fj;
+    for (StrackJob *j : group)
+        fj.push_back(j->fused_part());
+    const int w = group[0]->w, slot = group[0]->kPeak;
+    if (w == 16)
+        launch_cpx_fused<16>(ctx, slot, fj);
+    else if (w == 32)
+        launch_cpx_fused<32>(ctx, slot, fj);
+    else
+        launch_cpx_fused<64>(ctx, slot, fj);
+    for (StrackJob *j : group)
+        j->fused_finish();
+}
+
+// Reserve the arena for a batch: every call's fixed part and fused
+// results, plus stage scratch for the largest staged / amplitude batch; a
+// budget the device cannot hold is halved until it fits.
+void reserve_batch(st_ctx *ctx, std::vector<StrackJob *> &jobs)
+{
+    int ncu = 0;
+    ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    size_t budget = ctx->scratchBudget;
+    const size_t kMaxBatch = 8192;  // keeps 2 x batch inside the 65535 grid-z limit of the transposes
+    for (;;) {
+        size_t need = 65536, stage = 0;
+        bool all1 = true;
+        for (StrackJob *j : jobs) {
+            j->Bc = (int)std::max<size_t>(1, std::min<size_t>({(size_t)j->npts, budget / j->perCpx, kMaxBatch}));
+            j->Ba = (int)std::max<size_t>(1, std::min<size_t>({(size_t)j->npts, budget / j->perAmp, kMaxBatch}));
+            all1 = all1 && j->Bc == 1 && j->Ba == 1;
+            need += j->fixed + j->fused_bytes(ncu) + 2048;
+            stage = std::max(stage, std::max((size_t)j->Bc * j->perCpx, (size_t)j->Ba * j->perAmp));
+        }
+        need += stage;
+        try {
+            ctx->arena_reserve(need);
+            return;
+        } catch (const StError &e) {
+            if (e.code != ST_ERR_NOMEM || all1)
+                throw;
+            budget /= 2;
         }
     }
-    std::vector<uint8_t> azv;
-    std::vector<int> azs;
-    if (diag && diag->azShift) {
-        azv.resize(npts);
-        azs.resize(npts);
-        ST_HIP(hipMemcpyAsync(azv.data(), S.azValid, npts, hipMemcpyDeviceToHost, st));
-        ST_HIP(hipMemcpyAsync(azs.data(), S.azShift, npts * 4, hipMemcpyDeviceToHost, st));
+}
+
+// The calls of a batch, start to end.  Results of each call are those it
+// gives alone: every kernel computes its items independently.
+void run_strack_batch(st_ctx *ctx, std::vector<StrackJob *> &jobs)
+{
+    init_dirichlet(ctx);
+    ctx->stats_reset();
+    for (StrackJob *j : jobs)
+        j->plan();
+    reserve_batch(ctx, jobs);
+    ST_HIP(hipEventRecord(ctx->ev[64], ctx->stream));
+    for (StrackJob *j : jobs)
+        j->setup();
+    const size_t mark = ctx->arena.used;
+    for (StrackJob *j : jobs)
+        j->mark = mark;
+    // the fused calls, grouped by window, sample type and corrections
+    std::map<std::tuple<int, int, int>, std::vector<StrackJob *>> groups;
+    for (StrackJob *j : jobs)
+        if (j->fusedPath && !j->mfmaPath && j->nCpx > 0)
+            groups[std::make_tuple(j->w, j->pr.stype, (j->P.baseFlag ? 1 : 0) | (j->P.intOn ? 2 : 0))].push_back(j);
+    for (auto &g : groups)
+        run_fused_group(ctx, g.second);
+    ctx->arena.used = mark;
+    for (StrackJob *j : jobs) {
+        j->complex_alone();
+        j->finish();
     }
-    ST_HIP(hipEventRecord(sev[5], st));
-    ST_HIP(hipStreamSynchronize(st));
-    parallel_copy(spread);
-    ctx->flush_timings();
-    {
-        float ms;
-        ST_HIP(hipEventElapsedTime(&ms, sev[0], sev[1]));
-        ctx->stats.stageMs[0] = ms;
-        ST_HIP(hipEventElapsedTime(&ms, sev[1], sev[2]));
-        ctx->stats.stageMs[1] = ms;
-        ST_HIP(hipEventElapsedTime(&ms, sev[2], sev[3]));
-        ctx->stats.stageMs[2] = ms;
-        ST_HIP(hipEventElapsedTime(&ms, sev[3], sev[4]));
-        ctx->stats.stageMs[3] = ms;
-        ST_HIP(hipEventElapsedTime(&ms, sev[0], sev[5]));
-        ctx->stats.totalMs = ms;
-        ctx->stats.nPoints = npts;
-        for (int t = 0; t < 3; t++)
-            ctx->stats.nCorr[t] = counts[CNT_CORR0 + t];
+}
+
+}  // namespace
+
+void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, st_track_out *out,
+                           st_track_diag *diag, st_cull_params *cull)
+{
+    StrackJob job(ctx, pr, par, out, diag, cull);
+    std::vector<StrackJob *> jobs{&job};
+    run_strack_batch(ctx, jobs);
+}
+
+void st_speckle_track_batch_impl(st_ctx *ctx, int n, StPair *const *prs, const st_track_params *const *pars,
+                                 st_track_out *const *outs)
+{
+    std::vector<std::unique_ptr<StrackJob>> own;
+    std::vector<StrackJob *> jobs;
+    for (int k = 0; k < n; k++) {
+        own.emplace_back(new StrackJob(ctx, *prs[k], *pars[k], outs[k], nullptr, nullptr));
+        jobs.push_back(own.back().get());
     }
-    if (diag && diag->azShift)
-        for (int k = 0; k < npts; k++)
-            diag->azShift[k] = azv[k] ? azs[k] : -1;
-    out->nComplex = counts[CNT_COMPLEX];
-    out->nAmp = counts[CNT_AMP];
-    out->nAmpL = counts[CNT_AMPL];
-    out->nFail = counts[CNT_FAIL];
+    run_strack_batch(ctx, jobs);
 }

@@ -1087,6 +1087,8 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
                     diag->peakGap[t][k] = 0;
                 if (diag->fineGap[t])
                     diag->fineGap[t][k] = 0;
+                if (diag->snr[t])
+                    diag->snr[t][k] = 0;
             }
         }
     }

@@ -33,6 +33,7 @@ struct PeakRes {
     int I, J;         // coarse peak in the fftshifted surface, -1000 = rejected
     float raw;        // |C| at the coarse peak (findCPeak's cMax), -1 when rejected
     float gap;        // (best - runner-up) / best on the coarse surface
+    float snr;        // coarse peak / mean |C| over the surface
     int fi, fj;       // oversampled-box argmax (0..47) or -1
     double fineMax;   // |Q| at the refined peak
     float fineGap;

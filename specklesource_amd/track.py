@@ -46,6 +46,8 @@ def load_library(path=LIB_PATH):
     lib.st_speckle_track.argtypes = [vp, C.c_int32, C.POINTER(abi.TrackParams),
                                      C.POINTER(abi.TrackOut), C.POINTER(abi.TrackDiag)]
     lib.st_corr_track_fast.argtypes = lib.st_speckle_track.argtypes
+    lib.st_speckle_track_batch.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(abi.TrackParams),
+                                           C.POINTER(abi.TrackOut)]
     for name in ("st_cull_st_data", "st_cull_stats", "st_cull_smooth",
                  "st_cull_islands", "st_cull_run", "st_cull_run_noalloc"):
         getattr(lib, name).argtypes = [vp, C.POINTER(abi.CullParams)]
@@ -127,7 +129,7 @@ def alloc_outputs(par, want_diag=False):
                            ("fineI", np.int32, C.c_int32), ("fineJ", np.int32, C.c_int32),
                            ("cmax", np.float64, C.c_double), ("sigRg", np.float32, C.c_float),
                            ("sigAz", np.float32, C.c_float), ("peakGap", np.float32, C.c_float),
-                           ("fineGap", np.float32, C.c_float)):
+                           ("fineGap", np.float32, C.c_float), ("snr", np.float32, C.c_float)):
             arrs = [np.zeros(shape, dt) for _ in range(3)]
             d[nm] = arrs
             fld = getattr(dg, nm)
@@ -215,6 +217,20 @@ class Device:
                                        C.byref(dg) if dg is not None else None)
         self._check(rc, "st_speckle_track")
         return finish(res, out)
+
+    def speckle_track_batch(self, pairs, pars):
+        """Several strack calls as one batch (st_speckle_track_batch); a
+        list of TrackResult in the order of pairs / pars."""
+        n = len(pairs)
+        res, outs = [], (abi.TrackOut * n)()
+        for k, p in enumerate(pars):
+            r, o, _ = alloc_outputs(p)
+            res.append(r)
+            outs[k] = o
+        pa = (abi.TrackParams * n)(*pars)
+        pi = (C.c_int32 * n)(*pairs)
+        self._check(self.lib.st_speckle_track_batch(self.ctx, n, pi, pa, outs), "st_speckle_track_batch")
+        return [finish(r, outs[k]) for k, r in enumerate(res)]
 
     def corr_track_fast(self, pair, par, diag=False):
         res, out, dg = alloc_outputs(par, diag)

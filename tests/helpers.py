@@ -16,6 +16,9 @@ THRESH_TOL = 1e-4
 # ratio, the .azd product: the GPU's exp and 3x3 solve round differently
 # from the CPU's in the last bits, and five iterations amplify that.
 LM_RTOL = 2e-3
+# Relative tolerance on the SNR diagnostic (peak over the mean |C| of the
+# surface: an FP32 sum over up to 512^2 values against the FP64 oracle's).
+SNR_RTOL = 1e-3
 
 
 def periodic_pair(w, nr, na, dr, da, seed=0, band=True):
@@ -126,6 +129,18 @@ def compare_strack(par, g, o32, o64, tie=TIE, thresh_tol=THRESH_TOL, fine_tie=1e
                 probs.append((nm, t, float(a[k]), float(b[k])))
     za, zb = o32.azd[all_same].astype(np.float64), g.azd[all_same].astype(np.float64)
     ez = np.abs(za - zb) / np.maximum(np.abs(za), 1e-6)
+    # SNR against the FP64 oracle wherever the attempt's peak agrees
+    d64 = o64.diag
+    snr_err = []
+    for t in range(3):
+        same = (dg["peakI"][t] > -1000) & (d64["peakI"][t] == dg["peakI"][t]) & (d64["peakJ"][t] == dg["peakJ"][t])
+        a, b = dg["snr"][t][same].astype(np.float64), d64["snr"][t][same].astype(np.float64)
+        e = np.abs(a - b) / np.maximum(np.abs(b), 1e-12)
+        snr_err += list(e)
+        for k in np.nonzero(e > SNR_RTOL)[0]:
+            probs.append(("snr", t, float(b[k]), float(a[k])))
+    rep["snr_relerr_median"] = float(np.median(snr_err)) if snr_err else None
+    rep["snr_relerr_max"] = float(np.max(snr_err)) if snr_err else None
     rep["lm_sigma_relerr_max"] = lm_err
     rep["azd_compared"] = int(all_same.sum())
     rep["azd_relerr_max"] = float(ez.max()) if ez.size else 0.0

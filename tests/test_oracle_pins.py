@@ -90,11 +90,13 @@ def _find_cpeak(o, mag_shifted):
     Cz = np.ascontiguousarray(Cs.astype(o.dtype))
     real = np.float32 if o.prec == "f32" else np.float64
     mag = np.zeros((WA, WR), real)
-    i, j, c = C.c_int(), C.c_int(), C.c_double()
+    i, j, c, snr = C.c_int(), C.c_int(), C.c_double(), C.c_double()
     o.lib.orc_find_cpeak.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int),
-                                     C.POINTER(C.c_int), C.POINTER(C.c_double)]
-    o.lib.orc_find_cpeak(Cz.ctypes.data, WR, WA, mag.ctypes.data, C.byref(i), C.byref(j), C.byref(c))
+                                     C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    o.lib.orc_find_cpeak(Cz.ctypes.data, WR, WA, mag.ctypes.data, C.byref(i), C.byref(j), C.byref(c),
+                         C.byref(snr))
     assert np.allclose(mag, mag_shifted)
+    _find_cpeak.snr = snr.value
     return i.value, j.value, c.value
 
 
@@ -206,3 +208,18 @@ def test_get_shifts_takes_the_largest_corner_next_to_a_hole(oracle32, oracle64):
     par4, keep4 = _grid_params(dr2 * 0 + 1, da4)
     for o in (oracle32, oracle64):
         assert _shift(o, par4, 105.0, 60.0) == (-2e9, -2e9)
+
+
+# ------------------------------------------------------- SNR diagnostic
+def test_snr_is_peak_over_mean_surface_magnitude(oracle32, oracle64):
+    """SNR = the coarse peak over the mean |C| of the whole surface (a
+    diagnostic the reference does not compute): a flat surface of ones with
+    one peak of 9 on a 32 x 32 surface gives 9 * 1024 / (1023 + 9); a flat
+    surface gives 1."""
+    mag = np.ones((32, 32))
+    mag[20, 12] = 9.0
+    for o in (oracle32, oracle64):
+        _find_cpeak(o, mag)
+        assert _find_cpeak.snr == pytest.approx(9 * 1024 / 1032, rel=1e-6)
+        _find_cpeak(o, np.ones((32, 32)) * 3.0)
+        assert _find_cpeak.snr == pytest.approx(1.0, rel=1e-6)
