@@ -357,6 +357,7 @@ def served_cpu_and_accuracy(pairs, budget_s=12.0):
     acc = {}
     for prec_name, prec in (("fp32", abi.ST_PREC_FP32), ("mfma", abi.ST_PREC_MFMA)):
         tot = agree = ties = unexpl = 0
+        kinds = {}
         er, ea, cr, sr = [], [], [], []
         for sub, img1, img2 in jobs:
             q = abi.TrackParams.from_buffer_copy(sub)
@@ -372,6 +373,8 @@ def served_cpu_and_accuracy(pairs, budget_s=12.0):
             agree += round(rep["peak_agree"] * rep["attempts"])
             ties += rep["near_ties"]
             unexpl += len(probs)
+            for pr in probs:
+                kinds[str(pr[0])] = kinds.get(str(pr[0]), 0) + 1
             both = (g.type > 0) & (r64.type == g.type)
             er += list((g.offR - r64.offR)[both].astype(np.float64))
             ea += list((g.offA - r64.offA)[both].astype(np.float64))
@@ -384,7 +387,7 @@ def served_cpu_and_accuracy(pairs, budget_s=12.0):
         acc[prec_name] = {
             "sample_points": int(sum(j[0].nR * j[0].nA for j in jobs)), "attempts": tot,
             "integer_peak_agreement_vs_fp32_oracle": agree / max(tot, 1), "near_ties": ties,
-            "unexplained_mismatches": unexpl,
+            "unexplained_mismatches": unexpl, "mismatch_kinds": kinds,
             "rms_offset_px_vs_fp64": [float(np.sqrt(np.mean(np.square(er)))) if er else None,
                                       float(np.sqrt(np.mean(np.square(ea)))) if ea else None],
             "corr_relerr_median_vs_fp64": float(np.median(cr)) if cr else None,
@@ -431,6 +434,7 @@ def bench_served(args):
     from specklesource_amd import abi, configs, track
     pairs = configs.cfg3_pairs(n_pairs=args.pairs, scale=args.scale)
     srv = track.Server(0, slots=args.slots, hbm_budget=200 << 30)
+    srv.set_batch(args.batch)
     t0 = time.perf_counter()
     pids = [srv.synth_pair(s) for s, _ in pairs]
     print("generated %d pairs in HBM in %.1f s" % (len(pids), time.perf_counter() - t0), file=sys.stderr, flush=True)
@@ -447,7 +451,8 @@ def bench_served(args):
         "vs_baseline": None, "dtype": "fp32 (fp64 sub-pixel refinement and normalisation sums)",
         "data": "synthetic (generated in HBM)",
         "config": {"workload": "%d uneven pairs 1024-8192 px, w in {16,32,64,128}, grid w/2, search w/4, "
-                               "served on %d slots" % (len(pairs), args.slots), "windows": ws},
+                               "served on %d slots" % (len(pairs), args.slots), "windows": ws,
+                   "batch": args.batch},
         "latency_ms": {"p50": float(np.percentile(l32, 50)), "p99": float(np.percentile(l32, 99))},
         "precisions": {
             "fp32": {"value": v32, "latency_ms": {"p50": float(np.percentile(l32, 50)),
@@ -557,6 +562,7 @@ def main():
     ap.add_argument("--workload", choices=["served", "fullscene", "amponly", "strackw", "cull", "batched"],
                     default="served")
     ap.add_argument("--no-roofline", action="store_true", help="served workload: skip the one-context roofline pass")
+    ap.add_argument("--batch", type=int, default=1, help="served workload: strack jobs per fused batch (1: off)")
     ap.add_argument("--precision", choices=["fp32", "mfma"], default="fp32")
     ap.add_argument("--pairs", type=int, default=64)
     ap.add_argument("--slots", type=int, default=4)

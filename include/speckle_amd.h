@@ -357,6 +357,12 @@ int st_server_pair_upload(st_server *srv, const void *img1, int32_t nr1, int32_t
  * every submitted job that names it has ended, then frees it.  A second
  * release of the same pair fails with ST_ERR_ARG. */
 int st_server_pair_release(st_server *srv, int32_t pair);
+/* Up to maxJobs ready strack jobs whose complex matches run on the fused
+ * FP32 kernel (windows 16, 32, 64) are taken by one slot as a batch and
+ * share one persistent launch per window class (st_speckle_track_batch);
+ * each job's results are the ones it gives alone, and all of them end
+ * together.  1 (the default) turns batching off. */
+int st_server_set_batch(st_server *srv, int32_t maxJobs);
 /* Queue a job; returns its id at once. */
 int st_submit(st_server *srv, const st_job *job, int64_t *id);
 /* State of a job without waiting (info may be NULL). */

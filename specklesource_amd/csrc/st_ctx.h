@@ -100,9 +100,14 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
                            st_track_diag *diag, st_cull_params *cull = nullptr);
 void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, st_track_out *out,
                              st_track_diag *diag);
-// n strack calls as one batch (shared fused launches per window class).
+// n strack calls as one batch (shared fused launches per window class);
+// culls may be null, or hold a cull chain (or null) per call.
 void st_speckle_track_batch_impl(st_ctx *ctx, int n, StPair *const *prs, const st_track_params *const *pars,
-                                 st_track_out *const *outs);
+                                 st_track_out *const *outs, st_cull_params *const *culls = nullptr,
+                                 int64_t *patches = nullptr);
+// Whether a strack call's complex matches run on the fused FP32 kernel
+// (what a batch shares).
+bool strack_batchable(const st_track_params &p);
 
 // The cullst call order (Cullst/cullst.c:75-100) on grids already on the
 // device: offR / offA / type are culled in place, corr is read; the other

@@ -75,7 +75,6 @@ struct st_server {
     size_t slotScratch = 0;
     uint64_t used = 0;  // slot arenas + resident pairs + staged images
     std::vector<st_ctx *> ctxs;
-    std::vector<int64_t> running;  // job id per slot, -1 when idle
     std::vector<std::thread> workers;
     std::thread stagerThread;
     std::mutex mu;
@@ -90,6 +89,8 @@ struct st_server {
     int64_t nextId = 1;
     bool stop = false;
     hipStream_t up = nullptr;
+    int maxBatch = 1;  // strack jobs per fused batch (st_server_set_batch)
+    static constexpr size_t kMaxBatchPoints = (size_t)4 << 20;  // grid points per batch
 
     // Called with mu held: the job has ended (any state); drop its pair
     // reference and wake whoever waits.
@@ -220,37 +221,111 @@ struct st_server {
         jb.status = rc;
     }
 
+    static bool batchable(const Job &jb)
+    {
+        return jb.j.kind == ST_JOB_STRACK && strack_batchable(jb.j.par);
+    }
+    static size_t points(const Job &jb) { return (size_t)std::max(0, jb.j.par.nR) * std::max(0, jb.j.par.nA); }
+
+    // Several ready strack jobs on the fused path run as one batch: their
+    // complex matches share one persistent launch per window class.  If
+    // the batch fails (a job's own error, or a cancel), every job that was
+    // not cancelled runs again alone, so each gets its own status.
+    void run_batch(int slot, std::vector<Job *> &batch)
+    {
+        st_ctx *c = ctxs[slot];
+        const int n = (int)batch.size();
+        std::vector<StPair> prs(n);
+        std::vector<StPair *> pp(n);
+        std::vector<const st_track_params *> ps(n);
+        std::vector<st_track_out *> os(n);
+        std::vector<st_cull_params *> cs(n);
+        std::vector<int64_t> pat(n, 0);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (int k = 0; k < n; k++) {
+                Job &jb = *batch[k];
+                prs[k] = jb.j.pair >= 0 ? pairs.at(jb.j.pair).p : jb.img;
+                pp[k] = &prs[k];
+                ps[k] = &jb.j.par;
+                os[k] = &jb.j.out;
+                cs[k] = jb.j.cull;
+            }
+        }
+        const int rc = st_guard(c, [&]() {
+            st_speckle_track_batch_impl(c, n, pp.data(), ps.data(), os.data(), cs.data(), pat.data());
+        });
+        if (rc == ST_OK) {
+            for (int k = 0; k < n; k++) {
+                batch[k]->status = ST_OK;
+                batch[k]->patches = pat[k];
+            }
+            return;
+        }
+        for (Job *jb : batch) {
+            bool cancelled;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                cancelled = jb->cancel;
+                c->cancel.store(cancelled);
+            }
+            if (cancelled) {
+                jb->status = ST_ERR_CANCELLED;
+                jb->err = "job cancelled";
+            } else {
+                run(slot, *jb);
+            }
+        }
+    }
+
     void worker(int slot)
     {
         for (;;) {
-            Job *jb = nullptr;
-            int64_t id = -1;
+            std::vector<Job *> batch;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 work.wait(lk, [&]() { return !ready.empty() || (stop && queue.empty() && !any_staging()); });
                 if (ready.empty())
                     return;
-                id = ready.front();
+                batch.push_back(&jobs.at(ready.front()));
                 ready.pop_front();
-                jb = &jobs.at(id);
-                jb->state = ST_JOB_RUNNING;
-                jb->slot = slot;
-                jb->tStart = Clock::now();
-                running[slot] = id;
-                ctxs[slot]->cancel.store(jb->cancel);
+                if (maxBatch > 1 && batchable(*batch[0])) {
+                    size_t pts = points(*batch[0]);
+                    for (auto it = ready.begin(); it != ready.end() && (int)batch.size() < maxBatch;) {
+                        Job &o = jobs.at(*it);
+                        if (batchable(o) && pts + points(o) <= kMaxBatchPoints) {
+                            pts += points(o);
+                            batch.push_back(&o);
+                            it = ready.erase(it);
+                        } else {
+                            ++it;
+                        }
+                    }
+                }
+                const auto now = Clock::now();
+                for (Job *jb : batch) {
+                    jb->state = ST_JOB_RUNNING;
+                    jb->slot = slot;
+                    jb->tStart = now;
+                }
+                ctxs[slot]->cancel.store(false);
             }
-            stage.notify_all();  // a ready place freed up
-            run(slot, *jb);
+            stage.notify_all();  // ready places freed up
+            if (batch.size() == 1)
+                run(slot, *batch[0]);
+            else
+                run_batch(slot, batch);
             {
                 std::lock_guard<std::mutex> lk(mu);
-                running[slot] = -1;
                 ctxs[slot]->cancel.store(false);
-                if (jb->j.pair < 0 && jb->j.kind != ST_JOB_CULL) {
-                    free_pair(jb->img);
-                    used -= jb->bytes;
+                for (Job *jb : batch) {
+                    if (jb->j.pair < 0 && jb->j.kind != ST_JOB_CULL) {
+                        free_pair(jb->img);
+                        used -= jb->bytes;
+                    }
+                    const bool cancelled = jb->status == ST_ERR_CANCELLED;
+                    finish(*jb, jb->status == ST_OK ? ST_JOB_DONE : (cancelled ? ST_JOB_CANCELLED : ST_JOB_FAILED));
                 }
-                const bool cancelled = jb->status == ST_ERR_CANCELLED;
-                finish(*jb, jb->status == ST_OK ? ST_JOB_DONE : (cancelled ? ST_JOB_CANCELLED : ST_JOB_FAILED));
             }
             stage.notify_all();
             work.notify_all();
@@ -325,7 +400,6 @@ extern "C" int st_server_create(int32_t device, int32_t slots, uint64_t hbmBudge
     s->used = (uint64_t)slots * s->slotScratch;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess)
         return fail(ST_ERR_HIP);
-    s->running.assign(slots, -1);
     for (int k = 0; k < slots; k++)
         s->workers.emplace_back([s, k]() { s->worker(k); });
     s->stagerThread = std::thread([s]() { s->stager(); });
@@ -416,6 +490,15 @@ extern "C" int st_server_pair_release(st_server *s, int32_t pair)
     s->used -= it->second.bytes;
     s->pairs.erase(it);
     s->stage.notify_all();
+    return ST_OK;
+}
+
+extern "C" int st_server_set_batch(st_server *s, int32_t maxJobs)
+{
+    if (!s || maxJobs < 1 || maxJobs > 64)
+        return ST_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->maxBatch = maxJobs;
     return ST_OK;
 }
 

@@ -2645,6 +2645,7 @@ struct StrackJob {
     const int *dR2 = nullptr, *dA2 = nullptr;
     int nCpx = 0;
     size_t mark = 0;  // arena position shared by the batch's stage scratch
+    int64_t patches = 0;  // patch correlations of this call
     CpxBatch C{};     // fused path: the per-item results of this call
     struct OvfBuf {
         void *p = nullptr;
@@ -3154,8 +3155,10 @@ struct StrackJob {
             ST_HIP(hipEventElapsedTime(&ms, sev[0], sev[5]));
             ctx->stats.totalMs = ms;
             ctx->stats.nPoints += npts;
-            for (int t = 0; t < 3; t++)
+            for (int t = 0; t < 3; t++) {
                 ctx->stats.nCorr[t] += counts[CNT_CORR0 + t];
+                patches += counts[CNT_CORR0 + t];
+            }
         }
         if (diag && diag->azShift)
             for (int k = 0; k < npts; k++)
@@ -3256,13 +3259,22 @@ void st_speckle_track_impl(st_ctx *ctx, StPair &pr, const st_track_params &par, 
 }
 
 void st_speckle_track_batch_impl(st_ctx *ctx, int n, StPair *const *prs, const st_track_params *const *pars,
-                                 st_track_out *const *outs)
+                                 st_track_out *const *outs, st_cull_params *const *culls, int64_t *patches)
 {
     std::vector<std::unique_ptr<StrackJob>> own;
     std::vector<StrackJob *> jobs;
     for (int k = 0; k < n; k++) {
-        own.emplace_back(new StrackJob(ctx, *prs[k], *pars[k], outs[k], nullptr, nullptr));
+        own.emplace_back(new StrackJob(ctx, *prs[k], *pars[k], outs[k], nullptr, culls ? culls[k] : nullptr));
         jobs.push_back(own.back().get());
     }
     run_strack_batch(ctx, jobs);
+    if (patches)
+        for (int k = 0; k < n; k++)
+            patches[k] = jobs[k]->patches;
+}
+
+bool strack_batchable(const st_track_params &p)
+{
+    return p.wR == p.wA && p.wRa == p.wAa && (p.wR == 16 || p.wR == 32 || p.wR == 64) &&
+           p.precision == ST_PREC_FP32 && !p.gaussFlag && !p.intf && !p.baseFlag && !p.noComplex && !staged_only();
 }

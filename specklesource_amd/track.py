@@ -65,6 +65,7 @@ def load_library(path=LIB_PATH):
     lib.st_poll.argtypes = [vp, C.c_int64, C.POINTER(abi.JobInfo)]
     lib.st_wait.argtypes = [vp, C.c_int64, C.POINTER(abi.JobInfo)]
     lib.st_cancel.argtypes = [vp, C.c_int64]
+    lib.st_server_set_batch.argtypes = [vp, C.c_int32]
     lib.st_job_release.argtypes = [vp, C.c_int64]
     lib.st_set_profiling.argtypes = [vp, C.c_int32]
     lib.st_synth_host.argtypes = [C.POINTER(abi.SynthSpec), vp, vp]
@@ -354,6 +355,11 @@ class Server:
         if self.lib.st_poll(self.srv, jid, C.byref(info)) != abi.ST_OK:
             raise RuntimeError("unknown job %d" % jid)
         return info
+
+    def set_batch(self, max_jobs):
+        """Ready strack jobs on the fused path taken as one batch (1: off)."""
+        if self.lib.st_server_set_batch(self.srv, max_jobs) != abi.ST_OK:
+            raise RuntimeError("st_server_set_batch failed")
 
     def cancel(self, jid):
         """Ask the server to cancel a job (st_cancel); False when the job

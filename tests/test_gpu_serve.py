@@ -186,3 +186,31 @@ def test_pair_release_waits_for_its_jobs_and_refuses_new_ones(device):
             srv.release_pair(rp)
     finally:
         srv.close()
+
+
+def test_batched_jobs_equal_jobs_alone(device):
+    """Many small fused-path jobs submitted at once are taken in batches by
+    the slots (one fused launch per window class per batch); with batching
+    off the same jobs run one by one.  Every job gives the same grids both
+    ways and alone."""
+    jobs = []
+    for k in range(12):
+        w = (16, 32, 64)[k % 3]
+        n = 256 + 32 * (k % 4)
+        s = configs._spec(n, n, 700 + k, dr0=0.3 * (k % 5) - 0.6, da0=0.2, gmin=0.5, gmax=0.9)
+        jobs.append((s, small_params(w, n, n, step=w // 2, rShiftPoly=[0.3 * (k % 5) - 0.6, 0, 0],
+                                     aShiftPoly=[0.2, 0, 0])))
+    imgs = [track.synth_host(s) for s, _ in jobs]
+    alone = [_alone(device, abi.ST_JOB_STRACK, p, a, b, None)[0] for (_, p), (a, b) in zip(jobs, imgs)]
+    for batch in (8, 1):
+        srv = track.Server(0, slots=2, hbm_budget=32 << 30)
+        try:
+            srv.set_batch(batch)
+            pids = [srv.upload_pair(a, b) for a, b in imgs]
+            ids = [srv.submit(p, pair=pid) for (_, p), pid in zip(jobs, pids)]
+            for k, jid in enumerate(ids):
+                r, info, _ = srv.wait(jid)
+                assert info.patches > 0
+                _same(r, alone[k])
+        finally:
+            srv.close()
