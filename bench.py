@@ -440,21 +440,33 @@ def bench_served(args):
     print("generated %d pairs in HBM in %.1f s" % (len(pids), time.perf_counter() - t0), file=sys.stderr, flush=True)
     v32, w32, l32 = served_passes(srv, pairs, pids, abi.ST_PREC_FP32, args.warmup, args.steps, "fp32")
     vmf, wmf, lmf = served_passes(srv, pairs, pids, abi.ST_PREC_MFMA, args.warmup, args.steps, "mfma")
+    vau, wau, lau = served_passes(srv, pairs, pids, abi.ST_PREC_AUTO, args.warmup, args.steps, "auto")
     srv.close()
+    dev = track.Device(0)
+    tune = {w: dev.tune(w) for w in (16, 32, 64, 128)}
+    dev.close()
     ws = {}
     for _, p in pairs:
         ws[p.wR] = ws.get(p.wR, 0) + 1
     line = {
         "metric": "patch correlations/sec + p99 job latency, 1 MI355X, at <=0.02 px offset error",
-        "value": v32, "unit": "patch correlations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * sum(w32) / len(w32), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32 (fp64 sub-pixel refinement and normalisation sums)",
+        "value": vau, "unit": "patch correlations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * sum(wau) / len(wau), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "per window as tuned (%s; fp64 sub-pixel refinement and normalisation sums)"
+                 % ", ".join("%d: %s" % (w, "fp32" if not t or t[0] == 0 else "bf16 split")
+                             for w, t in tune.items()),
         "data": "synthetic (generated in HBM)",
         "config": {"workload": "%d uneven pairs 1024-8192 px, w in {16,32,64,128}, grid w/2, search w/4, "
                                "served on %d slots" % (len(pairs), args.slots), "windows": ws,
                    "batch": args.batch},
-        "latency_ms": {"p50": float(np.percentile(l32, 50)), "p99": float(np.percentile(l32, 99))},
+        "latency_ms": {"p50": float(np.percentile(lau, 50)), "p99": float(np.percentile(lau, 99))},
+        "tune": {str(w): (None if t is None else {"precision": ["fp32", "mfma"][t[0]], "wg_per_cu": t[1],
+                                                  "staged_batch": t[2], "us_per_patch": t[3]})
+                 for w, t in tune.items()},
         "precisions": {
+            "auto": {"value": vau, "latency_ms": {"p50": float(np.percentile(lau, 50)),
+                                                  "p99": float(np.percentile(lau, 99))}},
             "fp32": {"value": v32, "latency_ms": {"p50": float(np.percentile(l32, 50)),
                                                   "p99": float(np.percentile(l32, 99))}},
             "mfma": {"value": vmf, "latency_ms": {"p50": float(np.percentile(lmf, 50)),

@@ -59,6 +59,7 @@ extern "C" {
  * contractions on the matrix cores (FP32 accumulate). */
 #define ST_PREC_FP32 0
 #define ST_PREC_MFMA 1
+#define ST_PREC_AUTO 2 /* the precision st_autotune found faster for the window (FP32 without a tune entry) */
 
 typedef struct st_ctx st_ctx;
 
@@ -190,6 +191,22 @@ int st_speckle_track_batch(st_ctx *ctx, int32_t n, const int32_t *pairs, const s
  * Replaces corrTrackFast (Strackw/strackw.h:1, corrTrackFast.c:89-231). */
 int st_corr_track_fast(st_ctx *ctx, int32_t pair, const st_track_params *par,
                        st_track_out *out, st_track_diag *diag);
+
+/* ---- autotuning (no counterpart in the reference) -------------------- */
+/* Times the complex-match stage of a synthetic job per window (16, 32,
+ * 64, 128) over the launch choices -- precision for ST_PREC_AUTO, the
+ * fused kernel's workgroups per CU, the staged path's batch cap -- keeps
+ * the fastest for this process and device, and writes them to cacheFile
+ * (text, one line per window: arch window precision wg_per_cu staged_batch
+ * us_per_patch; NULL: not written).  A context reads its architecture's
+ * cache at creation ($SPECKLE_AMD_TUNE, else tune/<arch>.txt beside the
+ * library directory). */
+int st_autotune(st_ctx *ctx, const int32_t *windows, int32_t n, const char *cacheFile);
+/* The tuned choices of a window (ST_ERR_ARG when it has none). */
+int st_tune_get(st_ctx *ctx, int32_t window, int32_t *precision, int32_t *wgPerCu, int32_t *stagedBatch,
+                float *usPerPatch);
+/* Read a tune cache into the process's table for ctx's device. */
+int st_tune_load(st_ctx *ctx, const char *cacheFile);
 
 /* Work counters and device timings of the last tracking call on a context.
  * Not in the reference (it prints line summaries and stage timers instead,

@@ -22,6 +22,7 @@ ST_NODATA = -2.0e9
 
 ST_PREC_FP32 = 0
 ST_PREC_MFMA = 1
+ST_PREC_AUTO = 2
 
 _f = C.POINTER(C.c_float)
 _d = C.POINTER(C.c_double)

@@ -160,6 +160,7 @@ extern "C" int st_ctx_create(int32_t device, st_ctx **out)
         c->scratchBudget = std::min(c->scratchBudget, std::max<size_t>(freeB / 4, (size_t)64 << 20));
     try {
         cull_noalloc_init(device);
+        tune_init(device);
     } catch (const StError &e) {
         (void)hipStreamDestroy(c->stream);
         delete c;

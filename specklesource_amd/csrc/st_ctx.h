@@ -121,3 +121,18 @@ void st_cull_chain_device(st_ctx *ctx, st_cull_params *cp, float *dOffR, float *
 // Resolves the device address of the no-allocation cull's scratch array on
 // `device` (once per device; st_ctx_create calls it).
 void cull_noalloc_init(int device);
+
+// Launch choices of one window class (st_autotune and its cache).
+struct TuneEntry {
+    int window = 0;
+    int precision = ST_PREC_FP32;  // what ST_PREC_AUTO runs
+    int wgPerCu = 0;               // fused kernel's persistent grid per CU (0: occupancy limit)
+    int stagedBatch = 0;           // staged path's batch cap (0: the scratch budget decides)
+    float usPerPatch = 0;          // measured complex-stage time per attempt
+};
+// Reads the device's tune cache once (st_ctx_create calls it).
+void tune_init(int device);
+// The entry of `window` on `device`; false when there is none.
+bool tune_find(int device, int window, TuneEntry *out);
+// Resident workgroups per CU of the fused kernel at window w (16, 32, 64).
+int fused_occupancy(int w, int stype);

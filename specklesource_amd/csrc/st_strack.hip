@@ -2369,6 +2369,9 @@ void launch_fused_kernel_o(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS,
     constexpr int NTH = 2 * FusedLanes<W>::L * W;
     ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused<W, STYPE, BASE, OCC>, NTH, lds));
     ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    TuneEntry te;
+    if (tune_find(ctx->device, W, &te) && te.wgPerCu > 0 && te.wgPerCu < perCU)
+        perCU = te.wgPerCu;
     const int grid = std::max(1, std::min(total, std::max(1, perCU) * ncu));
     float *mag = ctx->arena.take<float>((size_t)grid * 4 * W * W);
     ctx->time_begin(slot);
@@ -2684,7 +2687,12 @@ struct StrackJob {
         intOn = par.intf != nullptr && !par.noComplex;
         // -gauss fits its peak on the staged path's magnitude surfaces
         gauss = par.gaussFlag != 0;
-        mfmaPath = par.precision == ST_PREC_MFMA && (par.wR == 16 || par.wR == 32) && !par.baseFlag &&
+        int precision = par.precision;
+        if (precision == ST_PREC_AUTO) {
+            TuneEntry te;
+            precision = tune_find(ctx->device, par.wR, &te) ? te.precision : ST_PREC_FP32;
+        }
+        mfmaPath = precision == ST_PREC_MFMA && (par.wR == 16 || par.wR == 32) && !par.baseFlag &&
                        !intOn && !gauss && !staged_only();
         fusedPath = (par.wR == 16 || par.wR == 32 || par.wR == 64) && !gauss && !staged_only();
         kPeak = ctx->kernel_slot(mfmaPath ? "k_cpx_mfma" : (fusedPath ? "k_cpx_fused" : "peak_refine"));
@@ -2902,7 +2910,10 @@ struct StrackJob {
             ST_HIP(hipGetLastError());
             ar.used = mark;
         } else if (nCpx > 0 && !fusedPath) {
-            const int B = std::min(Bc, nCpx);
+            int B = std::min(Bc, nCpx);
+            TuneEntry te;
+            if (tune_find(ctx->device, w, &te) && te.stagedBatch > 0)
+                B = std::min(B, te.stagedBatch);
             CpxBatch C;
             C.P = ar.take<float2>((size_t)2 * B * w * w);
             C.S = ar.take<float2>((size_t)B * 4 * w * w);
@@ -3272,6 +3283,8 @@ void st_speckle_track_batch_impl(st_ctx *ctx, int n, StPair *const *prs, const s
         for (int k = 0; k < n; k++)
             patches[k] = jobs[k]->patches;
 }
+
+int fused_occupancy(int w, int stype) { return fused_per_cu_w(w, stype); }
 
 bool strack_batchable(const st_track_params &p)
 {

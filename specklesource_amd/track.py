@@ -65,6 +65,10 @@ def load_library(path=LIB_PATH):
     lib.st_poll.argtypes = [vp, C.c_int64, C.POINTER(abi.JobInfo)]
     lib.st_wait.argtypes = [vp, C.c_int64, C.POINTER(abi.JobInfo)]
     lib.st_cancel.argtypes = [vp, C.c_int64]
+    lib.st_autotune.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, C.c_char_p]
+    lib.st_tune_get.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                C.POINTER(C.c_float)]
+    lib.st_tune_load.argtypes = [vp, C.c_char_p]
     lib.st_server_set_batch.argtypes = [vp, C.c_int32]
     lib.st_job_release.argtypes = [vp, C.c_int64]
     lib.st_set_profiling.argtypes = [vp, C.c_int32]
@@ -218,6 +222,23 @@ class Device:
                                        C.byref(dg) if dg is not None else None)
         self._check(rc, "st_speckle_track")
         return finish(res, out)
+
+    def autotune(self, windows, cache_file=None):
+        """st_autotune over the given windows; writes cache_file when given."""
+        w = (C.c_int32 * len(windows))(*windows)
+        self._check(self.lib.st_autotune(self.ctx, w, len(windows), cache_file.encode() if cache_file else None),
+                    "st_autotune")
+
+    def tune(self, window):
+        """(precision, wg_per_cu, staged_batch, us_per_patch) of a window, or
+        None without an entry."""
+        p, g, b, u = C.c_int32(), C.c_int32(), C.c_int32(), C.c_float()
+        if self.lib.st_tune_get(self.ctx, window, C.byref(p), C.byref(g), C.byref(b), C.byref(u)) != abi.ST_OK:
+            return None
+        return p.value, g.value, b.value, u.value
+
+    def tune_load(self, cache_file):
+        self._check(self.lib.st_tune_load(self.ctx, cache_file.encode()), "st_tune_load")
 
     def speckle_track_batch(self, pairs, pars):
         """Several strack calls as one batch (st_speckle_track_batch); a
