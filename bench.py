@@ -587,8 +587,8 @@ def main():
                     help="run only a 1 GiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(sharding design in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(each GPU would track its own share of the grid rows); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()

@@ -88,7 +88,23 @@ static inline oreal absc(ocplx x)
 }
 
 /* ------------------------------------------------------------------ */
-/* computeHanning, speckleTrack.c:1830-1852 (square windows) */
+/* computeHanning, speckleTrack.c:1830-1852.  The reference stores the
+ * window as one contiguous wA x wR float block with rows of wR
+ * (mallocFloatMat(wA, wR), speckleTrack.c:1775) and indexes its columns
+ * with wA / 2 (:1843-1844).  For square windows that is the separable
+ * taper; otherwise hanning[i][j] lands at flat offset i * wR + j, which can
+ * fall in a neighbouring row or outside the block.  Restated on the flat
+ * block in the reference's loop order: writes inside [0, wA wR) land where
+ * the reference's land (a later write to the same cell wins), writes
+ * outside it are dropped (the reference corrupts its heap there) and cells
+ * never written hold 0 (the reference leaves them indeterminate). */
+static void hann_put(oreal *h, int wA, int wR, int i, int j, oreal v)
+{
+    const long f = (long)i * wR + j;
+    if (f >= 0 && f < (long)wA * wR)
+        h[f] = v;
+}
+
 static void compute_hanning(orc_st *s)
 {
     int wA = s->wA, wR = s->wR;
@@ -100,11 +116,12 @@ static void compute_hanning(orc_st *s)
         for (int j = 0; j < wR / 2; j++) {
             int j1 = wA / 2 + j, j2 = wA / 2 - j - 1;
             double wj = 0.5 * (1.0 + cos((2.0 * ORC_PI) * (double)j / (double)(wR - 1)));
-            oreal h = (oreal)(wi * wj);
-            s->hanning[i1 * wR + j1] = h;
-            s->hanning[i2 * wR + j2] = h;
-            s->hanning[i2 * wR + j1] = h;
-            s->hanning[i1 * wR + j2] = h;
+            /* stored as float (hanning is float**) */
+            oreal h = (oreal)(float)(wi * wj);
+            hann_put(s->hanning, wA, wR, i1, j1, h);
+            hann_put(s->hanning, wA, wR, i2, j2, h);
+            hann_put(s->hanning, wA, wR, i2, j1, h);
+            hann_put(s->hanning, wA, wR, i1, j2, h);
         }
     }
 }
@@ -1073,7 +1090,10 @@ static void amp_match(orc_st *s, int large, int iOut, int jOut, int *pI, int *pJ
 int orc_speckle_track(const st_track_params *par, const void *img1, int nr1, int na1, const void *img2, int nr2,
                       int na2, int stype, st_track_out *out, st_track_diag *dg)
 {
-    if (par->wR != par->wA || par->wRa != par->wAa || (par->wR & 1) || (par->wRa & 1))
+    /* windows of any even size, range and azimuth independent
+     * (parseTrack.c:167-171) */
+    if ((par->wR & 1) || (par->wA & 1) || (par->wRa & 1) || (par->wAa & 1) || par->wR < 8 || par->wA < 8 ||
+        par->wRa < 8 || par->wAa < 8)
         return ST_ERR_UNSUPPORTED;
     if (par->intf && !par->noComplex) {
         /* the corrected window must lie inside the oversampled patch (the
@@ -1100,6 +1120,10 @@ int orc_speckle_track(const st_track_params *par, const void *img1, int nr1, int
     s->wRa = par->wRa;
     s->wAa = par->wAa;
     int wmax = s->wA > s->wAa * LA ? s->wA : s->wAa * LA;
+    if (s->wR > wmax)
+        wmax = s->wR;
+    if (s->wRa * LA > wmax)
+        wmax = s->wRa * LA;
     size_t big = (size_t)(2 * wmax) * (2 * wmax);
     s->hanning = (oreal *)calloc((size_t)s->wA * s->wR, sizeof(oreal));
     s->patch1 = (ocplx *)calloc((size_t)s->wA * s->wR, sizeof(ocplx));
@@ -1293,17 +1317,20 @@ float orc_gauss_sigma(const float *y, float x0, int azOrRg)
     return gauss_peak_1d(mag, NGAUSS, NGAUSS / 2, NGAUSS / 2, azOrRg) + 0.0f * x0;
 }
 
-void orc_hanning(int w, float *out)
+void orc_hanning2(int wA, int wR, float *out)
 {
     orc_st s;
     memset(&s, 0, sizeof(s));
-    s.wA = s.wR = w;
-    s.hanning = (oreal *)calloc((size_t)w * w, sizeof(oreal));
+    s.wA = wA;
+    s.wR = wR;
+    s.hanning = (oreal *)calloc((size_t)wA * wR, sizeof(oreal));
     compute_hanning(&s);
-    for (int k = 0; k < w * w; k++)
+    for (int k = 0; k < wA * wR; k++)
         out[k] = (float)s.hanning[k];
     free(s.hanning);
 }
+
+void orc_hanning(int w, float *out) { orc_hanning2(w, w, out); }
 
 /* cmpTrackGauss alone, on a float magnitude surface of row stride W_R. */
 void orc_gauss_fit_2d(const float *magIn, int W_R, int W_A, int iMax, int jMax, double cMax, int wR, int wA,
@@ -1329,9 +1356,9 @@ void orc_int_correct(const st_track_params *par, int r1, int a1, ocplx *patch)
     phase_correct_int(&s, r1, a1);
 }
 
-/* cmpPSWithPad + the 2w x 2w inverse + findCPeak's fftshifted magnitude,
- * for two w x w patches already in the frequency domain's input form. */
-void orc_corr_surface(const ocplx *p1, const ocplx *p2, int w, int azShift, oreal *magOut)
+/* cmpPSWithPad + the 2wA x 2wR inverse + findCPeak's fftshifted magnitude,
+ * for two wA x wR patches (rows of wR) as getPatches leaves them. */
+void orc_corr_surface2(const ocplx *p1, const ocplx *p2, int wA, int wR, int azShift, oreal *magOut)
 {
     orc_st s;
     memset(&s, 0, sizeof(s));
@@ -1340,19 +1367,21 @@ void orc_corr_surface(const ocplx *p1, const ocplx *p2, int w, int azShift, orea
     par.nR = par.nA = 1;
     par.hanningFlag = 0;
     s.par = &par;
-    s.wA = s.wR = w;
+    s.wA = wA;
+    s.wR = wR;
     s.azShift = azShift;
     s.p1 = s.p2 = 1.0;
-    s.patch1 = (ocplx *)calloc((size_t)w * w, sizeof(ocplx));
-    s.patch2 = (ocplx *)calloc((size_t)w * w, sizeof(ocplx));
-    s.ps = (ocplx *)calloc((size_t)4 * w * w, sizeof(ocplx));
-    s.cbuf = (ocplx *)calloc((size_t)4 * w * w, sizeof(ocplx));
+    const size_t n = (size_t)wA * wR;
+    s.patch1 = (ocplx *)calloc(n, sizeof(ocplx));
+    s.patch2 = (ocplx *)calloc(n, sizeof(ocplx));
+    s.ps = (ocplx *)calloc(4 * n, sizeof(ocplx));
+    s.cbuf = (ocplx *)calloc(4 * n, sizeof(ocplx));
     s.mag = magOut;
     s.cFastOver = (ocplx *)calloc((size_t)NFAST * NOVER * NFAST * NOVER, sizeof(ocplx));
     float azd;
     s.azd = &azd;
-    memcpy(s.patch1, p1, sizeof(ocplx) * (size_t)w * w);
-    memcpy(s.patch2, p2, sizeof(ocplx) * (size_t)w * w);
+    memcpy(s.patch1, p1, sizeof(ocplx) * n);
+    memcpy(s.patch2, p2, sizeof(ocplx) * n);
     int pI, pJ, fI, fJ;
     double cMax;
     float r1, a1;
@@ -1362,6 +1391,11 @@ void orc_corr_surface(const ocplx *p1, const ocplx *p2, int w, int azShift, orea
     free(s.ps);
     free(s.cbuf);
     free(s.cFastOver);
+}
+
+void orc_corr_surface(const ocplx *p1, const ocplx *p2, int w, int azShift, oreal *magOut)
+{
+    orc_corr_surface2(p1, p2, w, w, azShift, magOut);
 }
 
 /* findCPeak alone on a W_A x W_R complex surface in the inverse

@@ -389,13 +389,13 @@ __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *lis
     __shared__ int shi[NT / 64];
     const int b = blockIdx.x;
     const int pt = list[off + b];
-    const int w = P.wA, N = w * w;
+    const int wa = P.wA, wr = P.wR, N = wa * wr;  // wA rows of wR samples
     const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
     const bool isf = P.im.stype == ST_SAMPLE_C64;
     const float thr = isf ? 1e-15f : 1e-9f;
     const float thrAll = thr * P.wA * P.wR;
     auto get = [&](int e, float2 &x, float2 &y) {
-        const int i = e / w, j = e - (e / w) * w;
+        const int i = e / wr, j = e - (e / wr) * wr;
         x = load_sample(P.im.img1, P.im.stype, P.im.nr1, a1 + i, r1 + j);
         y = load_sample(P.im.img2, P.im.stype, P.im.nr2, a2 + i, r2 + j);
         if (isf) {
@@ -412,7 +412,7 @@ __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *lis
     float2 *P1 = C.P + (size_t)b * N, *P2 = C.P + ((size_t)B + b) * N;
     __shared__ IntTab<128> itab;
     if (P.intOn)
-        int_tables<128>(P, off + b, w, itab, threadIdx.x, NT);
+        int_tables<128>(P, off + b, imax(wa, wr), itab, threadIdx.x, NT);
     int nZero = 0;
     double p1 = 0, p2 = 0;
     for (int e = threadIdx.x; e < N; e += NT) {
@@ -425,7 +425,7 @@ __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *lis
         p2 += pw2;
         P2[e] = y;
         if (P.baseFlag) {
-            const int i = e / w, j = e - (e / w) * w;
+            const int i = e / wr, j = e - (e / wr) * wr;
             double xx = (a1 - P.wA / 2 - (P.nSlpA1 / 2)) / (double)P.nSlpA1;
             xx += i * (1. / (double)P.nSlpA1);
             double x0 = (a1 - P.wA / 2 - (P.nSlpA1 / 2)) / (double)P.nSlpA1;
@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *lis
             x = make_float2((float)(cr * pr - ci * pim), (float)(cr * pim + ci * pr));
         }
         if (P.intOn) {
-            const int i = e / w, j = e - (e / w) * w;
+            const int i = e / wr, j = e - (e / wr) * wr;
             double cr = x.x, ci = x.y;
             int_apply<128>(P, itab, off + b, i, j, cr, ci);
             x = make_float2((float)cr, (float)ci);
@@ -455,7 +455,7 @@ __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *lis
             C.have[b] = 0;
         return;
     }
-    const int az = doppler_shift(get, w, w, w, P.wA, sh);
+    const int az = doppler_shift(get, wa, wr, wa, P.wA, sh);
     if (threadIdx.x == 0) {
         C.have[b] = 1;
         C.az[b] = az;
@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *lis
 // the per-item powers p (p1, p2) and their factor pScale, the Hanning
 // correction hc (cmpTrackGauss's scale, speckleTrack.c:2249-2254).
 struct GaussArg {
-    int on, wWin;
+    int on, wWinA, wWinR;
     double hc, pScale;
     const double *p;
 };
@@ -482,9 +482,9 @@ struct GaussArg {
 // The part of the peak stage after the coarse scan: border check, FP64
 // refinement of the 8 x 8 neighbourhood on the searched 48 x 48 box (or,
 // in -gauss mode, the 2-D Gaussian fit), and the samples of the two 1-D
-// Gaussian fits, from the fftshifted magnitude surface M.
+// Gaussian fits, from the fftshifted WA x WR magnitude surface M.
 template <bool GAUSS>
-__device__ void peak_finish(const float *M, int W, int mode, int I, int J, float raw, float sec, float snr,
+__device__ void peak_finish(const float *M, int WA, int WR, int mode, int I, int J, float raw, float sec, float snr,
                             PeakRes *out, const GaussArg &G, int item)
 {
     __shared__ ArgMaxD shd[NT / 64];
@@ -500,7 +500,7 @@ __device__ void peak_finish(const float *M, int W, int mode, int I, int J, float
     r.fineGap = 0;
     r.gI = r.gJ = -2000000000;
     r.gC = 0;
-    if (I < NFAST / 2 || I > W - NFAST / 2 || J < NFAST / 2 || J > W - NFAST / 2) {
+    if (I < NFAST / 2 || I > WA - NFAST / 2 || J < NFAST / 2 || J > WR - NFAST / 2) {
         r.I = r.J = -1000;
         r.raw = -1.f;
         if (threadIdx.x == 0)
@@ -514,11 +514,11 @@ __device__ void peak_finish(const float *M, int W, int mode, int I, int J, float
         // the fit itself runs in k_gauss_fit, one thread per attempt; the
         // amplitude mode's 1-D fits sit at the coarse peak and are known now
         if (threadIdx.x == 0) {
-            if (mode == 1 && I > NGAUSS / 2 && I < W - NGAUSS / 2 && J > NGAUSS / 2 && J < W - NGAUSS / 2) {
+            if (mode == 1 && I > NGAUSS / 2 && I < WA - NGAUSS / 2 && J > NGAUSS / 2 && J < WR - NGAUSS / 2) {
                 r.lmOk = 1;
                 for (int t = 0; t < NGAUSS; t++) {
-                    r.lmY[0][t] = M[I * W + (J - NGAUSS / 2 + t)];
-                    r.lmY[1][t] = M[(I - NGAUSS / 2 + t) * W + J];
+                    r.lmY[0][t] = M[I * WR + (J - NGAUSS / 2 + t)];
+                    r.lmY[1][t] = M[(I - NGAUSS / 2 + t) * WR + J];
                 }
                 r.lmX0[0] = (float)(J - NGAUSS / 2);
                 r.lmX0[1] = (float)(I - NGAUSS / 2);
@@ -531,7 +531,7 @@ __device__ void peak_finish(const float *M, int W, int mode, int I, int J, float
     }
     // FP64 refinement: Q = A P A^T on the searched 48 x 48 box
     if (threadIdx.x < NFAST * NFAST)
-        Pm[threadIdx.x] = (double)M[(I - NFAST / 2 + threadIdx.x / NFAST) * W + (J - NFAST / 2 + threadIdx.x % NFAST)];
+        Pm[threadIdx.x] = (double)M[(I - NFAST / 2 + threadIdx.x / NFAST) * WR + (J - NFAST / 2 + threadIdx.x % NFAST)];
     __syncthreads();
     for (int t = threadIdx.x; t < NFAST * kFine; t += NT) {
         const int mm = t / kFine, v = t - mm * kFine;
@@ -588,17 +588,17 @@ __device__ void peak_finish(const float *M, int W, int mode, int I, int J, float
         const int jF = J * NOVER + (r.fj + 24) - (NOVER * NFAST) / 2;
         cI = iF / NOVER;
         cJ = jF / NOVER;
-        ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < W && (cJ - NGAUSS / 2) >= 0 && (cJ + NGAUSS / 2) < W;
+        ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < WA && (cJ - NGAUSS / 2) >= 0 && (cJ + NGAUSS / 2) < WR;
     } else {
         cI = I;
         cJ = J;
-        ok = I > NGAUSS / 2 && I < W - NGAUSS / 2 && J > NGAUSS / 2 && J < W - NGAUSS / 2;
+        ok = I > NGAUSS / 2 && I < WA - NGAUSS / 2 && J > NGAUSS / 2 && J < WR - NGAUSS / 2;
     }
     r.lmOk = ok ? 1 : 0;
     if (ok) {
         for (int t = 0; t < NGAUSS; t++) {
-            r.lmY[0][t] = M[cI * W + (cJ - NGAUSS / 2 + t)];
-            r.lmY[1][t] = M[(cI - NGAUSS / 2 + t) * W + cJ];
+            r.lmY[0][t] = M[cI * WR + (cJ - NGAUSS / 2 + t)];
+            r.lmY[1][t] = M[(cI - NGAUSS / 2 + t) * WR + cJ];
         }
         r.lmX0[0] = (float)(cJ - NGAUSS / 2);
         r.lmX0[1] = (float)(cI - NGAUSS / 2);
@@ -612,7 +612,7 @@ __device__ void peak_finish(const float *M, int W, int mode, int I, int J, float
 // speckleTrack.c:2205-2303), so a wave runs 64 fits at once in the
 // reference's sequential operation order.  In the complex mode the 1-D fit
 // samples follow at the fitted peak / NOVER (speckleTrack.c:377-386).
-__global__ void __launch_bounds__(128) k_gauss_fit(const float *mag, int W, int mode, PeakRes *res, int B,
+__global__ void __launch_bounds__(128) k_gauss_fit(const float *mag, int WA, int WR, int mode, PeakRes *res, int B,
                                                    GaussArg G)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -621,25 +621,25 @@ __global__ void __launch_bounds__(128) k_gauss_fit(const float *mag, int W, int 
     PeakRes &r = res[b];
     if (r.I < -999)
         return;  // rejected coarse peak: the fit's edge test fails (defaults stand)
-    const float *M = mag + (size_t)b * W * W;
+    const float *M = mag + (size_t)b * WA * WR;
     const double p1 = G.p[2 * b] * G.pScale, p2 = G.p[2 * b + 1] * G.pScale;
-    const double scale = G.hc / (sqrt(p1 * p2) * pow((double)(G.wWin * G.wWin), 2));
+    const double scale = G.hc / (sqrt(p1 * p2) * pow((double)(G.wWinR * G.wWinA), 2));
     int gI, gJ;
     double gC;
-    gauss_fit_2d(M, W, r.I, r.J, (double)r.raw, G.wWin, G.wWin, scale, &gI, &gJ, &gC);
+    gauss_fit_2d(M, WR, r.I, r.J, (double)r.raw, G.wWinR, G.wWinA, scale, &gI, &gJ, &gC);
     r.gI = gI;
     r.gJ = gJ;
     r.gC = gC;
     if (mode != 0)
         return;
     const int cI = gI / NOVER, cJ = gJ / NOVER;
-    const bool ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < W && (cJ - NGAUSS / 2) >= 0 &&
-                    (cJ + NGAUSS / 2) < W;
+    const bool ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < WA && (cJ - NGAUSS / 2) >= 0 &&
+                    (cJ + NGAUSS / 2) < WR;
     r.lmOk = ok ? 1 : 0;
     if (ok) {
         for (int t = 0; t < NGAUSS; t++) {
-            r.lmY[0][t] = M[cI * W + (cJ - NGAUSS / 2 + t)];
-            r.lmY[1][t] = M[(cI - NGAUSS / 2 + t) * W + cJ];
+            r.lmY[0][t] = M[cI * WR + (cJ - NGAUSS / 2 + t)];
+            r.lmY[1][t] = M[(cI - NGAUSS / 2 + t) * WR + cJ];
         }
         r.lmX0[0] = (float)(cJ - NGAUSS / 2);
         r.lmX0[1] = (float)(cI - NGAUSS / 2);
@@ -820,11 +820,13 @@ __global__ void k_amp_list(KPar P, Pts S, int npts, int *list)
 // :398-484, ampTrackFast :489-597)
 struct AmpPart;
 struct Top2;
+// ws = wsA x wsR: the patch (wAa x wRa, doubled on the large try);
+// W = WA x WR = 2 wsA x 2 wsR: the interpolated surfaces.
 struct AmpBatch {
-    float2 *I;       // [2][B][ws^2] patches, then their range spectra
-    float2 *F;       // [2][B][W^2]: interpolation columns [2][B][W][ws], then range spectra Z [B][W^2]
-    float2 *S;       // [B][W^2]: amplitudes [2][B][W^2] (float), then the correlation's azimuth half
-    float *mag;      // [B][W^2]
+    float2 *I;       // [2][B][ws] patches, then their range spectra
+    float2 *F;       // [2][B][W]: interpolation columns [2][B][WA][wsR], then range spectra Z [B][W]
+    float2 *S;       // [B][W]: amplitudes [2][B][W] (float), then the correlation's azimuth half
+    float *mag;      // [B][W]
     PeakRes *res;
     float *sig;
     uint8_t *state;  // 0 = bounds failed, 1 = no data, 2 = matched
@@ -843,7 +845,7 @@ __global__ void __launch_bounds__(NT) k_amp_gather(KPar P, Pts S, const int *lis
     const int b = blockIdx.x;
     const int pt = list[off + b];
     const int sc = large ? LA : 1;
-    const int ws = P.wAa * sc;
+    const int wsA = P.wAa * sc, wsR = P.wRa * sc;
     const int i = pt / P.nR, j = pt - (pt / P.nR) * P.nR;
     const int a1 = P.aStart + i * P.deltaA - P.wA / 2 * sc;
     const int r1 = P.rStart + j * P.deltaR - P.wR / 2 * sc;
@@ -855,18 +857,19 @@ __global__ void __launch_bounds__(NT) k_amp_gather(KPar P, Pts S, const int *lis
             A.state[b] = 0;
         return;
     }
-    const int N = ws * ws;
+    const int N = wsA * wsR;
     float2 *I1 = A.I + (size_t)b * N, *I2 = A.I + ((size_t)B + b) * N;
     for (int e = threadIdx.x; e < N; e += NT) {
-        const int ii = e / ws, jj = e - (e / ws) * ws;
+        const int ii = e / wsR, jj = e - (e / wsR) * wsR;
         I1[e] = load_sample(P.im.img1, P.im.stype, P.im.nr1, a1 + ii, r1 + jj);
         I2[e] = load_sample(P.im.img2, P.im.stype, P.im.nr2, a2 + ii, r2 + jj);
     }
     int azUse = S.azUse[pt];
     if (P.noComplex && !large) {
-        const int wsA = P.wAa;
+        // the small try's patch: wAa rows of wRa samples
+        const int rl = P.wRa;
         auto get = [&](int e, float2 &x, float2 &y) {
-            const int ii = e / wsA, jj = e - (e / wsA) * wsA;
+            const int ii = e / rl, jj = e - (e / rl) * rl;
             x = load_sample(P.im.img1, P.im.stype, P.im.nr1, a1 + ii, r1 + jj);
             y = load_sample(P.im.img2, P.im.stype, P.im.nr2, a2 + ii, r2 + jj);
         };
@@ -876,7 +879,7 @@ __global__ void __launch_bounds__(NT) k_amp_gather(KPar P, Pts S, const int *lis
     }
     if (threadIdx.x == 0) {
         A.state[b] = 2;
-        A.az[b] = (int)((float)azUse * (float)ws / (float)P.wA);
+        A.az[b] = (int)((float)azUse * (float)wsA / (float)P.wA);
     }
 }
 
@@ -902,7 +905,7 @@ __global__ void k_amp_finalize(KPar P, Pts S, const int *list, int off, int B, i
         return;
     }
     const int sc = large ? LA : 1;
-    const int W = 2 * P.wAa * sc;
+    const int WA = 2 * P.wAa * sc, WR = 2 * P.wRa * sc;
     const PeakRes &r = A.res[b];
     float sRg = A.sig[2 * b], sAz = A.sig[2 * b + 1];
     if (r.lmOk)
@@ -921,7 +924,7 @@ __global__ void k_amp_finalize(KPar P, Pts S, const int *list, int off, int B, i
         jMax = r.gJ;
     } else {
         const double p1 = A.p[2 * b], p2 = A.p[2 * b + 1];
-        cMax = r.fineMax / (pow((double)W, 2) * pow((double)W, 2) * pow((double)NFAST, 2));
+        cMax = r.fineMax / (pow((double)WA, 2) * pow((double)WR, 2) * pow((double)NFAST, 2));
         cMax = cMax / sqrt(p1 * p2);
         iMax = r.I * NOVER + (r.fi + 24) - (NOVER * NFAST) / 2;
         jMax = r.J * NOVER + (r.fj + 24) - (NOVER * NFAST) / 2;
@@ -948,7 +951,6 @@ __global__ void k_amp_finalize(KPar P, Pts S, const int *list, int off, int B, i
     S.peakGap[o] = r.gap;
     S.fineGap[o] = r.fineGap;
     S.snr[o] = r.snr;
-    (void)W;
 }
 
 // Final offsets (speckleTrack.c:301-323).
@@ -1177,35 +1179,36 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
 // half (pad, inverse, |C|) on rows.
 constexpr int kStElems = 4096;  // elements per workgroup (LDS and register staging)
 
-// Azimuth half: C adjacent spectrum columns la of one patch per workgroup;
-// out T[b][n][la], n < 2w.  Workgroups are numbered so that the column
-// blocks sharing cache lines run on the same XCD.
-__global__ void __launch_bounds__(NT) k_cpx_colx(const float2 *P, int B, int w, const int *az, float2 *T, int lc,
-                                                  stfft::PlanArgs Pf, stfft::PlanArgs Pi, stfft::FastDiv dW)
+// Azimuth half: C adjacent spectrum columns la (of wr) of one patch per
+// workgroup, each wa long; out T[b][n][la], n < 2wa.  Workgroups are
+// numbered so that the column blocks sharing cache lines run on the same XCD.
+__global__ void __launch_bounds__(NT) k_cpx_colx(const float2 *P, int B, int wa, int wr, const int *az, float2 *T,
+                                                  int lc, stfft::PlanArgs Pf, stfft::PlanArgs Pi, stfft::FastDiv dW)
 {
     extern __shared__ float2 ldsx[];
     constexpr int IT = kStElems / NT;
-    const int C = 1 << lc, W = 2 * w, h = w / 2;
+    const int C = 1 << lc, W = 2 * wa, h = wa / 2;
     const int nbx = gridDim.x, total = nbx * gridDim.y, flat = blockIdx.y * nbx + blockIdx.x;
     const int per = total / 8;
     const int Lb = flat < per * 8 ? (flat % 8) * per + flat / 8 : flat;
     const int b = Lb / nbx, c0 = (Lb - (Lb / nbx) * nbx) * C;
-    const int nc = imin(C, w - c0);
-    const int px = W + 1, py = w + 1;
+    const int nc = imin(C, wr - c0);
+    const int px = W + 1, py = wa + 1;
     // F1 [C][py] and F2 [C][py] back to back, so one transform takes both;
     // the padded columns [C][px] then overwrite them from the start of the
     // buffer (every read of the pad pass precedes its first write)
     float2 *X = ldsx, *Y = ldsx + (size_t)C * py;
-    const float2 *P1 = P + (size_t)b * w * w + c0, *P2 = P + ((size_t)B + b) * w * w + c0;
+    const size_t N = (size_t)wa * wr;
+    const float2 *P1 = P + (size_t)b * N + c0, *P2 = P + ((size_t)B + b) * N + c0;
     {
-        const int cnt = w << lc;
+        const int cnt = wa << lc;
         float2 u[IT], v[IT];
 #pragma unroll
         for (int it = 0; it < IT; it++) {
             const int t = threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1);
             const bool ok = t < cnt && c < nc;
-            u[it] = ok ? P1[(size_t)a * w + c] : make_float2(0.f, 0.f);
-            v[it] = ok ? P2[(size_t)a * w + c] : make_float2(0.f, 0.f);
+            u[it] = ok ? P1[(size_t)a * wr + c] : make_float2(0.f, 0.f);
+            v[it] = ok ? P2[(size_t)a * wr + c] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int it = 0; it < IT; it++) {
@@ -1229,9 +1232,9 @@ __global__ void __launch_bounds__(NT) k_cpx_colx(const float2 *P, int B, int w, 
                 const int c = stfft::fdiv(t, dW), n = t - c * W;
                 const int k = n < h ? n : (n >= W - h ? n - W : (1 << 20));
                 if (k != (1 << 20)) {
-                    int ka = k + w + azb;  // in [0, 3w)
-                    ka -= ka >= w ? w : 0;
-                    ka -= ka >= w ? w : 0;
+                    int ka = k + wa + azb;  // in [0, 3wa)
+                    ka -= ka >= wa ? wa : 0;
+                    ka -= ka >= wa ? wa : 0;
                     v[it] = cmulconj(X[c * py + ka], Y[c * py + ka]);
                 }
             }
@@ -1248,30 +1251,31 @@ __global__ void __launch_bounds__(NT) k_cpx_colx(const float2 *P, int B, int w, 
         __syncthreads();
     }
     stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
-    float2 *Tb = T + (size_t)b * W * w + c0;
+    float2 *Tb = T + (size_t)b * W * wr + c0;
     const int cnt = W << lc;
 #pragma unroll
     for (int it = 0; it < IT; it++) {
         const int t = threadIdx.x + it * NT, n = t >> lc, c = t & (C - 1);
         if (t < cnt && c < nc)
-            Tb[(size_t)n * w + c] = X[c * px + n];
+            Tb[(size_t)n * wr + c] = X[c * px + n];
     }
 }
 
-// Range half: R rows of one patch per workgroup: the w range frequencies
-// of T spread over 2w (Nyquist on the negative side, zeros between),
-// inverse transform, |C| into the fftshifted surface M, and this block's
-// top two in findCPeak's scan-rank order (speckleTrack.c:744-805).
-__global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R, float *mag, Top2 *part,
+// Range half: R rows (of 2wa) of one patch per workgroup: the wr range
+// frequencies of T spread over 2wr (Nyquist on the negative side, zeros
+// between), inverse transform, |C| into the fftshifted 2wa x 2wr surface M,
+// and this block's top two in findCPeak's scan-rank order
+// (speckleTrack.c:744-805).
+__global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int wa, int wr, int R, float *mag, Top2 *part,
                                                     stfft::PlanArgs Pi, stfft::FastDiv dW)
 {
     extern __shared__ float2 ldsx[];
     __shared__ Top2 sht[NT / 64];
     constexpr int IT = kStElems / NT;
-    const int W = 2 * w, h = w / 2, H = w;
-    const int b = blockIdx.y, n0 = blockIdx.x * R, nrow = imin(R, W - n0);
-    const float2 *Tb = T + ((size_t)b * W + n0) * w;
-    const int cnt = nrow * W;
+    const int WA = 2 * wa, WR = 2 * wr, h = wr / 2, HA = wa, HR = wr;
+    const int b = blockIdx.y, n0 = blockIdx.x * R, nrow = imin(R, WA - n0);
+    const float2 *Tb = T + ((size_t)b * WA + n0) * wr;
+    const int cnt = nrow * WR;
     {
         float2 v[IT];
 #pragma unroll
@@ -1279,10 +1283,10 @@ __global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R
             const int t = threadIdx.x + it * NT;
             v[it] = make_float2(0.f, 0.f);
             if (t < cnt) {
-                const int i = stfft::fdiv(t, dW), m = t - i * W;
-                const int la = m < h ? m : (m >= W - h ? m - w : -1);
+                const int i = stfft::fdiv(t, dW), m = t - i * WR;
+                const int la = m < h ? m : (m >= WR - h ? m - wr : -1);
                 if (la >= 0)
-                    v[it] = Tb[(size_t)i * w + la];
+                    v[it] = Tb[(size_t)i * wr + la];
             }
         }
 #pragma unroll
@@ -1293,20 +1297,20 @@ __global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R
         }
     }
     __syncthreads();
-    stfft::lds_fft<kStElems, NT>(ldsx, W, nrow, Pi, +1);
-    float *M = mag + (size_t)b * W * W;
+    stfft::lds_fft<kStElems, NT>(ldsx, WR, nrow, Pi, +1);
+    float *M = mag + (size_t)b * WA * WR;
     Top2 best{0.0f, 0x7fffffff, 0.0f, 0.0f};
     for (int t = threadIdx.x; t < cnt; t += NT) {
 #pragma clang fp contract(off)
-        const int i = stfft::fdiv(t, dW), m = t - i * W, n = n0 + i;
+        const int i = stfft::fdiv(t, dW), m = t - i * WR, n = n0 + i;
         const float2 z = ldsx[t];
         const float x2 = z.x * z.x + z.y * z.y;
         const float v = sqrtf(x2);
-        const int I = n < H ? n + H : n - H, J = m < H ? m + H : m - H;
-        M[(size_t)I * W + J] = v;
-        const int i0 = I < H ? I : I - H, j0 = J < H ? J : J - H;
-        const int q = (I < H) ? (J < H ? 0 : 3) : (J < H ? 2 : 1);
-        const int rank = (i0 * H + j0) * 4 + q;
+        const int I = n < HA ? n + HA : n - HA, J = m < HR ? m + HR : m - HR;
+        M[(size_t)I * WR + J] = v;
+        const int i0 = I < HA ? I : I - HA, j0 = J < HR ? J : J - HR;
+        const int q = (I < HA) ? (J < HR ? 0 : 3) : (J < HR ? 2 : 1);
+        const int rank = (i0 * HR + j0) * 4 + q;
         if (v > 0.0f)
             best = top2_combine(best, Top2{v, rank, 0.0f, v});
         else
@@ -1320,10 +1324,10 @@ __global__ void __launch_bounds__(NT) k_cpx_rowmag(const float2 *T, int w, int R
 // Top two over the row blocks of each patch (in block order), then the
 // refinement and the Gaussian-fit samples (peak_finish).
 template <bool GAUSS>
-__global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, const float *mag, int W, int mode,
-                                                    PeakRes *res, GaussArg G)
+__global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, const float *mag, int WA, int WR,
+                                                    int mode, PeakRes *res, GaussArg G)
 {
-    const int b = blockIdx.x, H = W / 2;
+    const int b = blockIdx.x, HA = WA / 2, HR = WR / 2;
     Top2 best = part[(size_t)b * nPart];
     for (int k = 1; k < nPart; k++)
         best = top2_combine(best, part[(size_t)b * nPart + k]);
@@ -1331,13 +1335,13 @@ __global__ void __launch_bounds__(NT) k_peak_merge(const Top2 *part, int nPart, 
     float raw = 0.f;
     if (best.r1 != 0x7fffffff) {
         const int q = best.r1 & 3, ij = best.r1 >> 2;
-        const int i = ij / H, j = ij - (ij / H) * H;
-        I = (q == 0 || q == 3) ? i : i + H;
-        J = (q == 0 || q == 2) ? j : j + H;
+        const int i = ij / HR, j = ij - (ij / HR) * HR;
+        I = (q == 0 || q == 3) ? i : i + HA;
+        J = (q == 0 || q == 2) ? j : j + HR;
         raw = best.v1;
     }
-    const float snr = best.sum > 0.0f ? best.v1 * (float)(W * W) / best.sum : 0.0f;
-    peak_finish<GAUSS>(mag + (size_t)b * W * W, W, mode, I, J, raw, best.v2, snr, res + b, G, b);
+    const float snr = best.sum > 0.0f ? best.v1 * (float)(WA * WR) / best.sum : 0.0f;
+    peak_finish<GAUSS>(mag + (size_t)b * WA * WR, WA, WR, mode, I, J, raw, best.v2, snr, res + b, G, b);
 }
 
 // ---------------------------------------------------------------------
@@ -1378,30 +1382,30 @@ __device__ inline void xcd_item_block(int nbx, int &item, int &blk)
 }
 
 // zeroPadFFT's azimuth half (speckleTrack.c:2014-2038): C adjacent range
-// frequencies la of one image's row spectra per workgroup; item z = image
-// * B + patch.
-__global__ void __launch_bounds__(NT) k_amp_colx(const float2 *I, int B, int ws, const int *az, const uint8_t *state,
-                                                  float2 *T, int lc, stfft::PlanArgs Pf, stfft::PlanArgs Pi,
-                                                  stfft::FastDiv dW)
+// frequencies la (of wsR) of one image's row spectra per workgroup, each
+// column wsA long, padded to WA = 2 wsA; item z = image * B + patch.
+__global__ void __launch_bounds__(NT) k_amp_colx(const float2 *I, int B, int wsA, int wsR, const int *az,
+                                                  const uint8_t *state, float2 *T, int lc, stfft::PlanArgs Pf,
+                                                  stfft::PlanArgs Pi, stfft::FastDiv dW)
 {
     extern __shared__ float2 ldsx[];
     constexpr int IT = kStElems / NT;
-    const int C = 1 << lc, W = 2 * ws, h = ws / 2;
+    const int C = 1 << lc, W = 2 * wsA, h = wsA / 2;
     int z, cb;
     xcd_item_block(gridDim.x, z, cb);
     const int b = z % B, c0 = cb * C;
     if (state[b] != 2)
         return;
-    const int nc = imin(C, ws - c0), px = C > 1 ? W + 1 : W;
+    const int nc = imin(C, wsR - c0), px = C > 1 ? W + 1 : W;
     float2 *X = ldsx;  // [C][px]
-    const float2 *Ib = I + (size_t)z * ws * ws + c0;
+    const float2 *Ib = I + (size_t)z * wsA * wsR + c0;
     {
-        const int cnt = ws << lc;
+        const int cnt = wsA << lc;
         float2 u[IT];
 #pragma unroll
         for (int it = 0; it < IT; it++) {
             const int t = threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1);
-            u[it] = t < cnt && c < nc ? Ib[(size_t)a * ws + c] : make_float2(0.f, 0.f);
+            u[it] = t < cnt && c < nc ? Ib[(size_t)a * wsR + c] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int it = 0; it < IT; it++) {
@@ -1423,7 +1427,7 @@ __global__ void __launch_bounds__(NT) k_amp_colx(const float2 *I, int B, int ws,
                 const int c = stfft::fdiv(t, dW), n = t - c * W;
                 const int k = n < h ? n : (n >= W - h ? n - W : (1 << 20));
                 if (k != (1 << 20))
-                    v[it] = X[c * px + (k + ws + azb) % ws];
+                    v[it] = X[c * px + (k + wsA + azb) % wsA];
             }
         }
         __syncthreads();
@@ -1438,21 +1442,21 @@ __global__ void __launch_bounds__(NT) k_amp_colx(const float2 *I, int B, int ws,
         __syncthreads();
     }
     stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
-    float2 *Tb = T + (size_t)z * W * ws + c0;
+    float2 *Tb = T + (size_t)z * W * wsR + c0;
     const int cnt = W << lc;
 #pragma unroll
     for (int it = 0; it < IT; it++) {
         const int t = threadIdx.x + it * NT, n = t >> lc, c = t & (C - 1);
         if (t < cnt && c < nc)
-            Tb[(size_t)n * ws + c] = X[c * px + n];
+            Tb[(size_t)n * wsR + c] = X[c * px + n];
     }
 }
 
 // zeroPadFFT's range half, then detection and amplitude
-// (speckleTrack.c:1166-1188): R rows of both images of one patch per
-// workgroup (the zero test counts cells where either image is below the
-// threshold).
-__global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int B, int ws, int R,
+// (speckleTrack.c:1166-1188): R rows (of WA = 2 wsA) of both images of one
+// patch per workgroup, each row's wsR frequencies padded to WR = 2 wsR (the
+// zero test counts cells where either image is below the threshold).
+__global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int B, int wsA, int wsR, int R,
                                                     const uint8_t *state, float *amp, AmpPart *part,
                                                     stfft::PlanArgs Pi, stfft::FastDiv dW)
 {
@@ -1460,16 +1464,16 @@ __global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int 
     __shared__ double sh[4 * (NT / 64)];
     __shared__ int shi[NT / 64];
     constexpr int IT = kStElems / NT;
-    const int W = 2 * ws, h = ws / 2;
+    const int WA = 2 * wsA, W = 2 * wsR, h = wsR / 2;
     int b, blk;
     xcd_item_block(gridDim.x, b, blk);
     if (state[b] != 2)
         return;
-    const int n0 = blk * R, nrow = imin(R, W - n0), cnt = nrow * W;
+    const int n0 = blk * R, nrow = imin(R, WA - n0), cnt = nrow * W;
     // the second image's rows follow the first's, so one transform can take
     // both (2 nrow sequences) where they fit the register staging
     float2 *X1 = ldsx, *X2 = ldsx + (size_t)nrow * W;
-    const float2 *T1 = T + ((size_t)b * W + n0) * ws, *T2 = T + (((size_t)B + b) * W + n0) * ws;
+    const float2 *T1 = T + ((size_t)b * WA + n0) * wsR, *T2 = T + (((size_t)B + b) * WA + n0) * wsR;
     for (int t0 = 0; t0 < cnt; t0 += kStElems) {
         float2 u[IT], v[IT];
 #pragma unroll
@@ -1478,10 +1482,10 @@ __global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int 
             u[it] = v[it] = make_float2(0.f, 0.f);
             if (t < cnt) {
                 const int i = stfft::fdiv(t, dW), m = t - i * W;
-                const int la = m < h ? m : (m >= W - h ? m - ws : -1);
+                const int la = m < h ? m : (m >= W - h ? m - wsR : -1);
                 if (la >= 0) {
-                    u[it] = T1[(size_t)i * ws + la];
-                    v[it] = T2[(size_t)i * ws + la];
+                    u[it] = T1[(size_t)i * wsR + la];
+                    v[it] = T2[(size_t)i * wsR + la];
                 }
             }
         }
@@ -1501,10 +1505,10 @@ __global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int 
         stfft::lds_fft<kStElems, NT>(X1, W, nrow, Pi, +1);
         stfft::lds_fft<kStElems, NT>(X2, W, nrow, Pi, +1);
     }
-    const size_t sws = (size_t)ws;
-    const float scale = (float)(1. / (double)(sws * sws * sws * sws * 8));
+    const size_t swR = (size_t)wsR, swA = (size_t)wsA;
+    const float scale = (float)(1. / (double)(swR * swA * swR * swA * 8));
     const float thr = P.im.stype == ST_SAMPLE_C64 ? 1.0e-15f : 1.0e-6f;
-    float *A1 = amp + ((size_t)b * W + n0) * W, *A2 = amp + (((size_t)B + b) * W + n0) * W;
+    float *A1 = amp + ((size_t)b * WA + n0) * W, *A2 = amp + (((size_t)B + b) * WA + n0) * W;
     int nz = 0;
     double s1 = 0, s2 = 0, m1 = 0, m2 = 0;
     for (int t = threadIdx.x; t < cnt; t += NT) {
@@ -1557,7 +1561,8 @@ __global__ void __launch_bounds__(NT) k_amp_rowdet(KPar P, const float2 *T, int 
 // The zero test and the means of ampMatch (speckleTrack.c:437-467) from
 // the detection partials: one wave per patch, lanes over the partials, then
 // a fixed-order wave sum; a rejected patch is marked "no data" (state 1).
-__global__ void __launch_bounds__(64) k_amp_stat(const AmpPart *dpart, int nPd, int W, uint8_t *state, double2 *mean)
+__global__ void __launch_bounds__(64) k_amp_stat(const AmpPart *dpart, int nPd, int WA, int WR, uint8_t *state,
+                                                 double2 *mean)
 {
     const int b = blockIdx.x, l = threadIdx.x;
     if (state[b] != 2)
@@ -1583,17 +1588,17 @@ __global__ void __launch_bounds__(64) k_amp_stat(const AmpPart *dpart, int nPd, 
         if (nz > 20 || s1 < 1.0e-6 || s2 < 1.0e-6)
             state[b] = 1;
         else
-            mean[b] = make_double2(m1 / (double)((size_t)W * W), m2 / (double)((size_t)W * W));
+            mean[b] = make_double2(m1 / (double)((size_t)WA * WR), m2 / (double)((size_t)WA * WR));
     }
 }
 
 // Mean removal (ampMatch's rules) and the forward range transforms of the
 // correlation.  Both images are real, so one complex transform of
 // v1 + i v2 carries both spectra (k_amp_colx2 separates them); out
-// Z [B][W][W] and this block's partial powers.
-__global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int W, int R, const double2 *mean,
-                                                    const uint8_t *state, float2 *Z, double2 *ppart,
-                                                    stfft::PlanArgs Pf)
+// Z [B][WA][W] and this block's partial powers.
+__global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int WA, int W, int R,
+                                                    const double2 *mean, const uint8_t *state, float2 *Z,
+                                                    double2 *ppart, stfft::PlanArgs Pf)
 {
     extern __shared__ float2 ldsx[];
     __shared__ double sh[2 * (NT / 64)];
@@ -1602,9 +1607,9 @@ __global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int 
     if (state[b] != 2)
         return;
     const double m1 = mean[b].x, m2 = mean[b].y;
-    const int n0 = blk * R, nrow = imin(R, W - n0), cnt = nrow * W;
+    const int n0 = blk * R, nrow = imin(R, WA - n0), cnt = nrow * W;
     float2 *X = ldsx;
-    const float *A1 = amp + ((size_t)b * W + n0) * W, *A2 = amp + (((size_t)B + b) * W + n0) * W;
+    const float *A1 = amp + ((size_t)b * WA + n0) * W, *A2 = amp + (((size_t)B + b) * WA + n0) * W;
     constexpr int IT = kStElems / NT;
     double p1 = 0, p2 = 0;
     for (int t0 = 0; t0 < cnt; t0 += kStElems) {
@@ -1651,7 +1656,7 @@ __global__ void __launch_bounds__(NT) k_amp_rowfwd(const float *amp, int B, int 
         ppart[(size_t)b * gridDim.x + blk] = make_double2(q1, q2);
     }
     stfft::lds_fft<kStElems, NT>(X, W, nrow, Pf, -1);
-    float2 *Zb = Z + ((size_t)b * W + n0) * W;
+    float2 *Zb = Z + ((size_t)b * WA + n0) * W;
     for (int t = threadIdx.x; t < cnt; t += NT)
         Zb[t] = X[t];
 }
@@ -1666,22 +1671,22 @@ __device__ inline float2 amp_cross_sep(float2 za, float2 zb)
 }
 
 // The correlation's azimuth half: slot c of a workgroup holds column
-// l = c0 + c (l < W/2) in X and its mirror -l in Y (column W/2 for l = 0);
-// forward transforms, separation and cross spectra of both columns,
-// inverse transforms; out S [B][W][W].
-__global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *Z, int W, const uint8_t *state, float2 *S, int lc,
-                                                   stfft::PlanArgs Pf, stfft::PlanArgs Pi, stfft::FastDiv dW)
+// l = c0 + c (l < WR/2) in X and its mirror -l in Y (column WR/2 for l = 0);
+// forward transforms (WA long), separation and cross spectra of both
+// columns, inverse transforms; out S [B][WA][WR].
+__global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *Z, int WA, int WR, const uint8_t *state, float2 *S,
+                                                   int lc, stfft::PlanArgs Pf, stfft::PlanArgs Pi, stfft::FastDiv dW)
 {
     extern __shared__ float2 ldsx[];
     constexpr int IT = kStElems / NT;
-    const int C = 1 << lc, H = W / 2;
+    const int C = 1 << lc, H = WR / 2, W = WA;
     int b, cb;
     xcd_item_block(gridDim.x, b, cb);
     if (state[b] != 2)
         return;
     const int c0 = cb * C, nc = imin(C, H - c0), px = C > 1 ? W + 1 : W;
     float2 *X = ldsx, *Y = ldsx + (size_t)C * px;
-    const float2 *Zb = Z + (size_t)b * W * W;
+    const float2 *Zb = Z + (size_t)b * WA * WR;
     const int cnt = W << lc;
     for (int t0 = 0; t0 < cnt; t0 += kStElems) {
         float2 u[IT], v[IT];
@@ -1689,8 +1694,8 @@ __global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *Z, int W, const 
         for (int it = 0; it < IT; it++) {
             const int t = t0 + threadIdx.x + it * NT, a = t >> lc, c = t & (C - 1), l = c0 + c;
             const bool ok = t < cnt && c < nc;
-            u[it] = ok ? Zb[(size_t)a * W + l] : make_float2(0.f, 0.f);
-            v[it] = ok ? Zb[(size_t)a * W + (l == 0 ? H : W - l)] : make_float2(0.f, 0.f);
+            u[it] = ok ? Zb[(size_t)a * WR + l] : make_float2(0.f, 0.f);
+            v[it] = ok ? Zb[(size_t)a * WR + (l == 0 ? H : WR - l)] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int it = 0; it < IT; it++) {
@@ -1718,7 +1723,7 @@ __global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *Z, int W, const 
             u[it] = v[it] = make_float2(0.f, 0.f);
             if (t < cnt) {
                 const int c = stfft::fdiv(t, dW), k = t - c * W, km = k == 0 ? 0 : W - k;
-                if (c0 + c == 0) {  // columns 0 and W/2 are their own mirrors
+                if (c0 + c == 0) {  // columns 0 and WR/2 are their own mirrors
                     u[it] = amp_cross_sep(X[k], X[km]);
                     v[it] = amp_cross_sep(Y[k], Y[km]);
                 } else {
@@ -1740,44 +1745,44 @@ __global__ void __launch_bounds__(NT) k_amp_colx2(const float2 *Z, int W, const 
         __syncthreads();
     }
     // s = F^-1(F1 conj(F2)) is real, so T(n, -l) = conj T(n, l): only the
-    // columns l <= W/2 are transformed back and stored
+    // columns l <= WR/2 are transformed back and stored
     if (c0 != 0) {
         stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
     } else if ((C + 1) * W <= kStElems) {
-        stfft::lds_fft<kStElems, NT>(X, px, C + 1, Pi, +1);  // Y's slot 0 (column W/2) is sequence C
+        stfft::lds_fft<kStElems, NT>(X, px, C + 1, Pi, +1);  // Y's slot 0 (column WR/2) is sequence C
     } else {
         stfft::lds_fft<kStElems, NT>(X, px, C, Pi, +1);
         stfft::lds_fft<kStElems, NT>(Y, px, 1, Pi, +1);
     }
-    float2 *Sb = S + (size_t)b * W * W;
+    float2 *Sb = S + (size_t)b * WA * WR;
     for (int t = threadIdx.x; t < cnt; t += NT) {
         const int n = t >> lc, c = t & (C - 1);
         if (c < nc)
-            Sb[(size_t)n * W + c0 + c] = X[c * px + n];
+            Sb[(size_t)n * WR + c0 + c] = X[c * px + n];
     }
     if (c0 == 0)
         for (int n = threadIdx.x; n < W; n += NT)
-            Sb[(size_t)n * W + H] = Y[n];
+            Sb[(size_t)n * WR + H] = Y[n];
 }
 
-// The correlation's range half: from the stored columns l <= W/2 of T,
-// two rows per complex inverse transform (row n0 + 2q in the real part,
+// The correlation's range half: from the stored columns l <= W/2 of T
+// (WA rows of W = WR), two rows per complex inverse transform (row n0 + 2q in the real part,
 // n0 + 2q + 1 in the imaginary part, each transform real), then |s| into
 // the fftshifted surface and this block's top two, as k_cpx_rowmag.
-__global__ void __launch_bounds__(NT) k_amp_rowmag(const float2 *T, int W, int R, float *mag, Top2 *part,
+__global__ void __launch_bounds__(NT) k_amp_rowmag(const float2 *T, int WA, int W, int R, float *mag, Top2 *part,
                                                     stfft::PlanArgs Pi, stfft::FastDiv dW, const uint8_t *state)
 {
     extern __shared__ float2 ldsx[];
     __shared__ Top2 sht[NT / 64];
     constexpr int IT = kStElems / NT;
-    const int H = W / 2;
-    const int b = blockIdx.y, n0 = blockIdx.x * R, nrow = imin(R, W - n0), npair = (nrow + 1) / 2;
+    const int H = W / 2, HA = WA / 2;
+    const int b = blockIdx.y, n0 = blockIdx.x * R, nrow = imin(R, WA - n0), npair = (nrow + 1) / 2;
     if (state[b] != 2) {
         if (threadIdx.x == 0)
             part[(size_t)b * gridDim.x + blockIdx.x] = Top2{0.0f, 0x7fffffff, 0.0f, 0.0f};
         return;
     }
-    const float2 *Tb = T + ((size_t)b * W + n0) * W;
+    const float2 *Tb = T + ((size_t)b * WA + n0) * W;
     const int cnt = npair * W;
     {
         float2 v[IT];
@@ -1805,16 +1810,16 @@ __global__ void __launch_bounds__(NT) k_amp_rowmag(const float2 *T, int W, int R
     }
     __syncthreads();
     stfft::lds_fft<kStElems, NT>(ldsx, W, npair, Pi, +1);
-    float *M = mag + (size_t)b * W * W;
+    float *M = mag + (size_t)b * WA * W;
     Top2 best{0.0f, 0x7fffffff, 0.0f, 0.0f};
     for (int t = threadIdx.x; t < nrow * W; t += NT) {
         const int i = stfft::fdiv(t, dW), m = t - i * W, n = n0 + i;
         const float2 z = ldsx[(i >> 1) * W + m];
         const float v = fabsf((i & 1) ? z.y : z.x);
-        const int I = n < H ? n + H : n - H, J = m < H ? m + H : m - H;
+        const int I = n < HA ? n + HA : n - HA, J = m < H ? m + H : m - H;
         M[(size_t)I * W + J] = v;
-        const int i0 = I < H ? I : I - H, j0 = J < H ? J : J - H;
-        const int q = (I < H) ? (J < H ? 0 : 3) : (J < H ? 2 : 1);
+        const int i0 = I < HA ? I : I - HA, j0 = J < H ? J : J - H;
+        const int q = (I < HA) ? (J < H ? 0 : 3) : (J < H ? 2 : 1);
         const int rank = (i0 * H + j0) * 4 + q;
         if (v > 0.0f)
             best = top2_combine(best, Top2{v, rank, 0.0f, v});
@@ -1827,7 +1832,7 @@ __global__ void __launch_bounds__(NT) k_amp_rowmag(const float2 *T, int W, int R
 }
 
 // The mean powers of ampMatch from the partials, in block order.
-__global__ void k_amp_psum(const double2 *ppart, int nPf, int B, int W, const uint8_t *state, double *p)
+__global__ void k_amp_psum(const double2 *ppart, int nPf, int B, int WA, int WR, const uint8_t *state, double *p)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B || state[b] != 2)
@@ -1837,7 +1842,7 @@ __global__ void k_amp_psum(const double2 *ppart, int nPf, int B, int W, const ui
         p1 += ppart[(size_t)b * nPf + k].x;
         p2 += ppart[(size_t)b * nPf + k].y;
     }
-    const double N = (double)((size_t)W * W);
+    const double N = (double)((size_t)WA * WR);
     p[2 * b] = p1 / N;
     p[2 * b + 1] = p2 / N;
 }
@@ -2235,8 +2240,20 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
 
 void host_hanning(int wA, int wR, std::vector<float> &h)
 {
-    // computeHanning, speckleTrack.c:1830-1852 (square windows)
-    h.assign((size_t)wA * wR, 0.f);
+    // computeHanning, speckleTrack.c:1830-1852, on the reference's storage:
+    // one wA x wR block with rows of wR (speckleTrack.c:1775) whose columns
+    // are indexed with wA / 2 (:1843-1844).  For wR != wA an element
+    // [i][j] lands at flat offset i wR + j, in a neighbouring row or outside
+    // the block: writes inside it land where the reference's do, in its
+    // loop order; writes outside it are dropped (the reference's corrupt
+    // its heap) and cells never written stay 0 (indeterminate there).
+    const long n = (long)wA * wR;
+    h.assign((size_t)n, 0.f);
+    auto put = [&](int i, int j, float v) {
+        const long f = (long)i * wR + j;
+        if (f >= 0 && f < n)
+            h[(size_t)f] = v;
+    };
     for (int i = 0; i < wA / 2; i++) {
         int i1 = wA / 2 + i, i2 = wA / 2 - i - 1;
         double wi = 0.5 * (1.0 + cos((2.0 * M_PI) * (double)i / (double)(wA - 1)));
@@ -2244,10 +2261,10 @@ void host_hanning(int wA, int wR, std::vector<float> &h)
             int j1 = wA / 2 + j, j2 = wA / 2 - j - 1;
             double wj = 0.5 * (1.0 + cos((2.0 * M_PI) * (double)j / (double)(wR - 1)));
             float v = (float)(wi * wj);
-            h[(size_t)i1 * wR + j1] = v;
-            h[(size_t)i2 * wR + j2] = v;
-            h[(size_t)i2 * wR + j1] = v;
-            h[(size_t)i1 * wR + j2] = v;
+            put(i1, j1, v);
+            put(i2, j2, v);
+            put(i2, j1, v);
+            put(i1, j2, v);
         }
     }
 }
@@ -2635,7 +2652,7 @@ struct StrackJob {
     st_cull_params *cull = nullptr;
     bool intOn = false, gauss = false, mfmaPath = false, fusedPath = false;
     int kPeak = 0, kAmpPeak = 0, kGather = 0, kFft = 0, kAmpDet = 0;
-    int npts = 0, w = 0, intPs = 0;
+    int npts = 0, w = 0, wa = 0, intPs = 0;  // w: range window, wa: azimuth window
     std::vector<float> hann;
     std::vector<int> hr2, ha2;
     size_t fixed = 0;
@@ -2674,11 +2691,17 @@ struct StrackJob {
     void plan()
     {
         const st_track_params &par = *pp;
-        ST_REQUIRE(par.wR == par.wA && par.wRa == par.wAa, ST_ERR_UNSUPPORTED, "non-square windows");
-        ST_REQUIRE(par.wR >= 8 && par.wRa >= 8 && !(par.wR & 1) && !(par.wRa & 1), ST_ERR_ARG, "window sizes");
-        ST_REQUIRE(stfft::supported_size(par.wR) && stfft::supported_size(2 * par.wR) &&
-                       stfft::supported_size(par.wRa * LA) && stfft::supported_size(4 * par.wRa),
-                   ST_ERR_UNSUPPORTED, "window size not supported by the FFT engine");
+        // range and azimuth sizes are independent (parseTrack.c:167-171)
+        ST_REQUIRE(par.wR >= 8 && par.wA >= 8 && par.wRa >= 8 && par.wAa >= 8 && !(par.wR & 1) && !(par.wA & 1) &&
+                       !(par.wRa & 1) && !(par.wAa & 1),
+                   ST_ERR_ARG, "window sizes");
+        for (int ws : {par.wR, par.wA})
+            ST_REQUIRE(stfft::supported_size(ws) && stfft::supported_size(2 * ws), ST_ERR_UNSUPPORTED,
+                       "window size not supported by the FFT engine");
+        for (int ws : {par.wRa, par.wAa})
+            ST_REQUIRE(stfft::supported_size(ws) && stfft::supported_size(2 * ws) && stfft::supported_size(ws * LA) &&
+                           stfft::supported_size(2 * ws * LA),
+                       ST_ERR_UNSUPPORTED, "window size not supported by the FFT engine");
         ST_REQUIRE(par.nR > 0 && par.nA > 0 && par.scaleFactor > 0, ST_ERR_ARG, "grid");
         ST_REQUIRE(par.polyShift || (par.initDr && par.initDa && par.initNr > 0 && par.initNa > 0), ST_ERR_ARG,
                    "initial offsets: polyShift=0 needs initDr/initDa");
@@ -2687,14 +2710,17 @@ struct StrackJob {
         intOn = par.intf != nullptr && !par.noComplex;
         // -gauss fits its peak on the staged path's magnitude surfaces
         gauss = par.gaussFlag != 0;
+        // the matrix-core and fused kernels take square windows; the staged
+        // kernels take any supported wA x wR
+        const bool square = par.wR == par.wA;
         int precision = par.precision;
         if (precision == ST_PREC_AUTO) {
             TuneEntry te;
-            precision = tune_find(ctx->device, par.wR, &te) ? te.precision : ST_PREC_FP32;
+            precision = square && tune_find(ctx->device, par.wR, &te) ? te.precision : ST_PREC_FP32;
         }
-        mfmaPath = precision == ST_PREC_MFMA && (par.wR == 16 || par.wR == 32) && !par.baseFlag &&
+        mfmaPath = precision == ST_PREC_MFMA && square && (par.wR == 16 || par.wR == 32) && !par.baseFlag &&
                        !intOn && !gauss && !staged_only();
-        fusedPath = (par.wR == 16 || par.wR == 32 || par.wR == 64) && !gauss && !staged_only();
+        fusedPath = square && (par.wR == 16 || par.wR == 32 || par.wR == 64) && !gauss && !staged_only();
         kPeak = ctx->kernel_slot(mfmaPath ? "k_cpx_mfma" : (fusedPath ? "k_cpx_fused" : "peak_refine"));
         kAmpPeak = ctx->kernel_slot("amp_peak");
         kGather = ctx->kernel_slot("cpx_gather");
@@ -2702,6 +2728,7 @@ struct StrackJob {
         kAmpDet = ctx->kernel_slot("amp_detect");
         npts = par.nR * par.nA;
         w = par.wR;
+        wa = par.wA;
 
         // per-point state + inputs
         host_hanning(par.wA, par.wR, hann);
@@ -2733,9 +2760,9 @@ struct StrackJob {
                        ST_ERR_ARG, "cull grids must be the tracking outputs");
             fixed += cull_device_bytes((size_t)npts);
         }
-        const int wsMax = par.wAa * (par.maxTries >= 2 ? LA : 1);
-        perCpx = (size_t)w * w * (16 + 32 + 32 + 16) + 2048;
-        perAmp = (size_t)wsMax * wsMax * 128 + (size_t)2 * wsMax * 96 + 4096;
+        const int sMax = par.maxTries >= 2 ? LA : 1, wsA = par.wAa * sMax, wsR = par.wRa * sMax;
+        perCpx = (size_t)w * wa * (16 + 32 + 32 + 16) + 2048;
+        perAmp = (size_t)wsA * wsR * 128 + (size_t)(wsA + wsR) * 96 + 4096;
     }
 
     // Bytes of the fused path's per-item results (an upper bound: every
@@ -2912,30 +2939,34 @@ struct StrackJob {
         } else if (nCpx > 0 && !fusedPath) {
             int B = std::min(Bc, nCpx);
             TuneEntry te;
-            if (tune_find(ctx->device, w, &te) && te.stagedBatch > 0)
+            if (w == wa && tune_find(ctx->device, w, &te) && te.stagedBatch > 0)
                 B = std::min(B, te.stagedBatch);
+            const size_t N = (size_t)wa * w;  // wA rows of wR samples
             CpxBatch C;
-            C.P = ar.take<float2>((size_t)2 * B * w * w);
-            C.S = ar.take<float2>((size_t)B * 4 * w * w);
-            C.tmp = ar.take<float2>((size_t)B * 4 * w * w);
-            C.mag = ar.take<float>((size_t)B * 4 * w * w);
+            C.P = ar.take<float2>(2 * B * N);
+            C.S = ar.take<float2>(B * 4 * N);
+            C.tmp = ar.take<float2>(B * 4 * N);
+            C.mag = ar.take<float>(B * 4 * N);
             C.res = ar.take<PeakRes>(B);
             C.sig = ar.take<float>(2 * (size_t)B);
             C.have = ar.take<uint8_t>(B);
             C.az = ar.take<int>(B);
             C.p = ar.take<double>(2 * (size_t)B);
             // blocking of the two passes after the range transforms: Cx spectrum
-            // columns per k_cpx_colx workgroup, Rr rows per k_cpx_rowmag workgroup
-            const int W2 = 2 * w;
-            ST_REQUIRE(W2 <= kStElems, ST_ERR_UNSUPPORTED, "window too large for the staged complex path");
+            // columns (each 2 wA long) per k_cpx_colx workgroup, Rr rows (each
+            // 2 wR long) per k_cpx_rowmag workgroup
+            const int WA = 2 * wa, WR = 2 * w;
+            ST_REQUIRE(WA <= kStElems && WR <= kStElems, ST_ERR_UNSUPPORTED,
+                       "window too large for the staged complex path");
             int lcx = 0;
-            while ((2 << lcx) <= 16 && (2 << lcx) * W2 <= kStElems)
+            while ((2 << lcx) <= 16 && (2 << lcx) <= w && (2 << lcx) * WA <= kStElems)
                 lcx++;
-            const int Cx = 1 << lcx, Rr = kStElems / W2, nPart = (W2 + Rr - 1) / Rr;
-            const size_t ldsX = (size_t)Cx * std::max(2 * (w + 1), W2 + 1) * sizeof(float2),
-                         ldsR = (size_t)Rr * W2 * sizeof(float2);
-            const stfft::PlanArgs PfW = ctx->fft.plan(w, -1), PiW2 = ctx->fft.plan(W2, +1);
-            const stfft::FastDiv dW2 = stfft::make_fastdiv((uint32_t)W2);
+            const int Cx = 1 << lcx, Rr = kStElems / WR, nPart = (WA + Rr - 1) / Rr;
+            const size_t ldsX = (size_t)Cx * std::max(2 * (wa + 1), WA + 1) * sizeof(float2),
+                         ldsR = (size_t)Rr * WR * sizeof(float2);
+            const stfft::PlanArgs PfA = ctx->fft.plan(wa, -1), PiA2 = ctx->fft.plan(WA, +1),
+                                  PiR2 = ctx->fft.plan(WR, +1);
+            const stfft::FastDiv dA2 = stfft::make_fastdiv((uint32_t)WA), dR2 = stfft::make_fastdiv((uint32_t)WR);
             Top2 *part = (Top2 *)C.tmp;  // [B][nPart]
             for (int off = 0; off < nCpx; off += B) {
                 ctx->check_cancel();
@@ -2944,20 +2975,21 @@ struct StrackJob {
                 hipLaunchKernelGGL(k_cpx_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, C);
                 ctx->time_end(kGather);
                 ctx->time_begin(kFft);
-                ctx->fft.rows(C.P, (long)2 * nb * w, w, -1, st);
+                ctx->fft.rows(C.P, (long)2 * nb * wa, w, -1, st);
                 ctx->time_end(kFft);
                 ctx->time_begin(kPeak);
-                hipLaunchKernelGGL(k_cpx_colx, dim3((w + Cx - 1) / Cx, nb), dim3(NT), ldsX, st, C.P, nb, w, C.az, C.S, lcx,
-                                   PfW, PiW2, dW2);
-                hipLaunchKernelGGL(k_cpx_rowmag, dim3(nPart, nb), dim3(NT), ldsR, st, C.S, w, Rr, C.mag, part, PiW2, dW2);
-                const GaussArg G{gauss ? 1 : 0, w, par.hanningFlag ? 1.5 : 1.0, 1.0, C.p};
+                hipLaunchKernelGGL(k_cpx_colx, dim3((w + Cx - 1) / Cx, nb), dim3(NT), ldsX, st, C.P, nb, wa, w, C.az, C.S,
+                                   lcx, PfA, PiA2, dA2);
+                hipLaunchKernelGGL(k_cpx_rowmag, dim3(nPart, nb), dim3(NT), ldsR, st, C.S, wa, w, Rr, C.mag, part, PiR2,
+                                   dR2);
+                const GaussArg G{gauss ? 1 : 0, wa, w, par.hanningFlag ? 1.5 : 1.0, 1.0, C.p};
                 if (gauss) {
-                    hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0, C.res,
-                                       G);
-                    hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, C.mag, 2 * w, 0, C.res, nb,
+                    hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, WA, WR, 0,
+                                       C.res, G);
+                    hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, C.mag, WA, WR, 0, C.res, nb,
                                        G);
                 } else
-                    hipLaunchKernelGGL(k_peak_merge<false>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, 2 * w, 0,
+                    hipLaunchKernelGGL(k_peak_merge<false>, dim3(nb), dim3(NT), 0, st, part, nPart, C.mag, WA, WR, 0,
                                        C.res, G);
                 ctx->time_end(kPeak);
                 hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, C.res, nb, C.sig);
@@ -3017,33 +3049,36 @@ struct StrackJob {
             const int nA = read_count(ctx, S.counts, CNT_LIST);
             if (nA == 0)
                 break;
-            const int ws = par.wAa * (large ? LA : 1), W = 2 * ws;
+            // patches wsA x wsR, interpolated surfaces WA x WR
+            const int wsA = par.wAa * (large ? LA : 1), wsR = par.wRa * (large ? LA : 1), WA = 2 * wsA, WR = 2 * wsR;
+            ST_REQUIRE(WA <= kStElems && WR <= kStElems, ST_ERR_UNSUPPORTED,
+                       "window too large for the staged amplitude path");
             // blocking: Cx interpolation columns per k_amp_colx workgroup, Rd
             // rows per k_amp_rowdet / k_amp_rowfwd workgroup, Cc correlation
             // columns per k_amp_colx2 workgroup, Rm rows per k_amp_rowmag one
             int lcx = 0, lcc = 0;
-            while ((2 << lcx) <= ws && (2 << lcx) * W <= kStElems)
+            while ((2 << lcx) <= wsR && (2 << lcx) * WA <= kStElems)
                 lcx++;
-            while ((2 << lcc) <= ws && (2 << lcc) * W * 2 <= kStElems)
+            while ((2 << lcc) <= wsR && (2 << lcc) * WA * 2 <= kStElems)
                 lcc++;
             const int Cx = 1 << lcx, Cc = 1 << lcc;
-            const int Rd = std::max(1, kStElems / (2 * W)), nPd = (W + Rd - 1) / Rd;
-            const int Rf = kStElems / W, nPf = (W + Rf - 1) / Rf;
-            const int Rm = 2 * (kStElems / W), nPm = (W + Rm - 1) / Rm;
-            const size_t ldsX = (size_t)Cx * (Cx > 1 ? W + 1 : W) * sizeof(float2);
-            const size_t ldsD = (size_t)2 * Rd * W * sizeof(float2), ldsF = (size_t)Rf * W * sizeof(float2);
-            const size_t ldsC = (size_t)2 * Cc * (Cc > 1 ? W + 1 : W) * sizeof(float2);
-            const size_t ldsM = (size_t)(Rm / 2) * W * sizeof(float2);
-            const size_t per = (size_t)ws * ws * 128 + (size_t)nPd * (sizeof(AmpPart) + sizeof(double2)) +
+            const int Rd = std::max(1, kStElems / (2 * WR)), nPd = (WA + Rd - 1) / Rd;
+            const int Rf = kStElems / WR, nPf = (WA + Rf - 1) / Rf;
+            const int Rm = 2 * (kStElems / WR), nPm = (WA + Rm - 1) / Rm;
+            const size_t ldsX = (size_t)Cx * (Cx > 1 ? WA + 1 : WA) * sizeof(float2);
+            const size_t ldsD = (size_t)2 * Rd * WR * sizeof(float2), ldsF = (size_t)Rf * WR * sizeof(float2);
+            const size_t ldsC = (size_t)2 * Cc * (Cc > 1 ? WA + 1 : WA) * sizeof(float2);
+            const size_t ldsM = (size_t)(Rm / 2) * WR * sizeof(float2);
+            const size_t per = (size_t)wsA * wsR * 128 + (size_t)nPd * (sizeof(AmpPart) + sizeof(double2)) +
                                (size_t)nPm * sizeof(Top2) + 2048;
             // the amplitude kernels put 2 x batch on grid y (at most 65535)
             const size_t kMaxAmpBatch = 16384;
             const int B = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nA, (ar.cap - ar.used) / per, kMaxAmpBatch}));
             AmpBatch A;
-            A.I = ar.take<float2>((size_t)2 * B * ws * ws);
-            A.F = ar.take<float2>((size_t)2 * B * W * W);
-            A.S = ar.take<float2>((size_t)B * W * W);
-            A.mag = ar.take<float>((size_t)B * W * W);
+            A.I = ar.take<float2>((size_t)2 * B * wsA * wsR);
+            A.F = ar.take<float2>((size_t)2 * B * WA * WR);
+            A.S = ar.take<float2>((size_t)B * WA * WR);
+            A.mag = ar.take<float>((size_t)B * WA * WR);
             A.res = ar.take<PeakRes>(B);
             A.sig = ar.take<float>(2 * (size_t)B);
             A.state = ar.take<uint8_t>(B);
@@ -3053,36 +3088,40 @@ struct StrackJob {
             A.ppart = ar.take<double2>((size_t)B * nPd);
             A.mean = ar.take<double2>(B);
             A.part = ar.take<Top2>((size_t)B * nPm);
-            const stfft::PlanArgs Pfs = ctx->fft.plan(ws, -1), PfW = ctx->fft.plan(W, -1), PiW = ctx->fft.plan(W, +1);
-            const stfft::FastDiv dW = stfft::make_fastdiv((uint32_t)W);
+            const stfft::PlanArgs PfsA = ctx->fft.plan(wsA, -1), PfA = ctx->fft.plan(WA, -1),
+                                  PiA = ctx->fft.plan(WA, +1), PfR = ctx->fft.plan(WR, -1),
+                                  PiR = ctx->fft.plan(WR, +1);
+            const stfft::FastDiv dA = stfft::make_fastdiv((uint32_t)WA), dR = stfft::make_fastdiv((uint32_t)WR);
             for (int off = 0; off < nA; off += B) {
                 ctx->check_cancel();
                 const int nb = std::min(B, nA - off);
                 hipLaunchKernelGGL(k_amp_gather, dim3(nb), dim3(NT), 0, st, P, S, list, off, nb, large, A);
-                ctx->fft.rows(A.I, (long)2 * nb * ws, ws, -1, st);
-                hipLaunchKernelGGL(k_amp_colx, dim3((ws + Cx - 1) / Cx, 2 * nb), dim3(NT), ldsX, st, A.I, nb, ws, A.az,
-                                   A.state, A.F, lcx, Pfs, PiW, dW);
+                ctx->fft.rows(A.I, (long)2 * nb * wsA, wsR, -1, st);
+                hipLaunchKernelGGL(k_amp_colx, dim3((wsR + Cx - 1) / Cx, 2 * nb), dim3(NT), ldsX, st, A.I, nb, wsA, wsR,
+                                   A.az, A.state, A.F, lcx, PfsA, PiA, dA);
                 ctx->time_begin(kAmpDet);
-                hipLaunchKernelGGL(k_amp_rowdet, dim3(nPd, nb), dim3(NT), ldsD, st, P, A.F, nb, ws, Rd, A.state,
-                                   (float *)A.S, A.dpart, PiW, dW);
+                hipLaunchKernelGGL(k_amp_rowdet, dim3(nPd, nb), dim3(NT), ldsD, st, P, A.F, nb, wsA, wsR, Rd, A.state,
+                                   (float *)A.S, A.dpart, PiR, dR);
                 ctx->time_end(kAmpDet);
-                hipLaunchKernelGGL(k_amp_stat, dim3(nb), dim3(64), 0, st, A.dpart, nPd, W, A.state, A.mean);
-                hipLaunchKernelGGL(k_amp_rowfwd, dim3(nPf, nb), dim3(NT), ldsF, st, (const float *)A.S, nb, W, Rf, A.mean,
-                                   A.state, A.F, A.ppart, PfW);
-                hipLaunchKernelGGL(k_amp_colx2, dim3((ws + Cc - 1) / Cc, nb), dim3(NT), ldsC, st, A.F, W, A.state, A.S,
-                                   lcc, PfW, PiW, dW);
-                hipLaunchKernelGGL(k_amp_psum, dim3((nb + 127) / 128), dim3(128), 0, st, A.ppart, nPf, nb, W, A.state,
-                                   A.p);
+                hipLaunchKernelGGL(k_amp_stat, dim3(nb), dim3(64), 0, st, A.dpart, nPd, WA, WR, A.state, A.mean);
+                hipLaunchKernelGGL(k_amp_rowfwd, dim3(nPf, nb), dim3(NT), ldsF, st, (const float *)A.S, nb, WA, WR, Rf,
+                                   A.mean, A.state, A.F, A.ppart, PfR);
+                hipLaunchKernelGGL(k_amp_colx2, dim3((wsR + Cc - 1) / Cc, nb), dim3(NT), ldsC, st, A.F, WA, WR, A.state,
+                                   A.S, lcc, PfA, PiA, dA);
+                hipLaunchKernelGGL(k_amp_psum, dim3((nb + 127) / 128), dim3(128), 0, st, A.ppart, nPf, nb, WA, WR,
+                                   A.state, A.p);
                 ctx->time_begin(kAmpPeak);
-                hipLaunchKernelGGL(k_amp_rowmag, dim3(nPm, nb), dim3(NT), ldsM, st, A.S, W, Rm, A.mag, A.part, PiW, dW,
-                                   A.state);
-                const GaussArg G{gauss ? 1 : 0, W / 2, 1.0, 16.0, A.p};
+                hipLaunchKernelGGL(k_amp_rowmag, dim3(nPm, nb), dim3(NT), ldsM, st, A.S, WA, WR, Rm, A.mag, A.part, PiR,
+                                   dR, A.state);
+                const GaussArg G{gauss ? 1 : 0, wsA, wsR, 1.0, 16.0, A.p};
                 if (gauss) {
-                    hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, A.part, nPm, A.mag, W, 1, A.res, G);
-                    hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, A.mag, W, 1, A.res, nb, G);
-                } else
-                    hipLaunchKernelGGL(k_peak_merge<false>, dim3(nb), dim3(NT), 0, st, A.part, nPm, A.mag, W, 1, A.res,
+                    hipLaunchKernelGGL(k_peak_merge<true>, dim3(nb), dim3(NT), 0, st, A.part, nPm, A.mag, WA, WR, 1,
+                                       A.res, G);
+                    hipLaunchKernelGGL(k_gauss_fit, dim3((nb + 127) / 128), dim3(128), 0, st, A.mag, WA, WR, 1, A.res, nb,
                                        G);
+                } else
+                    hipLaunchKernelGGL(k_peak_merge<false>, dim3(nb), dim3(NT), 0, st, A.part, nPm, A.mag, WA, WR, 1,
+                                       A.res, G);
                 ctx->time_end(kAmpPeak);
                 hipLaunchKernelGGL(k_lm, dim3((2 * nb + 127) / 128), dim3(128), 0, st, A.res, nb, A.sig);
                 hipLaunchKernelGGL(k_amp_finalize, dim3((nb + 127) / 128), dim3(128), 0, st, P, S, list, off, nb, large, A);

@@ -22,20 +22,22 @@ SNR_RTOL = 1e-3
 
 
 def periodic_pair(w, nr, na, dr, da, seed=0, band=True):
-    """Image 1 tiles one w x w pattern; image 2 is image 1 shifted by the
-    integer offset (dr, da), so every w x w window of image 2 is a circular
-    shift of the matching window of image 1 (a closed-form case: the
-    correlation peak is exactly at the shift)."""
+    """Image 1 tiles one wA x wR pattern (w is wA = wR or a pair (wA, wR));
+    image 2 is image 1 shifted by the integer offset (dr, da), so every
+    wA x wR window of image 2 is a circular shift of the matching window of
+    image 1 (a closed-form case: the correlation peak is exactly at the
+    shift)."""
+    wa, wr = (w, w) if np.isscalar(w) else w
     rng = np.random.default_rng(seed)
-    pat = (rng.standard_normal((w, w)) + 1j * rng.standard_normal((w, w))) / np.sqrt(2)
+    pat = (rng.standard_normal((wa, wr)) + 1j * rng.standard_normal((wa, wr))) / np.sqrt(2)
     if band:
         # keep the pattern band-limited around a Doppler centroid
         F = np.fft.fft2(pat)
-        fa = np.fft.fftfreq(w)[:, None]
-        fr = np.fft.fftfreq(w)[None, :]
+        fa = np.fft.fftfreq(wa)[:, None]
+        fr = np.fft.fftfreq(wr)[None, :]
         keep = (np.abs(((fa - 0.12 + 0.5) % 1.0) - 0.5) < 0.4) & (np.abs(fr) < 0.43)
         pat = np.fft.ifft2(F * keep)
-    reps = (na // w + 2, nr // w + 2)
+    reps = (na // wa + 2, nr // wr + 2)
     big = np.tile(pat, reps)
     img1 = big[:na, :nr].astype(np.complex64)
     img2 = np.roll(np.roll(big, da, axis=0), dr, axis=1)[:na, :nr].astype(np.complex64)

@@ -124,11 +124,11 @@ def test_poll_reports_states(device):
         srv.wait(ids[-1])
         assert all(srv.poll(j).state == abi.ST_JOB_DONE for j in ids[:-1])
         assert all(srv.wait(j)[1].state == abi.ST_JOB_DONE for j in ids[:-1])
-        bad = track.default_params(rStart=0, aStart=0, deltaR=8, deltaA=8, nR=4, nA=4, wR=24, wA=32, wRa=24,
+        bad = track.default_params(rStart=0, aStart=0, deltaR=8, deltaA=8, nR=4, nA=4, wR=25, wA=32, wRa=24,
                                    wAa=32)
         jid = srv.submit(bad, images=(a, b), kind=kind)
         assert srv.poll(jid).state in (abi.ST_JOB_QUEUED, abi.ST_JOB_RUNNING, abi.ST_JOB_FAILED)
-        with pytest.raises(RuntimeError, match="non-square"):
+        with pytest.raises(RuntimeError, match="window sizes"):
             srv.wait(jid)
         # wait() released the job: its id is no longer known
         with pytest.raises(RuntimeError, match="unknown job"):
