@@ -214,14 +214,15 @@ int st_tune_load(st_ctx *ctx, const char *cacheFile);
  * correlation" is a window pair that reaches the FFT stage: every complex
  * match whose patches pass the data tests, every amplitude try whose
  * windows are in bounds, every strackw patch. */
+#define ST_KERNEL_SLOTS 16
 typedef struct {
     int64_t nPoints;
     int64_t nCorr[3];      /* patch correlations: complex, amp small, amp large */
     double stageMs[4];     /* device time: geometry, complex, amp small, amp large */
     double totalMs;        /* device time of the whole call */
-    double kernelMs[8];    /* summed duration of the dominant kernels (see kernelNames) */
-    int64_t kernelLaunches[8];
-    char kernelNames[8][32];
+    double kernelMs[ST_KERNEL_SLOTS]; /* summed duration of the dominant kernels (see kernelNames) */
+    int64_t kernelLaunches[ST_KERNEL_SLOTS];
+    char kernelNames[ST_KERNEL_SLOTS][32];
 } st_track_stats;
 
 int st_last_stats(st_ctx *ctx, st_track_stats *stats);

@@ -13,6 +13,7 @@ ST_ERR_UNSUPPORTED = -3
 ST_ERR_NOMEM = -4
 ST_ERR_NODEV = -5
 ST_ERR_CANCELLED = -7
+ST_KERNEL_SLOTS = 16  # kernel timing slots of st_track_stats
 
 ST_SAMPLE_C64 = 0
 ST_SAMPLE_CI16 = 1
@@ -91,8 +92,8 @@ class TrackDiag(C.Structure):
 class TrackStats(C.Structure):
     _fields_ = [
         ("nPoints", C.c_int64), ("nCorr", C.c_int64 * 3), ("stageMs", C.c_double * 4),
-        ("totalMs", C.c_double), ("kernelMs", C.c_double * 8), ("kernelLaunches", C.c_int64 * 8),
-        ("kernelNames", (C.c_char * 32) * 8),
+        ("totalMs", C.c_double), ("kernelMs", C.c_double * ST_KERNEL_SLOTS),
+        ("kernelLaunches", C.c_int64 * ST_KERNEL_SLOTS), ("kernelNames", (C.c_char * 32) * ST_KERNEL_SLOTS),
     ]
 
 

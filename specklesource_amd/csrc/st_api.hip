@@ -88,7 +88,7 @@ void st_ctx::stats_reset()
 
 int st_ctx::kernel_slot(const char *name)
 {
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < ST_KERNEL_SLOTS; k++) {
         if (stats.kernelNames[k][0] == 0) {
             strncpy(stats.kernelNames[k], name, 31);
             return k;
@@ -96,7 +96,7 @@ int st_ctx::kernel_slot(const char *name)
         if (strncmp(stats.kernelNames[k], name, 31) == 0)
             return k;
     }
-    return 7;
+    return ST_KERNEL_SLOTS - 1;
 }
 
 void st_ctx::time_begin(int slot)

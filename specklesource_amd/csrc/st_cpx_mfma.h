@@ -68,9 +68,7 @@ struct MfGeom {
     static constexpr int RP = 8 * W * LDK;
     static constexpr int RX = 4 * W * LDK;
     static constexpr int RL = 8 * W * LDK > 16 * W * W ? 8 * W * LDK : 16 * W * W;
-    static constexpr int PM = RP + RX - 8 * 8 * 8;  // offset of Pm
     static constexpr int BYTES = RP + RX + RL;
-    static_assert(8 * 48 * 16 <= PM, "T overlaps Pm");
     static constexpr int WPP = W == 16 ? 1 : 4;  // waves per patch
     static constexpr int PPB = 4 / WPP;          // patches per 256-thread block
 };
@@ -151,7 +149,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1
     const int tp = wv * 64 + lane;    // thread index within the patch
     unsigned char *base = mf_lds + (size_t)slot * G::BYTES;
     __bf16 *bP = (__bf16 *)base;                  // P: [g][hi/lo] planes; later G: [hi/lo] 2W rows
-    double2 *T = (double2 *)base;                 // refinement rows (8 x 48), after G and X~ are dead
     __bf16 *bX = (__bf16 *)(base + G::RP);        // X~: [hi/lo]
     __bf16 *bL = (__bf16 *)(base + G::RP + G::RX);  // L: [g][hi/lo]
     float *Mg = (float *)(base + G::RP + G::RX);    // |C| (2W x 2W), after L is dead
@@ -390,44 +387,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1
             res->raw = raw;
         }
         // ---------------- FP64 refinement of the 8 x 8 neighbourhood
-        // (overSampleC, speckleTrack.c:852-863): rows first, T[mm][v]
-        double *Pm = (double *)(base + G::PM);
+        // (overSampleC, speckleTrack.c:852-863) in this patch's LDS region
+        // (R from constant memory: the scratch must end before the surface Mg)
+        static_assert(G::RP + G::RX >= RefLayout<false>::BYTES, "refinement scratch overlaps the surface");
+        unsigned char *scr = (unsigned char *)base;
+        double *Pm = (double *)(scr + kRefPm);
         if (tp < NFAST * NFAST)
             Pm[tp] = (double)Mg[(J - NFAST / 2 + tp % NFAST) * WW + (I - NFAST / 2 + tp / NFAST)];
         mf_sync<WPP>();
-#pragma unroll 1
-        for (int e = tp; e < NFAST * kFine; e += TPP) {
-            const int mm = e / kFine, v = e - mm * kFine;
-            double re = 0, im = 0;
-            for (int nn = 0; nn < NFAST; nn++) {
-                const double2 d = c_dir96[(v + 24) - 12 * nn + 96];
-                const double pv = Pm[mm * NFAST + nn];
-                re += pv * d.x;
-                im += pv * d.y;
-            }
-            T[e] = make_double2(re, im);
-        }
-        mf_sync<WPP>();
-        double f1 = 0.0, f2 = 0.0;
-        int fr = 0x7fffffff;
-#pragma unroll 1
-        for (int e = tp; e < kFine * kFine; e += TPP) {
-            const int uu = e / kFine, v = e - uu * kFine;
-            double re = 0, im = 0;
-            for (int mm = 0; mm < NFAST; mm++) {
-                const double2 d = c_dir96[(uu + 24) - 12 * mm + 96];
-                const double2 tv = T[mm * kFine + v];
-                re += d.x * tv.x - d.y * tv.y;
-                im += d.x * tv.y + d.y * tv.x;
-            }
-            const double a = re * re + im * im;  // |Q|^2: same order as |Q|
-            if (a > f1 || (a == f1 && e < fr)) {
-                f2 = fmax(f2, f1);
-                f1 = a;
-                fr = e;
-            } else
-                f2 = fmax(f2, a);
-        }
+        double f1, f2;
+        int fr;
+        refine_fine<TPP, false, 2>(scr, tp, [] { mf_sync<WPP>(); }, f1, f2, fr);
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const double b1 = __shfl_xor(f1, o), b2 = __shfl_xor(f2, o);

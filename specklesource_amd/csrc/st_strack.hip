@@ -31,7 +31,8 @@
 using namespace sttrack;
 
 namespace sttrack {
-__constant__ double2 c_dir96[193];
+__constant__ double c_refR[kFine * NFAST];
+__constant__ double2 c_refB[16];
 }
 namespace fftreg {
 __constant__ float2 c_tw128[128];
@@ -381,6 +382,7 @@ struct CpxBatch {
     int *az;          // [B]
     double *p;        // [B][2]
     const int *list;  // fused and matrix-core paths: item -> grid point
+    float *nb;        // fused path: [B][kNb^2] |C| around each coarse peak
 };
 
 __global__ void __launch_bounds__(NT) k_cpx_gather(KPar P, Pts S, const int *list, int off, int B, CpxBatch C)
@@ -487,10 +489,9 @@ template <bool GAUSS>
 __device__ void peak_finish(const float *M, int WA, int WR, int mode, int I, int J, float raw, float sec, float snr,
                             PeakRes *out, const GaussArg &G, int item)
 {
-    __shared__ ArgMaxD shd[NT / 64];
-    __shared__ double shdd[NT / 64];
-    __shared__ double Pm[NFAST * NFAST];
-    __shared__ double2 T[NFAST * kFine];
+    __shared__ double shd[2 * (NT / 64)];
+    __shared__ int shr[NT / 64];
+    __shared__ __align__(16) unsigned char scr[RefLayout<true>::BYTES];
     PeakRes r;
     r.gap = raw > 0 ? (raw - sec) / raw : 0.f;
     r.snr = snr;
@@ -529,53 +530,20 @@ __device__ void peak_finish(const float *M, int WA, int WR, int mode, int I, int
         (void)item;
         return;
     }
-    // FP64 refinement: Q = A P A^T on the searched 48 x 48 box
+    // FP64 refinement on the searched 48 x 48 box
+    double *Pm = (double *)(scr + kRefPm);
     if (threadIdx.x < NFAST * NFAST)
         Pm[threadIdx.x] = (double)M[(I - NFAST / 2 + threadIdx.x / NFAST) * WR + (J - NFAST / 2 + threadIdx.x % NFAST)];
     __syncthreads();
-    for (int t = threadIdx.x; t < NFAST * kFine; t += NT) {
-        const int mm = t / kFine, v = t - mm * kFine;
-        double re = 0, im = 0;
-        for (int nn = 0; nn < NFAST; nn++) {
-            const double2 d = c_dir96[(v + 24) - 12 * nn + 96];
-            const double p = Pm[mm * NFAST + nn];
-            re += p * d.x;
-            im += p * d.y;
-        }
-        T[t] = make_double2(re, im);
-    }
-    __syncthreads();
-    ArgMaxD fb{0.0, 0x7fffffff};
-    double qv[(kFine * kFine + NT - 1) / NT];
-    int nq = 0;
-    for (int t = threadIdx.x; t < kFine * kFine; t += NT) {
-        const int u = t / kFine, v = t - u * kFine;
-        double re = 0, im = 0;
-        for (int mm = 0; mm < NFAST; mm++) {
-            const double2 d = c_dir96[(u + 24) - 12 * mm + 96];
-            const double2 tv = T[mm * kFine + v];
-            re += d.x * tv.x - d.y * tv.y;
-            im += d.x * tv.y + d.y * tv.x;
-        }
-        const double a = re * re + im * im;  // |Q|^2: same order as |Q|
-        qv[nq++] = a;
-        if (a > 0.0) {
-            ArgMaxD c{a, t};
-            fb = argmaxd_combine(fb, c);
-        }
-    }
-    fb = block_argmax_d<NT>(fb, shd);
-    double fsec = 0;
-    nq = 0;
-    for (int t = threadIdx.x; t < kFine * kFine; t += NT, nq++)
-        if (t != fb.rank)
-            fsec = fmax(fsec, qv[nq]);
-    fsec = block_max_d<NT>(fsec, shdd);
-    if (fb.rank != 0x7fffffff) {
-        r.fi = fb.rank / kFine;
-        r.fj = fb.rank - r.fi * kFine;
-        r.fineMax = sqrt(fb.v);
-        r.fineGap = (float)((sqrt(fb.v) - sqrt(fsec)) / sqrt(fb.v));
+    double f1, f2;
+    int fr;
+    refine_fine<NT, true>(scr, threadIdx.x, [] { __syncthreads(); }, f1, f2, fr);
+    block_top2_d<NT>(f1, f2, fr, shd, shr);
+    if (fr != 0x7fffffff) {
+        r.fi = fr / kFine;
+        r.fj = fr - r.fi * kFine;
+        r.fineMax = sqrt(f1);
+        r.fineGap = (float)((sqrt(f1) - sqrt(f2)) / sqrt(f1));
     } else {
         r.fi = r.fj = 0;
         r.fineMax = 0;
@@ -1847,6 +1815,84 @@ __global__ void k_amp_psum(const double2 *ppart, int nPf, int B, int WA, int WR,
     p[2 * b + 1] = p2 / N;
 }
 
+// Sub-pixel refinement and Gaussian-fit samples of the fused path's
+// attempts (overSampleC and the central-box argmax, speckleTrack.c:686-700,
+// 852-863; the samples of complexMatch's fits, :377-386), one wave per
+// attempt and four per workgroup, from the kNb x kNb |C| block that
+// k_cpx_fused left around each coarse peak.  Kept out of the transform
+// kernel, whose LDS and barriers would otherwise hold its workgroups
+// through this latency-bound stretch.
+__global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int WW)
+{
+    __shared__ double Rt[kFine * NFAST];
+    __shared__ double2 Pc[4][NFAST * NFAST];
+    __shared__ double2 Tq[4][NFAST * kFine];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int e = threadIdx.x; e < kFine * NFAST; e += 256)
+        Rt[e] = c_refR[e];
+    __syncthreads();
+    // from here on each wave works alone (no workgroup barrier follows)
+    const int it = blockIdx.x * 4 + wv;
+    if (it >= count || !C.have[it])
+        return;
+    PeakRes *res = &C.res[it];
+    const int I = res->I, J = res->J;
+    if (I < -999)
+        return;
+    const float *nb = C.nb + (size_t)it * kNb * kNb;
+    auto wsync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    {
+        const int m = lane / NFAST, n = lane % NFAST;
+        const double pm = (double)nb[(kNb / 2 - NFAST / 2 + m) * kNb + (kNb / 2 - NFAST / 2 + n)];
+        const double2 b = c_refB[m + n];
+        Pc[wv][lane] = make_double2(b.x * pm, b.y * pm);
+    }
+    wsync();
+    double f1, f2;
+    int fr;
+    refine_core<64, 3>(Pc[wv], Tq[wv], Rt, lane, wsync, f1, f2, fr);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double b1 = __shfl_xor(f1, o), b2 = __shfl_xor(f2, o);
+        const int br = __shfl_xor(fr, o);
+        if (b1 > f1 || (b1 == f1 && br < fr)) {
+            f2 = fmax(f1, b2);
+            f1 = b1;
+            fr = br;
+        } else
+            f2 = fmax(f2, b1);
+    }
+    int fi = 0, fj = 0;
+    if (fr != 0x7fffffff) {
+        fi = fr / kFine;
+        fj = fr - fi * kFine;
+    }
+    const int iF = I * NOVER + (fi + 24) - (NOVER * NFAST) / 2;
+    const int jF = J * NOVER + (fj + 24) - (NOVER * NFAST) / 2;
+    const int cI = iF / NOVER, cJ = jF / NOVER;
+    const bool ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < WW && (cJ - NGAUSS / 2) >= 0 &&
+                    (cJ + NGAUSS / 2) < WW;
+    if (lane == 0) {
+        res->fi = fi;
+        res->fj = fj;
+        res->fineMax = fr != 0x7fffffff ? sqrt(f1) : 0.0;
+        res->fineGap = fr != 0x7fffffff ? (float)((sqrt(f1) - sqrt(f2)) / sqrt(f1)) : 0.f;
+        res->lmOk = ok ? 1 : 0;
+        res->lmX0[0] = (float)(cJ - NGAUSS / 2);
+        res->lmX0[1] = (float)(cI - NGAUSS / 2);
+    }
+    if (ok && lane < 2 * NGAUSS) {
+        // cI, cJ within 2 samples of (I, J): inside the block
+        const int k = lane % NGAUSS, bi = cI - I + kNb / 2, bj = cJ - J + kNb / 2;
+        res->lmY[lane / NGAUSS][k] = lane < NGAUSS ? nb[bi * kNb + (bj - NGAUSS / 2 + k)]
+                                                   : nb[(bi - NGAUSS / 2 + k) * kNb + bj];
+    }
+}
+
 // Lanes per transform of the fused kernel.  Lane pairs at every window
 // size: lane quads at w = 64 (16 points per lane, 512-thread blocks) were
 // measured slower on the full-scene job (55.6 ms vs 43.6 ms per launch) --
@@ -1881,7 +1927,7 @@ struct FusedLanes {
 template <int W, int STYPE, int BASE, int OCC>
 __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgpu_waves_per_eu(OCC, 8))) k_cpx_fused(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
                                                      const int *__restrict__ jobStart, int njobs,
-                                                     const CpxBatch *__restrict__ Cp, float *magScratch)
+                                                     const CpxBatch *__restrict__ Cp)
 {
     const int count = jobStart[njobs];
     using namespace ::fftreg;
@@ -1889,8 +1935,6 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
     using FL = FusedLanes<W>;
     using LF = LaneFFT<W, L>;
     extern __shared__ float2 lds[];  // [2][BUF]
-    __shared__ double shd[32];
-    __shared__ int shr[16];
     __shared__ Top2 sht[16];
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
 
@@ -1925,7 +1969,6 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         const int r = (t / L) % W;
         const int h = t & (L - 1);
         const int blk = LF::blk(h);
-        float *M = magScratch + (size_t)blockIdx.x * WW * WW;
         const int pt = C.list[it];
         const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
         if (!C.have[it])
@@ -2035,12 +2078,14 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         __syncthreads();
         // ---------------- polyphase inverse: components (pp, q = g)
         const int krow = ((r - az) % W + W) % W;
-        const int ks = krow < W / 2 ? krow : krow - W;
         float2 *Z = lds + (size_t)g * BUF;
         Top2 best{0.0f, 0x7fffffff, 0.0f, 0.0f};
-        // exp(i pi (ks pp + ls q) / w) = row factor (per lane, per pp) x
-        // column factor (per element, q = g only): the column factor is
-        // applied once, before the component loop
+        // exp(i pi (ks pp + ls q) / w) = row factor (ks: signed frequency of
+        // the row, pp) x column factor (per element, q = g only).  The column
+        // factor is applied before the row transform; the row factor is
+        // constant along a row, so one row transform serves both components
+        // and the factor of pp = 1 multiplies its output on the column pass
+        // (element k of lane h is row blk H + k there).
         if (g) {
 #pragma unroll
             for (int k = 0; k < H; k++) {
@@ -2056,38 +2101,25 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
                 x[k] = cmul_f(x[k], cf);
             }
         }
-#pragma unroll 1
-        for (int pp = 0; pp < 2; pp++) {
-            float2 y[H];
-            const float2 rowf = c_tw128[(ks * pp * (64 / W)) & 127];
+        LF::inv(x, h);
+        // x[n'] = Z[krow][L n' + h]
 #pragma unroll
-            for (int k = 0; k < H; k++)
-                y[k] = cmul_f(x[k], rowf);
-            LF::inv(y, h);
-            // y[n'] = Z[krow][L n' + h]
-#pragma unroll
-            for (int n = 0; n < H; n++)
-                Z[FL::at(krow, L * n + h)] = y[n];
-            __syncthreads();
-            // columns: lane h holds k in [blk H, blk H + H)
-#pragma unroll
-            for (int k = 0; k < H; k++)
-                y[k] = Z[FL::at(blk * H + k, r)];
-            __syncthreads();
-            LF::inv(y, h);
-            // y[j] = C_pp,g[L j + h][r]  ->  surface row n = 2 (L j + h) + pp, col m = 2 r + g
-            const int m = 2 * r + g;
-            const int J = (m + W) & (WW - 1);
+        for (int n = 0; n < H; n++)
+            Z[FL::at(krow, L * n + h)] = x[n];
+        __syncthreads();
+        // surface row n = 2 (L j + h) + pp, column m = 2 r + g, fftshifted
+        const int Jc = (2 * r + g + W) & (WW - 1);
+        float mg0[H];  // |C| of component pp = 0, kept until the LDS is free
+        auto scan = [&](const float2 *y, int pp, float *mg) {
 #pragma unroll
             for (int jj = 0; jj < H; jj++) {
 #pragma clang fp contract(off)
-                const int n = 2 * (L * jj + h) + pp;
-                const int I = (n + W) & (WW - 1);
+                const int I = (2 * (L * jj + h) + pp + W) & (WW - 1);
                 const float x2 = y[jj].x * y[jj].x + y[jj].y * y[jj].y;
                 const float v = sqrtf(x2);
-                M[I * WW + J] = v;
-                const int i0 = I < W ? I : I - W, j0 = J < W ? J : J - W;
-                const int q = (I < W) ? (J < W ? 0 : 3) : (J < W ? 2 : 1);
+                mg[jj] = v;
+                const int i0 = I < W ? I : I - W, j0 = Jc < W ? Jc : Jc - W;
+                const int q = (I < W) ? (Jc < W ? 0 : 3) : (Jc < W ? 2 : 1);
                 const int rank = (i0 * W + j0) * 4 + q;
                 if (v > 0.0f) {
                     Top2 c{v, rank, 0.0f, v};
@@ -2095,6 +2127,42 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
                 } else {
                     best.v2 = fmaxf(best.v2, v);
                 }
+            }
+        };
+        {
+            float2 y[H];
+#pragma unroll
+            for (int k = 0; k < H; k++)
+                y[k] = Z[FL::at(blk * H + k, r)];
+            LF::inv(y, h);
+            scan(y, 0, mg0);
+        }
+        {
+            float2 y[H];
+#pragma unroll
+            for (int k = 0; k < H; k++) {
+                // row blk H + k, signed frequency k - blk H (L = 2)
+                float2 rf = c_tw128[(k * (64 / W)) & 127];
+#pragma unroll
+                for (int b2 = 1; b2 < L; b2++) {
+                    const int rho = b2 * H + k;
+                    const int rs = rho < W / 2 ? rho : rho - W;
+                    const float2 cb = c_tw128[(rs * (64 / W)) & 127];
+                    rf = blk == b2 ? cb : rf;
+                }
+                y[k] = cmul_f(Z[FL::at(blk * H + k, r)], rf);
+            }
+            __syncthreads();  // every read of Z done: the LDS takes the surface
+            LF::inv(y, h);
+            float mg1[H];
+            scan(y, 1, mg1);
+            // the 2w x 2w |C| surface in LDS (16 w^2 bytes <= the two transpose buffers)
+            float *Ms = (float *)lds;
+#pragma unroll
+            for (int jj = 0; jj < H; jj++) {
+                const int n = 2 * (L * jj + h);
+                Ms[((n + W) & (WW - 1)) * WW + Jc] = mg0[jj];
+                Ms[((n + 1 + W) & (WW - 1)) * WW + Jc] = mg1[jj];
             }
         }
         best = wg_top2<NTH>(best, sht);
@@ -2128,107 +2196,14 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
                 res->J = J;
                 res->raw = raw;
             }
-            double *Pm = (double *)lds;
-            double2 *T = (double2 *)(lds + 64);
-            if (t < NFAST * NFAST)
-                Pm[t] = (double)M[(I - NFAST / 2 + t / NFAST) * WW + (J - NFAST / 2 + t % NFAST)];
-            __syncthreads();
-#pragma unroll 1
-            for (int e = t; e < NFAST * kFine; e += NTH) {
-                const int mm = e / kFine, v = e - mm * kFine;
-                double re = 0, im = 0;
-                for (int nn = 0; nn < NFAST; nn++) {
-                    const double2 d = c_dir96[(v + 24) - 12 * nn + 96];
-                    const double pv = Pm[mm * NFAST + nn];
-                    re += pv * d.x;
-                    im += pv * d.y;
-                }
-                T[e] = make_double2(re, im);
-            }
-            __syncthreads();
-            double f1 = 0.0, f2 = 0.0;
-            int fr = 0x7fffffff;
-#pragma unroll 1
-            for (int e = t; e < kFine * kFine; e += NTH) {
-                const int u = e / kFine, v = e - u * kFine;
-                double re = 0, im = 0;
-                for (int mm = 0; mm < NFAST; mm++) {
-                    const double2 d = c_dir96[(u + 24) - 12 * mm + 96];
-                    const double2 tv = T[mm * kFine + v];
-                    re += d.x * tv.x - d.y * tv.y;
-                    im += d.x * tv.y + d.y * tv.x;
-                }
-                const double a = re * re + im * im;  // |Q|^2: same order as |Q|
-                if (a > f1 || (a == f1 && e < fr)) {
-                    f2 = fmax(f2, f1);
-                    f1 = a;
-                    fr = e;
-                } else
-                    f2 = fmax(f2, a);
-            }
-            // workgroup top-2 in double
-#pragma unroll
-            for (int o = 1; o < (NTH < 64 ? NTH : 64); o <<= 1) {
-                const double b1 = __shfl_xor(f1, o), b2 = __shfl_xor(f2, o);
-                const int br = __shfl_xor(fr, o);
-                if (b1 > f1 || (b1 == f1 && br < fr)) {
-                    f2 = fmax(f1, b2);
-                    f1 = b1;
-                    fr = br;
-                } else
-                    f2 = fmax(f2, b1);
-            }
-            if constexpr (NTH > 64) {
-                constexpr int NW = NTH / 64;
-                double *s1 = shd;  // [0, NW): best, [NW, 2 NW): second
-                int *sr = shr;
-                __syncthreads();
-                if ((t & 63) == 0) {
-                    s1[t >> 6] = f1;
-                    s1[NW + (t >> 6)] = f2;
-                    sr[t >> 6] = fr;
-                }
-                __syncthreads();
-                f1 = s1[0];
-                f2 = s1[NW];
-                fr = sr[0];
-                for (int k = 1; k < NW; k++) {
-                    const double b1 = s1[k], b2 = s1[NW + k];
-                    const int br = sr[k];
-                    if (b1 > f1 || (b1 == f1 && br < fr)) {
-                        f2 = fmax(f1, b2);
-                        f1 = b1;
-                        fr = br;
-                    } else
-                        f2 = fmax(f2, b1);
-                }
-                __syncthreads();
-            }
-            int fi = 0, fj = 0;
-            if (fr != 0x7fffffff) {
-                fi = fr / kFine;
-                fj = fr - fi * kFine;
-            }
-            if (t == 0) {
-                res->fi = fi;
-                res->fj = fj;
-                res->fineMax = fr != 0x7fffffff ? sqrt(f1) : 0.0;
-                res->fineGap = fr != 0x7fffffff ? (float)((sqrt(f1) - sqrt(f2)) / sqrt(f1)) : 0.f;
-            }
-            const int iF = I * NOVER + (fi + 24) - (NOVER * NFAST) / 2;
-            const int jF = J * NOVER + (fj + 24) - (NOVER * NFAST) / 2;
-            const int cI = iF / NOVER, cJ = jF / NOVER;
-            const bool ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < WW && (cJ - NGAUSS / 2) >= 0 &&
-                            (cJ + NGAUSS / 2) < WW;
-            if (t == 0) {
-                res->lmOk = ok ? 1 : 0;
-                res->lmX0[0] = (float)(cJ - NGAUSS / 2);
-                res->lmX0[1] = (float)(cI - NGAUSS / 2);
-            }
-            if (ok && t < 2 * NGAUSS) {
-                const int k = t % NGAUSS;
-                res->lmY[t / NGAUSS][k] = t < NGAUSS ? M[cI * WW + (cJ - NGAUSS / 2 + k)]
-                                                     : M[(cI - NGAUSS / 2 + k) * WW + cJ];
+            // the |C| block around the peak out of the LDS surface, for
+            // k_cpx_refine (cells off the surface are never read: the fits'
+            // edge tests reject them)
+            const float *Ms = (const float *)lds;
+            float *nb = C.nb + (size_t)it * kNb * kNb;
+            for (int e = t; e < kNb * kNb; e += NTH) {
+                const int ri = I - kNb / 2 + e / kNb, ci = J - kNb / 2 + e % kNb;
+                nb[e] = (ri >= 0 && ri < WW && ci >= 0 && ci < WW) ? Ms[ri * WW + ci] : 0.0f;
             }
         }
         __syncthreads();
@@ -2350,17 +2325,22 @@ void init_dirichlet(st_ctx *ctx)
     ST_REQUIRE(ctx->device >= 0 && ctx->device < 64, ST_ERR_ARG, "device index");
     if (done[ctx->device])
         return;
-    double2 tab[193];
-    for (int t = -96; t <= 96; t++) {
-        double re = 0, im = 0;
-        for (int k = -4; k <= 3; k++) {
-            double ang = 2.0 * M_PI * (double)k * (double)t / 96.0;
-            re += cos(ang);
-            im += sin(ang);
+    // R(t) = sum_{j = 1, 3, 5, 7} 2 cos(pi j t / 96) at t = u + 24 - 12 m, and
+    // b_k = exp(i pi k / 8) (refine_fine)
+    double rt[kFine * NFAST];
+    for (int u = 0; u < kFine; u++)
+        for (int m = 0; m < NFAST; m++) {
+            const double tt = (double)(u + 24 - 12 * m);
+            double v = 0;
+            for (int j = 1; j <= 7; j += 2)
+                v += 2.0 * cos(M_PI * j * tt / 96.0);
+            rt[u * NFAST + m] = v;
         }
-        tab[t + 96] = make_double2(re, im);
-    }
-    ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_dir96), tab, sizeof(tab)));
+    ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_refR), rt, sizeof(rt)));
+    double2 bt[16];
+    for (int k = 0; k < 16; k++)
+        bt[k] = make_double2(cos(M_PI * k / 8.0), sin(M_PI * k / 8.0));
+    ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_refB), bt, sizeof(bt)));
     float2 t128[128];
     for (int j = 0; j < 128; j++)
         t128[j] = make_float2((float)cos(2.0 * M_PI * j / 128.0), (float)sin(2.0 * M_PI * j / 128.0));
@@ -2378,6 +2358,16 @@ T *upload(st_ctx *ctx, const T *h, size_t n)
 
 #include "st_cpx_mfma.h"
 
+// Dynamic LDS of the fused kernel: the two transpose buffers, which later
+// hold the |C| surface (16 w^2 bytes) and then the refinement scratch.
+template <int W>
+constexpr size_t fused_lds_bytes()
+{
+    constexpr size_t t = (size_t)2 * FusedLanes<W>::BUF * sizeof(float2);
+    static_assert(t >= (size_t)16 * W * W, "surface does not fit the transpose buffers");
+    return t > (size_t)RefLayout<true>::BYTES ? t : (size_t)RefLayout<true>::BYTES;
+}
+
 template <int W, int STYPE, int BASE, int OCC>
 void launch_fused_kernel_o(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *dStart, int njobs,
                            int total, const CpxBatch *dC, size_t lds)
@@ -2390,10 +2380,9 @@ void launch_fused_kernel_o(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS,
     if (tune_find(ctx->device, W, &te) && te.wgPerCu > 0 && te.wgPerCu < perCU)
         perCU = te.wgPerCu;
     const int grid = std::max(1, std::min(total, std::max(1, perCU) * ncu));
-    float *mag = ctx->arena.take<float>((size_t)grid * 4 * W * W);
     ctx->time_begin(slot);
     hipLaunchKernelGGL((k_cpx_fused<W, STYPE, BASE, OCC>), dim3(grid), dim3(NTH), lds, ctx->stream, dP, dS, dStart,
-                       njobs, dC, mag);
+                       njobs, dC);
     ctx->time_end(slot);
     ST_HIP(hipGetLastError());
 }
@@ -2417,7 +2406,7 @@ template <int W, int STYPE>
 void launch_cpx_fused_t(st_ctx *ctx, int slot, const std::vector<FusedJob> &jobs, const KPar *dP, const Pts *dS,
                         const CpxBatch *dC, const int *dStart, int total, int base)
 {
-    const size_t lds = std::max<size_t>((size_t)2 * FusedLanes<W>::BUF * sizeof(float2), 8192);
+    const size_t lds = fused_lds_bytes<W>();
     const int prepSlot = ctx->kernel_slot("k_cpx_prep");
     ctx->time_begin(prepSlot);
     for (size_t j = 0; j < jobs.size(); j++) {
@@ -2438,7 +2427,7 @@ void launch_cpx_fused_t(st_ctx *ctx, int slot, const std::vector<FusedJob> &jobs
 template <int W, int OCC>
 int fused_per_cu_o(int stype)
 {
-    const size_t lds = std::max<size_t>((size_t)2 * FusedLanes<W>::BUF * sizeof(float2), 8192);
+    const size_t lds = fused_lds_bytes<W>();
     constexpr int NTH = 2 * FusedLanes<W>::L * W;
     int perCU = 0;
     if (stype == ST_SAMPLE_C64)
@@ -2765,15 +2754,16 @@ struct StrackJob {
         perAmp = (size_t)wsA * wsR * 128 + (size_t)(wsA + wsR) * 96 + 4096;
     }
 
-    // Bytes of the fused path's per-item results (an upper bound: every
-    // point an attempt) and of the |C| scratch of its resident workgroups.
-    size_t fused_bytes(int ncu) const
+    // Bytes of the fused path's per-item results and |C| blocks (an upper
+    // bound: every point an attempt; the kernel keeps its |C| surfaces in
+    // LDS).
+    size_t fused_bytes() const
     {
         if (!fusedPath)
             return 0;
-        const size_t perPt = sizeof(PeakRes) + 2 * sizeof(float) + 1 + sizeof(int) + 2 * sizeof(double);
-        const int perCU = fused_per_cu_w(w, pr.stype);
-        return (size_t)npts * perPt + 8 * 256 + (size_t)std::min(npts, perCU * ncu) * 4 * w * w * sizeof(float);
+        const size_t perPt =
+            sizeof(PeakRes) + 2 * sizeof(float) + 1 + sizeof(int) + 2 * sizeof(double) + kNb * kNb * sizeof(float);
+        return (size_t)npts * perPt + 8 * 256;
     }
 
     // Per-point state, parameter block, uploads, geometry, the count of
@@ -3011,6 +3001,7 @@ struct StrackJob {
         C.az = ar.take<int>(nCpx);
         C.p = ar.take<double>(2 * (size_t)nCpx);
         C.list = list;
+        C.nb = ar.take<float>((size_t)nCpx * kNb * kNb);
         return FusedJob{P, S, C, nCpx};
     }
 
@@ -3018,6 +3009,10 @@ struct StrackJob {
     void fused_finish()
     {
         hipStream_t st = ctx->stream;
+        const int kRef = ctx->kernel_slot("k_cpx_refine");
+        ctx->time_begin(kRef);
+        hipLaunchKernelGGL(k_cpx_refine, dim3((nCpx + 3) / 4), dim3(256), 0, st, C, nCpx, 2 * w);
+        ctx->time_end(kRef);
         hipLaunchKernelGGL(k_lm, dim3((2 * nCpx + 127) / 128), dim3(128), 0, st, C.res, nCpx, C.sig);
         hipLaunchKernelGGL(k_cpx_finalize, dim3((nCpx + 127) / 128), dim3(128), 0, st, P, S, list, 0, nCpx, C);
         ST_HIP(hipGetLastError());
@@ -3243,8 +3238,6 @@ void run_fused_group(st_ctx *ctx, std::vector<StrackJob *> &group)
 // budget the device cannot hold is halved until it fits.
 void reserve_batch(st_ctx *ctx, std::vector<StrackJob *> &jobs)
 {
-    int ncu = 0;
-    ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     size_t budget = ctx->scratchBudget;
     const size_t kMaxBatch = 8192;  // keeps 2 x batch inside the 65535 grid-z limit of the transposes
     for (;;) {
@@ -3254,7 +3247,7 @@ void reserve_batch(st_ctx *ctx, std::vector<StrackJob *> &jobs)
             j->Bc = (int)std::max<size_t>(1, std::min<size_t>({(size_t)j->npts, budget / j->perCpx, kMaxBatch}));
             j->Ba = (int)std::max<size_t>(1, std::min<size_t>({(size_t)j->npts, budget / j->perAmp, kMaxBatch}));
             all1 = all1 && j->Bc == 1 && j->Ba == 1;
-            need += j->fixed + j->fused_bytes(ncu) + 2048;
+            need += j->fixed + j->fused_bytes() + 2048;
             stage = std::max(stage, std::max((size_t)j->Bc * j->perCpx, (size_t)j->Ba * j->perAmp));
         }
         need += stage;

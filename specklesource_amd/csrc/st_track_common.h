@@ -12,6 +12,10 @@ constexpr int NFAST = 8;   // speckleTrack.c:16
 constexpr int LA = 2;      // speckleTrack.c:18
 constexpr int NGAUSS = 18; // speckleTrack.c:29
 constexpr int kFine = 2 * NOVER * 2;  // 48: side of the searched oversampled box
+// |C| block around a coarse peak (I, J): rows and columns I - kNb / 2 ..
+// I + kNb / 2 - 1 cover the 8 x 8 refinement box and both 18-sample
+// Gaussian fits at the refined peak (within 2 samples of the coarse one)
+constexpr int kNb = 22;
 constexpr int NT = 256;
 
 struct Images {
@@ -44,9 +48,169 @@ struct PeakRes {
     double gC;        // -gauss mode: fitted amplitude, capped at 1
 };
 
-// Dirichlet kernel of the 8 -> 96 spectral zero pad (speckleTrack.c:852-863,
-// 2014-2038): D(t) = sum_{k=-4}^{3} exp(2 pi i k t / 96), t = -96 .. 96.
-extern __constant__ double2 c_dir96[193];
+// Sub-pixel refinement: overSampleC (speckleTrack.c:852-863, 2014-2038)
+// zero-pads the 8 x 8 neighbourhood P of the coarse peak to 96 x 96, so
+// the oversampled surface is Q = A P A^T with A[u][m] = D(u + 24 - 12 m)
+// on the searched 48 x 48 box (:686-700; u, v = 24 .. 71 of the 96 grid)
+// and D(t) = sum_{k=-4}^{3} exp(2 pi i k t / 96) = exp(-i pi t / 96) R(t),
+// R(t) = sum_{j=1,3,5,7} 2 cos(pi j t / 96) real.  The phases factor out of
+// |Q|: |Q[u][v]| = |sum_m R_um sum_n b_{m+n} P[m][n] R_vn| with
+// R_um = R(u + 24 - 12 m) and b_k = exp(i pi k / 8), two real-by-complex
+// products instead of complex-by-complex ones.
+// c_refR[u * 8 + m] = R_um (u < 48, m < 8); c_refB[k] = b_k (k < 15).
+extern __constant__ double c_refR[kFine * NFAST];
+extern __constant__ double2 c_refB[16];
+
+// Workgroup best / runner-up of refine_fine's per-thread results (value,
+// smaller rank first among equal values); every thread leaves with the
+// result.  sd: 2 NTH / 64 doubles, sr: NTH / 64 ints of shared memory.
+template <int NTH>
+__device__ inline void block_top2_d(double &f1, double &f2, int &fr, double *sd, int *sr)
+{
+#pragma unroll
+    for (int o = 1; o < (NTH < 64 ? NTH : 64); o <<= 1) {
+        const double b1 = __shfl_xor(f1, o), b2 = __shfl_xor(f2, o);
+        const int br = __shfl_xor(fr, o);
+        if (b1 > f1 || (b1 == f1 && br < fr)) {
+            f2 = fmax(f1, b2);
+            f1 = b1;
+            fr = br;
+        } else
+            f2 = fmax(f2, b1);
+    }
+    if constexpr (NTH > 64) {
+        constexpr int NW = NTH / 64;
+        const int t = threadIdx.x;
+        __syncthreads();
+        if ((t & 63) == 0) {
+            sd[t >> 6] = f1;
+            sd[NW + (t >> 6)] = f2;
+            sr[t >> 6] = fr;
+        }
+        __syncthreads();
+        f1 = sd[0];
+        f2 = sd[NW];
+        fr = sr[0];
+        for (int k = 1; k < NW; k++) {
+            const double b1 = sd[k], b2 = sd[NW + k];
+            const int br = sr[k];
+            if (b1 > f1 || (b1 == f1 && br < fr)) {
+                f2 = fmax(f1, b2);
+                f1 = b1;
+                fr = br;
+            } else
+                f2 = fmax(f2, b1);
+        }
+        __syncthreads();
+    }
+}
+
+// LDS scratch of refine_fine: P (64 doubles, written by the caller), b P
+// (64 double2), T = (b P) R^T (8 x 48 double2) and, with RLDS, a copy of R
+// (384 doubles; without it R is read from constant memory).
+constexpr int kRefPm = 0, kRefPc = 512;
+template <bool RLDS>
+struct RefLayout {
+    static constexpr int T = 1536, R = 1536 + 6144, BYTES = RLDS ? 1536 + 6144 + 3072 : 1536 + 6144;
+};
+
+// T = (b P) R^T from Pc = b P (8 x 8), then |Q|^2 = |R T|^2 over the box
+// in TU x TU tiles; Rt: R (48 x 8) in LDS or constant memory.  Each thread
+// leaves with the best (f1, rank fr) and runner-up f2 of its own tiles.
+template <int NTH, int TU, class Sync>
+__device__ inline void refine_core(const double2 *Pc, double2 *Tq, const double *Rt, int t, Sync sync, double &f1,
+                                   double &f2, int &fr)
+{
+    // T[m][v] = sum_n (b P)[m][n] R_vn
+#pragma unroll 1
+    for (int e = t; e < NFAST * kFine; e += NTH) {
+        const int m = e / kFine, v = e - m * kFine;
+        double re = 0, im = 0;
+#pragma unroll
+        for (int n = 0; n < NFAST; n++) {
+            const double r = Rt[v * NFAST + n];
+            const double2 pc = Pc[m * NFAST + n];
+            re = fma(pc.x, r, re);
+            im = fma(pc.y, r, im);
+        }
+        Tq[e] = make_double2(re, im);
+    }
+    sync();
+    f1 = 0.0;
+    f2 = 0.0;
+    fr = 0x7fffffff;
+    static_assert(kFine % TU == 0, "tile side");
+    constexpr int NTS = kFine / TU;
+#pragma unroll 1
+    for (int ti = t; ti < NTS * NTS; ti += NTH) {
+        const int u0 = (ti / NTS) * TU, v0 = (ti % NTS) * TU;
+        double ar[TU][TU], ai[TU][TU];
+#pragma unroll
+        for (int a = 0; a < TU; a++)
+#pragma unroll
+            for (int b = 0; b < TU; b++)
+                ar[a][b] = ai[a][b] = 0.0;
+#pragma unroll 1
+        for (int m = 0; m < NFAST; m++) {
+            double r[TU];
+            double2 tv[TU];
+#pragma unroll
+            for (int a = 0; a < TU; a++)
+                r[a] = Rt[(u0 + a) * NFAST + m];
+#pragma unroll
+            for (int b = 0; b < TU; b++)
+                tv[b] = Tq[m * kFine + v0 + b];
+#pragma unroll
+            for (int a = 0; a < TU; a++)
+#pragma unroll
+                for (int b = 0; b < TU; b++) {
+                    ar[a][b] = fma(r[a], tv[b].x, ar[a][b]);
+                    ai[a][b] = fma(r[a], tv[b].y, ai[a][b]);
+                }
+        }
+#pragma unroll
+        for (int a = 0; a < TU; a++)
+#pragma unroll
+            for (int b = 0; b < TU; b++) {
+                const double q = ar[a][b] * ar[a][b] + ai[a][b] * ai[a][b];  // |Q|^2: same order as |Q|
+                const int e = (u0 + a) * kFine + v0 + b;
+                if (q > f1 || (q == f1 && e < fr)) {
+                    f2 = fmax(f2, f1);
+                    f1 = q;
+                    fr = e;
+                } else
+                    f2 = fmax(f2, q);
+            }
+    }
+}
+
+// The whole refinement for a caller-written P at scr + kRefPm (layout
+// RefLayout<RLDS>): every thread leaves with its own best (value f1, rank
+// fr, smaller rank first among equal values) and runner-up value f2, for
+// the caller's reduction.  All NTH threads call it, t = 0 .. NTH - 1;
+// sync() is the caller's barrier.  TU x TU tiles ((48 / TU)^2 of them;
+// TU = 2 for kernels short of registers).
+template <int NTH, bool RLDS, int TU = 3, class Sync>
+__device__ inline void refine_fine(unsigned char *scr, int t, Sync sync, double &f1, double &f2, int &fr)
+{
+    using LY = RefLayout<RLDS>;
+    const double *Pm = (const double *)(scr + kRefPm);
+    double2 *Pc = (double2 *)(scr + kRefPc);
+    double2 *Tq = (double2 *)(scr + LY::T);
+    const double *Rt = c_refR;
+    for (int e = t; e < NFAST * NFAST; e += NTH) {
+        const double2 b = c_refB[e / NFAST + e % NFAST];
+        Pc[e] = make_double2(b.x * Pm[e], b.y * Pm[e]);
+    }
+    if constexpr (RLDS) {
+        double *Rl = (double *)(scr + LY::R);
+        for (int e = t; e < kFine * NFAST; e += NTH)
+            Rl[e] = c_refR[e];
+        Rt = Rl;
+    }
+    sync();
+    refine_core<NTH, TU>(Pc, Tq, Rt, t, sync, f1, f2, fr);
+}
 
 #pragma clang fp contract(off)
 // 1-D Gaussian LM fit (gaussPeak1D / mrqminMod / mrqcofMod / myGauss1D,

@@ -41,7 +41,8 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(st_track_params), sizeof(st_track_out),
          sizeof(st_track_diag), sizeof(st_cull_params), sizeof(st_synth_spec), sizeof(st_job),
          sizeof(st_job_info), sizeof(st_offsets_meta), sizeof(st_cull_meta));
-  printf("%zu %zu %zu\n", sizeof(st_cull_ls_params), sizeof(st_cull_ls_meta), sizeof(st_track_par_file));
+  printf("%zu %zu %zu %zu\n", sizeof(st_cull_ls_params), sizeof(st_cull_ls_meta), sizeof(st_track_par_file),
+         sizeof(st_track_stats));
   return 0;
 }
 '''
@@ -56,7 +57,8 @@ int main(void) {
     assert sizes == [C.sizeof(abi.TrackParams), C.sizeof(abi.TrackOut), C.sizeof(abi.TrackDiag),
                      C.sizeof(abi.CullParams), C.sizeof(abi.SynthSpec), C.sizeof(abi.Job),
                      C.sizeof(abi.JobInfo), C.sizeof(files.OffsetsMeta), C.sizeof(files.CullMeta),
-                     C.sizeof(abi.CullLSParams), C.sizeof(files.CullLSMeta), C.sizeof(files.TrackParFile)]
+                     C.sizeof(abi.CullLSParams), C.sizeof(files.CullLSMeta), C.sizeof(files.TrackParFile),
+                     C.sizeof(abi.TrackStats)]
 
 
 def test_ctx_create_without_gpu_reports_nodev():

@@ -29,7 +29,7 @@ def _run(device, par, a, b, stype=abi.ST_SAMPLE_C64):
         device.set_profiling(False)
     finally:
         device.release_pair(pid)
-    names = {st.kernelNames[i].value.decode(): st.kernelLaunches[i] for i in range(8)}
+    names = {st.kernelNames[i].value.decode(): st.kernelLaunches[i] for i in range(len(st.kernelNames))}
     return g, names
 
 

@@ -179,7 +179,7 @@ def test_parity_baseline_phase(device, oracle32, oracle64):
 
 def _kernel_names(device):
     st = device.last_stats()
-    return {st.kernelNames[i].value.decode(): st.kernelLaunches[i] for i in range(8)}
+    return {st.kernelNames[i].value.decode(): st.kernelLaunches[i] for i in range(len(st.kernelNames))}
 
 
 @pytest.mark.parametrize("w", [16, 32, 64])

@@ -71,7 +71,7 @@ def test_swap_throughput_full_scene_pair(device):
     try:
         pid = device.upload_pair_raw(raw, (na, nr), raw, (na, nr), abi.ST_SAMPLE_C64, 1)
         st = device.last_stats()
-        names = {st.kernelNames[i].value.decode(): st.kernelMs[i] for i in range(8)}
+        names = {st.kernelNames[i].value.decode(): st.kernelMs[i] for i in range(len(st.kernelNames))}
         device.release_pair(pid)
     finally:
         device.set_profiling(False)
