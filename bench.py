@@ -51,6 +51,7 @@ def flops_amp_patch(wa):
 
 
 TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "k_cpx_fused_fullscene_hbm_traffic.md")
+SERVED_TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "k_cpx_fused_served_hbm_traffic.md")
 
 
 def measured_traffic():
@@ -59,6 +60,20 @@ def measured_traffic():
     absent)."""
     try:
         with open(TRAFFIC_PROFILE) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return int(float(ln.split(":")[1]))
+    except OSError:
+        pass
+    return None
+
+
+def served_traffic():
+    """HBM bytes per fused launch (k_cpx_fused plus its k_cpx_refine) of the
+    served workload's one-context pass, from the committed FETCH_SIZE /
+    WRITE_SIZE profile (None when absent)."""
+    try:
+        with open(SERVED_TRAFFIC_PROFILE) as f:
             for ln in f:
                 if ln.startswith("traffic_bytes_per_launch:"):
                     return int(float(ln.split(":")[1]))
@@ -239,10 +254,11 @@ def served_roofline(pairs, precision):
     pids = [dev.synth_pair(s) for s, _ in pairs]
     dev.set_profiling(True)
     table, work = {}, {}
-    for (s, p), pid in zip(pairs, pids):
+    for k, ((s, p), pid) in enumerate(zip(pairs, pids)):
         q = abi.TrackParams.from_buffer_copy(p)
         q.precision = precision
         dev.speckle_track(pid, q)
+        print("roofline pass: job %d of %d done" % (k + 1, len(pairs)), file=sys.stderr, flush=True)
         st = _kernel_table(dev, table)
         names = [st.kernelNames[i].value.decode() for i in range(len(st.kernelNames))]
         kern = "k_cpx_mfma" if "k_cpx_mfma" in names else ("k_cpx_fused" if "k_cpx_fused" in names else "staged")
@@ -258,16 +274,21 @@ def served_roofline(pairs, precision):
     flops, patches, byts = work[dom]
     mfma = dom == "k_cpx_mfma"
     label = dom
+    by_kernel = {dom: table[dom][0] / table[dom][1]}
+    traffic = None
     if dom == "k_cpx_fused" and "k_cpx_refine" in table:
         # the fused path's sub-pixel refinement runs in its own kernel right
         # after it: the complex match's flops are divided by both
         ms += table["k_cpx_refine"][0]
+        by_kernel["k_cpx_refine"] = table["k_cpx_refine"][0] / table["k_cpx_refine"][1]
         label = "k_cpx_fused + k_cpx_refine"
+        traffic = served_traffic()
     achieved = flops / (ms * 1e-3) / 1e12
     peak = PEAK_BF16_TFLOPS if mfma else PEAK_FP32_TFLOPS
     return {"bound": "mfma" if mfma else "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": None, "kernel": label, "launches": launches,
-            "launch_ms_avg": ms / launches, "patches": patches,
+            "frac": achieved / peak, "traffic": traffic, "kernel": label, "launches": launches,
+            "launch_ms_avg": ms / launches, "launch_ms_avg_by_kernel": by_kernel, "patches": patches,
+            "algorithmic_bytes_per_launch": byts / launches,
             "flops_total": flops, "algorithmic_bytes_total": byts,
             "kernel_ms_per_pass": {k: v[0] for k, v in sorted(table.items(), key=lambda kv: -kv[1][0])},
             "measured": "one context, the same %d jobs in sequence, HIP events on the library stream" % len(pairs)}
@@ -435,8 +456,8 @@ def bench_served(args):
     sides drawn from 1024-8192 px, windows from {16, 32, 64, 128} on a w/2
     grid, affine offsets within +/-20 px, resident in HBM; one step submits
     the 64 jobs to a 4-slot server and waits for all of them.  Both the FP32
-    path and the split-bf16 matrix-core path are timed; `value` is the
-    FP32 path (the faster one)."""
+    path and the split-bf16 matrix-core path are timed; `value` is the run
+    with each window's tuned precision (ST_PREC_AUTO)."""
     from specklesource_amd import abi, configs, track
     pairs = configs.cfg3_pairs(n_pairs=args.pairs, scale=args.scale)
     srv = track.Server(0, slots=args.slots, hbm_budget=200 << 30)
@@ -580,6 +601,8 @@ def main():
     ap.add_argument("--workload", choices=["served", "fullscene", "amponly", "strackw", "cull", "batched"],
                     default="served")
     ap.add_argument("--no-roofline", action="store_true", help="served workload: skip the one-context roofline pass")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="served workload: only the one-context roofline pass at --precision (for profilers)")
     ap.add_argument("--batch", type=int, default=1, help="served workload: strack jobs per fused batch (1: off)")
     ap.add_argument("--precision", choices=["fp32", "mfma"], default="fp32")
     ap.add_argument("--pairs", type=int, default=64)
@@ -609,7 +632,13 @@ def main():
         bench_batched(args)
         return
     if args.workload == "served":
-        bench_served(args)
+        if args.roofline_only:
+            from specklesource_amd import abi, configs
+            pairs = configs.cfg3_pairs(n_pairs=args.pairs, scale=args.scale)
+            prec = abi.ST_PREC_MFMA if args.precision == "mfma" else abi.ST_PREC_FP32
+            print(json.dumps({"roofline": served_roofline(pairs, prec)}))
+        else:
+            bench_served(args)
         return
 
     from specklesource_amd import abi, configs, track
