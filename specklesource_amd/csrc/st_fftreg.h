@@ -13,11 +13,11 @@
 
 namespace fftreg {
 
-// exp(-2 pi i j / 64), j = 0..63: (float)cos and (float)sin of the double
-// angle -2 pi j / 64, written as literals so the compiler folds them into
+// exp(-2 pi i j / 128), j = 0..127: (float)cos and (float)sin of the double
+// angle -2 pi j / 128, written as literals so the compiler folds them into
 // instructions instead of holding them in SGPRs.
-constexpr float kTw64Re[64] = {0x1p+0f, 0x1.fd88dap-1f, 0x1.f6297cp-1f, 0x1.e9f416p-1f, 0x1.d906bcp-1f, 0x1.c38b3p-1f, 0x1.a9b662p-1f, 0x1.8bc806p-1f, 0x1.6a09e6p-1f, 0x1.44cf32p-1f, 0x1.1c73b4p-1f, 0x1.e2b5d4p-2f, 0x1.87de2ap-2f, 0x1.294062p-2f, 0x1.8f8b84p-3f, 0x1.917a6cp-4f, 0x1.1a6264p-54f, -0x1.917a6cp-4f, -0x1.8f8b84p-3f, -0x1.294062p-2f, -0x1.87de2ap-2f, -0x1.e2b5d4p-2f, -0x1.1c73b4p-1f, -0x1.44cf32p-1f, -0x1.6a09e6p-1f, -0x1.8bc806p-1f, -0x1.a9b662p-1f, -0x1.c38b3p-1f, -0x1.d906bcp-1f, -0x1.e9f416p-1f, -0x1.f6297cp-1f, -0x1.fd88dap-1f, -0x1p+0f, -0x1.fd88dap-1f, -0x1.f6297cp-1f, -0x1.e9f416p-1f, -0x1.d906bcp-1f, -0x1.c38b3p-1f, -0x1.a9b662p-1f, -0x1.8bc806p-1f, -0x1.6a09e6p-1f, -0x1.44cf32p-1f, -0x1.1c73b4p-1f, -0x1.e2b5d4p-2f, -0x1.87de2ap-2f, -0x1.294062p-2f, -0x1.8f8b84p-3f, -0x1.917a6cp-4f, -0x1.a79394p-53f, 0x1.917a6cp-4f, 0x1.8f8b84p-3f, 0x1.294062p-2f, 0x1.87de2ap-2f, 0x1.e2b5d4p-2f, 0x1.1c73b4p-1f, 0x1.44cf32p-1f, 0x1.6a09e6p-1f, 0x1.8bc806p-1f, 0x1.a9b662p-1f, 0x1.c38b3p-1f, 0x1.d906bcp-1f, 0x1.e9f416p-1f, 0x1.f6297cp-1f, 0x1.fd88dap-1f};
-constexpr float kTw64Im[64] = {-0x0p+0f, -0x1.917a6cp-4f, -0x1.8f8b84p-3f, -0x1.294062p-2f, -0x1.87de2ap-2f, -0x1.e2b5d4p-2f, -0x1.1c73b4p-1f, -0x1.44cf32p-1f, -0x1.6a09e6p-1f, -0x1.8bc806p-1f, -0x1.a9b662p-1f, -0x1.c38b3p-1f, -0x1.d906bcp-1f, -0x1.e9f416p-1f, -0x1.f6297cp-1f, -0x1.fd88dap-1f, -0x1p+0f, -0x1.fd88dap-1f, -0x1.f6297cp-1f, -0x1.e9f416p-1f, -0x1.d906bcp-1f, -0x1.c38b3p-1f, -0x1.a9b662p-1f, -0x1.8bc806p-1f, -0x1.6a09e6p-1f, -0x1.44cf32p-1f, -0x1.1c73b4p-1f, -0x1.e2b5d4p-2f, -0x1.87de2ap-2f, -0x1.294062p-2f, -0x1.8f8b84p-3f, -0x1.917a6cp-4f, -0x1.1a6264p-53f, 0x1.917a6cp-4f, 0x1.8f8b84p-3f, 0x1.294062p-2f, 0x1.87de2ap-2f, 0x1.e2b5d4p-2f, 0x1.1c73b4p-1f, 0x1.44cf32p-1f, 0x1.6a09e6p-1f, 0x1.8bc806p-1f, 0x1.a9b662p-1f, 0x1.c38b3p-1f, 0x1.d906bcp-1f, 0x1.e9f416p-1f, 0x1.f6297cp-1f, 0x1.fd88dap-1f, 0x1p+0f, 0x1.fd88dap-1f, 0x1.f6297cp-1f, 0x1.e9f416p-1f, 0x1.d906bcp-1f, 0x1.c38b3p-1f, 0x1.a9b662p-1f, 0x1.8bc806p-1f, 0x1.6a09e6p-1f, 0x1.44cf32p-1f, 0x1.1c73b4p-1f, 0x1.e2b5d4p-2f, 0x1.87de2ap-2f, 0x1.294062p-2f, 0x1.8f8b84p-3f, 0x1.917a6cp-4f};
+constexpr float kTw128Re[128] = {0x1p+0f, 0x1.ff621ep-1f, 0x1.fd88dap-1f, 0x1.fa7558p-1f, 0x1.f6297cp-1f, 0x1.f0a7fp-1f, 0x1.e9f416p-1f, 0x1.e2121p-1f, 0x1.d906bcp-1f, 0x1.ced7bp-1f, 0x1.c38b3p-1f, 0x1.b72834p-1f, 0x1.a9b662p-1f, 0x1.9b3e04p-1f, 0x1.8bc806p-1f, 0x1.7b5df2p-1f, 0x1.6a09e6p-1f, 0x1.57d694p-1f, 0x1.44cf32p-1f, 0x1.30ff8p-1f, 0x1.1c73b4p-1f, 0x1.07387ap-1f, 0x1.e2b5d4p-2f, 0x1.b5d1p-2f, 0x1.87de2ap-2f, 0x1.58f9a8p-2f, 0x1.294062p-2f, 0x1.f19f98p-3f, 0x1.8f8b84p-3f, 0x1.2c8106p-3f, 0x1.917a6cp-4f, 0x1.91f66p-5f, 0x1.1a6264p-54f, -0x1.91f66p-5f, -0x1.917a6cp-4f, -0x1.2c8106p-3f, -0x1.8f8b84p-3f, -0x1.f19f98p-3f, -0x1.294062p-2f, -0x1.58f9a8p-2f, -0x1.87de2ap-2f, -0x1.b5d1p-2f, -0x1.e2b5d4p-2f, -0x1.07387ap-1f, -0x1.1c73b4p-1f, -0x1.30ff8p-1f, -0x1.44cf32p-1f, -0x1.57d694p-1f, -0x1.6a09e6p-1f, -0x1.7b5df2p-1f, -0x1.8bc806p-1f, -0x1.9b3e04p-1f, -0x1.a9b662p-1f, -0x1.b72834p-1f, -0x1.c38b3p-1f, -0x1.ced7bp-1f, -0x1.d906bcp-1f, -0x1.e2121p-1f, -0x1.e9f416p-1f, -0x1.f0a7fp-1f, -0x1.f6297cp-1f, -0x1.fa7558p-1f, -0x1.fd88dap-1f, -0x1.ff621ep-1f, -0x1p+0f, -0x1.ff621ep-1f, -0x1.fd88dap-1f, -0x1.fa7558p-1f, -0x1.f6297cp-1f, -0x1.f0a7fp-1f, -0x1.e9f416p-1f, -0x1.e2121p-1f, -0x1.d906bcp-1f, -0x1.ced7bp-1f, -0x1.c38b3p-1f, -0x1.b72834p-1f, -0x1.a9b662p-1f, -0x1.9b3e04p-1f, -0x1.8bc806p-1f, -0x1.7b5df2p-1f, -0x1.6a09e6p-1f, -0x1.57d694p-1f, -0x1.44cf32p-1f, -0x1.30ff8p-1f, -0x1.1c73b4p-1f, -0x1.07387ap-1f, -0x1.e2b5d4p-2f, -0x1.b5d1p-2f, -0x1.87de2ap-2f, -0x1.58f9a8p-2f, -0x1.294062p-2f, -0x1.f19f98p-3f, -0x1.8f8b84p-3f, -0x1.2c8106p-3f, -0x1.917a6cp-4f, -0x1.91f66p-5f, -0x1.a79394p-53f, 0x1.91f66p-5f, 0x1.917a6cp-4f, 0x1.2c8106p-3f, 0x1.8f8b84p-3f, 0x1.f19f98p-3f, 0x1.294062p-2f, 0x1.58f9a8p-2f, 0x1.87de2ap-2f, 0x1.b5d1p-2f, 0x1.e2b5d4p-2f, 0x1.07387ap-1f, 0x1.1c73b4p-1f, 0x1.30ff8p-1f, 0x1.44cf32p-1f, 0x1.57d694p-1f, 0x1.6a09e6p-1f, 0x1.7b5df2p-1f, 0x1.8bc806p-1f, 0x1.9b3e04p-1f, 0x1.a9b662p-1f, 0x1.b72834p-1f, 0x1.c38b3p-1f, 0x1.ced7bp-1f, 0x1.d906bcp-1f, 0x1.e2121p-1f, 0x1.e9f416p-1f, 0x1.f0a7fp-1f, 0x1.f6297cp-1f, 0x1.fa7558p-1f, 0x1.fd88dap-1f, 0x1.ff621ep-1f};
+constexpr float kTw128Im[128] = {-0x0p+0f, -0x1.91f66p-5f, -0x1.917a6cp-4f, -0x1.2c8106p-3f, -0x1.8f8b84p-3f, -0x1.f19f98p-3f, -0x1.294062p-2f, -0x1.58f9a8p-2f, -0x1.87de2ap-2f, -0x1.b5d1p-2f, -0x1.e2b5d4p-2f, -0x1.07387ap-1f, -0x1.1c73b4p-1f, -0x1.30ff8p-1f, -0x1.44cf32p-1f, -0x1.57d694p-1f, -0x1.6a09e6p-1f, -0x1.7b5df2p-1f, -0x1.8bc806p-1f, -0x1.9b3e04p-1f, -0x1.a9b662p-1f, -0x1.b72834p-1f, -0x1.c38b3p-1f, -0x1.ced7bp-1f, -0x1.d906bcp-1f, -0x1.e2121p-1f, -0x1.e9f416p-1f, -0x1.f0a7fp-1f, -0x1.f6297cp-1f, -0x1.fa7558p-1f, -0x1.fd88dap-1f, -0x1.ff621ep-1f, -0x1p+0f, -0x1.ff621ep-1f, -0x1.fd88dap-1f, -0x1.fa7558p-1f, -0x1.f6297cp-1f, -0x1.f0a7fp-1f, -0x1.e9f416p-1f, -0x1.e2121p-1f, -0x1.d906bcp-1f, -0x1.ced7bp-1f, -0x1.c38b3p-1f, -0x1.b72834p-1f, -0x1.a9b662p-1f, -0x1.9b3e04p-1f, -0x1.8bc806p-1f, -0x1.7b5df2p-1f, -0x1.6a09e6p-1f, -0x1.57d694p-1f, -0x1.44cf32p-1f, -0x1.30ff8p-1f, -0x1.1c73b4p-1f, -0x1.07387ap-1f, -0x1.e2b5d4p-2f, -0x1.b5d1p-2f, -0x1.87de2ap-2f, -0x1.58f9a8p-2f, -0x1.294062p-2f, -0x1.f19f98p-3f, -0x1.8f8b84p-3f, -0x1.2c8106p-3f, -0x1.917a6cp-4f, -0x1.91f66p-5f, -0x1.1a6264p-53f, 0x1.91f66p-5f, 0x1.917a6cp-4f, 0x1.2c8106p-3f, 0x1.8f8b84p-3f, 0x1.f19f98p-3f, 0x1.294062p-2f, 0x1.58f9a8p-2f, 0x1.87de2ap-2f, 0x1.b5d1p-2f, 0x1.e2b5d4p-2f, 0x1.07387ap-1f, 0x1.1c73b4p-1f, 0x1.30ff8p-1f, 0x1.44cf32p-1f, 0x1.57d694p-1f, 0x1.6a09e6p-1f, 0x1.7b5df2p-1f, 0x1.8bc806p-1f, 0x1.9b3e04p-1f, 0x1.a9b662p-1f, 0x1.b72834p-1f, 0x1.c38b3p-1f, 0x1.ced7bp-1f, 0x1.d906bcp-1f, 0x1.e2121p-1f, 0x1.e9f416p-1f, 0x1.f0a7fp-1f, 0x1.f6297cp-1f, 0x1.fa7558p-1f, 0x1.fd88dap-1f, 0x1.ff621ep-1f, 0x1p+0f, 0x1.ff621ep-1f, 0x1.fd88dap-1f, 0x1.fa7558p-1f, 0x1.f6297cp-1f, 0x1.f0a7fp-1f, 0x1.e9f416p-1f, 0x1.e2121p-1f, 0x1.d906bcp-1f, 0x1.ced7bp-1f, 0x1.c38b3p-1f, 0x1.b72834p-1f, 0x1.a9b662p-1f, 0x1.9b3e04p-1f, 0x1.8bc806p-1f, 0x1.7b5df2p-1f, 0x1.6a09e6p-1f, 0x1.57d694p-1f, 0x1.44cf32p-1f, 0x1.30ff8p-1f, 0x1.1c73b4p-1f, 0x1.07387ap-1f, 0x1.e2b5d4p-2f, 0x1.b5d1p-2f, 0x1.87de2ap-2f, 0x1.58f9a8p-2f, 0x1.294062p-2f, 0x1.f19f98p-3f, 0x1.8f8b84p-3f, 0x1.2c8106p-3f, 0x1.917a6cp-4f, 0x1.91f66p-5f};
 // exp(+2 pi i j / 128), j = 0..127
 extern __constant__ float2 c_tw128[128];
 
@@ -26,11 +26,11 @@ __device__ inline float2 cmul_f(float2 a, float2 b)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// exp(SIGN 2 pi i k / len) for len dividing 64
+// exp(SIGN 2 pi i k / len) for len dividing 128
 template <int SIGN>
 __device__ inline float2 tw(int k, int len)
 {
-    float2 w = make_float2(kTw64Re[k * (64 / len)], kTw64Im[k * (64 / len)]);
+    float2 w = make_float2(kTw128Re[k * (128 / len)], kTw128Im[k * (128 / len)]);
     if (SIGN > 0)
         w.y = -w.y;
     return w;
@@ -142,6 +142,103 @@ __device__ inline void inv_pair(float2 (&x)[W / 2], int h)
         x[n] = h ? b : a;
     }
     fft<H, +1>(x);
+}
+
+// ---- 128 points over a lane quad (lanes 4i .. 4i + 3, h = lane & 3), 32
+// values per lane.  Forward: lane h holds samples 4m + h; each lane runs a
+// 32-point transform, lane pairs (h, h ^ 2) combine them into the 64-point
+// transforms of the even and odd samples, and pairs (h, h ^ 1) into the
+// 128-point one; lane h leaves with outputs qblk(h) * 32 + k.  Inverse:
+// lane h holds inputs qblk(h) * 32 + n, the same two exchanges in the
+// opposite order (decimation in frequency), then the 32-point transforms;
+// lane h leaves with outputs 4j + h.
+// exp(+2 pi i j / 256), j = 0..31 (the polyphase factors at w = 128)
+constexpr float kTw256Re[32] = {0x1p+0f, 0x1.ffd886p-1f, 0x1.ff621ep-1f, 0x1.fe9cdap-1f, 0x1.fd88dap-1f, 0x1.fc2648p-1f, 0x1.fa7558p-1f, 0x1.f8765p-1f, 0x1.f6297cp-1f, 0x1.f38f3ap-1f, 0x1.f0a7fp-1f, 0x1.ed740ep-1f, 0x1.e9f416p-1f, 0x1.e6288ep-1f, 0x1.e2121p-1f, 0x1.ddb13cp-1f, 0x1.d906bcp-1f, 0x1.d4134ep-1f, 0x1.ced7bp-1f, 0x1.c954b2p-1f, 0x1.c38b3p-1f, 0x1.bd7c0ap-1f, 0x1.b72834p-1f, 0x1.b090a6p-1f, 0x1.a9b662p-1f, 0x1.a29a7ap-1f, 0x1.9b3e04p-1f, 0x1.93a224p-1f, 0x1.8bc806p-1f, 0x1.83b0ep-1f, 0x1.7b5df2p-1f, 0x1.72d084p-1f};
+constexpr float kTw256Im[32] = {0x0p+0f, 0x1.92156p-6f, 0x1.91f66p-5f, 0x1.2d520ap-4f, 0x1.917a6cp-4f, 0x1.f564e6p-4f, 0x1.2c8106p-3f, 0x1.5e2144p-3f, 0x1.8f8b84p-3f, 0x1.c0b826p-3f, 0x1.f19f98p-3f, 0x1.111d26p-2f, 0x1.294062p-2f, 0x1.4135cap-2f, 0x1.58f9a8p-2f, 0x1.708854p-2f, 0x1.87de2ap-2f, 0x1.9ef794p-2f, 0x1.b5d1p-2f, 0x1.cc66eap-2f, 0x1.e2b5d4p-2f, 0x1.f8ba4ep-2f, 0x1.07387ap-1f, 0x1.11eb36p-1f, 0x1.1c73b4p-1f, 0x1.26d054p-1f, 0x1.30ff8p-1f, 0x1.3affa2p-1f, 0x1.44cf32p-1f, 0x1.4e6cacp-1f, 0x1.57d694p-1f, 0x1.610b76p-1f};
+
+__device__ inline int qblk(int h) { return ((h & 1) << 1) | (h >> 1); }
+
+// Value of lane ^ 2 via DPP quad_perm [2,3,0,1].
+__device__ inline float dpp_swap2(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+
+template <int XOR>
+__device__ inline float2 shfl_quad(float2 v)
+{
+    if constexpr (XOR == 1)
+        return make_float2(dpp_swap1(v.x), dpp_swap1(v.y));
+    else
+        return make_float2(dpp_swap2(v.x), dpp_swap2(v.y));
+}
+
+// exp(SIGN 2 pi i k / len) with the exact values at k = 0 and k = len / 4.
+template <int SIGN>
+__device__ inline float2 twx(int k, int len)
+{
+    if (k == 0)
+        return make_float2(1.f, 0.f);
+    if (4 * k == len)
+        return make_float2(0.f, SIGN < 0 ? -1.f : 1.f);
+    return tw<SIGN>(k, len);
+}
+
+// One radix-2 combine between lane h and lane h ^ XOR: role 0 holds the
+// even-sample transform E, role 1 the odd-sample one O, with twiddle
+// exp(SIGN 2 pi i (k + off) / len) where off = len / 4 on lanes with `rot`
+// set (applied as an exact multiplication by SIGN i, so every twiddle stays
+// a compile-time operand instead of a per-lane register).  Role 0 leaves
+// with E + w O, role 1 with E - w O.
+template <int XOR, int SIGN, int LEN>
+__device__ inline void quad_fwd_stage(float2 (&x)[32], int role, int rot)
+{
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+        float2 m = cmul_f(x[k], twx<SIGN>(k, LEN));
+        if (rot)  // times exp(SIGN i pi / 2) = SIGN i
+            m = SIGN < 0 ? make_float2(m.y, -m.x) : make_float2(-m.y, m.x);
+        const float2 t = role ? m : x[k];
+        const float2 y = shfl_quad<XOR>(t);
+        x[k] = role ? make_float2(y.x - t.x, y.y - t.y) : make_float2(y.x + t.x, y.y + t.y);
+    }
+}
+
+// Inverse combine: role 0 holds the first half a, role 1 the second half
+// b (same index n); role 0 leaves with a + b, role 1 with (a - b) w (w as
+// in quad_fwd_stage).
+template <int XOR, int SIGN, int LEN>
+__device__ inline void quad_inv_stage(float2 (&x)[32], int role, int rot)
+{
+#pragma unroll
+    for (int n = 0; n < 32; n++) {
+        const float2 y = shfl_quad<XOR>(x[n]);
+        const float2 a = make_float2(x[n].x + y.x, x[n].y + y.y);
+        const float2 d = make_float2(y.x - x[n].x, y.y - x[n].y);
+        float2 m = cmul_f(d, twx<SIGN>(n, LEN));
+        if (rot)
+            m = SIGN < 0 ? make_float2(m.y, -m.x) : make_float2(-m.y, m.x);
+        x[n] = role ? m : a;
+    }
+}
+
+__device__ inline void fwd_quad128(float2 (&x)[32], int h)
+{
+    fft<32, -1>(x);
+    // (h, h ^ 2): 64-point transforms of the even (h = 0, 2) and odd (1, 3) samples
+    quad_fwd_stage<2, -1, 64>(x, (h >> 1) & 1, 0);
+    // (h, h ^ 1): lanes 0 / 1 hold outputs 0..31 of E / O, lanes 2 / 3
+    // outputs 32..63 (twiddle exp(-2 pi i (32 + k) / 128))
+    quad_fwd_stage<1, -1, 128>(x, h & 1, (h >> 1) & 1);
+}
+
+__device__ inline void inv_quad128(float2 (&x)[32], int h)
+{
+    // (h, h ^ 1): inputs n and n + 64 (blocks 0 / 2 on lanes 0 / 1, 1 / 3 on 2 / 3)
+    quad_inv_stage<1, +1, 128>(x, h & 1, (h >> 1) & 1);
+    // (h, h ^ 2): within each half, inputs n and n + 32
+    quad_inv_stage<2, +1, 64>(x, (h >> 1) & 1, 0);
+    fft<32, +1>(x);
 }
 
 // Lane-group FFTs (lane pairs): the block of outputs (forward) / inputs

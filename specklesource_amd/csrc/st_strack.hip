@@ -956,9 +956,9 @@ __global__ void k_compose(KPar P, Pts S, float *offR, float *offA, int npts)
 //   * the zero-padded 2w x 2w inverse is done as its four polyphase
 //     components C[2n+p][2m+q] = IFFT_w(X[(k+az) mod w][l] exp(i pi (k p +
 //     l q) / w)) (k, l signed), so no padded array is ever formed;
-//   * the fftshifted magnitude surface goes to a per-workgroup scratch
-//     slot (L2 resident) for the argmax neighbourhood and the Gaussian-fit
-//     samples.
+//   * the fftshifted magnitude surface stays in LDS for the argmax; the
+//     22 x 22 block around the coarse peak goes to k_cpx_refine, which
+//     refines it and takes the Gaussian-fit samples.
 // Thread t: group g = t / 2w (patch g in the forward part, column parity q
 // = g in the inverse part), index r = (t / 2) mod w, lane half h = t & 1.
 struct Top2 {
@@ -1013,9 +1013,10 @@ __device__ inline Top2 wg_top2(Top2 a, Top2 *sh)
 }
 
 // Per-patch statistics of getPatches (speckleTrack.c:1194-1330) with one
-// lane per window column: 64 / W patch pairs per wave, each lane walks its
-// column down both windows (coalesced row reads), so every sum is a lane
-// sum followed by a segmented wave reduction -- no LDS, no barriers.  The
+// lane per window column (two at w = 128): 64 / W patch pairs per wave (one
+// at w = 128), each lane walks its columns down both windows (coalesced row
+// reads), so every sum is a lane sum followed by a segmented wave
+// reduction -- no LDS, no barriers.  The
 // Doppler lag-1 sum of estDopCarrier1 (dopplerEstimation.c:42-88) is
 // expanded around the patch mean:
 //   sum (x1 - m) conj(x0 - m) = sum x1 conj(x0) - conj(m) sum x1
@@ -1028,7 +1029,8 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
     const KPar &P = *Pp;
     const Pts &S = *Sp;
     const CpxBatch &C = *Cp;
-    constexpr int PPW = 64 / W;
+    // lanes per patch, patches per wave, columns per lane
+    constexpr int LPP = W < 64 ? W : 64, PPW = 64 / LPP, CPL = W / LPP;
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
     const float thr = isf ? 1e-15f : 1e-9f;
     const float thrAll = thr * W * W;
@@ -1037,7 +1039,7 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
     // rows hit that XCD's L2: workgroups are dealt round-robin over the 8 XCDs
     const int nb = gridDim.x, per = nb / 8, b = blockIdx.x;
     const int blk = b < per * 8 ? (b % 8) * per + b / 8 : b;
-    const int it = blk * PPW + lane / W, c = lane % W;
+    const int it = blk * PPW + lane / LPP, c0 = lane % LPP;
     const bool act = it < count;
     double ps[2] = {0, 0}, T[2][2] = {{0, 0}, {0, 0}}, Lg[2][2] = {{0, 0}, {0, 0}};
     double F0[2][2] = {{0, 0}, {0, 0}}, FL[2][2] = {{0, 0}, {0, 0}};
@@ -1046,6 +1048,9 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
         const int pt = list[it];
         const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
         const bool useH = P.hanning != 0;
+#pragma unroll 1
+        for (int cc = 0; cc < CPL; cc++) {
+        const int c = c0 + cc * LPP;
         float2 prev[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
 #pragma unroll 4
         for (int a = 0; a < W; a++) {
@@ -1075,8 +1080,8 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
                 T[g][0] += x.x;
                 T[g][1] += x.y;
                 if (a == 0) {
-                    F0[g][0] = x.x;
-                    F0[g][1] = x.y;
+                    F0[g][0] += x.x;
+                    F0[g][1] += x.y;
                 } else {
                     // x conj(prev): re = x.re p.re + x.im p.im, im = x.im p.re - x.re p.im.
                     // A product of two floats is exact in double, so each fma
@@ -1085,17 +1090,18 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
                     Lg[g][1] = fma(-(double)x.x, (double)prev[g].y, fma((double)x.y, (double)prev[g].x, Lg[g][1]));
                 }
                 if (a == W - 1) {
-                    FL[g][0] = x.x;
-                    FL[g][1] = x.y;
+                    FL[g][0] += x.x;
+                    FL[g][1] += x.y;
                 }
                 prev[g] = x;
             }
             nz += (pw[0] < thr || pw[1] < thr) ? 1 : 0;
         }
+        }
     }
-    // segmented reduction over the W lanes of each patch pair
+    // segmented reduction over the LPP lanes of each patch pair
 #pragma unroll
-    for (int o = 1; o < W; o <<= 1) {
+    for (int o = 1; o < LPP; o <<= 1) {
         for (int g = 0; g < 2; g++) {
             ps[g] += __shfl_xor(ps[g], o);
             for (int k = 0; k < 2; k++) {
@@ -1107,7 +1113,7 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
         }
         nz += __shfl_xor(nz, o);
     }
-    if (!act || c != 0)
+    if (!act || c0 != 0)
         return;
     if (nz > 20 || ps[0] < thrAll || ps[1] < thrAll) {
         C.have[it] = 0;
@@ -2211,6 +2217,243 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
 }
 
 // ---------------------------------------------------------------------
+// Fused complex match at w = 128: one workgroup of 512 threads per patch
+// pair (thread t: index r = t / 4, lane quarter h = t & 3), persistent over
+// the work list.  A 128 x 128 complex array fills most of a CU's LDS, so
+// the two windows go through one transpose buffer in turn: window 1's
+// spectrum waits in a per-workgroup global slot (cache resident) and is
+// multiplied into window 2's spectrum row by row; the cross spectrum goes
+// back to the slot, which both column phases of the polyphase inverse read;
+// and the 256 x 256 |C| surface goes to a second slot, from which the
+// block around the coarse peak is taken for k_cpx_refine.  Otherwise as
+// k_cpx_fused, with each 128-point transform on a lane quad
+// (fwd_quad128 / inv_quad128): the components C[2n+p][2m+q], one row
+// transform per column phase, findCPeak's rank order.
+constexpr int kF128 = 128;
+constexpr size_t kF128SlotX = (size_t)kF128 * kF128;      // float2 per workgroup: F1, then X
+constexpr size_t kF128SlotM = (size_t)4 * kF128 * kF128;  // float per workgroup: |C|
+
+// exp(i pi (blk 32 + k - (blk >= 2 ? 128 : 0)) / 128) = s_blk exp(i pi k / 128):
+// the polyphase factor of the signed frequency of element k of block blk
+__device__ inline float2 f128_factor(float2 sblk, int k)
+{
+    return ::fftreg::cmul_f(sblk, make_float2(::fftreg::kTw256Re[k], ::fftreg::kTw256Im[k]));
+}
+
+template <int STYPE>
+__global__ void __launch_bounds__(512) k_cpx_fused128(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
+                                                      const int *__restrict__ jobStart, int njobs,
+                                                      const CpxBatch *__restrict__ Cp, float2 *slotX, float *slotM)
+{
+    constexpr int W = kF128, L = 4, H = W / L, NTH = 512, WW = 2 * W;
+    using FL = FusedLanes<W>;
+    extern __shared__ float2 lds[];  // [BUF]: the transpose buffer
+    __shared__ Top2 sht[NTH / 64];
+    constexpr bool isf = STYPE == ST_SAMPLE_C64;
+    const int count = jobStart[njobs];
+    // items in XCD-contiguous order (as k_cpx_fused)
+    const int G = gridDim.x, Q = G / 8;
+    const int base = (G % 8 == 0) ? (blockIdx.x % 8) * Q + blockIdx.x / 8 : blockIdx.x;
+    float2 *Xs = slotX + (size_t)blockIdx.x * kF128SlotX;
+    float *M = slotM + (size_t)blockIdx.x * kF128SlotM;
+    for (int k0 = 0; k0 < count; k0 += G) {
+        const int itg = k0 + base;
+        if (itg >= count)
+            break;
+        int jb = 0;
+        for (int lo = 0, hi = njobs - 1; lo < hi;) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobStart[mid] <= itg)
+                lo = jb = mid;
+            else
+                hi = mid - 1;
+        }
+        const int it = itg - jobStart[jb];
+        const KPar &P = Pp[jb];
+        const Pts &S = Sp[jb];
+        const CpxBatch &C = Cp[jb];
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        const int r = t >> 2;
+        const int pt = C.list[it];
+        if (!C.have[it])
+            continue;
+        const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
+        const int az = C.az[it];
+        // ---------------- forward 2-D transforms, window 1 then window 2
+#pragma unroll 1
+        for (int g = 0; g < 2; g++) {
+            // per-lane values behind an opaque copy in every loop body, so the
+            // compiler does not hoist the lane-selected twiddles and factors
+            // out of the loops (they would hold registers throughout)
+            int h = t & 3;
+            asm volatile("" : "+v"(h));
+            const int blk = ::fftreg::qblk(h);
+            float2 x[H];
+            const void *img = g ? P.im.img2 : P.im.img1;
+            const int nr = g ? P.im.nr2 : P.im.nr1;
+            const int ra = (g ? r2 : r1) + r, aa = (g ? a2 : a1) + h;
+            const bool useH = (g == 1) && P.hanning;
+#pragma unroll
+            for (int m = 0; m < H; m++) {
+                const size_t o = (size_t)(aa + L * m) * nr + ra;
+                float2 v;
+                if constexpr (isf) {
+                    v = ((const float2 *)img)[o];
+                } else {
+                    const short2 q = ((const short2 *)img)[o];
+                    v = make_float2((float)q.x, (float)q.y);
+                }
+                x[m] = v;
+            }
+#pragma unroll
+            for (int m = 0; m < H; m++) {
+#pragma clang fp contract(off)
+                float2 v = x[m];
+                if constexpr (isf) {
+                    v.x *= 1000;
+                    v.y *= 1000;
+                }
+                if (useH) {
+                    const float hw = P.hann[(L * m + h) * W + r];
+                    v.x *= hw;
+                    v.y *= hw;
+                }
+                x[m] = v;
+            }
+            ::fftreg::fwd_quad128(x, h);  // column r: outputs blk H + k
+#pragma unroll
+            for (int k = 0; k < H; k++)
+                lds[FL::at(blk * H + k, r)] = x[k];
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < H; m++)
+                x[m] = lds[FL::at(r, L * m + h)];
+            __syncthreads();
+            ::fftreg::fwd_quad128(x, h);  // row r: x[k] = F_g[r][blk H + k]
+            // slot element k of thread t at Xs[k * NTH + t] (coalesced)
+            if (g == 0) {
+#pragma unroll
+                for (int k = 0; k < H; k++)
+                    Xs[k * NTH + t] = x[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < H; k++) {
+#pragma clang fp contract(off)
+                    const float2 f1 = Xs[k * NTH + t];
+                    Xs[k * NTH + t] = cmulconj(f1, x[k]);  // X = F1 conj(F2)
+                }
+            }
+        }
+        // ---------------- polyphase inverse, one column phase q at a time
+        const int krow = ((r - az) % W + W) % W;
+        // s_blk = exp(i pi (blk 32 - (blk >= 2 ? 128 : 0)) / 128): 1, exp(i pi / 4), -i, -exp(i 3 pi / 4)
+        auto sblk_of = [](int b) {
+            const float c45 = 0x1.6a09e6p-1f;
+            return b == 0 ? make_float2(1.f, 0.f)
+                   : b == 1 ? make_float2(c45, c45) : b == 2 ? make_float2(0.f, -1.f) : make_float2(c45, -c45);
+        };
+        const int Jq = (2 * r + W) & (WW - 1);  // surface column of q = 0 (q = 1: + 1)
+        Top2 best{0.0f, 0x7fffffff, 0.0f, 0.0f};
+#pragma unroll 1
+        for (int q = 0; q < 2; q++) {
+            {
+                int h = t & 3;
+                asm volatile("" : "+v"(h));
+                const int blk = ::fftreg::qblk(h);
+                const float2 sblk = sblk_of(blk);
+                float2 x[H];
+#pragma unroll
+                for (int k = 0; k < H; k++) {
+                    x[k] = Xs[k * NTH + t];
+                    if (q)  // column l = blk 32 + k: exp(i pi ls / 128)
+                        x[k] = ::fftreg::cmul_f(x[k], f128_factor(sblk, k));
+                }
+                ::fftreg::inv_quad128(x, h);  // x[n'] = Z[krow][L n' + h]
+#pragma unroll
+                for (int n = 0; n < H; n++)
+                    lds[FL::at(krow, L * n + h)] = x[n];
+            }
+            __syncthreads();
+            const int J = Jq + q;
+#pragma unroll 1
+            for (int pp = 0; pp < 2; pp++) {
+                int h = t & 3;
+                asm volatile("" : "+v"(h));
+                const int blk = ::fftreg::qblk(h);
+                const float2 sblk = sblk_of(blk);
+                float2 y[H];
+#pragma unroll
+                for (int k = 0; k < H; k++) {
+                    y[k] = lds[FL::at(blk * H + k, r)];
+                    if (pp)  // row rho = blk 32 + k: exp(i pi rs / 128)
+                        y[k] = ::fftreg::cmul_f(y[k], f128_factor(sblk, k));
+                }
+                ::fftreg::inv_quad128(y, h);  // y[j] = C_pp,q[L j + h][r]
+#pragma unroll
+                for (int jj = 0; jj < H; jj++) {
+#pragma clang fp contract(off)
+                    const int I = (2 * (L * jj + h) + pp + W) & (WW - 1);
+                    const float x2 = y[jj].x * y[jj].x + y[jj].y * y[jj].y;
+                    const float v = sqrtf(x2);
+                    M[I * WW + J] = v;
+                    const int i0 = I < W ? I : I - W, j0 = J < W ? J : J - W;
+                    const int qd = (I < W) ? (J < W ? 0 : 3) : (J < W ? 2 : 1);
+                    const int rank = (i0 * W + j0) * 4 + qd;
+                    if (v > 0.0f) {
+                        Top2 c{v, rank, 0.0f, v};
+                        best = top2_combine(best, c);
+                    } else {
+                        best.v2 = fmaxf(best.v2, v);
+                    }
+                }
+            }
+            __syncthreads();  // every read of Z done before the next phase writes it
+        }
+        best = wg_top2<NTH>(best, sht);
+        __syncthreads();
+        PeakRes *res = &C.res[it];
+        int I = -1000, J = -1000;
+        float raw = 0.f;
+        if (best.r1 != 0x7fffffff) {
+            const int qd = best.r1 & 3, ij = best.r1 >> 2;
+            const int i0 = ij / W, j0 = ij - (ij / W) * W;
+            I = (qd == 0 || qd == 3) ? i0 : i0 + W;
+            J = (qd == 0 || qd == 2) ? j0 : j0 + W;
+            raw = best.v1;
+        }
+        if (t == 0) {
+            res->gap = raw > 0 ? (raw - best.v2) / raw : 0.f;
+            res->snr = best.sum > 0.0f ? best.v1 * (float)(WW * WW) / best.sum : 0.0f;
+            res->lmOk = 0;
+            res->fi = res->fj = -1;
+            res->fineMax = 0;
+            res->fineGap = 0;
+        }
+        if (I < NFAST / 2 || I > WW - NFAST / 2 || J < NFAST / 2 || J > WW - NFAST / 2) {
+            if (t == 0) {
+                res->I = res->J = -1000;
+                res->raw = -1.f;
+            }
+        } else {
+            if (t == 0) {
+                res->I = I;
+                res->J = J;
+                res->raw = raw;
+            }
+            // the |C| block around the peak, for k_cpx_refine (cells off the
+            // surface are never read: the fits' edge tests reject them)
+            float *nb = C.nb + (size_t)it * kNb * kNb;
+            for (int e = t; e < kNb * kNb; e += NTH) {
+                const int ri = I - kNb / 2 + e / kNb, ci = J - kNb / 2 + e % kNb;
+                nb[e] = (ri >= 0 && ri < WW && ci >= 0 && ci < WW) ? M[ri * WW + ci] : 0.0f;
+            }
+        }
+        __syncthreads();  // block reads of M done before the next item writes it
+    }
+}
+
+// ---------------------------------------------------------------------
 // host side
 
 void host_hanning(int wA, int wR, std::vector<float> &h)
@@ -2345,6 +2588,7 @@ void init_dirichlet(st_ctx *ctx)
     for (int j = 0; j < 128; j++)
         t128[j] = make_float2((float)cos(2.0 * M_PI * j / 128.0), (float)sin(2.0 * M_PI * j / 128.0));
     ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fftreg::c_tw128), t128, sizeof(t128)));
+
     done[ctx->device] = true;
 }
 
@@ -2402,20 +2646,55 @@ struct FusedJob {
     int count;   // its complex attempts
 };
 
+// w = 128: one workgroup per CU (the transpose buffer fills the LDS), with
+// its two global slots.
+constexpr size_t kF128Lds = (size_t)FusedLanes<kF128>::BUF * sizeof(float2);
+constexpr size_t kF128SlotBytes = kF128SlotX * sizeof(float2) + kF128SlotM * sizeof(float);
+
+template <int STYPE>
+int fused128_per_cu()
+{
+    int perCU = 0;
+    ST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_cpx_fused128<STYPE>, 512, kF128Lds));
+    return std::max(1, perCU);
+}
+
+template <int STYPE>
+void launch_fused128_kernel(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *dStart, int njobs,
+                            int total, const CpxBatch *dC)
+{
+    int ncu = 0;
+    ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const int grid = std::max(1, std::min(total, fused128_per_cu<STYPE>() * ncu));
+    float2 *sx = ctx->arena.take<float2>((size_t)grid * kF128SlotX);
+    float *sm = ctx->arena.take<float>((size_t)grid * kF128SlotM);
+    ctx->time_begin(slot);
+    hipLaunchKernelGGL((k_cpx_fused128<STYPE>), dim3(grid), dim3(512), kF128Lds, ctx->stream, dP, dS, dStart, njobs,
+                       dC, sx, sm);
+    ctx->time_end(slot);
+    ST_HIP(hipGetLastError());
+}
+
 template <int W, int STYPE>
 void launch_cpx_fused_t(st_ctx *ctx, int slot, const std::vector<FusedJob> &jobs, const KPar *dP, const Pts *dS,
                         const CpxBatch *dC, const int *dStart, int total, int base)
 {
-    const size_t lds = fused_lds_bytes<W>();
     const int prepSlot = ctx->kernel_slot("k_cpx_prep");
     ctx->time_begin(prepSlot);
     for (size_t j = 0; j < jobs.size(); j++) {
         const int count = jobs[j].count;
-        hipLaunchKernelGGL((k_cpx_prep_cols<W, STYPE>), dim3((count + 64 / W - 1) / (64 / W)), dim3(64), 0,
+        constexpr int PPW = W < 64 ? 64 / W : 1;  // patches per prep wave
+        hipLaunchKernelGGL((k_cpx_prep_cols<W, STYPE>), dim3((count + PPW - 1) / PPW), dim3(64), 0,
                            ctx->stream, dP + j, dS + j, jobs[j].C.list, count, dC + j);
     }
     ctx->time_end(prepSlot);
     const int n = (int)jobs.size();
+    if constexpr (W == kF128) {
+        (void)base;  // the w = 128 kernel takes no phase corrections (plan() sends those calls to the staged path)
+        launch_fused128_kernel<STYPE>(ctx, slot, dP, dS, dStart, n, total, dC);
+        return;
+    }
+    const size_t lds = fused_lds_bytes<W>();
     switch (base) {
     case 0: launch_fused_kernel<W, STYPE, 0>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
     case 1: launch_fused_kernel<W, STYPE, 1>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
@@ -2446,6 +2725,8 @@ int fused_per_cu(int stype)
 
 int fused_per_cu_w(int w, int stype)
 {
+    if (w == kF128)
+        return stype == ST_SAMPLE_C64 ? fused128_per_cu<ST_SAMPLE_C64>() : fused128_per_cu<ST_SAMPLE_CI16>();
     return w == 16 ? fused_per_cu<16>(stype) : (w == 32 ? fused_per_cu<32>(stype) : fused_per_cu<64>(stype));
 }
 
@@ -2709,7 +2990,9 @@ struct StrackJob {
         }
         mfmaPath = precision == ST_PREC_MFMA && square && (par.wR == 16 || par.wR == 32) && !par.baseFlag &&
                        !intOn && !gauss && !staged_only();
-        fusedPath = square && (par.wR == 16 || par.wR == 32 || par.wR == 64) && !gauss && !staged_only();
+        fusedPath = square && (par.wR == 16 || par.wR == 32 || par.wR == 64 ||
+                               (par.wR == kF128 && !par.baseFlag && !intOn)) &&
+                    !gauss && !staged_only();
         kPeak = ctx->kernel_slot(mfmaPath ? "k_cpx_mfma" : (fusedPath ? "k_cpx_fused" : "peak_refine"));
         kAmpPeak = ctx->kernel_slot("amp_peak");
         kGather = ctx->kernel_slot("cpx_gather");
@@ -2763,7 +3046,14 @@ struct StrackJob {
             return 0;
         const size_t perPt =
             sizeof(PeakRes) + 2 * sizeof(float) + 1 + sizeof(int) + 2 * sizeof(double) + kNb * kNb * sizeof(float);
-        return (size_t)npts * perPt + 8 * 256;
+        size_t slots = 0;
+        if (w == kF128) {
+            // the w = 128 kernel's per-workgroup slots
+            int ncu = 0;
+            ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+            slots = (size_t)fused_per_cu_w(w, pr.stype) * ncu * kF128SlotBytes + 512;
+        }
+        return (size_t)npts * perPt + 8 * 256 + slots;
     }
 
     // Per-point state, parameter block, uploads, geometry, the count of
@@ -3227,8 +3517,10 @@ void run_fused_group(st_ctx *ctx, std::vector<StrackJob *> &group)
         launch_cpx_fused<16>(ctx, slot, fj);
     else if (w == 32)
         launch_cpx_fused<32>(ctx, slot, fj);
-    else
+    else if (w == 64)
         launch_cpx_fused<64>(ctx, slot, fj);
+    else
+        launch_cpx_fused<kF128>(ctx, slot, fj);
     for (StrackJob *j : group)
         j->fused_finish();
 }
@@ -3320,6 +3612,6 @@ int fused_occupancy(int w, int stype) { return fused_per_cu_w(w, stype); }
 
 bool strack_batchable(const st_track_params &p)
 {
-    return p.wR == p.wA && p.wRa == p.wAa && (p.wR == 16 || p.wR == 32 || p.wR == 64) &&
+    return p.wR == p.wA && p.wRa == p.wAa && (p.wR == 16 || p.wR == 32 || p.wR == 64 || p.wR == kF128) &&
            p.precision == ST_PREC_FP32 && !p.gaussFlag && !p.intf && !p.baseFlag && !p.noComplex && !staged_only();
 }

@@ -6,8 +6,10 @@
 //   - the precision ST_PREC_AUTO runs (FP32 fused kernel, or the split-bf16
 //     matrix-core kernel at windows 16 and 32);
 //   - the persistent grid of the fused kernel, as workgroups per CU (from
-//     the occupancy limit down to half of it);
-//   - the batch cap of the staged path (windows it handles alone, 128).
+//     the occupancy limit down to half of it).
+// The cache format also carries a batch cap for the staged path; the
+// tuner leaves it at 0 (scratch-budget batches), since every tuned window
+// now runs a fused kernel.
 // The choices are kept per device in a process-wide table and in a text
 // cache, one line per window:
 //
@@ -236,15 +238,11 @@ extern "C" int st_autotune(st_ctx *ctx, const int32_t *windows, int32_t n, const
         // candidates
         std::vector<TuneEntry> cand;
         int maxPerCu = 0;
-        rc = st_guard(ctx, [&]() { maxPerCu = w <= 64 ? fused_occupancy(w, ST_SAMPLE_C64) : 0; });
-        if (w <= 64)
-            for (int g = maxPerCu; g >= std::max(1, maxPerCu / 2); g--)
-                cand.push_back(TuneEntry{w, ST_PREC_FP32, g, 0, 0});
+        rc = st_guard(ctx, [&]() { maxPerCu = fused_occupancy(w, ST_SAMPLE_C64); });
+        for (int g = maxPerCu; g >= std::max(1, maxPerCu / 2); g--)
+            cand.push_back(TuneEntry{w, ST_PREC_FP32, g, 0, 0});
         if (w <= 32)
             cand.push_back(TuneEntry{w, ST_PREC_MFMA, 0, 0, 0});
-        if (w == 128)
-            for (int b : {0, 256, 1024, 4096})
-                cand.push_back(TuneEntry{w, ST_PREC_FP32, 0, b, 0});
         // cand[0] is the default choice (FP32, occupancy limit, budget
         // batches); another one must beat it by kMinGain to replace it
         // every candidate runs as kLoad copies of the job at once, one per

@@ -182,15 +182,16 @@ def _kernel_names(device):
     return {st.kernelNames[i].value.decode(): st.kernelLaunches[i] for i in range(len(st.kernelNames))}
 
 
-@pytest.mark.parametrize("w", [16, 32, 64])
+@pytest.mark.parametrize("w", [16, 32, 64, 128])
 def test_staged_path_parity(device, oracle32, oracle64, monkeypatch, w):
     """The staged complex path (range transforms, then k_cpx_colx,
     k_cpx_rowmag and k_peak_merge) forced at the fused kernels' window
     sizes, against the same oracle bars."""
     monkeypatch.setenv("SPECKLE_AMD_PATH", "staged")
-    s = configs._spec(480, 480, 300 + w, dr0=0.8, drR=1e-3, da0=-1.3, daA=2e-3, gmin=0.35, gmax=0.85,
+    n = 480 if w <= 64 else 1100
+    s = configs._spec(n, n, 300 + w, dr0=0.8, drR=1e-3, da0=-1.3, daA=2e-3, gmin=0.35, gmax=0.85,
                       gLr=300, gLa=250)
-    p = small_params(w, 480, 480, step=w // 2, rShiftPoly=[0.8, 1e-3, 0], aShiftPoly=[-1.3, 0, 2e-3])
+    p = small_params(w, n, n, step=w // 2, rShiftPoly=[0.8, 1e-3, 0], aShiftPoly=[-1.3, 0, 2e-3])
     a, b = track.synth_host(s)
     device.set_profiling(True)
     try:
@@ -203,8 +204,8 @@ def test_staged_path_parity(device, oracle32, oracle64, monkeypatch, w):
 
 
 def test_circular_shift_known_answer_w128(device):
-    """w = 128 runs the staged path; circular shifts give exact integer
-    offsets and correlation 1."""
+    """w = 128 (the lane-quad fused kernel): circular shifts give exact
+    integer offsets and correlation 1."""
     w = 128
     img1, img2 = periodic_pair(w, 6 * w, 6 * w, 5, -3, seed=w)
     par = small_params(w, 6 * w, 6 * w, step=w // 2, hanningFlag=0, rShiftPoly=[5, 0, 0], aShiftPoly=[-3, 0, 0])
@@ -223,7 +224,13 @@ def test_parity_window_128(device, oracle32, oracle64):
                       gLa=300)
     p = small_params(128, 1100, 1100, step=48, rShiftPoly=[2.3, 2e-3, 0], aShiftPoly=[-1.7, 0, 1e-3])
     a, b = track.synth_host(s)
-    g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
+    device.set_profiling(True)
+    try:
+        g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a, b)
+        names = _kernel_names(device)
+    finally:
+        device.set_profiling(False)
+    assert names.get("k_cpx_fused", 0) > 0 and names.get("peak_refine", 0) == 0
     assert (g.type == abi.ST_CMATCH).mean() > 0.5
     _check(rep, probs)
 
