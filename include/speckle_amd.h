@@ -179,8 +179,9 @@ int st_speckle_track(st_ctx *ctx, int32_t pair, const st_track_params *par,
                      st_track_out *out, st_track_diag *diag);
 
 /* Several strack calls (pairs, grids and parameters of their own) as one
- * batch: the calls whose complex matches run on the fused FP32 kernel
- * (windows 16, 32, 64) share one persistent launch per window class, the
+ * batch: the calls whose complex matches run on the fused FP32 kernels
+ * (square windows 16, 32, 64, 128) share one persistent launch per window
+ * class, the
  * rest of each call runs as st_speckle_track does.  Each call's outputs are
  * the ones it gives alone.  pars[k] and outs[k] describe call k; no
  * diagnostics.  No counterpart in the reference (one call per process). */
@@ -194,8 +195,9 @@ int st_corr_track_fast(st_ctx *ctx, int32_t pair, const st_track_params *par,
 
 /* ---- autotuning (no counterpart in the reference) -------------------- */
 /* Times the complex-match stage of a synthetic job per window (16, 32,
- * 64, 128) over the launch choices -- precision for ST_PREC_AUTO, the
- * fused kernel's workgroups per CU, the staged path's batch cap -- keeps
+ * 64, 128) over the launch choices -- precision for ST_PREC_AUTO and the
+ * fused kernel's workgroups per CU (the cache's staged-path batch cap is
+ * written as 0) -- keeps
  * the fastest for this process and device, and writes them to cacheFile
  * (text, one line per window: arch window precision wg_per_cu staged_batch
  * us_per_patch; NULL: not written).  A context reads its architecture's
