@@ -1041,6 +1041,7 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
     const int blk = b < per * 8 ? (b % 8) * per + b / 8 : b;
     const int it = blk * PPW + lane / LPP, c0 = lane % LPP;
     const bool act = it < count;
+    // per column of the lane (CPL independent accumulation chains), summed after the loop
     double ps[2] = {0, 0}, T[2][2] = {{0, 0}, {0, 0}}, Lg[2][2] = {{0, 0}, {0, 0}};
     double F0[2][2] = {{0, 0}, {0, 0}}, FL[2][2] = {{0, 0}, {0, 0}};
     int nz = 0;
@@ -1048,56 +1049,77 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
         const int pt = list[it];
         const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
         const bool useH = P.hanning != 0;
-#pragma unroll 1
-        for (int cc = 0; cc < CPL; cc++) {
-        const int c = c0 + cc * LPP;
-        float2 prev[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-#pragma unroll 4
-        for (int a = 0; a < W; a++) {
-#pragma clang fp contract(off)
-            float pw[2];
+        double cps[CPL][2], cT[CPL][2][2], cLg[CPL][2][2];
+        float2 prev[CPL][2];
+#pragma unroll
+        for (int cc = 0; cc < CPL; cc++)
 #pragma unroll
             for (int g = 0; g < 2; g++) {
-                const void *img = g ? P.im.img2 : P.im.img1;
-                const int nr = g ? P.im.nr2 : P.im.nr1;
-                const size_t o = (size_t)((g ? a2 : a1) + a) * nr + (g ? r2 : r1) + c;
-                float2 x;
-                if constexpr (isf) {
-                    x = ((const float2 *)img)[o];
-                    x.x *= 1000;
-                    x.y *= 1000;
-                } else {
-                    const short2 q = ((const short2 *)img)[o];
-                    x = make_float2((float)q.x, (float)q.y);
-                }
-                if (g == 1 && useH) {
-                    const float hw = P.hann[a * W + c];
-                    x.x *= hw;
-                    x.y *= hw;
-                }
-                pw[g] = x.x * x.x + x.y * x.y;
-                ps[g] += pw[g];
-                T[g][0] += x.x;
-                T[g][1] += x.y;
-                if (a == 0) {
-                    F0[g][0] += x.x;
-                    F0[g][1] += x.y;
-                } else {
-                    // x conj(prev): re = x.re p.re + x.im p.im, im = x.im p.re - x.re p.im.
-                    // A product of two floats is exact in double, so each fma
-                    // rounds once, like the add it replaces.
-                    Lg[g][0] = fma((double)x.y, (double)prev[g].y, fma((double)x.x, (double)prev[g].x, Lg[g][0]));
-                    Lg[g][1] = fma(-(double)x.x, (double)prev[g].y, fma((double)x.y, (double)prev[g].x, Lg[g][1]));
-                }
-                if (a == W - 1) {
-                    FL[g][0] += x.x;
-                    FL[g][1] += x.y;
-                }
-                prev[g] = x;
+                cps[cc][g] = 0;
+                cT[cc][g][0] = cT[cc][g][1] = 0;
+                cLg[cc][g][0] = cLg[cc][g][1] = 0;
+                prev[cc][g] = make_float2(0.f, 0.f);
             }
-            nz += (pw[0] < thr || pw[1] < thr) ? 1 : 0;
+#pragma unroll(CPL > 1 ? 2 : 4)
+        for (int a = 0; a < W; a++) {
+#pragma unroll
+            for (int cc = 0; cc < CPL; cc++) {
+#pragma clang fp contract(off)
+                const int c = c0 + cc * LPP;
+                float pw[2];
+#pragma unroll
+                for (int g = 0; g < 2; g++) {
+                    const void *img = g ? P.im.img2 : P.im.img1;
+                    const int nr = g ? P.im.nr2 : P.im.nr1;
+                    const size_t o = (size_t)((g ? a2 : a1) + a) * nr + (g ? r2 : r1) + c;
+                    float2 x;
+                    if constexpr (isf) {
+                        x = ((const float2 *)img)[o];
+                        x.x *= 1000;
+                        x.y *= 1000;
+                    } else {
+                        const short2 q = ((const short2 *)img)[o];
+                        x = make_float2((float)q.x, (float)q.y);
+                    }
+                    if (g == 1 && useH) {
+                        const float hw = P.hann[a * W + c];
+                        x.x *= hw;
+                        x.y *= hw;
+                    }
+                    pw[g] = x.x * x.x + x.y * x.y;
+                    cps[cc][g] += pw[g];
+                    cT[cc][g][0] += x.x;
+                    cT[cc][g][1] += x.y;
+                    if (a == 0) {
+                        F0[g][0] += x.x;
+                        F0[g][1] += x.y;
+                    } else {
+                        // x conj(prev): re = x.re p.re + x.im p.im, im = x.im p.re - x.re p.im.
+                        // A product of two floats is exact in double, so each fma
+                        // rounds once, like the add it replaces.
+                        const float2 pv = prev[cc][g];
+                        cLg[cc][g][0] = fma((double)x.y, (double)pv.y, fma((double)x.x, (double)pv.x, cLg[cc][g][0]));
+                        cLg[cc][g][1] = fma(-(double)x.x, (double)pv.y, fma((double)x.y, (double)pv.x, cLg[cc][g][1]));
+                    }
+                    if (a == W - 1) {
+                        FL[g][0] += x.x;
+                        FL[g][1] += x.y;
+                    }
+                    prev[cc][g] = x;
+                }
+                nz += (pw[0] < thr || pw[1] < thr) ? 1 : 0;
+            }
         }
-        }
+#pragma unroll
+        for (int cc = 0; cc < CPL; cc++)
+#pragma unroll
+            for (int g = 0; g < 2; g++) {
+                ps[g] += cps[cc][g];
+                T[g][0] += cT[cc][g][0];
+                T[g][1] += cT[cc][g][1];
+                Lg[g][0] += cLg[cc][g][0];
+                Lg[g][1] += cLg[cc][g][1];
+            }
     }
     // segmented reduction over the LPP lanes of each patch pair
 #pragma unroll
@@ -1860,7 +1882,7 @@ __global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int W
     wsync();
     double f1, f2;
     int fr;
-    refine_core<64, 3>(Pc[wv], Tq[wv], Rt, lane, wsync, f1, f2, fr);
+    refine_core<64, 3, 2>(Pc[wv], Tq[wv], Rt, lane, wsync, f1, f2, fr);
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const double b1 = __shfl_xor(f1, o), b2 = __shfl_xor(f2, o);
@@ -2692,14 +2714,14 @@ void launch_cpx_fused_t(st_ctx *ctx, int slot, const std::vector<FusedJob> &jobs
     if constexpr (W == kF128) {
         (void)base;  // the w = 128 kernel takes no phase corrections (plan() sends those calls to the staged path)
         launch_fused128_kernel<STYPE>(ctx, slot, dP, dS, dStart, n, total, dC);
-        return;
-    }
-    const size_t lds = fused_lds_bytes<W>();
-    switch (base) {
-    case 0: launch_fused_kernel<W, STYPE, 0>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
-    case 1: launch_fused_kernel<W, STYPE, 1>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
-    case 2: launch_fused_kernel<W, STYPE, 2>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
-    default: launch_fused_kernel<W, STYPE, 3>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
+    } else {
+        const size_t lds = fused_lds_bytes<W>();
+        switch (base) {
+        case 0: launch_fused_kernel<W, STYPE, 0>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
+        case 1: launch_fused_kernel<W, STYPE, 1>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
+        case 2: launch_fused_kernel<W, STYPE, 2>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
+        default: launch_fused_kernel<W, STYPE, 3>(ctx, slot, dP, dS, dStart, n, total, dC, lds); break;
+        }
     }
 }
 

@@ -115,9 +115,10 @@ struct RefLayout {
 };
 
 // T = (b P) R^T from Pc = b P (8 x 8), then |Q|^2 = |R T|^2 over the box
-// in TU x TU tiles; Rt: R (48 x 8) in LDS or constant memory.  Each thread
-// leaves with the best (f1, rank fr) and runner-up f2 of its own tiles.
-template <int NTH, int TU, class Sync>
+// in TU x TU tiles; Rt: R (48 x 8) in LDS or constant memory; UM: unroll of
+// the inner product loop.  Each thread leaves with the best (f1, rank fr)
+// and runner-up f2 of its own tiles.
+template <int NTH, int TU, int UM = 1, class Sync>
 __device__ inline void refine_core(const double2 *Pc, double2 *Tq, const double *Rt, int t, Sync sync, double &f1,
                                    double &f2, int &fr)
 {
@@ -150,7 +151,7 @@ __device__ inline void refine_core(const double2 *Pc, double2 *Tq, const double 
 #pragma unroll
             for (int b = 0; b < TU; b++)
                 ar[a][b] = ai[a][b] = 0.0;
-#pragma unroll 1
+#pragma unroll(UM)
         for (int m = 0; m < NFAST; m++) {
             double r[TU];
             double2 tv[TU];
