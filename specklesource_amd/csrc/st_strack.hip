@@ -1921,6 +1921,14 @@ __global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int W
     }
 }
 
+// global-address-space views of pointers held in parameter blocks (loads
+// through them are global loads, not flat ones that also count against
+// the LDS wait counter)
+typedef __attribute__((address_space(1))) const char gchar;
+typedef __attribute__((address_space(1))) const float gfloat;
+typedef __attribute__((address_space(1))) const unsigned long long gu64;
+typedef __attribute__((address_space(1))) const unsigned gu32;
+
 // Lanes per transform of the fused kernel.  Lane pairs at every window
 // size: lane quads at w = 64 (16 points per lane, 512-thread blocks) were
 // measured slower on the full-scene job (55.6 ms vs 43.6 ms per launch) --
@@ -2012,10 +2020,10 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         for (int m = 0; m < H; m++) {
             const size_t o = (size_t)(aa + L * m) * nr + ra;
             if constexpr (isf) {
-                x[m] = ((const float2 *)img)[o];
+                x[m] = __builtin_bit_cast(float2, ((const gu64 *)img)[o]);
             } else {
-                const short2 q = ((const short2 *)img)[o];
-                x[m] = make_float2((float)q.x, (float)q.y);
+                const unsigned q = ((const gu32 *)img)[o];
+                x[m] = make_float2((float)(short)(q & 0xffffu), (float)(short)(q >> 16));
             }
         }
 #pragma unroll
@@ -2026,7 +2034,7 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
                 v.x *= 1000;
                 v.y *= 1000;
             }
-            const float hw = P.hann[(L * m + h) * W + r];
+            const float hw = ((const gfloat *)P.hann)[(L * m + h) * W + r];
             v.x = useH ? v.x * hw : v.x;
             v.y = useH ? v.y * hw : v.y;
             x[m] = v;
@@ -2243,16 +2251,15 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
 // pair (thread t: index r = t / 4, lane quarter h = t & 3), persistent over
 // the work list.  A 128 x 128 complex array fills most of a CU's LDS, so
 // the two windows go through one transpose buffer in turn: window 1's
-// spectrum waits in a per-workgroup global slot (cache resident) and is
-// multiplied into window 2's spectrum row by row; the cross spectrum goes
-// back to the slot, which both column phases of the polyphase inverse read;
-// and the 256 x 256 |C| surface goes to a second slot, from which the
-// block around the coarse peak is taken for k_cpx_refine.  Otherwise as
+// spectrum waits in registers (64 per thread) while window 2's is formed,
+// the cross spectrum stays there for both column phases of the polyphase
+// inverse, and the 256 x 256 |C| surface goes to a per-workgroup global
+// slot (phase-major, so every store fills whole cache lines), from which
+// the block around the coarse peak is taken for k_cpx_refine.  Otherwise as
 // k_cpx_fused, with each 128-point transform on a lane quad
 // (fwd_quad128 / inv_quad128): the components C[2n+p][2m+q], one row
 // transform per column phase, findCPeak's rank order.
 constexpr int kF128 = 128;
-constexpr size_t kF128SlotX = (size_t)kF128 * kF128;      // float2 per workgroup: F1, then X
 constexpr size_t kF128SlotM = (size_t)4 * kF128 * kF128;  // float per workgroup: |C|
 
 // exp(i pi (blk 32 + k - (blk >= 2 ? 128 : 0)) / 128) = s_blk exp(i pi k / 128):
@@ -2265,7 +2272,7 @@ __device__ inline float2 f128_factor(float2 sblk, int k)
 template <int STYPE>
 __global__ void __launch_bounds__(512) k_cpx_fused128(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
                                                       const int *__restrict__ jobStart, int njobs,
-                                                      const CpxBatch *__restrict__ Cp, float2 *slotX, float *slotM)
+                                                      const CpxBatch *__restrict__ Cp, float *slotM)
 {
     constexpr int W = kF128, L = 4, H = W / L, NTH = 512, WW = 2 * W;
     using FL = FusedLanes<W>;
@@ -2276,7 +2283,6 @@ __global__ void __launch_bounds__(512) k_cpx_fused128(const KPar *__restrict__ P
     // items in XCD-contiguous order (as k_cpx_fused)
     const int G = gridDim.x, Q = G / 8;
     const int base = (G % 8 == 0) ? (blockIdx.x % 8) * Q + blockIdx.x / 8 : blockIdx.x;
-    float2 *Xs = slotX + (size_t)blockIdx.x * kF128SlotX;
     float *M = slotM + (size_t)blockIdx.x * kF128SlotM;
     for (int k0 = 0; k0 < count; k0 += G) {
         const int itg = k0 + base;
@@ -2297,37 +2303,45 @@ __global__ void __launch_bounds__(512) k_cpx_fused128(const KPar *__restrict__ P
         int t = threadIdx.x;
         asm volatile("" : "+v"(t));
         const int r = t >> 2;
-        const int pt = C.list[it];
-        if (!C.have[it])
+        // the item's values are the same on every lane: scalar registers
+        const int pt = __builtin_amdgcn_readfirstlane(C.list[it]);
+        if (!__builtin_amdgcn_readfirstlane(C.have[it]))
             continue;
-        const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
-        const int az = C.az[it];
+        const int r1 = __builtin_amdgcn_readfirstlane(S.r1[pt]), a1 = __builtin_amdgcn_readfirstlane(S.a1[pt]);
+        const int r2 = __builtin_amdgcn_readfirstlane(S.r2[pt]), a2 = __builtin_amdgcn_readfirstlane(S.a2[pt]);
+        const int az = __builtin_amdgcn_readfirstlane(C.az[it]);
         // ---------------- forward 2-D transforms, window 1 then window 2
-#pragma unroll 1
-        for (int g = 0; g < 2; g++) {
-            // per-lane values behind an opaque copy in every loop body, so the
+        // fwd2d(g, x): window g's spectrum, row r, columns blk H + k in x[k]
+        auto fwd2d = [&](int g, float2 (&x)[H]) {
+            // per-lane values behind an opaque copy in every body, so the
             // compiler does not hoist the lane-selected twiddles and factors
-            // out of the loops (they would hold registers throughout)
+            // out of it (they would hold registers throughout)
             int h = t & 3;
             asm volatile("" : "+v"(h));
             const int blk = ::fftreg::qblk(h);
-            float2 x[H];
-            const void *img = g ? P.im.img2 : P.im.img1;
             const int nr = g ? P.im.nr2 : P.im.nr1;
-            const int ra = (g ? r2 : r1) + r, aa = (g ? a2 : a1) + h;
             const bool useH = (g == 1) && P.hanning;
+            // window origin (uniform) + row step L m nr (uniform) + the
+            // lane's 32-bit offset h nr + r: one address register per lane
+            constexpr size_t ES = isf ? sizeof(float2) : sizeof(short2);
+            const char *org = (const char *)(g ? P.im.img2 : P.im.img1) +
+                              ((size_t)(g ? a2 : a1) * nr + (g ? r2 : r1)) * ES;
+            const unsigned lo = (unsigned)((h * nr + r) * ES);
 #pragma unroll
             for (int m = 0; m < H; m++) {
-                const size_t o = (size_t)(aa + L * m) * nr + ra;
+                const gchar *pm = (const gchar *)org + (size_t)(L * m) * nr * ES;
                 float2 v;
                 if constexpr (isf) {
-                    v = ((const float2 *)img)[o];
+                    v = __builtin_bit_cast(float2, *(const gu64 *)(pm + lo));
                 } else {
-                    const short2 q = ((const short2 *)img)[o];
-                    v = make_float2((float)q.x, (float)q.y);
+                    const unsigned q = *(const gu32 *)(pm + lo);
+                    v = make_float2((float)(short)(q & 0xffffu), (float)(short)(q >> 16));
                 }
                 x[m] = v;
             }
+            // the taper is read on both windows and selected (a uniform
+            // branch per element would wait for every load in flight)
+            const gfloat *hn = (const gfloat *)P.hann + h * W + r;
 #pragma unroll
             for (int m = 0; m < H; m++) {
 #pragma clang fp contract(off)
@@ -2336,11 +2350,9 @@ __global__ void __launch_bounds__(512) k_cpx_fused128(const KPar *__restrict__ P
                     v.x *= 1000;
                     v.y *= 1000;
                 }
-                if (useH) {
-                    const float hw = P.hann[(L * m + h) * W + r];
-                    v.x *= hw;
-                    v.y *= hw;
-                }
+                const float hw = hn[L * m * W];
+                v.x = useH ? v.x * hw : v.x;
+                v.y = useH ? v.y * hw : v.y;
                 x[m] = v;
             }
             ::fftreg::fwd_quad128(x, h);  // column r: outputs blk H + k
@@ -2353,18 +2365,19 @@ __global__ void __launch_bounds__(512) k_cpx_fused128(const KPar *__restrict__ P
                 x[m] = lds[FL::at(r, L * m + h)];
             __syncthreads();
             ::fftreg::fwd_quad128(x, h);  // row r: x[k] = F_g[r][blk H + k]
-            // slot element k of thread t at Xs[k * NTH + t] (coalesced)
-            if (g == 0) {
+        };
+        // window 1's spectrum stays in registers while window 2's is formed,
+        // and the cross spectrum X = F1 conj(F2) stays there for both
+        // column phases of the inverse
+        float2 Xr[H];
+        fwd2d(0, Xr);
+        {
+            float2 x[H];
+            fwd2d(1, x);
 #pragma unroll
-                for (int k = 0; k < H; k++)
-                    Xs[k * NTH + t] = x[k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < H; k++) {
+            for (int k = 0; k < H; k++) {
 #pragma clang fp contract(off)
-                    const float2 f1 = Xs[k * NTH + t];
-                    Xs[k * NTH + t] = cmulconj(f1, x[k]);  // X = F1 conj(F2)
-                }
+                Xr[k] = cmulconj(Xr[k], x[k]);
             }
         }
         // ---------------- polyphase inverse, one column phase q at a time
@@ -2386,11 +2399,8 @@ __global__ void __launch_bounds__(512) k_cpx_fused128(const KPar *__restrict__ P
                 const float2 sblk = sblk_of(blk);
                 float2 x[H];
 #pragma unroll
-                for (int k = 0; k < H; k++) {
-                    x[k] = Xs[k * NTH + t];
-                    if (q)  // column l = blk 32 + k: exp(i pi ls / 128)
-                        x[k] = ::fftreg::cmul_f(x[k], f128_factor(sblk, k));
-                }
+                for (int k = 0; k < H; k++)  // column l = blk 32 + k: exp(i pi ls / 128) on phase 1
+                    x[k] = q ? ::fftreg::cmul_f(Xr[k], f128_factor(sblk, k)) : Xr[k];
                 ::fftreg::inv_quad128(x, h);  // x[n'] = Z[krow][L n' + h]
 #pragma unroll
                 for (int n = 0; n < H; n++)
@@ -2418,7 +2428,7 @@ __global__ void __launch_bounds__(512) k_cpx_fused128(const KPar *__restrict__ P
                     const int I = (2 * (L * jj + h) + pp + W) & (WW - 1);
                     const float x2 = y[jj].x * y[jj].x + y[jj].y * y[jj].y;
                     const float v = sqrtf(x2);
-                    M[I * WW + J] = v;
+                    M[((pp * 2 + q) * H + jj) * NTH + t] = v;  // full cache lines per store
                     const int i0 = I < W ? I : I - W, j0 = J < W ? J : J - W;
                     const int qd = (I < W) ? (J < W ? 0 : 3) : (J < W ? 2 : 1);
                     const int rank = (i0 * W + j0) * 4 + qd;
@@ -2468,7 +2478,11 @@ __global__ void __launch_bounds__(512) k_cpx_fused128(const KPar *__restrict__ P
             float *nb = C.nb + (size_t)it * kNb * kNb;
             for (int e = t; e < kNb * kNb; e += NTH) {
                 const int ri = I - kNb / 2 + e / kNb, ci = J - kNb / 2 + e % kNb;
-                nb[e] = (ri >= 0 && ri < WW && ci >= 0 && ci < WW) ? M[ri * WW + ci] : 0.0f;
+                // cell (ri, ci): phase (pp, q), row index L jj + h, column r
+                const int u = (ri - W) & (WW - 1), v = (ci - W) & (WW - 1);
+                const int ip = u >> 1;
+                const size_t o = ((size_t)(((u & 1) * 2 + (v & 1)) * H + (ip >> 2)) * NTH) + (v >> 1) * L + (ip & 3);
+                nb[e] = (ri >= 0 && ri < WW && ci >= 0 && ci < WW) ? M[o] : 0.0f;
             }
         }
         __syncthreads();  // block reads of M done before the next item writes it
@@ -2671,7 +2685,7 @@ struct FusedJob {
 // w = 128: one workgroup per CU (the transpose buffer fills the LDS), with
 // its two global slots.
 constexpr size_t kF128Lds = (size_t)FusedLanes<kF128>::BUF * sizeof(float2);
-constexpr size_t kF128SlotBytes = kF128SlotX * sizeof(float2) + kF128SlotM * sizeof(float);
+constexpr size_t kF128SlotBytes = kF128SlotM * sizeof(float);
 
 template <int STYPE>
 int fused128_per_cu()
@@ -2688,11 +2702,10 @@ void launch_fused128_kernel(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS
     int ncu = 0;
     ST_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     const int grid = std::max(1, std::min(total, fused128_per_cu<STYPE>() * ncu));
-    float2 *sx = ctx->arena.take<float2>((size_t)grid * kF128SlotX);
     float *sm = ctx->arena.take<float>((size_t)grid * kF128SlotM);
     ctx->time_begin(slot);
     hipLaunchKernelGGL((k_cpx_fused128<STYPE>), dim3(grid), dim3(512), kF128Lds, ctx->stream, dP, dS, dStart, njobs,
-                       dC, sx, sm);
+                       dC, sm);
     ctx->time_end(slot);
     ST_HIP(hipGetLastError());
 }
