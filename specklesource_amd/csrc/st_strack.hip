@@ -1867,7 +1867,7 @@ __global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int W
     const int I = res->I, J = res->J;
     if (I < -999)
         return;
-    const float *nb = C.nb + (size_t)it * kNb * kNb;
+    const float *nb = C.nb + (size_t)it * kNbc;
     auto wsync = [] {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1875,7 +1875,7 @@ __global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int W
     };
     {
         const int m = lane / NFAST, n = lane % NFAST;
-        const double pm = (double)nb[(kNb / 2 - NFAST / 2 + m) * kNb + (kNb / 2 - NFAST / 2 + n)];
+        const double pm = (double)nb[kNbMap.off[(kNb / 2 - NFAST / 2 + m) * kNb + (kNb / 2 - NFAST / 2 + n)]];
         const double2 b = c_refB[m + n];
         Pc[wv][lane] = make_double2(b.x * pm, b.y * pm);
     }
@@ -1916,8 +1916,8 @@ __global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int W
     if (ok && lane < 2 * NGAUSS) {
         // cI, cJ within 2 samples of (I, J): inside the block
         const int k = lane % NGAUSS, bi = cI - I + kNb / 2, bj = cJ - J + kNb / 2;
-        res->lmY[lane / NGAUSS][k] = lane < NGAUSS ? nb[bi * kNb + (bj - NGAUSS / 2 + k)]
-                                                   : nb[(bi - NGAUSS / 2 + k) * kNb + bj];
+        res->lmY[lane / NGAUSS][k] = lane < NGAUSS ? nb[kNbMap.off[bi * kNb + (bj - NGAUSS / 2 + k)]]
+                                                   : nb[kNbMap.off[(bi - NGAUSS / 2 + k) * kNb + bj]];
     }
 }
 
@@ -2236,9 +2236,9 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
             // k_cpx_refine (cells off the surface are never read: the fits'
             // edge tests reject them)
             const float *Ms = (const float *)lds;
-            float *nb = C.nb + (size_t)it * kNb * kNb;
-            for (int e = t; e < kNb * kNb; e += NTH) {
-                const int ri = I - kNb / 2 + e / kNb, ci = J - kNb / 2 + e % kNb;
+            float *nb = C.nb + (size_t)it * kNbc;
+            for (int e = t; e < kNbc; e += NTH) {
+                const int ri = I - kNb / 2 + kNbMap.ri[e], ci = J - kNb / 2 + kNbMap.ci[e];
                 nb[e] = (ri >= 0 && ri < WW && ci >= 0 && ci < WW) ? Ms[ri * WW + ci] : 0.0f;
             }
         }
@@ -2475,9 +2475,9 @@ __global__ void __launch_bounds__(512) k_cpx_fused128(const KPar *__restrict__ P
             }
             // the |C| block around the peak, for k_cpx_refine (cells off the
             // surface are never read: the fits' edge tests reject them)
-            float *nb = C.nb + (size_t)it * kNb * kNb;
-            for (int e = t; e < kNb * kNb; e += NTH) {
-                const int ri = I - kNb / 2 + e / kNb, ci = J - kNb / 2 + e % kNb;
+            float *nb = C.nb + (size_t)it * kNbc;
+            for (int e = t; e < kNbc; e += NTH) {
+                const int ri = I - kNb / 2 + kNbMap.ri[e], ci = J - kNb / 2 + kNbMap.ci[e];
                 // cell (ri, ci): phase (pp, q), row index L jj + h, column r
                 const int u = (ri - W) & (WW - 1), v = (ci - W) & (WW - 1);
                 const int ip = u >> 1;
@@ -3080,7 +3080,7 @@ struct StrackJob {
         if (!fusedPath)
             return 0;
         const size_t perPt =
-            sizeof(PeakRes) + 2 * sizeof(float) + 1 + sizeof(int) + 2 * sizeof(double) + kNb * kNb * sizeof(float);
+            sizeof(PeakRes) + 2 * sizeof(float) + 1 + sizeof(int) + 2 * sizeof(double) + kNbc * sizeof(float);
         size_t slots = 0;
         if (w == kF128) {
             // the w = 128 kernel's per-workgroup slots
@@ -3326,7 +3326,7 @@ struct StrackJob {
         C.az = ar.take<int>(nCpx);
         C.p = ar.take<double>(2 * (size_t)nCpx);
         C.list = list;
-        C.nb = ar.take<float>((size_t)nCpx * kNb * kNb);
+        C.nb = ar.take<float>((size_t)nCpx * kNbc);
         return FusedJob{P, S, C, nCpx};
     }
 

@@ -16,6 +16,64 @@ constexpr int kFine = 2 * NOVER * 2;  // 48: side of the searched oversampled bo
 // I + kNb / 2 - 1 cover the 8 x 8 refinement box and both 18-sample
 // Gaussian fits at the refined peak (within 2 samples of the coarse one)
 constexpr int kNb = 22;
+
+// The cells of that block the refinement can read, stored compactly: the
+// 8 x 8 box (block rows and columns 7..14) and the two fit crosses.  The
+// refined peak (cI, cJ) = (I, J) + floor((f - 24) / 12), f in 0..47, lies
+// within -2..+1 of the coarse one, so each cross runs along block row (or
+// column) 9..12 over block columns (rows) 0..20.  168 of the 484 cells.
+struct NbMap {
+    short off[kNb * kNb];  // block cell -> compact index, -1 if never read
+    unsigned char ri[kNb * kNb], ci[kNb * kNb];  // compact index -> block cell
+    int n;
+};
+constexpr bool nb_needed(int i, int j)
+{
+    const int b0 = kNb / 2 - NFAST / 2, c0 = kNb / 2 - 2, c1 = kNb / 2 + 1;
+    const int lo = c0 - NGAUSS / 2, hi = c1 + NGAUSS / 2 - 1;
+    const bool box = i >= b0 && i < b0 + NFAST && j >= b0 && j < b0 + NFAST;
+    const bool rowX = i >= c0 && i <= c1 && j >= lo && j <= hi;
+    const bool colX = j >= c0 && j <= c1 && i >= lo && i <= hi;
+    return box || rowX || colX;
+}
+constexpr NbMap make_nb_map()
+{
+    NbMap m{};
+    m.n = 0;
+    for (int i = 0; i < kNb; i++)
+        for (int j = 0; j < kNb; j++) {
+            m.off[i * kNb + j] = -1;
+            if (nb_needed(i, j)) {
+                m.off[i * kNb + j] = (short)m.n;
+                m.ri[m.n] = (unsigned char)i;
+                m.ci[m.n] = (unsigned char)j;
+                m.n++;
+            }
+        }
+    return m;
+}
+constexpr NbMap kNbMap = make_nb_map();
+constexpr int kNbc = kNbMap.n;  // floats per patch in CpxBatch::nb
+// every cell k_cpx_refine can read is stored: the box, and the crosses at
+// every refined peak offset floor((f - kFine / 2) / NOVER), f in 0..kFine-1
+constexpr bool nb_map_covers_reads()
+{
+    for (int m = 0; m < NFAST; m++)
+        for (int n = 0; n < NFAST; n++)
+            if (kNbMap.off[(kNb / 2 - NFAST / 2 + m) * kNb + kNb / 2 - NFAST / 2 + n] < 0)
+                return false;
+    const int I = NFAST / 2;  // the smallest accepted coarse peak
+    for (int fi = 0; fi < kFine; fi++)
+        for (int fj = 0; fj < kFine; fj += kFine / 4) {
+            const int bi = kNb / 2 + (I * NOVER + fi + kFine / 2 - (NOVER * NFAST) / 2) / NOVER - I;
+            const int bj = kNb / 2 + (I * NOVER + fj + kFine / 2 - (NOVER * NFAST) / 2) / NOVER - I;
+            for (int k = 0; k < NGAUSS; k++)
+                if (kNbMap.off[bi * kNb + bj - NGAUSS / 2 + k] < 0 || kNbMap.off[(bi - NGAUSS / 2 + k) * kNb + bj] < 0)
+                    return false;
+        }
+    return true;
+}
+static_assert(kNbc == 168 && nb_map_covers_reads(), "compact peak block");
 constexpr int NT = 256;
 
 struct Images {
