@@ -40,6 +40,14 @@ def flops_complex_patch(w):
             + 16 * w * w + f_os)
 
 
+def flops_window_sums(w):
+    """Flops of the window sums the fused kernel forms for w <= 64
+    (getPatches' statistics and the Doppler lag sum): per sample of each of
+    the two windows, |x|^2 (3), the running sum (2) and the lag product
+    accumulated (8).  At w = 128 a separate kernel forms them."""
+    return 26 * w * w if w <= 64 else 0
+
+
 def flops_amp_patch(wa):
     """Amplitude try: forward wa^2 x2, inverse (2wa)^2 x2 (upsampling),
     forward (2wa)^2 x2, inverse (2wa)^2, detection and cross products,
@@ -262,7 +270,7 @@ def served_roofline(pairs, precision):
         st = _kernel_table(dev, table)
         names = [st.kernelNames[i].value.decode() for i in range(len(st.kernelNames))]
         kern = "k_cpx_mfma" if "k_cpx_mfma" in names else ("k_cpx_fused" if "k_cpx_fused" in names else "staged")
-        fl = flops_mfma_patch(p.wR) if kern == "k_cpx_mfma" else flops_complex_patch(p.wR)
+        fl = flops_mfma_patch(p.wR) if kern == "k_cpx_mfma" else flops_complex_patch(p.wR) + flops_window_sums(p.wR)
         f, n, b = work.get(kern, (0.0, 0, 0))
         work[kern] = (f + fl * st.nCorr[0], n + st.nCorr[0], b + 16 * p.wR * p.wR * st.nCorr[0])
     dev.set_profiling(False)
@@ -696,7 +704,8 @@ def main():
         # the fused kernel and the refinement kernel that follows it
         kname = "k_cpx_fused + k_cpx_refine"
         launch_ms = (kern["k_cpx_fused"][0] + kern.get("k_cpx_refine", (0.0, 0))[0]) / kern["k_cpx_fused"][1]
-        per_launch = flops_complex_patch(par.wR) * ncpx * args.steps / kern["k_cpx_fused"][1]
+        per_launch = ((flops_complex_patch(par.wR) + flops_window_sums(par.wR)) * ncpx * args.steps /
+                      kern["k_cpx_fused"][1])
     else:
         kname = "complex-match stage"
         launch_ms = stages[1] / args.steps

@@ -33,6 +33,7 @@ using namespace sttrack;
 namespace sttrack {
 __constant__ double c_refR[kFine * NFAST];
 __constant__ double2 c_refB[16];
+__constant__ double2 c_root256[256];
 }
 namespace fftreg {
 __constant__ float2 c_tw128[128];
@@ -947,18 +948,22 @@ __global__ void k_compose(KPar P, Pts S, float *offR, float *offA, int npts)
 
 // ---------------------------------------------------------------------
 // Fused complex match for w in {16, 32, 64}: one workgroup of 4w threads
-// per patch pair, persistent over the work list.  With the window
-// statistics and the Doppler shift from k_cpx_prep_cols, the chain
-// windows -> cmpPSWithPad -> 2w x 2w inverse -> findCPeak -> overSampleC
-// stays on chip:
+// per patch pair, persistent over the work list.  The chain
+// windows -> getPatches' sums and acceptance test -> cmpPSWithPad -> 2w x 2w
+// inverse -> findCPeak -> overSampleC stays on chip:
+//   * the window sums and the Doppler lag sum are formed from the loaded
+//     samples (lane_sums) and reduced across the workgroup while the
+//     forward transforms run; one thread applies the acceptance test and
+//     finds the shift (patch_accept) before the inverse needs it;
 //   * the w x w forward FFTs run in registers, each length-w transform
 //     split over a lane pair (st_fftreg.h), with one LDS transpose;
 //   * the zero-padded 2w x 2w inverse is done as its four polyphase
 //     components C[2n+p][2m+q] = IFFT_w(X[(k+az) mod w][l] exp(i pi (k p +
 //     l q) / w)) (k, l signed), so no padded array is ever formed;
 //   * the fftshifted magnitude surface stays in LDS for the argmax; the
-//     22 x 22 block around the coarse peak goes to k_cpx_refine, which
-//     refines it and takes the Gaussian-fit samples.
+//     cells of the 22 x 22 block around the coarse peak that k_cpx_refine
+//     reads (kNbMap) go to it, which refines the peak and takes the
+//     Gaussian-fit samples.
 // Thread t: group g = t / 2w (patch g in the forward part, column parity q
 // = g in the inverse part), index r = (t / 2) mod w, lane half h = t & 1.
 struct Top2 {
@@ -1012,6 +1017,121 @@ __device__ inline Top2 wg_top2(Top2 a, Top2 *sh)
     }
 }
 
+// Sums of getPatches (speckleTrack.c:1194-1330) and estDopCarrier1
+// (dopplerEstimation.c:42-88) over one window: [0] sum |x|^2, [1..2] sum x,
+// [3..4] the lag-1 sum along azimuth sum x[a] conj(x[a - 1]), [5..6] the
+// sum over row 0, [7..8] the sum over row W - 1.
+struct PatchSums {
+    double v[2][9];
+};
+
+// The acceptance test (more than 20 near-zero samples or a near-zero
+// window rejects the pair) and the Doppler shift from the sums, with the
+// lag-1 sum expanded around the patch mean m:
+//   sum (x1 - m) conj(x0 - m) = sum x1 conj(x0) - conj(m) sum x1
+//                               - m sum conj(x0) + (W - 1) W |m|^2
+// (the row-0 / row-(W-1) sums taken out of the totals).  Writes the item's
+// have / az / p; returns have.
+template <int W, int STYPE>
+__device__ inline int patch_accept(const KPar &P, const CpxBatch &C, int it, const PatchSums &s, int nz, int &az)
+{
+    const float thr = STYPE == ST_SAMPLE_C64 ? 1e-15f : 1e-9f;
+    const float thrAll = thr * W * W;
+    az = 0;
+    if (nz > 20 || s.v[0][0] < thrAll || s.v[1][0] < thrAll) {
+        C.have[it] = 0;
+        return 0;
+    }
+    const double cnt = (double)W * (double)W;
+    double zr[2], zi[2];
+    for (int g = 0; g < 2; g++) {
+        const double *T = s.v[g] + 1, *Lg = s.v[g] + 3, *F0 = s.v[g] + 5, *FL = s.v[g] + 7;
+        const double mr = T[0] / cnt, mi = T[1] / cnt;
+        // sums over the lag pairs: x1 over rows 1..W-1, x0 over rows 0..W-2
+        const double s1r = T[0] - F0[0], s1i = T[1] - F0[1];
+        const double s0r = T[0] - FL[0], s0i = T[1] - FL[1];
+        // - conj(m) s1 - m conj(s0) + (W - 1) W |m|^2
+        zr[g] = Lg[0] - (mr * s1r + mi * s1i) - (mr * s0r + mi * s0i) +
+                (double)(W - 1) * (double)W * (mr * mr + mi * mi);
+        zi[g] = Lg[1] - (mr * s1i - mi * s1r) - (mi * s0r - mr * s0i);
+    }
+    const int shift = doppler_shift(zr[0], zi[0], zr[1], zi[1], W, c_root256);
+    az = (int)((float)shift * (float)P.wA / (float)W);
+    C.have[it] = 1;
+    C.az[it] = az;
+    C.p[2 * it] = s.v[0][0] / cnt;
+    C.p[2 * it + 1] = s.v[1][0] / cnt;
+    return 1;
+}
+
+// Value of lane h - 1 of the lane's group of L (lane L - 1 for h = 0).
+template <int L>
+__device__ inline float2 lane_prev(float2 v)
+{
+    constexpr int ctl = L == 2 ? 0xB1 : 0x93;  // quad_perm [1,0,3,2] / [3,0,1,2]
+    return make_float2(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.x), ctl, 0xF, 0xF, false)),
+                       __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.y), ctl, 0xF, 0xF, false)));
+}
+
+// One lane's part of the window sums: it holds rows a = L m + h (m < H) of
+// a column in x (scaled and tapered as the transforms take them).  The
+// previous row of the lag sum comes from lane h - 1 of the group, or from
+// lane L - 1 one step earlier; the per-sample arithmetic is k_cpx_prep_cols'.
+// Bit m of mask: |x[m]|^2 below the near-zero threshold.
+template <int H, int L>
+__device__ inline void lane_sums(const float2 (&x)[H], int h, float thr, double (&acc)[9], unsigned &mask)
+{
+    float2 carry = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+        acc[k] = 0.0;
+    mask = 0u;
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+#pragma clang fp contract(off)
+        const float2 v = x[m];
+        const float2 nb = lane_prev<L>(v);
+        const float2 pv = h ? nb : carry;
+        carry = nb;
+        const float pw = v.x * v.x + v.y * v.y;
+        acc[0] += pw;
+        acc[1] += v.x;
+        acc[2] += v.y;
+        if (m == 0 && h == 0) {
+            acc[5] += v.x;
+            acc[6] += v.y;
+        } else {
+            acc[3] = fma((double)v.y, (double)pv.y, fma((double)v.x, (double)pv.x, acc[3]));
+            acc[4] = fma(-(double)v.x, (double)pv.y, fma((double)v.y, (double)pv.x, acc[4]));
+        }
+        if (m == H - 1 && h == L - 1) {
+            acc[7] += v.x;
+            acc[8] += v.y;
+        }
+        mask |= (pw < thr ? 1u : 0u) << m;
+    }
+}
+
+// Butterfly sum over aligned groups of SPAN lanes (SPAN <= 64).
+template <int SPAN>
+__device__ inline void group_sum(double (&acc)[9])
+{
+#pragma unroll
+    for (int o = 1; o < SPAN; o <<= 1)
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            acc[k] += __shfl_xor(acc[k], o);
+}
+
+template <int SPAN>
+__device__ inline int group_sum_i(int v)
+{
+#pragma unroll
+    for (int o = 1; o < SPAN; o <<= 1)
+        v += __shfl_xor(v, o);
+    return v;
+}
+
 // Per-patch statistics of getPatches (speckleTrack.c:1194-1330) with one
 // lane per window column (two at w = 128): 64 / W patch pairs per wave (one
 // at w = 128), each lane walks its columns down both windows (coalesced row
@@ -1033,7 +1153,6 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
     constexpr int LPP = W < 64 ? W : 64, PPW = 64 / LPP, CPL = W / LPP;
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
     const float thr = isf ? 1e-15f : 1e-9f;
-    const float thrAll = thr * W * W;
     const int lane = threadIdx.x;
     // consecutive items (overlapping patches) on one XCD, so their shared
     // rows hit that XCD's L2: workgroups are dealt round-robin over the 8 XCDs
@@ -1137,31 +1256,18 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
     }
     if (!act || c0 != 0)
         return;
-    if (nz > 20 || ps[0] < thrAll || ps[1] < thrAll) {
-        C.have[it] = 0;
-        return;
-    }
-    const double cnt = (double)W * (double)W;
-    double ang[2];
+    PatchSums ss;
     for (int g = 0; g < 2; g++) {
-        const double mr = T[g][0] / cnt, mi = T[g][1] / cnt;
-        // sums over the lag pairs: x1 over rows 1..W-1, x0 over rows 0..W-2
-        const double s1r = T[g][0] - F0[g][0], s1i = T[g][1] - F0[g][1];
-        const double s0r = T[g][0] - FL[g][0], s0i = T[g][1] - FL[g][1];
-        // - conj(m) s1 - m conj(s0) + (W - 1) W |m|^2
-        const double zr = Lg[g][0] - (mr * s1r + mi * s1i) - (mr * s0r + mi * s0i) +
-                          (double)(W - 1) * (double)W * (mr * mr + mi * mi);
-        const double zi = Lg[g][1] - (mr * s1i - mi * s1r) - (mi * s0r - mr * s0i);
-        ang[g] = (zr == 0.0 && zi == 0.0) ? NAN : atan2(zi, zr);
+        ss.v[g][0] = ps[g];
+        for (int k = 0; k < 2; k++) {
+            ss.v[g][1 + k] = T[g][k];
+            ss.v[g][3 + k] = Lg[g][k];
+            ss.v[g][5 + k] = F0[g][k];
+            ss.v[g][7 + k] = FL[g][k];
+        }
     }
-    const double v = (double)(W / 2) * (ang[0] + ang[1]) * .5 / 3.14159265358979323846;
-    int shift = isnan(v) ? 0 : ((v >= 0.0) ? (int)(v + 0.5) : (int)(v - 0.5));
-    if (shift < 0)
-        shift += W;
-    C.have[it] = 1;
-    C.az[it] = (int)((float)shift * (float)P.wA / (float)W);
-    C.p[2 * it] = ps[0] / cnt;
-    C.p[2 * it + 1] = ps[1] / cnt;
+    int az;
+    patch_accept<W, STYPE>(P, C, it, ss, nz, az);
 }
 
 // ---------------------------------------------------------------------
@@ -1972,7 +2078,14 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
     using LF = LaneFFT<W, L>;
     extern __shared__ float2 lds[];  // [2][BUF]
     __shared__ Top2 sht[16];
+    // window sums: one partial per group of SPAN lanes (NP / 2 per window),
+    // near-zero counts per wave, the sample masks, the acceptance result
+    constexpr int SPAN = 2 * W < 64 ? 2 * W : 64, NP = NTH / SPAN;
+    __shared__ double stp[NP][9];
+    __shared__ int stz[NTH / 64], sflag[2];
+    __shared__ unsigned smask[NTH];
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
+    const float thr = isf ? 1e-15f : 1e-9f;
 
     // Items in XCD-contiguous order: at every round the workgroups of one
     // XCD (dealt round-robin, b % 8) take a contiguous run of items, so the
@@ -2007,9 +2120,6 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         const int blk = LF::blk(h);
         const int pt = C.list[it];
         const int r1 = S.r1[pt], a1 = S.a1[pt], r2 = S.r2[pt], a2 = S.a2[pt];
-        if (!C.have[it])
-            continue;
-        const int az = C.az[it];
         // ---------------- load column r of patch g: samples a = 2m + h
         float2 x[H];
         const void *img = g ? P.im.img2 : P.im.img1;
@@ -2039,6 +2149,23 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
             v.y = useH ? v.y * hw : v.y;
             x[m] = v;
         }
+        // ---------------- window sums (getPatches' statistics and the
+        // Doppler lag sum) from the loaded samples, reduced to one partial
+        // per SPAN lanes; the pair's acceptance test and shift follow once
+        // both windows' near-zero masks are in LDS
+        {
+            double acc[9];
+            unsigned mk;
+            lane_sums<H, L>(x, h, thr, acc, mk);
+            group_sum<SPAN>(acc);
+            if ((t & (SPAN - 1)) == 0) {
+#pragma unroll
+                for (int k = 0; k < 9; k++)
+                    stp[t / SPAN][k] = acc[k];
+            }
+            smask[t] = mk;
+        }
+        __builtin_amdgcn_sched_barrier(0);  // the sums finish before the transforms start
         // ---------------- phase corrections on patch 1 (in LDS)
         if constexpr (BASE != 0) {
             float2 *R = lds + (size_t)g * BUF;
@@ -2091,16 +2218,47 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         for (int k = 0; k < H; k++)
             F[FL::at(blk * H + k, r)] = x[k];
         __syncthreads();
+        {
+            // samples near zero in either window (thread t + 2w holds the
+            // same samples of window 2)
+            const int nzl = g == 0 ? __popc(smask[t] | smask[t + 2 * W]) : 0;
+            const int nzw = group_sum_i<64>(nzl);
+            if ((t & 63) == 0)
+                stz[t >> 6] = nzw;
+        }
         // ---------------- forward FFT along range (row r)
 #pragma unroll
         for (int m = 0; m < H; m++)
             x[m] = F[FL::at(r, L * m + h)];
         __syncthreads();
+        if (t == 0) {
+            PatchSums ss;
+#pragma unroll
+            for (int gg = 0; gg < 2; gg++)
+#pragma unroll
+                for (int k = 0; k < 9; k++) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int q = 0; q < NP / 2; q++)
+                        a += stp[gg * (NP / 2) + q][k];
+                    ss.v[gg][k] = a;
+                }
+            int nz = 0;
+#pragma unroll
+            for (int q = 0; q < NTH / 64; q++)
+                nz += stz[q];
+            int azv;
+            sflag[0] = patch_accept<W, STYPE>(P, C, it, ss, nz, azv);
+            sflag[1] = azv;
+        }
         LF::fwd(x, h);
 #pragma unroll
         for (int k = 0; k < H; k++)
             F[FL::at(r, blk * H + k)] = x[k];
         __syncthreads();
+        // a pair that failed the acceptance test (have = 0, az = 0) runs on;
+        // nothing downstream reads its results (skipping here costs registers)
+        const int az = sflag[1];
         // ---------------- cross spectrum X = F1 conj(F2), row r, half h
         {
             const float2 *Fo = lds + (size_t)(1 - g) * BUF;
@@ -2624,6 +2782,10 @@ void init_dirichlet(st_ctx *ctx)
     for (int j = 0; j < 128; j++)
         t128[j] = make_float2((float)cos(2.0 * M_PI * j / 128.0), (float)sin(2.0 * M_PI * j / 128.0));
     ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fftreg::c_tw128), t128, sizeof(t128)));
+    double2 r256[256];
+    for (int j = 0; j < 256; j++)
+        r256[j] = make_double2(cos(2.0 * M_PI * j / 256.0), sin(2.0 * M_PI * j / 256.0));
+    ST_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_root256), r256, sizeof(r256)));
 
     done[ctx->device] = true;
 }
@@ -2639,13 +2801,13 @@ T *upload(st_ctx *ctx, const T *h, size_t n)
 #include "st_cpx_mfma.h"
 
 // Dynamic LDS of the fused kernel: the two transpose buffers, which later
-// hold the |C| surface (16 w^2 bytes) and then the refinement scratch.
+// hold the |C| surface (16 w^2 bytes).
 template <int W>
 constexpr size_t fused_lds_bytes()
 {
     constexpr size_t t = (size_t)2 * FusedLanes<W>::BUF * sizeof(float2);
     static_assert(t >= (size_t)16 * W * W, "surface does not fit the transpose buffers");
-    return t > (size_t)RefLayout<true>::BYTES ? t : (size_t)RefLayout<true>::BYTES;
+    return t;
 }
 
 template <int W, int STYPE, int BASE, int OCC>
@@ -2667,11 +2829,21 @@ void launch_fused_kernel_o(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS,
     ST_HIP(hipGetLastError());
 }
 
+// Minimum waves per SIMD each instantiation is compiled for: what its LDS
+// allows (w = 16: about 17 one-wave workgroups per CU; w = 32: 9 two-wave
+// ones; w = 64: 2 four-wave ones), or less where the phase corrections
+// need the registers.
+template <int W, int BASE>
+constexpr int fused_occ()
+{
+    return W == 16 ? 4 : W == 32 ? (BASE == 0 ? 4 : 3) : 2;
+}
+
 template <int W, int STYPE, int BASE>
 void launch_fused_kernel(st_ctx *ctx, int slot, const KPar *dP, const Pts *dS, const int *dStart, int njobs,
                          int total, const CpxBatch *dC, size_t lds)
 {
-    launch_fused_kernel_o<W, STYPE, BASE, 2>(ctx, slot, dP, dS, dStart, njobs, total, dC, lds);
+    launch_fused_kernel_o<W, STYPE, BASE, fused_occ<W, BASE>()>(ctx, slot, dP, dS, dStart, njobs, total, dC, lds);
 }
 
 // The complex attempts of one or more tracking calls of window class W.
@@ -2714,15 +2886,18 @@ template <int W, int STYPE>
 void launch_cpx_fused_t(st_ctx *ctx, int slot, const std::vector<FusedJob> &jobs, const KPar *dP, const Pts *dS,
                         const CpxBatch *dC, const int *dStart, int total, int base)
 {
-    const int prepSlot = ctx->kernel_slot("k_cpx_prep");
-    ctx->time_begin(prepSlot);
-    for (size_t j = 0; j < jobs.size(); j++) {
-        const int count = jobs[j].count;
-        constexpr int PPW = W < 64 ? 64 / W : 1;  // patches per prep wave
-        hipLaunchKernelGGL((k_cpx_prep_cols<W, STYPE>), dim3((count + PPW - 1) / PPW), dim3(64), 0,
-                           ctx->stream, dP + j, dS + j, jobs[j].C.list, count, dC + j);
+    if constexpr (W == kF128) {
+        // the window sums and acceptance test in their own kernel (the fused
+        // kernels for w <= 64 form them from the windows they load)
+        const int prepSlot = ctx->kernel_slot("k_cpx_prep");
+        ctx->time_begin(prepSlot);
+        for (size_t j = 0; j < jobs.size(); j++) {
+            const int count = jobs[j].count;
+            hipLaunchKernelGGL((k_cpx_prep_cols<W, STYPE>), dim3(count), dim3(64), 0, ctx->stream, dP + j, dS + j,
+                               jobs[j].C.list, count, dC + j);
+        }
+        ctx->time_end(prepSlot);
     }
-    ctx->time_end(prepSlot);
     const int n = (int)jobs.size();
     if constexpr (W == kF128) {
         (void)base;  // the w = 128 kernel takes no phase corrections (plan() sends those calls to the staged path)
@@ -2755,7 +2930,7 @@ int fused_per_cu_o(int stype)
 template <int W>
 int fused_per_cu(int stype)
 {
-    return fused_per_cu_o<W, 2>(stype);
+    return fused_per_cu_o<W, fused_occ<W, 0>()>(stype);
 }
 
 int fused_per_cu_w(int w, int stype)

@@ -115,6 +115,67 @@ struct PeakRes {
 // |Q|: |Q[u][v]| = |sum_m R_um sum_n b_{m+n} P[m][n] R_vn| with
 // R_um = R(u + 24 - 12 m) and b_k = exp(i pi k / 8), two real-by-complex
 // products instead of complex-by-complex ones.
+// exp(2 pi i j / 256), j = 0..255 (doppler_shift's roots of unity)
+extern __constant__ double2 c_root256[256];
+
+// The Doppler shift of the acceptance test (estDopCarrier1 per window,
+// dopplerEstimation.c:42-88, then speckleTrack.c's rounding):
+//   k = round(W (a0 + a1) / (4 pi)), a_g = atan2(zi_g, zr_g),
+// half away from zero, + W when negative, 0 when either lag sum is zero;
+// found without evaluating an angle.  Each a_g is split as m_g d + r_g on
+// the grid d = 2 pi / W (r_g in [0, d], m_g an integer) by a binary search
+// of sign tests against the roots of unity in tab; then
+// a0 + a1 = M d + (r0 + r1), and round((a0 + a1) / 2d) is (M + 1) / 2 for
+// odd M, and M / 2 or M / 2 + 1 for even M as r0 + r1 stays below d or
+// not (one more sign test).  Each test is exact up to the rounding of one
+// product, so the result differs from rounding the atan2 sum only for
+// angles within about 1e-16 of a rounding boundary.
+__host__ __device__ inline int doppler_shift(double zr0, double zi0, double zr1, double zi1, int W, const double2 *tab)
+{
+    if ((zr0 == 0.0 && zi0 == 0.0) || (zr1 == 0.0 && zi1 == 0.0))
+        return 0;
+    const int step = 256 / W;  // table stride of d
+    double pr[2], pi[2];
+    int M = 0;
+    for (int g = 0; g < 2; g++) {
+        const double zr = g ? zr1 : zr0, zi = g ? zi1 : zi0;
+        // a in [0, pi] (zi > 0 or +0), else a = -b, b in [0, pi]
+        const bool upper = zi > 0.0 || (zi == 0.0 && !__builtin_signbit(zi));
+        const double yi = upper ? zi : -zi;
+        // largest m in [0, W / 2] with m d <= b: b - m d in [0, pi] <=> cross(e_m, (zr, yi)) >= 0
+        int lo = 0, hi = W / 2;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            const double2 e = tab[mid * step];
+            if (e.x * yi - e.y * zr >= 0.0)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        // upper: m = lo, rotate by -m d; lower: m = -(lo + 1), rotate by +(lo + 1) d
+        const int j = upper ? lo : lo + 1;
+        const double2 e = tab[(j * step) & 255];
+        const double es = upper ? -e.y : e.y;
+        double rr = zr * e.x - zi * es, ri = zr * es + zi * e.x;
+        // scaled by a power of two (exact) so that the product below can
+        // neither underflow nor overflow
+        const int ex = ilogb(fmax(fabs(rr), fabs(ri)));
+        pr[g] = ldexp(rr, -ex);
+        pi[g] = ldexp(ri, -ex);
+        M += upper ? lo : -(lo + 1);
+    }
+    int k;
+    if (M & 1) {
+        k = (M + 1) / 2;
+    } else {
+        // r0 + r1 >= d <=> cross(e_1, rho0 rho1) >= 0 (r0 + r1 in [0, 2d] with 2d < pi)
+        const double qr = pr[0] * pr[1] - pi[0] * pi[1], qi = pr[0] * pi[1] + pi[0] * pr[1];
+        const double2 e1 = tab[step];
+        k = M / 2 + ((e1.x * qi - e1.y * qr >= 0.0) ? 1 : 0);
+    }
+    return k < 0 ? k + W : k;
+}
+
 // c_refR[u * 8 + m] = R_um (u < 48, m < 8); c_refB[k] = b_k (k < 15).
 extern __constant__ double c_refR[kFine * NFAST];
 extern __constant__ double2 c_refB[16];
