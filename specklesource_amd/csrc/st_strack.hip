@@ -39,6 +39,14 @@ namespace fftreg {
 __constant__ float2 c_tw128[128];
 }  // namespace fftreg
 
+// global-address-space views of pointers held in parameter blocks (loads
+// through them are global loads, not flat ones that also count against
+// the LDS wait counter)
+typedef __attribute__((address_space(1))) const char gchar;
+typedef __attribute__((address_space(1))) const float gfloat;
+typedef __attribute__((address_space(1))) const unsigned long long gu64;
+typedef __attribute__((address_space(1))) const unsigned gu32;
+
 namespace {
 
 // Kept bins of one point's interferogram spectrum (phaseCorrectInt): the
@@ -1112,24 +1120,53 @@ __device__ inline void lane_sums(const float2 (&x)[H], int h, float thr, double 
     }
 }
 
-// Butterfly sum over aligned groups of SPAN lanes (SPAN <= 64).
-template <int SPAN>
-__device__ inline void group_sum(double (&acc)[9])
+// Lane values moved by DPP control CTRL within the rows of row_mask RM
+// (lanes of other rows, and lanes whose source lies outside the row, get 0).
+template <int CTRL, int RM>
+__device__ inline double dpp_d(double v)
 {
-#pragma unroll
-    for (int o = 1; o < SPAN; o <<= 1)
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-            acc[k] += __shfl_xor(acc[k], o);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, RM, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, RM, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-template <int SPAN>
-__device__ inline int group_sum_i(int v)
+template <int CTRL, int RM>
+__device__ inline int dpp_i(int v)
 {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, RM, 0xF, true);
+}
+
+// Sum over aligned groups of SPAN (32 or 64) lanes, complete in the
+// group's last lane: prefix sums within each 16-lane row (row_shr 1, 2, 4,
+// 8), then row 0's total into row 1 and row 2's into row 3 (row_bcast:15),
+// and for SPAN = 64 lane 31's into rows 2 and 3 (row_bcast:31).
+template <int SPAN, class T, int N>
+__device__ inline void group_sum_last(T (&acc)[N])
+{
+    static_assert(SPAN == 32 || SPAN == 64, "group span");
+    auto step = [&](auto f) {
 #pragma unroll
-    for (int o = 1; o < SPAN; o <<= 1)
-        v += __shfl_xor(v, o);
-    return v;
+        for (int k = 0; k < N; k++)
+            acc[k] += f(acc[k]);
+    };
+    if constexpr (sizeof(T) == 8) {
+        step([](T v) { return dpp_d<0x111, 0xF>(v); });
+        step([](T v) { return dpp_d<0x112, 0xF>(v); });
+        step([](T v) { return dpp_d<0x114, 0xF>(v); });
+        step([](T v) { return dpp_d<0x118, 0xF>(v); });
+        step([](T v) { return dpp_d<0x142, 0xA>(v); });
+        if constexpr (SPAN == 64)
+            step([](T v) { return dpp_d<0x143, 0xC>(v); });
+    } else {
+        step([](T v) { return dpp_i<0x111, 0xF>(v); });
+        step([](T v) { return dpp_i<0x112, 0xF>(v); });
+        step([](T v) { return dpp_i<0x114, 0xF>(v); });
+        step([](T v) { return dpp_i<0x118, 0xF>(v); });
+        step([](T v) { return dpp_i<0x142, 0xA>(v); });
+        if constexpr (SPAN == 64)
+            step([](T v) { return dpp_i<0x143, 0xC>(v); });
+    }
 }
 
 // Per-patch statistics of getPatches (speckleTrack.c:1194-1330) with one
@@ -1142,8 +1179,16 @@ __device__ inline int group_sum_i(int v)
 //   sum (x1 - m) conj(x0 - m) = sum x1 conj(x0) - conj(m) sum x1
 //                               - m sum conj(x0) + (W - 1) W |m|^2
 // with the first / last row sums taken out of the totals.
+// Row segments of the prep kernel (one wave each): at w = 128 a single
+// wave per patch leaves too few waves in flight to cover the loads.
+template <int W>
+constexpr int prep_segs()
+{
+    return W == 128 ? 4 : 1;
+}
+
 template <int W, int STYPE>
-__global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
+__global__ void __launch_bounds__(64 * prep_segs<W>()) k_cpx_prep_cols(const KPar *__restrict__ Pp, const Pts *__restrict__ Sp,
                                                       const int *list, int count, const CpxBatch *__restrict__ Cp)
 {
     const KPar &P = *Pp;
@@ -1151,9 +1196,12 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
     const CpxBatch &C = *Cp;
     // lanes per patch, patches per wave, columns per lane
     constexpr int LPP = W < 64 ? W : 64, PPW = 64 / LPP, CPL = W / LPP;
+    // row segments of RS rows, one wave each (the wave of segment g > 0
+    // starts its lag sum from the last row of segment g - 1)
+    constexpr int SEG = prep_segs<W>(), RS = W / SEG;
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
     const float thr = isf ? 1e-15f : 1e-9f;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
     // consecutive items (overlapping patches) on one XCD, so their shared
     // rows hit that XCD's L2: workgroups are dealt round-robin over the 8 XCDs
     const int nb = gridDim.x, per = nb / 8, b = blockIdx.x;
@@ -1170,6 +1218,29 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
         const bool useH = P.hanning != 0;
         double cps[CPL][2], cT[CPL][2][2], cLg[CPL][2][2];
         float2 prev[CPL][2];
+        // sample (a, c) of window g, scaled and tapered as the transforms take it
+        auto sample = [&](int g, int a, int c) {
+#pragma clang fp contract(off)
+            const void *img = g ? P.im.img2 : P.im.img1;
+            const int nr = g ? P.im.nr2 : P.im.nr1;
+            const size_t o = (size_t)((g ? a2 : a1) + a) * nr + (g ? r2 : r1) + c;
+            float2 x;
+            if constexpr (isf) {
+                x = __builtin_bit_cast(float2, ((const gu64 *)img)[o]);
+                x.x *= 1000;
+                x.y *= 1000;
+            } else {
+                const unsigned q = ((const gu32 *)img)[o];
+                x = make_float2((float)(short)(q & 0xffffu), (float)(short)(q >> 16));
+            }
+            if (g == 1 && useH) {
+                const float hw = ((const gfloat *)P.hann)[a * W + c];
+                x.x *= hw;
+                x.y *= hw;
+            }
+            return x;
+        };
+        const int a0 = seg * RS;
 #pragma unroll
         for (int cc = 0; cc < CPL; cc++)
 #pragma unroll
@@ -1177,10 +1248,10 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
                 cps[cc][g] = 0;
                 cT[cc][g][0] = cT[cc][g][1] = 0;
                 cLg[cc][g][0] = cLg[cc][g][1] = 0;
-                prev[cc][g] = make_float2(0.f, 0.f);
+                prev[cc][g] = a0 > 0 ? sample(g, a0 - 1, c0 + cc * LPP) : make_float2(0.f, 0.f);
             }
 #pragma unroll(CPL > 1 ? 2 : 4)
-        for (int a = 0; a < W; a++) {
+        for (int a = a0; a < a0 + RS; a++) {
 #pragma unroll
             for (int cc = 0; cc < CPL; cc++) {
 #pragma clang fp contract(off)
@@ -1188,23 +1259,7 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
                 float pw[2];
 #pragma unroll
                 for (int g = 0; g < 2; g++) {
-                    const void *img = g ? P.im.img2 : P.im.img1;
-                    const int nr = g ? P.im.nr2 : P.im.nr1;
-                    const size_t o = (size_t)((g ? a2 : a1) + a) * nr + (g ? r2 : r1) + c;
-                    float2 x;
-                    if constexpr (isf) {
-                        x = ((const float2 *)img)[o];
-                        x.x *= 1000;
-                        x.y *= 1000;
-                    } else {
-                        const short2 q = ((const short2 *)img)[o];
-                        x = make_float2((float)q.x, (float)q.y);
-                    }
-                    if (g == 1 && useH) {
-                        const float hw = P.hann[a * W + c];
-                        x.x *= hw;
-                        x.y *= hw;
-                    }
+                    const float2 x = sample(g, a, c);
                     pw[g] = x.x * x.x + x.y * x.y;
                     cps[cc][g] += pw[g];
                     cT[cc][g][0] += x.x;
@@ -1253,6 +1308,40 @@ __global__ void __launch_bounds__(64) k_cpx_prep_cols(const KPar *__restrict__ P
             }
         }
         nz += __shfl_xor(nz, o);
+    }
+    if constexpr (SEG > 1) {
+        // the segments' sums, added in segment order by wave 0 (PPW = 1 here)
+        static_assert(PPW == 1, "one patch per wave when rows are split");
+        __shared__ double segs[SEG][19];
+        if (c0 == 0) {
+            double *d = segs[seg];
+            for (int g = 0; g < 2; g++) {
+                d[g * 9] = ps[g];
+                for (int k = 0; k < 2; k++) {
+                    d[g * 9 + 1 + k] = T[g][k];
+                    d[g * 9 + 3 + k] = Lg[g][k];
+                    d[g * 9 + 5 + k] = F0[g][k];
+                    d[g * 9 + 7 + k] = FL[g][k];
+                }
+            }
+            d[18] = nz;
+        }
+        __syncthreads();
+        if (seg != 0 || c0 != 0 || !act)
+            return;
+        for (int q = 1; q < SEG; q++) {
+            const double *d = segs[q];
+            for (int g = 0; g < 2; g++) {
+                ps[g] += d[g * 9];
+                for (int k = 0; k < 2; k++) {
+                    T[g][k] += d[g * 9 + 1 + k];
+                    Lg[g][k] += d[g * 9 + 3 + k];
+                    F0[g][k] += d[g * 9 + 5 + k];
+                    FL[g][k] += d[g * 9 + 7 + k];
+                }
+            }
+            nz += (int)d[18];
+        }
     }
     if (!act || c0 != 0)
         return;
@@ -2027,14 +2116,6 @@ __global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int W
     }
 }
 
-// global-address-space views of pointers held in parameter blocks (loads
-// through them are global loads, not flat ones that also count against
-// the LDS wait counter)
-typedef __attribute__((address_space(1))) const char gchar;
-typedef __attribute__((address_space(1))) const float gfloat;
-typedef __attribute__((address_space(1))) const unsigned long long gu64;
-typedef __attribute__((address_space(1))) const unsigned gu32;
-
 // Lanes per transform of the fused kernel.  Lane pairs at every window
 // size: lane quads at w = 64 (16 points per lane, 512-thread blocks) were
 // measured slower on the full-scene job (55.6 ms vs 43.6 ms per launch) --
@@ -2157,8 +2238,8 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
             double acc[9];
             unsigned mk;
             lane_sums<H, L>(x, h, thr, acc, mk);
-            group_sum<SPAN>(acc);
-            if ((t & (SPAN - 1)) == 0) {
+            group_sum_last<SPAN>(acc);
+            if ((t & (SPAN - 1)) == SPAN - 1) {
 #pragma unroll
                 for (int k = 0; k < 9; k++)
                     stp[t / SPAN][k] = acc[k];
@@ -2221,10 +2302,10 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         {
             // samples near zero in either window (thread t + 2w holds the
             // same samples of window 2)
-            const int nzl = g == 0 ? __popc(smask[t] | smask[t + 2 * W]) : 0;
-            const int nzw = group_sum_i<64>(nzl);
-            if ((t & 63) == 0)
-                stz[t >> 6] = nzw;
+            int nzl[1] = {g == 0 ? (int)__popc(smask[t] | smask[t + 2 * W]) : 0};
+            group_sum_last<64>(nzl);
+            if ((t & 63) == 63)
+                stz[t >> 6] = nzl[0];
         }
         // ---------------- forward FFT along range (row r)
 #pragma unroll
@@ -2893,8 +2974,8 @@ void launch_cpx_fused_t(st_ctx *ctx, int slot, const std::vector<FusedJob> &jobs
         ctx->time_begin(prepSlot);
         for (size_t j = 0; j < jobs.size(); j++) {
             const int count = jobs[j].count;
-            hipLaunchKernelGGL((k_cpx_prep_cols<W, STYPE>), dim3(count), dim3(64), 0, ctx->stream, dP + j, dS + j,
-                               jobs[j].C.list, count, dC + j);
+            hipLaunchKernelGGL((k_cpx_prep_cols<W, STYPE>), dim3(count), dim3(64 * prep_segs<W>()), 0, ctx->stream,
+                               dP + j, dS + j, jobs[j].C.list, count, dC + j);
         }
         ctx->time_end(prepSlot);
     }
