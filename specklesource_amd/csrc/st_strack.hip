@@ -2041,21 +2041,14 @@ __global__ void k_amp_psum(const double2 *ppart, int nPf, int B, int WA, int WR,
 // Sub-pixel refinement and Gaussian-fit samples of the fused path's
 // attempts (overSampleC and the central-box argmax, speckleTrack.c:686-700,
 // 852-863; the samples of complexMatch's fits, :377-386), one wave per
-// attempt and four per workgroup, from the kNb x kNb |C| block that
-// k_cpx_fused left around each coarse peak.  Kept out of the transform
-// kernel, whose LDS and barriers would otherwise hold its workgroups
-// through this latency-bound stretch.
+// attempt and four per workgroup, from the block cells k_cpx_fused left
+// around each coarse peak.  The product runs in registers (refine_cols):
+// the 8 x 8 box is read by every lane alike and the Dirichlet rows come as
+// scalars, so no LDS is involved.
 __global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int WW)
 {
-    __shared__ double Rt[kFine * NFAST];
-    __shared__ double2 Pc[4][NFAST * NFAST];
-    __shared__ double2 Tq[4][NFAST * kFine];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int e = threadIdx.x; e < kFine * NFAST; e += 256)
-        Rt[e] = c_refR[e];
-    __syncthreads();
-    // from here on each wave works alone (no workgroup barrier follows)
-    const int it = blockIdx.x * 4 + wv;
+    const int lane = threadIdx.x & 63;
+    const int it = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (it >= count || !C.have[it])
         return;
     PeakRes *res = &C.res[it];
@@ -2063,21 +2056,13 @@ __global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int W
     if (I < -999)
         return;
     const float *nb = C.nb + (size_t)it * kNbc;
-    auto wsync = [] {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
-    {
-        const int m = lane / NFAST, n = lane % NFAST;
-        const double pm = (double)nb[kNbMap.off[(kNb / 2 - NFAST / 2 + m) * kNb + (kNb / 2 - NFAST / 2 + n)]];
-        const double2 b = c_refB[m + n];
-        Pc[wv][lane] = make_double2(b.x * pm, b.y * pm);
-    }
-    wsync();
     double f1, f2;
     int fr;
-    refine_core<64, 3, 2>(Pc[wv], Tq[wv], Rt, lane, wsync, f1, f2, fr);
+    refine_cols(
+        [&](int e) {
+            return (double)nb[kNbMap.off[(kNb / 2 - NFAST / 2 + e / NFAST) * kNb + (kNb / 2 - NFAST / 2 + e % NFAST)]];
+        },
+        lane, f1, f2, fr);
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const double b1 = __shfl_xor(f1, o), b2 = __shfl_xor(f2, o);

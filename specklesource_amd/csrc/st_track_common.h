@@ -304,6 +304,61 @@ __device__ inline void refine_core(const double2 *Pc, double2 *Tq, const double 
     }
 }
 
+// refine_core's product for one wave and one attempt without LDS: lane
+// v < kFine owns column v of the search box.  It forms T[m][v] (m < 8) in
+// registers from the 8 x 8 box P (pm(e) = P[e / 8][e % 8], read by every
+// lane alike) and its own row of R, then walks the rows u of the box with
+// R[u][m] uniform across the wave (scalar loads), keeping its best and
+// runner-up.  The products and their order are refine_core's, so each
+// |Q|^2 is the same number.  Lanes v >= kFine leave with nothing found.
+template <class PM>
+__device__ inline void refine_cols(PM pm, int lane, double &f1, double &f2, int &fr)
+{
+    f1 = 0.0;
+    f2 = 0.0;
+    fr = 0x7fffffff;
+    if (lane >= kFine)
+        return;
+    const int v = lane;
+    double tr[NFAST], ti[NFAST];
+    {
+        double rv[NFAST];
+#pragma unroll
+        for (int n = 0; n < NFAST; n++)
+            rv[n] = c_refR[v * NFAST + n];
+#pragma unroll
+        for (int m = 0; m < NFAST; m++) {
+            double re = 0, im = 0;
+#pragma unroll
+            for (int n = 0; n < NFAST; n++) {
+                const double2 b = c_refB[m + n];
+                const double p = pm(m * NFAST + n);
+                const double2 pc = make_double2(b.x * p, b.y * p);
+                re = fma(pc.x, rv[n], re);
+                im = fma(pc.y, rv[n], im);
+            }
+            tr[m] = re;
+            ti[m] = im;
+        }
+    }
+#pragma unroll 2
+    for (int u = 0; u < kFine; u++) {
+        double ar = 0.0, ai = 0.0;
+#pragma unroll
+        for (int m = 0; m < NFAST; m++) {
+            const double r = c_refR[u * NFAST + m];
+            ar = fma(r, tr[m], ar);
+            ai = fma(r, ti[m], ai);
+        }
+        const double q = ar * ar + ai * ai;  // |Q|^2: refine_core's expression
+        // ranks u kFine + v rise with u, so a later equal value never wins
+        const bool up = q > f1;
+        f2 = fmax(f2, up ? f1 : q);
+        fr = up ? u * kFine + v : fr;
+        f1 = up ? q : f1;
+    }
+}
+
 // The whole refinement for a caller-written P at scr + kRefPm (layout
 // RefLayout<RLDS>): every thread leaves with its own best (value f1, rank
 // fr, smaller rank first among equal values) and runner-up value f2, for
