@@ -1063,7 +1063,7 @@ __device__ inline int patch_accept(const KPar &P, const CpxBatch &C, int it, con
                 (double)(W - 1) * (double)W * (mr * mr + mi * mi);
         zi[g] = Lg[1] - (mr * s1i - mi * s1r) - (mi * s0r - mr * s0i);
     }
-    const int shift = doppler_shift(zr[0], zi[0], zr[1], zi[1], W, c_root256);
+    const int shift = doppler_round(zr[0], zi[0], zr[1], zi[1], W, c_root256);
     az = (int)((float)shift * (float)P.wA / (float)W);
     C.have[it] = 1;
     C.az[it] = az;

@@ -115,7 +115,7 @@ struct PeakRes {
 // |Q|: |Q[u][v]| = |sum_m R_um sum_n b_{m+n} P[m][n] R_vn| with
 // R_um = R(u + 24 - 12 m) and b_k = exp(i pi k / 8), two real-by-complex
 // products instead of complex-by-complex ones.
-// exp(2 pi i j / 256), j = 0..255 (doppler_shift's roots of unity)
+// exp(2 pi i j / 256), j = 0..255 (doppler_round's roots of unity)
 extern __constant__ double2 c_root256[256];
 
 // The Doppler shift of the acceptance test (estDopCarrier1 per window,
@@ -130,7 +130,7 @@ extern __constant__ double2 c_root256[256];
 // not (one more sign test).  Each test is exact up to the rounding of one
 // product, so the result differs from rounding the atan2 sum only for
 // angles within about 1e-16 of a rounding boundary.
-__host__ __device__ inline int doppler_shift(double zr0, double zi0, double zr1, double zi1, int W, const double2 *tab)
+__host__ __device__ inline int doppler_round(double zr0, double zi0, double zr1, double zi1, int W, const double2 *tab)
 {
     if ((zr0 == 0.0 && zi0 == 0.0) || (zr1 == 0.0 && zi1 == 0.0))
         return 0;

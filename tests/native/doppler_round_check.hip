@@ -1,4 +1,4 @@
-// Host check of doppler_shift (st_track_common.h) against the rounding it
+// Host check of doppler_round (st_track_common.h) against the rounding it
 // replaces: round(W (atan2(zi0, zr0) + atan2(zi1, zr1)) / (4 pi)), half
 // away from zero, + W when negative, 0 for a zero lag sum
 // (dopplerEstimation.c:42-88 per window, speckleTrack.c's rounding).
@@ -39,7 +39,7 @@ int main()
         const int W = Ws[it & 3];
         const double s0 = std::pow(10.0, (int)(rng() % 40) - 20), s1 = std::pow(10.0, (int)(rng() % 40) - 20);
         const double a = nd(rng) * s0, b = nd(rng) * s0, c = nd(rng) * s1, d = nd(rng) * s1;
-        bad1 += doppler_shift(a, b, c, d, W, tab) != by_atan2(a, b, c, d, W);
+        bad1 += doppler_round(a, b, c, d, W, tab) != by_atan2(a, b, c, d, W);
         n1++;
     }
     // signed zeros, axes, extreme magnitudes
@@ -50,7 +50,7 @@ int main()
             for (double b : sp)
                 for (double c : sp)
                     for (double d : sp) {
-                        bad2 += doppler_shift(a, b, c, d, W, tab) != by_atan2(a, b, c, d, W);
+                        bad2 += doppler_round(a, b, c, d, W, tab) != by_atan2(a, b, c, d, W);
                         n2++;
                     }
     long bad3 = 0, n3 = 0;
@@ -64,7 +64,7 @@ int main()
                 if (a0 <= -M_PI || a0 > M_PI || a1 <= -M_PI || a1 > M_PI)
                     continue;
                 const double zr0 = 3.3 * cos(a0), zi0 = 3.3 * sin(a0), zr1 = 0.7 * cos(a1), zi1 = 0.7 * sin(a1);
-                bad3 += doppler_shift(zr0, zi0, zr1, zi1, W, tab) != by_atan2(zr0, zi0, zr1, zi1, W);
+                bad3 += doppler_round(zr0, zi0, zr1, zi1, W, tab) != by_atan2(zr0, zi0, zr1, zi1, W);
                 n3++;
             }
     printf("random %ld %ld special %ld %ld near %ld %ld\n", bad1, n1, bad2, n2, bad3, n3);

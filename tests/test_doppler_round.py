@@ -1,4 +1,4 @@
-"""doppler_shift (the acceptance test's Doppler rounding by sign tests
+"""doppler_round (the acceptance test's Doppler rounding by sign tests
 against roots of unity, st_track_common.h) against the atan2 rounding of
 estDopCarrier1 (dopplerEstimation.c:42-88): random lag sums over 40 decades
 of magnitude, signed zeros / axes / extreme magnitudes, and angle sums
@@ -15,9 +15,9 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
-def test_doppler_shift_matches_atan2_rounding(tmp_path):
-    exe = str(tmp_path / "doppler_shift_check")
-    src = os.path.join(HERE, "native", "doppler_shift_check.hip")
+def test_doppler_round_matches_atan2_rounding(tmp_path):
+    exe = str(tmp_path / "doppler_round_check")
+    src = os.path.join(HERE, "native", "doppler_round_check.hip")
     inc = os.path.join(ROOT, "specklesource_amd", "csrc")
     subprocess.run([HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950", "-I", inc, "-I",
                     os.path.join(ROOT, "include"), src, "-o", exe], check=True, capture_output=True)
