@@ -2040,40 +2040,67 @@ __global__ void k_amp_psum(const double2 *ppart, int nPf, int B, int WA, int WR,
 
 // Sub-pixel refinement and Gaussian-fit samples of the fused path's
 // attempts (overSampleC and the central-box argmax, speckleTrack.c:686-700,
-// 852-863; the samples of complexMatch's fits, :377-386), one wave per
-// attempt and four per workgroup, from the block cells k_cpx_fused left
-// around each coarse peak.  The product runs in registers (refine_cols):
-// the 8 x 8 box is read by every lane alike and the Dirichlet rows come as
-// scalars, so no LDS is involved.
-__global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int WW)
+// 852-863; the samples of complexMatch's fits, :377-386), from the block
+// cells k_cpx_fused left around each coarse peak.  Four attempts per
+// workgroup of 192 threads: thread 48 p + v owns column v of attempt p's
+// 48 x 48 search box (refine_cols), so every lane of the three waves works
+// (one attempt per wave would leave 16 lanes idle); the Dirichlet rows come
+// as scalars, the 8 x 8 boxes (times b) through LDS, and wave 0 merges each
+// attempt's 48 column results and writes them.
+constexpr int kRefPer = 4, kRefThreads = kRefPer * kFine;
+__global__ void __launch_bounds__(kRefThreads) k_cpx_refine(CpxBatch C, int count, int WW)
 {
-    const int lane = threadIdx.x & 63;
-    const int it = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (it >= count || !C.have[it])
+    __shared__ double2 Pc[kRefPer][NFAST * NFAST];
+    __shared__ double s1[kRefThreads], s2[kRefThreads];
+    __shared__ int sr[kRefThreads];
+    const int tid = threadIdx.x, base = blockIdx.x * kRefPer;
+    // an attempt is refined when the fused kernel accepted it and found a coarse peak
+    auto live = [&](int it) { return it < count && C.have[it] && C.res[it].I >= -999; };
+    for (int e2 = tid; e2 < kRefPer * NFAST * NFAST; e2 += kRefThreads) {
+        const int pp = e2 / (NFAST * NFAST), e = e2 % (NFAST * NFAST), it = base + pp;
+        double2 c = make_double2(0.0, 0.0);
+        if (live(it)) {
+            const int m = e / NFAST, n = e % NFAST;
+            const double pm = (double)C.nb[(size_t)it * kNbc +
+                                           kNbMap.off[(kNb / 2 - NFAST / 2 + m) * kNb + (kNb / 2 - NFAST / 2 + n)]];
+            const double2 b = c_refB[m + n];
+            c = make_double2(b.x * pm, b.y * pm);
+        }
+        Pc[pp][e] = c;
+    }
+    __syncthreads();
+    {
+        const int p = tid / kFine, v = tid % kFine;
+        double f1, f2;
+        int fr;
+        refine_cols([&](int e) { return Pc[p][e]; }, v, f1, f2, fr);
+        s1[tid] = f1;
+        s2[tid] = f2;
+        sr[tid] = fr;
+    }
+    __syncthreads();
+    if (tid >= 64)
+        return;
+    // wave 0: 16 lanes per attempt, three columns each, then a butterfly over the 16
+    const int pp = tid >> 4, l = tid & 15, it = base + pp;
+    double f1 = 0.0, f2 = 0.0;
+    int fr = 0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < kFine / 16; k++) {
+        const int c = pp * kFine + l * (kFine / 16) + k;
+        top2_merge(f1, f2, fr, s1[c], s2[c], sr[c]);
+    }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        const double b1 = __shfl_xor(f1, o), b2 = __shfl_xor(f2, o);
+        const int br = __shfl_xor(fr, o);
+        top2_merge(f1, f2, fr, b1, b2, br);
+    }
+    if (!live(it))
         return;
     PeakRes *res = &C.res[it];
     const int I = res->I, J = res->J;
-    if (I < -999)
-        return;
     const float *nb = C.nb + (size_t)it * kNbc;
-    double f1, f2;
-    int fr;
-    refine_cols(
-        [&](int e) {
-            return (double)nb[kNbMap.off[(kNb / 2 - NFAST / 2 + e / NFAST) * kNb + (kNb / 2 - NFAST / 2 + e % NFAST)]];
-        },
-        lane, f1, f2, fr);
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const double b1 = __shfl_xor(f1, o), b2 = __shfl_xor(f2, o);
-        const int br = __shfl_xor(fr, o);
-        if (b1 > f1 || (b1 == f1 && br < fr)) {
-            f2 = fmax(f1, b2);
-            f1 = b1;
-            fr = br;
-        } else
-            f2 = fmax(f2, b1);
-    }
     int fi = 0, fj = 0;
     if (fr != 0x7fffffff) {
         fi = fr / kFine;
@@ -2084,7 +2111,7 @@ __global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int W
     const int cI = iF / NOVER, cJ = jF / NOVER;
     const bool ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < WW && (cJ - NGAUSS / 2) >= 0 &&
                     (cJ + NGAUSS / 2) < WW;
-    if (lane == 0) {
+    if (l == 0) {
         res->fi = fi;
         res->fj = fj;
         res->fineMax = fr != 0x7fffffff ? sqrt(f1) : 0.0;
@@ -2093,11 +2120,14 @@ __global__ void __launch_bounds__(256) k_cpx_refine(CpxBatch C, int count, int W
         res->lmX0[0] = (float)(cJ - NGAUSS / 2);
         res->lmX0[1] = (float)(cI - NGAUSS / 2);
     }
-    if (ok && lane < 2 * NGAUSS) {
+    if (ok) {
         // cI, cJ within 2 samples of (I, J): inside the block
-        const int k = lane % NGAUSS, bi = cI - I + kNb / 2, bj = cJ - J + kNb / 2;
-        res->lmY[lane / NGAUSS][k] = lane < NGAUSS ? nb[kNbMap.off[bi * kNb + (bj - NGAUSS / 2 + k)]]
-                                                   : nb[kNbMap.off[(bi - NGAUSS / 2 + k) * kNb + bj]];
+        const int bi = cI - I + kNb / 2, bj = cJ - J + kNb / 2;
+        for (int q = l; q < 2 * NGAUSS; q += 16) {
+            const int k = q % NGAUSS;
+            res->lmY[q / NGAUSS][k] = q < NGAUSS ? nb[kNbMap.off[bi * kNb + (bj - NGAUSS / 2 + k)]]
+                                                 : nb[kNbMap.off[(bi - NGAUSS / 2 + k) * kNb + bj]];
+        }
     }
 }
 
@@ -3577,7 +3607,8 @@ struct StrackJob {
         hipStream_t st = ctx->stream;
         const int kRef = ctx->kernel_slot("k_cpx_refine");
         ctx->time_begin(kRef);
-        hipLaunchKernelGGL(k_cpx_refine, dim3((nCpx + 3) / 4), dim3(256), 0, st, C, nCpx, 2 * w);
+        hipLaunchKernelGGL(k_cpx_refine, dim3((nCpx + kRefPer - 1) / kRefPer), dim3(kRefThreads), 0, st, C, nCpx,
+                           2 * w);
         ctx->time_end(kRef);
         hipLaunchKernelGGL(k_lm, dim3((2 * nCpx + 127) / 128), dim3(128), 0, st, C.res, nCpx, C.sig);
         hipLaunchKernelGGL(k_cpx_finalize, dim3((nCpx + 127) / 128), dim3(128), 0, st, P, S, list, 0, nCpx, C);

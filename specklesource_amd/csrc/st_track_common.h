@@ -304,22 +304,18 @@ __device__ inline void refine_core(const double2 *Pc, double2 *Tq, const double 
     }
 }
 
-// refine_core's product for one wave and one attempt without LDS: lane
-// v < kFine owns column v of the search box.  It forms T[m][v] (m < 8) in
-// registers from the 8 x 8 box P (pm(e) = P[e / 8][e % 8], read by every
-// lane alike) and its own row of R, then walks the rows u of the box with
-// R[u][m] uniform across the wave (scalar loads), keeping its best and
-// runner-up.  The products and their order are refine_core's, so each
-// |Q|^2 is the same number.  Lanes v >= kFine leave with nothing found.
-template <class PM>
-__device__ inline void refine_cols(PM pm, int lane, double &f1, double &f2, int &fr)
+// refine_core's product without the LDS tiles: the calling lane owns
+// column v < kFine of one attempt's search box.  It forms T[m][v] (m < 8)
+// in registers from pc(e) = (b P)[e / 8][e % 8] and its own row of R, then
+// walks the rows u of the box with R[u][m] uniform across the workgroup
+// (scalar loads), keeping its best and runner-up.  The products and their
+// order are refine_core's, so each |Q|^2 is the same number.
+template <class PC>
+__device__ inline void refine_cols(PC pc, int v, double &f1, double &f2, int &fr)
 {
     f1 = 0.0;
     f2 = 0.0;
     fr = 0x7fffffff;
-    if (lane >= kFine)
-        return;
-    const int v = lane;
     double tr[NFAST], ti[NFAST];
     {
         double rv[NFAST];
@@ -331,11 +327,9 @@ __device__ inline void refine_cols(PM pm, int lane, double &f1, double &f2, int 
             double re = 0, im = 0;
 #pragma unroll
             for (int n = 0; n < NFAST; n++) {
-                const double2 b = c_refB[m + n];
-                const double p = pm(m * NFAST + n);
-                const double2 pc = make_double2(b.x * p, b.y * p);
-                re = fma(pc.x, rv[n], re);
-                im = fma(pc.y, rv[n], im);
+                const double2 c = pc(m * NFAST + n);
+                re = fma(c.x, rv[n], re);
+                im = fma(c.y, rv[n], im);
             }
             tr[m] = re;
             ti[m] = im;
@@ -356,6 +350,19 @@ __device__ inline void refine_cols(PM pm, int lane, double &f1, double &f2, int 
         f2 = fmax(f2, up ? f1 : q);
         fr = up ? u * kFine + v : fr;
         f1 = up ? q : f1;
+    }
+}
+
+// Best / runner-up merge: the larger value wins, the smaller rank among
+// equal values; the runner-up is the larger of everything else.
+__device__ inline void top2_merge(double &f1, double &f2, int &fr, double b1, double b2, int br)
+{
+    if (b1 > f1 || (b1 == f1 && br < fr)) {
+        f2 = fmax(f1, b2);
+        f1 = b1;
+        fr = br;
+    } else {
+        f2 = fmax(f2, b1);
     }
 }
 
