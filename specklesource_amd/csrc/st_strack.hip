@@ -1025,6 +1025,63 @@ __device__ inline Top2 wg_top2(Top2 a, Top2 *sh)
     }
 }
 
+// The peak scan without ranks: each thread keeps the largest value, the
+// runner-up (the larger of all others) and the sum; the merge gives the
+// same three numbers as top2_combine, in the same order of additions.
+__device__ inline Top2 max2_combine(Top2 a, Top2 b)
+{
+    Top2 o;
+    o.sum = a.sum + b.sum;
+    o.v1 = fmaxf(a.v1, b.v1);
+    o.v2 = fmaxf(fmaxf(a.v2, b.v2), fminf(a.v1, b.v1));
+    o.r1 = 0x7fffffff;
+    return o;
+}
+
+__device__ inline void max2_add(Top2 &a, float v)
+{
+    a.sum = a.sum + v;
+    a.v2 = fmaxf(a.v2, fminf(a.v1, v));
+    a.v1 = fmaxf(a.v1, v);
+}
+
+template <int NTH>
+__device__ inline Top2 wg_max2(Top2 a, Top2 *sh)
+{
+#pragma unroll
+    for (int o = 1; o < (NTH < 64 ? NTH : 64); o <<= 1) {
+        Top2 b;
+        b.v1 = __shfl_xor(a.v1, o);
+        b.v2 = __shfl_xor(a.v2, o);
+        b.sum = __shfl_xor(a.sum, o);
+        a = max2_combine(a, b);
+    }
+    if constexpr (NTH <= 64) {
+        return a;
+    } else {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        __syncthreads();
+        if (lane == 0)
+            sh[wv] = a;
+        __syncthreads();
+        Top2 r = sh[0];
+        for (int k = 1; k < NTH / 64; k++)
+            r = max2_combine(r, sh[k]);
+        __syncthreads();
+        return r;
+    }
+}
+
+// findCPeak's rank of a surface cell (I, J) of the 2w x 2w fftshifted
+// surface: row-major within the unshifted quadrant, quadrant order 0-3.
+template <int W>
+__device__ inline int surf_rank(int I, int J)
+{
+    const int i0 = I < W ? I : I - W, j0 = J < W ? J : J - W;
+    const int q = (I < W) ? (J < W ? 0 : 3) : (J < W ? 2 : 1);
+    return (i0 * W + j0) * 4 + q;
+}
+
 // Sums of getPatches (speckleTrack.c:1194-1330) and estDopCarrier1
 // (dopplerEstimation.c:42-88) over one window: [0] sum |x|^2, [1..2] sum x,
 // [3..4] the lag-1 sum along azimuth sum x[a] conj(x[a - 1]), [5..6] the
@@ -2180,6 +2237,7 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
     __shared__ double stp[NP][9];
     __shared__ int stz[NTH / 64], sflag[2];
     __shared__ unsigned smask[NTH];
+    __shared__ int srank;  // rank of the surface maximum (atomicMin over its holders)
     constexpr bool isf = STYPE == ST_SAMPLE_C64;
     const float thr = isf ? 1e-15f : 1e-9f;
 
@@ -2370,6 +2428,8 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         const int krow = ((r - az) % W + W) % W;
         float2 *Z = lds + (size_t)g * BUF;
         Top2 best{0.0f, 0x7fffffff, 0.0f, 0.0f};
+        if (t == 0)
+            srank = 0x7fffffff;  // the previous item's last reads of it are behind barriers
         // exp(i pi (ks pp + ls q) / w) = row factor (ks: signed frequency of
         // the row, pp) x column factor (per element, q = g only).  The column
         // factor is applied before the row transform; the row factor is
@@ -2404,19 +2464,10 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
 #pragma unroll
             for (int jj = 0; jj < H; jj++) {
 #pragma clang fp contract(off)
-                const int I = (2 * (L * jj + h) + pp + W) & (WW - 1);
                 const float x2 = y[jj].x * y[jj].x + y[jj].y * y[jj].y;
                 const float v = sqrtf(x2);
                 mg[jj] = v;
-                const int i0 = I < W ? I : I - W, j0 = Jc < W ? Jc : Jc - W;
-                const int q = (I < W) ? (Jc < W ? 0 : 3) : (Jc < W ? 2 : 1);
-                const int rank = (i0 * W + j0) * 4 + q;
-                if (v > 0.0f) {
-                    Top2 c{v, rank, 0.0f, v};
-                    best = top2_combine(best, c);
-                } else {
-                    best.v2 = fmaxf(best.v2, v);
-                }
+                max2_add(best, v);
             }
         };
         {
@@ -2426,6 +2477,9 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
                 y[k] = Z[FL::at(blk * H + k, r)];
             LF::inv(y, h);
             scan(y, 0, mg0);
+            // keep the second component's loads behind this scan (hoisted
+            // above it, both components' values would be live at once)
+            __builtin_amdgcn_sched_barrier(0);
         }
         {
             float2 y[H];
@@ -2455,8 +2509,24 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
                 Ms[((n + 1 + W) & (WW - 1)) * WW + Jc] = mg1[jj];
             }
         }
-        best = wg_top2<NTH>(best, sht);
-        __syncthreads();
+        {
+            // the maximum's rank: its holders (normally one thread) look it
+            // up among their own cells of the LDS surface
+            const float mine = best.v1;
+            best = wg_max2<NTH>(best, sht);
+            if (best.v1 > 0.0f && mine == best.v1) {
+                const float *Ms = (const float *)lds;
+                int rk = 0x7fffffff;
+                for (int e = 0; e < 2 * H; e++) {
+                    const int I = (2 * (L * (e >> 1) + h) + (e & 1) + W) & (WW - 1);
+                    if (Ms[I * WW + Jc] == best.v1)
+                        rk = min(rk, surf_rank<W>(I, Jc));
+                }
+                atomicMin(&srank, rk);
+            }
+            __syncthreads();
+            best.r1 = srank;
+        }
         PeakRes *res = &C.res[it];
         int I = -1000, J = -1000;
         float raw = 0.f;
