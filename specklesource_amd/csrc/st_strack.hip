@@ -1025,6 +1025,21 @@ __device__ inline Top2 wg_top2(Top2 a, Top2 *sh)
     }
 }
 
+// sqrtf for x >= 2^-96 (finite): the hardware square root, then the two
+// one-ulp corrections of the correctly rounded sequence the compiler emits
+// for sqrtf (its input scaling below 2^-96 and its zero / infinity pass-
+// through change nothing in this range), so the result is sqrtf(x) bit for
+// bit.
+__device__ inline float sqrt_rn_normal(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __int_as_float(__float_as_int(s) - 1), sp = __int_as_float(__float_as_int(s) + 1);
+    const float rm = __builtin_fmaf(-sm, s, x), rp = __builtin_fmaf(-sp, s, x);
+    float r = rm <= 0.0f ? sm : s;
+    r = rp > 0.0f ? sp : r;
+    return r;
+}
+
 // The peak scan without ranks: each thread keeps the largest value, the
 // runner-up (the larger of all others) and the sum; the merge gives the
 // same three numbers as top2_combine, in the same order of additions.
@@ -2461,14 +2476,27 @@ __global__ void __launch_bounds__(2 * FusedLanes<W>::L * W) __attribute__((amdgp
         const int Jc = (2 * r + g + W) & (WW - 1);
         float mg0[H];  // |C| of component pp = 0, kept until the LDS is free
         auto scan = [&](const float2 *y, int pp, float *mg) {
+            // |C|^2 first; a wave with no value below 2^-96 (nearly always)
+            // takes the short form of the correctly rounded square root
+            bool tiny = false;
 #pragma unroll
             for (int jj = 0; jj < H; jj++) {
 #pragma clang fp contract(off)
-                const float x2 = y[jj].x * y[jj].x + y[jj].y * y[jj].y;
-                const float v = sqrtf(x2);
-                mg[jj] = v;
-                max2_add(best, v);
+                mg[jj] = y[jj].x * y[jj].x + y[jj].y * y[jj].y;
+                tiny |= !(mg[jj] >= 0x1p-96f);
             }
+            if (__builtin_amdgcn_ballot_w64(tiny) == 0) {
+#pragma unroll
+                for (int jj = 0; jj < H; jj++)
+                    mg[jj] = sqrt_rn_normal(mg[jj]);
+            } else {
+#pragma unroll
+                for (int jj = 0; jj < H; jj++)
+                    mg[jj] = sqrtf(mg[jj]);
+            }
+#pragma unroll
+            for (int jj = 0; jj < H; jj++)
+                max2_add(best, mg[jj]);
         };
         {
             float2 y[H];
