@@ -2774,12 +2774,29 @@ __global__ void __launch_bounds__(512) k_cpx_fused128(const KPar *__restrict__ P
                         y[k] = ::fftreg::cmul_f(y[k], f128_factor(sblk, k));
                 }
                 ::fftreg::inv_quad128(y, h);  // y[j] = C_pp,q[L j + h][r]
+                // |C|^2 first; a wave with no value below 2^-96 (nearly always)
+                // takes the short form of the correctly rounded square root
+                float mg[H];
+                bool tiny = false;
 #pragma unroll
                 for (int jj = 0; jj < H; jj++) {
 #pragma clang fp contract(off)
+                    mg[jj] = y[jj].x * y[jj].x + y[jj].y * y[jj].y;
+                    tiny |= !(mg[jj] >= 0x1p-96f);
+                }
+                if (__builtin_amdgcn_ballot_w64(tiny) == 0) {
+#pragma unroll
+                    for (int jj = 0; jj < H; jj++)
+                        mg[jj] = sqrt_rn_normal(mg[jj]);
+                } else {
+#pragma unroll
+                    for (int jj = 0; jj < H; jj++)
+                        mg[jj] = sqrtf(mg[jj]);
+                }
+#pragma unroll
+                for (int jj = 0; jj < H; jj++) {
                     const int I = (2 * (L * jj + h) + pp + W) & (WW - 1);
-                    const float x2 = y[jj].x * y[jj].x + y[jj].y * y[jj].y;
-                    const float v = sqrtf(x2);
+                    const float v = mg[jj];
                     M[((pp * 2 + q) * H + jj) * NTH + t] = v;  // full cache lines per store
                     const int i0 = I < W ? I : I - W, j0 = J < W ? J : J - W;
                     const int qd = (I < W) ? (J < W ? 0 : 3) : (J < W ? 2 : 1);
