@@ -345,11 +345,12 @@ __device__ inline void refine_cols(PC pc, int v, double &f1, double &f2, int &fr
             ai = fma(r, ti[m], ai);
         }
         const double q = ar * ar + ai * ai;  // |Q|^2: refine_core's expression
-        // ranks u kFine + v rise with u, so a later equal value never wins
-        const bool up = q > f1;
-        f2 = fmax(f2, up ? f1 : q);
-        fr = up ? u * kFine + v : fr;
-        f1 = up ? q : f1;
+        // ranks u kFine + v rise with u, so a later equal value never wins;
+        // the runner-up is the larger of the old one and the smaller of
+        // (best, q) (all values are finite and non-negative)
+        fr = q > f1 ? u * kFine + v : fr;
+        f2 = __builtin_fmax(f2, __builtin_fmin(f1, q));
+        f1 = __builtin_fmax(f1, q);
     }
 }
 
