@@ -106,6 +106,32 @@ def test_parity_complex_int16(device, oracle32, oracle64):
     _check(rep, probs)
 
 
+@pytest.mark.parametrize("w", [16, 64, 128])
+def test_parity_complex_int16_fused_windows(device, oracle32, oracle64, w):
+    """Integer (CI16) samples through the fused kernels of the other window
+    classes: their 16-bit loads are instantiations of their own."""
+    n = 480 if w <= 64 else 1100
+    s = configs._spec(n, n, 40 + w, dr0=1.1, drR=1e-3, da0=-0.6, daA=1e-3, gmin=0.5, gmax=0.85, gLr=300,
+                      gLa=250)
+    a, b = track.synth_host(s)
+    a16 = np.zeros(a.shape + (2,), np.int16)
+    b16 = np.zeros_like(a16)
+    a16[..., 0] = np.round(a.real * 300)
+    a16[..., 1] = np.round(a.imag * 300)
+    b16[..., 0] = np.round(b.real * 300)
+    b16[..., 1] = np.round(b.imag * 300)
+    p = small_params(w, n, n, step=w // 2 if w <= 64 else 48, rShiftPoly=[1.1, 1e-3, 0], aShiftPoly=[-0.6, 0, 1e-3])
+    device.set_profiling(True)
+    try:
+        g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, p, a16, b16, abi.ST_SAMPLE_CI16)
+        names = _kernel_names(device)
+    finally:
+        device.set_profiling(False)
+    assert names.get("k_cpx_fused", 0) > 0
+    assert (g.type == abi.ST_CMATCH).mean() > 0.5
+    _check(rep, probs)
+
+
 def test_parity_initial_grid_with_holes(device, oracle32, oracle64):
     """findImage2Pos from an initial-offset grid; grid holes keep the
     previous point's window position (speckleTrack.c:823-826)."""
