@@ -261,7 +261,7 @@ def served_roofline(pairs, precision):
     dev = track.Device(0)
     pids = [dev.synth_pair(s) for s, _ in pairs]
     dev.set_profiling(True)
-    table, work = {}, {}
+    table, work, refine = {}, {}, {}
     for k, ((s, p), pid) in enumerate(zip(pairs, pids)):
         q = abi.TrackParams.from_buffer_copy(p)
         q.precision = precision
@@ -273,6 +273,11 @@ def served_roofline(pairs, precision):
         fl = flops_mfma_patch(p.wR) if kern == "k_cpx_mfma" else flops_complex_patch(p.wR) + flops_window_sums(p.wR)
         f, n, b = work.get(kern, (0.0, 0, 0))
         work[kern] = (f + fl * st.nCorr[0], n + st.nCorr[0], b + 16 * p.wR * p.wR * st.nCorr[0])
+        # this call's refinement-kernel time, charged to its own match kernel
+        if "k_cpx_refine" in names:
+            k = names.index("k_cpx_refine")
+            rms, rn = refine.get(kern, (0.0, 0))
+            refine[kern] = (rms + st.kernelMs[k], rn + st.kernelLaunches[k])
     dev.set_profiling(False)
     for pid in pids:
         dev.release_pair(pid)
@@ -284,11 +289,11 @@ def served_roofline(pairs, precision):
     label = dom
     by_kernel = {dom: table[dom][0] / table[dom][1]}
     traffic = None
-    if dom == "k_cpx_fused" and "k_cpx_refine" in table:
+    if dom == "k_cpx_fused" and dom in refine:
         # the fused path's sub-pixel refinement runs in its own kernel right
         # after it: the complex match's flops are divided by both
-        ms += table["k_cpx_refine"][0]
-        by_kernel["k_cpx_refine"] = table["k_cpx_refine"][0] / table["k_cpx_refine"][1]
+        ms += refine[dom][0]
+        by_kernel["k_cpx_refine"] = refine[dom][0] / refine[dom][1]
         label = "k_cpx_fused + k_cpx_refine"
         traffic = served_traffic()
     achieved = flops / (ms * 1e-3) / 1e12

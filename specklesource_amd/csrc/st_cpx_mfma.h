@@ -386,76 +386,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 1
             res->J = J;
             res->raw = raw;
         }
-        // ---------------- FP64 refinement of the 8 x 8 neighbourhood
-        // (overSampleC, speckleTrack.c:852-863) in this patch's LDS region
-        // (R from constant memory: the scratch must end before the surface Mg)
-        static_assert(G::RP + G::RX >= RefLayout<false>::BYTES, "refinement scratch overlaps the surface");
-        unsigned char *scr = (unsigned char *)base;
-        double *Pm = (double *)(scr + kRefPm);
-        if (tp < NFAST * NFAST)
-            Pm[tp] = (double)Mg[(J - NFAST / 2 + tp % NFAST) * WW + (I - NFAST / 2 + tp / NFAST)];
-        mf_sync<WPP>();
-        double f1, f2;
-        int fr;
-        refine_fine<TPP, false, 2>(scr, tp, [] { mf_sync<WPP>(); }, f1, f2, fr);
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const double b1 = __shfl_xor(f1, o), b2 = __shfl_xor(f2, o);
-            const int br = __shfl_xor(fr, o);
-            if (b1 > f1 || (b1 == f1 && br < fr)) {
-                f2 = fmax(f1, b2);
-                f1 = b1;
-                fr = br;
-            } else
-                f2 = fmax(f2, b1);
-        }
-        if constexpr (WPP > 1) {
-            __syncthreads();
-            if (lane == 0) {
-                shd[wv][2] = f1;
-                shd[wv][3] = f2;
-                shi[wv][1] = fr;
-            }
-            __syncthreads();
-            f1 = shd[0][2];
-            f2 = shd[0][3];
-            fr = shi[0][1];
-            for (int k = 1; k < WPP; k++) {
-                const double b1 = shd[k][2], b2 = shd[k][3];
-                const int br = shi[k][1];
-                if (b1 > f1 || (b1 == f1 && br < fr)) {
-                    f2 = fmax(f1, b2);
-                    f1 = b1;
-                    fr = br;
-                } else
-                    f2 = fmax(f2, b1);
-            }
-        }
-        int fi = 0, fj = 0;
-        if (fr != 0x7fffffff) {
-            fi = fr / kFine;
-            fj = fr - fi * kFine;
-        }
-        if (tp == 0) {
-            res->fi = fi;
-            res->fj = fj;
-            res->fineMax = fr != 0x7fffffff ? sqrt(f1) : 0.0;
-            res->fineGap = fr != 0x7fffffff ? (float)((sqrt(f1) - sqrt(f2)) / sqrt(f1)) : 0.f;
-        }
-        const int iF = I * NOVER + (fi + 24) - (NOVER * NFAST) / 2;
-        const int jF = J * NOVER + (fj + 24) - (NOVER * NFAST) / 2;
-        const int cI = iF / NOVER, cJ = jF / NOVER;
-        const bool ok = (cI - NGAUSS / 2) >= 0 && (cI + NGAUSS / 2) < WW && (cJ - NGAUSS / 2) >= 0 &&
-                        (cJ + NGAUSS / 2) < WW;
-        if (tp == 0) {
-            res->lmOk = ok ? 1 : 0;
-            res->lmX0[0] = (float)(cJ - NGAUSS / 2);
-            res->lmX0[1] = (float)(cI - NGAUSS / 2);
-        }
-        if (ok && tp < 2 * NGAUSS) {
-            const int k = tp % NGAUSS;
-            res->lmY[tp / NGAUSS][k] = tp < NGAUSS ? Mg[(cJ - NGAUSS / 2 + k) * WW + cI]
-                                                   : Mg[cJ * WW + (cI - NGAUSS / 2 + k)];
+        // the block cells k_cpx_refine reads (the refinement and the fit
+        // samples run there, as on the FP32 fused path); Mg is transposed
+        float *nb = C.nb + (size_t)it * kNbc;
+        for (int e = tp; e < kNbc; e += TPP) {
+            const int ri = I - kNb / 2 + kNbMap.ri[e], ci = J - kNb / 2 + kNbMap.ci[e];
+            nb[e] = (ri >= 0 && ri < WW && ci >= 0 && ci < WW) ? Mg[ci * WW + ri] : 0.0f;
         }
         mf_sync<WPP>();
     }

@@ -3629,10 +3629,16 @@ struct StrackJob {
             C.have = ar.take<uint8_t>(nCpx);
             C.az = ar.take<int>(nCpx);
             C.p = ar.take<double>(2 * (size_t)nCpx);
+            C.nb = ar.take<float>((size_t)nCpx * kNbc);
             if (w == 16)
                 launch_cpx_mfma<16>(ctx, kPeak, P, S, list, nCpx, C);
             else
                 launch_cpx_mfma<32>(ctx, kPeak, P, S, list, nCpx, C);
+            const int kRef = ctx->kernel_slot("k_cpx_refine");
+            ctx->time_begin(kRef);
+            hipLaunchKernelGGL(k_cpx_refine, dim3((nCpx + kRefPer - 1) / kRefPer), dim3(kRefThreads), 0, st, C,
+                               nCpx, 2 * w);
+            ctx->time_end(kRef);
             hipLaunchKernelGGL(k_lm, dim3((2 * nCpx + 127) / 128), dim3(128), 0, st, C.res, nCpx, C.sig);
             hipLaunchKernelGGL(k_cpx_finalize, dim3((nCpx + 127) / 128), dim3(128), 0, st, P, S, list, 0, nCpx, C);
             ST_HIP(hipGetLastError());
