@@ -623,8 +623,11 @@ __global__ void __launch_bounds__(128) k_gauss_fit(const float *mag, int WA, int
     }
 }
 
-// Two fits per attempt: [0] along range, [1] along azimuth.
-__global__ void k_lm(const PeakRes *res, int B, float *sig)
+// Two fits per attempt: [0] along range, [1] along azimuth.  Launched in
+// 128-thread blocks: the bound lets a fit keep its samples, the 3 x 3 system
+// and the trial parameters in registers (at the default 1024-thread bound
+// they spill to scratch).
+__global__ void __launch_bounds__(128) k_lm(const PeakRes *res, int B, float *sig)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * B)
