@@ -14,6 +14,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -28,7 +29,12 @@ namespace {
 
 constexpr float kNoData = -2.0e9f;
 constexpr int kMaxIter = 48;
-constexpr int kMaxBox = 15;  // box sides up to 15 (225 cells per box)
+// The median kernels keep each wave's box values (range and azimuth) in
+// dynamic LDS sized for the box of the call, at most kCullLds bytes per
+// workgroup: box sides up to 45 (2025 cells; 4 waves x 2 x 4 B per cell).
+constexpr int kMaxBox = 45;
+constexpr size_t kCullLds = 65536;
+constexpr int cull_box_cells(int bR, int bA) { return ((bR / 2) * 2 + 1) * ((bA / 2) * 2 + 1); }
 
 __device__ inline bool valid2(float r, float a)
 {
@@ -206,12 +212,13 @@ template <bool LS>
 __global__ void __launch_bounds__(256) k_cull_iter(CullDev D, int bR, int bA, int nGood, float maxA, float maxR,
                                                    LsArg L)
 {
-    __shared__ float lr[4][kMaxBox * kMaxBox], la[4][kMaxBox * kMaxBox];
-    const int w = threadIdx.x >> 6;
+    extern __shared__ float cull_lds[];  // per wave: range values, then azimuth values
+    const int w = threadIdx.x >> 6, cells = cull_box_cells(bR, bA);
     const int c = blockIdx.x * 4 + w;
     if (c >= D.nR * D.nA)
         return;
-    const bool cull = cull_decide<LS>(D, c, bR, bA, nGood, maxA, maxR, D.cPrev, lr[w], la[w], L);
+    float *lr = cull_lds + (size_t)w * 2 * cells, *la = lr + cells;
+    const bool cull = cull_decide<LS>(D, c, bR, bA, nGood, maxA, maxR, D.cPrev, lr, la, L);
     if ((threadIdx.x & 63) == 0) {
         D.cNew[c] = cull ? 1 : 0;
         if ((D.cPrev[c] != 0) != cull)
@@ -219,14 +226,16 @@ __global__ void __launch_bounds__(256) k_cull_iter(CullDev D, int bR, int bA, in
     }
 }
 
-// Exact sequential order for the leftover case: one workgroup sweeps the
-// wavefront steps, one wave per cell of a step.
+// Exact sequential order for the leftover case: one workgroup (up to 16
+// waves, fewer for large boxes) sweeps the wavefront steps, one wave per
+// cell of a step.
 template <bool LS>
 __global__ void __launch_bounds__(1024) k_cull_wavefront(CullDev D, int bR, int bA, int nGood, float maxA, float maxR,
                                                          LsArg L)
 {
-    __shared__ float lr[16][kMaxBox * kMaxBox], la[16][kMaxBox * kMaxBox];
-    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    extern __shared__ float cull_lds[];
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6, cells = cull_box_cells(bR, bA);
+    float *lr = cull_lds + (size_t)w * 2 * cells, *la = lr + cells;
     const int s = bR / 2 + 1;
     const int steps = (D.nA - 1) * s + D.nR;
     for (int T = 0; T < steps; T++) {
@@ -235,7 +244,7 @@ __global__ void __launch_bounds__(1024) k_cull_wavefront(CullDev D, int bR, int 
         for (int i = iLo + w; i <= iHi; i += nw) {
             const int j = T - i * s;
             const int c = i * D.nR + j;
-            const bool cull = cull_decide<LS>(D, c, bR, bA, nGood, maxA, maxR, D.cNew, lr[w], la[w], L);
+            const bool cull = cull_decide<LS>(D, c, bR, bA, nGood, maxA, maxR, D.cNew, lr, la, L);
             if ((threadIdx.x & 63) == 0)
                 D.cNew[c] = cull ? 1 : 0;
         }
@@ -749,13 +758,16 @@ struct CullRun {
     int median_pass(int bR, int bA, int nGood, float maxA, float maxR, const LsArg &L)
     {
         ST_REQUIRE(bR >= 1 && bA >= 1 && (bR / 2) * 2 + 1 <= kMaxBox && (bA / 2) * 2 + 1 <= kMaxBox,
-                   ST_ERR_UNSUPPORTED, "box sizes above 15");
+                   ST_ERR_UNSUPPORTED, "box sizes above 45");
         hipStream_t st = ctx->stream;
+        const size_t perWave = 2 * sizeof(float) * (size_t)cull_box_cells(bR, bA);
+        const int wfWaves = (int)std::min<size_t>(16, kCullLds / perWave);
         bool settled = false;
         const int maxIter = max_iterations();
         for (int it = 0; it < maxIter; it++) {
             ST_HIP(hipMemsetAsync(D.counters, 0, sizeof(int), st));
-            hipLaunchKernelGGL(k_cull_iter<LS>, dim3((n + 3) / 4), dim3(256), 0, st, D, bR, bA, nGood, maxA, maxR, L);
+            hipLaunchKernelGGL(k_cull_iter<LS>, dim3((n + 3) / 4), dim3(256), 4 * perWave, st, D, bR, bA, nGood, maxA,
+                               maxR, L);
             ST_HIP(hipGetLastError());
             std::swap(D.cPrev, D.cNew);
             if (counter(0) == 0) {
@@ -765,7 +777,8 @@ struct CullRun {
         }
         std::swap(D.cPrev, D.cNew);  // cNew holds the last decisions
         if (!settled)
-            hipLaunchKernelGGL(k_cull_wavefront<LS>, dim3(1), dim3(1024), 0, st, D, bR, bA, nGood, maxA, maxR, L);
+            hipLaunchKernelGGL(k_cull_wavefront<LS>, dim3(1), dim3(64 * wfWaves), wfWaves * perWave, st, D, bR, bA,
+                               nGood, maxA, maxR, L);
         ST_HIP(hipMemsetAsync(D.counters + 1, 0, sizeof(int), st));
         hipLaunchKernelGGL(k_cull_apply, dim3(nb(n)), dim3(256), 0, st, D);
         ST_HIP(hipGetLastError());

@@ -38,7 +38,9 @@ constexpr int kN21 = WNFAST + 1;
 constexpr int kFineLo = (kN21 * WNOVER) / 2 - 2 * WNOVER;  // 136: searched oversampled window
 constexpr int kFineN = 4 * WNOVER;                         // 64
 constexpr int kTile = 32;
-constexpr int kMaxNavg = 8;
+// Boxcar half-widths up to 24: k_w_detect's two haloed tiles,
+// (32 + 2 * 24)^2 floats each, stay within 64 KB of LDS with its output tile.
+constexpr int kMaxNavg = 24;
 constexpr int kAmpMatch = 2;
 // Dirichlet kernel of the 21 -> 336 zero pad, D(t), t = kDirLo .. kDirLo + kDirN - 1
 constexpr int kDirLo = kFineLo - WNOVER * WNFAST;  // -184
@@ -324,8 +326,7 @@ __global__ void __launch_bounds__(NT) k_w_colinterp(WPar p, WBatch Bt, int B, in
 // edge-pad frame zero.
 __global__ void __launch_bounds__(256) k_w_detect(WPar p, WBatch Bt, int B)
 {
-    __shared__ float d1[(kTile + 2 * kMaxNavg) * (kTile + 2 * kMaxNavg)];
-    __shared__ float d2[(kTile + 2 * kMaxNavg) * (kTile + 2 * kMaxNavg)];
+    extern __shared__ float w_det_lds[];  // the two haloed tiles, sized by detect_lds_bytes
     __shared__ float sv[kTile][kTile + 1];
     const int b = blockIdx.z;
     const int NN = p.W_A * p.W_R;
@@ -334,6 +335,7 @@ __global__ void __launch_bounds__(256) k_w_detect(WPar p, WBatch Bt, int B)
     const int hA = box ? p.navgA : 0, hR = box ? p.navgR : 0;
     const int y0 = blockIdx.y * kTile, x0 = blockIdx.x * kTile;
     const int TH = kTile + 2 * hA, TW = kTile + 2 * hR;
+    float *d1 = w_det_lds, *d2 = w_det_lds + TH * TW;
     const int c0 = p.epA * WOS, c1 = c0 + p.wA2;  // chip rows [c0, c1)
     const int e0 = p.epR * WOS, e1 = e0 + p.wR2;  // chip cols [e0, e1)
     const bool chipTile = y0 < c1 && y0 + kTile > c0 && x0 < e1 && x0 + kTile > e0;
@@ -390,6 +392,15 @@ __global__ void __launch_bounds__(256) k_w_detect(WPar p, WBatch Bt, int B)
         cs[0] = a1;
         cs[p.W_R] = a2;
     }
+}
+
+// k_w_detect's dynamic LDS: the haloed search and chip tiles (the boxcar
+// runs only when a half-width exceeds 1, detectPatch's test).
+size_t detect_lds_bytes(const WPar &p)
+{
+    const bool box = p.navgA > 1 || p.navgR > 1;
+    const size_t th = kTile + 2 * (box ? p.navgA : 0), tw = kTile + 2 * (box ? p.navgR : 0);
+    return 2 * th * tw * sizeof(float);
 }
 
 // Chip mean and variance (computeMeanSigma, corrTrackFast.c:418-431):
@@ -907,7 +918,7 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
                    stfft::supported_size(p.W_A),
                ST_ERR_UNSUPPORTED, "window size not supported by the FFT engine");
     ST_REQUIRE(p.navgR >= 0 && p.navgA >= 0 && p.navgR <= kMaxNavg && p.navgA <= kMaxNavg, ST_ERR_UNSUPPORTED,
-               "boxcar half-widths above 8");
+               "boxcar half-widths above 24");
     ST_REQUIRE(par.polyShift || (par.initDr && par.initDa && par.initNr > 1 && par.initNa > 1), ST_ERR_ARG,
                "initial offsets: polyShift=0 needs initDr/initDa");
     // corrTrackFast.c:333-334
@@ -1019,7 +1030,7 @@ void st_corr_track_fast_impl(st_ctx *ctx, StPair &pr, const st_track_params &par
             ctx->time_end(kFftSlot);
             ctx->time_begin(kDet);
             hipLaunchKernelGGL(k_w_detect, dim3((p.W_R + kTile - 1) / kTile, (p.W_A + kTile - 1) / kTile, nb),
-                               dim3(256), 0, st, p, Bt, nb);
+                               dim3(256), detect_lds_bytes(p), st, p, Bt, nb);
             ctx->time_end(kDet);
             ctx->time_begin(kStat);
             hipLaunchKernelGGL(k_w_chipstats, dim3(nb), dim3(NT), 0, st, p, Bt);

@@ -85,14 +85,38 @@ def test_cull_prepass_mask_corr_singlemt(device, oracle32):
         assert np.array_equal(ga["type"], oa["type"]), kw
 
 
-@pytest.mark.parametrize("box", [(3, 3), (5, 11), (15, 15), (4, 6)])
+@pytest.mark.parametrize("box", [(3, 3), (5, 11), (15, 15), (4, 6), (21, 21), (17, 31), (45, 45), (44, 20)])
 def test_cull_box_sizes(device, oracle32, box):
+    """Boxes up to 45 x 45 (the per-wave LDS lists are sized per call; an
+    even side b spans b + 1 cells, as in cullSTData.c:73-80)."""
     offR, offA, corr, typ = make_cull_grid(45, 52, seed=8, outlier_frac=0.1)
     bR, bA = box
-    (gc, ga), (oc, oa) = _pair(offR, offA, corr, typ, bR=bR, bA=bA, nGood=max(1, bR * bA // 5))
+    (gc, ga), (oc, oa) = _pair(offR, offA, corr, typ, bR=bR, bA=bA, nGood=min(60, max(1, bR * bA // 5)))
     device.cull("st_cull_st_data", gc)
     oracle32.cull("st_cull_st_data", oc)
+    assert gc.nCulled == oc.nCulled
     assert np.array_equal(ga["offR"], oa["offR"]) and np.array_equal(ga["offA"], oa["offA"])
+
+
+@pytest.mark.parametrize("iters", ["48", "0"])
+def test_cull_large_box_schedules(device, oracle32, monkeypatch, iters):
+    """A 41 x 41 box through both schedules: the fixpoint kernel, and the
+    sequential wavefront kernel, which runs fewer waves when the box lists
+    would not fit 16 of them in the workgroup's LDS."""
+    monkeypatch.setenv("SPECKLE_AMD_CULL_ITERS", iters)
+    offR, offA, corr, typ = make_cull_grid(30, 60, seed=12, outlier_frac=0.2)
+    (gc, ga), (oc, oa) = _pair(offR, offA, corr, typ, bR=41, bA=41, nGood=40, maxR=0.3, maxA=0.4)
+    device.cull("st_cull_st_data", gc)
+    oracle32.cull("st_cull_st_data", oc)
+    assert gc.nCulled == oc.nCulled
+    assert np.array_equal(ga["offR"], oa["offR"]) and np.array_equal(ga["offA"], oa["offA"])
+
+
+def test_cull_box_above_limit_refused(device):
+    offR, offA, corr, typ = make_cull_grid(20, 20, seed=13)
+    gc, _ = cull_params(offR, offA, corr, typ, bR=47, bA=9)
+    with pytest.raises(RuntimeError, match="box sizes above 45"):
+        device.cull("st_cull_st_data", gc)
 
 
 def test_cull_stats_sigma(device, oracle32):

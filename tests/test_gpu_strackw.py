@@ -100,6 +100,38 @@ def test_parity_synthetic(device, oracle32, oracle64, w, edge, navg):
     _check(rep, probs)
 
 
+@pytest.mark.parametrize("navg", [12, 24])
+def test_parity_wide_boxcar(device, oracle32, oracle64, navg):
+    """Boxcar half-widths up to 24 (k_w_detect's haloed tiles are sized
+    per call) on a coarse grid, so the oracle's (2 navg + 1)^2 taps per
+    pixel stay quick.  A 49 x 49 boxcar over a 64 px window leaves a flat
+    NCC surface on which FP32 itself (the FP32 oracle) lands up to ~0.1 px
+    from FP64, so the GPU's offsets are held to the FP32 oracle's (equal
+    peaks give equal offsets; differing peaks must be near-ties) and its
+    2-D offset error against FP64 to within RMS_TOL of the FP32 oracle's
+    (a near-tie resolved the other way moves a point along either axis, so
+    the axes are not bounded one by one)."""
+    s, par = _scene(480, 480, 70 + navg, 64, 8, navg, step=96)
+    a, b = track.synth_host(s)
+    g = _run(device, par, a, b)
+    o32 = oracle32.corr_track_fast(par, a, b, diag=True)
+    o64 = oracle64.corr_track_fast(par, a, b, diag=True)
+    rep, probs = compare(g, o32, o64)
+    ok = (o32.offR > -1e9) & (o64.offR > -1e9)
+    base = [float(np.sqrt(np.mean((x[ok] - y[ok]).astype(np.float64) ** 2)))
+            for x, y in ((o32.offR, o64.offR), (o32.offA, o64.offA))]
+    print("\n", rep, "fp32 oracle rms vs fp64:", base)
+    assert not probs, probs[:10]
+    assert np.hypot(rep["rms_r_vs_f64"], rep["rms_a_vs_f64"]) <= np.hypot(*base) + RMS_TOL
+
+
+def test_boxcar_above_limit_refused(device):
+    s, par = _scene(480, 480, 99, 64, 8, 25, step=96)
+    a, b = track.synth_host(s)
+    with pytest.raises(RuntimeError, match="boxcar half-widths above 24"):
+        _run(device, par, a, b)
+
+
 def test_parity_cfg4_window(device, oracle32, oracle64):
     """The chip-128 strackw window of the sheared-flow workload (wRa = 384,
     edge pad 123 -> 768^2 transforms, 513^2 lags, 5 x 5 boxcar) on a
