@@ -115,7 +115,8 @@ __device__ inline bool in_bounds(int r1, int a1, int wR, int wA, int sR, int sA)
 // conj(y) / |y|, or left as it is where |Re y| and |Im y| are both below
 // 1e-6.  Spectrum and correction run in FP64.
 constexpr int kIntThreads = 256;
-constexpr int kIntMaxPs = 64;
+constexpr int kIntMaxPs = 64;    // k_int_bins: the patch and its transforms in LDS
+constexpr int kIntMaxPsG = 128;  // k_int_bins_g: in a global workspace per workgroup
 constexpr int kIntPer = kIntMaxPs * kIntMaxPs / kIntThreads;
 
 __device__ inline void int_correct(const KPar &P, int it, int i, int j, double &cr, double &ci)
@@ -312,6 +313,114 @@ __global__ void __launch_bounds__(kIntThreads) k_int_bins(KPar P, Pts S, const i
 }
 
 size_t int_lds_bytes(int ps) { return (size_t)ps * ps * sizeof(double2) + ps * sizeof(double2) + (size_t)ps * ps * sizeof(float2); }
+
+// k_int_bins for patches above kIntMaxPs samples a side (up to kIntMaxPsG,
+// e.g. w = 128 without looks): the same transforms, sums and selection, with
+// the patch, the range transform and the spectrum in a workspace of
+// int_ws_bytes(ps) per workgroup (they do not fit the LDS), the workgroups
+// walking the attempts as a persistent grid.  Each thread reads back only
+// the spectrum cells it wrote; the other cells cross threads after a barrier.
+// (each array starts on a 256-byte boundary: ps may be odd)
+__host__ __device__ inline size_t int_ws_al(size_t b) { return (b + 255) & ~(size_t)255; }
+__host__ __device__ inline size_t int_ws_bytes(int ps)
+{
+    const size_t n = (size_t)ps * ps;
+    return int_ws_al(n * sizeof(float2)) + int_ws_al(n * sizeof(double2)) + 2 * int_ws_al(n * sizeof(double));
+}
+
+__global__ void __launch_bounds__(kIntThreads) k_int_bins_g(KPar P, Pts S, const int *list, int count, IntSet *set,
+                                                            IntBin *ovf, int *ovfCount, int pass, unsigned char *ws)
+{
+    __shared__ double2 tw[kIntMaxPsG];
+    __shared__ double red[kIntThreads / 64];
+    __shared__ int nKept;
+    const int ps = P.intPs, ps2 = ps * ps, h = ps / 2;
+    unsigned char *mine = ws + (size_t)blockIdx.x * int_ws_bytes(ps);
+    float2 *xin = (float2 *)mine;
+    double2 *T = (double2 *)(mine + int_ws_al((size_t)ps2 * sizeof(float2)));
+    double *Xr = (double *)((unsigned char *)T + int_ws_al((size_t)ps2 * sizeof(double2)));
+    double *Pw = (double *)((unsigned char *)Xr + int_ws_al((size_t)ps2 * sizeof(double)));
+    for (int t = threadIdx.x; t < ps; t += kIntThreads) {
+        double sn, cs;
+        sincospi(-2.0 * t / ps, &sn, &cs);
+        tw[t] = make_double2(cs, sn);
+    }
+    const float2 *src = (const float2 *)P.intf;
+    for (int it = blockIdx.x; it < count; it += gridDim.x) {
+        IntSet *me = set + it;
+        if (pass == 1 && me->n <= kIntInline)
+            continue;
+        const int pt = list[it];
+        const int ia1 = (S.a1[pt] + P.wA / 2) / P.intNal - ps / 2;
+        const int ir1 = (S.r1[pt] + P.wR / 2) / P.intNrl - ps / 2;
+        if (!in_bounds(ir1, ia1, ps, ps, P.intNr, P.intNa)) {
+            if (threadIdx.x == 0) {
+                me->n = 0;
+                me->ovf = 0;
+            }
+            continue;
+        }
+        for (int e = threadIdx.x; e < ps2; e += kIntThreads) {
+            const int i = e / ps, j = e - (e / ps) * ps;
+            xin[e] = src[(size_t)(ia1 + i) * P.intNr + ir1 + j];
+        }
+        if (threadIdx.x == 0)
+            nKept = 0;
+        __syncthreads();
+        for (int e = threadIdx.x; e < ps2; e += kIntThreads) {
+            const int i = e / ps, l = e - (e / ps) * ps;
+            double sr = 0, si = 0;
+            int idx = 0;
+            for (int j = 0; j < ps; j++) {
+                const float2 x = xin[i * ps + j];
+                const double2 w = tw[idx];
+                sr += (double)x.x * w.x - (double)x.y * w.y;
+                si += (double)x.x * w.y + (double)x.y * w.x;
+                idx += l;
+                idx -= idx >= ps ? ps : 0;
+            }
+            T[e] = make_double2(sr, si);
+        }
+        __syncthreads();
+        double mx = 0;
+        for (int e = threadIdx.x; e < ps2; e += kIntThreads) {
+            const int k = e / ps, l = e - (e / ps) * ps;
+            double sr = 0, si = 0;
+            int idx = 0;
+            for (int i = 0; i < ps; i++) {
+                const double2 t = T[i * ps + l], w = tw[idx];
+                sr += t.x * w.x - t.y * w.y;
+                si += t.x * w.y + t.y * w.x;
+                idx += k;
+                idx -= idx >= ps ? ps : 0;
+            }
+            Xr[e] = sr;
+            Pw[e] = sr * sr + si * si;
+            mx = fmax(mx, Pw[e]);
+        }
+        mx = block_max_d<kIntThreads>(mx, red);
+        const double thr = 0.99 * mx;
+        IntBin *dst = pass == 0 ? me->b : ovf + (size_t)me->ovf * ps2;
+        const int cap = pass == 0 ? kIntInline : ps2;
+        for (int e = threadIdx.x; e < ps2; e += kIntThreads) {
+            const double xr = Xr[e];
+            if (!(Pw[e] < thr) && xr != 0.0) {
+                const int k = e / ps, l = e - (e / ps) * ps;
+                if (k < 2 * h && l < 2 * h) {
+                    const int pos = atomicAdd(&nKept, 1);
+                    if (pos < cap)
+                        dst[pos] = IntBin{k < h ? k : P.intPa - 2 * h + k, l < h ? l : P.intPr - 2 * h + l, xr};
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && pass == 0) {
+            me->n = nKept;
+            me->ovf = nKept > kIntInline ? atomicAdd(ovfCount, 1) : 0;
+        }
+        __syncthreads();
+    }
+}
 
 // ---------------------------------------------------------------------
 // geometry: findImage2Pos (polynomial) or host-computed positions,
@@ -3371,7 +3480,7 @@ struct StrackJob {
             if (p)
                 (void)hipFree(p);
         }
-    } intOvfBuf;
+    } intOvfBuf, intWsBuf;
 
     StrackJob(st_ctx *c, StPair &p, const st_track_params &par, st_track_out *o, st_track_diag *d,
               st_cull_params *cu)
@@ -3441,8 +3550,8 @@ struct StrackJob {
             ST_REQUIRE(par.intNr > 0 && par.intNa > 0 && par.intNrl > 0 && par.intNal > 0, ST_ERR_ARG,
                        "interferogram size and looks");
             intPs = std::max(std::max(par.wA / par.intNal, par.wR / par.intNrl), 8);
-            ST_REQUIRE(intPs <= kIntMaxPs, ST_ERR_UNSUPPORTED,
-                       "interferogram patch above 64 x 64 samples (window / looks)");
+            ST_REQUIRE(intPs <= kIntMaxPsG, ST_ERR_UNSUPPORTED,
+                       "interferogram patch above 128 x 128 samples (window / looks)");
             ST_REQUIRE((intPs * par.intNal) / 2 >= par.wA / 2 && (intPs * par.intNrl) / 2 >= par.wR / 2, ST_ERR_ARG,
                        "window not covered by the oversampled interferogram patch (window not a multiple of the looks)");
             ST_REQUIRE((long)intPs * par.intNal <= 32768 && (long)intPs * par.intNrl <= 32768, ST_ERR_ARG,
@@ -3598,9 +3707,21 @@ struct StrackJob {
             P.intSet = set;
             const size_t lds = int_lds_bytes(intPs);
             const int kInt = ctx->kernel_slot("k_int_bins");
+            // patches above kIntMaxPs: k_int_bins_g over a persistent grid with its workspace
+            const bool big = intPs > kIntMaxPs;
+            const int gridG = std::min(nCpx, 512);
+            if (big)
+                ST_HIP(hipMalloc(&intWsBuf.p, (size_t)gridG * int_ws_bytes(intPs)));
+            auto bins = [&](IntBin *ovfp, int pass) {
+                if (big)
+                    hipLaunchKernelGGL(k_int_bins_g, dim3(gridG), dim3(kIntThreads), 0, st, P, S, list, nCpx, set, ovfp,
+                                       nOvfDev, pass, (unsigned char *)intWsBuf.p);
+                else
+                    hipLaunchKernelGGL(k_int_bins, dim3(nCpx), dim3(kIntThreads), lds, st, P, S, list, set, ovfp,
+                                       nOvfDev, pass);
+            };
             ctx->time_begin(kInt);
-            hipLaunchKernelGGL(k_int_bins, dim3(nCpx), dim3(kIntThreads), lds, st, P, S, list, set, (IntBin *)nullptr,
-                               nOvfDev, 0);
+            bins((IntBin *)nullptr, 0);
             ctx->time_end(kInt);
             ST_HIP(hipGetLastError());
             const int nOvf = read_count(ctx, nOvfDev, 0);
@@ -3609,8 +3730,7 @@ struct StrackJob {
                 ST_HIP(hipMalloc(&intOvfBuf.p, (size_t)nOvf * intPs * intPs * sizeof(IntBin)));
                 P.intOvf = (const IntBin *)intOvfBuf.p;
                 ctx->time_begin(kInt);
-                hipLaunchKernelGGL(k_int_bins, dim3(nCpx), dim3(kIntThreads), lds, st, P, S, list, set,
-                                   (IntBin *)intOvfBuf.p, nOvfDev, 1);
+                bins((IntBin *)intOvfBuf.p, 1);
                 ctx->time_end(kInt);
                 ST_HIP(hipGetLastError());
             }

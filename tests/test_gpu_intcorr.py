@@ -6,7 +6,7 @@ by conj(y) / |y| before the complex match, so the matches that the fringe
 removes come back.  The same bars as the other strack parity tests apply
 (integer peaks and match types against the FP32-faithful oracle up to
 near-ties, sub-pixel RMS against FP64).  The cases cover the fused kernel
-at w = 16, 32, 64, the staged path, the correction after the baseline
+at w = 16, 32, 64, patches above 64 samples, the staged path, the correction after the baseline
 ramp, spectra with two kept bins and flat (chirp) spectra whose kept bins
 exceed the per-point record, and the sizes the GPU path refuses."""
 import ctypes as C
@@ -33,6 +33,23 @@ def test_int_parity_fused(device, oracle32, oracle64, w, nrl, nal, kfa, kfr):
     n = par.nR * par.nA
     assert names.get("k_cpx_fused", 0) > 0 and names.get("k_int_bins", 0) == 1
     assert g.nComplex > 0.7 * n, (g.nComplex, n)
+    _check(rep, probs)
+
+
+@pytest.mark.parametrize("w,kfa,kfr", [(96, 2, 5), (128, 3, 7)])
+def test_int_parity_large_patch(device, oracle32, oracle64, w, kfa, kfr):
+    """Correction patches above 64 samples a side (w = 96 and 128 without
+    looks, ps = w), whose spectra go through a global workspace."""
+    par, a, b, intf, ps = fringe_scene(12 * w, w, 1, 1, kfa, kfr, seed=w + 1)
+    assert ps == w
+    device.set_profiling(True)
+    try:
+        g, o32, o64, rep, probs = _run_all(device, oracle32, oracle64, par, a, b)
+        names = _kernel_names(device)
+    finally:
+        device.set_profiling(False)
+    assert names.get("k_int_bins", 0) >= 1
+    assert g.nComplex > 0.7 * par.nR * par.nA, (g.nComplex, par.nR * par.nA)
     _check(rep, probs)
 
 
@@ -115,7 +132,7 @@ def test_int_many_kept_bins(device, oracle32, oracle64, monkeypatch, path):
 
 
 def test_int_refused_sizes(device):
-    """ps above 64 (w = 128 without looks) is refused as unsupported; a
+    """ps above 128 (w = 192 without looks) is refused as unsupported; a
     window not covered by ps x looks (w = 40 with 3 x 3 looks: ps = 13,
     Pa = 39) as an argument error, as the oracle does."""
     img = np.zeros((600, 600), np.complex64)
@@ -123,10 +140,10 @@ def test_int_refused_sizes(device):
     try:
         intf = np.ones((300, 600), np.float32)
         from tests.helpers import small_params
-        p = small_params(128, 600, 600)
+        p = small_params(192, 600, 600)
         p.intf = abi.ptr(intf, C.c_float)
         p.intNa, p.intNr, p.intNrl, p.intNal = 300, 300, 1, 1
-        with pytest.raises(RuntimeError, match=r"\(-3\)"):
+        with pytest.raises(RuntimeError, match=r"\(-3\).*interferogram patch above 128"):
             device.speckle_track(pid, p)
         p = small_params(40, 600, 600)
         p.intf = abi.ptr(intf, C.c_float)
