@@ -397,6 +397,18 @@ extern "C" int st_server_create(int32_t device, int32_t slots, uint64_t hbmBudge
         c->arenaLimit = s->slotScratch;
         s->ctxs.push_back(c);
     }
+    // each slot's arena is held at its charged size from the start, so no
+    // job frees and reallocates it (a hipFree waits for the whole device,
+    // every other slot included)
+    for (st_ctx *c : s->ctxs) {
+        try {
+            if (hipSetDevice(device) != hipSuccess)
+                return fail(ST_ERR_HIP);
+            c->arena_reserve(s->slotScratch);
+        } catch (const StError &e) {
+            return fail(e.code);
+        }
+    }
     s->used = (uint64_t)slots * s->slotScratch;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess)
         return fail(ST_ERR_HIP);
